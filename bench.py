@@ -1,0 +1,191 @@
+#!/usr/bin/env python3
+"""Ray-cast microbenchmark of the mi355x-lbvh accelerator on one MI355X.
+
+Workload: a 1024 x 1024 fractal heightfield (2,093,058 triangles) plus
+100,000 random triangles above it; 8,388,608 coherent primary rays
+(4096 x 2048 pinhole) and 8,388,608 incoherent rays (origins uniform in the
+scene box, directions uniform on the sphere); shadow rays are the incoherent
+rays with tmax uniform in (0, 1].
+
+A step is one closest-hit pass over the 8,388,608 incoherent rays, with rays
+and BVH already resident in HBM.  The coherent closest-hit and incoherent
+any-hit rates are reported alongside.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Mrays/s incoherent closest-hit; time-to-relMSE 1e-3, guided vs plain GPU PT"
+HBM_PEAK_GBS = 8000.0
+RAY_BYTES = 32
+HIT_BYTES = 16
+NODE_BYTES = 64
+TRI_BYTES = 48
+OCC_BYTES = 4  # any-hit output per ray, padded (the kernel writes 1 bit)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--cpu-sample", type=int, default=2_097_152,
+                   help="incoherent rays timed on the CPU kd-tree port (0: skip)")
+    p.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cores)")
+    p.add_argument("--res", type=int, default=1024)
+    p.add_argument("--n-random", type=int, default=100_000)
+    p.add_argument("--quiet", action="store_true")
+    return p.parse_args()
+
+
+def log(args, msg):
+    if not args.quiet:
+        print(msg, file=sys.stderr, flush=True)
+
+
+def timed_passes(fn, steps, warmup, torch):
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(steps):
+        fn()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall / steps * 1e3, ev0.elapsed_time(ev1) / steps
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not supported",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+    from libyafaray_amd import scenes
+    from libyafaray_amd.accel import GpuAccelerator
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no HIP device"}))
+        sys.exit(1)
+    torch.cuda.set_device(0)
+
+    t0 = time.perf_counter()
+    verts, tris, coherent, incoherent, shadow = scenes.raycast_workload(args.res, args.n_random)
+    log(args, f"workload: {len(tris)} tris, {len(coherent)} coherent + {len(incoherent)} incoherent rays "
+              f"({time.perf_counter() - t0:.1f}s)")
+    acc = GpuAccelerator(verts, tris)
+    info = acc.info()
+    log(args, f"lbvh: {info['n_nodes']} nodes, depth {info['max_depth']}, build {info['build_ms']:.1f} ms")
+
+    d_inc = torch.from_numpy(incoherent).cuda()
+    d_coh = torch.from_numpy(coherent).cuda()
+    d_sha = torch.from_numpy(shadow).cuda()
+    n = d_inc.shape[0]
+    hits = torch.empty((n, 4), dtype=torch.int32, device="cuda")
+    occ = torch.empty(((n + 31) // 32,), dtype=torch.int32, device="cuda")
+
+    wall_ms, ev_ms = timed_passes(lambda: acc.intersect_device(d_inc, hits), args.steps, args.warmup, torch)
+    value = n / (wall_ms * 1e-3) / 1e6
+    log(args, f"incoherent closest: {ev_ms:.3f} ms/pass (events), {value:.1f} Mrays/s")
+    _, coh_ms = timed_passes(lambda: acc.intersect_device(d_coh, hits), max(3, args.steps // 2), 2, torch)
+    _, sha_ms = timed_passes(lambda: acc.intersect_shadow_device(d_sha, occ), max(3, args.steps // 2), 2, torch)
+
+    # counting replay of the same kernels on the same batches
+    nf_inc, nt_inc = acc.traversal_stats(d_inc, 0)
+    nf_coh, nt_coh = acc.traversal_stats(d_coh, 0)
+    nf_sha, nt_sha = acc.traversal_stats(d_sha, 1)
+    nodes_per_ray = nf_inc / n
+    tris_per_ray = nt_inc / n
+    bytes_per_ray = RAY_BYTES + HIT_BYTES + NODE_BYTES * nodes_per_ray + TRI_BYTES * tris_per_ray
+    achieved = bytes_per_ray * n / (ev_ms * 1e-3) / 1e9
+
+    traffic = None
+    tfile = ROOT / "profiles" / "raycast_traffic.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get("closest_incoherent_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if args.cpu_sample > 0:
+        from oracle import pyoracle
+        threads = args.cpu_threads or max(1, min(16, os.cpu_count() or 1))
+        sample = incoherent[: args.cpu_sample]
+        tb = time.perf_counter()
+        kd = pyoracle.KdTree(verts, tris)
+        build_s = time.perf_counter() - tb
+        tq = time.perf_counter()
+        kd.closest(sample, threads=threads)
+        q_s = time.perf_counter() - tq
+        cpu = {"value": round(len(sample) / q_s / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+               "sample": f"{len(sample)} of the incoherent rays, closest-hit, C port of the reference kd-tree "
+                         f"(SAH build {build_s:.1f}s not timed)"}
+        log(args, f"cpu kd-tree port: {cpu['value']} Mrays/s on {threads} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"raycast heightfield {args.res}x{args.res} + {args.n_random} random tris "
+                        f"({len(tris)} tris); {n} incoherent closest-hit rays per step",
+            "triangles": int(len(tris)),
+            "rays_per_step": int(n),
+            "query": "closest-hit",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "bytes_per_ray": round(bytes_per_ray, 1),
+            "nodes_per_ray": round(nodes_per_ray, 2),
+            "tris_per_ray": round(tris_per_ray, 2),
+            "kernel_ms": round(ev_ms, 4),
+        },
+        "cpu_baseline": cpu,
+        "extra": {
+            "coherent_closest_mrays": round(len(coherent) / (coh_ms * 1e-3) / 1e6, 1),
+            "incoherent_anyhit_mrays": round(n / (sha_ms * 1e-3) / 1e6, 1),
+            "coherent_nodes_per_ray": round(nf_coh / len(coherent), 2),
+            "coherent_tris_per_ray": round(nt_coh / len(coherent), 2),
+            "anyhit_nodes_per_ray": round(nf_sha / n, 2),
+            "anyhit_tris_per_ray": round(nt_sha / n, 2),
+            "bvh_build_ms": round(info["build_ms"], 2),
+            "bvh_nodes": info["n_nodes"],
+            "bvh_depth": info["max_depth"],
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

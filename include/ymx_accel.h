@@ -1,0 +1,144 @@
+/*
+ * ymx_accel.h -- C ABI of the MI355X ray-cast accelerator (libymx.so).
+ *
+ * This is the drop-in boundary for libYafaRay's Accelerator plugin
+ * (include/accelerator/accelerator.h:37-87 in the reference).  A scene selects
+ * it through the accelerator "type" param ("mi355x-lbvh"); the host-side
+ * Accelerator subclass in libyafaray_amd/host wraps these entry points.
+ *
+ * Conventions
+ *   - Every entry point returns an int status (YMX_OK == 0, negative on error);
+ *     nothing throws across the ABI.  ymx_last_error() describes the last
+ *     failure of the calling thread.
+ *   - Ray and hit buffers passed to the query calls are DEVICE pointers owned
+ *     by the caller.  `stream` is a hipStream_t (NULL: the default stream);
+ *     queries are asynchronous on it.
+ *   - A handle is immutable after ymx_accel_create.  Queries on one handle
+ *     share a small per-handle work area (a work counter and a stack spill
+ *     area), so queries on the SAME handle must not overlap in time: issue
+ *     them on one stream, or synchronise between streams.
+ *   - FP32 throughout.  Closest-hit primitive ids are bit-exact with the
+ *     reference's kd-tree except for exact ties in t, which this library
+ *     resolves to the lowest primitive id.
+ */
+#ifndef YMX_ACCEL_H
+#define YMX_ACCEL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define YMX_OK 0
+#define YMX_ERR_INVALID_ARG (-1)
+#define YMX_ERR_HIP (-2)
+#define YMX_ERR_OUT_OF_MEMORY (-3)
+#define YMX_ERR_UNSUPPORTED (-4)
+
+#define YMX_NO_HIT 0xFFFFFFFFu
+
+/* Primitive visibility bits, as include/common/visibility.h:28-33 (Visible,
+ * CastsShadows).  The mask passed per primitive is the AND of the primitive's
+ * and its material's visibility (include/accelerator/accelerator.h:126-128). */
+#define YMX_VIS_VISIBLE 1u
+#define YMX_VIS_CASTS_SHADOWS 2u
+
+/* Ray record (32 B): include/geometry/ray.h:36-51 without time/differentials.
+ * tmax < 0 means unbounded, as Ray::tmax_ = -1 (include/geometry/ray.h:47). */
+typedef struct ymx_ray
+{
+	float org[3];
+	float tmin;
+	float dir[3];
+	float tmax;
+} ymx_ray;
+
+/* Hit record (16 B): the fields of IntersectData
+ * (include/accelerator/intersect_data.h:30-39) a caller needs, with the
+ * Primitive pointer replaced by the primitive's index in the list given to
+ * ymx_accel_create (YMX_NO_HIT on a miss; t, u, v are then 0). */
+typedef struct ymx_hit
+{
+	uint32_t prim;
+	float t;
+	float u;
+	float v;
+} ymx_hit;
+
+typedef struct ymx_accel ymx_accel;
+
+typedef struct ymx_accel_params
+{
+	uint32_t struct_size;   /* sizeof(ymx_accel_params) */
+	int32_t max_leaf_size;  /* triangles per leaf after collapse (1..8), 0 = default */
+	int32_t reserved[6];
+} ymx_accel_params;
+
+typedef struct ymx_accel_info
+{
+	uint32_t struct_size;
+	uint32_t n_tris;
+	uint32_t n_nodes;        /* 64-byte nodes */
+	uint32_t max_depth;      /* longest root-to-leaf path in nodes */
+	uint64_t node_bytes;
+	uint64_t tri_bytes;
+	float build_ms;          /* device time of the GPU build kernels */
+	float bound_min[3];      /* tree bound (getBound()) */
+	float bound_max[3];
+} ymx_accel_info;
+
+/* Build the accelerator over n_tris triangles on the current HIP device.
+ * verts: host, 3 floats per vertex; tri_idx: host, 3 vertex indices per
+ * triangle (all < n_verts, checked); vis: host, one visibility mask per
+ * triangle, or NULL for "normal" (both bits).
+ * Replaces Accelerator::factory -> AcceleratorKdTree::AcceleratorKdTree
+ * (src/accelerator/accelerator.cc:44-55,
+ *  src/accelerator/accelerator_kdtree_original.cc:65-130). */
+int ymx_accel_create(const float *verts, uint32_t n_verts, const uint32_t *tri_idx,
+                     const uint8_t *vis, uint32_t n_tris, const ymx_accel_params *params,
+                     ymx_accel **out);
+
+/* Release the handle and its device memory (blocks until its work is done). */
+int ymx_accel_destroy(ymx_accel *accel);
+
+/* Accelerator::getBound() (include/accelerator/accelerator.h:53): the union of
+ * the triangle bounds grown by 0.001 x extent per axis
+ * (src/accelerator/accelerator_kdtree_original.cc:89-103). */
+int ymx_accel_bound(const ymx_accel *accel, float bound_min[3], float bound_max[3]);
+
+int ymx_accel_get_info(const ymx_accel *accel, ymx_accel_info *info);
+
+/* Closest hit for n rays: Accelerator::intersect(ray, t_max) with
+ * t_max = ray.tmax >= 0 ? ray.tmax : FLT_MAX
+ * (include/accelerator/accelerator.h:50,89-101; traversal semantics of
+ * include/accelerator/accelerator_kdtree_common.h:107-255, Nearest).
+ * t_min = max(ray.tmin, dynamic bias of the tree-bound crossing). */
+int ymx_accel_intersect(const ymx_accel *accel, const ymx_ray *d_rays, ymx_hit *d_hits,
+                        uint32_t n_rays, void *stream);
+
+/* Any hit for n rays: Accelerator::intersectShadow(ray, t_max) with
+ * t_max = ray.tmax >= 0 ? ray.tmax : FLT_MAX; ray.tmin is ignored, the
+ * dynamic bias alone is t_min (include/accelerator/accelerator.h:51,
+ * include/accelerator/accelerator_kdtree_common.h:139).
+ * d_occluded: bit i % 32 of word i / 32 is ray i's occlusion; (n + 31) / 32
+ * words are written.  d_prim (may be NULL): occluder id or YMX_NO_HIT; any
+ * occluder may be reported, as the reference returns the first it meets. */
+int ymx_accel_intersect_shadow(const ymx_accel *accel, const ymx_ray *d_rays,
+                               uint32_t *d_occluded, uint32_t *d_prim, uint32_t n_rays,
+                               void *stream);
+
+/* Counting replay of the query kernels: total 64-byte nodes fetched and
+ * triangles tested over the batch.  query: 0 closest, 1 shadow.  Synchronous;
+ * the counts are written to host memory. */
+int ymx_accel_traversal_stats(const ymx_accel *accel, const ymx_ray *d_rays, uint32_t n_rays,
+                              int query, uint64_t *node_fetches, uint64_t *tri_tests, void *stream);
+
+/* Message of the calling thread's last failure ("" if none). */
+const char *ymx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

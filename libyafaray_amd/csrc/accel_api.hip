@@ -1,0 +1,241 @@
+// accel_api.hip -- the extern "C" entry points declared in include/ymx_accel.h.
+#include "accel_internal.h"
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+
+namespace {
+thread_local char g_last_error[512] = "";
+
+void free_accel(ymx_accel *acc)
+{
+	if(!acc) return;
+	if(acc->d_nodes) (void)hipFree(acc->d_nodes);
+	if(acc->d_tris) (void)hipFree(acc->d_tris);
+	if(acc->d_spill) (void)hipFree(acc->d_spill);
+	if(acc->d_work) (void)hipFree(acc->d_work);
+	if(acc->d_stats) (void)hipFree(acc->d_stats);
+	delete acc;
+}
+}  // namespace
+
+void ymx_set_error(const char *fmt, ...)
+{
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(g_last_error, sizeof g_last_error, fmt, ap);
+	va_end(ap);
+}
+
+extern "C" {
+
+const char *ymx_last_error(void) { return g_last_error; }
+
+int ymx_accel_create(const float *verts, uint32_t n_verts, const uint32_t *tri_idx, const uint8_t *vis,
+                     uint32_t n_tris, const ymx_accel_params *params, ymx_accel **out)
+{
+	g_last_error[0] = '\0';
+	if(!out)
+	{
+		ymx_set_error("ymx_accel_create: out is NULL");
+		return YMX_ERR_INVALID_ARG;
+	}
+	*out = nullptr;
+	if(n_tris > 0 && (!verts || !tri_idx || n_verts == 0))
+	{
+		ymx_set_error("ymx_accel_create: NULL geometry with %u triangles", n_tris);
+		return YMX_ERR_INVALID_ARG;
+	}
+	if(n_tris > YMX_LEAF_MAX_FIRST)
+	{
+		ymx_set_error("ymx_accel_create: %u triangles exceed the limit of %u", n_tris, YMX_LEAF_MAX_FIRST);
+		return YMX_ERR_UNSUPPORTED;
+	}
+	int max_leaf = 1;
+	if(params)
+	{
+		if(params->struct_size != sizeof(ymx_accel_params))
+		{
+			ymx_set_error("ymx_accel_create: params.struct_size %u != %zu", params->struct_size,
+			              sizeof(ymx_accel_params));
+			return YMX_ERR_INVALID_ARG;
+		}
+		if(params->max_leaf_size < 0 || params->max_leaf_size > 8)
+		{
+			ymx_set_error("ymx_accel_create: max_leaf_size %d outside 0..8", params->max_leaf_size);
+			return YMX_ERR_INVALID_ARG;
+		}
+		if(params->max_leaf_size > 0) max_leaf = params->max_leaf_size;
+	}
+	for(size_t i = 0; i < 3 * (size_t)n_tris; ++i)
+	{
+		if(tri_idx[i] >= n_verts)
+		{
+			ymx_set_error("ymx_accel_create: triangle %zu references vertex %u >= n_verts %u", i / 3, tri_idx[i],
+			              n_verts);
+			return YMX_ERR_INVALID_ARG;
+		}
+	}
+	ymx_accel *acc = new(std::nothrow) ymx_accel();
+	if(!acc)
+	{
+		ymx_set_error("ymx_accel_create: host allocation failed");
+		return YMX_ERR_OUT_OF_MEMORY;
+	}
+	memset(acc, 0, sizeof *acc);
+	if(hipGetDevice(&acc->device) != hipSuccess)
+	{
+		ymx_set_error("ymx_accel_create: no HIP device");
+		delete acc;
+		return YMX_ERR_HIP;
+	}
+	int rc = ymx_build_device(acc, verts, n_verts, tri_idx, vis, n_tris, max_leaf);
+	if(rc == YMX_OK) rc = ymx_trace_setup(acc);
+	if(rc != YMX_OK)
+	{
+		free_accel(acc);
+		return rc;
+	}
+	*out = acc;
+	return YMX_OK;
+}
+
+int ymx_accel_destroy(ymx_accel *accel)
+{
+	if(!accel) return YMX_OK;
+	(void)hipDeviceSynchronize();
+	free_accel(accel);
+	return YMX_OK;
+}
+
+int ymx_accel_bound(const ymx_accel *accel, float bound_min[3], float bound_max[3])
+{
+	if(!accel || !bound_min || !bound_max)
+	{
+		ymx_set_error("ymx_accel_bound: NULL argument");
+		return YMX_ERR_INVALID_ARG;
+	}
+	memcpy(bound_min, accel->bmin, 3 * sizeof(float));
+	memcpy(bound_max, accel->bmax, 3 * sizeof(float));
+	return YMX_OK;
+}
+
+int ymx_accel_get_info(const ymx_accel *accel, ymx_accel_info *info)
+{
+	if(!accel || !info || info->struct_size != sizeof(ymx_accel_info))
+	{
+		ymx_set_error("ymx_accel_get_info: NULL argument or struct_size mismatch");
+		return YMX_ERR_INVALID_ARG;
+	}
+	info->n_tris = accel->n_tris;
+	info->n_nodes = accel->n_nodes;
+	info->max_depth = accel->max_depth;
+	info->node_bytes = 64ull * accel->n_nodes;
+	info->tri_bytes = 48ull * (accel->n_tris ? accel->n_tris : 1);
+	info->build_ms = accel->build_ms;
+	memcpy(info->bound_min, accel->bmin, sizeof info->bound_min);
+	memcpy(info->bound_max, accel->bmax, sizeof info->bound_max);
+	return YMX_OK;
+}
+
+static int check_query(const ymx_accel *accel, const void *rays, const void *out, const char *fn)
+{
+	if(!accel)
+	{
+		ymx_set_error("%s: NULL accelerator", fn);
+		return YMX_ERR_INVALID_ARG;
+	}
+	if(!rays || !out)
+	{
+		ymx_set_error("%s: NULL ray or result buffer", fn);
+		return YMX_ERR_INVALID_ARG;
+	}
+	if(((uintptr_t)rays & 15u) || ((uintptr_t)out & 3u))
+	{
+		ymx_set_error("%s: ray buffer must be 16-byte aligned", fn);
+		return YMX_ERR_INVALID_ARG;
+	}
+	return YMX_OK;
+}
+
+static int check_count(uint32_t n_rays, const char *fn)
+{
+	// the persistent kernels hand out rays in chunks of 64 from a 32-bit counter
+	if(n_rays > 0x80000000u)
+	{
+		ymx_set_error("%s: %u rays exceed the per-call limit of 2^31", fn, n_rays);
+		return YMX_ERR_INVALID_ARG;
+	}
+	return YMX_OK;
+}
+
+int ymx_accel_intersect(const ymx_accel *accel, const ymx_ray *d_rays, ymx_hit *d_hits, uint32_t n_rays,
+                        void *stream)
+{
+	if(n_rays == 0) return YMX_OK;
+	int rc = check_count(n_rays, "ymx_accel_intersect");
+	if(rc == YMX_OK) rc = check_query(accel, d_rays, d_hits, "ymx_accel_intersect");
+	if(rc != YMX_OK) return rc;
+	if((uintptr_t)d_hits & 15u)
+	{
+		ymx_set_error("ymx_accel_intersect: hit buffer must be 16-byte aligned");
+		return YMX_ERR_INVALID_ARG;
+	}
+	return ymx_launch_closest(accel, d_rays, d_hits, n_rays, (hipStream_t)stream, nullptr);
+}
+
+int ymx_accel_intersect_shadow(const ymx_accel *accel, const ymx_ray *d_rays, uint32_t *d_occluded,
+                               uint32_t *d_prim, uint32_t n_rays, void *stream)
+{
+	if(n_rays == 0) return YMX_OK;
+	int rc = check_count(n_rays, "ymx_accel_intersect_shadow");
+	if(rc == YMX_OK) rc = check_query(accel, d_rays, d_occluded, "ymx_accel_intersect_shadow");
+	if(rc != YMX_OK) return rc;
+	return ymx_launch_shadow(accel, d_rays, d_occluded, d_prim, n_rays, (hipStream_t)stream, nullptr);
+}
+
+int ymx_accel_traversal_stats(const ymx_accel *accel, const ymx_ray *d_rays, uint32_t n_rays, int query,
+                              uint64_t *node_fetches, uint64_t *tri_tests, void *stream)
+{
+	if(!node_fetches || !tri_tests || (query != 0 && query != 1))
+	{
+		ymx_set_error("ymx_accel_traversal_stats: bad arguments");
+		return YMX_ERR_INVALID_ARG;
+	}
+	*node_fetches = 0;
+	*tri_tests = 0;
+	if(n_rays == 0) return YMX_OK;
+	int rc = check_query(accel, d_rays, accel ? (const void *)accel->d_stats : nullptr, "ymx_accel_traversal_stats");
+	if(rc != YMX_OK) return rc;
+	hipStream_t s = (hipStream_t)stream;
+	void *scratch_out = nullptr;
+	const size_t out_bytes = query == 0 ? sizeof(ymx_hit) * (size_t)n_rays : sizeof(uint32_t) * (((size_t)n_rays + 31) / 32);
+	YMX_HIP_CHECK(hipMalloc(&scratch_out, out_bytes));
+	unsigned long long host[2] = {0, 0};
+	hipError_t e = hipMemsetAsync(accel->d_stats, 0, sizeof host, s);
+	if(e == hipSuccess)
+	{
+		rc = query == 0 ? ymx_launch_closest(accel, d_rays, (ymx_hit *)scratch_out, n_rays, s, accel->d_stats)
+		                : ymx_launch_shadow(accel, d_rays, (uint32_t *)scratch_out, nullptr, n_rays, s, accel->d_stats);
+		if(rc == YMX_OK) e = hipMemcpyAsync(host, accel->d_stats, sizeof host, hipMemcpyDeviceToHost, s);
+		if(rc == YMX_OK && e == hipSuccess) e = hipStreamSynchronize(s);
+	}
+	(void)hipFree(scratch_out);
+	if(rc != YMX_OK) return rc;
+	YMX_HIP_CHECK(e);
+	uint32_t work[2] = {0, 0};
+	YMX_HIP_CHECK(hipMemcpy(work, accel->d_work, sizeof work, hipMemcpyDeviceToHost));
+	if(work[1])
+	{
+		ymx_set_error("ymx_accel_traversal_stats: traversal stack overflow (depth %u)", accel->max_depth);
+		return YMX_ERR_UNSUPPORTED;
+	}
+	*node_fetches = host[0];
+	*tri_tests = host[1];
+	return YMX_OK;
+}
+
+}  // extern "C"

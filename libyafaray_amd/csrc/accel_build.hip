@@ -1,0 +1,519 @@
+// accel_build.hip -- GPU construction of the linear BVH used by the
+// "mi355x-lbvh" accelerator, and the tree bound the reference's kd-tree
+// exposes through getBound().
+//
+// Pipeline (all on the device, one stream):
+//   prep     one thread per triangle: exact FP32 bounds, centroid, the 48-byte
+//            triangle record (v0, v1 - v0, v2 - v0); wave-reduced scene and
+//            centroid bounds folded in with ordered-int atomics.
+//   morton   63-bit Morton code of the centroid (21 bits per axis).
+//   sort     radix sort of (code, triangle) pairs.
+//   karras   one thread per internal node finds its key range and split
+//            (T. Karras, "Maximizing Parallelism in the Construction of BVHs,
+//            Octrees, and k-d Trees", HPG 2012), writing child refs and
+//            parent links.
+//   reorder  triangle records into leaf order.
+//   refit    one thread per leaf climbs to the root; the second arrival at a
+//            node unions both child boxes (atomic counter per node).
+//   depth    longest root-to-leaf chain, sizing the traversal stack.
+//
+// The reference builds a SAH kd-tree on the CPU instead
+// (src/accelerator/accelerator_kdtree_original.cc:65-645); only its query
+// results and its tree bound (:89-103) are part of the contract.
+#include "accel_internal.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <float.h>
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+namespace {
+
+constexpr int kBuildBlock = 256;
+
+__device__ __forceinline__ int ordered_from_float(float f)
+{
+	const int i = __float_as_int(f);
+	return i >= 0 ? i : i ^ 0x7FFFFFFF;
+}
+__host__ inline float float_from_ordered(int i)
+{
+	const int b = i >= 0 ? i : i ^ 0x7FFFFFFF;
+	float f;
+	memcpy(&f, &b, sizeof f);
+	return f;
+}
+
+// bounds buffer layout (ordered ints): [0..2] scene min, [3..5] scene max,
+// [6..8] centroid min, [9..11] centroid max
+__global__ void k_init_bounds(int *b)
+{
+	const int i = threadIdx.x;
+	if(i < 12) b[i] = ((i / 3) % 2 == 0) ? 0x7FFFFFFF : (int)0x80000000;
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_prep(const float *__restrict__ verts,
+                                                      const uint32_t *__restrict__ idx,
+                                                      const uint8_t *__restrict__ vis, uint32_t n,
+                                                      float4 *__restrict__ tris_unsorted,
+                                                      float *__restrict__ prim_box,
+                                                      float *__restrict__ centroid, int *bounds)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+	float clo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, chi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+	if(i < n)
+	{
+		const uint32_t a = idx[3 * (size_t)i], b = idx[3 * (size_t)i + 1], c = idx[3 * (size_t)i + 2];
+		float v[3][3];
+		for(int k = 0; k < 3; ++k)
+		{
+			v[0][k] = verts[3 * (size_t)a + k];
+			v[1][k] = verts[3 * (size_t)b + k];
+			v[2][k] = verts[3 * (size_t)c + k];
+		}
+		for(int k = 0; k < 3; ++k)
+		{
+			lo[k] = fminf(v[0][k], fminf(v[1][k], v[2][k]));
+			hi[k] = fmaxf(v[0][k], fmaxf(v[1][k], v[2][k]));
+			clo[k] = chi[k] = 0.5f * (lo[k] + hi[k]);
+			prim_box[6 * (size_t)i + k] = lo[k];
+			prim_box[6 * (size_t)i + 3 + k] = hi[k];
+			centroid[3 * (size_t)i + k] = clo[k];
+		}
+		const uint32_t vmask = vis ? (uint32_t)vis[i] : 3u;
+		tris_unsorted[3 * (size_t)i + 0] = make_float4(v[0][0], v[0][1], v[0][2], __uint_as_float(i));
+		tris_unsorted[3 * (size_t)i + 1] = make_float4(v[1][0] - v[0][0], v[1][1] - v[0][1],
+		                                               v[1][2] - v[0][2], __uint_as_float(vmask));
+		tris_unsorted[3 * (size_t)i + 2] = make_float4(v[2][0] - v[0][0], v[2][1] - v[0][1],
+		                                               v[2][2] - v[0][2], 0.f);
+	}
+	// wave64 reduction, then one atomic per wave
+	for(int off = 32; off > 0; off >>= 1)
+	{
+		for(int k = 0; k < 3; ++k)
+		{
+			lo[k] = fminf(lo[k], __shfl_xor(lo[k], off));
+			hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], off));
+			clo[k] = fminf(clo[k], __shfl_xor(clo[k], off));
+			chi[k] = fmaxf(chi[k], __shfl_xor(chi[k], off));
+		}
+	}
+	if((threadIdx.x & 63) == 0)
+	{
+		for(int k = 0; k < 3; ++k)
+		{
+			atomicMin(&bounds[k], ordered_from_float(lo[k]));
+			atomicMax(&bounds[3 + k], ordered_from_float(hi[k]));
+			atomicMin(&bounds[6 + k], ordered_from_float(clo[k]));
+			atomicMax(&bounds[9 + k], ordered_from_float(chi[k]));
+		}
+	}
+}
+
+__device__ __forceinline__ uint64_t spread_bits_21(uint32_t x)
+{
+	uint64_t v = x & 0x1FFFFFu;
+	v = (v | (v << 32)) & 0x1F00000000FFFFull;
+	v = (v | (v << 16)) & 0x1F0000FF0000FFull;
+	v = (v | (v << 8)) & 0x100F00F00F00F00Full;
+	v = (v | (v << 4)) & 0x10C30C30C30C30C3ull;
+	v = (v | (v << 2)) & 0x1249249249249249ull;
+	return v;
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_morton(const float *__restrict__ centroid, uint32_t n,
+                                                        float cx, float cy, float cz, float sx, float sy,
+                                                        float sz, uint64_t *__restrict__ keys,
+                                                        uint32_t *__restrict__ vals)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= n) return;
+	const float scale = 2097151.f;
+	const float fx = fminf(fmaxf((centroid[3 * (size_t)i + 0] - cx) * sx, 0.f), 1.f) * scale;
+	const float fy = fminf(fmaxf((centroid[3 * (size_t)i + 1] - cy) * sy, 0.f), 1.f) * scale;
+	const float fz = fminf(fmaxf((centroid[3 * (size_t)i + 2] - cz) * sz, 0.f), 1.f) * scale;
+	const uint64_t code = (spread_bits_21((uint32_t)fx) << 2) | (spread_bits_21((uint32_t)fy) << 1) |
+	                      spread_bits_21((uint32_t)fz);
+	keys[i] = code;
+	vals[i] = i;
+}
+
+__device__ __forceinline__ int karras_delta(const uint64_t *__restrict__ keys, int n, int i, int j)
+{
+	if(j < 0 || j >= n) return -1;
+	const uint64_t a = keys[i], b = keys[j];
+	if(a == b) return 64 + __clz((uint32_t)(i ^ j));
+	return __clzll((long long)(a ^ b));
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_karras(const uint64_t *__restrict__ keys, int n,
+                                                        float4 *__restrict__ nodes,
+                                                        int32_t *__restrict__ parent_internal,
+                                                        int32_t *__restrict__ parent_leaf)
+{
+	const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+	if(i >= n - 1) return;
+	const int d = (karras_delta(keys, n, i, i + 1) - karras_delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+	const int dmin = karras_delta(keys, n, i, i - d);
+	int lmax = 2;
+	while(karras_delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+	int l = 0;
+	for(int t = lmax >> 1; t >= 1; t >>= 1)
+		if(karras_delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+	const int j = i + l * d;
+	const int dnode = karras_delta(keys, n, i, j);
+	int s = 0;
+	int t = l;
+	do
+	{
+		t = (t + 1) >> 1;
+		if(karras_delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+	} while(t > 1);
+	const int gamma = i + s * d + (d < 0 ? -1 : 0);
+	const int lo = i < j ? i : j;
+	const int hi = i < j ? j : i;
+	int32_t c0, c1;
+	if(lo == gamma)
+	{
+		c0 = ymx_leaf_ref((uint32_t)gamma, 1u);
+		parent_leaf[gamma] = i;
+	}
+	else
+	{
+		c0 = gamma;
+		parent_internal[gamma] = i;
+	}
+	if(hi == gamma + 1)
+	{
+		c1 = ymx_leaf_ref((uint32_t)(gamma + 1), 1u);
+		parent_leaf[gamma + 1] = i;
+	}
+	else
+	{
+		c1 = gamma + 1;
+		parent_internal[gamma + 1] = i;
+	}
+	nodes[4 * (size_t)i + 3] = make_float4(__int_as_float(c0), __int_as_float(c1), 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_reorder(const float4 *__restrict__ src,
+                                                         const uint32_t *__restrict__ order, uint32_t n,
+                                                         float4 *__restrict__ dst)
+{
+	const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+	if(j >= n) return;
+	const uint32_t s = order[j];
+	dst[3 * (size_t)j + 0] = src[3 * (size_t)s + 0];
+	dst[3 * (size_t)j + 1] = src[3 * (size_t)s + 1];
+	dst[3 * (size_t)j + 2] = src[3 * (size_t)s + 2];
+}
+
+__device__ __forceinline__ float load_coherent(const float *p)
+{
+	return __uint_as_float(__hip_atomic_load((const uint32_t *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+__device__ __forceinline__ void store_slot(float4 *nodes, int node, int slot, const float lo[3], const float hi[3])
+{
+	float *f = (float *)(nodes + 4 * (size_t)node);
+	// slot 0: f0 = (lo.x, hi.x, lo.y, hi.y), f2.xy = (lo.z, hi.z); slot 1: f1, f2.zw
+	float *xy = f + 4 * slot;
+	xy[0] = lo[0];
+	xy[1] = hi[0];
+	xy[2] = lo[1];
+	xy[3] = hi[1];
+	f[8 + 2 * slot] = lo[2];
+	f[9 + 2 * slot] = hi[2];
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_refit(float4 *nodes, const float *__restrict__ prim_box,
+                                                       const uint32_t *__restrict__ order, uint32_t n,
+                                                       const int32_t *__restrict__ parent_internal,
+                                                       const int32_t *__restrict__ parent_leaf,
+                                                       uint32_t *flags, float pad)
+{
+	const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+	if(j >= n) return;
+	const uint32_t p = order[j];
+	float lo[3], hi[3];
+	for(int k = 0; k < 3; ++k)
+	{
+		lo[k] = prim_box[6 * (size_t)p + k] - pad;
+		hi[k] = prim_box[6 * (size_t)p + 3 + k] + pad;
+	}
+	int32_t child_ref = ymx_leaf_ref(j, 1u);
+	int32_t node = parent_leaf[j];
+	while(node >= 0)
+	{
+		const int32_t c0 = __float_as_int(nodes[4 * (size_t)node + 3].x);
+		const int slot = (c0 == child_ref) ? 0 : 1;
+		store_slot(nodes, node, slot, lo, hi);
+		__threadfence();
+		const uint32_t old = atomicAdd(&flags[node], 1u);
+		if(old == 0) return;  // sibling not done yet; it will continue upwards
+		__threadfence();
+		const float *f = (const float *)(nodes + 4 * (size_t)node);
+		const int other = 1 - slot;
+		const float olo[3] = {load_coherent(f + 4 * other + 0), load_coherent(f + 4 * other + 2),
+		                      load_coherent(f + 8 + 2 * other)};
+		const float ohi[3] = {load_coherent(f + 4 * other + 1), load_coherent(f + 4 * other + 3),
+		                      load_coherent(f + 9 + 2 * other)};
+		for(int k = 0; k < 3; ++k)
+		{
+			lo[k] = fminf(lo[k], olo[k]);
+			hi[k] = fmaxf(hi[k], ohi[k]);
+		}
+		child_ref = node;
+		node = (node == 0) ? -1 : parent_internal[node];
+	}
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_depth(const int32_t *__restrict__ parent_internal,
+                                                       const int32_t *__restrict__ parent_leaf, uint32_t n,
+                                                       uint32_t *max_depth)
+{
+	const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+	uint32_t d = 0;
+	if(j < n)
+	{
+		int32_t node = parent_leaf[j];
+		while(node >= 0)
+		{
+			++d;
+			node = (node == 0) ? -1 : parent_internal[node];
+		}
+	}
+	for(int off = 32; off > 0; off >>= 1) d = max(d, (uint32_t)__shfl_xor((int)d, off));
+	if((threadIdx.x & 63) == 0) atomicMax(max_depth, d);
+}
+
+inline unsigned grid_for(uint64_t n) { return (unsigned)((n + kBuildBlock - 1) / kBuildBlock); }
+
+struct DevBuf
+{
+	void *p = nullptr;
+	~DevBuf()
+	{
+		if(p) (void)hipFree(p);
+	}
+	hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 16); }
+	template <typename T> T *as() const { return static_cast<T *>(p); }
+	void *release()
+	{
+		void *r = p;
+		p = nullptr;
+		return r;
+	}
+};
+
+struct EventPair
+{
+	hipEvent_t a = nullptr, b = nullptr;
+	~EventPair()
+	{
+		if(a) (void)hipEventDestroy(a);
+		if(b) (void)hipEventDestroy(b);
+	}
+};
+
+// Tree bound of src/accelerator/accelerator_kdtree_original.cc:89-103: the
+// union of primitive bounds, each axis grown by 0.001 x (float) extent
+// evaluated in double and narrowed back to float.
+void reference_tree_bound(const float lo_in[3], const float hi_in[3], uint32_t n_tris, float bmin[3], float bmax[3])
+{
+	for(int k = 0; k < 3; ++k)
+	{
+		float a = n_tris ? lo_in[k] : 0.f;
+		float g = n_tris ? hi_in[k] : 0.f;
+		const double offset = static_cast<double>(g - a) * 0.001;
+		a -= static_cast<float>(offset);
+		g += static_cast<float>(offset);
+		bmin[k] = a;
+		bmax[k] = g;
+	}
+}
+
+}  // namespace
+
+int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const uint32_t *idx,
+                     const uint8_t *vis, uint32_t n_tris, int max_leaf)
+{
+	(void)max_leaf;
+	acc->n_tris = n_tris;
+	acc->build_ms = 0.f;
+	hipStream_t stream = nullptr;
+
+	if(n_tris < 2)
+	{
+		// A single node whose two children are the same one-triangle leaf
+		// (testing it twice changes nothing).  With no triangle the leaf holds
+		// a degenerate, invisible triangle that no ray can hit, so every query
+		// is a miss.  (Slab tests take per-axis min/max, so an inverted "empty"
+		// box would not be rejected; the tree therefore never has empty children.)
+		float4 node[4];
+		float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {0.f, 0.f, 0.f};
+		float4 tri[3] = {};
+		const int32_t c0 = ymx_leaf_ref(0, 1);
+		if(n_tris == 1)
+		{
+			float v[3][3];
+			for(int c = 0; c < 3; ++c)
+				for(int k = 0; k < 3; ++k) v[c][k] = verts[3 * (size_t)idx[c] + k];
+			for(int k = 0; k < 3; ++k)
+			{
+				lo[k] = fminf(v[0][k], fminf(v[1][k], v[2][k]));
+				hi[k] = fmaxf(v[0][k], fmaxf(v[1][k], v[2][k]));
+			}
+			uint32_t zero = 0, vm = vis ? vis[0] : 3u;
+			float fz, fv;
+			memcpy(&fz, &zero, 4);
+			memcpy(&fv, &vm, 4);
+			tri[0] = make_float4(v[0][0], v[0][1], v[0][2], fz);
+			tri[1] = make_float4(v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2], fv);
+			tri[2] = make_float4(v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2], 0.f);
+		}
+		else
+		{
+			// degenerate triangle at the origin, visibility 0: never accepted
+			tri[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+		}
+		float pad = 0.f;
+		if(n_tris == 1)
+		{
+			float ext = 0.f;
+			for(int k = 0; k < 3; ++k) ext = fmaxf(ext, hi[k] - lo[k]);
+			pad = ext * 0x1p-20f + 1e-30f;
+		}
+		float l0[3], h0[3];
+		for(int k = 0; k < 3; ++k)
+		{
+			l0[k] = lo[k] - pad;
+			h0[k] = hi[k] + pad;
+		}
+		float fc0;
+		memcpy(&fc0, &c0, 4);
+		node[0] = make_float4(l0[0], h0[0], l0[1], h0[1]);
+		node[1] = node[0];
+		node[2] = make_float4(l0[2], h0[2], l0[2], h0[2]);
+		node[3] = make_float4(fc0, fc0, 0.f, 0.f);
+		DevBuf dn, dt;
+		YMX_HIP_CHECK(dn.alloc(sizeof node));
+		YMX_HIP_CHECK(dt.alloc(sizeof tri));
+		YMX_HIP_CHECK(hipMemcpy(dn.p, node, sizeof node, hipMemcpyHostToDevice));
+		YMX_HIP_CHECK(hipMemcpy(dt.p, tri, sizeof tri, hipMemcpyHostToDevice));
+		acc->d_nodes = dn.as<float4>();
+		acc->d_tris = dt.as<float4>();
+		dn.release();
+		dt.release();
+		acc->n_nodes = 1;
+		acc->max_depth = 1;
+		reference_tree_bound(lo, hi, n_tris, acc->bmin, acc->bmax);
+		return YMX_OK;
+	}
+
+	DevBuf d_verts, d_idx, d_vis, d_tris_unsorted, d_prim_box, d_centroid, d_bounds, d_keys, d_keys_alt,
+	    d_vals, d_vals_alt, d_nodes, d_tris, d_parent_int, d_parent_leaf, d_flags, d_depth, d_temp;
+	YMX_HIP_CHECK(d_verts.alloc(sizeof(float) * 3 * (size_t)n_verts));
+	YMX_HIP_CHECK(d_idx.alloc(sizeof(uint32_t) * 3 * (size_t)n_tris));
+	if(vis) YMX_HIP_CHECK(d_vis.alloc(n_tris));
+	YMX_HIP_CHECK(d_tris_unsorted.alloc(sizeof(float4) * 3 * (size_t)n_tris));
+	YMX_HIP_CHECK(d_prim_box.alloc(sizeof(float) * 6 * (size_t)n_tris));
+	YMX_HIP_CHECK(d_centroid.alloc(sizeof(float) * 3 * (size_t)n_tris));
+	YMX_HIP_CHECK(d_bounds.alloc(sizeof(int) * 12));
+	YMX_HIP_CHECK(d_keys.alloc(sizeof(uint64_t) * (size_t)n_tris));
+	YMX_HIP_CHECK(d_keys_alt.alloc(sizeof(uint64_t) * (size_t)n_tris));
+	YMX_HIP_CHECK(d_vals.alloc(sizeof(uint32_t) * (size_t)n_tris));
+	YMX_HIP_CHECK(d_vals_alt.alloc(sizeof(uint32_t) * (size_t)n_tris));
+	YMX_HIP_CHECK(d_nodes.alloc(sizeof(float4) * 4 * (size_t)(n_tris - 1)));
+	YMX_HIP_CHECK(d_tris.alloc(sizeof(float4) * 3 * (size_t)n_tris));
+	YMX_HIP_CHECK(d_parent_int.alloc(sizeof(int32_t) * (size_t)(n_tris - 1)));
+	YMX_HIP_CHECK(d_parent_leaf.alloc(sizeof(int32_t) * (size_t)n_tris));
+	YMX_HIP_CHECK(d_flags.alloc(sizeof(uint32_t) * (size_t)(n_tris - 1)));
+	YMX_HIP_CHECK(d_depth.alloc(sizeof(uint32_t)));
+
+	YMX_HIP_CHECK(hipMemcpy(d_verts.p, verts, sizeof(float) * 3 * (size_t)n_verts, hipMemcpyHostToDevice));
+	YMX_HIP_CHECK(hipMemcpy(d_idx.p, idx, sizeof(uint32_t) * 3 * (size_t)n_tris, hipMemcpyHostToDevice));
+	if(vis) YMX_HIP_CHECK(hipMemcpy(d_vis.p, vis, n_tris, hipMemcpyHostToDevice));
+
+	EventPair ev;
+	YMX_HIP_CHECK(hipEventCreate(&ev.a));
+	YMX_HIP_CHECK(hipEventCreate(&ev.b));
+	YMX_HIP_CHECK(hipEventRecord(ev.a, stream));
+
+	k_init_bounds<<<1, 64, 0, stream>>>(d_bounds.as<int>());
+	k_prep<<<grid_for(n_tris), kBuildBlock, 0, stream>>>(d_verts.as<float>(), d_idx.as<uint32_t>(),
+	                                                     vis ? d_vis.as<uint8_t>() : nullptr, n_tris,
+	                                                     d_tris_unsorted.as<float4>(), d_prim_box.as<float>(),
+	                                                     d_centroid.as<float>(), d_bounds.as<int>());
+	YMX_HIP_CHECK(hipGetLastError());
+	int hb[12];
+	YMX_HIP_CHECK(hipMemcpyAsync(hb, d_bounds.p, sizeof hb, hipMemcpyDeviceToHost, stream));
+	YMX_HIP_CHECK(hipStreamSynchronize(stream));
+	float scene_lo[3], scene_hi[3], c_lo[3], c_hi[3];
+	for(int k = 0; k < 3; ++k)
+	{
+		scene_lo[k] = float_from_ordered(hb[k]);
+		scene_hi[k] = float_from_ordered(hb[3 + k]);
+		c_lo[k] = float_from_ordered(hb[6 + k]);
+		c_hi[k] = float_from_ordered(hb[9 + k]);
+	}
+	float inv_ext[3];
+	for(int k = 0; k < 3; ++k)
+	{
+		const float e = c_hi[k] - c_lo[k];
+		inv_ext[k] = e > 0.f ? 1.f / e : 0.f;
+	}
+	k_morton<<<grid_for(n_tris), kBuildBlock, 0, stream>>>(d_centroid.as<float>(), n_tris, c_lo[0], c_lo[1],
+	                                                       c_lo[2], inv_ext[0], inv_ext[1], inv_ext[2],
+	                                                       d_keys.as<uint64_t>(), d_vals.as<uint32_t>());
+	YMX_HIP_CHECK(hipGetLastError());
+
+	hipcub::DoubleBuffer<uint64_t> kb(d_keys.as<uint64_t>(), d_keys_alt.as<uint64_t>());
+	hipcub::DoubleBuffer<uint32_t> vb(d_vals.as<uint32_t>(), d_vals_alt.as<uint32_t>());
+	size_t temp_bytes = 0;
+	YMX_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, kb, vb, (int)n_tris, 0, 63, stream));
+	YMX_HIP_CHECK(d_temp.alloc(temp_bytes));
+	YMX_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(d_temp.p, temp_bytes, kb, vb, (int)n_tris, 0, 63, stream));
+	const uint64_t *keys = kb.Current();
+	const uint32_t *order = vb.Current();
+
+	k_karras<<<grid_for(n_tris - 1), kBuildBlock, 0, stream>>>(keys, (int)n_tris, d_nodes.as<float4>(),
+	                                                           d_parent_int.as<int32_t>(),
+	                                                           d_parent_leaf.as<int32_t>());
+	YMX_HIP_CHECK(hipGetLastError());
+	k_reorder<<<grid_for(n_tris), kBuildBlock, 0, stream>>>(d_tris_unsorted.as<float4>(), order, n_tris,
+	                                                        d_tris.as<float4>());
+	YMX_HIP_CHECK(hipGetLastError());
+	YMX_HIP_CHECK(hipMemsetAsync(d_flags.p, 0, sizeof(uint32_t) * (size_t)(n_tris - 1), stream));
+	YMX_HIP_CHECK(hipMemsetAsync(d_depth.p, 0, sizeof(uint32_t), stream));
+	float ext = 0.f;
+	for(int k = 0; k < 3; ++k) ext = fmaxf(ext, scene_hi[k] - scene_lo[k]);
+	const float pad = ext * 0x1p-20f + 1e-30f;
+	k_refit<<<grid_for(n_tris), kBuildBlock, 0, stream>>>(d_nodes.as<float4>(), d_prim_box.as<float>(), order,
+	                                                      n_tris, d_parent_int.as<int32_t>(),
+	                                                      d_parent_leaf.as<int32_t>(), d_flags.as<uint32_t>(), pad);
+	YMX_HIP_CHECK(hipGetLastError());
+	k_depth<<<grid_for(n_tris), kBuildBlock, 0, stream>>>(d_parent_int.as<int32_t>(), d_parent_leaf.as<int32_t>(),
+	                                                      n_tris, d_depth.as<uint32_t>());
+	YMX_HIP_CHECK(hipGetLastError());
+	YMX_HIP_CHECK(hipEventRecord(ev.b, stream));
+	uint32_t depth = 0;
+	YMX_HIP_CHECK(hipMemcpyAsync(&depth, d_depth.p, sizeof depth, hipMemcpyDeviceToHost, stream));
+	YMX_HIP_CHECK(hipStreamSynchronize(stream));
+	float ms = 0.f;
+	YMX_HIP_CHECK(hipEventElapsedTime(&ms, ev.a, ev.b));
+
+	acc->build_ms = ms;
+	acc->n_nodes = n_tris - 1;
+	acc->max_depth = depth;
+	acc->d_nodes = d_nodes.as<float4>();
+	acc->d_tris = d_tris.as<float4>();
+	d_nodes.release();
+	d_tris.release();
+	reference_tree_bound(scene_lo, scene_hi, n_tris, acc->bmin, acc->bmax);
+	return YMX_OK;
+}

@@ -1,0 +1,95 @@
+// accel_internal.h -- device data layout shared by the LBVH build and the
+// traversal kernels of libymx.so.
+//
+// HBM layout (all 16-byte aligned, read with float4 loads):
+//   node i (64 B, 4 x float4), a binary BVH node holding BOTH children's boxes
+//   so one node fetch decides the next step for a ray:
+//     f0 = (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y)
+//     f1 = (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y)
+//     f2 = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)
+//     f3 = (ref c0, ref c1, -, -) as int bits
+//   child ref >= 0 : internal node index
+//   child ref <  0 : leaf = 0x80000000 | first_tri << 4 | (count - 1)
+//   child ref == YMX_REF_EMPTY : no child (its box is empty)
+//   triangle j (48 B, 3 x float4), in leaf order:
+//     t0 = (v0.xyz, prim id bits)   t1 = (e1 = v1 - v0, visibility bits)
+//     t2 = (e2 = v2 - v0, 0)
+//   e1/e2 are the same FP32 differences the reference forms in
+//   ShapePolygon::intersect (include/geometry/shape/shape_polygon.h:130-131).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ymx_accel.h"
+
+#define YMX_REF_EMPTY ((int32_t)0xFFFFFFFF)
+#define YMX_LEAF_BIT 0x80000000u
+#define YMX_LEAF_MAX_FIRST ((1u << 27) - 2u)
+#define YMX_LDS_STACK 16
+
+static __host__ __device__ inline int32_t ymx_leaf_ref(uint32_t first, uint32_t count)
+{
+	return (int32_t)(YMX_LEAF_BIT | (first << 4) | (count - 1u));
+}
+static __host__ __device__ inline uint32_t ymx_leaf_first(int32_t ref) { return ((uint32_t)ref & 0x7FFFFFFFu) >> 4; }
+static __host__ __device__ inline uint32_t ymx_leaf_count(int32_t ref) { return ((uint32_t)ref & 15u) + 1u; }
+
+struct ymx_accel
+{
+	int device;
+	uint32_t n_tris;
+	uint32_t n_nodes;
+	uint32_t max_depth;
+	float bmin[3], bmax[3];  // tree bound with the 0.1 % enlargement
+	float build_ms;
+	float4 *d_nodes;
+	float4 *d_tris;
+	// query work area (shared by the queries of this handle)
+	uint32_t *d_work;        // [0] ray counter, [1] overflow flag
+	int32_t *d_spill;        // stack spill: spill_depth entries per persistent lane
+	uint32_t spill_depth;
+	uint32_t grid_threads_closest;
+	uint32_t grid_threads_shadow;
+	unsigned long long *d_stats;  // [0] node fetches, [1] triangle tests
+};
+
+struct ymx_trace_args
+{
+	const float4 *nodes;
+	const float4 *tris;
+	const ymx_ray *rays;
+	uint32_t n_rays;
+	float bmin[3], bmax[3];
+	uint32_t *work;
+	int32_t *spill;
+	uint32_t spill_depth;
+	uint32_t grid_threads;
+	ymx_hit *hits;
+	uint32_t *occluded;
+	uint32_t *occ_prim;
+	unsigned long long *stats;
+};
+
+// Build entry (accel_build.hip) and launch entries (accel_trace.hip).
+int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const uint32_t *idx,
+                     const uint8_t *vis, uint32_t n_tris, int max_leaf);
+int ymx_launch_closest(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t n,
+                       hipStream_t stream, unsigned long long *stats);
+int ymx_launch_shadow(const ymx_accel *acc, const ymx_ray *rays, uint32_t *occ, uint32_t *prim,
+                      uint32_t n, hipStream_t stream, unsigned long long *stats);
+int ymx_trace_setup(ymx_accel *acc);
+
+void ymx_set_error(const char *fmt, ...);
+
+#define YMX_HIP_CHECK(expr)                                                                  \
+	do                                                                                       \
+	{                                                                                        \
+		hipError_t ymx_e_ = (expr);                                                          \
+		if(ymx_e_ != hipSuccess)                                                             \
+		{                                                                                    \
+			ymx_set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(ymx_e_), __FILE__, \
+			              __LINE__);                                                         \
+			return ymx_e_ == hipErrorOutOfMemory ? YMX_ERR_OUT_OF_MEMORY : YMX_ERR_HIP;        \
+		}                                                                                    \
+	} while(0)

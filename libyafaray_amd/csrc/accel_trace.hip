@@ -1,0 +1,428 @@
+// accel_trace.hip -- wave64 closest-hit and any-hit traversal of the LBVH.
+//
+// Persistent kernels: a grid that fills the device once; each wave takes
+// 64 consecutive rays at a time from a device counter, so ray loads, hit
+// stores and the any-hit occlusion ballot stay contiguous and coalesced.
+// Per lane: one ray, a 16-entry traversal stack in LDS (lane-major so lanes
+// never share a bank) spilling its oldest entries to a per-lane area in HBM
+// when a path is deeper than 16.
+//
+// Bit-exactness with the reference (its FP32 x86 build, no FMA):
+//   - this file is compiled with -ffp-contract=off and IEEE division;
+//   - the tree-bound crossing, the dynamic ray bias and the triangle test are
+//     evaluated in the reference's operation order;
+//   - box tests only cull: boxes are padded, slab intervals are widened by
+//     (1 + 2 gamma_3) (T. Ize, "Robust BVH Ray Traversal", JCGT 2013) and the
+//     accepted [t_min, t_best] window is widened by 2^-12 relative, so every
+//     triangle whose exact test would be accepted is tested.
+#include "accel_internal.h"
+
+#include <float.h>
+
+namespace {
+
+constexpr int kTraceBlock = 128;
+constexpr int kStack = YMX_LDS_STACK;
+constexpr float kBiasScale = 0.1f * 0.00005f;  // include/accelerator/accelerator.h:64,85
+constexpr float kSlabWiden = 1.0000004f;       // >= 1 + 2 gamma_3 for FP32
+constexpr float kWindow = 0x1p-12f;
+
+struct RayState
+{
+	float o[3], d[3], idir[3];
+	float tlo, thi;
+};
+
+// include/geometry/bound.h:157-198 on the tree bound, exactly as the reference
+// evaluates it.
+__device__ __forceinline__ bool tree_cross(const float a[3], const float g[3], const float from[3],
+                                           const float dir[3], float t_max, float &enter, float &leave)
+{
+	float p[3];
+	for(int k = 0; k < 3; ++k) p[k] = from[k] - a[k];
+	float lmin = -FLT_MAX, lmax = FLT_MAX;
+#pragma unroll
+	for(int axis = 0; axis < 3; ++axis)
+	{
+		if(dir[axis] != 0.f)
+		{
+			const float inv_dir = 1.f / dir[axis];
+			float ltmin, ltmax;
+			if(inv_dir > 0.f)
+			{
+				ltmin = -p[axis] * inv_dir;
+				ltmax = ((g[axis] - a[axis]) - p[axis]) * inv_dir;
+			}
+			else
+			{
+				ltmin = ((g[axis] - a[axis]) - p[axis]) * inv_dir;
+				ltmax = -p[axis] * inv_dir;
+			}
+			if(axis == 0)
+			{
+				lmin = ltmin;
+				lmax = ltmax;
+			}
+			else
+			{
+				lmin = (lmin < ltmin) ? ltmin : lmin;
+				lmax = (lmax < ltmax) ? lmax : ltmax;
+			}
+			if((lmax < 0.f) || (lmin > t_max)) return false;
+		}
+	}
+	if((lmin <= lmax) && (lmax >= 0.f) && (lmin <= t_max))
+	{
+		enter = lmin;
+		leave = lmax;
+		return true;
+	}
+	return false;
+}
+
+// include/geometry/shape/shape_polygon.h:126-176 (N == 3) on a pre-formed
+// record (v0, e1, e2).  Returns t > 0 on a hit, 0 otherwise.
+__device__ __forceinline__ float tri_test(const float4 t0, const float4 t1, const float4 t2, const float o[3],
+                                          const float d[3], float &u_out, float &v_out)
+{
+	const float e1x = t1.x, e1y = t1.y, e1z = t1.z;
+	const float e2x = t2.x, e2y = t2.y, e2z = t2.z;
+	// pvec = dir ^ edge_2
+	const float px = d[1] * e2z - d[2] * e2y;
+	const float py = d[2] * e2x - d[0] * e2z;
+	const float pz = d[0] * e2y - d[1] * e2x;
+	const float det = e1x * px + e1y * py + e1z * pz;
+	if(det == 0.f) return 0.f;
+	const float inv_det = 1.f / det;
+	const float tx = o[0] - t0.x, ty = o[1] - t0.y, tz = o[2] - t0.z;
+	const float u = (tx * px + ty * py + tz * pz) * inv_det;
+	if(!(u >= 0.f && u <= 1.f)) return 0.f;
+	// qvec = tvec ^ edge_1
+	const float qx = ty * e1z - tz * e1y;
+	const float qy = tz * e1x - tx * e1z;
+	const float qz = tx * e1y - ty * e1x;
+	const float v = (d[0] * qx + d[1] * qy + d[2] * qz) * inv_det;
+	if(!(v >= 0.f && (u + v) <= 1.f)) return 0.f;
+	const float t = (e2x * qx + e2y * qy + e2z * qz) * inv_det;
+	if(!(t > 0.f)) return 0.f;
+	u_out = u;
+	v_out = v;
+	return t;
+}
+
+// Slab test of both children of one node.  Returns bit 0 / bit 1 for child
+// 0 / 1 and their entry distances.
+__device__ __forceinline__ int box2(const float4 f0, const float4 f1, const float4 f2, const RayState &r,
+                                    float thi, float &tn0, float &tn1)
+{
+	const float a0x = (f0.x - r.o[0]) * r.idir[0], b0x = (f0.y - r.o[0]) * r.idir[0];
+	const float a0y = (f0.z - r.o[1]) * r.idir[1], b0y = (f0.w - r.o[1]) * r.idir[1];
+	const float a0z = (f2.x - r.o[2]) * r.idir[2], b0z = (f2.y - r.o[2]) * r.idir[2];
+	const float a1x = (f1.x - r.o[0]) * r.idir[0], b1x = (f1.y - r.o[0]) * r.idir[0];
+	const float a1y = (f1.z - r.o[1]) * r.idir[1], b1y = (f1.w - r.o[1]) * r.idir[1];
+	const float a1z = (f2.z - r.o[2]) * r.idir[2], b1z = (f2.w - r.o[2]) * r.idir[2];
+	const float n0 = fmaxf(fmaxf(fminf(a0x, b0x), fminf(a0y, b0y)), fmaxf(fminf(a0z, b0z), r.tlo));
+	const float x0 = fminf(fminf(fmaxf(a0x, b0x), fmaxf(a0y, b0y)), fmaxf(a0z, b0z)) * kSlabWiden;
+	const float n1 = fmaxf(fmaxf(fminf(a1x, b1x), fminf(a1y, b1y)), fmaxf(fminf(a1z, b1z), r.tlo));
+	const float x1 = fminf(fminf(fmaxf(a1x, b1x), fmaxf(a1y, b1y)), fmaxf(a1z, b1z)) * kSlabWiden;
+	tn0 = n0;
+	tn1 = n1;
+	int m = 0;
+	if(n0 <= fminf(x0, thi)) m |= 1;
+	if(n1 <= fminf(x1, thi)) m |= 2;
+	return m;
+}
+
+struct LaneStack
+{
+	int32_t *lds;     // this lane's column: lds[k * kTraceBlock]
+	int32_t *spill;   // this lane's spill column: spill[k * grid_threads]
+	uint32_t stride;  // grid_threads
+	uint32_t cap;     // spill capacity (entries)
+	uint32_t sp;
+	bool overflow;
+
+	__device__ __forceinline__ void push(int32_t ref)
+	{
+		const uint32_t slot = sp & (kStack - 1);
+		if(sp >= (uint32_t)kStack)
+		{
+			const uint32_t k = sp - kStack;
+			if(k >= cap)
+			{
+				overflow = true;
+				return;
+			}
+			spill[(size_t)k * stride] = lds[slot * kTraceBlock];
+		}
+		lds[slot * kTraceBlock] = ref;
+		++sp;
+	}
+	__device__ __forceinline__ int32_t pop()
+	{
+		if(sp == 0) return YMX_REF_EMPTY;
+		--sp;
+		const uint32_t slot = sp & (kStack - 1);
+		const int32_t ref = lds[slot * kTraceBlock];
+		if(sp >= (uint32_t)kStack) lds[slot * kTraceBlock] = spill[(size_t)(sp - kStack) * stride];
+		return ref;
+	}
+};
+
+__device__ __forceinline__ void setup_ray(const ymx_ray &ray, RayState &r)
+{
+	for(int k = 0; k < 3; ++k)
+	{
+		r.o[k] = ray.org[k];
+		r.d[k] = ray.dir[k];
+		const float dk = (fabsf(ray.dir[k]) > 1e-30f) ? ray.dir[k] : copysignf(1e-30f, ray.dir[k]);
+		r.idir[k] = 1.f / dk;
+	}
+}
+
+__device__ __forceinline__ ymx_ray load_ray(const ymx_ray *rays, uint32_t i)
+{
+	const float4 *p = reinterpret_cast<const float4 *>(rays + i);
+	const float4 a = p[0], b = p[1];
+	ymx_ray r;
+	r.org[0] = a.x;
+	r.org[1] = a.y;
+	r.org[2] = a.z;
+	r.tmin = a.w;
+	r.dir[0] = b.x;
+	r.dir[1] = b.y;
+	r.dir[2] = b.z;
+	r.tmax = b.w;
+	return r;
+}
+
+__device__ __forceinline__ uint32_t wave_fetch(uint32_t *counter)
+{
+	uint32_t base = 0;
+	if(__lane_id() == 0) base = atomicAdd(counter, 64u);
+	return __shfl(base, 0);
+}
+
+template <bool kShadow, bool kStats>
+__global__ __launch_bounds__(kTraceBlock) void k_trace(ymx_trace_args args)
+{
+	__shared__ int32_t lds_stack[kStack * kTraceBlock];
+	const uint32_t gtid = blockIdx.x * kTraceBlock + threadIdx.x;
+	LaneStack st;
+	st.lds = lds_stack + threadIdx.x;
+	st.spill = args.spill + gtid;
+	st.stride = args.grid_threads;
+	st.cap = args.spill_depth;
+	st.overflow = false;
+	unsigned long long n_fetch = 0, n_tests = 0;
+
+	for(;;)
+	{
+		const uint32_t base = wave_fetch(args.work);
+		if(base >= args.n_rays) break;
+		const uint32_t rid = base + __lane_id();
+		const bool valid = rid < args.n_rays;
+
+		ymx_ray ray = {};
+		if(valid) ray = load_ray(args.rays, rid);
+		RayState r;
+		setup_ray(ray, r);
+		const float t_max = (ray.tmax >= 0.f) ? ray.tmax : FLT_MAX;
+		float enter = 0.f, leave = 0.f;
+		bool live = valid && tree_cross(args.bmin, args.bmax, ray.org, ray.dir, t_max, enter, leave);
+		// include/accelerator/accelerator_kdtree_common.h:139
+		const float bias = kBiasScale * fabsf(leave - enter);
+		const float t_min = kShadow ? bias : ((ray.tmin < bias) ? bias : ray.tmin);
+		r.tlo = t_min - t_min * kWindow;
+		float best_t = t_max;
+		uint32_t best_prim = YMX_NO_HIT;
+		float best_u = 0.f, best_v = 0.f;
+		bool occluded = false;
+
+		st.sp = 0;
+		int32_t ref = live ? 0 : YMX_REF_EMPTY;
+		while(ref != YMX_REF_EMPTY)
+		{
+			// descend internal nodes until this lane holds a leaf (or nothing)
+			while(ref >= 0)
+			{
+				const float4 *np = args.nodes + 4 * (size_t)ref;
+				const float4 f0 = np[0], f1 = np[1], f2 = np[2], f3 = np[3];
+				if(kStats) ++n_fetch;
+				const float thi = best_t + best_t * kWindow;
+				float tn0, tn1;
+				const int m = box2(f0, f1, f2, r, thi, tn0, tn1);
+				int32_t c0 = __float_as_int(f3.x), c1 = __float_as_int(f3.y);
+				if(m == 3)
+				{
+					if(tn1 < tn0)
+					{
+						const int32_t tmp = c0;
+						c0 = c1;
+						c1 = tmp;
+					}
+					st.push(c1);
+					ref = c0;
+				}
+				else if(m == 1) ref = c0;
+				else if(m == 2) ref = c1;
+				else ref = st.pop();
+			}
+			if(ref == YMX_REF_EMPTY) break;
+			// leaf
+			const uint32_t first = ymx_leaf_first(ref), count = ymx_leaf_count(ref);
+			for(uint32_t k = 0; k < count; ++k)
+			{
+				const float4 *tp = args.tris + 3 * (size_t)(first + k);
+				const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+				if(kStats) ++n_tests;
+				float u, v;
+				const float t = tri_test(t0, t1, t2, r.o, r.d, u, v);
+				if(t <= 0.f || t < t_min) continue;
+				const uint32_t prim = __float_as_uint(t0.w);
+				const uint32_t vmask = __float_as_uint(t1.w);
+				if(kShadow)
+				{
+					// include/accelerator/accelerator.h:135-145
+					if(t >= t_max || !(vmask & YMX_VIS_CASTS_SHADOWS)) continue;
+					occluded = true;
+					best_prim = prim;
+					break;
+				}
+				else
+				{
+					// include/accelerator/accelerator.h:122-133; exact ties keep the lowest id
+					if(!(vmask & YMX_VIS_VISIBLE)) continue;
+					const bool closer = (t < best_t) || (t == best_t && best_prim != YMX_NO_HIT && prim < best_prim);
+					if(!closer) continue;
+					best_t = t;
+					best_prim = prim;
+					best_u = u;
+					best_v = v;
+				}
+			}
+			if(kShadow && occluded) break;
+			ref = st.pop();
+		}
+
+		if(kShadow)
+		{
+			const unsigned long long mask = __ballot(occluded);
+			if(__lane_id() == 0)
+			{
+				const uint32_t w = base >> 5;
+				const uint32_t n_words = (args.n_rays + 31u) >> 5;
+				args.occluded[w] = (uint32_t)mask;
+				if(w + 1 < n_words) args.occluded[w + 1] = (uint32_t)(mask >> 32);
+			}
+			if(valid && args.occ_prim) args.occ_prim[rid] = occluded ? best_prim : YMX_NO_HIT;
+		}
+		else if(valid)
+		{
+			ymx_hit h;
+			if(best_prim != YMX_NO_HIT)
+			{
+				h.prim = best_prim;
+				h.t = best_t;
+				h.u = best_u;
+				h.v = best_v;
+			}
+			else
+			{
+				h.prim = YMX_NO_HIT;
+				h.t = 0.f;
+				h.u = 0.f;
+				h.v = 0.f;
+			}
+			reinterpret_cast<float4 *>(args.hits)[rid] =
+			    make_float4(__uint_as_float(h.prim), h.t, h.u, h.v);
+		}
+	}
+	if(st.overflow) atomicOr(args.work + 1, 1u);
+	if(kStats)
+	{
+		for(int off = 32; off > 0; off >>= 1)
+		{
+			n_fetch += __shfl_xor(n_fetch, off);
+			n_tests += __shfl_xor(n_tests, off);
+		}
+		if(__lane_id() == 0)
+		{
+			atomicAdd(args.stats + 0, n_fetch);
+			atomicAdd(args.stats + 1, n_tests);
+		}
+	}
+}
+
+__global__ void k_reset_work(uint32_t *work) { work[0] = 0u; }
+
+template <bool kShadow, bool kStats>
+int launch(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t *occ, uint32_t *prim, uint32_t n,
+           hipStream_t stream, unsigned long long *stats)
+{
+	if(n == 0) return YMX_OK;
+	ymx_trace_args a;
+	a.nodes = acc->d_nodes;
+	a.tris = acc->d_tris;
+	a.rays = rays;
+	a.n_rays = n;
+	for(int k = 0; k < 3; ++k)
+	{
+		a.bmin[k] = acc->bmin[k];
+		a.bmax[k] = acc->bmax[k];
+	}
+	a.work = acc->d_work;
+	a.spill = acc->d_spill;
+	a.spill_depth = acc->spill_depth;
+	a.grid_threads = kShadow ? acc->grid_threads_shadow : acc->grid_threads_closest;
+	a.hits = hits;
+	a.occluded = occ;
+	a.occ_prim = prim;
+	a.stats = stats;
+	const uint32_t needed = ((n + 63u) / 64u) * 64u;
+	uint32_t threads = a.grid_threads < needed ? a.grid_threads : needed;
+	const uint32_t blocks = (threads + kTraceBlock - 1) / kTraceBlock;
+	k_reset_work<<<1, 1, 0, stream>>>(acc->d_work);
+	k_trace<kShadow, kStats><<<blocks, kTraceBlock, 0, stream>>>(a);
+	YMX_HIP_CHECK(hipGetLastError());
+	return YMX_OK;
+}
+
+}  // namespace
+
+int ymx_trace_setup(ymx_accel *acc)
+{
+	int dev = 0;
+	YMX_HIP_CHECK(hipGetDevice(&dev));
+	hipDeviceProp_t prop;
+	YMX_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+	int per_cu_c = 0, per_cu_s = 0;
+	YMX_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_c, k_trace<false, false>, kTraceBlock, 0));
+	YMX_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s, k_trace<true, false>, kTraceBlock, 0));
+	if(per_cu_c < 1) per_cu_c = 1;
+	if(per_cu_s < 1) per_cu_s = 1;
+	acc->grid_threads_closest = (uint32_t)(prop.multiProcessorCount * per_cu_c * kTraceBlock);
+	acc->grid_threads_shadow = (uint32_t)(prop.multiProcessorCount * per_cu_s * kTraceBlock);
+	const uint32_t gt = acc->grid_threads_closest > acc->grid_threads_shadow ? acc->grid_threads_closest
+	                                                                         : acc->grid_threads_shadow;
+	acc->spill_depth = acc->max_depth > (uint32_t)kStack ? acc->max_depth - (uint32_t)kStack + 1u : 1u;
+	YMX_HIP_CHECK(hipMalloc(&acc->d_spill, sizeof(int32_t) * (size_t)acc->spill_depth * gt));
+	YMX_HIP_CHECK(hipMalloc(&acc->d_work, 64));
+	YMX_HIP_CHECK(hipMemset(acc->d_work, 0, 64));
+	YMX_HIP_CHECK(hipMalloc(&acc->d_stats, 2 * sizeof(unsigned long long)));
+	return YMX_OK;
+}
+
+int ymx_launch_closest(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t n, hipStream_t stream,
+                       unsigned long long *stats)
+{
+	if(stats) return launch<false, true>(acc, rays, hits, nullptr, nullptr, n, stream, stats);
+	return launch<false, false>(acc, rays, hits, nullptr, nullptr, n, stream, nullptr);
+}
+
+int ymx_launch_shadow(const ymx_accel *acc, const ymx_ray *rays, uint32_t *occ, uint32_t *prim, uint32_t n,
+                      hipStream_t stream, unsigned long long *stats)
+{
+	if(stats) return launch<true, true>(acc, rays, nullptr, occ, prim, n, stream, stats);
+	return launch<true, false>(acc, rays, nullptr, occ, prim, n, stream, nullptr);
+}

@@ -1,0 +1,87 @@
+"""ctypes binding of libymx.so (include/ymx_accel.h).
+
+The shared library is the product; this module only loads it and describes
+its structs.  It fails loudly when the library has not been built: there is no
+fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libymx.so"
+
+YMX_OK = 0
+YMX_NO_HIT = 0xFFFFFFFF
+YMX_VIS_VISIBLE = 1
+YMX_VIS_CASTS_SHADOWS = 2
+
+
+class YmxRay(C.Structure):
+    _fields_ = [("org", C.c_float * 3), ("tmin", C.c_float), ("dir", C.c_float * 3), ("tmax", C.c_float)]
+
+
+class YmxHit(C.Structure):
+    _fields_ = [("prim", C.c_uint32), ("t", C.c_float), ("u", C.c_float), ("v", C.c_float)]
+
+
+class YmxAccelParams(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("max_leaf_size", C.c_int32), ("reserved", C.c_int32 * 6)]
+
+
+class YmxAccelInfo(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32),
+        ("n_tris", C.c_uint32),
+        ("n_nodes", C.c_uint32),
+        ("max_depth", C.c_uint32),
+        ("node_bytes", C.c_uint64),
+        ("tri_bytes", C.c_uint64),
+        ("build_ms", C.c_float),
+        ("bound_min", C.c_float * 3),
+        ("bound_max", C.c_float * 3),
+    ]
+
+
+# Every symbol include/ymx_accel.h declares, with its ctypes signature.
+SIGNATURES = {
+    "ymx_accel_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
+                                   C.POINTER(YmxAccelParams), C.POINTER(C.c_void_p)]),
+    "ymx_accel_destroy": (C.c_int, [C.c_void_p]),
+    "ymx_accel_bound": (C.c_int, [C.c_void_p, C.c_float * 3, C.c_float * 3]),
+    "ymx_accel_get_info": (C.c_int, [C.c_void_p, C.POINTER(YmxAccelInfo)]),
+    "ymx_accel_intersect": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "ymx_accel_intersect_shadow": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                             C.c_void_p]),
+    "ymx_accel_traversal_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int,
+                                            C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_void_p]),
+    "ymx_last_error": (C.c_char_p, []),
+}
+
+_lib = None
+
+
+class YmxError(RuntimeError):
+    pass
+
+
+def lib() -> C.CDLL:
+    """Load libymx.so (built by __graft_entry__.build()); raise if absent."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise YmxError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != YMX_OK:
+        msg = lib().ymx_last_error().decode(errors="replace")
+        raise YmxError(f"{what} failed ({rc}): {msg}")

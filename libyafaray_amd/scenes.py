@@ -1,0 +1,230 @@
+"""Seeded synthetic scenes and ray batches of the benchmark workloads.
+
+Every generator is deterministic in its seed (numpy PCG64) and returns plain
+float32 / uint32 arrays, so the GPU library and the CPU oracle see identical
+inputs.
+
+Workloads
+  cornell_random_tris  -- Cornell-style box of 5 unit walls (10 triangles)
+                          plus n random small triangles inside it.
+  heightfield_scene    -- a res x res fractal value-noise heightfield
+                          (6 octaves, amplitude 0.2 over the unit square,
+                          2 (res-1)^2 triangles) plus n random triangles
+                          above it; the ray-cast microbenchmark geometry.
+  camera_rays          -- pinhole camera rays, one per pixel centre, built as
+                          PerspectiveCamera::shootRay does
+                          (src/camera/camera.cc:89-110,
+                          src/camera/camera_perspective.cc:97-111,165-180).
+  incoherent_rays      -- origins uniform in a box, directions uniform on S^2.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+F32 = np.float32
+
+
+def _hash2(ix: np.ndarray, iy: np.ndarray, seed: int) -> np.ndarray:
+    """Integer lattice hash -> [0, 1) (splitmix64 finaliser)."""
+    with np.errstate(over="ignore"):
+        h = (ix.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)) ^ \
+            (iy.astype(np.uint64) * np.uint64(0xC2B2AE3D27D4EB4F)) ^ np.uint64(seed * 0x165667B19E3779F9 & (2**64 - 1))
+        h ^= h >> np.uint64(30)
+        h *= np.uint64(0xBF58476D1CE4E5B9)
+        h ^= h >> np.uint64(27)
+        h *= np.uint64(0x94D049BB133111EB)
+        h ^= h >> np.uint64(31)
+    return (h >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+def value_noise(x: np.ndarray, y: np.ndarray, seed: int) -> np.ndarray:
+    ix = np.floor(x)
+    iy = np.floor(y)
+    fx = x - ix
+    fy = y - iy
+    ix = ix.astype(np.int64)
+    iy = iy.astype(np.int64)
+    sx = fx * fx * (3.0 - 2.0 * fx)
+    sy = fy * fy * (3.0 - 2.0 * fy)
+    v00 = _hash2(ix, iy, seed)
+    v10 = _hash2(ix + 1, iy, seed)
+    v01 = _hash2(ix, iy + 1, seed)
+    v11 = _hash2(ix + 1, iy + 1, seed)
+    a = v00 + (v10 - v00) * sx
+    b = v01 + (v11 - v01) * sx
+    return a + (b - a) * sy
+
+
+def fractal_heights(res: int, octaves: int = 6, amplitude: float = 0.2, base_freq: float = 4.0,
+                    seed: int = 0) -> np.ndarray:
+    t = np.linspace(0.0, 1.0, res)
+    x, y = np.meshgrid(t, t, indexing="xy")
+    h = np.zeros_like(x)
+    amp = 1.0
+    freq = base_freq
+    for o in range(octaves):
+        h += amp * value_noise(x * freq, y * freq, seed * 131 + o)
+        amp *= 0.5
+        freq *= 2.0
+    h = (h - h.min()) / max(h.max() - h.min(), 1e-30) * amplitude
+    return h
+
+
+def grid_mesh(heights: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Vertices over [0,1]^2 with z = heights; two triangles per cell."""
+    res = heights.shape[0]
+    t = np.linspace(0.0, 1.0, res)
+    x, y = np.meshgrid(t, t, indexing="xy")
+    verts = np.stack([x, y, heights], axis=-1).reshape(-1, 3).astype(F32)
+    i, j = np.meshgrid(np.arange(res - 1), np.arange(res - 1), indexing="ij")
+    v00 = (i * res + j).reshape(-1)
+    v01 = v00 + 1
+    v10 = v00 + res
+    v11 = v10 + 1
+    tris = np.empty((2 * v00.size, 3), dtype=np.uint32)
+    tris[0::2] = np.stack([v00, v01, v11], axis=-1)
+    tris[1::2] = np.stack([v00, v11, v10], axis=-1)
+    return verts, tris
+
+
+def unit_vectors(rng: np.random.Generator, n: int) -> np.ndarray:
+    z = rng.uniform(-1.0, 1.0, n)
+    phi = rng.uniform(0.0, 2.0 * np.pi, n)
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    return np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=-1)
+
+
+def random_triangles(rng: np.random.Generator, n: int, lo, hi, edge=(0.01, 0.03)) -> np.ndarray:
+    """n triangles (n, 3 vertices, xyz): centroid uniform in the box [lo, hi],
+    each vertex at a distance U(edge) from the centroid along a random unit
+    direction."""
+    c = np.column_stack([rng.uniform(lo[k], hi[k], n) for k in range(3)])
+    out = np.empty((n, 3, 3))
+    for k in range(3):
+        length = rng.uniform(edge[0], edge[1], n)[:, None]
+        out[:, k, :] = c + length * unit_vectors(rng, n)
+    return out.astype(F32)
+
+
+def merge(meshes: list[tuple[np.ndarray, np.ndarray]]) -> tuple[np.ndarray, np.ndarray]:
+    verts, tris, off = [], [], 0
+    for v, t in meshes:
+        verts.append(v)
+        tris.append(t.astype(np.int64) + off)
+        off += v.shape[0]
+    return np.concatenate(verts).astype(F32), np.concatenate(tris).astype(np.uint32)
+
+
+def soup(tri_verts: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    n = tri_verts.shape[0]
+    return tri_verts.reshape(-1, 3).astype(F32), np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+
+
+def heightfield_scene(res: int = 1024, n_random: int = 100_000, seed: int = 0):
+    """Ray-cast microbenchmark geometry: the heightfield plus n_random triangles
+    with centroids in [0,1]^2 x [0.22, 0.5], above the surface."""
+    rng = np.random.default_rng(seed)
+    hf = grid_mesh(fractal_heights(res, seed=seed))
+    if not n_random:
+        return hf
+    rnd = soup(random_triangles(rng, n_random, (0.0, 0.0, 0.22), (1.0, 1.0, 0.5)))
+    return merge([hf, rnd])
+
+
+def cornell_walls() -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """5 axis-aligned unit walls of [0,1]^3 (open at y = 0), 2 triangles each.
+    Returns verts, tris and a wall index per triangle (0 floor, 1 ceiling,
+    2 back, 3 left, 4 right)."""
+    quads = [
+        [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0)],  # floor z=0
+        [(0, 0, 1), (0, 1, 1), (1, 1, 1), (1, 0, 1)],  # ceiling z=1
+        [(0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)],  # back y=1
+        [(0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1)],  # left x=0
+        [(1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0)],  # right x=1
+    ]
+    verts = np.array([p for q in quads for p in q], dtype=F32)
+    tris = []
+    wall = []
+    for w in range(5):
+        b = 4 * w
+        tris += [[b, b + 1, b + 2], [b, b + 2, b + 3]]
+        wall += [w, w]
+    return verts, np.array(tris, dtype=np.uint32), np.array(wall, dtype=np.int32)
+
+
+def cornell_random_tris(n_random: int = 10_000, seed: int = 0):
+    rng = np.random.default_rng(seed)
+    wv, wt, _ = cornell_walls()
+    rnd = soup(random_triangles(rng, n_random, (0.2, 0.2, 0.2), (0.8, 0.8, 0.8)))
+    return merge([(wv, wt), rnd])
+
+
+def _normalize32(v: np.ndarray) -> np.ndarray:
+    """Vec::normalize on float32 rows: v * (1 / sqrt(x^2 + y^2 + z^2))."""
+    v = v.astype(F32)
+    l2 = (v[..., 0] * v[..., 0] + v[..., 1] * v[..., 1]) + v[..., 2] * v[..., 2]
+    inv = (F32(1.0) / np.sqrt(l2)).astype(F32)
+    return (v * inv[..., None]).astype(F32)
+
+
+def camera_rays(resx: int, resy: int, eye, target, up_point, focal: float) -> np.ndarray:
+    """Pinhole rays through pixel centres (x + 0.5, y + 0.5), laid out row-major."""
+    eye = np.asarray(eye, dtype=F32)
+    cam_y = (np.asarray(up_point, dtype=F32) - eye).astype(F32)
+    cam_z = (np.asarray(target, dtype=F32) - eye).astype(F32)
+    cam_x = np.cross(cam_z, cam_y).astype(F32)
+    cam_y = np.cross(cam_z, cam_x).astype(F32)
+    cam_x, cam_y, cam_z = (_normalize32(a) for a in (cam_x, cam_y, cam_z))
+    aspect = F32(resy) / F32(resx)
+    vright = cam_x
+    vup = (aspect * cam_y).astype(F32)
+    vto = (cam_z * F32(focal) - F32(0.5) * (vup + vright)).astype(F32)
+    vup = (vup / F32(resy)).astype(F32)
+    vright = (vright / F32(resx)).astype(F32)
+    px = (np.arange(resx, dtype=F32) + F32(0.5))
+    py = (np.arange(resy, dtype=F32) + F32(0.5))
+    PX, PY = np.meshgrid(px, py, indexing="xy")
+    d = (vright[None, None, :] * PX[..., None] + vup[None, None, :] * PY[..., None]).astype(F32) + vto
+    d = _normalize32(d.reshape(-1, 3))
+    rays = np.zeros((d.shape[0], 8), dtype=F32)
+    rays[:, 0:3] = eye
+    rays[:, 3] = 0.0
+    rays[:, 4:7] = d
+    rays[:, 7] = -1.0
+    return rays
+
+
+def incoherent_rays(rng: np.random.Generator, n: int, lo, hi, tmax=None) -> np.ndarray:
+    lo = np.asarray(lo, dtype=np.float64)
+    hi = np.asarray(hi, dtype=np.float64)
+    rays = np.zeros((n, 8), dtype=F32)
+    rays[:, 0:3] = rng.uniform(lo, hi, size=(n, 3))
+    rays[:, 4:7] = unit_vectors(rng, n)
+    rays[:, 3] = 0.0
+    rays[:, 7] = -1.0 if tmax is None else tmax
+    return rays
+
+
+def shadow_tmax(rng: np.random.Generator, n: int) -> np.ndarray:
+    """tmax uniform in (0, 1]."""
+    return (1.0 - rng.uniform(0.0, 1.0, n)).astype(F32)
+
+
+# Ray-cast microbenchmark (16M rays on the ~2.2M-triangle heightfield scene).
+RAYCAST_CAMERA = dict(eye=(0.5, -0.45, 0.75), target=(0.5, 0.55, 0.05), up_point=(0.5, -0.45, 1.75),
+                      focal=1.0 / (2.0 * np.tan(np.radians(20.0))))
+
+
+def raycast_workload(res: int = 1024, n_random: int = 100_000, cam_res=(4096, 2048), n_incoherent=None,
+                     seed: int = 0):
+    """Geometry plus coherent, incoherent and shadow ray batches."""
+    verts, tris = heightfield_scene(res, n_random, seed)
+    rng = np.random.default_rng(seed + 1)
+    coherent = camera_rays(cam_res[0], cam_res[1], **RAYCAST_CAMERA)
+    n_inc = coherent.shape[0] if n_incoherent is None else n_incoherent
+    lo = verts.min(axis=0)
+    hi = verts.max(axis=0)
+    incoherent = incoherent_rays(rng, n_inc, lo, hi)
+    shadow = incoherent.copy()
+    shadow[:, 7] = shadow_tmax(rng, n_inc)
+    return verts, tris, coherent, incoherent, shadow

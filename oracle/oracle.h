@@ -1,0 +1,113 @@
+/*
+ * oracle.h -- CPU restatement of libYafaRay's ray-cast path (TEST INFRASTRUCTURE).
+ *
+ * This library is the checker, never the product.  Only tests/, the smoke()
+ * entry point and the cpu_baseline leg of bench.py may load it.  The shipped
+ * path (libyafaray_amd) never links or calls it.
+ *
+ * Every function names the reference file:line whose behaviour it restates.
+ * Arithmetic is FP32 in the reference's operation order, compiled without FMA
+ * contraction (-ffp-contract=off), so results are bit-identical to the
+ * reference's x86-64 SSE build of the same expressions.
+ *
+ * Pinning (see tests/test_oracle_*.py):
+ *   ora_triangle_intersect  -- closed-form hits (axis-aligned, exactly
+ *                              representable), double-precision re-derivation,
+ *                              barycentric reconstruction, mutation cases.
+ *   ora_bound_cross         -- closed-form slab intervals, the zero-direction
+ *                              quirk, early-out cases.
+ *   ora_tree_bound          -- closed form of the 0.1% enlargement.
+ *   ora_closest_brute       -- analytic heightfield hits (vertical rays),
+ *                              double-precision brute force, kd-tree port.
+ *   ora_kd_*                -- equals ora_*_brute on every non-tie ray.
+ */
+#ifndef YMX_ORACLE_H
+#define YMX_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Ray record shared with the product ABI: origin, tmin, direction, tmax
+ * (tmax < 0 means "unbounded", include/geometry/ray.h:45-47). */
+typedef struct { float org[3]; float tmin; float dir[3]; float tmax; } ora_ray;
+
+/* Hit record: prim id (0xFFFFFFFF = miss), t, u, v. */
+typedef struct { uint32_t prim; float t; float u; float v; } ora_hit;
+
+#define ORA_NO_HIT 0xFFFFFFFFu
+#define ORA_VIS_VISIBLE 1u   /* include/common/visibility.h:28-33 */
+#define ORA_VIS_SHADOWS 2u
+
+/* include/geometry/shape/shape_polygon.h:126-176 (N == 3). Returns 1 on hit. */
+int ora_triangle_intersect(const float v0[3], const float v1[3], const float v2[3],
+                           const float from[3], const float dir[3],
+                           float *t, float *u, float *v);
+
+/* include/geometry/bound.h:157-198. Returns 1 when crossed. */
+int ora_bound_cross(const float a[3], const float g[3], const float from[3], const float dir[3],
+                    float t_max, float *enter, float *leave);
+
+/* src/accelerator/accelerator_kdtree_original.cc:89-103 */
+void ora_tree_bound(const float *verts, const uint32_t *tri_idx, uint32_t n_tris,
+                    float a_out[3], float g_out[3]);
+
+/* include/accelerator/accelerator.h:64 */
+float ora_dynamic_bias(float enter, float leave);
+
+/* Brute-force closest hit: Accelerator::intersect(ray, camera) semantics
+ * (include/accelerator/accelerator.h:89-101, 122-133) over every primitive in
+ * id order; an exact tie in t keeps the lowest id. */
+void ora_closest_brute(const float *verts, const uint32_t *tri_idx, const uint8_t *vis,
+                       uint32_t n_tris, const ora_ray *rays, uint32_t n_rays,
+                       ora_hit *hits, int threads);
+
+/* Brute-force any hit: Accelerator::intersectShadow(ray, t_max) semantics with
+ * t_max = ray.tmax (< 0: FLT_MAX), include/accelerator/accelerator_kdtree_common.h:139,
+ * include/accelerator/accelerator.h:135-145.  occluded[i] = 0/1. */
+void ora_shadow_brute(const float *verts, const uint32_t *tri_idx, const uint8_t *vis,
+                      uint32_t n_tris, const ora_ray *rays, uint32_t n_rays,
+                      uint8_t *occluded, int threads);
+
+/* Re-run the triangle test for given (ray, primitive) pairs: out[i] is the
+ * ora_hit of ray i against primitive prim[i] (prim[i] == ORA_NO_HIT: skipped,
+ * out[i] = {ORA_NO_HIT, 0, 0, 0}). */
+void ora_retest(const float *verts, const uint32_t *tri_idx, const ora_ray *rays, const uint32_t *prim,
+                uint32_t n_rays, ora_hit *out);
+
+/* ---- kd-tree port (src/accelerator/accelerator_kdtree_original.cc) ---- */
+typedef struct ora_kdtree ora_kdtree;
+
+/* Build with the reference's SAH (pigeonhole bins for > 128 prims, sorted
+ * edges below), max depth 7 + 1.66 ln N, leaf size 1, cost ratio 0.8, empty
+ * bonus 0.33.  Polygon clipping of small nodes (PRIMITIVE_CLIPPING) is not
+ * restated; nodes keep whole-primitive bounds, which changes tree shape but
+ * not query results.  Returns NULL on allocation failure. */
+ora_kdtree *ora_kd_build(const float *verts, const uint32_t *tri_idx, const uint8_t *vis,
+                         uint32_t n_tris);
+void ora_kd_free(ora_kdtree *kd);
+void ora_kd_bound(const ora_kdtree *kd, float a[3], float g[3]);
+uint32_t ora_kd_num_nodes(const ora_kdtree *kd);
+
+/* include/accelerator/accelerator_kdtree_common.h:107-255, Nearest. */
+void ora_kd_closest(const ora_kdtree *kd, const ora_ray *rays, uint32_t n_rays,
+                    ora_hit *hits, int threads);
+/* Same, Shadow: occluded[i] = 0/1 and, if prim != NULL, the occluder id. */
+void ora_kd_shadow(const ora_kdtree *kd, const ora_ray *rays, uint32_t n_rays,
+                   uint8_t *occluded, uint32_t *prim, int threads);
+
+/* include/math/math.h:116-145 (FAST_MATH log2/exp2/pow) and :184-190 (log). */
+float ora_fast_log2(float x);
+float ora_fast_exp2(float x);
+float ora_fast_pow(float a, float b);
+float ora_fast_log(float a);
+/* include/math/math.h:228-265 (FAST_TRIG sin/cos). */
+float ora_fast_sin(float x);
+float ora_fast_cos(float x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

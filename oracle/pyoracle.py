@@ -1,0 +1,173 @@
+"""ctypes wrapper of oracle/liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+The oracle is the checker: only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py import this module.  The shipped package never
+does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+NO_HIT = 0xFFFFFFFF
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built (make -C oracle)")
+        L = C.CDLL(str(LIB_PATH))
+        fp = C.POINTER(C.c_float)
+        vp = C.c_void_p
+        L.ora_triangle_intersect.argtypes = [fp, fp, fp, fp, fp, fp, fp, fp]
+        L.ora_triangle_intersect.restype = C.c_int
+        L.ora_bound_cross.argtypes = [fp, fp, fp, fp, C.c_float, fp, fp]
+        L.ora_bound_cross.restype = C.c_int
+        L.ora_tree_bound.argtypes = [vp, vp, C.c_uint32, fp, fp]
+        L.ora_dynamic_bias.argtypes = [C.c_float, C.c_float]
+        L.ora_dynamic_bias.restype = C.c_float
+        L.ora_closest_brute.argtypes = [vp, vp, vp, C.c_uint32, vp, C.c_uint32, vp, C.c_int]
+        L.ora_shadow_brute.argtypes = [vp, vp, vp, C.c_uint32, vp, C.c_uint32, vp, C.c_int]
+        L.ora_retest.argtypes = [vp, vp, vp, vp, C.c_uint32, vp]
+        L.ora_kd_build.argtypes = [vp, vp, vp, C.c_uint32]
+        L.ora_kd_build.restype = vp
+        L.ora_kd_free.argtypes = [vp]
+        L.ora_kd_bound.argtypes = [vp, fp, fp]
+        L.ora_kd_num_nodes.argtypes = [vp]
+        L.ora_kd_num_nodes.restype = C.c_uint32
+        L.ora_kd_closest.argtypes = [vp, vp, C.c_uint32, vp, C.c_int]
+        L.ora_kd_shadow.argtypes = [vp, vp, C.c_uint32, vp, vp, C.c_int]
+        for name in ("ora_fast_log2", "ora_fast_exp2", "ora_fast_log", "ora_fast_sin", "ora_fast_cos"):
+            getattr(L, name).argtypes = [C.c_float]
+            getattr(L, name).restype = C.c_float
+        L.ora_fast_pow.argtypes = [C.c_float, C.c_float]
+        L.ora_fast_pow.restype = C.c_float
+        _lib = L
+    return _lib
+
+
+def default_threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def _f3(a) -> C.Array:
+    return (C.c_float * 3)(*[float(x) for x in a])
+
+
+def triangle_intersect(v0, v1, v2, org, dir):
+    t, u, v = C.c_float(), C.c_float(), C.c_float()
+    hit = lib().ora_triangle_intersect(_f3(v0), _f3(v1), _f3(v2), _f3(org), _f3(dir),
+                                       C.byref(t), C.byref(u), C.byref(v))
+    return bool(hit), t.value, u.value, v.value
+
+
+def bound_cross(a, g, org, dir, t_max):
+    e, l = C.c_float(), C.c_float()
+    hit = lib().ora_bound_cross(_f3(a), _f3(g), _f3(org), _f3(dir), C.c_float(t_max), C.byref(e), C.byref(l))
+    return bool(hit), e.value, l.value
+
+
+def tree_bound(verts, tris):
+    verts = np.ascontiguousarray(verts, dtype=np.float32)
+    tris = np.ascontiguousarray(tris, dtype=np.uint32)
+    a, g = (C.c_float * 3)(), (C.c_float * 3)()
+    lib().ora_tree_bound(verts.ctypes.data, tris.ctypes.data, tris.shape[0], a, g)
+    return np.array(a[:], dtype=np.float32), np.array(g[:], dtype=np.float32)
+
+
+def dynamic_bias(enter, leave) -> float:
+    return lib().ora_dynamic_bias(C.c_float(enter), C.c_float(leave))
+
+
+def _prep(verts, tris, vis):
+    verts = np.ascontiguousarray(verts, dtype=np.float32)
+    tris = np.ascontiguousarray(tris, dtype=np.uint32)
+    if vis is None:
+        vis = np.full(tris.shape[0], 3, dtype=np.uint8)
+    vis = np.ascontiguousarray(vis, dtype=np.uint8)
+    return verts, tris, vis
+
+
+def closest_brute(verts, tris, rays, vis=None, threads=None) -> dict:
+    verts, tris, vis = _prep(verts, tris, vis)
+    rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+    out = np.zeros((rays.shape[0], 4), dtype=np.uint32)
+    lib().ora_closest_brute(verts.ctypes.data, tris.ctypes.data, vis.ctypes.data, tris.shape[0],
+                            rays.ctypes.data, rays.shape[0], out.ctypes.data, threads or default_threads())
+    return _hits(out)
+
+
+def shadow_brute(verts, tris, rays, vis=None, threads=None) -> np.ndarray:
+    verts, tris, vis = _prep(verts, tris, vis)
+    rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+    occ = np.zeros(rays.shape[0], dtype=np.uint8)
+    lib().ora_shadow_brute(verts.ctypes.data, tris.ctypes.data, vis.ctypes.data, tris.shape[0],
+                           rays.ctypes.data, rays.shape[0], occ.ctypes.data, threads or default_threads())
+    return occ
+
+
+def retest(verts, tris, rays, prim) -> dict:
+    """Re-run the triangle test of each ray against its reported primitive."""
+    verts, tris, _ = _prep(verts, tris, None)
+    rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+    prim = np.ascontiguousarray(prim, dtype=np.uint32)
+    out = np.zeros((rays.shape[0], 4), dtype=np.uint32)
+    lib().ora_retest(verts.ctypes.data, tris.ctypes.data, rays.ctypes.data, prim.ctypes.data, rays.shape[0],
+                     out.ctypes.data)
+    return _hits(out)
+
+
+def _hits(out: np.ndarray) -> dict:
+    return {"prim": out[:, 0].copy(), "t": out[:, 1].view(np.float32).copy(),
+            "u": out[:, 2].view(np.float32).copy(), "v": out[:, 3].view(np.float32).copy()}
+
+
+class KdTree:
+    """Port of the reference's SAH kd-tree (CPU baseline and large-batch oracle)."""
+
+    def __init__(self, verts, tris, vis=None):
+        self.verts, self.tris, self.vis = _prep(verts, tris, vis)
+        self._h = lib().ora_kd_build(self.verts.ctypes.data, self.tris.ctypes.data, self.vis.ctypes.data,
+                                     self.tris.shape[0])
+        if not self._h:
+            raise MemoryError("ora_kd_build failed")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().ora_kd_free(self._h)
+            self._h = None
+
+    def bound(self):
+        a, g = (C.c_float * 3)(), (C.c_float * 3)()
+        lib().ora_kd_bound(self._h, a, g)
+        return np.array(a[:], dtype=np.float32), np.array(g[:], dtype=np.float32)
+
+    def num_nodes(self) -> int:
+        return int(lib().ora_kd_num_nodes(self._h))
+
+    def closest(self, rays, threads=None) -> dict:
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        out = np.zeros((rays.shape[0], 4), dtype=np.uint32)
+        lib().ora_kd_closest(self._h, rays.ctypes.data, rays.shape[0], out.ctypes.data,
+                             threads or default_threads())
+        return _hits(out)
+
+    def shadow(self, rays, threads=None):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        occ = np.zeros(rays.shape[0], dtype=np.uint8)
+        prim = np.zeros(rays.shape[0], dtype=np.uint32)
+        lib().ora_kd_shadow(self._h, rays.ctypes.data, rays.shape[0], occ.ctypes.data, prim.ctypes.data,
+                            threads or default_threads())
+        return occ, prim
+
+
+def fast_pow(a, b) -> float:
+    return lib().ora_fast_pow(C.c_float(a), C.c_float(b))

@@ -1,0 +1,263 @@
+"""GPU parity of the mi355x-lbvh accelerator against the CPU oracle, through
+the C ABI (libymx.so).
+
+Bar: closest-hit prim ids, t, u, v bit-exact against the brute-force oracle
+(both resolve exact ties in t to the lowest id); any-hit occlusion bits
+bit-exact.  Against the kd-tree port (ties in kd leaf order, like the
+reference) every differing id must be an exact tie in t.  At the full
+benchmark size the properties are size-independent: every reported hit
+re-tests bit-exactly, and shadow rays clipped at the closest hit agree with
+it."""
+import sys
+
+import numpy as np
+import pytest
+
+from libyafaray_amd import scenes
+from libyafaray_amd.accel import GpuAccelerator
+
+pytestmark = pytest.mark.gpu
+NO_HIT = 0xFFFFFFFF
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def bits(x):
+    return np.ascontiguousarray(x, dtype=np.float32).view(np.uint32)
+
+
+def assert_hits_equal(got, exp, what):
+    for k in ("prim",):
+        bad = np.nonzero(got[k] != exp[k])[0]
+        assert bad.size == 0, f"{what}: {bad.size} prim mismatches, first {bad[:5]} got {got[k][bad[:5]]} exp {exp[k][bad[:5]]}"
+    for k in ("t", "u", "v"):
+        bad = np.nonzero(bits(got[k]) != bits(exp[k]))[0]
+        assert bad.size == 0, f"{what}: {bad.size} {k} mismatches, first {bad[:5]}"
+
+
+def small_scene(seed=0, res=64, n_random=2000):
+    verts, tris = scenes.heightfield_scene(res=res, n_random=n_random, seed=seed)
+    rng = np.random.default_rng(seed + 100)
+    lo, hi = verts.min(0), verts.max(0)
+    rays = np.concatenate([
+        scenes.incoherent_rays(rng, 20000, lo, hi),
+        scenes.camera_rays(160, 80, **scenes.RAYCAST_CAMERA),
+        # origins outside the scene box, aimed anywhere
+        scenes.incoherent_rays(rng, 3000, lo - 2.0, hi + 2.0),
+    ])
+    return verts, tris, rays, rng
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_closest_bit_exact_small(oracle, seed):
+    verts, tris, rays, _ = small_scene(seed)
+    say(f"closest small seed={seed} tris={len(tris)} rays={len(rays)}")
+    acc = GpuAccelerator(verts, tris)
+    a, g = acc.bound()
+    ea, eg = oracle.tree_bound(verts, tris)
+    assert np.array_equal(a, ea) and np.array_equal(g, eg)
+    got = acc.intersect(rays)
+    exp = oracle.closest_brute(verts, tris, rays)
+    assert (exp["prim"] != NO_HIT).mean() > 0.3
+    assert_hits_equal(got, exp, "closest")
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_shadow_bit_exact_small(oracle, seed):
+    verts, tris, rays, rng = small_scene(seed)
+    rays = rays.copy()
+    rays[:, 7] = scenes.shadow_tmax(rng, len(rays))
+    say(f"shadow small seed={seed} rays={len(rays)}")
+    acc = GpuAccelerator(verts, tris)
+    occ, prim = acc.intersect_shadow(rays)
+    exp = oracle.shadow_brute(verts, tris, rays)
+    assert 0.05 < exp.mean() < 0.95
+    assert np.array_equal(occ, exp)
+    # every reported occluder really blocks the ray inside [bias, tmax)
+    assert np.all(prim[occ == 0] == NO_HIT)
+    r = oracle.retest(verts, tris, rays, np.where(occ == 1, prim, NO_HIT).astype(np.uint32))
+    t = r["t"][occ == 1]
+    assert np.all(r["prim"][occ == 1] == prim[occ == 1])
+    assert np.all((t > 0) & (t < rays[occ == 1, 7]))
+
+
+def test_visibility_masks(oracle):
+    verts, tris, rays, rng = small_scene(2, res=40, n_random=800)
+    vis = rng.integers(0, 4, size=len(tris)).astype(np.uint8)
+    say(f"visibility rays={len(rays)}")
+    acc = GpuAccelerator(verts, tris, vis)
+    assert_hits_equal(acc.intersect(rays), oracle.closest_brute(verts, tris, rays, vis=vis), "closest+vis")
+    srays = rays.copy()
+    srays[:, 7] = scenes.shadow_tmax(rng, len(rays)) * 2
+    occ, _ = acc.intersect_shadow(srays)
+    assert np.array_equal(occ, oracle.shadow_brute(verts, tris, srays, vis=vis))
+
+
+def test_ties_on_grid_vertices(oracle):
+    """Vertical rays through heightfield vertices and edges meet several
+    triangles at the same t; the lowest id must win, as in the oracle."""
+    verts, tris = scenes.heightfield_scene(res=33, n_random=0, seed=3)
+    g = np.linspace(0.0, 1.0, 65, dtype=np.float32)  # every vertex and edge midpoint
+    xy = np.stack(np.meshgrid(g, g, indexing="xy"), -1).reshape(-1, 2)
+    rays = np.zeros((len(xy), 8), np.float32)
+    rays[:, 0:2] = xy
+    rays[:, 2] = 1.0
+    rays[:, 6] = -1.0
+    rays[:, 7] = -1.0
+    say(f"ties rays={len(rays)}")
+    acc = GpuAccelerator(verts, tris)
+    got = acc.intersect(rays)
+    exp = oracle.closest_brute(verts, tris, rays)
+    assert_hits_equal(got, exp, "ties")
+
+
+@pytest.mark.parametrize("n_tris", [0, 1, 2, 3])
+def test_tiny_scenes(oracle, n_tris):
+    rng = np.random.default_rng(n_tris)
+    tv = scenes.random_triangles(rng, max(n_tris, 1), (0.3, 0.3, 0.3), (0.7, 0.7, 0.7), edge=(0.1, 0.3))[:n_tris]
+    verts, tris = scenes.soup(tv) if n_tris else (np.zeros((3, 3), np.float32), np.zeros((0, 3), np.uint32))
+    rays = scenes.incoherent_rays(rng, 5000, (0.2, 0.2, 0.2), (0.8, 0.8, 0.8))
+    say(f"tiny n_tris={n_tris}")
+    acc = GpuAccelerator(verts, tris)
+    got = acc.intersect(rays)
+    if n_tris == 0:
+        assert np.all(got["prim"] == NO_HIT)
+        occ, _ = acc.intersect_shadow(rays)
+        assert not occ.any()
+        return
+    assert_hits_equal(got, oracle.closest_brute(verts, tris, rays), f"tiny{n_tris}")
+    srays = rays.copy()
+    srays[:, 7] = scenes.shadow_tmax(rng, len(rays))
+    occ, _ = acc.intersect_shadow(srays)
+    assert np.array_equal(occ, oracle.shadow_brute(verts, tris, srays))
+
+
+@pytest.mark.parametrize("n_rays", [1, 63, 64, 65, 1000, 4097])
+def test_ragged_batches(oracle, n_rays):
+    verts, tris, rays, rng = small_scene(4, res=24, n_random=200)
+    rays = rays[:n_rays].copy()
+    say(f"ragged n_rays={n_rays}")
+    acc = GpuAccelerator(verts, tris)
+    assert_hits_equal(acc.intersect(rays), oracle.closest_brute(verts, tris, rays), "ragged")
+    rays[:, 7] = scenes.shadow_tmax(rng, len(rays))
+    occ, _ = acc.intersect_shadow(rays)
+    assert np.array_equal(occ, oracle.shadow_brute(verts, tris, rays))
+
+
+def test_empty_batch_and_bad_args():
+    import torch
+    from libyafaray_amd import native
+    verts, tris = scenes.heightfield_scene(res=8, n_random=0)
+    acc = GpuAccelerator(verts, tris)
+    r = torch.zeros((0, 8), dtype=torch.float32, device="cuda")
+    assert acc.intersect_device(r).shape == (0, 4)
+    with pytest.raises(native.YmxError):
+        GpuAccelerator(verts, np.array([[0, 1, 10_000]], np.uint32))
+    with pytest.raises(ValueError):
+        acc.intersect_device(torch.zeros((4, 7), dtype=torch.float32, device="cuda"))
+
+
+def test_axis_aligned_and_degenerate_rays(oracle):
+    verts, tris, _, rng = small_scene(5, res=32, n_random=500)
+    lo, hi = verts.min(0), verts.max(0)
+    n = 6000
+    rays = scenes.incoherent_rays(rng, n, lo, hi)
+    axis = rng.integers(0, 3, n)
+    sign = rng.choice([-1.0, 1.0], n)
+    rays[:, 4:7] = 0.0
+    rays[np.arange(n), 4 + axis] = sign
+    # a third keep one extra non-zero component
+    extra = rng.random(n) < 0.33
+    rays[extra, 4 + (axis[extra] + 1) % 3] = 0.5
+    say("axis-aligned rays")
+    acc = GpuAccelerator(verts, tris)
+    assert_hits_equal(acc.intersect(rays), oracle.closest_brute(verts, tris, rays), "axis")
+
+
+def test_deterministic_repeat():
+    verts, tris, rays, _ = small_scene(6)
+    acc = GpuAccelerator(verts, tris)
+    a = acc.intersect(rays)
+    b = acc.intersect(rays)
+    for k in a:
+        assert np.array_equal(bits(a[k]) if k != "prim" else a[k], bits(b[k]) if k != "prim" else b[k])
+
+
+def test_medium_scene_against_kdtree_port(oracle):
+    """~140k triangles: GPU vs the reference-style kd-tree (ties listed)."""
+    verts, tris = scenes.heightfield_scene(res=256, n_random=10_000, seed=7)
+    rng = np.random.default_rng(70)
+    lo, hi = verts.min(0), verts.max(0)
+    rays = np.concatenate([scenes.incoherent_rays(rng, 150_000, lo, hi),
+                           scenes.camera_rays(512, 256, **scenes.RAYCAST_CAMERA)])
+    say(f"medium tris={len(tris)} rays={len(rays)}")
+    acc = GpuAccelerator(verts, tris)
+    got = acc.intersect(rays)
+    kd = oracle.KdTree(verts, tris)
+    exp = kd.closest(rays)
+    diff = np.nonzero(got["prim"] != exp["prim"])[0]
+    say(f"  exact-tie ids (kd order vs lowest id): {diff.size}")
+    assert np.array_equal(bits(got["t"]), bits(exp["t"]))
+    assert diff.size <= 1e-3 * len(rays)
+    if diff.size:
+        # both candidates hit at the same t
+        r = oracle.retest(verts, tris, rays[diff], exp["prim"][diff])
+        assert np.array_equal(bits(r["t"]), bits(got["t"][diff]))
+        assert np.all(got["prim"][diff] < exp["prim"][diff])
+    # brute force on a subset settles the tie-break exactly
+    sub = rng.choice(len(rays), 4000, replace=False)
+    assert_hits_equal({k: v[sub] for k, v in got.items()}, oracle.closest_brute(verts, tris, rays[sub]), "medium")
+
+
+@pytest.mark.timeout(600)
+def test_full_raycast_workload_properties(oracle):
+    """The benchmark scene (~2.19M triangles) with all 16.8M rays."""
+    import torch
+    verts, tris, coherent, incoherent, shadow = scenes.raycast_workload()
+    say(f"full tris={len(tris)} rays={len(coherent) + len(incoherent)}")
+    acc = GpuAccelerator(verts, tris)
+    info = acc.info()
+    say(f"  nodes={info['n_nodes']} depth={info['max_depth']} build_ms={info['build_ms']:.1f}")
+    for name, rays in (("coherent", coherent), ("incoherent", incoherent)):
+        d_rays = torch.from_numpy(rays).cuda()
+        hits = acc.intersect_device(d_rays)
+        torch.cuda.synchronize()
+        h = hits.cpu().numpy()
+        prim = h[:, 0].view(np.uint32)
+        t = h[:, 1].view(np.float32)
+        hit = prim != NO_HIT
+        say(f"  {name}: hit fraction {hit.mean():.4f}")
+        # every reported hit re-tests bit-exactly
+        r = oracle.retest(verts, tris, rays, prim)
+        assert np.array_equal(r["prim"], prim)
+        assert np.array_equal(bits(r["t"]), h[:, 1].view(np.uint32))
+        assert np.array_equal(bits(r["u"]), h[:, 2].view(np.uint32))
+        # shadow rays clipped at the closest hit: nothing lies strictly before it
+        s = rays.copy()
+        s[hit, 7] = t[hit]
+        occ_at = acc.intersect_shadow_device(torch.from_numpy(s).cuda())
+        s[hit, 7] = np.nextafter(t[hit], np.float32(np.inf))
+        occ_after = acc.intersect_shadow_device(torch.from_numpy(s).cuda())
+        torch.cuda.synchronize()
+        from libyafaray_amd.accel import unpack_bits
+        at = unpack_bits(occ_at.cpu().numpy(), len(rays))
+        after = unpack_bits(occ_after.cpu().numpy(), len(rays))
+        assert not at[hit].any()
+        assert after[hit].all()
+        assert not after[~hit].any()
+        del d_rays, hits
+    # subset against the kd-tree port
+    rng = np.random.default_rng(9)
+    sub = rng.choice(len(incoherent), 300_000, replace=False)
+    kd = oracle.KdTree(verts, tris)
+    exp = kd.closest(incoherent[sub])
+    got = acc.intersect(incoherent[sub])
+    diff = np.nonzero(got["prim"] != exp["prim"])[0]
+    say(f"  kd-port subset: {diff.size} exact-tie ids of {len(sub)}")
+    assert np.array_equal(bits(got["t"]), bits(exp["t"]))
+    occ_g, _ = acc.intersect_shadow(shadow[sub])
+    occ_k, _ = kd.shadow(shadow[sub])
+    assert np.array_equal(occ_g, occ_k)
