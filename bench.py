@@ -45,7 +45,22 @@ def parse():
     p.add_argument("--res", type=int, default=1024)
     p.add_argument("--n-random", type=int, default=100_000)
     p.add_argument("--quiet", action="store_true")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="run only a plain 1 GiB device-to-device copy (HBM counter calibration)")
     return p.parse_args()
+
+
+def calibrate_copy():
+    """A plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters give the
+    scale between the counters and bytes moved."""
+    import torch
+    n = 1 << 28  # 1 GiB of float32
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": 4 * n, "launches": 3}), flush=True)
 
 
 def log(args, msg):
@@ -84,6 +99,9 @@ def main():
         print(json.dumps({"error": "no HIP device"}))
         sys.exit(1)
     torch.cuda.set_device(0)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
 
     t0 = time.perf_counter()
     verts, tris, coherent, incoherent, shadow = scenes.raycast_workload(args.res, args.n_random)
