@@ -21,6 +21,8 @@ SOURCES = [
     "accel_build.hip",
     "accel_trace.hip",
     "accel_api.hip",
+    "pt_kernels.hip",
+    "pt_api.hip",
 ]
 
 # -ffp-contract=off / IEEE division / denormals kept: FP32 results must match

@@ -59,7 +59,8 @@ struct ymx_trace_args
 	const float4 *nodes;
 	const float4 *tris;
 	const ymx_ray *rays;
-	uint32_t n_rays;
+	uint32_t n_rays;          // ray count, or the capacity when n_dev is set
+	const uint32_t *n_dev;    // optional device-side ray count (queues)
 	float bmin[3], bmax[3];
 	uint32_t *work;
 	int32_t *spill;
@@ -79,6 +80,11 @@ int ymx_launch_closest(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits,
 int ymx_launch_shadow(const ymx_accel *acc, const ymx_ray *rays, uint32_t *occ, uint32_t *prim,
                       uint32_t n, hipStream_t stream, unsigned long long *stats);
 int ymx_trace_setup(ymx_accel *acc);
+// Queue variants: the ray count is read on the device from *n_dev (<= capacity).
+int ymx_launch_closest_queue(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, const uint32_t *n_dev,
+                             uint32_t capacity, hipStream_t stream);
+int ymx_launch_shadow_queue(const ymx_accel *acc, const ymx_ray *rays, uint32_t *occ, const uint32_t *n_dev,
+                            uint32_t capacity, hipStream_t stream);
 
 void ymx_set_error(const char *fmt, ...);
 

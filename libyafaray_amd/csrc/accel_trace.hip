@@ -215,13 +215,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(ymx_trace_args args)
 	st.cap = args.spill_depth;
 	st.overflow = false;
 	unsigned long long n_fetch = 0, n_tests = 0;
+	const uint32_t n_rays = args.n_dev ? *args.n_dev : args.n_rays;
 
 	for(;;)
 	{
 		const uint32_t base = wave_fetch(args.work);
-		if(base >= args.n_rays) break;
+		if(base >= n_rays) break;
 		const uint32_t rid = base + __lane_id();
-		const bool valid = rid < args.n_rays;
+		const bool valid = rid < n_rays;
 
 		ymx_ray ray = {};
 		if(valid) ray = load_ray(args.rays, rid);
@@ -311,7 +312,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(ymx_trace_args args)
 			if(__lane_id() == 0)
 			{
 				const uint32_t w = base >> 5;
-				const uint32_t n_words = (args.n_rays + 31u) >> 5;
+				const uint32_t n_words = (n_rays + 31u) >> 5;
 				args.occluded[w] = (uint32_t)mask;
 				if(w + 1 < n_words) args.occluded[w + 1] = (uint32_t)(mask >> 32);
 			}
@@ -358,7 +359,7 @@ __global__ void k_reset_work(uint32_t *work) { work[0] = 0u; }
 
 template <bool kShadow, bool kStats>
 int launch(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t *occ, uint32_t *prim, uint32_t n,
-           hipStream_t stream, unsigned long long *stats)
+           hipStream_t stream, unsigned long long *stats, const uint32_t *n_dev = nullptr)
 {
 	if(n == 0) return YMX_OK;
 	ymx_trace_args a;
@@ -366,6 +367,7 @@ int launch(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t *o
 	a.tris = acc->d_tris;
 	a.rays = rays;
 	a.n_rays = n;
+	a.n_dev = n_dev;
 	for(int k = 0; k < 3; ++k)
 	{
 		a.bmin[k] = acc->bmin[k];
@@ -418,6 +420,18 @@ int ymx_launch_closest(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits,
 {
 	if(stats) return launch<false, true>(acc, rays, hits, nullptr, nullptr, n, stream, stats);
 	return launch<false, false>(acc, rays, hits, nullptr, nullptr, n, stream, nullptr);
+}
+
+int ymx_launch_closest_queue(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, const uint32_t *n_dev,
+                             uint32_t capacity, hipStream_t stream)
+{
+	return launch<false, false>(acc, rays, hits, nullptr, nullptr, capacity, stream, nullptr, n_dev);
+}
+
+int ymx_launch_shadow_queue(const ymx_accel *acc, const ymx_ray *rays, uint32_t *occ, const uint32_t *n_dev,
+                            uint32_t capacity, hipStream_t stream)
+{
+	return launch<true, false>(acc, rays, nullptr, occ, nullptr, capacity, stream, nullptr, n_dev);
 }
 
 int ymx_launch_shadow(const ymx_accel *acc, const ymx_ray *rays, uint32_t *occ, uint32_t *prim, uint32_t n,

@@ -1,0 +1,435 @@
+// pt_api.hip -- host side of the path tracer: the extern "C" entry points of
+// include/ymx_pt.h, scene constants, and the per-pass launch loop.
+#include "pt_internal.h"
+
+#include <float.h>
+#include <math.h>
+#include <string.h>
+
+#include <new>
+#include <vector>
+
+using namespace ymx;
+
+namespace {
+
+// Faure's scrambling permutations (H. Faure, J. Number Theory 42, 1992):
+// sigma_2 = (0 1); sigma_{2c}(i) = 2 sigma_c(i), sigma_{2c}(c + i) = 2 sigma_c(i) + 1;
+// sigma_{2c+1}(c) = c, the rest is sigma_{2c} with values >= c moved up by one.
+std::vector<std::vector<int>> faure_permutations(int max_base)
+{
+	std::vector<std::vector<int>> t(max_base + 1);
+	t[1] = {0};
+	t[2] = {0, 1};
+	for(int b = 3; b <= max_base; ++b)
+	{
+		t[b].resize(b);
+		if(b % 2 == 0)
+		{
+			const int c = b / 2;
+			for(int i = 0; i < c; ++i)
+			{
+				t[b][i] = 2 * t[c][i];
+				t[b][c + i] = 2 * t[c][i] + 1;
+			}
+		}
+		else
+		{
+			const int c = (b - 1) / 2;
+			for(int i = 0; i < b; ++i)
+			{
+				if(i == c)
+				{
+					t[b][i] = c;
+					continue;
+				}
+				const int s = t[b - 1][i < c ? i : i - 1];
+				t[b][i] = s >= c ? s + 1 : s;
+			}
+		}
+	}
+	return t;
+}
+
+// The 50 bases of Halton::lowDiscrepancySampling (src/sampler/halton.cc:401-403).
+constexpr int kBases[kLdsDims] = {1, 2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53,
+                                  59, 61, 67, 71, 73, 79, 83, 89, 97, 101, 103, 107, 109, 113, 127, 131,
+                                  137, 139, 149, 151, 157, 163, 167, 173, 179, 181, 191, 193, 197, 199, 211, 223, 227};
+
+void free_pt(ymx_pt *pt)
+{
+	if(!pt) return;
+	void *ptrs[] = {pt->d_prim, pt->d_materials, pt->d_lights, pt->d_const, pt->d_faure, pt->d_inv_prims, pt->d_prims,
+	                pt->d_film, pt->d_film_w, pt->st.col, pt->st.path_col, pt->st.tp_next, pt->st.tp_vertex,
+	                pt->st.pwo, pt->st.cam_dir, pt->st.cam_hit, pt->st.dxy, pt->st.offset, pt->st.flags, pt->st.light,
+	                pt->st.vert_mat, pt->st.ext_rays[0], pt->st.ext_rays[1], pt->st.ext_path[0], pt->st.ext_path[1],
+	                pt->st.ext_count, pt->st.ext_hits, pt->st.slot_contrib, pt->st.slot_queue, pt->st.sh_rays,
+	                pt->st.sh_occ, pt->st.sh_count, pt->st.totals};
+	for(void *p : ptrs)
+		if(p) (void)hipFree(p);
+	if(pt->ev0) (void)hipEventDestroy(pt->ev0);
+	if(pt->ev1) (void)hipEventDestroy(pt->ev1);
+	delete pt;
+}
+
+template <typename T> hipError_t dalloc(T *&p, size_t count) { return hipMalloc((void **)&p, sizeof(T) * (count ? count : 1)); }
+
+// Gauss filter table of ImageFilm (src/render/imagefilm.cc:231-264, include/math/filter.h:32-37).
+void filter_table(float pixelwidth, PtConst &c)
+{
+	float fw = pixelwidth * 0.5f;
+	fw *= 2.f;
+	if(fw < 0.501f) fw = 0.501f;
+	if(fw > 0.5f * 8.f) fw = 0.5f * 8.f;
+	const float scale = 1.f / 16.f;
+	int k = 0;
+	for(int y = 0; y < 16; ++y)
+		for(int x = 0; x < 16; ++x)
+		{
+			const float dx = ((float)x + .5f) * scale, dy = ((float)y + .5f) * scale;
+			const float r_2 = dx * dx + dy * dy;
+			const float v = fast_exp((float)-6 * r_2) - 0.00247875f;
+			c.filter_table[k++] = v > 0.f ? v : 0.f;
+		}
+	c.filter_table_scale = 0.9999f * 16.f / fw;
+	c.filter_width = fw;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, const uint32_t *tri_idx,
+                  const uint32_t *mat_id, uint32_t n_tris, const ymx_material *materials, uint32_t n_materials,
+                  const ymx_light *lights, uint32_t n_lights, const ymx_pt_settings *st, ymx_pt **out)
+{
+	if(!out || !accel || !st || (n_tris && (!verts || !tri_idx || !mat_id)) || (n_materials && !materials) ||
+	   (n_lights && !lights))
+	{
+		ymx_set_error("ymx_pt_create: NULL argument");
+		return YMX_ERR_INVALID_ARG;
+	}
+	*out = nullptr;
+	ymx_accel_info info;
+	info.struct_size = sizeof info;
+	if(ymx_accel_get_info(accel, &info) != YMX_OK || info.n_tris != n_tris)
+	{
+		ymx_set_error("ymx_pt_create: accelerator holds %u triangles, %u given", info.n_tris, n_tris);
+		return YMX_ERR_INVALID_ARG;
+	}
+	if(st->resx <= 0 || st->resy <= 0 || (int64_t)st->resx * st->resy > (1 << 26) || st->bounces < 1 ||
+	   st->bounces > 12 || st->aa_samples < 1 || st->path_samples < 0 || n_lights > (uint32_t)kMaxLights)
+	{
+		ymx_set_error("ymx_pt_create: unsupported settings (res %dx%d, bounces %d, AA %d, lights %u)", st->resx,
+		              st->resy, st->bounces, st->aa_samples, n_lights);
+		return YMX_ERR_UNSUPPORTED;
+	}
+	for(uint32_t i = 0; i < 3 * (size_t)n_tris; ++i)
+		if(tri_idx[i] >= n_verts)
+		{
+			ymx_set_error("ymx_pt_create: vertex index out of range");
+			return YMX_ERR_INVALID_ARG;
+		}
+	for(uint32_t i = 0; i < n_tris; ++i)
+		if(mat_id[i] >= n_materials)
+		{
+			ymx_set_error("ymx_pt_create: triangle %u uses material %u of %u", i, mat_id[i], n_materials);
+			return YMX_ERR_INVALID_ARG;
+		}
+	for(uint32_t m = 0; m < n_materials; ++m)
+		if(materials[m].type != YMX_MAT_SHINYDIFFUSE)
+		{
+			ymx_set_error("ymx_pt_create: material %u has unsupported type %u", m, materials[m].type);
+			return YMX_ERR_UNSUPPORTED;
+		}
+	for(uint32_t l = 0; l < n_lights; ++l)
+		if((lights[l].type != YMX_LIGHT_AREA && lights[l].type != YMX_LIGHT_POINT) ||
+		   (lights[l].type == YMX_LIGHT_AREA && (lights[l].samples < 1 || lights[l].samples > 64)))
+		{
+			ymx_set_error("ymx_pt_create: light %u unsupported (type %u, samples %d)", l, lights[l].type,
+			              lights[l].samples);
+			return YMX_ERR_UNSUPPORTED;
+		}
+
+	ymx_pt *pt = new(std::nothrow) ymx_pt();
+	if(!pt)
+	{
+		ymx_set_error("ymx_pt_create: host allocation failed");
+		return YMX_ERR_OUT_OF_MEMORY;
+	}
+	memset(pt, 0, sizeof *pt);
+	pt->accel = accel;
+	pt->n_tris = n_tris;
+	pt->n_materials = n_materials;
+	(void)hipGetDevice(&pt->device);
+
+#define PT_TRY(expr)                      \
+	do                                    \
+	{                                     \
+		hipError_t e_ = (expr);           \
+		if(e_ != hipSuccess)              \
+		{                                 \
+			ymx_set_error("%s: %s", #expr, hipGetErrorString(e_)); \
+			free_pt(pt);                  \
+			return e_ == hipErrorOutOfMemory ? YMX_ERR_OUT_OF_MEMORY : YMX_ERR_HIP; \
+		}                                 \
+	} while(0)
+
+	// per-primitive face normal ((v1 - v0) ^ (v2 - v0)).normalize() and material
+	// (include/geometry/shape/shape_polygon.h:58-62)
+	std::vector<float4> prim(n_tris ? n_tris : 1);
+	for(uint32_t i = 0; i < n_tris; ++i)
+	{
+		const V3 p0 = load3(verts + 3 * (size_t)tri_idx[3 * (size_t)i]);
+		const V3 p1 = load3(verts + 3 * (size_t)tri_idx[3 * (size_t)i + 1]);
+		const V3 p2 = load3(verts + 3 * (size_t)tri_idx[3 * (size_t)i + 2]);
+		const V3 ng = normalize(cross(sub(p1, p0), sub(p2, p0)));
+		uint32_t m = mat_id[i];
+		float fm;
+		memcpy(&fm, &m, 4);
+		prim[i] = make_float4(ng.x, ng.y, ng.z, fm);
+	}
+	PT_TRY(dalloc(pt->d_prim, prim.size()));
+	PT_TRY(hipMemcpy(pt->d_prim, prim.data(), sizeof(float4) * prim.size(), hipMemcpyHostToDevice));
+
+	// materials (src/material/material_shiny_diffuse.cc:131-183)
+	std::vector<PtMaterial> mats(n_materials ? n_materials : 1);
+	for(uint32_t m = 0; m < n_materials; ++m)
+	{
+		const ymx_material &s = materials[m];
+		PtMaterial &d = mats[m];
+		d.color = rgb(s.color[0], s.color[1], s.color[2]);
+		d.diffuse_reflect = s.diffuse_reflect;
+		d.emit_color = cscale(d.color, s.emit);
+		d.flags = 0;
+		if(s.diffuse_reflect > 0.00001f) d.flags |= kDiffuse | kReflect;
+		if(s.emit > 0.f) d.flags |= kEmit;
+		d.receive_shadows = s.receive_shadows;
+	}
+	PT_TRY(dalloc(pt->d_materials, mats.size()));
+	PT_TRY(hipMemcpy(pt->d_materials, mats.data(), sizeof(PtMaterial) * mats.size(), hipMemcpyHostToDevice));
+
+	// lights (src/light/light_area.cc, include/light/light_area.h:70-80, include/light/light_point.h:64)
+	std::vector<PtLight> lts(n_lights ? n_lights : 1);
+	uint32_t slots_camera = 0, slots_path = 0;
+	for(uint32_t l = 0; l < n_lights; ++l)
+	{
+		const ymx_light &s = lights[l];
+		PtLight &d = lts[l];
+		memset(&d, 0, sizeof d);
+		d.type = s.type;
+		d.cast_shadows = s.cast_shadows;
+		d.samples = s.samples;
+		const Rgb cp = cscale(rgb(s.color[0], s.color[1], s.color[2]), s.power);
+		if(s.type == YMX_LIGHT_AREA)
+		{
+			const V3 c0 = load3(s.corner), p1 = load3(s.point1), p2 = load3(s.point2);
+			d.color = cscale(cp, kPi);
+			d.corner = c0;
+			d.to_x = sub(p1, c0);
+			d.to_y = sub(p2, c0);
+			d.normal_flipped = cross(sub(p2, c0), sub(p1, c0));
+			const V3 nn = normalize(d.normal_flipped);
+			d.area = sqrtf(dot(nn, nn));
+			d.quad[0] = c0;
+			d.quad[1] = p1;
+			d.quad[2] = sub(add(p1, p2), c0);
+			d.quad[3] = p2;
+		}
+		else
+		{
+			d.color = cp;
+			d.from = load3(s.from);
+		}
+		const uint32_t k = s.type == YMX_LIGHT_AREA ? 2u * (uint32_t)s.samples : 1u;
+		slots_camera += k;
+		slots_path = k > slots_path ? k : slots_path;
+	}
+	PT_TRY(dalloc(pt->d_lights, lts.size()));
+	PT_TRY(hipMemcpy(pt->d_lights, lts.data(), sizeof(PtLight) * lts.size(), hipMemcpyHostToDevice));
+
+	// constants: camera (src/camera/camera.cc:89-104, camera_perspective.cc:97-111)
+	PtConst &c = pt->host_const;
+	memset(&c, 0, sizeof c);
+	{
+		const V3 from = load3(st->from);
+		V3 cy = sub(load3(st->up), from);
+		V3 cz = sub(load3(st->to), from);
+		V3 cx = cross(cz, cy);
+		cy = cross(cz, cx);
+		cx = normalize(cx);
+		cy = normalize(cy);
+		cz = normalize(cz);
+		const float aspect = st->aspect_factor * (float)st->resy / (float)st->resx;
+		c.vright = cx;
+		c.vup = mul(aspect, cy);
+		c.vto = sub(mul(st->focal, cz), mul(0.5f, add(c.vup, c.vright)));
+		c.vup = v3(c.vup.x / (float)st->resy, c.vup.y / (float)st->resy, c.vup.z / (float)st->resy);
+		c.vright = v3(c.vright.x / (float)st->resx, c.vright.y / (float)st->resx, c.vright.z / (float)st->resx);
+		c.cam_from = from;
+	}
+	c.resx = st->resx;
+	c.resy = st->resy;
+	c.path_samples = st->path_samples;
+	c.bounces = st->bounces;
+	c.rr_min_bounces = st->rr_min_bounces;
+	c.aa_samples = st->aa_samples;
+	c.n_lights = n_lights;
+	c.slots_camera = slots_camera;
+	c.slots_path = slots_path;
+	c.has_background = st->has_background ? 1u : 0u;
+	c.background = rgb(st->background[0], st->background[1], st->background[2]);
+	c.seed = st->seed;
+	filter_table(st->aa_pixelwidth, c);
+	PT_TRY(dalloc(pt->d_const, 1));
+	PT_TRY(hipMemcpy(pt->d_const, &c, sizeof c, hipMemcpyHostToDevice));
+
+	// low-discrepancy tables (src/sampler/halton.cc:25-440)
+	{
+		const auto perms = faure_permutations(kLdsMaxBase);
+		std::vector<int32_t> faure((size_t)kLdsDims * kLdsMaxBase, 0);
+		std::vector<double> inv(kLdsDims);
+		std::vector<uint32_t> bases(kLdsDims);
+		for(int d = 0; d < kLdsDims; ++d)
+		{
+			const int b = kBases[d];
+			const int table = d <= 2 ? 3 : b;  // the first three dimensions share base 3's table
+			for(int i = 0; i < (int)perms[table].size() && i < kLdsMaxBase; ++i) faure[(size_t)d * kLdsMaxBase + i] = perms[table][i];
+			inv[d] = floor(1e9 / (double)b + 0.5) / 1e9;  // 1/p rounded to nine decimals
+			bases[d] = (uint32_t)b;
+		}
+		PT_TRY(dalloc(pt->d_faure, faure.size()));
+		PT_TRY(hipMemcpy(pt->d_faure, faure.data(), sizeof(int32_t) * faure.size(), hipMemcpyHostToDevice));
+		PT_TRY(dalloc(pt->d_inv_prims, inv.size()));
+		PT_TRY(hipMemcpy(pt->d_inv_prims, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice));
+		PT_TRY(dalloc(pt->d_prims, bases.size()));
+		PT_TRY(hipMemcpy(pt->d_prims, bases.data(), sizeof(uint32_t) * bases.size(), hipMemcpyHostToDevice));
+	}
+
+	// wavefront state
+	const uint32_t n = (uint32_t)(st->resx * st->resy);
+	PtState &s = pt->st;
+	s.capacity = n;
+	s.slots = slots_camera > slots_path ? slots_camera : slots_path;
+	if(s.slots == 0) s.slots = 1;
+	const size_t slot_cap = (size_t)n * s.slots;
+	if(slot_cap > 0xFFFFFFFFull)
+	{
+		ymx_set_error("ymx_pt_create: %zu shadow slots per wavefront exceed 2^32", slot_cap);
+		free_pt(pt);
+		return YMX_ERR_UNSUPPORTED;
+	}
+	PT_TRY(dalloc(pt->d_film, n));
+	PT_TRY(dalloc(pt->d_film_w, n));
+	PT_TRY(hipMemset(pt->d_film, 0, sizeof(float4) * n));
+	PT_TRY(hipMemset(pt->d_film_w, 0, sizeof(float) * n));
+	PT_TRY(dalloc(s.col, n));
+	PT_TRY(dalloc(s.path_col, n));
+	PT_TRY(dalloc(s.tp_next, n));
+	PT_TRY(dalloc(s.tp_vertex, n));
+	PT_TRY(dalloc(s.pwo, n));
+	PT_TRY(dalloc(s.cam_dir, n));
+	PT_TRY(dalloc(s.cam_hit, n));
+	PT_TRY(dalloc(s.dxy, n));
+	PT_TRY(dalloc(s.offset, n));
+	PT_TRY(dalloc(s.flags, n));
+	PT_TRY(dalloc(s.light, n));
+	PT_TRY(dalloc(s.vert_mat, n));
+	PT_TRY(dalloc(s.ext_rays[0], n));
+	PT_TRY(dalloc(s.ext_rays[1], n));
+	PT_TRY(dalloc(s.ext_path[0], n));
+	PT_TRY(dalloc(s.ext_path[1], n));
+	PT_TRY(dalloc(s.ext_count, 2));
+	PT_TRY(hipMemset(s.ext_count, 0, 2 * sizeof(uint32_t)));
+	PT_TRY(dalloc(s.ext_hits, n));
+	PT_TRY(dalloc(s.slot_contrib, slot_cap));
+	PT_TRY(dalloc(s.slot_queue, slot_cap));
+	PT_TRY(dalloc(s.sh_rays, slot_cap));
+	PT_TRY(dalloc(s.sh_occ, (slot_cap + 31) / 32 + 2));
+	PT_TRY(dalloc(s.sh_count, 1));
+	PT_TRY(hipMemset(s.sh_count, 0, sizeof(uint32_t)));
+	PT_TRY(dalloc(s.totals, 3));
+	PT_TRY(hipMemset(s.totals, 0, 3 * sizeof(unsigned long long)));
+	PT_TRY(hipEventCreate(&pt->ev0));
+	PT_TRY(hipEventCreate(&pt->ev1));
+#undef PT_TRY
+	*out = pt;
+	return YMX_OK;
+}
+
+int ymx_pt_destroy(ymx_pt *pt)
+{
+	if(!pt) return YMX_OK;
+	(void)hipDeviceSynchronize();
+	free_pt(pt);
+	return YMX_OK;
+}
+
+int ymx_pt_clear_film(ymx_pt *pt, void *stream)
+{
+	if(!pt)
+	{
+		ymx_set_error("ymx_pt_clear_film: NULL handle");
+		return YMX_ERR_INVALID_ARG;
+	}
+	const uint32_t n = pt->st.capacity;
+	hipStream_t s = (hipStream_t)stream;
+	YMX_HIP_CHECK(hipMemsetAsync(pt->d_film, 0, sizeof(float4) * n, s));
+	YMX_HIP_CHECK(hipMemsetAsync(pt->d_film_w, 0, sizeof(float) * n, s));
+	return YMX_OK;
+}
+
+int ymx_pt_render(ymx_pt *pt, int32_t first_sample, int32_t n_samples, void *stream)
+{
+	if(!pt || first_sample < 0 || n_samples < 0)
+	{
+		ymx_set_error("ymx_pt_render: bad arguments");
+		return YMX_ERR_INVALID_ARG;
+	}
+	hipStream_t s = (hipStream_t)stream;
+	YMX_HIP_CHECK(hipMemsetAsync(pt->st.totals, 0, 3 * sizeof(unsigned long long), s));
+	YMX_HIP_CHECK(hipEventRecord(pt->ev0, s));
+	for(int32_t k = 0; k < n_samples; ++k)
+	{
+		const int rc = ymx_pt_trace_pass(pt, first_sample + k, s);
+		if(rc != YMX_OK) return rc;
+	}
+	YMX_HIP_CHECK(hipEventRecord(pt->ev1, s));
+	pt->stats.struct_size = sizeof(ymx_pt_stats);
+	return YMX_OK;
+}
+
+int ymx_pt_read_film(ymx_pt *pt, float *rgba, float *weight)
+{
+	if(!pt || !rgba || !weight)
+	{
+		ymx_set_error("ymx_pt_read_film: NULL argument");
+		return YMX_ERR_INVALID_ARG;
+	}
+	YMX_HIP_CHECK(hipDeviceSynchronize());
+	const uint32_t n = pt->st.capacity;
+	YMX_HIP_CHECK(hipMemcpy(rgba, pt->d_film, sizeof(float4) * n, hipMemcpyDeviceToHost));
+	YMX_HIP_CHECK(hipMemcpy(weight, pt->d_film_w, sizeof(float) * n, hipMemcpyDeviceToHost));
+	return YMX_OK;
+}
+
+int ymx_pt_get_stats(const ymx_pt *pt, ymx_pt_stats *stats)
+{
+	if(!pt || !stats || stats->struct_size != sizeof(ymx_pt_stats))
+	{
+		ymx_set_error("ymx_pt_get_stats: NULL argument or struct_size mismatch");
+		return YMX_ERR_INVALID_ARG;
+	}
+	YMX_HIP_CHECK(hipEventSynchronize(pt->ev1));
+	unsigned long long t[3] = {0, 0, 0};
+	YMX_HIP_CHECK(hipMemcpy(t, pt->st.totals, sizeof t, hipMemcpyDeviceToHost));
+	float ms = 0.f;
+	YMX_HIP_CHECK(hipEventElapsedTime(&ms, pt->ev0, pt->ev1));
+	stats->camera_rays = t[0];
+	stats->extension_rays = t[1];
+	stats->shadow_rays = t[2];
+	stats->render_ms = ms;
+	return YMX_OK;
+}
+
+}  // extern "C"

@@ -44,8 +44,21 @@ class YmxAccelInfo(C.Structure):
     ]
 
 
-# Every symbol include/ymx_accel.h declares, with its ctypes signature.
+class YmxPtStats(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("camera_rays", C.c_uint64), ("extension_rays", C.c_uint64),
+                ("shadow_rays", C.c_uint64), ("render_ms", C.c_float)]
+
+
+# Every symbol include/ymx_accel.h and include/ymx_pt.h declare, with its ctypes signature.
 SIGNATURES = {
+    "ymx_pt_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
+                                C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                C.POINTER(C.c_void_p)]),
+    "ymx_pt_destroy": (C.c_int, [C.c_void_p]),
+    "ymx_pt_clear_film": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "ymx_pt_render": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
+    "ymx_pt_read_film": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ymx_pt_get_stats": (C.c_int, [C.c_void_p, C.POINTER(YmxPtStats)]),
     "ymx_accel_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
                                    C.POINTER(YmxAccelParams), C.POINTER(C.c_void_p)]),
     "ymx_accel_destroy": (C.c_int, [C.c_void_p]),

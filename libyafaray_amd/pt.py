@@ -1,0 +1,137 @@
+"""Python handle on the wavefront path tracer (C ABI in include/ymx_pt.h).
+
+Mirrors the render side of libYafaRay's `pathtracing` SurfaceIntegrator:
+`render(first, n)` traces AA samples like TiledIntegrator::renderPass and
+`film()` returns ImageFilm's colour and weight sums.  Scenes are the dicts
+built by libyafaray_amd.scenes (geometry, materials, lights, camera,
+settings with the reference's param names).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import native
+from .accel import GpuAccelerator
+from .native import check
+
+
+class YmxMaterial(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("color", C.c_float * 3), ("diffuse_reflect", C.c_float),
+                ("emit", C.c_float), ("receive_shadows", C.c_uint32)]
+
+
+class YmxLight(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("corner", C.c_float * 3), ("point1", C.c_float * 3),
+                ("point2", C.c_float * 3), ("from_", C.c_float * 3), ("color", C.c_float * 3),
+                ("power", C.c_float), ("samples", C.c_int32), ("cast_shadows", C.c_uint32)]
+
+
+class YmxPtSettings(C.Structure):
+    _fields_ = [("from_", C.c_float * 3), ("to", C.c_float * 3), ("up", C.c_float * 3), ("focal", C.c_float),
+                ("aspect_factor", C.c_float), ("resx", C.c_int32), ("resy", C.c_int32),
+                ("path_samples", C.c_int32), ("bounces", C.c_int32), ("rr_min_bounces", C.c_int32),
+                ("aa_samples", C.c_int32), ("aa_pixelwidth", C.c_float), ("has_background", C.c_int32),
+                ("background", C.c_float * 3), ("seed", C.c_uint32)]
+
+
+YmxPtStats = native.YmxPtStats
+
+
+def _f3(v):
+    return (C.c_float * 3)(*[float(x) for x in v])
+
+
+def build_structs(scene):
+    mats = (YmxMaterial * max(1, len(scene["materials"])))()
+    for i, m in enumerate(scene["materials"]):
+        mats[i].type = 0
+        mats[i].color = _f3(m["color"])
+        mats[i].diffuse_reflect = float(m.get("diffuse_reflect", 1.0))
+        mats[i].emit = float(m.get("emit", 0.0))
+        mats[i].receive_shadows = 1 if m.get("receive_shadows", True) else 0
+    lights = (YmxLight * max(1, len(scene["lights"])))()
+    for i, l in enumerate(scene["lights"]):
+        lights[i].type = 0 if l["type"] == "area" else 1
+        for k in ("corner", "point1", "point2"):
+            setattr(lights[i], k, _f3(l.get(k, (0, 0, 0))))
+        lights[i].from_ = _f3(l.get("from", (0, 0, 0)))
+        lights[i].color = _f3(l.get("color", (1, 1, 1)))
+        lights[i].power = float(l.get("power", 1.0))
+        lights[i].samples = int(l.get("samples", 4))
+        lights[i].cast_shadows = 1 if l.get("cast_shadows", True) else 0
+    cam, s = scene["camera"], scene["settings"]
+    st = YmxPtSettings()
+    st.from_ = _f3(cam["from_"])
+    st.to = _f3(cam["to"])
+    st.up = _f3(cam["up"])
+    st.focal = float(cam["focal"])
+    st.aspect_factor = float(cam.get("aspect_factor", 1.0))
+    st.resx, st.resy = int(cam["resx"]), int(cam["resy"])
+    st.path_samples = int(s["path_samples"])
+    st.bounces = int(s["bounces"])
+    st.rr_min_bounces = int(s["rr_min_bounces"])
+    st.aa_samples = int(s["aa_samples"])
+    st.aa_pixelwidth = float(s.get("aa_pixelwidth", 1.5))
+    bg = s.get("background")
+    st.has_background = 0 if bg is None else 1
+    st.background = _f3(bg if bg is not None else (0, 0, 0))
+    st.seed = int(s.get("seed", 0))
+    return mats, lights, st
+
+
+class GpuPathTracer:
+    """Path tracer over a GpuAccelerator of the same triangles."""
+
+    def __init__(self, scene, accel: GpuAccelerator | None = None):
+        self._lib = native.lib()
+        self.verts = np.ascontiguousarray(scene["verts"], dtype=np.float32)
+        self.tris = np.ascontiguousarray(scene["tris"], dtype=np.uint32)
+        self.mat_id = np.ascontiguousarray(scene["mat_id"], dtype=np.uint32)
+        self.accel = accel or GpuAccelerator(self.verts, self.tris)
+        self.mats, self.lights, self.st = build_structs(scene)
+        self.resx, self.resy = self.st.resx, self.st.resy
+        h = C.c_void_p()
+        rc = self._lib.ymx_pt_create(self.accel.handle, self.verts.ctypes.data, self.verts.shape[0],
+                                     self.tris.ctypes.data, self.mat_id.ctypes.data, self.tris.shape[0],
+                                     C.addressof(self.mats), len(scene["materials"]), C.addressof(self.lights),
+                                     len(scene["lights"]), C.addressof(self.st), C.byref(h))
+        check(rc, "ymx_pt_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.ymx_pt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _stream():
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+
+    def clear(self):
+        check(self._lib.ymx_pt_clear_film(self._h, self._stream()), "ymx_pt_clear_film")
+
+    def render(self, first_sample: int = 0, n_samples: int | None = None):
+        n = self.st.aa_samples if n_samples is None else n_samples
+        check(self._lib.ymx_pt_render(self._h, int(first_sample), int(n), self._stream()), "ymx_pt_render")
+
+    def film(self):
+        rgba = np.zeros((self.resy, self.resx, 4), dtype=np.float32)
+        w = np.zeros((self.resy, self.resx), dtype=np.float32)
+        check(self._lib.ymx_pt_read_film(self._h, rgba.ctypes.data, w.ctypes.data), "ymx_pt_read_film")
+        return rgba, w
+
+    def stats(self) -> dict:
+        s = YmxPtStats()
+        s.struct_size = C.sizeof(YmxPtStats)
+        check(self._lib.ymx_pt_get_stats(self._h, C.byref(s)), "ymx_pt_get_stats")
+        return {"camera_rays": s.camera_rays, "extension_rays": s.extension_rays, "shadow_rays": s.shadow_rays,
+                "render_ms": s.render_ms}

@@ -228,3 +228,34 @@ def raycast_workload(res: int = 1024, n_random: int = 100_000, cam_res=(4096, 20
     shadow = incoherent.copy()
     shadow[:, 7] = shadow_tmax(rng, n_inc)
     return verts, tris, coherent, incoherent, shadow
+
+
+# ---------------------------------------------------------------- path tracing
+def cornell_path_scene(n_random: int = 10_000, seed: int = 0, res: int = 256, spp: int = 16):
+    """Cornell-style box + random triangles lit by one 0.2 x 0.2 ceiling area light
+    of radiance 10 (256 x 256, 16 spp, 5 bounces, Russian roulette from depth 3).
+
+    Materials are shinydiffuse (Lambert): walls 0.7 grey, left wall red, right
+    wall green, each random triangle its own grey albedo U[0.2, 0.8].  The area
+    light emits colour * power * pi (src/light/light_area.h:78) and its pdf
+    carries the quad area through the unnormalised normal (:76), so radiance L
+    corresponds to colour * power = L / pi with the reference's pi-free Lambert.
+    It is a pure light (no mesh), so camera and path rays do not see it.
+    """
+    rng = np.random.default_rng(seed)
+    wv, wt, wall = cornell_walls()
+    tv = random_triangles(rng, n_random, (0.2, 0.2, 0.2), (0.8, 0.8, 0.8))
+    verts, tris = merge([(wv, wt), soup(tv)])
+    wall_mat = np.array([0, 0, 0, 1, 2])  # floor, ceiling, back grey; left red; right green
+    mat_id = np.concatenate([wall_mat[wall], 3 + np.arange(n_random)]).astype(np.uint32)
+    albedo = rng.uniform(0.2, 0.8, n_random)
+    materials = [dict(color=(0.7, 0.7, 0.7)), dict(color=(0.7, 0.1, 0.1)), dict(color=(0.1, 0.7, 0.1))]
+    materials += [dict(color=(a, a, a)) for a in albedo]
+    lights = [dict(type="area", corner=(0.4, 0.4, 0.999), point1=(0.4, 0.6, 0.999), point2=(0.6, 0.4, 0.999),
+                   color=(1.0, 1.0, 1.0), power=10.0 / np.pi, samples=4)]
+    camera = dict(from_=(0.5, -1.6, 0.5), to=(0.5, 0.5, 0.5), up=(0.5, -1.6, 1.5),
+                  focal=float(1.0 / (2.0 * np.tan(np.radians(20.0)))), resx=res, resy=res)
+    settings = dict(path_samples=1, bounces=5, rr_min_bounces=2, aa_samples=spp, aa_pixelwidth=1.5,
+                    background=None, seed=0)
+    return dict(verts=verts, tris=tris, mat_id=mat_id, materials=materials, lights=lights, camera=camera,
+                settings=settings)

@@ -555,6 +555,11 @@ static int kd_intersect(const ora_kdtree *kd, const ora_ray *ray, float t_max, i
 	return 0;
 }
 
+int ora_kd_intersect1(const ora_kdtree *kd, const ora_ray *ray, float t_max, int shadow, ora_hit *hit)
+{
+	return kd_intersect(kd, ray, t_max, shadow ? TEST_SHADOW : TEST_NEAREST, hit);
+}
+
 typedef struct
 {
 	const ora_kdtree *kd;

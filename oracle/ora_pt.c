@@ -1,0 +1,897 @@
+/*
+ * ora_pt.c -- CPU restatement of the reference's path tracer for the C0
+ * feature set (TEST INFRASTRUCTURE ONLY; see ora_pt.h for the scope and the
+ * deterministic replacements).  Each function names the reference lines it
+ * follows; FP32 expressions keep the reference's evaluation order.
+ */
+#include "ora_pt.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ */
+/* small vector / colour helpers with the reference's operation order   */
+
+typedef struct { float x, y, z; } v3;
+typedef struct { float r, g, b; } rgb;
+
+static inline v3 V(float x, float y, float z) { v3 r = {x, y, z}; return r; }
+static inline v3 vadd(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 vsub(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 vmul(float f, v3 a) { return V(f * a.x, f * a.y, f * a.z); }  /* vector.h:175-187 */
+static inline v3 vneg(v3 a) { return V(-a.x, -a.y, -a.z); }
+static inline float vdot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline v3 vcross(v3 a, v3 b)
+{
+	return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+/* Vec::normalize, include/geometry/vector.h:306-315 */
+static inline v3 vnorm(v3 a)
+{
+	const float l2 = vdot(a, a);
+	if(l2 != 0.f)
+	{
+		const float inv = 1.f / sqrtf(l2);
+		a.x *= inv; a.y *= inv; a.z *= inv;
+	}
+	return a;
+}
+static inline v3 vload(const float *p) { return V(p[0], p[1], p[2]); }
+
+static inline rgb C(float r, float g, float b) { rgb c = {r, g, b}; return c; }
+static inline rgb cadd(rgb a, rgb b) { return C(a.r + b.r, a.g + b.g, a.b + b.b); }
+static inline rgb cmul(rgb a, rgb b) { return C(a.r * b.r, a.g * b.g, a.b * b.b); }
+static inline rgb cscale(rgb a, float f) { return C(f * a.r, f * a.g, f * a.b); }   /* color.h:250-253 */
+static inline rgb cdiv(rgb a, float f) { return C(a.r / f, a.g / f, a.b / f); }     /* color.h:255-258 */
+static inline float cmax(rgb a) { float m = a.g > a.b ? a.g : a.b; return a.r > m ? a.r : m; }  /* color.h:88 */
+static inline int cblack(rgb a) { return a.r == 0.f && a.g == 0.f && a.b == 0.f; }
+
+static const float K_PI = (float)3.1415926535897932384626433832795L;
+static const float K_2PI = (float)6.283185307179586476925286766559L;
+static const float K_1_PI = (float)0.31830988618379067153776752674503L;
+
+/* ------------------------------------------------------------------ */
+/* samplers                                                             */
+
+/* include/sampler/sample.h:100-108 */
+float ora_ri_vdc(uint32_t bits, uint32_t r)
+{
+	bits = (bits << 16) | (bits >> 16);
+	bits = ((bits & 0x00ff00ffu) << 8) | ((bits & 0xff00ff00u) >> 8);
+	bits = ((bits & 0x0f0f0f0fu) << 4) | ((bits & 0xf0f0f0f0u) >> 4);
+	bits = ((bits & 0x33333333u) << 2) | ((bits & 0xccccccccu) >> 2);
+	bits = ((bits & 0x55555555u) << 1) | ((bits & 0xaaaaaaaau) >> 1);
+	const float v = (float)((double)(bits ^ r) * 0.00000000023283064365386962890625);
+	return v < 0.f ? 0.f : (v > 1.f ? 1.f : v);
+}
+
+/* include/sampler/sample.h:110-115 */
+float ora_ri_s(uint32_t i, uint32_t r)
+{
+	for(uint32_t v = 1u << 31; i; i >>= 1, v ^= v >> 1)
+		if(i & 1u) r ^= v;
+	const float f = (float)((double)r * 0.00000000023283064365386962890625);
+	return f < 0.f ? 0.f : (f > 1.f ? 1.f : f);
+}
+
+/* include/sampler/sample.h:117-122 */
+float ora_ri_lp(uint32_t i, uint32_t r)
+{
+	for(uint32_t v = 1u << 31; i; i >>= 1, v |= v >> 1)
+		if(i & 1u) r ^= v;
+	const float f = (float)((double)r * 0.00000000023283064365386962890625);
+	return f < 0.f ? 0.f : (f > 1.f ? 1.f : f);
+}
+
+/* include/sampler/sample.h:130-146 (FNV-1a over the 4 little-endian bytes) */
+uint32_t ora_fnv32a(uint32_t value)
+{
+	uint32_t hash = 0x811c9dc5u;
+	for(int k = 0; k < 4; ++k)
+	{
+		hash ^= (value >> (8 * k)) & 0xffu;
+		hash *= 0x01000193u;
+	}
+	return hash;
+}
+
+/* Faure's scrambling permutations (H. Faure, "Good permutations for extreme
+ * discrepancy", J. Number Theory 42, 1992): sigma_2 = (0 1); for b = 2c,
+ * sigma_b(i) = 2 sigma_c(i), sigma_b(c + i) = 2 sigma_c(i) + 1; for b = 2c + 1,
+ * sigma_b(c) = c and the other entries are sigma_{2c} with values >= c shifted up
+ * by one.  src/sampler/halton.cc tabulates the prime bases up to 229. */
+#define FAURE_MAX_BASE 230
+static int g_faure[FAURE_MAX_BASE][FAURE_MAX_BASE];
+static pthread_once_t g_faure_once = PTHREAD_ONCE_INIT;
+
+static void faure_init(void)
+{
+	g_faure[1][0] = 0;
+	g_faure[2][0] = 0;
+	g_faure[2][1] = 1;
+	for(int b = 3; b < FAURE_MAX_BASE; ++b)
+	{
+		if(b % 2 == 0)
+		{
+			const int c = b / 2;
+			for(int i = 0; i < c; ++i)
+			{
+				g_faure[b][i] = 2 * g_faure[c][i];
+				g_faure[b][c + i] = 2 * g_faure[c][i] + 1;
+			}
+		}
+		else
+		{
+			const int c = (b - 1) / 2;
+			for(int i = 0; i < b; ++i)
+			{
+				if(i == c) { g_faure[b][i] = c; continue; }
+				const int s = g_faure[b - 1][i < c ? i : i - 1];
+				g_faure[b][i] = s >= c ? s + 1 : s;
+			}
+		}
+	}
+}
+
+int ora_faure_perm(int base, int digit)
+{
+	pthread_once(&g_faure_once, faure_init);
+	return g_faure[base][digit];
+}
+
+/* faure::prims and faure::inv_prims, src/sampler/halton.cc:401-416: the inverse
+ * bases are 1/p rounded to nine decimals (not exact reciprocals). */
+static const int k_prims[50] = {1, 2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67, 71, 73,
+                                79, 83, 89, 97, 101, 103, 107, 109, 113, 127, 131, 137, 139, 149, 151, 157, 163, 167,
+                                173, 179, 181, 191, 193, 197, 199, 211, 223, 227};
+
+static double inv_prim9(int p)
+{
+	/* nearest double to round(1e9 / p) / 1e9, as a 9-decimal literal parses */
+	return floor(1e9 / (double)p + 0.5) / 1e9;
+}
+
+/* Halton::lowDiscrepancySampling, src/sampler/halton.cc:419-440 (dim < 50). */
+double ora_low_discrepancy(int dim, uint32_t n)
+{
+	double value = 0.0;
+	if(dim < 50 && dim >= 1)
+	{
+		pthread_once(&g_faure_once, faure_init);
+		const uint32_t base = (uint32_t)k_prims[dim];
+		const int table = dim <= 2 ? 3 : (int)base;  /* all_tables[0..2] are base_3 */
+		const double f = inv_prim9(k_prims[dim]);
+		double dn = (double)n;
+		double factor = f;
+		while(n > 0)
+		{
+			value += (double)g_faure[table][n % base] * factor;
+			dn *= f;
+			n = (uint32_t)dn;
+			factor *= f;
+		}
+	}
+	return value;
+}
+
+/* Halton(base, start).getNext(), include/sampler/halton.h:54-80 */
+typedef struct { uint32_t base; double inv_base; double value; } halton;
+
+static void halton_start(halton *h, int base, uint32_t start)
+{
+	h->base = (uint32_t)base;
+	h->inv_base = 1.0 / (double)base;
+	double factor = h->inv_base;
+	h->value = 0.0;
+	while(start > 0)
+	{
+		h->value += (double)(start % h->base) * factor;
+		start /= h->base;
+		factor *= h->inv_base;
+	}
+}
+
+static float halton_next(halton *h)
+{
+	const double r = 0.9999999999 - h->value;
+	if(h->inv_base < r) h->value += h->inv_base;
+	else
+	{
+		double hh = 0.0, hv = h->inv_base;
+		while(hv >= r)
+		{
+			hh = hv;
+			hv *= h->inv_base;
+		}
+		h->value += hh + hv - 1.0;
+	}
+	const float v = (float)h->value;
+	return v < 0.f ? 0.f : (v > 1.f ? 1.f : v);
+}
+
+float ora_halton_next(int base, uint32_t start)
+{
+	halton h;
+	halton_start(&h, base, start);
+	return halton_next(&h);
+}
+
+/* Counter-based replacement of the Russian-roulette generator. */
+float ora_rr_random(uint32_t pixel, uint32_t sample, uint32_t depth, uint32_t seed)
+{
+	uint32_t h = pixel * 0x9E3779B1u;
+	h ^= (sample + 0x7F4A7C15u) * 0x85EBCA77u;
+	h ^= (depth + 0x165667B1u) * 0xC2B2AE3Du;
+	h ^= seed * 0x27D4EB2Fu;
+	h ^= h >> 16;
+	h *= 0x85EBCA6Bu;
+	h ^= h >> 13;
+	h *= 0xC2B2AE35u;
+	h ^= h >> 16;
+	return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
+
+/* math::filter::gauss with FAST_MATH exp (include/math/filter.h:32-37) and the
+ * table of src/render/imagefilm.cc:231-264. */
+void ora_filter_table(float pixelwidth, float table[256], float *table_scale, float *filter_width)
+{
+	float fw = pixelwidth * 0.5f;
+	fw *= 2.f;                                   /* Gauss */
+	if(fw < 0.501f) fw = 0.501f;
+	if(fw > 0.5f * 8.f) fw = 0.5f * 8.f;         /* max_filter_size_ = 8 */
+	const float scale = 1.f / 16.f;
+	int k = 0;
+	for(int y = 0; y < 16; ++y)
+	{
+		for(int x = 0; x < 16; ++x)
+		{
+			const float dx = ((float)x + .5f) * scale, dy = ((float)y + .5f) * scale;
+			const float r_2 = dx * dx + dy * dy;
+			const float e = ora_fast_exp2((float)1.4426950408889634073599246810019L * ((float)-6 * r_2));
+			const float v = e - 0.00247875f;
+			table[k++] = v > 0.f ? v : 0.f;
+		}
+	}
+	*table_scale = 0.9999f * 16.f / fw;
+	*filter_width = fw;
+}
+
+/* ------------------------------------------------------------------ */
+/* scene access                                                          */
+
+typedef struct
+{
+	const ora_kdtree *kd;
+	const float *verts;
+	const uint32_t *idx;
+	const uint32_t *mat_id;
+	const ora_material *mats;
+	const ora_light *lights;
+	uint32_t n_lights;
+	const ora_pt_settings *st;
+	/* camera (src/camera/camera.cc:89-110, camera_perspective.cc:97-111) */
+	v3 cam_from, vright, vup, vto;
+	uint32_t total_samples;
+} pt_ctx;
+
+enum
+{
+	F_NONE = 0, F_SPECULAR = 1, F_GLOSSY = 2, F_DIFFUSE = 4, F_DISPERSIVE = 8, F_REFLECT = 16, F_TRANSMIT = 32,
+	F_FILTER = 64, F_EMIT = 128, F_VOLUMETRIC = 256, F_ALL = 511
+}; /* include/material/bsdf.h:25-67 */
+
+typedef struct
+{
+	v3 p, n, ng;
+	v3 u, v;     /* uvn_ */
+	uint32_t prim;
+	const ora_material *mat;
+	uint32_t flags;
+} surf;
+
+static uint32_t mat_flags(const ora_material *m)
+{
+	uint32_t f = 0;
+	if(m->diffuse_reflect > 0.00001f) f |= F_DIFFUSE | F_REFLECT;  /* material_shiny_diffuse.cc:175-183 */
+	if(m->emit > 0.f) f |= F_EMIT;                                 /* :134 */
+	return f;
+}
+
+/* Vec3f::createCoordsSystem, include/geometry/vector.h:399-421 */
+static void coords_system(v3 n, v3 *u, v3 *v)
+{
+	if(n.x == 0.f && n.y == 0.f)
+	{
+		*u = n.z < 0.f ? V(-1.f, 0.f, 0.f) : V(1.f, 0.f, 0.f);
+		*v = V(0.f, 1.f, 0.f);
+	}
+	else
+	{
+		const float d = 1.f / sqrtf(n.y * n.y + n.x * n.x);
+		*u = V(n.y * d, -n.x * d, 0.f);
+		*v = vcross(n, *u);
+	}
+}
+
+/* Accelerator::intersect(ray, camera) + PrimitivePolygon::getSurfacePolygon
+ * (include/accelerator/accelerator.h:89-101, src/geometry/primitive/primitive_polygon.cc:37-115) */
+static int intersect_surface(const pt_ctx *c, v3 from, v3 dir, float tmin, float tmax, surf *sp)
+{
+	ora_ray r = {{from.x, from.y, from.z}, tmin, {dir.x, dir.y, dir.z}, tmax};
+	const float t_max = (tmax >= 0.f) ? tmax : FLT_MAX;
+	ora_hit h;
+	if(!ora_kd_intersect1(c->kd, &r, t_max, 0, &h) || h.prim == ORA_NO_HIT) return 0;
+	sp->p = vadd(from, vmul(h.t, dir));
+	const uint32_t *ix = c->idx + 3 * (size_t)h.prim;
+	const v3 p0 = vload(c->verts + 3 * (size_t)ix[0]);
+	const v3 p1 = vload(c->verts + 3 * (size_t)ix[1]);
+	const v3 p2 = vload(c->verts + 3 * (size_t)ix[2]);
+	sp->ng = vnorm(vcross(vsub(p1, p0), vsub(p2, p0)));   /* shape_polygon.h:58-62 */
+	sp->n = sp->ng;
+	coords_system(sp->n, &sp->u, &sp->v);
+	sp->prim = h.prim;
+	sp->mat = &c->mats[c->mat_id[h.prim]];
+	sp->flags = mat_flags(sp->mat);
+	return 1;
+}
+
+/* Accelerator::isShadowed, include/accelerator/accelerator.h:103-111 */
+static int is_shadowed(const pt_ctx *c, v3 from, v3 dir, float tmin, float tmax)
+{
+	const v3 f = vadd(from, vmul(tmin, dir));
+	const float t_max = (tmax >= 0.f) ? tmax - 2 * tmin : FLT_MAX;
+	ora_ray r = {{f.x, f.y, f.z}, tmin, {dir.x, dir.y, dir.z}, tmax};
+	ora_hit h;
+	return ora_kd_intersect1(c->kd, &r, t_max, 1, &h);
+}
+
+/* SurfacePoint::normalFaceForward, include/geometry/surface.h:202-205 */
+static inline v3 face_forward(v3 ng, v3 n, v3 wo) { return vdot(ng, wo) < 0.f ? vneg(n) : n; }
+
+/* ShinyDiffuseMaterial::eval (diffuse only), material_shiny_diffuse.cc:297-340 */
+static rgb bsdf_eval(const surf *sp, v3 wo, v3 wl, uint32_t bsdfs)
+{
+	const float cos_ng_wo = vdot(sp->ng, wo);
+	const float cos_ng_wl = vdot(sp->ng, wl);
+	const v3 n = face_forward(sp->ng, sp->n, wo);
+	if(!(bsdfs & (sp->flags & F_DIFFUSE))) return C(0.f, 0.f, 0.f);
+	(void)cos_ng_wl;
+	(void)cos_ng_wo;
+	/* kr = 1 (no Fresnel), m_t = 1, no translucency */
+	if(vdot(n, wl) < 0.0) return C(0.f, 0.f, 0.f);
+	const float m_d = 1.f * (1.f - 0.f) * sp->mat->diffuse_reflect;
+	return cscale(C(sp->mat->color[0], sp->mat->color[1], sp->mat->color[2]), m_d);
+}
+
+/* ShinyDiffuseMaterial::pdf, material_shiny_diffuse.cc:441-483 */
+static float bsdf_pdf(const surf *sp, v3 wo, v3 wi, uint32_t bsdfs)
+{
+	if(!(bsdfs & F_DIFFUSE)) return 0.f;
+	const v3 n = face_forward(sp->ng, sp->n, wo);
+	if(!(sp->flags & F_DIFFUSE)) return 0.f;
+	float pdf = 0.f, sum = 0.f;
+	int n_match = 0;
+	if(bsdfs & (F_DIFFUSE | F_REFLECT))
+	{
+		const float width = sp->mat->diffuse_reflect;  /* accumulate(): accum[3] */
+		sum += width;
+		pdf += fabsf(vdot(wi, n)) * width;
+		++n_match;
+	}
+	if(!n_match || sum < 0.00001) return 0.f;
+	return pdf / sum;
+}
+
+/* sample::cosHemisphere, include/sampler/sample.h:43-52 */
+static v3 cos_hemisphere(v3 n, v3 ru, v3 rv, float s1, float s2)
+{
+	if(s1 >= 1.0f) return n;
+	const float z1 = s1;
+	const float z2 = s2 * K_2PI;
+	return vadd(vmul(sqrtf(1.f - z1), vadd(vmul(ora_fast_cos(z2), ru), vmul(ora_fast_sin(z2), rv))),
+	            vmul(sqrtf(z1), n));
+}
+
+typedef struct { float s1, s2; uint32_t flags; uint32_t sampled; float pdf; } bsample;
+
+/* ShinyDiffuseMaterial::sample (diffuse component), material_shiny_diffuse.cc:349-439 */
+static rgb bsdf_sample(const surf *sp, v3 wo, v3 *wi, bsample *s, float *w)
+{
+	const float cos_ng_wo = vdot(sp->ng, wo);
+	const v3 n = face_forward(sp->ng, sp->n, wo);
+	const uint32_t cflag = F_DIFFUSE | F_REFLECT;
+	float sum = 0.f, width = 0.f, val = 0.f;
+	int n_match = 0;
+	if((sp->flags & F_DIFFUSE) && (s->flags & cflag) == cflag)
+	{
+		width = sp->mat->diffuse_reflect;
+		sum += width;
+		val = sum;
+		++n_match;
+	}
+	if(!n_match || sum < 0.00001)
+	{
+		s->sampled = F_NONE;
+		s->pdf = 0.f;
+		return C(1.f, 1.f, 1.f);
+	}
+	const float inv_sum = 1.f / sum;
+	val *= inv_sum;
+	width *= inv_sum;
+	(void)val;
+	const float s1 = s->s1 / width;
+	rgb scolor = C(0.f, 0.f, 0.f);
+	*wi = cos_hemisphere(n, sp->u, sp->v, s1, s->s2);
+	if(cos_ng_wo * vdot(sp->ng, *wi) > 0)
+		scolor = cscale(C(sp->mat->color[0], sp->mat->color[1], sp->mat->color[2]), sp->mat->diffuse_reflect);
+	s->pdf = fabsf(vdot(*wi, n)) * width;
+	s->sampled = cflag;
+	*w = fabsf(vdot(*wi, sp->n)) / (s->pdf * 0.99f + 0.01f);
+	/* getAlpha() == 1 for a non-transparent shinydiffuse: w * 1 + 1 * 0 */
+	*w = *w * 1.f + 1.f * (1.f - 1.f);
+	return scolor;
+}
+
+static rgb mat_emit(const surf *sp)
+{
+	const ora_material *m = sp->mat;
+	return cscale(C(m->color[0], m->color[1], m->color[2]), m->emit);  /* emit_color_ = emit * color */
+}
+
+/* ------------------------------------------------------------------ */
+/* lights                                                                */
+
+static rgb light_color(const ora_light *l)
+{
+	const rgb c = cscale(C(l->color[0], l->color[1], l->color[2]), l->power);
+	return l->type == ORA_LIGHT_AREA ? cscale(c, K_PI) : c;  /* light_area.h:78, light_point.h:64 */
+}
+
+static v3 area_normal_flipped(const ora_light *l)
+{
+	/* (point2 - corner) ^ (point1 - corner), light_area.h:76 */
+	return vcross(vsub(vload(l->point2), vload(l->corner)), vsub(vload(l->point1), vload(l->corner)));
+}
+
+/* area_ = normal_flipped_.normalized().length(), light_area.h:79 */
+static float area_area(const ora_light *l)
+{
+	const v3 nf = vnorm(area_normal_flipped(l));
+	return sqrtf(vdot(nf, nf));
+}
+
+/* ShapePolygon<float, 4>::intersect, include/geometry/shape/shape_polygon.h:126-176 */
+static float quad_intersect(const v3 q[4], v3 from, v3 dir)
+{
+	const v3 e1 = vsub(q[1], q[0]);
+	const v3 e2 = vsub(q[2], q[0]);
+	const v3 p2 = vcross(dir, e2);
+	const float det12 = vdot(e1, p2);
+	if(det12 != 0.f)
+	{
+		const float inv12 = 1.f / det12;
+		const v3 tvec = vsub(from, q[0]);
+		float u = vdot(tvec, p2) * inv12;
+		if(u >= 0.f && u <= 1.f)
+		{
+			const v3 q1 = vcross(tvec, e1);
+			const float v = vdot(dir, q1) * inv12;
+			if(v >= 0.f && (u + v) <= 1.f)
+			{
+				const float t = vdot(e2, q1) * inv12;
+				if(t > 0.f) return t;
+			}
+		}
+		else
+		{
+			const v3 e3 = vsub(q[3], q[0]);
+			const v3 p3 = vcross(dir, e3);
+			const float det23 = vdot(e2, p3);
+			if(det23 != 0.f)
+			{
+				const float inv23 = 1.f / det23;
+				u = vdot(tvec, p3) * inv23;
+				if(u >= 0.f && u <= 1.f)
+				{
+					const v3 q2 = vcross(tvec, e2);
+					const float v = vdot(dir, q2) * inv23;
+					if(v >= 0.f && (u + v) <= 1.f)
+					{
+						const float t = vdot(e3, q2) * inv23;
+						if(t > 0.f) return t;
+					}
+				}
+			}
+		}
+	}
+	return 0.f;
+}
+
+/* AreaLight::intersect, src/light/light_area.cc:146-155.  Returns ipdf > 0 on hit. */
+static int area_intersect(const ora_light *l, v3 from, v3 dir, float *t, float *ipdf)
+{
+	const v3 nf = area_normal_flipped(l);
+	const float cos_angle = vdot(dir, nf);
+	if(cos_angle <= 0.f) return 0;
+	const v3 c0 = vload(l->corner), p1 = vload(l->point1), p2 = vload(l->point2);
+	const v3 q[4] = {c0, p1, vsub(vadd(p1, p2), c0), p2};
+	const float th = quad_intersect(q, from, dir);
+	if(th <= 0.f) return 0;
+	*t = th;
+	*ipdf = 1.f / (th * th) * area_area(l) * cos_angle * K_1_PI;
+	return 1;
+}
+
+/* MonteCarloIntegrator::doLightEstimation and the area / dirac paths,
+ * src/integrator/surface/integrator_montecarlo.cc:118-440. */
+static rgb light_estimation(const pt_ctx *c, const ora_light *l, const surf *sp, v3 wo, uint32_t loffs,
+                            uint32_t psd_sample, uint32_t psd_offset)
+{
+	rgb col = C(0.f, 0.f, 0.f);
+	const int cast_shadows = l->cast_shadows && sp->mat->receive_shadows;
+	const float plen = sqrtf(vdot(sp->p, sp->p));
+	const float shadow_tmin = 0.0005f * (1.f > plen ? 1.f : plen);   /* shadow_bias_auto_ */
+	if(l->type == ORA_LIGHT_POINT)
+	{
+		/* PointLight::illuminate, src/light/light_point.cc:70-81; diracLight :118-190 */
+		v3 ldir = vsub(vload(l->from), sp->p);
+		const float dist_sqr = ldir.x * ldir.x + ldir.y * ldir.y + ldir.z * ldir.z;
+		const float dist = sqrtf(dist_sqr);
+		if(dist == 0.f) return col;
+		const float idist_sqr = 1.f / dist_sqr;
+		ldir = vmul(1.f / dist, ldir);
+		const rgb lcol = cscale(light_color(l), idist_sqr);
+		int shadowed = 0;
+		if(cast_shadows) shadowed = is_shadowed(c, sp->p, ldir, shadow_tmin, dist);
+		const float angle = fabsf(vdot(sp->n, ldir));
+		if(!shadowed)
+		{
+			const rgb surf_col = bsdf_eval(sp, wo, ldir, F_ALL);
+			col = cadd(col, cmul(cscale(cmul(surf_col, lcol), angle), C(1.f, 1.f, 1.f)));
+		}
+		return col;
+	}
+	const uint32_t l_offs = loffs * 4567u;
+	const int num_samples = (int)ceilf((float)l->samples * 1.f);
+	const float inv_num_samples = 1.f / (float)num_samples;
+	const uint32_t offs = (uint32_t)num_samples * psd_sample + psd_offset + l_offs;
+	const rgb lcolor = light_color(l);
+	const v3 nf = area_normal_flipped(l);
+	const float area = area_area(l);
+	const v3 c0 = vload(l->corner);
+	const v3 to_x = vsub(vload(l->point1), c0), to_y = vsub(vload(l->point2), c0);
+	halton h2, h3;
+	/* areaLightSampleLight, :192-316 */
+	halton_start(&h2, 2, offs - 1);
+	halton_start(&h3, 3, offs - 1);
+	rgb acc = C(0.f, 0.f, 0.f);
+	for(int i = 0; i < num_samples; ++i)
+	{
+		const float s1 = halton_next(&h2);
+		const float s2 = halton_next(&h3);
+		/* AreaLight::illumSample, src/light/light_area.cc:99-124 */
+		const v3 p = vadd(vadd(c0, vmul(s1, to_x)), vmul(s2, to_y));
+		v3 ldir = vsub(p, sp->p);
+		const float dist_sqr = vdot(ldir, ldir);
+		const float dist = sqrtf(dist_sqr);
+		if(dist <= 0.f) continue;
+		ldir = vmul(1.f / dist, ldir);
+		const float cos_angle = vdot(ldir, nf);
+		if(cos_angle <= 0.f) continue;
+		const float lpdf = dist_sqr * K_PI / (area * cos_angle);
+		int shadowed = 0;
+		if(cast_shadows) shadowed = is_shadowed(c, sp->p, ldir, shadow_tmin, dist);
+		if(!shadowed && lpdf > 1e-6f)
+		{
+			const rgb surf_col = bsdf_eval(sp, wo, ldir, F_ALL);
+			const float angle = fabsf(vdot(sp->n, ldir));
+			float w = 1.f;
+			const float m_pdf = bsdf_pdf(sp, wo, ldir, F_GLOSSY | F_DIFFUSE | F_DISPERSIVE | F_REFLECT | F_TRANSMIT);
+			if(m_pdf > 1e-6f)
+			{
+				const float l_2 = lpdf * lpdf;
+				const float m_2 = m_pdf * m_pdf;
+				w = l_2 / (l_2 + m_2);
+			}
+			acc = cadd(acc, cdiv(cscale(cscale(cmul(surf_col, lcolor), angle), w), lpdf));
+		}
+	}
+	col = cadd(col, cscale(acc, inv_num_samples));
+	/* areaLightSampleMaterial, :318-415 */
+	halton_start(&h2, 2, offs - 1);
+	halton_start(&h3, 3, offs - 1);
+	acc = C(0.f, 0.f, 0.f);
+	const float btmin = 0.00005f * (1.f > plen ? 1.f : plen);   /* ray_min_dist_auto_ */
+	for(int i = 0; i < num_samples; ++i)
+	{
+		bsample s;
+		s.s1 = halton_next(&h2);
+		s.s2 = halton_next(&h3);
+		s.flags = F_GLOSSY | F_DIFFUSE | F_DISPERSIVE | F_REFLECT | F_TRANSMIT;
+		s.pdf = 0.f;
+		float W = 0.f;
+		v3 dir = V(0.f, 0.f, 0.f);
+		const rgb surf_col = bsdf_sample(sp, wo, &dir, &s, &W);
+		float t, ipdf;
+		if(area_intersect(l, sp->p, dir, &t, &ipdf) && s.pdf > 1e-6f)
+		{
+			int shadowed = 0;
+			if(cast_shadows) shadowed = is_shadowed(c, sp->p, dir, btmin, t);
+			if(!shadowed && ipdf > 1e-6f)
+			{
+				const float l_pdf = 1.f / ipdf;
+				const float l_2 = l_pdf * l_pdf;
+				const float m_2 = s.pdf * s.pdf;
+				const float w = m_2 / (l_2 + m_2);
+				acc = cadd(acc, cscale(cscale(cmul(surf_col, lcolor), w), W));
+			}
+		}
+	}
+	col = cadd(col, cscale(acc, inv_num_samples));
+	return col;
+}
+
+/* estimateAllDirectLight, integrator_montecarlo.cc:92-106 */
+static rgb direct_all(const pt_ctx *c, const surf *sp, v3 wo, uint32_t psd_sample, uint32_t psd_offset)
+{
+	rgb col = C(0.f, 0.f, 0.f);
+	for(uint32_t l = 0; l < c->n_lights; ++l)
+		col = cadd(col, light_estimation(c, &c->lights[l], sp, wo, l, psd_sample, psd_offset));
+	return col;
+}
+
+/* estimateOneDirectLight, integrator_montecarlo.cc:108-116, with the
+ * deterministic light-pick index of ora_pt.h. */
+static rgb direct_one(const pt_ctx *c, const surf *sp, v3 wo, uint32_t pixel, uint32_t psd_sample,
+                      uint32_t psd_offset, uint32_t sub, uint32_t depth)
+{
+	const int num_lights = (int)c->n_lights;
+	if(num_lights == 0) return C(0.f, 0.f, 0.f);
+	const uint32_t bounces = (uint32_t)c->st->bounces;
+	const uint32_t corr = (pixel * c->total_samples + psd_sample) * ((uint32_t)c->st->path_samples * bounces) +
+	                      sub * bounces + depth + 1u;
+	const float h = ora_halton_next(2, corr - 1u);
+	int lnum = (int)(h * (float)num_lights);
+	if(lnum > num_lights - 1) lnum = num_lights - 1;
+	return cscale(light_estimation(c, &c->lights[lnum], sp, wo, (uint32_t)lnum, psd_sample, psd_offset),
+	              (float)num_lights);
+}
+
+/* PathIntegrator::integrate, src/integrator/surface/integrator_path_tracer.cc:138-314 */
+static void integrate(const pt_ctx *c, v3 from, v3 dir, uint32_t pixel, uint32_t psd_sample, uint32_t psd_offset,
+                      float out[4])
+{
+	const ora_pt_settings *st = c->st;
+	rgb col = C(0.f, 0.f, 0.f);
+	float alpha = 1.f;
+	float w = 0.f;
+	surf sp;
+	if(!intersect_surface(c, from, dir, 0.f, -1.f, &sp))
+	{
+		if(st->has_background) col = C(st->background[0], st->background[1], st->background[2]);
+		out[0] = col.r; out[1] = col.g; out[2] = col.b; out[3] = alpha;
+		return;
+	}
+	const uint32_t mat_bsdfs = sp.flags;
+	const v3 wo = vneg(dir);
+	if(mat_bsdfs & F_EMIT) col = cadd(col, mat_emit(&sp));
+	if(mat_bsdfs & F_DIFFUSE) col = cadd(col, direct_all(c, &sp, wo, psd_sample, psd_offset));
+	uint32_t path_flags = F_DIFFUSE;
+	if(mat_bsdfs & path_flags)
+	{
+		rgb path_col = C(0.f, 0.f, 0.f);
+		path_flags |= F_DIFFUSE | F_REFLECT | F_TRANSMIT;
+		const int n_samples = st->path_samples > 1 ? st->path_samples : 1;
+		for(int i = 0; i < n_samples; ++i)
+		{
+			const uint32_t offs = (uint32_t)st->path_samples * psd_sample + psd_offset + (uint32_t)i;
+			rgb throughput;
+			v3 pwo = wo;
+			bsample s;
+			s.s1 = ora_ri_vdc(offs, 0);
+			s.s2 = (float)ora_low_discrepancy(2, offs);
+			s.flags = path_flags;
+			s.pdf = 0.f;
+			s.sampled = F_NONE;
+			v3 pdir = V(0.f, 0.f, 0.f);
+			rgb scol = bsdf_sample(&sp, pwo, &pdir, &s, &w);
+			scol = cscale(scol, w);
+			throughput = scol;
+			surf hit;
+			if(!intersect_surface(c, sp.p, pdir, 0.00005f, -1.f, &hit)) continue;
+			if(s.sampled != F_NONE) pwo = vneg(pdir);
+			rgb lcol = direct_one(c, &hit, pwo, pixel, psd_sample, psd_offset, (uint32_t)i, 0u);
+			const uint32_t mat_bsd_fs = hit.flags;
+			if(mat_bsd_fs & F_EMIT) lcol = cadd(lcol, mat_emit(&hit));
+			path_col = cadd(path_col, cmul(lcol, throughput));
+			int caustic = 0;
+			for(int depth = 1; depth < st->bounces; ++depth)
+			{
+				const int d_4 = 4 * depth;
+				s.s1 = (float)ora_low_discrepancy(d_4 + 3, offs);
+				s.s2 = (float)ora_low_discrepancy(d_4 + 4, offs);
+				s.flags = F_ALL;
+				scol = bsdf_sample(&hit, pwo, &pdir, &s, &w);
+				scol = cscale(scol, w);
+				if(cblack(scol)) break;
+				throughput = cmul(throughput, scol);
+				caustic = (s.sampled & (F_SPECULAR | F_GLOSSY | F_FILTER)) != 0;
+				surf nh;
+				if(!intersect_surface(c, hit.p, pdir, 0.00005f, -1.f, &nh)) break;
+				hit = nh;
+				pwo = vneg(pdir);
+				if(mat_bsd_fs & F_DIFFUSE) lcol = direct_one(c, &hit, pwo, pixel, psd_sample, psd_offset, (uint32_t)i, (uint32_t)depth);
+				else lcol = C(0.f, 0.f, 0.f);
+				if(depth > st->rr_min_bounces)
+				{
+					const float random_value = ora_rr_random(pixel, psd_sample * (uint32_t)st->path_samples + (uint32_t)i, (uint32_t)depth, st->seed);
+					const float probability = cmax(throughput);
+					if(probability <= 0.f || probability < random_value) break;
+					throughput = cscale(throughput, 1.f / probability);
+				}
+				if((mat_bsd_fs & F_EMIT) && caustic) lcol = cadd(lcol, mat_emit(&hit));
+				path_col = cadd(path_col, cmul(lcol, throughput));
+			}
+		}
+		col = cadd(col, cdiv(path_col, (float)n_samples));
+	}
+	/* recursiveRaytrace: no glossy / specular / filter components in this material set */
+	out[0] = col.r; out[1] = col.g; out[2] = col.b; out[3] = alpha;
+}
+
+static void setup_ctx(pt_ctx *c, const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx,
+                      const uint32_t *mat_id, const ora_material *mats, const ora_light *lights, uint32_t n_lights,
+                      const ora_pt_settings *st)
+{
+	c->kd = kd; c->verts = verts; c->idx = tri_idx; c->mat_id = mat_id; c->mats = mats;
+	c->lights = lights; c->n_lights = n_lights; c->st = st;
+	c->total_samples = (uint32_t)st->aa_samples;
+	/* Camera::Camera, src/camera/camera.cc:89-104 */
+	const v3 from = vload(st->from);
+	v3 cy = vsub(vload(st->up), from);
+	v3 cz = vsub(vload(st->to), from);
+	v3 cx = vcross(cz, cy);
+	cy = vcross(cz, cx);
+	cx = vnorm(cx);
+	cy = vnorm(cy);
+	cz = vnorm(cz);
+	const float aspect = st->aspect_factor * (float)st->resy / (float)st->resx;
+	/* PerspectiveCamera::setAxis, camera_perspective.cc:97-111 */
+	c->vright = cx;
+	c->vup = vmul(aspect, cy);
+	c->vto = vsub(vmul(st->focal, cz), vmul(0.5f, vadd(c->vup, c->vright)));
+	c->vup = V(c->vup.x / (float)st->resy, c->vup.y / (float)st->resy, c->vup.z / (float)st->resy);
+	c->vright = V(c->vright.x / (float)st->resx, c->vright.y / (float)st->resx, c->vright.z / (float)st->resx);
+	c->cam_from = from;
+}
+
+/* TiledIntegrator::renderTile per-sample set-up, integrator_tiled.cc:290-335 (one pass) */
+static void pixel_sample(const pt_ctx *c, int x, int y, int sample, float col[4], float *dx_out, float *dy_out)
+{
+	const ora_pt_settings *st = c->st;
+	const uint32_t pixel = (uint32_t)(st->resx * y + x);
+	const uint32_t offset = ora_fnv32a((uint32_t)y * ora_fnv32a((uint32_t)x));
+	const uint32_t psd_sample = (uint32_t)sample;
+	float dx = 0.5f, dy = 0.5f;
+	if(st->aa_samples > 1)
+	{
+		const float d_1 = 1.f / (float)st->aa_samples;
+		dx = (0.5f + (float)sample) * d_1;
+		dy = ora_ri_lp((uint32_t)sample + offset, 0);
+	}
+	/* PerspectiveCamera::shootRay, camera_perspective.cc:165-180 (pinhole) */
+	const float px = (float)x + dx, py = (float)y + dy;
+	v3 dir = vadd(vadd(vmul(px, c->vright), vmul(py, c->vup)), c->vto);
+	dir = vnorm(dir);
+	integrate(c, c->cam_from, dir, pixel, psd_sample, offset, col);
+	*dx_out = dx;
+	*dy_out = dy;
+}
+
+void ora_pt_pixel_sample(const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx, const uint32_t *mat_id,
+                         const ora_material *mats, const ora_light *lights, uint32_t n_lights,
+                         const ora_pt_settings *st, int x, int y, int sample, float col_alpha[4], float dxdy[2])
+{
+	pt_ctx c;
+	setup_ctx(&c, kd, verts, tri_idx, mat_id, mats, lights, n_lights, st);
+	pixel_sample(&c, x, y, sample, col_alpha, &dxdy[0], &dxdy[1]);
+}
+
+/* ------------------------------------------------------------------ */
+/* film                                                                  */
+
+typedef struct
+{
+	const pt_ctx *c;
+	int sample;
+	float *cols;   /* resx * resy * 4 */
+	float *dxy;    /* resx * resy * 2 */
+	int tid, nthreads;
+} pass_job;
+
+static void *pass_worker(void *arg)
+{
+	pass_job *j = (pass_job *)arg;
+	const ora_pt_settings *st = j->c->st;
+	for(int y = j->tid; y < st->resy; y += j->nthreads)
+		for(int x = 0; x < st->resx; ++x)
+		{
+			const size_t k = (size_t)y * st->resx + x;
+			pixel_sample(j->c, x, y, j->sample, j->cols + 4 * k, &j->dxy[2 * k], &j->dxy[2 * k + 1]);
+		}
+	return NULL;
+}
+
+/* roundToIntWithBias, include/render/imagefilm.h:344-348 */
+static inline int round_bias(double v) { return (int)(v + (.5 - 1.4e-11)); }
+
+/* ImageFilm::addSample, src/render/imagefilm.cc:771-822 */
+static void add_sample(int resx, int resy, const float table[256], float table_scale, float filter_width, int x,
+                       int y, float dx, float dy, const float col[4], float *film_rgba, float *film_w)
+{
+	int dx0 = round_bias((double)dx - filter_width);
+	int dx1 = round_bias((double)dx + filter_width - 1.0);
+	int dy0 = round_bias((double)dy - filter_width);
+	int dy1 = round_bias((double)dy + filter_width - 1.0);
+	if(-x > dx0) dx0 = -x;
+	if((resx - 1) - x < dx1) dx1 = (resx - 1) - x;
+	if(-y > dy0) dy0 = -y;
+	if((resy - 1) - y < dy1) dy1 = (resy - 1) - y;
+	int xi[9], yi[9];
+	const double x_offs = dx - 0.5;
+	for(int i = dx0, n = 0; i <= dx1; ++i, ++n) xi[n] = (int)floor(fabs(((double)i - x_offs) * table_scale));
+	const double y_offs = dy - 0.5;
+	for(int i = dy0, n = 0; i <= dy1; ++i, ++n) yi[n] = (int)floor(fabs(((double)i - y_offs) * table_scale));
+	for(int j = y + dy0; j <= y + dy1; ++j)
+		for(int i = x + dx0; i <= x + dx1; ++i)
+		{
+			const float wt = table[yi[j - (y + dy0)] * 16 + xi[i - (x + dx0)]];
+			const size_t p = (size_t)j * resx + i;
+			film_w[p] = film_w[p] + wt;
+			for(int k = 0; k < 4; ++k) film_rgba[4 * p + k] += wt * col[k];
+		}
+}
+
+void ora_pt_render(const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx, const uint32_t *mat_id,
+                   const ora_material *mats, const ora_light *lights, uint32_t n_lights,
+                   const ora_pt_settings *st, float *film_rgba, float *film_w, int threads)
+{
+	pt_ctx c;
+	setup_ctx(&c, kd, verts, tri_idx, mat_id, mats, lights, n_lights, st);
+	float table[256], table_scale, filter_width;
+	ora_filter_table(st->aa_pixelwidth, table, &table_scale, &filter_width);
+	const size_t npix = (size_t)st->resx * st->resy;
+	float *cols = (float *)malloc(npix * 4 * sizeof(float));
+	float *dxy = (float *)malloc(npix * 2 * sizeof(float));
+	if(!cols || !dxy) { free(cols); free(dxy); return; }
+	if(threads < 1) threads = 1;
+	if(threads > 256) threads = 256;
+	pass_job jobs[256];
+	pthread_t th[256];
+	for(int s = 0; s < st->aa_samples; ++s)
+	{
+		for(int t = 0; t < threads; ++t)
+		{
+			jobs[t].c = &c; jobs[t].sample = s; jobs[t].cols = cols; jobs[t].dxy = dxy;
+			jobs[t].tid = t; jobs[t].nthreads = threads;
+		}
+		if(threads == 1) pass_worker(&jobs[0]);
+		else
+		{
+			for(int t = 0; t < threads; ++t) pthread_create(&th[t], NULL, pass_worker, &jobs[t]);
+			for(int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
+		}
+		for(int y = 0; y < st->resy; ++y)
+			for(int x = 0; x < st->resx; ++x)
+			{
+				const size_t k = (size_t)y * st->resx + x;
+				add_sample(st->resx, st->resy, table, table_scale, filter_width, x, y, dxy[2 * k], dxy[2 * k + 1],
+				           cols + 4 * k, film_rgba, film_w);
+			}
+	}
+	free(cols);
+	free(dxy);
+}

@@ -1,0 +1,110 @@
+/*
+ * ora_pt.h -- CPU restatement of libYafaRay's `pathtracing` SurfaceIntegrator
+ * (TEST INFRASTRUCTURE ONLY: the checker of the GPU path tracer, and the CPU
+ * baseline of bench.py; never linked into the shipped library).
+ *
+ * Restated: TiledIntegrator::renderTile's sampling
+ * (src/integrator/surface/integrator_tiled.cc:253-395), PathIntegrator::integrate
+ * (src/integrator/surface/integrator_path_tracer.cc:138-314), the Monte-Carlo
+ * direct lighting (src/integrator/surface/integrator_montecarlo.cc:92-440),
+ * ShinyDiffuseMaterial (src/material/material_shiny_diffuse.cc:141-485),
+ * AreaLight / PointLight (src/light/light_area.cc:99-163,
+ * src/light/light_point.cc:70-97), PerspectiveCamera::shootRay
+ * (src/camera/camera_perspective.cc:97-111,165-180), the samplers
+ * (include/sampler/sample.h:43-148, include/sampler/halton.h:30-81,
+ * src/sampler/halton.cc:419-440) and ImageFilm::addSample with the Gauss
+ * filter table (src/render/imagefilm.cc:231-264,771-822).
+ *
+ * Deterministic replacements for the reference's two non-reproducible
+ * streams (the same in the GPU path tracer):
+ *   - Russian roulette draws ymx_rr_random(pixel, sample, depth, seed) instead
+ *     of the tile-seeded multiply-with-carry generator seeded by rand()
+ *     (integrator_tiled.cc:256);
+ *   - the one-light pick of estimateOneDirectLight uses the Halton base-2
+ *     index c = (pixel * total_samples + sample) * bounces + k + 1 (k: the
+ *     path vertex) instead of the per-thread correlative counter
+ *     (integrator_montecarlo.cc:110-114).
+ * Film samples are splatted sample pass by sample pass, pixels in row-major
+ * order (the reference's order depends on its thread schedule).
+ */
+#ifndef YMX_ORA_PT_H
+#define YMX_ORA_PT_H
+
+#include <stdint.h>
+
+#include "oracle.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORA_MAT_SHINYDIFFUSE 0u
+
+typedef struct
+{
+	uint32_t type;          /* ORA_MAT_* */
+	float color[3];         /* "color" */
+	float diffuse_reflect;  /* "diffuse_reflect" (default 1) */
+	float emit;             /* "emit" (default 0) */
+	uint32_t receive_shadows;
+} ora_material;
+
+#define ORA_LIGHT_AREA 0u
+#define ORA_LIGHT_POINT 1u
+
+typedef struct
+{
+	uint32_t type;
+	float corner[3], point1[3], point2[3]; /* area light */
+	float from[3];                          /* point light */
+	float color[3];
+	float power;
+	int32_t samples;                        /* area light samples (default 4) */
+	uint32_t cast_shadows;
+} ora_light;
+
+typedef struct
+{
+	float from[3], to[3], up[3];
+	float focal;
+	float aspect_factor;
+	int32_t resx, resy;
+	int32_t path_samples;
+	int32_t bounces;
+	int32_t rr_min_bounces;
+	int32_t aa_samples;      /* AA_minsamples (one pass) */
+	float aa_pixelwidth;     /* Gauss filter, default 1.5 */
+	int32_t has_background;
+	float background[3];
+	uint32_t seed;
+} ora_pt_settings;
+
+/* Render aa_samples samples per pixel into film sums: film_rgba[4 * (y * resx + x)]
+ * (colour * weight, alpha * weight) and film_w[y * resx + x] (weights).
+ * Buffers are accumulated into (zero them first). */
+void ora_pt_render(const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx, const uint32_t *mat_id,
+                   const ora_material *mats, const ora_light *lights, uint32_t n_lights,
+                   const ora_pt_settings *st, float *film_rgba, float *film_w, int threads);
+
+/* Same, but only the sample values of one pixel sample (no film), for
+ * sample-level checks: col_alpha[4], and the sub-pixel offsets dx, dy. */
+void ora_pt_pixel_sample(const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx, const uint32_t *mat_id,
+                         const ora_material *mats, const ora_light *lights, uint32_t n_lights,
+                         const ora_pt_settings *st, int x, int y, int sample, float col_alpha[4], float dxdy[2]);
+
+/* Samplers (include/sampler/sample.h:100-148, halton.h, halton.cc:419-440). */
+float ora_ri_vdc(uint32_t bits, uint32_t r);
+float ora_ri_s(uint32_t i, uint32_t r);
+float ora_ri_lp(uint32_t i, uint32_t r);
+uint32_t ora_fnv32a(uint32_t value);
+double ora_low_discrepancy(int dim, uint32_t n);
+float ora_halton_next(int base, uint32_t start);  /* Halton(base, start).getNext() */
+int ora_faure_perm(int base, int digit);
+float ora_rr_random(uint32_t pixel, uint32_t sample, uint32_t depth, uint32_t seed);
+/* Gauss filter table of ImageFilm (16 x 16 floats) and its scale. */
+void ora_filter_table(float pixelwidth, float table[256], float *table_scale, float *filter_width);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -91,6 +91,10 @@ void ora_kd_free(ora_kdtree *kd);
 void ora_kd_bound(const ora_kdtree *kd, float a[3], float g[3]);
 uint32_t ora_kd_num_nodes(const ora_kdtree *kd);
 
+/* One ray through the kd-tree port: shadow = 0 closest (t_max rule of
+ * Accelerator::intersect applied by the caller), 1 any hit.  Returns 1 on hit. */
+int ora_kd_intersect1(const ora_kdtree *kd, const ora_ray *ray, float t_max, int shadow, ora_hit *hit);
+
 /* include/accelerator/accelerator_kdtree_common.h:107-255, Nearest. */
 void ora_kd_closest(const ora_kdtree *kd, const ora_ray *rays, uint32_t n_rays,
                     ora_hit *hits, int threads);

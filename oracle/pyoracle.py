@@ -171,3 +171,124 @@ class KdTree:
 
 def fast_pow(a, b) -> float:
     return lib().ora_fast_pow(C.c_float(a), C.c_float(b))
+
+
+# ---------------------------------------------------------------- path tracer
+class OraMaterial(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("color", C.c_float * 3), ("diffuse_reflect", C.c_float),
+                ("emit", C.c_float), ("receive_shadows", C.c_uint32)]
+
+
+class OraLight(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("corner", C.c_float * 3), ("point1", C.c_float * 3),
+                ("point2", C.c_float * 3), ("from_", C.c_float * 3), ("color", C.c_float * 3),
+                ("power", C.c_float), ("samples", C.c_int32), ("cast_shadows", C.c_uint32)]
+
+
+class OraPtSettings(C.Structure):
+    _fields_ = [("from_", C.c_float * 3), ("to", C.c_float * 3), ("up", C.c_float * 3), ("focal", C.c_float),
+                ("aspect_factor", C.c_float), ("resx", C.c_int32), ("resy", C.c_int32),
+                ("path_samples", C.c_int32), ("bounces", C.c_int32), ("rr_min_bounces", C.c_int32),
+                ("aa_samples", C.c_int32), ("aa_pixelwidth", C.c_float), ("has_background", C.c_int32),
+                ("background", C.c_float * 3), ("seed", C.c_uint32)]
+
+
+def _f3v(v):
+    return (C.c_float * 3)(*[float(x) for x in v])
+
+
+def pt_structs(scene):
+    mats = (OraMaterial * len(scene["materials"]))()
+    for i, m in enumerate(scene["materials"]):
+        mats[i].type = 0
+        mats[i].color = _f3v(m["color"])
+        mats[i].diffuse_reflect = float(m.get("diffuse_reflect", 1.0))
+        mats[i].emit = float(m.get("emit", 0.0))
+        mats[i].receive_shadows = 1
+    lights = (OraLight * max(1, len(scene["lights"])))()
+    for i, l in enumerate(scene["lights"]):
+        lights[i].type = 0 if l["type"] == "area" else 1
+        for k in ("corner", "point1", "point2"):
+            setattr(lights[i], k, _f3v(l.get(k, (0, 0, 0))))
+        lights[i].from_ = _f3v(l.get("from", (0, 0, 0)))
+        lights[i].color = _f3v(l.get("color", (1, 1, 1)))
+        lights[i].power = float(l.get("power", 1.0))
+        lights[i].samples = int(l.get("samples", 4))
+        lights[i].cast_shadows = 1
+    cam, s = scene["camera"], scene["settings"]
+    st = OraPtSettings()
+    st.from_ = _f3v(cam["from_"])
+    st.to = _f3v(cam["to"])
+    st.up = _f3v(cam["up"])
+    st.focal = cam["focal"]
+    st.aspect_factor = 1.0
+    st.resx, st.resy = cam["resx"], cam["resy"]
+    st.path_samples, st.bounces, st.rr_min_bounces = s["path_samples"], s["bounces"], s["rr_min_bounces"]
+    st.aa_samples = s["aa_samples"]
+    st.aa_pixelwidth = s["aa_pixelwidth"]
+    bg = s.get("background")
+    st.has_background = 0 if bg is None else 1
+    st.background = _f3v(bg if bg is not None else (0, 0, 0))
+    st.seed = int(s.get("seed", 0))
+    return mats, lights, st
+
+
+def _pt_lib():
+    L = lib()
+    if not getattr(L, "_pt_ready", False):
+        vp = C.c_void_p
+        L.ora_pt_render.argtypes = [vp, vp, vp, vp, vp, vp, C.c_uint32, vp, vp, vp, C.c_int]
+        L.ora_pt_pixel_sample.argtypes = [vp, vp, vp, vp, vp, vp, C.c_uint32, vp, C.c_int, C.c_int, C.c_int, vp, vp]
+        L.ora_ri_vdc.argtypes = [C.c_uint32, C.c_uint32]
+        L.ora_ri_vdc.restype = C.c_float
+        L.ora_ri_s.argtypes = [C.c_uint32, C.c_uint32]
+        L.ora_ri_s.restype = C.c_float
+        L.ora_ri_lp.argtypes = [C.c_uint32, C.c_uint32]
+        L.ora_ri_lp.restype = C.c_float
+        L.ora_fnv32a.argtypes = [C.c_uint32]
+        L.ora_fnv32a.restype = C.c_uint32
+        L.ora_low_discrepancy.argtypes = [C.c_int, C.c_uint32]
+        L.ora_low_discrepancy.restype = C.c_double
+        L.ora_halton_next.argtypes = [C.c_int, C.c_uint32]
+        L.ora_halton_next.restype = C.c_float
+        L.ora_faure_perm.argtypes = [C.c_int, C.c_int]
+        L.ora_faure_perm.restype = C.c_int
+        L.ora_rr_random.argtypes = [C.c_uint32] * 4
+        L.ora_rr_random.restype = C.c_float
+        L.ora_filter_table.argtypes = [C.c_float, vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L._pt_ready = True
+    return L
+
+
+class PathTracer:
+    """CPU restatement of the reference's pathtracing integrator (oracle)."""
+
+    def __init__(self, scene, kd: "KdTree | None" = None):
+        self.scene = scene
+        self.verts = np.ascontiguousarray(scene["verts"], dtype=np.float32)
+        self.tris = np.ascontiguousarray(scene["tris"], dtype=np.uint32)
+        self.mat_id = np.ascontiguousarray(scene["mat_id"], dtype=np.uint32)
+        self.kd = kd or KdTree(self.verts, self.tris)
+        self.mats, self.lights, self.st = pt_structs(scene)
+        self.n_lights = len(scene["lights"])
+
+    def render(self, threads=None, aa_samples=None):
+        L = _pt_lib()
+        st = self.st
+        if aa_samples is not None:
+            st.aa_samples = int(aa_samples)
+        rgba = np.zeros((st.resy, st.resx, 4), dtype=np.float32)
+        w = np.zeros((st.resy, st.resx), dtype=np.float32)
+        L.ora_pt_render(self.kd._h, self.verts.ctypes.data, self.tris.ctypes.data, self.mat_id.ctypes.data,
+                        C.addressof(self.mats), C.addressof(self.lights), self.n_lights, C.addressof(st),
+                        rgba.ctypes.data, w.ctypes.data, threads or default_threads())
+        return rgba, w
+
+    def pixel_sample(self, x, y, sample):
+        L = _pt_lib()
+        out = np.zeros(4, dtype=np.float32)
+        dxdy = np.zeros(2, dtype=np.float32)
+        L.ora_pt_pixel_sample(self.kd._h, self.verts.ctypes.data, self.tris.ctypes.data, self.mat_id.ctypes.data,
+                              C.addressof(self.mats), C.addressof(self.lights), self.n_lights, C.addressof(self.st),
+                              int(x), int(y), int(sample), out.ctypes.data, dxdy.ctypes.data)
+        return out, dxdy
