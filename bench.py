@@ -47,7 +47,50 @@ def parse():
     p.add_argument("--quiet", action="store_true")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run only a plain 1 GiB device-to-device copy (HBM counter calibration)")
+    p.add_argument("--no-pt", action="store_true", help="skip the path-tracing measurement")
+    p.add_argument("--pt-only", action="store_true", help="only the path-tracing measurement")
     return p.parse_args()
+
+
+def measure_path_tracer(args, torch):
+    """Cornell box + 10k random triangles, 256 x 256, 16 spp, 5 bounces: GPU
+    wavefront render time and rays/s over all rays cast, beside the CPU
+    restatement of the reference's integrator on the host cores."""
+    from libyafaray_amd import scenes
+    from libyafaray_amd.pt import GpuPathTracer
+    sc = scenes.cornell_path_scene()
+    pt = GpuPathTracer(sc)
+    pt.clear()
+    pt.render(0, 16)  # warm-up (kernel loads, first-touch)
+    torch.cuda.synchronize()
+    times, st = [], None
+    for _ in range(3):
+        pt.clear()
+        t0 = time.perf_counter()
+        pt.render(0, 16)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        st = pt.stats()
+    pt.close()
+    rays = st["camera_rays"] + st["extension_rays"] + st["shadow_rays"]
+    best = min(times)
+    out = {"workload": "cornell box + 10000 random tris, 256x256, 16 spp, bounces 5, RR from depth 3",
+           "render_ms": round(best * 1e3, 2), "rays": int(rays),
+           "mrays_per_s": round(rays / best / 1e6, 1),
+           "camera_rays": int(st["camera_rays"]), "extension_rays": int(st["extension_rays"]),
+           "shadow_rays": int(st["shadow_rays"])}
+    if args.cpu_sample > 0:
+        from oracle import pyoracle
+        threads = args.cpu_threads or max(1, min(16, os.cpu_count() or 1))
+        cpu = pyoracle.PathTracer(sc)
+        t0 = time.perf_counter()
+        cpu.render(threads=threads)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_port_render_ms"] = round(cpu_s * 1e3, 1)
+        out["cpu_port_threads"] = threads
+        out["speedup_vs_cpu_port"] = round(cpu_s / best, 1)
+    log(args, f"path tracer: {out}")
+    return out
 
 
 def calibrate_copy():
@@ -101,6 +144,9 @@ def main():
     torch.cuda.set_device(0)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.pt_only:
+        print(json.dumps({"path_tracing": measure_path_tracer(args, torch)}), flush=True)
         return
 
     t0 = time.perf_counter()
@@ -202,6 +248,8 @@ def main():
             "bvh_depth": info["max_depth"],
         },
     }
+    if not args.no_pt:
+        out["extra"]["path_tracing"] = measure_path_tracer(args, torch)
     print(json.dumps(out), flush=True)
 
 

@@ -791,6 +791,17 @@ static void pixel_sample(const pt_ctx *c, int x, int y, int sample, float col[4]
 	*dy_out = dy;
 }
 
+void ora_pt_camera_dir(const ora_pt_settings *st, float px, float py, float dir[3])
+{
+	pt_ctx c;
+	memset(&c, 0, sizeof c);
+	setup_ctx(&c, NULL, NULL, NULL, NULL, NULL, NULL, 0, st);
+	const v3 d = vnorm(vadd(vadd(vmul(px, c.vright), vmul(py, c.vup)), c.vto));
+	dir[0] = d.x;
+	dir[1] = d.y;
+	dir[2] = d.z;
+}
+
 void ora_pt_pixel_sample(const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx, const uint32_t *mat_id,
                          const ora_material *mats, const ora_light *lights, uint32_t n_lights,
                          const ora_pt_settings *st, int x, int y, int sample, float col_alpha[4], float dxdy[2])

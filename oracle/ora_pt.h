@@ -92,6 +92,9 @@ void ora_pt_pixel_sample(const ora_kdtree *kd, const float *verts, const uint32_
                          const ora_material *mats, const ora_light *lights, uint32_t n_lights,
                          const ora_pt_settings *st, int x, int y, int sample, float col_alpha[4], float dxdy[2]);
 
+/* PerspectiveCamera::shootRay direction for film coordinates (px, py). */
+void ora_pt_camera_dir(const ora_pt_settings *st, float px, float py, float dir[3]);
+
 /* Samplers (include/sampler/sample.h:100-148, halton.h, halton.cc:419-440). */
 float ora_ri_vdc(uint32_t bits, uint32_t r);
 float ora_ri_s(uint32_t i, uint32_t r);

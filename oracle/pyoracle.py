@@ -292,3 +292,20 @@ class PathTracer:
                               C.addressof(self.mats), C.addressof(self.lights), self.n_lights, C.addressof(self.st),
                               int(x), int(y), int(sample), out.ctypes.data, dxdy.ctypes.data)
         return out, dxdy
+
+
+def camera_dir(scene, px, py):
+    _, _, st = pt_structs(scene)
+    L = _pt_lib()
+    L.ora_pt_camera_dir.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_void_p]
+    out = np.zeros(3, dtype=np.float32)
+    L.ora_pt_camera_dir(C.addressof(st), float(px), float(py), out.ctypes.data)
+    return out
+
+
+def filter_table(pixelwidth=1.5):
+    L = _pt_lib()
+    t = np.zeros(256, dtype=np.float32)
+    scale, width = C.c_float(), C.c_float()
+    L.ora_filter_table(C.c_float(pixelwidth), t.ctypes.data, C.byref(scale), C.byref(width))
+    return t.reshape(16, 16), scale.value, width.value

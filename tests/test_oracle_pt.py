@@ -1,0 +1,197 @@
+"""Pins the CPU restatement of the reference's path tracer (oracle/ora_pt.c):
+samplers against the published sequences and the reference's own tables,
+the filter table against its closed form, the camera against its field of
+view, and the direct-lighting estimator against an analytic irradiance."""
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from libyafaray_amd import scenes
+
+REF_HALTON = Path("/root/reference/src/sampler/halton.cc")
+
+
+@pytest.fixture(scope="module")
+def L(oracle):
+    return oracle._pt_lib()
+
+
+# ---------------------------------------------------------------- samplers
+def test_van_der_corput_known_values(L):
+    assert [L.ora_ri_vdc(i, 0) for i in range(8)] == [0, 0.5, 0.25, 0.75, 0.125, 0.625, 0.375, 0.875]
+    assert L.ora_ri_vdc(0, 1 << 31) == 0.5  # the scramble XORs the reversed bits
+
+
+def test_sobol_and_lp_known_values(L):
+    # second Sobol' dimension, direction numbers m = 1, 3, 5 (v = 1/2, 3/4, 5/8), and the
+    # Larcher-Pillichshammer sequence, v = 1/2, 3/4, 7/8 (Kollig & Keller 2002); x(i) = XOR of v_k over bits of i
+    assert [L.ora_ri_s(i, 0) for i in range(6)] == [0, 0.5, 0.75, 0.25, 0.625, 0.125]
+    assert [L.ora_ri_lp(i, 0) for i in range(6)] == [0, 0.5, 0.75, 0.25, 0.875, 0.375]
+
+
+def fnv1a_bytes(data: bytes) -> int:
+    h = 2166136261
+    for b in data:
+        h ^= b
+        h = (h * 16777619) & 0xFFFFFFFF
+    return h
+
+
+def test_fnv32a_is_fnv1a_of_little_endian_bytes(L):
+    rng = np.random.default_rng(0)
+    for v in [0, 1, 255, 256, 0xDEADBEEF, *rng.integers(0, 2**32, 200).tolist()]:
+        assert L.ora_fnv32a(int(v)) == fnv1a_bytes(int(v).to_bytes(4, "little"))
+
+
+def test_faure_permutations_match_reference_tables(L):
+    if not REF_HALTON.exists():
+        pytest.skip("reference sources not mounted")
+    text = REF_HALTON.read_text()
+    tables = {int(b): [int(x) for x in body.replace("\n", " ").split(",") if x.strip()]
+              for b, body in re.findall(r"base_(\d+)\s*\{([^}]*)\}", text)}
+    assert len(tables) >= 49
+    for base, perm in tables.items():
+        assert [L.ora_faure_perm(base, d) for d in range(base)] == perm, base
+
+
+def test_faure_permutations_published_values(L):
+    # Faure (1992): sigma_5 = (0 3 2 1 4), sigma_7 = (0 2 5 3 1 4 6), sigma_11 = (0 7 4 2 9 5 1 8 6 3 10)
+    assert [L.ora_faure_perm(5, d) for d in range(5)] == [0, 3, 2, 1, 4]
+    assert [L.ora_faure_perm(7, d) for d in range(7)] == [0, 2, 5, 3, 1, 4, 6]
+    assert [L.ora_faure_perm(11, d) for d in range(11)] == [0, 7, 4, 2, 9, 5, 1, 8, 6, 3, 10]
+    for b in (13, 17, 97, 227):
+        assert sorted(L.ora_faure_perm(b, d) for d in range(b)) == list(range(b))
+
+
+PRIMS = [1, 2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67, 71, 73, 79, 83, 89, 97,
+         101, 103, 107, 109, 113, 127, 131, 137, 139, 149, 151, 157, 163, 167, 173, 179, 181, 191, 193, 197,
+         199, 211, 223, 227]
+
+
+def test_low_discrepancy_scrambled_radical_inverse(L):
+    """Equals the Faure-scrambled radical inverse wherever the reference's
+    9-decimal reciprocal extracts the same digits as integer division."""
+    checked = 0
+    for dim in (2, 3, 5, 7, 8, 11, 12, 20, 40):
+        b = PRIMS[dim]
+        table = 3 if dim <= 2 else b
+        f = round(1.0 / b, 9)
+        for n in range(1, 3000):
+            digits, m, dn, same = [], n, float(n), True
+            while m > 0:
+                digits.append(m % b)
+                dn *= f
+                nxt = int(dn)
+                if nxt != m // b:
+                    same = False
+                    break
+                m = nxt
+            if not same:
+                continue
+            exact = sum(L.ora_faure_perm(table, dg) / b ** (k + 1) for k, dg in enumerate(digits))
+            assert abs(L.ora_low_discrepancy(dim, n) - exact) < 1e-7, (dim, n)
+            checked += 1
+    assert checked > 10000
+
+
+def test_low_discrepancy_truncated_reciprocal_quirk(L):
+    # 3 * 0.333333333 = 0.999999999 truncates to 0: the second digit of n = 3 is lost
+    assert L.ora_low_discrepancy(2, 3) == 0.0
+    assert L.ora_low_discrepancy(2, 1) == 0.333333333
+
+
+def test_halton_next_is_next_radical_inverse(L):
+    def radical(b, n):
+        v, f = 0.0, 1.0 / b
+        while n:
+            v += (n % b) * f
+            n //= b
+            f /= b
+        return v
+    for b in (2, 3):
+        for s in range(0, 2000):
+            assert abs(L.ora_halton_next(b, s) - radical(b, s + 1)) < 1e-6
+
+
+def test_rr_random_is_uniform(L):
+    vals = np.array([L.ora_rr_random(p, s, d, 0) for p in range(64) for s in range(8) for d in range(8)])
+    assert vals.min() >= 0.0 and vals.max() < 1.0
+    assert abs(vals.mean() - 0.5) < 0.02 and abs(vals.var() - 1 / 12) < 0.01
+
+
+# ---------------------------------------------------------------- film / camera
+def test_gauss_filter_table(oracle):
+    t, scale, width = oracle.filter_table(1.5)
+    assert width == 1.5 and abs(scale - 0.9999 * 16 / 1.5) < 1e-6
+    x = (np.arange(16) + 0.5) / 16
+    r2 = x[None, :] ** 2 + x[:, None] ** 2
+    closed = np.maximum(0.0, np.exp(-6 * r2) - 0.00247875)
+    assert np.allclose(t, closed, rtol=4e-4, atol=1e-6)
+
+
+def test_camera_axis_and_field_of_view(oracle):
+    sc = scenes.cornell_path_scene(res=64)
+    cam = sc["camera"]
+    to = np.subtract(cam["to"], cam["from_"])
+    centre = oracle.camera_dir(sc, 32.0, 32.0)
+    assert np.allclose(centre, to / np.linalg.norm(to), atol=1e-6)
+    left = oracle.camera_dir(sc, 0.0, 32.0)
+    half_fov = math.degrees(math.acos(float(np.dot(left, centre))))
+    assert abs(half_fov - 20.0) < 1e-3
+    # image rows grow downwards (towards -z for this camera)
+    assert oracle.camera_dir(sc, 32.0, 63.0)[2] < centre[2]
+
+
+# ---------------------------------------------------------------- estimator
+def plane_under_light(side=0.1, height=1.0, albedo=0.6, res=4, spp=64):
+    """A large Lambertian floor lit by a small downward-facing area light; a
+    camera with a very narrow field looks straight down at the spot below
+    the light (the light has no geometry, so camera rays pass it)."""
+    verts = np.array([[-50, -50, 0], [50, -50, 0], [50, 50, 0], [-50, 50, 0]], np.float32)
+    tris = np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+    h = side / 2
+    light = dict(type="area", corner=(-h, -h, height), point1=(-h, h, height), point2=(h, -h, height),
+                 color=(1.0, 1.0, 1.0), power=1.0, samples=4)
+    return dict(verts=verts, tris=tris, mat_id=np.zeros(2, np.uint32),
+                materials=[dict(color=(albedo, albedo, albedo))], lights=[light],
+                camera=dict(from_=(0.0, 0.0, 5.0), to=(0.0, 0.0, 0.0), up=(0.0, 1.0, 5.0), focal=2000.0,
+                            resx=res, resy=res),
+                settings=dict(path_samples=1, bounces=1, rr_min_bounces=0, aa_samples=spp, aa_pixelwidth=1.5,
+                              background=None, seed=0))
+
+
+def test_direct_light_matches_form_factor(oracle):
+    """Reference conventions: Lambert eval = albedo (no 1/pi), light colour
+    scaled by pi and pdf carrying pi and the quad area, so the pixel under
+    the light reads albedo * power * integral of cos cos' / r^2 over the quad."""
+    side, height, albedo = 0.1, 1.0, 0.6
+    sc = plane_under_light(side, height, albedo)
+    pt = oracle.PathTracer(sc)
+    rgba, w = pt.render()
+    img = rgba[..., 0] / w
+    g = (np.arange(400) + 0.5) / 400 * side - side / 2
+    X, Y = np.meshgrid(g, g)
+    integral = (height ** 2 / (X ** 2 + Y ** 2 + height ** 2) ** 2).mean() * side * side
+    expected = albedo * 1.0 * integral
+    assert np.allclose(img, expected, rtol=0.01)
+
+
+def test_power_scales_linearly_and_shadows_block(oracle):
+    sc = plane_under_light()
+    base = oracle.PathTracer(sc).render()
+    sc["lights"][0]["power"] = 3.0
+    tripled = oracle.PathTracer(sc).render()
+    assert np.allclose(tripled[0][..., :3], 3 * base[0][..., :3], rtol=1e-5)
+    # an occluder between floor and light removes the direct light
+    sc = plane_under_light()
+    occ = np.array([[-1, -1, 0.5], [1, -1, 0.5], [1, 1, 0.5], [-1, 1, 0.5]], np.float32)
+    sc["verts"] = np.concatenate([sc["verts"], occ])
+    sc["tris"] = np.concatenate([sc["tris"], np.array([[4, 5, 6], [4, 6, 7]], np.uint32)])
+    sc["mat_id"] = np.zeros(4, np.uint32)
+    sc["camera"]["from_"] = (0.0, 0.0, 0.4)
+    sc["camera"]["up"] = (0.0, 1.0, 0.4)
+    rgba, w = oracle.PathTracer(sc).render()
+    assert np.all(rgba[..., :3] == 0.0)
