@@ -306,12 +306,20 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 		PT_TRY(hipMemcpy(pt->d_prims, bases.data(), sizeof(uint32_t) * bases.size(), hipMemcpyHostToDevice));
 	}
 
-	// wavefront state
-	const uint32_t n = (uint32_t)(st->resx * st->resy);
+	// wavefront state: `batch` AA samples of every pixel per wavefront, bounded
+	// by 16M paths and 256M shadow slots (about 13 GB of slot state)
+	const uint32_t npix = (uint32_t)(st->resx * st->resy);
 	PtState &s = pt->st;
-	s.capacity = n;
 	s.slots = slots_camera > slots_path ? slots_camera : slots_path;
 	if(s.slots == 0) s.slots = 1;
+	uint64_t max_paths = 1ull << 24;
+	if(max_paths * s.slots > (1ull << 28)) max_paths = (1ull << 28) / s.slots;
+	uint64_t batch = max_paths / npix;
+	if(batch < 1) batch = 1;
+	if(batch > (uint64_t)st->aa_samples) batch = (uint64_t)st->aa_samples;
+	pt->batch = (uint32_t)batch;
+	const uint32_t n = (uint32_t)(npix * batch);
+	s.capacity = n;
 	const size_t slot_cap = (size_t)n * s.slots;
 	if(slot_cap > 0xFFFFFFFFull)
 	{
@@ -319,10 +327,10 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 		free_pt(pt);
 		return YMX_ERR_UNSUPPORTED;
 	}
-	PT_TRY(dalloc(pt->d_film, n));
-	PT_TRY(dalloc(pt->d_film_w, n));
-	PT_TRY(hipMemset(pt->d_film, 0, sizeof(float4) * n));
-	PT_TRY(hipMemset(pt->d_film_w, 0, sizeof(float) * n));
+	PT_TRY(dalloc(pt->d_film, npix));
+	PT_TRY(dalloc(pt->d_film_w, npix));
+	PT_TRY(hipMemset(pt->d_film, 0, sizeof(float4) * npix));
+	PT_TRY(hipMemset(pt->d_film_w, 0, sizeof(float) * npix));
 	PT_TRY(dalloc(s.col, n));
 	PT_TRY(dalloc(s.path_col, n));
 	PT_TRY(dalloc(s.tp_next, n));
@@ -372,7 +380,7 @@ int ymx_pt_clear_film(ymx_pt *pt, void *stream)
 		ymx_set_error("ymx_pt_clear_film: NULL handle");
 		return YMX_ERR_INVALID_ARG;
 	}
-	const uint32_t n = pt->st.capacity;
+	const uint32_t n = (uint32_t)(pt->host_const.resx * pt->host_const.resy);  // film pixels
 	hipStream_t s = (hipStream_t)stream;
 	YMX_HIP_CHECK(hipMemsetAsync(pt->d_film, 0, sizeof(float4) * n, s));
 	YMX_HIP_CHECK(hipMemsetAsync(pt->d_film_w, 0, sizeof(float) * n, s));
@@ -389,9 +397,10 @@ int ymx_pt_render(ymx_pt *pt, int32_t first_sample, int32_t n_samples, void *str
 	hipStream_t s = (hipStream_t)stream;
 	YMX_HIP_CHECK(hipMemsetAsync(pt->st.totals, 0, 3 * sizeof(unsigned long long), s));
 	YMX_HIP_CHECK(hipEventRecord(pt->ev0, s));
-	for(int32_t k = 0; k < n_samples; ++k)
+	for(int32_t k = 0; k < n_samples; k += (int32_t)pt->batch)
 	{
-		const int rc = ymx_pt_trace_pass(pt, first_sample + k, s);
+		const int32_t count = (n_samples - k) < (int32_t)pt->batch ? (n_samples - k) : (int32_t)pt->batch;
+		const int rc = ymx_pt_trace_pass(pt, first_sample + k, count, s);
 		if(rc != YMX_OK) return rc;
 	}
 	YMX_HIP_CHECK(hipEventRecord(pt->ev1, s));
@@ -407,7 +416,7 @@ int ymx_pt_read_film(ymx_pt *pt, float *rgba, float *weight)
 		return YMX_ERR_INVALID_ARG;
 	}
 	YMX_HIP_CHECK(hipDeviceSynchronize());
-	const uint32_t n = pt->st.capacity;
+	const uint32_t n = (uint32_t)(pt->host_const.resx * pt->host_const.resy);  // film pixels
 	YMX_HIP_CHECK(hipMemcpy(rgba, pt->d_film, sizeof(float4) * n, hipMemcpyDeviceToHost));
 	YMX_HIP_CHECK(hipMemcpy(weight, pt->d_film_w, sizeof(float) * n, hipMemcpyDeviceToHost));
 	return YMX_OK;

@@ -104,10 +104,12 @@ struct ymx_pt
 	float4 *d_film;                  // resx * resy (colour sums, alpha sum)
 	float *d_film_w;                 // resx * resy
 	ymx::PtState st;
+	uint32_t batch;                  // AA samples per wavefront
 	ymx::PtConst host_const;
 	ymx_pt_stats stats;
 	hipEvent_t ev0, ev1;
 };
 
-// Kernel launches of one AA sample pass (pt_kernels.hip).
-int ymx_pt_trace_pass(ymx_pt *pt, int32_t sample, hipStream_t stream);
+// Kernel launches of one wavefront: AA samples [first_sample, first_sample + count)
+// of every pixel (pt_kernels.hip); count * resx * resy <= st.capacity.
+int ymx_pt_trace_pass(ymx_pt *pt, int32_t first_sample, int32_t count, hipStream_t stream);

@@ -42,10 +42,19 @@ struct PtArgs
 	PtState st;
 	float4 *film;
 	float *film_w;
-	int32_t sample;
+	int32_t first_sample;  // AA sample index of the wavefront's first slice
+	uint32_t npix;         // resx * resy
+	uint32_t n_paths;      // npix * count: path i traces pixel i % npix, sample first + i / npix
+	int32_t count;         // AA samples in this wavefront
 	int32_t sub;
 	int32_t depth;
 };
+
+__device__ __forceinline__ uint32_t pix_of(const PtArgs &a, uint32_t i) { return i % a.npix; }
+__device__ __forceinline__ uint32_t sample_of(const PtArgs &a, uint32_t i)
+{
+	return (uint32_t)a.first_sample + i / a.npix;
+}
 
 struct Surf
 {
@@ -433,7 +442,7 @@ __device__ __forceinline__ void queue_extension(const PtArgs &a, int qsel, uint3
 __global__ __launch_bounds__(kBlock) void k_camera(PtArgs a)
 {
 	const PtConst &c = *a.c;
-	const uint32_t n = (uint32_t)(c.resx * c.resy);
+	const uint32_t n = a.n_paths;
 	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
 	if(i == 0)
 	{
@@ -442,14 +451,16 @@ __global__ __launch_bounds__(kBlock) void k_camera(PtArgs a)
 		a.st.totals[0] += n;
 	}
 	if(i >= n) return;
-	const uint32_t x = i % (uint32_t)c.resx, y = i / (uint32_t)c.resx;
+	const uint32_t p = pix_of(a, i);
+	const uint32_t sample = sample_of(a, i);
+	const uint32_t x = p % (uint32_t)c.resx, y = p / (uint32_t)c.resx;
 	const uint32_t offset = path_offset(x, y);
 	float dx = 0.5f, dy = 0.5f;
 	if(c.aa_samples > 1)
 	{
 		const float d_1 = 1.f / (float)c.aa_samples;
-		dx = (0.5f + (float)a.sample) * d_1;
-		dy = ri_lp((uint32_t)a.sample + offset, 0);
+		dx = (0.5f + (float)sample) * d_1;
+		dy = ri_lp(sample + offset, 0);
 	}
 	const float px = (float)x + dx, py = (float)y + dy;
 	const V3 dir = normalize(add(add(mul(px, c.vright), mul(py, c.vup)), c.vto));
@@ -470,7 +481,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_camera(PtArgs a)
 {
 	const PtConst &c = *a.c;
 	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-	if(i >= (uint32_t)(c.resx * c.resy)) return;  // the camera queue is the identity
+	if(i >= a.n_paths) return;  // the camera queue is the identity
 	const ymx_hit h = a.st.ext_hits[i];
 	a.st.cam_hit[i] = h;
 	if(h.prim == YMX_NO_HIT)
@@ -490,12 +501,12 @@ __global__ __launch_bounds__(kBlock) void k_shade_camera(PtArgs a)
 	uint32_t f = 0u;
 	if(sp.flags & kDiffuse)
 	{
-		const uint32_t x = i % (uint32_t)c.resx, y = i / (uint32_t)c.resx;
-		const uint32_t offset = path_offset(x, y);
+		const uint32_t offset = a.st.offset[i];
+		const uint32_t sample = sample_of(a, i);
 		uint32_t slot = i * a.st.slots;
 		for(uint32_t l = 0; l < c.n_lights; ++l)
 		{
-			queue_light(a, sp, wo, a.lights[l], l, (uint32_t)a.sample, offset, slot);
+			queue_light(a, sp, wo, a.lights[l], l, sample, offset, slot);
 			slot += light_slots(a.lights[l]);
 		}
 		f |= kDirect;
@@ -516,7 +527,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve_camera(PtArgs a)
 		a.st.ext_count[0] = 0u;
 		a.st.ext_count[1] = 0u;
 	}
-	if(i >= (uint32_t)(c.resx * c.resy)) return;
+	if(i >= a.n_paths) return;
 	if(!(a.st.flags[i] & kDirect)) return;
 	Rgb col_all = rgb(0.f, 0.f, 0.f);
 	uint32_t slot = i * a.st.slots;
@@ -536,7 +547,7 @@ __global__ __launch_bounds__(kBlock) void k_subpath_init(PtArgs a)
 {
 	const PtConst &c = *a.c;
 	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-	if(i >= (uint32_t)(c.resx * c.resy)) return;
+	if(i >= a.n_paths) return;
 	uint32_t f = a.st.flags[i];
 	f &= ~(kActive | kAddEmit | kCaustic | kDirect | kVertex | kSampled);
 	a.st.flags[i] = f;
@@ -547,7 +558,7 @@ __global__ __launch_bounds__(kBlock) void k_subpath_init(PtArgs a)
 	Surf sp;
 	make_surf(a, c.cam_from, dir, h, sp);
 	const V3 wo = neg(dir);
-	const uint32_t offs = (uint32_t)c.path_samples * (uint32_t)a.sample + a.st.offset[i] + (uint32_t)a.sub;
+	const uint32_t offs = (uint32_t)c.path_samples * sample_of(a, i) + a.st.offset[i] + (uint32_t)a.sub;
 	BSample s;
 	s.s1 = ri_vdc(offs, 0);
 	s.s2 = (float)low_discrepancy(a, 2, offs);
@@ -590,8 +601,8 @@ __global__ __launch_bounds__(kBlock) void k_shade_path(PtArgs a)
 	const float4 tpn = a.st.tp_next[i];
 	Rgb tp = rgb(tpn.x, tpn.y, tpn.z);
 	uint32_t first_flags;
-	const uint32_t x = i % (uint32_t)c.resx, y = i / (uint32_t)c.resx;
-	const uint32_t pixel = (uint32_t)c.resx * y + x;
+	const uint32_t pixel = pix_of(a, i);  // camera_res_x * y + x
+	const uint32_t sample = sample_of(a, i);
 	const uint32_t offset = a.st.offset[i];
 	if(d == 0)
 	{
@@ -606,7 +617,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_path(PtArgs a)
 		first_flags = (f >> 8) & 0x1FFu;
 		if(d > c.rr_min_bounces)
 		{
-			const float random_value = rr_random(pixel, (uint32_t)a.sample * (uint32_t)c.path_samples + (uint32_t)a.sub,
+			const float random_value = rr_random(pixel, sample * (uint32_t)c.path_samples + (uint32_t)a.sub,
 			                                     (uint32_t)d, c.seed);
 			const float probability = cmaximum(tp);
 			if(probability <= 0.f || probability < random_value)
@@ -625,13 +636,13 @@ __global__ __launch_bounds__(kBlock) void k_shade_path(PtArgs a)
 	{
 		// estimateOneDirectLight (integrator_montecarlo.cc:108-116) with the
 		// deterministic Halton base-2 light-pick index
-		const uint32_t corr = (pixel * (uint32_t)c.aa_samples + (uint32_t)a.sample) *
+		const uint32_t corr = (pixel * (uint32_t)c.aa_samples + sample) *
 		                          ((uint32_t)c.path_samples * (uint32_t)c.bounces) +
 		                      (uint32_t)a.sub * (uint32_t)c.bounces + (uint32_t)d + 1u;
 		const float hv = halton_next(2, corr - 1u);
 		int lnum = (int)(hv * (float)c.n_lights);
 		if(lnum > (int)c.n_lights - 1) lnum = (int)c.n_lights - 1;
-		queue_light(a, sp, pwo, a.lights[lnum], (uint32_t)lnum, (uint32_t)a.sample, offset, i * a.st.slots);
+		queue_light(a, sp, pwo, a.lights[lnum], (uint32_t)lnum, sample, offset, i * a.st.slots);
 		a.st.light[i] = lnum;
 		f |= kDirect;
 	}
@@ -643,7 +654,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_path(PtArgs a)
 	// next segment (loop iteration depth = d + 1)
 	if(d + 1 < c.bounces)
 	{
-		const uint32_t offs = (uint32_t)c.path_samples * (uint32_t)a.sample + offset + (uint32_t)a.sub;
+		const uint32_t offs = (uint32_t)c.path_samples * sample + offset + (uint32_t)a.sub;
 		const int d_4 = 4 * (d + 1);
 		BSample s;
 		s.s1 = (float)low_discrepancy(a, d_4 + 3, offs);
@@ -679,7 +690,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve_path(PtArgs a)
 		*a.st.sh_count = 0u;
 		a.st.ext_count[a.depth & 1] = 0u;  // this segment's queue, consumed by shade_path
 	}
-	if(i >= (uint32_t)(c.resx * c.resy)) return;
+	if(i >= a.n_paths) return;
 	const uint32_t f = a.st.flags[i];
 	if(!(f & kVertex)) return;
 	Rgb lcol = rgb(0.f, 0.f, 0.f);
@@ -700,7 +711,7 @@ __global__ __launch_bounds__(kBlock) void k_finish(PtArgs a)
 {
 	const PtConst &c = *a.c;
 	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-	if(i >= (uint32_t)(c.resx * c.resy)) return;
+	if(i >= a.n_paths) return;
 	if(!(a.st.flags[i] & kPathBlock)) return;
 	const int n_samples = c.path_samples > 1 ? c.path_samples : 1;
 	const float4 p4 = a.st.path_col[i];
@@ -725,13 +736,15 @@ __global__ __launch_bounds__(kBlock) void k_film(PtArgs a)
 	float w = a.film_w[t];
 	float4 acc = a.film[t];
 	const double fw = (double)c.filter_width;
+	// sample slices in AA order, then source pixels in row-major order
+	for(int j = 0; j < a.count; ++j)
 	for(int sy = ty - radius; sy <= ty + radius; ++sy)
 	{
 		if(sy < 0 || sy >= c.resy) continue;
 		for(int sx = tx - radius; sx <= tx + radius; ++sx)
 		{
 			if(sx < 0 || sx >= c.resx) continue;
-			const uint32_t s = (uint32_t)(sy * c.resx + sx);
+			const uint32_t s = (uint32_t)j * a.npix + (uint32_t)(sy * c.resx + sx);
 			const float2 dd = a.st.dxy[s];
 			int dx0 = round_bias((double)dd.x - fw);
 			int dx1 = round_bias((double)dd.x + fw - 1.0);
@@ -765,11 +778,17 @@ inline unsigned blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 }  // namespace
 }  // namespace ymx
 
-int ymx_pt_trace_pass(ymx_pt *pt, int32_t sample, hipStream_t stream)
+int ymx_pt_trace_pass(ymx_pt *pt, int32_t first_sample, int32_t count, hipStream_t stream)
 {
 	using namespace ymx;
 	const PtConst &hc = pt->host_const;
-	const uint32_t n = (uint32_t)(hc.resx * hc.resy);
+	const uint32_t npix = (uint32_t)(hc.resx * hc.resy);
+	const uint32_t n = npix * (uint32_t)count;
+	if(count < 1 || n > pt->st.capacity)
+	{
+		ymx_set_error("ymx_pt_trace_pass: %d samples exceed the wavefront capacity", count);
+		return YMX_ERR_INVALID_ARG;
+	}
 	PtArgs a;
 	a.c = pt->d_const;
 	a.mats = pt->d_materials;
@@ -781,7 +800,10 @@ int ymx_pt_trace_pass(ymx_pt *pt, int32_t sample, hipStream_t stream)
 	a.st = pt->st;
 	a.film = pt->d_film;
 	a.film_w = pt->d_film_w;
-	a.sample = sample;
+	a.first_sample = first_sample;
+	a.npix = npix;
+	a.n_paths = n;
+	a.count = count;
 	a.sub = 0;
 	a.depth = 0;
 	const unsigned g = blocks_for(n);
@@ -825,7 +847,7 @@ int ymx_pt_trace_pass(ymx_pt *pt, int32_t sample, hipStream_t stream)
 	}
 	k_finish<<<g, kBlock, 0, stream>>>(a);
 	YMX_HIP_CHECK(hipGetLastError());
-	k_film<<<g, kBlock, 0, stream>>>(a);
+	k_film<<<blocks_for(npix), kBlock, 0, stream>>>(a);
 	YMX_HIP_CHECK(hipGetLastError());
 	return YMX_OK;
 }
