@@ -10,17 +10,24 @@
  *  src/render/imagefilm.cc:771-822), with the scene's accelerator built by
  * ymx_accel_create (include/ymx_accel.h).
  *
- * All pixels of one AA sample index are traced as one wavefront: camera
+ * Several AA sample indices of all pixels are traced as one wavefront: camera
  * rays, closest-hit extension, shading with next-event-estimation shadow rays
- * queued and traced in bulk, then a deterministic film gather.  Conventions
+ * queued and traced in bulk, the recursive specular / glossy rays level by
+ * level, then a deterministic film gather.  Conventions
  * as in ymx_accel.h: int status, ymx_last_error(), hipStream_t as void *.
  * A handle's calls must not overlap in time (it owns one set of wavefront
  * buffers and the film).
  *
- * Materials: shinydiffuse (diffuse colour, diffuse_reflect, emit).  Lights:
- * area (corner / point1 / point2, colour, power, samples) and point (from,
- * colour, power).  Param meanings and defaults follow the reference's
- * ParamMaps (include/material/material_shiny_diffuse.h:85-96,
+ * Materials: shinydiffuse (diffuse colour, diffuse_reflect, emit), glossy
+ * (isotropic Blinn / Ashikhmin-Shirley with optional Lambert diffuse), mirror,
+ * glass (no dispersion or absorption) and light_mat; specular and glossy
+ * (as_diffuse = 0) surfaces recurse as the reference's recursiveRaytrace does,
+ * up to `raydepth`.  Lights: area (corner / point1 / point2, colour, power,
+ * samples) and point (from, colour, power).  Param meanings and defaults follow
+ * the reference's ParamMaps (include/material/material_shiny_diffuse.h:85-96,
+ * include/material/material_glossy.h:68-80, include/material/material_mirror.h:57-58,
+ * include/material/material_glass.h:68-75, include/material/material_light.h:59-61,
+ * include/integrator/surface/integrator_montecarlo.h:47,
  * include/light/light_area.h:50-57, include/light/light_point.h:52-54,
  * include/integrator/surface/integrator_path_tracer.h:55-58,
  * include/render/imagefilm.h:196-225, include/camera/camera.h:82-89).
@@ -36,17 +43,35 @@
 extern "C" {
 #endif
 
-#define YMX_MAT_SHINYDIFFUSE 0u
+#define YMX_MAT_SHINYDIFFUSE 0u /* "shinydiffusemat" */
+#define YMX_MAT_GLOSSY 1u       /* "glossy" */
+#define YMX_MAT_MIRROR 2u       /* "mirror" */
+#define YMX_MAT_GLASS 3u        /* "glass" */
+#define YMX_MAT_LIGHT 4u        /* "light_mat" */
 #define YMX_LIGHT_AREA 0u
 #define YMX_LIGHT_POINT 1u
 
+/* One record for every material type; each type reads its own fields, named
+ * after the reference's params (defaults in brackets). */
 typedef struct ymx_material
 {
 	uint32_t type;            /* YMX_MAT_* */
-	float color[3];           /* "color" */
-	float diffuse_reflect;    /* "diffuse_reflect", default 1 */
-	float emit;               /* "emit", default 0 */
-	uint32_t receive_shadows; /* "receive_shadows", default 1 */
+	float color[3];           /* "color" of shinydiffuse, glossy, mirror, light_mat [1] */
+	float diffuse_reflect;    /* "diffuse_reflect": shinydiffuse [1], glossy [0] */
+	float emit;               /* shinydiffuse "emit" [0] */
+	uint32_t receive_shadows; /* "receive_shadows" [1] */
+	float diffuse_color[3];   /* glossy "diffuse_color" [1] */
+	float glossy_reflect;     /* glossy "glossy_reflect" [1] */
+	float exponent;           /* glossy "exponent" (Blinn) [50] */
+	uint32_t as_diffuse;      /* glossy "as_diffuse" [1] */
+	float reflect;            /* mirror "reflect" [1] */
+	float ior;                /* glass "IOR" [1.4] */
+	float filter_color[3];    /* glass "filter_color" [1] */
+	float transmit_filter;    /* glass "transmit_filter" [0] */
+	float mirror_color[3];    /* glass "mirror_color" [1] */
+	uint32_t fake_shadows;    /* glass "fake_shadows" [0] */
+	float power;              /* light_mat "power" [1] */
+	uint32_t double_sided;    /* light_mat "double_sided" [0] */
 } ymx_material;
 
 typedef struct ymx_light
@@ -74,6 +99,7 @@ typedef struct ymx_pt_settings
 	int32_t has_background;       /* constant background seen by camera rays */
 	float background[3];
 	uint32_t seed;                /* seed of the Russian-roulette stream */
+	int32_t raydepth;             /* "raydepth" of the recursive raytrace, default 5 (0..10) */
 } ymx_pt_settings;
 
 typedef struct ymx_pt ymx_pt;
@@ -99,12 +125,12 @@ int ymx_pt_render(ymx_pt *pt, int32_t first_sample, int32_t n_samples, void *str
  * filter weights; the pixel value is rgba / weight (ImageFilm::flush). */
 int ymx_pt_read_film(ymx_pt *pt, float *rgba, float *weight);
 
-/* Counters of the last ymx_pt_render: rays cast (camera + extension +
- * shadow), and kernel time in ms of the extension and shadow launches. */
+/* Counters of the last ymx_pt_render: rays cast, and its duration in ms on
+ * the stream (events around the whole render). */
 typedef struct ymx_pt_stats
 {
 	uint32_t struct_size;
-	uint64_t camera_rays;
+	uint64_t camera_rays;         /* camera rays, plus the rays of the recursive raytrace */
 	uint64_t extension_rays;
 	uint64_t shadow_rays;
 	float render_ms;

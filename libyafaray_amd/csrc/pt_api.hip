@@ -56,15 +56,30 @@ constexpr int kBases[kLdsDims] = {1, 2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37,
                                   59, 61, 67, 71, 73, 79, 83, 89, 97, 101, 103, 107, 109, 113, 127, 131,
                                   137, 139, 149, 151, 157, 163, 167, 173, 179, 181, 191, 193, 197, 199, 211, 223, 227};
 
+void free_nodes(PtNodes &nd)
+{
+	void *ptrs[] = {nd.ray, nd.info, nd.dec, nd.col, nd.wt, nd.child};
+	for(void *p : ptrs)
+		if(p) (void)hipFree(p);
+	nd.ray = nullptr;
+	nd.info = nullptr;
+	nd.dec = nullptr;
+	nd.col = nullptr;
+	nd.wt = nullptr;
+	nd.child = nullptr;
+	nd.cap = 0;
+}
+
 void free_pt(ymx_pt *pt)
 {
 	if(!pt) return;
+	free_nodes(pt->nd);
 	void *ptrs[] = {pt->d_prim, pt->d_materials, pt->d_lights, pt->d_const, pt->d_faure, pt->d_inv_prims, pt->d_prims,
-	                pt->d_film, pt->d_film_w, pt->st.col, pt->st.path_col, pt->st.tp_next, pt->st.tp_vertex,
-	                pt->st.pwo, pt->st.cam_dir, pt->st.cam_hit, pt->st.dxy, pt->st.offset, pt->st.flags, pt->st.light,
-	                pt->st.vert_mat, pt->st.ext_rays[0], pt->st.ext_rays[1], pt->st.ext_path[0], pt->st.ext_path[1],
-	                pt->st.ext_count, pt->st.ext_hits, pt->st.slot_contrib, pt->st.slot_queue, pt->st.sh_rays,
-	                pt->st.sh_occ, pt->st.sh_count, pt->st.totals};
+	                pt->d_film, pt->d_film_w, pt->st.path_col, pt->st.tp_next, pt->st.tp_vertex, pt->st.pwo,
+	                pt->st.cam_hit, pt->st.dxy, pt->st.flags, pt->st.light, pt->st.vert_emit, pt->st.ext_rays[0],
+	                pt->st.ext_rays[1], pt->st.ext_path[0], pt->st.ext_path[1], pt->st.ext_count, pt->st.ext_hits,
+	                pt->st.slot_contrib, pt->st.slot_queue, pt->st.sh_rays, pt->st.sh_occ, pt->st.sh_count,
+	                pt->st.totals, pt->nd.spawn};
 	for(void *p : ptrs)
 		if(p) (void)hipFree(p);
 	if(pt->ev0) (void)hipEventDestroy(pt->ev0);
@@ -73,6 +88,29 @@ void free_pt(ymx_pt *pt)
 }
 
 template <typename T> hipError_t dalloc(T *&p, size_t count) { return hipMalloc((void **)&p, sizeof(T) * (count ? count : 1)); }
+
+// Node arrays for `cap` nodes, the first `keep` copied from `old`.
+hipError_t alloc_nodes(PtNodes &nd, uint32_t cap, const PtNodes *old, uint32_t keep)
+{
+	hipError_t e;
+	if((e = dalloc(nd.ray, cap)) != hipSuccess) return e;
+	if((e = dalloc(nd.info, cap)) != hipSuccess) return e;
+	if((e = dalloc(nd.dec, cap)) != hipSuccess) return e;
+	if((e = dalloc(nd.col, cap)) != hipSuccess) return e;
+	if((e = dalloc(nd.wt, cap)) != hipSuccess) return e;
+	if((e = dalloc(nd.child, cap)) != hipSuccess) return e;
+	nd.cap = cap;
+	if(old && keep)
+	{
+		if((e = hipMemcpy(nd.ray, old->ray, sizeof(ymx_ray) * keep, hipMemcpyDeviceToDevice)) != hipSuccess) return e;
+		if((e = hipMemcpy(nd.info, old->info, sizeof(uint4) * keep, hipMemcpyDeviceToDevice)) != hipSuccess) return e;
+		if((e = hipMemcpy(nd.dec, old->dec, sizeof(float2) * keep, hipMemcpyDeviceToDevice)) != hipSuccess) return e;
+		if((e = hipMemcpy(nd.col, old->col, sizeof(float4) * keep, hipMemcpyDeviceToDevice)) != hipSuccess) return e;
+		if((e = hipMemcpy(nd.wt, old->wt, sizeof(float4) * keep, hipMemcpyDeviceToDevice)) != hipSuccess) return e;
+		if((e = hipMemcpy(nd.child, old->child, sizeof(uint2) * keep, hipMemcpyDeviceToDevice)) != hipSuccess) return e;
+	}
+	return hipSuccess;
+}
 
 // Gauss filter table of ImageFilm (src/render/imagefilm.cc:231-264, include/math/filter.h:32-37).
 void filter_table(float pixelwidth, PtConst &c)
@@ -97,6 +135,39 @@ void filter_table(float pixelwidth, PtConst &c)
 
 }  // namespace
 
+int ymx_pt_grow_nodes(ymx_pt *pt, uint64_t need, uint32_t keep)
+{
+	if(need > (1ull << 31))
+	{
+		ymx_set_error("ymx_pt_render: the recursive raytrace needs %llu ray nodes (at most 2^31)",
+		              (unsigned long long)need);
+		return YMX_ERR_UNSUPPORTED;
+	}
+	uint64_t cap = (uint64_t)pt->nd.cap * 2;
+	if(cap < need) cap = need;
+	if(cap > (1ull << 31)) cap = 1ull << 31;
+	YMX_HIP_CHECK(hipDeviceSynchronize());  // the old arrays may still be in use
+	PtNodes nd = pt->nd;
+	nd.ray = nullptr;
+	nd.info = nullptr;
+	nd.dec = nullptr;
+	nd.col = nullptr;
+	nd.wt = nullptr;
+	nd.child = nullptr;
+	const hipError_t e = alloc_nodes(nd, (uint32_t)cap, &pt->nd, keep);
+	if(e != hipSuccess)
+	{
+		free_nodes(nd);
+		ymx_set_error("ymx_pt_render: growing the ray-node pool to %llu nodes: %s", (unsigned long long)cap,
+		              hipGetErrorString(e));
+		return e == hipErrorOutOfMemory ? YMX_ERR_OUT_OF_MEMORY : YMX_ERR_HIP;
+	}
+	free_nodes(pt->nd);
+	nd.spawn = pt->nd.spawn;
+	pt->nd = nd;
+	return YMX_OK;
+}
+
 extern "C" {
 
 int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, const uint32_t *tri_idx,
@@ -118,10 +189,11 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 		return YMX_ERR_INVALID_ARG;
 	}
 	if(st->resx <= 0 || st->resy <= 0 || (int64_t)st->resx * st->resy > (1 << 26) || st->bounces < 1 ||
-	   st->bounces > 12 || st->aa_samples < 1 || st->path_samples < 0 || n_lights > (uint32_t)kMaxLights)
+	   st->bounces > 12 || st->aa_samples < 1 || st->path_samples < 0 || n_lights > (uint32_t)kMaxLights ||
+	   st->raydepth < 0 || st->raydepth > kMaxRayDepth)
 	{
-		ymx_set_error("ymx_pt_create: unsupported settings (res %dx%d, bounces %d, AA %d, lights %u)", st->resx,
-		              st->resy, st->bounces, st->aa_samples, n_lights);
+		ymx_set_error("ymx_pt_create: unsupported settings (res %dx%d, bounces %d, AA %d, lights %u, raydepth %d)",
+		              st->resx, st->resy, st->bounces, st->aa_samples, n_lights, st->raydepth);
 		return YMX_ERR_UNSUPPORTED;
 	}
 	for(uint32_t i = 0; i < 3 * (size_t)n_tris; ++i)
@@ -137,7 +209,7 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 			return YMX_ERR_INVALID_ARG;
 		}
 	for(uint32_t m = 0; m < n_materials; ++m)
-		if(materials[m].type != YMX_MAT_SHINYDIFFUSE)
+		if(materials[m].type > YMX_MAT_LIGHT)
 		{
 			ymx_set_error("ymx_pt_create: material %u has unsupported type %u", m, materials[m].type);
 			return YMX_ERR_UNSUPPORTED;
@@ -192,20 +264,61 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 	PT_TRY(dalloc(pt->d_prim, prim.size()));
 	PT_TRY(hipMemcpy(pt->d_prim, prim.data(), sizeof(float4) * prim.size(), hipMemcpyHostToDevice));
 
-	// materials (src/material/material_shiny_diffuse.cc:131-183)
+	// materials: the values each reference material derives at construction
 	std::vector<PtMaterial> mats(n_materials ? n_materials : 1);
+	bool any_glossy_branches = false, any_specular = false;
 	for(uint32_t m = 0; m < n_materials; ++m)
 	{
 		const ymx_material &s = materials[m];
 		PtMaterial &d = mats[m];
+		memset(&d, 0, sizeof d);
+		d.type = s.type;
+		d.receive_shadows = s.receive_shadows;
 		d.color = rgb(s.color[0], s.color[1], s.color[2]);
 		d.diffuse_reflect = s.diffuse_reflect;
-		d.emit_color = cscale(d.color, s.emit);
-		d.flags = 0;
-		if(s.diffuse_reflect > 0.00001f) d.flags |= kDiffuse | kReflect;
-		if(s.emit > 0.f) d.flags |= kEmit;
-		d.receive_shadows = s.receive_shadows;
+		switch(s.type)
+		{
+			case YMX_MAT_GLOSSY:  // src/material/material_glossy.cc:130-150
+				if(s.diffuse_reflect > 0.f) d.flags = kDiffuse | kReflect;
+				d.flags |= s.as_diffuse ? (kDiffuse | kReflect) : (kGlossy | kReflect);
+				d.as_diffuse = s.as_diffuse ? 1u : 0u;
+				d.glossy_reflect = s.glossy_reflect;
+				d.diffuse_color = rgb(s.diffuse_color[0], s.diffuse_color[1], s.diffuse_color[2]);
+				d.exponent = s.exponent;
+				d.p_diffuse = std_min(0.6f, 1.f - (s.glossy_reflect / (s.glossy_reflect + (1.f - s.glossy_reflect) *
+				                                                                             s.diffuse_reflect)));
+				break;
+			case YMX_MAT_MIRROR:  // src/material/material_mirror.cc:64-68, include/material/material_mirror.h:64
+				d.flags = kSpecular;
+				d.ref_col = cscale(d.color, std_min(s.reflect, 1.f));
+				break;
+			case YMX_MAT_GLASS:  // src/material/material_glass.cc:157-166, include/material/material_glass.h:92
+			{
+				d.flags = kSpecular | kReflect | kTransmit;
+				if(s.fake_shadows) d.flags |= kFilter;
+				d.transmit_flags = s.fake_shadows ? (kFilter | kTransmit) : (kSpecular | kTransmit);
+				d.ior = s.ior;
+				const float tf = s.transmit_filter;
+				d.filter_color = rgb(tf * s.filter_color[0] + (1.f - tf), tf * s.filter_color[1] + (1.f - tf),
+				                     tf * s.filter_color[2] + (1.f - tf));
+				d.mirror_color = rgb(s.mirror_color[0], s.mirror_color[1], s.mirror_color[2]);
+				break;
+			}
+			case YMX_MAT_LIGHT:  // src/material/material_light.cc:69-73, include/material/material_light.h:69
+				d.flags = kEmit;
+				d.emit_color = cscale(d.color, s.power);
+				d.double_sided = s.double_sided ? 1u : 0u;
+				break;
+			default:  // shinydiffuse, src/material/material_shiny_diffuse.cc:131-183
+				d.emit_color = cscale(d.color, s.emit);
+				if(s.diffuse_reflect > 0.00001f) d.flags |= kDiffuse | kReflect;
+				if(s.emit > 0.f) d.flags |= kEmit;
+		}
+		if(d.flags & kGlossy) any_glossy_branches = true;
+		if(d.flags & (kSpecular | kFilter)) any_specular = true;
 	}
+	pt->recursive = (any_glossy_branches || any_specular) && st->raydepth >= 1;
+	pt->max_fan = any_glossy_branches ? kGlossyBranches : 2u;
 	PT_TRY(dalloc(pt->d_materials, mats.size()));
 	PT_TRY(hipMemcpy(pt->d_materials, mats.data(), sizeof(PtMaterial) * mats.size(), hipMemcpyHostToDevice));
 
@@ -274,6 +387,7 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 	c.bounces = st->bounces;
 	c.rr_min_bounces = st->rr_min_bounces;
 	c.aa_samples = st->aa_samples;
+	c.raydepth = st->raydepth;
 	c.n_lights = n_lights;
 	c.slots_camera = slots_camera;
 	c.slots_path = slots_path;
@@ -331,18 +445,19 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 	PT_TRY(dalloc(pt->d_film_w, npix));
 	PT_TRY(hipMemset(pt->d_film, 0, sizeof(float4) * npix));
 	PT_TRY(hipMemset(pt->d_film_w, 0, sizeof(float) * npix));
-	PT_TRY(dalloc(s.col, n));
 	PT_TRY(dalloc(s.path_col, n));
 	PT_TRY(dalloc(s.tp_next, n));
 	PT_TRY(dalloc(s.tp_vertex, n));
 	PT_TRY(dalloc(s.pwo, n));
-	PT_TRY(dalloc(s.cam_dir, n));
 	PT_TRY(dalloc(s.cam_hit, n));
 	PT_TRY(dalloc(s.dxy, n));
-	PT_TRY(dalloc(s.offset, n));
 	PT_TRY(dalloc(s.flags, n));
 	PT_TRY(dalloc(s.light, n));
-	PT_TRY(dalloc(s.vert_mat, n));
+	PT_TRY(dalloc(s.vert_emit, n));
+	// node pool: the camera samples, plus room for their children when materials recurse
+	PT_TRY(alloc_nodes(pt->nd, pt->recursive ? 2u * n : n, nullptr, 0));
+	PT_TRY(dalloc(pt->nd.spawn, 1));
+	PT_TRY(hipMemset(pt->nd.spawn, 0, sizeof(uint32_t)));
 	PT_TRY(dalloc(s.ext_rays[0], n));
 	PT_TRY(dalloc(s.ext_rays[1], n));
 	PT_TRY(dalloc(s.ext_path[0], n));

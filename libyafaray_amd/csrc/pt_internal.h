@@ -10,17 +10,39 @@ namespace ymx {
 enum BsdfBits : uint32_t
 {
 	kNone = 0, kSpecular = 1, kGlossy = 2, kDiffuse = 4, kDispersive = 8, kReflect = 16, kTransmit = 32,
-	kFilter = 64, kEmit = 128, kVolumetric = 256, kAll = 511
-};  // include/material/bsdf.h:25-67
+	kFilter = 64, kEmit = 128, kVolumetric = 256, kAll = 127
+};  // include/material/bsdf.h:25-67 (All = the seven scattering bits, without Emit and Volumetric)
 
+// Material record on the device: the params of ymx_material plus the values
+// the reference derives from them at construction.
 struct PtMaterial
 {
-	Rgb color;
-	float diffuse_reflect;
-	Rgb emit_color;       // emit * color
-	uint32_t flags;       // BsdfBits
+	uint32_t type;            // YMX_MAT_*
+	uint32_t flags;           // BsdfBits of the material (bsdf_flags_)
 	uint32_t receive_shadows;
+	uint32_t as_diffuse;      // glossy
+	Rgb color;                // shinydiffuse diffuse colour, glossy colour
+	float diffuse_reflect;
+	Rgb emit_color;           // shinydiffuse emit * color; light_mat color * power
+	float glossy_reflect;
+	Rgb diffuse_color;        // glossy
+	float exponent;           // glossy
+	Rgb ref_col;              // mirror: color * min(reflect, 1)
+	float ior;                // glass
+	Rgb filter_color;         // glass: transmit_filter * filter_color + (1 - transmit_filter)
+	float p_diffuse;          // glossy: min(0.6, 1 - g / (g + (1 - g) d))
+	Rgb mirror_color;         // glass
+	uint32_t transmit_flags;  // glass: Specular|Transmit, or Filter|Transmit with fake shadows
+	uint32_t double_sided;    // light_mat
+	uint32_t pad[3];
 };
+
+// True when surfaces of this material spawn recursive rays (recursiveRaytrace):
+// glossy without as_diffuse, mirror, glass.
+YMX_HD bool spawns_children(uint32_t flags) { return (flags & (kGlossy | kSpecular | kFilter)) != 0; }
+
+constexpr int kMaxRayDepth = 10;
+constexpr uint32_t kGlossyBranches = 8;  // MonteCarloIntegrator::initial_ray_samples_glossy_
 
 struct PtLight
 {
@@ -44,6 +66,7 @@ struct PtConst
 	V3 cam_from, vright, vup, vto;
 	int32_t resx, resy;
 	int32_t path_samples, bounces, rr_min_bounces, aa_samples;
+	int32_t raydepth;
 	uint32_t n_lights;
 	uint32_t slots_camera;   // shadow slots of a camera vertex (all lights)
 	uint32_t slots_path;     // shadow slots of a path vertex (one light)
@@ -55,22 +78,35 @@ struct PtConst
 	float filter_table[256];
 };
 
-// Per-path state of one wavefront (one AA sample index for every pixel).
+// The integrate() calls of one wavefront ("nodes"): first the camera samples
+// (node i = path i), then their recursive children level by level.  A node's
+// children are the contiguous range child.x .. child.x + count.
+struct PtNodes
+{
+	ymx_ray *ray;       // origin, tmin, direction, tmax (-1)
+	uint4 *info;        // x: path (j * npix + pixel), y: level | division << 8, z: tag, w: ray-division offset
+	float2 *dec;        // ray-division decorrelation 1, 2
+	float4 *col;        // colour (local part, then with the children's added), w: alpha
+	float4 *wt;         // weight of this node's colour in its parent's sum
+	uint2 *child;       // x: first child, y: count | kind << 16 (kind 1 glossy branches, 2 specular)
+	uint32_t *spawn;    // children allocated so far in the level being shaded
+	uint32_t cap;       // nodes allocated
+};
+
+enum ChildKind : uint32_t { kChildGlossy = 1u, kChildSpecular = 2u };
+
+// Per-slot state of one chunk of nodes (slot i = node base + i).
 struct PtState
 {
-	float4 *col;        // rgb, alpha
 	float4 *path_col;   // rgb
 	float4 *tp_next;    // throughput including the next segment's BSDF weight
 	float4 *tp_vertex;  // throughput at the current vertex (after roulette)
 	float4 *pwo;        // outgoing direction at the current vertex
-	float4 *cam_dir;    // camera ray direction
-	ymx_hit *cam_hit;   // camera hit
-	float2 *dxy;        // sub-pixel offsets
-	uint32_t *offset;   // fnv32a(y * fnv32a(x))
-	uint32_t *flags;    // bit 0 active, bit 1 add emission, bit 2 caustic, bit 3 direct light queued,
-	                    // bits 8..16 first path vertex BSDF flags, bit 20 sampled != None
+	ymx_hit *cam_hit;   // hit of the node's ray
+	float2 *dxy;        // sub-pixel offsets of the camera samples (by path)
+	uint32_t *flags;    // StateBits of pt_kernels.hip; bits 8..16 first path vertex BSDF flags
 	int32_t *light;     // light picked at the current path vertex
-	uint32_t *vert_mat; // material of the current path vertex
+	float4 *vert_emit;  // emission of the current path vertex, when it is added
 	// extension queues (double-buffered: segment d reads queue d % 2)
 	ymx_ray *ext_rays[2];
 	uint32_t *ext_path[2]; // path index of each queued extension ray
@@ -82,9 +118,9 @@ struct PtState
 	ymx_ray *sh_rays;
 	uint32_t *sh_occ;      // bit per queued shadow ray
 	uint32_t *sh_count;
-	uint32_t capacity;     // paths per wavefront
+	uint32_t capacity;     // slots (nodes per chunk); also the camera paths per wavefront
 	uint32_t slots;        // shadow slots per path
-	unsigned long long *totals;  // rays cast: [0] camera, [1] extension, [2] shadow
+	unsigned long long *totals;  // rays cast: [0] camera + recursive, [1] extension, [2] shadow
 };
 
 }  // namespace ymx
@@ -104,6 +140,9 @@ struct ymx_pt
 	float4 *d_film;                  // resx * resy (colour sums, alpha sum)
 	float *d_film_w;                 // resx * resy
 	ymx::PtState st;
+	ymx::PtNodes nd;
+	bool recursive;                  // some material spawns recursive rays and raydepth >= 1
+	uint32_t max_fan;                // most children one node can spawn (8 glossy branches, 2 specular)
 	uint32_t batch;                  // AA samples per wavefront
 	ymx::PtConst host_const;
 	ymx_pt_stats stats;
@@ -113,3 +152,7 @@ struct ymx_pt
 // Kernel launches of one wavefront: AA samples [first_sample, first_sample + count)
 // of every pixel (pt_kernels.hip); count * resx * resy <= st.capacity.
 int ymx_pt_trace_pass(ymx_pt *pt, int32_t first_sample, int32_t count, hipStream_t stream);
+
+// Grows the node pool to hold at least `need` nodes, keeping the first `keep`
+// (pt_api.hip).  Synchronous.
+int ymx_pt_grow_nodes(ymx_pt *pt, uint64_t need, uint32_t keep);

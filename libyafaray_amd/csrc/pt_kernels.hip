@@ -1,24 +1,32 @@
 // pt_kernels.hip -- wavefront kernels of the path tracer.
 //
-// One AA sample index of every pixel is one wavefront.  Stage order per pass
-// (PathIntegrator::integrate, src/integrator/surface/integrator_path_tracer.cc:138-314):
-//   camera        camera rays of every pixel -> extension queue 0
-//   extend        closest hit (accel_trace.hip)
-//   shade_camera  emission + next-event estimation for all lights: shadow
-//                 rays queued, their contributions kept per slot
-//   shadow        any hit over the shadow queue, then resolve_camera sums the
-//                 unoccluded slot contributions in the reference's order
-//   per sub-path: subpath_init samples the first bounce; then for each
-//   segment d: extend, shade_path (roulette, one-light estimation, next BSDF
+// A wavefront holds `count` AA samples of every pixel (path i traces pixel
+// i % npix, sample first + i / npix).  Every call of the reference's
+// PathIntegrator::integrate (src/integrator/surface/integrator_path_tracer.cc:138-314)
+// is a *node*: the camera samples are the level-0 nodes; the recursive raytrace
+// of a mirror, glass or glossy surface (integrator_montecarlo.cc:506-748) adds
+// child nodes one level down.  Levels run in order, each in chunks of at most
+// `capacity` nodes; the stages of one chunk are
+//   extend        closest hit of the node rays (accel_trace.hip)
+//   shade_node    emission + next-event estimation for all lights (shadow rays
+//                 queued, contributions kept per slot); children written to the
+//                 next level
+//   shadow        any hit over the shadow queue; resolve_node sums the unoccluded
+//                 slots in the reference's order
+//   per sub-path: subpath_init samples the first bounce; then per segment d:
+//   extend, shade_path (roulette, one-light estimation, emission, next BSDF
 //   sample), shadow, resolve_path
 //   finish        col += path_col / path_samples
-//   film          gather-splat into the film with the Gauss filter table in
-//                 the order a row-major scatter would add (ImageFilm::addSample)
-// The shading arithmetic follows the reference's evaluation order; see
-// pt_math.h.  Compiled with -ffp-contract=off and IEEE division.
+// After the last level, combine adds each level's children into their parents
+// (deepest first) in the reference's order, and film gather-splats the camera
+// samples with the Gauss filter table in the order a row-major scatter would
+// add them (ImageFilm::addSample).  Compiled with -ffp-contract=off and IEEE
+// division: the shading keeps the reference's operation order (pt_math.h).
 #include "pt_internal.h"
 
 #include <float.h>
+
+#include <vector>
 
 namespace ymx {
 namespace {
@@ -40,20 +48,50 @@ struct PtArgs
 	const double *inv_prims;
 	const uint32_t *prims;
 	PtState st;
+	PtNodes nd;
 	float4 *film;
 	float *film_w;
 	int32_t first_sample;  // AA sample index of the wavefront's first slice
 	uint32_t npix;         // resx * resy
-	uint32_t n_paths;      // npix * count: path i traces pixel i % npix, sample first + i / npix
+	uint32_t n_paths;      // npix * count (level-0 nodes)
 	int32_t count;         // AA samples in this wavefront
+	uint32_t base;         // first node of the chunk: slot i is node base + i
+	uint32_t cnt;          // nodes in the chunk
+	uint32_t next_start;   // first node of the next level
+	uint32_t recursive;    // children may be spawned from this chunk
 	int32_t sub;
 	int32_t depth;
 };
 
-__device__ __forceinline__ uint32_t pix_of(const PtArgs &a, uint32_t i) { return i % a.npix; }
-__device__ __forceinline__ uint32_t sample_of(const PtArgs &a, uint32_t i)
+// What a kernel needs to know of the integrate() call behind slot i.
+struct NodeCtx
 {
-	return (uint32_t)a.first_sample + i / a.npix;
+	uint32_t n;          // node index
+	uint32_t path;       // camera sample (node index of the level-0 ancestor)
+	uint32_t pixel;      // camera_res_x * y + x
+	uint32_t sample;     // AA sample index
+	uint32_t offset;     // fnv32a(y * fnv32a(x))
+	uint32_t level;      // ray_level of the integrate() call
+	uint32_t division;   // RayDivision::division_
+	int32_t rd_offset;   // RayDivision::offset_
+	uint32_t tag;        // deterministic-stream tag of the call
+};
+
+__device__ __forceinline__ NodeCtx node_ctx(const PtArgs &a, uint32_t i)
+{
+	NodeCtx x;
+	x.n = a.base + i;
+	const uint4 info = a.nd.info[x.n];
+	x.path = info.x;
+	x.pixel = info.x % a.npix;
+	x.sample = (uint32_t)a.first_sample + info.x / a.npix;
+	const uint32_t px = x.pixel % (uint32_t)a.c->resx, py = x.pixel / (uint32_t)a.c->resx;
+	x.offset = fnv32a(py * fnv32a(px));
+	x.level = info.y & 0xFFu;
+	x.division = info.y >> 8;
+	x.tag = info.z;
+	x.rd_offset = (int32_t)info.w;
+	return x;
 }
 
 struct Surf
@@ -117,35 +155,12 @@ __device__ void make_surf(const PtArgs &a, V3 from, V3 dir, const ymx_hit &h, Su
 
 __device__ __forceinline__ V3 face_forward(V3 ng, V3 n, V3 wo) { return dot(ng, wo) < 0.f ? neg(n) : n; }
 
-// ShinyDiffuseMaterial::eval (src/material/material_shiny_diffuse.cc:297-340)
-__device__ Rgb bsdf_eval(const PtArgs &a, const Surf &sp, V3 wo, V3 wl, uint32_t bsdfs)
+struct BSample
 {
-	const V3 n = face_forward(sp.ng, sp.n, wo);
-	if(!(bsdfs & (sp.flags & kDiffuse))) return rgb(0.f, 0.f, 0.f);
-	if(dot(n, wl) < 0.0) return rgb(0.f, 0.f, 0.f);
-	const PtMaterial &m = a.mats[sp.mat];
-	const float m_d = 1.f * (1.f - 0.f) * m.diffuse_reflect;
-	return cscale(m.color, m_d);
-}
-
-// ShinyDiffuseMaterial::pdf (:441-483)
-__device__ float bsdf_pdf(const PtArgs &a, const Surf &sp, V3 wo, V3 wi, uint32_t bsdfs)
-{
-	if(!(bsdfs & kDiffuse)) return 0.f;
-	if(!(sp.flags & kDiffuse)) return 0.f;
-	const V3 n = face_forward(sp.ng, sp.n, wo);
-	float pdf = 0.f, sum = 0.f;
-	int n_match = 0;
-	if(bsdfs & (kDiffuse | kReflect))
-	{
-		const float width = a.mats[sp.mat].diffuse_reflect;
-		sum += width;
-		pdf += fabsf(dot(wi, n)) * width;
-		++n_match;
-	}
-	if(!n_match || sum < 0.00001) return 0.f;
-	return pdf / sum;
-}
+	float s1, s2;
+	uint32_t flags, sampled;
+	float pdf;
+};
 
 // sample::cosHemisphere, include/sampler/sample.h:43-52
 __device__ V3 cos_hemisphere(V3 n, V3 ru, V3 rv, float s1, float s2)
@@ -156,19 +171,42 @@ __device__ V3 cos_hemisphere(V3 n, V3 ru, V3 rv, float s1, float s2)
 	return add(mul(sqrtf(1.f - z1), add(mul(fast_cos(z2), ru), mul(fast_sin(z2), rv))), mul(sqrtf(z1), n));
 }
 
-struct BSample
-{
-	float s1, s2;
-	uint32_t flags, sampled;
-	float pdf;
-};
+// ---- ShinyDiffuseMaterial, diffuse component (src/material/material_shiny_diffuse.cc) ----
 
-// ShinyDiffuseMaterial::sample, diffuse component (:349-439)
-__device__ Rgb bsdf_sample(const PtArgs &a, const Surf &sp, V3 wo, V3 &wi, BSample &s, float &w)
+// eval (:297-340)
+__device__ Rgb sd_eval(const PtMaterial &m, const Surf &sp, V3 wo, V3 wl, uint32_t bsdfs)
+{
+	const V3 n = face_forward(sp.ng, sp.n, wo);
+	if(!(bsdfs & (sp.flags & kDiffuse))) return rgb(0.f, 0.f, 0.f);
+	if(dot(n, wl) < 0.0) return rgb(0.f, 0.f, 0.f);
+	const float m_d = 1.f * (1.f - 0.f) * m.diffuse_reflect;
+	return cscale(m.color, m_d);
+}
+
+// pdf (:441-483)
+__device__ float sd_pdf(const PtMaterial &m, const Surf &sp, V3 wo, V3 wi, uint32_t bsdfs)
+{
+	if(!(bsdfs & kDiffuse)) return 0.f;
+	if(!(sp.flags & kDiffuse)) return 0.f;
+	const V3 n = face_forward(sp.ng, sp.n, wo);
+	float pdf = 0.f, sum = 0.f;
+	int n_match = 0;
+	if(bsdfs & (kDiffuse | kReflect))
+	{
+		const float width = m.diffuse_reflect;
+		sum += width;
+		pdf += fabsf(dot(wi, n)) * width;
+		++n_match;
+	}
+	if(!n_match || sum < 0.00001) return 0.f;
+	return pdf / sum;
+}
+
+// sample (:349-439)
+__device__ Rgb sd_sample(const PtMaterial &m, const Surf &sp, V3 wo, V3 &wi, BSample &s, float &w)
 {
 	const float cos_ng_wo = dot(sp.ng, wo);
 	const V3 n = face_forward(sp.ng, sp.n, wo);
-	const PtMaterial &m = a.mats[sp.mat];
 	const uint32_t cflag = kDiffuse | kReflect;
 	float sum = 0.f, width = 0.f;
 	int n_match = 0;
@@ -195,6 +233,309 @@ __device__ Rgb bsdf_sample(const PtArgs &a, const Surf &sp, V3 wo, V3 &wi, BSamp
 	w = fabsf(dot(wi, sp.n)) / (s.pdf * 0.99f + 0.01f);
 	w = w * 1.f + 1.f * (1.f - 1.f);  // getAlpha() == 1
 	return scolor;
+}
+
+// ---- microfacet helpers (include/material/material_utils_microfacet.h) ----
+
+__device__ __forceinline__ float pdf_divisor(float cos) { return 8.f * kPi * (cos * 0.99f + 0.04f); }
+__device__ __forceinline__ float as_divisor(float cos_1, float cos_i, float cos_o)
+{
+	return 8.f * kPi * ((cos_1 * std_max(cos_i, cos_o)) * 0.99f + 0.04f);
+}
+__device__ __forceinline__ float blinn_d(float cos_h, float e) { return (e + 1.f) * fast_pow(cos_h, e); }
+__device__ __forceinline__ float blinn_pdf(float costheta, float cos_w_h, float e)
+{
+	return blinn_d(costheta, e) / pdf_divisor(cos_w_h);
+}
+__device__ V3 blinn_sample(float s_1, float s_2, float exponent)
+{
+	const float cos_theta = fast_pow(1.f - s_2, 1.f / (exponent + 1.f));
+	const float sin_theta = sqrtf(1.f - cos_theta * cos_theta);
+	const float phi = s_1 * k2Pi;
+	return v3(sin_theta * fast_cos(phi), sin_theta * fast_sin(phi), cos_theta);
+}
+__device__ __forceinline__ float schlick_fresnel(float costheta, float r)
+{
+	const float c_1 = (1.f - costheta);
+	const float c_2 = c_1 * c_1;
+	return r + ((1.f - r) * c_1 * c_2 * c_2);
+}
+// diffuseReflect: the scale in double (diffuse_ratio is a double constant), rounded to float
+__device__ Rgb diffuse_reflect(float wi_n, float wo_n, float glossy, float diffuse, Rgb diff_base)
+{
+	const double diffuse_ratio = 0.387507688;
+	float f_wi = 1.f - (0.5f * wi_n);
+	float temp = f_wi * f_wi;
+	f_wi = temp * temp * f_wi;
+	float f_wo = (1.f - (0.5f * wo_n));
+	temp = f_wo * f_wo;
+	f_wo = temp * temp * f_wo;
+	const float k = (float)(diffuse_ratio * diffuse * (1.f - glossy) * (1.f - f_wi) * (1.f - f_wo));
+	return cscale(diff_base, k);
+}
+
+// ---- GlossyMaterial, isotropic (src/material/material_glossy.cc) ----
+
+// eval (:200-250)
+__device__ Rgb glossy_eval(const PtMaterial &m, const Surf &sp, V3 wo, V3 wl, uint32_t bsdfs)
+{
+	if(!(bsdfs & kDiffuse) || ((dot(sp.ng, wl) * dot(sp.ng, wo)) < 0.f)) return rgb(0.f, 0.f, 0.f);
+	Rgb col = rgb(0.f, 0.f, 0.f);
+	const bool diffuse_flag = (bsdfs & kDiffuse) != 0;
+	const V3 n = face_forward(sp.ng, sp.n, wo);
+	const float wi_n = fabsf(dot(wl, n));
+	const float wo_n = fabsf(dot(wo, n));
+	if((m.as_diffuse && diffuse_flag) || (!m.as_diffuse && (bsdfs & kGlossy)))
+	{
+		const V3 h = normalize(add(wo, wl));
+		const float cos_wi_h = std_max(0.f, dot(wl, h));
+		const float glossy =
+			blinn_d(dot(h, n), m.exponent) * schlick_fresnel(cos_wi_h, m.glossy_reflect) / as_divisor(cos_wi_h, wo_n, wi_n);
+		col = cscale(m.color, glossy);
+	}
+	if(m.diffuse_reflect > 0.f && diffuse_flag)
+		col = cadd(col, cscale(m.diffuse_color, m.diffuse_reflect * (1.f - m.glossy_reflect)));
+	return col;
+}
+
+// pdf (:404-440)
+__device__ float glossy_pdf(const PtMaterial &m, const Surf &sp, V3 wo, V3 wi, uint32_t flags)
+{
+	if((dot(sp.ng, wo) * dot(sp.ng, wi)) < 0.f) return 0.f;
+	const V3 n = face_forward(sp.ng, sp.n, wo);
+	float pdf = 0.f;
+	const float cur_p_diffuse = m.p_diffuse;
+	const bool use_glossy = m.as_diffuse ? (flags & kDiffuse) != 0 : (flags & kGlossy) != 0;
+	const bool use_diffuse = m.diffuse_reflect > 0.f && (flags & kDiffuse);
+	if(use_diffuse)
+	{
+		pdf = fabsf(dot(wi, n));
+		if(use_glossy)
+		{
+			const V3 h = normalize(add(wi, wo));
+			const float cos_wo_h = dot(wo, h);
+			const float cos_n_h = dot(n, h);
+			pdf = pdf * cur_p_diffuse + blinn_pdf(cos_n_h, cos_wo_h, m.exponent) * (1.f - cur_p_diffuse);
+		}
+		return pdf;
+	}
+	if(use_glossy)
+	{
+		const V3 h = normalize(add(wi, wo));
+		const float cos_wo_h = dot(wo, h);
+		const float cos_n_h = dot(n, h);
+		pdf = blinn_pdf(cos_n_h, cos_wo_h, m.exponent);
+	}
+	return pdf;
+}
+
+// sample (:252-402); early returns leave w unchanged
+__device__ Rgb glossy_sample(const PtMaterial &m, const Surf &sp, V3 wo, V3 &wi, BSample &s, float &w)
+{
+	const float cos_ng_wo = dot(sp.ng, wo);
+	const V3 n = face_forward(sp.ng, sp.n, wo);
+	s.pdf = 0.f;
+	float wi_n = 0.f;
+	const float wo_n = fabsf(dot(wo, n));
+	Rgb scolor = rgb(0.f, 0.f, 0.f);
+	float s_1 = s.s1;
+	const float cur_p_diffuse = m.p_diffuse;
+	const bool use_glossy = m.as_diffuse ? (s.flags & kDiffuse) != 0 : (s.flags & kGlossy) != 0;
+	const bool use_diffuse = m.diffuse_reflect > 0.f && (s.flags & kDiffuse);
+	if(use_diffuse)
+	{
+		const float s_p_diffuse = use_glossy ? cur_p_diffuse : 1.f;
+		if(s_1 < s_p_diffuse)
+		{
+			s_1 /= s_p_diffuse;
+			wi = cos_hemisphere(n, sp.u, sp.v, s_1, s.s2);
+			const float cos_ng_wi = dot(sp.ng, wi);
+			if(cos_ng_wi * cos_ng_wo < 0.f) return scolor;
+			wi_n = fabsf(dot(wi, n));
+			s.pdf = wi_n;
+			float glossy = 0.f;
+			if(use_glossy)
+			{
+				const V3 h = normalize(add(wi, wo));
+				const float cos_wo_h = dot(wo, h);
+				const float cos_wi_h = fabsf(dot(wi, h));
+				const float cos_n_h = dot(n, h);
+				s.pdf = s.pdf * cur_p_diffuse + blinn_pdf(cos_n_h, cos_wo_h, m.exponent) * (1.f - cur_p_diffuse);
+				glossy = blinn_d(cos_n_h, m.exponent) * schlick_fresnel(cos_wi_h, m.glossy_reflect) /
+				         as_divisor(cos_wi_h, wo_n, wi_n);
+			}
+			s.sampled = kDiffuse | kReflect;
+			if(!(s.flags & kReflect)) return rgb(0.f, 0.f, 0.f);
+			scolor = cscale(m.color, glossy);
+			scolor = cadd(scolor, diffuse_reflect(wi_n, wo_n, m.glossy_reflect, m.diffuse_reflect, m.diffuse_color));
+			w = wi_n / (s.pdf * 0.99f + 0.01f);
+			return scolor;
+		}
+		s_1 -= cur_p_diffuse;
+		s_1 /= (1.f - cur_p_diffuse);
+	}
+	if(use_glossy)
+	{
+		const V3 hs = blinn_sample(s_1, s.s2, m.exponent);
+		V3 h = add(add(mul(hs.x, sp.u), mul(hs.y, sp.v)), mul(hs.z, n));
+		float cos_wo_h = dot(wo, h);
+		if(cos_wo_h < 0.f)
+		{
+			h = reflect_about(h, n);
+			cos_wo_h = dot(wo, h);
+		}
+		wi = reflect_dir(h, wo);
+		const float cos_ng_wi = dot(sp.ng, wi);
+		if(cos_ng_wo * cos_ng_wi < 0.f) return rgb(0.f, 0.f, 0.f);
+		wi_n = fabsf(dot(wi, n));
+		const float cos_hn = dot(h, n);
+		s.pdf = blinn_pdf(cos_hn, cos_wo_h, m.exponent);
+		const float glossy = blinn_d(cos_hn, m.exponent) * schlick_fresnel(cos_wo_h, m.glossy_reflect) /
+		                     as_divisor(cos_wo_h, wo_n, wi_n);
+		scolor = cscale(m.color, glossy);
+		s.sampled = m.as_diffuse ? (kDiffuse | kReflect) : (kGlossy | kReflect);
+	}
+	if(use_diffuse)
+	{
+		const Rgb add_col = diffuse_reflect(wi_n, wo_n, m.glossy_reflect, m.diffuse_reflect, m.diffuse_color);
+		s.pdf = wi_n * cur_p_diffuse + s.pdf * (1.f - cur_p_diffuse);
+		scolor = cadd(scolor, add_col);
+	}
+	w = wi_n / (s.pdf * 0.99f + 0.01f);
+	return scolor;
+}
+
+// ---- GlassMaterial without dispersion (src/material/material_glass.cc:168-378) ----
+
+// Vec3f::refract, include/geometry/vector.h:484-504
+__device__ bool refract(V3 n, V3 wi, V3 &wo, float ior)
+{
+	float eta;
+	float cos_v_n = dot(wi, n);
+	if(cos_v_n < 0.f)
+	{
+		eta = ior;
+		n = neg(n);
+		cos_v_n = -cos_v_n;
+	}
+	else eta = 1.f / ior;
+	const float k = 1.f - eta * eta * (1.f - cos_v_n * cos_v_n);
+	if(k <= 0.f) return false;
+	wo = normalize(add(mul(eta, neg(wi)), mul(eta * cos_v_n - sqrtf(k), n)));
+	return true;
+}
+
+// Vec3f::fresnel, include/geometry/vector.h:506-518
+__device__ void fresnel(V3 i, V3 n, float ior, float &kr, float &kt)
+{
+	const bool negative = dot(i, n) < 0.f;
+	const float c = dot(i, negative ? neg(n) : n);
+	float g = ior * ior + c * c - 1.f;
+	if(g <= 0.f) g = 0.f;
+	else g = sqrtf(g);
+	const float aux = c * (g + c);
+	kr = ((0.5f * (g - c) * (g - c)) / ((g + c) * (g + c))) *
+	     (1.f + ((aux - 1.f) * (aux - 1.f)) / ((aux + 1.f) * (aux + 1.f)));
+	kt = (kr < 1.f) ? 1.f - kr : 0.f;
+}
+
+// the normal bent towards wo (material_glass.cc:172-174, :307-317)
+__device__ V3 glass_normal(const Surf &sp, V3 wo, bool &outside)
+{
+	outside = dot(sp.ng, wo) > 0.f;
+	const float cos_wo_n = dot(sp.n, wo);
+	const bool keep = outside ? cos_wo_n >= 0.f : cos_wo_n <= 0.f;
+	return keep ? sp.n : normalize(sub(sp.n, mul(1.00001f * cos_wo_n, wo)));
+}
+
+__device__ Rgb glass_sample(const PtMaterial &m, const Surf &sp, V3 wo, V3 &wi, BSample &s, float &w)
+{
+	if(!(s.flags & kSpecular))
+	{
+		s.pdf = 0.f;
+		return rgb(0.f, 0.f, 0.f);
+	}
+	bool outside;
+	const V3 n = glass_normal(sp, wo, outside);
+	s.pdf = 1.f;
+	V3 refdir;
+	if(refract(n, wo, refdir, m.ior))
+	{
+		float kr, kt;
+		fresnel(wo, n, m.ior, kr, kt);
+		const float p_kr = (float)(0.01 + 0.99 * (double)kr), p_kt = (float)(0.01 + 0.99 * (double)kt);
+		if(s.s1 < p_kt && (s.flags & m.transmit_flags))
+		{
+			wi = refdir;
+			s.pdf = p_kt;
+			s.sampled = m.transmit_flags;
+			w = 1.f;
+			return m.filter_color;
+		}
+		else if(s.flags & (kSpecular | kReflect))
+		{
+			wi = reflect_about(wo, n);
+			s.pdf = p_kr;
+			s.sampled = kSpecular | kReflect;
+			w = 1.f;
+			return m.mirror_color;
+		}
+	}
+	else if(s.flags & (kSpecular | kReflect))
+	{
+		wi = reflect_about(wo, n);
+		s.sampled = kSpecular | kReflect;
+		w = 1.f;
+		return rgb(1.f, 1.f, 1.f);
+	}
+	s.pdf = 0.f;
+	return rgb(0.f, 0.f, 0.f);
+}
+
+// ---- dispatch over the material types ----
+
+__device__ Rgb bsdf_eval(const PtArgs &a, const Surf &sp, V3 wo, V3 wl, uint32_t bsdfs)
+{
+	const PtMaterial &m = a.mats[sp.mat];
+	if(m.type == YMX_MAT_SHINYDIFFUSE) return sd_eval(m, sp, wo, wl, bsdfs);
+	if(m.type == YMX_MAT_GLOSSY) return glossy_eval(m, sp, wo, wl, bsdfs);
+	return rgb(0.f, 0.f, 0.f);  // mirror, glass, light_mat
+}
+
+__device__ float bsdf_pdf(const PtArgs &a, const Surf &sp, V3 wo, V3 wi, uint32_t bsdfs)
+{
+	const PtMaterial &m = a.mats[sp.mat];
+	if(m.type == YMX_MAT_SHINYDIFFUSE) return sd_pdf(m, sp, wo, wi, bsdfs);
+	if(m.type == YMX_MAT_GLOSSY) return glossy_pdf(m, sp, wo, wi, bsdfs);
+	return 0.f;  // Material::pdf default
+}
+
+__device__ Rgb bsdf_sample(const PtArgs &a, const Surf &sp, V3 wo, V3 &wi, BSample &s, float &w)
+{
+	const PtMaterial &m = a.mats[sp.mat];
+	switch(m.type)
+	{
+		case YMX_MAT_SHINYDIFFUSE: return sd_sample(m, sp, wo, wi, s, w);
+		case YMX_MAT_GLOSSY: return glossy_sample(m, sp, wo, wi, s, w);
+		case YMX_MAT_MIRROR:  // MirrorMaterial::sample, src/material/material_mirror.cc:70-76
+			wi = reflect_dir(sp.n, wo);
+			s.sampled = kSpecular | kReflect;
+			w = 1.f;
+			return cscale(m.ref_col, 1.f / fabsf(dot(sp.n, wi)));
+		case YMX_MAT_GLASS: return glass_sample(m, sp, wo, wi, s, w);
+		default:  // LightMaterial::sample, src/material/material_light.cc:75-80
+			s.pdf = 0.f;
+			w = 0.f;
+			return rgb(0.f, 0.f, 0.f);
+	}
+}
+
+// Material::emit: shinydiffuse emit * color; LightMaterial::emit (src/material/material_light.cc:82-87)
+__device__ Rgb mat_emit(const PtArgs &a, const Surf &sp, V3 wo)
+{
+	const PtMaterial &m = a.mats[sp.mat];
+	if(m.type == YMX_MAT_LIGHT && !m.double_sided && !(dot(wo, sp.n) > 0.f)) return rgb(0.f, 0.f, 0.f);
+	return m.emit_color;
 }
 
 // ShapePolygon<float, 4>::intersect, include/geometry/shape/shape_polygon.h:126-176
@@ -244,9 +585,18 @@ __device__ float quad_intersect(const V3 q[4], V3 from, V3 dir)
 	return 0.f;
 }
 
-__device__ __forceinline__ uint32_t light_slots(const PtLight &l)
+// doLightEstimation's sample count: ceil(samples * 1), divided by the ray
+// division (integrator_montecarlo.cc:427-429)
+__device__ __forceinline__ int light_samples(const PtLight &l, uint32_t division)
 {
-	return l.type == YMX_LIGHT_AREA ? 2u * (uint32_t)l.samples : 1u;
+	int num_samples = (int)ceilf((float)l.samples * 1.f);
+	if(division > 1) num_samples = num_samples / (int)division > 1 ? num_samples / (int)division : 1;
+	return num_samples;
+}
+
+__device__ __forceinline__ uint32_t light_slots(const PtLight &l, uint32_t division)
+{
+	return l.type == YMX_LIGHT_AREA ? 2u * (uint32_t)light_samples(l, division) : 1u;
 }
 
 // Accelerator::isShadowed (include/accelerator/accelerator.h:103-111): origin
@@ -264,17 +614,31 @@ __device__ int32_t queue_shadow(const PtArgs &a, V3 from, V3 dir, float tmin, fl
 	return (int32_t)q;
 }
 
-__device__ __forceinline__ void set_slot(const PtArgs &a, uint32_t slot, Rgb c, bool valid, int32_t q)
+// A slot holds one light sample's contribution when unoccluded.  A zero
+// contribution is stored as no contribution (adding +0 changes no sum) and
+// needs no shadow ray.
+__device__ __forceinline__ void set_slot_empty(const PtArgs &a, uint32_t slot)
 {
-	a.st.slot_contrib[slot] = make_float4(c.r, c.g, c.b, valid ? 1.f : 0.f);
-	a.st.slot_queue[slot] = q;
+	a.st.slot_contrib[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+	a.st.slot_queue[slot] = -1;
+}
+__device__ __forceinline__ void set_slot(const PtArgs &a, uint32_t slot, Rgb c, bool cast, V3 from, V3 dir, float tmin,
+                                         float tmax)
+{
+	if(cblack(c))
+	{
+		set_slot_empty(a, slot);
+		return;
+	}
+	a.st.slot_contrib[slot] = make_float4(c.r, c.g, c.b, 1.f);
+	a.st.slot_queue[slot] = cast ? queue_shadow(a, from, dir, tmin, tmax) : -1;
 }
 
 // MonteCarloIntegrator::doLightEstimation with its area / dirac paths
 // (src/integrator/surface/integrator_montecarlo.cc:118-440), queuing one shadow
 // ray per sample instead of tracing it.
 __device__ void queue_light(const PtArgs &a, const Surf &sp, V3 wo, const PtLight &l, uint32_t loffs,
-                            uint32_t psd_sample, uint32_t psd_offset, uint32_t slot0)
+                            uint32_t division, uint32_t psd_sample, uint32_t psd_offset, uint32_t slot0)
 {
 	const PtMaterial &m = a.mats[sp.mat];
 	const bool cast_shadows = l.cast_shadows && m.receive_shadows;
@@ -289,7 +653,7 @@ __device__ void queue_light(const PtArgs &a, const Surf &sp, V3 wo, const PtLigh
 		const float dist = sqrtf(dist_sqr);
 		if(dist == 0.f)
 		{
-			set_slot(a, slot0, rgb(0.f, 0.f, 0.f), false, -1);
+			set_slot_empty(a, slot0);
 			return;
 		}
 		const float idist_sqr = 1.f / dist_sqr;
@@ -298,12 +662,11 @@ __device__ void queue_light(const PtArgs &a, const Surf &sp, V3 wo, const PtLigh
 		const float angle = fabsf(dot(sp.n, ldir));
 		const Rgb surf_col = bsdf_eval(a, sp, wo, ldir, kAll);
 		const Rgb c = cmul(cscale(cmul(surf_col, lcol), angle), rgb(1.f, 1.f, 1.f));
-		const int32_t q = cast_shadows ? queue_shadow(a, sp.p, ldir, shadow_tmin, dist) : -1;
-		set_slot(a, slot0, c, true, q);
+		set_slot(a, slot0, c, cast_shadows, sp.p, ldir, shadow_tmin, dist);
 		return;
 	}
 	const uint32_t l_offs = loffs * 4567u;
-	const int num_samples = (int)ceilf((float)l.samples * 1.f);
+	const int num_samples = light_samples(l, division);
 	const uint32_t offs = (uint32_t)num_samples * psd_sample + psd_offset + l_offs;
 	// areaLightSampleLight (:192-316) with AreaLight::illumSample (src/light/light_area.cc:99-124)
 	Halton h2, h3;
@@ -320,20 +683,20 @@ __device__ void queue_light(const PtArgs &a, const Surf &sp, V3 wo, const PtLigh
 		const float dist = sqrtf(dist_sqr);
 		if(dist <= 0.f)
 		{
-			set_slot(a, slot, rgb(0.f, 0.f, 0.f), false, -1);
+			set_slot_empty(a, slot);
 			continue;
 		}
 		ldir = mul(1.f / dist, ldir);
 		const float cos_angle = dot(ldir, l.normal_flipped);
 		if(cos_angle <= 0.f)
 		{
-			set_slot(a, slot, rgb(0.f, 0.f, 0.f), false, -1);
+			set_slot_empty(a, slot);
 			continue;
 		}
 		const float lpdf = dist_sqr * kPi / (l.area * cos_angle);
 		if(!(lpdf > 1e-6f))
 		{
-			set_slot(a, slot, rgb(0.f, 0.f, 0.f), false, -1);
+			set_slot_empty(a, slot);
 			continue;
 		}
 		const Rgb surf_col = bsdf_eval(a, sp, wo, ldir, kAll);
@@ -347,8 +710,7 @@ __device__ void queue_light(const PtArgs &a, const Surf &sp, V3 wo, const PtLigh
 			w = l_2 / (l_2 + m_2);
 		}
 		const Rgb c = cdiv(cscale(cscale(cmul(surf_col, l.color), angle), w), lpdf);
-		const int32_t q = cast_shadows ? queue_shadow(a, sp.p, ldir, shadow_tmin, dist) : -1;
-		set_slot(a, slot, c, true, q);
+		set_slot(a, slot, c, cast_shadows, sp.p, ldir, shadow_tmin, dist);
 	}
 	// areaLightSampleMaterial (:318-415) with AreaLight::intersect (src/light/light_area.cc:146-155)
 	const float btmin = 0.00005f * pscale;
@@ -366,10 +728,8 @@ __device__ void queue_light(const PtArgs &a, const Surf &sp, V3 wo, const PtLigh
 		float W = 0.f;
 		V3 dir = v3(0.f, 0.f, 0.f);
 		const Rgb surf_col = bsdf_sample(a, sp, wo, dir, s, W);
-		bool valid = false;
-		Rgb c = rgb(0.f, 0.f, 0.f);
-		int32_t q = -1;
 		const float cos_l = dot(dir, l.normal_flipped);
+		bool done = false;
 		if(cos_l > 0.f)
 		{
 			const float t = quad_intersect(l.quad, sp.p, dir);
@@ -382,13 +742,13 @@ __device__ void queue_light(const PtArgs &a, const Surf &sp, V3 wo, const PtLigh
 					const float l_2 = l_pdf * l_pdf;
 					const float m_2 = s.pdf * s.pdf;
 					const float w = m_2 / (l_2 + m_2);
-					c = cscale(cscale(cmul(surf_col, l.color), w), W);
-					valid = true;
-					q = cast_shadows ? queue_shadow(a, sp.p, dir, btmin, t) : -1;
+					const Rgb c = cscale(cscale(cmul(surf_col, l.color), w), W);
+					set_slot(a, slot, c, cast_shadows, sp.p, dir, btmin, t);
+					done = true;
 				}
 			}
 		}
-		set_slot(a, slot, c, valid, q);
+		if(!done) set_slot_empty(a, slot);
 	}
 }
 
@@ -403,7 +763,7 @@ __device__ __forceinline__ bool slot_lit(const PtArgs &a, uint32_t slot, Rgb &c)
 }
 
 // Sum of one light's slots in the order doLightEstimation accumulates them.
-__device__ Rgb resolve_light(const PtArgs &a, const PtLight &l, uint32_t slot0)
+__device__ Rgb resolve_light(const PtArgs &a, const PtLight &l, uint32_t division, uint32_t slot0)
 {
 	Rgb c;
 	if(l.type == YMX_LIGHT_POINT)
@@ -412,7 +772,7 @@ __device__ Rgb resolve_light(const PtArgs &a, const PtLight &l, uint32_t slot0)
 		if(slot_lit(a, slot0, c)) col = cadd(col, c);
 		return col;
 	}
-	const int num_samples = (int)ceilf((float)l.samples * 1.f);
+	const int num_samples = light_samples(l, division);
 	const float inv = 1.f / (float)num_samples;
 	Rgb col = rgb(0.f, 0.f, 0.f);
 	Rgb acc = rgb(0.f, 0.f, 0.f);
@@ -426,35 +786,43 @@ __device__ Rgb resolve_light(const PtArgs &a, const PtLight &l, uint32_t slot0)
 	return col;
 }
 
-__device__ __forceinline__ uint32_t path_offset(uint32_t x, uint32_t y) { return fnv32a(y * fnv32a(x)); }
-
-__device__ __forceinline__ void queue_extension(const PtArgs &a, int qsel, uint32_t path, V3 from, V3 dir, float tmin)
+__device__ __forceinline__ void queue_extension(const PtArgs &a, int qsel, uint32_t slot, V3 from, V3 dir, float tmin)
 {
 	const uint32_t q = atomicAdd(a.st.ext_count + qsel, 1u);
 	float4 *r = reinterpret_cast<float4 *>(a.st.ext_rays[qsel] + q);
 	r[0] = make_float4(from.x, from.y, from.z, tmin);
 	r[1] = make_float4(dir.x, dir.y, dir.z, -1.f);
-	a.st.ext_path[qsel][q] = path;
+	a.st.ext_path[qsel][q] = slot;
+}
+
+__device__ __forceinline__ void load_ray(const PtArgs &a, uint32_t n, V3 &from, V3 &dir)
+{
+	const float4 *r = reinterpret_cast<const float4 *>(a.nd.ray + n);
+	const float4 o = r[0], d = r[1];
+	from = v3(o.x, o.y, o.z);
+	dir = v3(d.x, d.y, d.z);
+}
+
+// max(1, path_samples / division) (integrator_path_tracer.cc:181)
+__device__ __forceinline__ int n_subpaths(const PtConst &c, uint32_t division)
+{
+	const int n = c.path_samples / (int)division;
+	return n > 1 ? n : 1;
 }
 
 // TiledIntegrator::renderTile sample set-up and PerspectiveCamera::shootRay
-// (src/integrator/surface/integrator_tiled.cc:290-335, src/camera/camera_perspective.cc:165-180)
+// (src/integrator/surface/integrator_tiled.cc:290-335, src/camera/camera_perspective.cc:165-180):
+// the level-0 nodes.
 __global__ __launch_bounds__(kBlock) void k_camera(PtArgs a)
 {
 	const PtConst &c = *a.c;
 	const uint32_t n = a.n_paths;
 	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-	if(i == 0)
-	{
-		a.st.ext_count[1] = 0u;
-		*a.st.sh_count = 0u;
-		a.st.totals[0] += n;
-	}
 	if(i >= n) return;
-	const uint32_t p = pix_of(a, i);
-	const uint32_t sample = sample_of(a, i);
+	const uint32_t p = i % a.npix;
+	const uint32_t sample = (uint32_t)a.first_sample + i / a.npix;
 	const uint32_t x = p % (uint32_t)c.resx, y = p / (uint32_t)c.resx;
-	const uint32_t offset = path_offset(x, y);
+	const uint32_t offset = fnv32a(y * fnv32a(x));
 	float dx = 0.5f, dy = 0.5f;
 	if(c.aa_samples > 1)
 	{
@@ -464,58 +832,199 @@ __global__ __launch_bounds__(kBlock) void k_camera(PtArgs a)
 	}
 	const float px = (float)x + dx, py = (float)y + dy;
 	const V3 dir = normalize(add(add(mul(px, c.vright), mul(py, c.vup)), c.vto));
-	a.st.col[i] = make_float4(0.f, 0.f, 0.f, 1.f);
-	a.st.path_col[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-	a.st.cam_dir[i] = make_float4(dir.x, dir.y, dir.z, 0.f);
 	a.st.dxy[i] = make_float2(dx, dy);
-	a.st.offset[i] = offset;
-	a.st.flags[i] = 0u;
-	float4 *r = reinterpret_cast<float4 *>(a.st.ext_rays[0] + i);
+	float4 *r = reinterpret_cast<float4 *>(a.nd.ray + i);
 	r[0] = make_float4(c.cam_from.x, c.cam_from.y, c.cam_from.z, 0.f);
 	r[1] = make_float4(dir.x, dir.y, dir.z, -1.f);
-	a.st.ext_path[0][i] = i;
+	a.nd.info[i] = make_uint4(i, 1u << 8, 0u, 0u);
+	a.nd.dec[i] = make_float2(0.f, 0.f);
 }
 
-// Camera vertex: emission and estimateAllDirectLight (integrator_path_tracer.cc:145-170)
-__global__ __launch_bounds__(kBlock) void k_shade_camera(PtArgs a)
+// One child of a recursive raytrace.
+__device__ void write_child(const PtArgs &a, uint32_t k, V3 from, V3 dir, Rgb wt, uint32_t path, uint32_t level,
+                            uint32_t division, int32_t rd_offset, float2 dec, uint32_t tag)
+{
+	float4 *r = reinterpret_cast<float4 *>(a.nd.ray + k);
+	r[0] = make_float4(from.x, from.y, from.z, 0.00005f);  // ray_min_dist_
+	r[1] = make_float4(dir.x, dir.y, dir.z, -1.f);
+	a.nd.info[k] = make_uint4(path, level | (division << 8), tag, (uint32_t)rd_offset);
+	a.nd.dec[k] = dec;
+	a.nd.wt[k] = make_float4(wt.r, wt.g, wt.b, 0.f);
+}
+
+// MonteCarloIntegrator::recursiveRaytrace (integrator_montecarlo.cc:698-748) for a
+// node at x.level: the glossy branches (:506-589 with glossyReflectNoTransmit
+// :623-645) or the perfect reflection / refraction (specularReflect/Refract,
+// :647-696, Material::getSpecular).  A child whose weight is exactly zero adds
+// nothing to its parent and is not traced.
+__device__ void spawn_children(const PtArgs &a, const NodeCtx &x, const Surf &sp, V3 wo)
+{
+	const PtConst &c = *a.c;
+	const int ray_level = (int)x.level + 1;
+	uint2 ch = make_uint2(0u, 0u);
+	if(!(ray_level <= c.raydepth && ray_level < 20 && spawns_children(sp.flags)))
+	{
+		a.nd.child[x.n] = ch;
+		return;
+	}
+	const PtMaterial &m = a.mats[sp.mat];
+	if(sp.flags & kGlossy)
+	{
+		if(!(sp.flags & kReflect))
+		{
+			a.nd.child[x.n] = ch;
+			return;
+		}
+		const uint32_t rsg = x.division > 1 ? (kGlossyBranches / x.division > 1 ? kGlossyBranches / x.division : 1u)
+		                                   : kGlossyBranches;
+		const uint32_t ndiv = x.division * rsg;
+		int32_t branch = (int32_t)ndiv * x.rd_offset;
+		const uint32_t offs = rsg * x.sample + x.offset;
+		Halton h2, h3;
+		h2.start(2, offs);
+		h3.start(3, offs);
+		V3 dirs[kGlossyBranches];
+		Rgb wts[kGlossyBranches];
+		float2 decs[kGlossyBranches];
+		int32_t offsets[kGlossyBranches];
+		uint32_t tags[kGlossyBranches];
+		uint32_t cnt = 0;
+		for(uint32_t ns = 0; ns < rsg; ++ns)
+		{
+			const float dec1 = (float)low_discrepancy(a, 2 * ray_level + 1, (uint32_t)branch + x.offset);
+			const float dec2 = (float)low_discrepancy(a, 2 * ray_level + 2, (uint32_t)branch + x.offset);
+			const int32_t boff = branch;
+			++branch;
+			BSample s;
+			s.s1 = h2.next();
+			s.s2 = h3.next();
+			s.flags = kGlossy | kReflect;
+			s.pdf = 0.f;
+			s.sampled = kNone;
+			float w = 0.f;
+			V3 wi = v3(0.f, 0.f, 0.f);
+			const Rgb mcol = bsdf_sample(a, sp, wo, wi, s, w);
+			const Rgb wt = cscale(mcol, w);
+			if(cblack(wt)) continue;
+			dirs[cnt] = wi;
+			wts[cnt] = wt;
+			decs[cnt] = make_float2(dec1, dec2);
+			offsets[cnt] = boff;
+			tags[cnt] = x.tag * 11u + 1u + ns;
+			++cnt;
+		}
+		if(cnt)
+		{
+			const uint32_t first = a.next_start + atomicAdd(a.nd.spawn, cnt);
+			for(uint32_t k = 0; k < cnt; ++k)
+				write_child(a, first + k, sp.p, dirs[k], wts[k], x.path, (uint32_t)ray_level, ndiv, offsets[k], decs[k],
+				            tags[k]);
+			ch = make_uint2(first, cnt | (kChildGlossy << 8) | (rsg << 16));
+		}
+		a.nd.child[x.n] = ch;
+		return;
+	}
+	// perfect specular: mirror (src/material/material_mirror.cc:78-86), glass (material_glass.cc:303-366)
+	bool has_r = false, has_t = false;
+	V3 dir_r = v3(0.f, 0.f, 0.f), dir_t = v3(0.f, 0.f, 0.f);
+	Rgb col_r = rgb(0.f, 0.f, 0.f), col_t = rgb(0.f, 0.f, 0.f);
+	if(m.type == YMX_MAT_MIRROR)
+	{
+		has_r = true;
+		col_r = m.ref_col;
+		dir_r = reflect_dir(face_forward(sp.ng, sp.n, wo), wo);
+	}
+	else if(m.type == YMX_MAT_GLASS)
+	{
+		bool outside;
+		const V3 n = glass_normal(sp, wo, outside);
+		V3 refdir;
+		if(refract(n, wo, refdir, m.ior))
+		{
+			float kr, kt;
+			fresnel(wo, n, m.ior, kr, kt);
+			has_t = true;
+			col_t = cscale(m.filter_color, kt);
+			dir_t = refdir;
+			if(outside || ray_level < 3)
+			{
+				has_r = true;
+				dir_r = reflect_about(wo, n);
+				col_r = cscale(m.mirror_color, kr);
+			}
+		}
+		else
+		{
+			has_r = true;
+			col_r = m.mirror_color;
+			dir_r = reflect_about(wo, n);
+		}
+	}
+	if(has_r && cblack(col_r)) has_r = false;
+	if(has_t && cblack(col_t)) has_t = false;
+	const uint32_t cnt = (has_r ? 1u : 0u) + (has_t ? 1u : 0u);
+	if(cnt)
+	{
+		// the ray division passes down unchanged (specularReflect / specularRefract)
+		const float2 dec = a.nd.dec[x.n];
+		const uint32_t first = a.next_start + atomicAdd(a.nd.spawn, cnt);
+		uint32_t k = first;
+		if(has_r)
+			write_child(a, k++, sp.p, dir_r, col_r, x.path, (uint32_t)ray_level, x.division, x.rd_offset, dec,
+			            x.tag * 11u + 9u);
+		if(has_t)
+			write_child(a, k, sp.p, dir_t, col_t, x.path, (uint32_t)ray_level, x.division, x.rd_offset, dec,
+			            x.tag * 11u + 10u);
+		ch = make_uint2(first, cnt | (kChildSpecular << 8));
+	}
+	a.nd.child[x.n] = ch;
+}
+
+// Node vertex: emission, estimateAllDirectLight and the children of the
+// recursive raytrace (integrator_path_tracer.cc:145-170, 289).
+__global__ __launch_bounds__(kBlock) void k_shade_node(PtArgs a)
 {
 	const PtConst &c = *a.c;
 	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-	if(i >= a.n_paths) return;  // the camera queue is the identity
+	if(i >= a.cnt) return;
+	const NodeCtx x = node_ctx(a, i);
 	const ymx_hit h = a.st.ext_hits[i];
 	a.st.cam_hit[i] = h;
 	if(h.prim == YMX_NO_HIT)
 	{
 		// TiledIntegrator::background (integrator_tiled.cc:693-706)
-		if(c.has_background) a.st.col[i] = make_float4(c.background.r, c.background.g, c.background.b, 1.f);
+		a.nd.col[x.n] = c.has_background ? make_float4(c.background.r, c.background.g, c.background.b, 1.f)
+		                                 : make_float4(0.f, 0.f, 0.f, 1.f);
+		a.nd.child[x.n] = make_uint2(0u, 0u);
+		a.st.flags[i] = 0u;
 		return;
 	}
-	const float4 d4 = a.st.cam_dir[i];
-	const V3 dir = v3(d4.x, d4.y, d4.z);
+	V3 from, dir;
+	load_ray(a, x.n, from, dir);
 	Surf sp;
-	make_surf(a, c.cam_from, dir, h, sp);
+	make_surf(a, from, dir, h, sp);
 	const V3 wo = neg(dir);
 	Rgb col = rgb(0.f, 0.f, 0.f);
-	if(sp.flags & kEmit) col = cadd(col, a.mats[sp.mat].emit_color);
-	a.st.col[i] = make_float4(col.r, col.g, col.b, 1.f);
+	if(sp.flags & kEmit) col = cadd(col, mat_emit(a, sp, wo));
+	a.nd.col[x.n] = make_float4(col.r, col.g, col.b, 1.f);
 	uint32_t f = 0u;
 	if(sp.flags & kDiffuse)
 	{
-		const uint32_t offset = a.st.offset[i];
-		const uint32_t sample = sample_of(a, i);
 		uint32_t slot = i * a.st.slots;
 		for(uint32_t l = 0; l < c.n_lights; ++l)
 		{
-			queue_light(a, sp, wo, a.lights[l], l, sample, offset, slot);
-			slot += light_slots(a.lights[l]);
+			queue_light(a, sp, wo, a.lights[l], l, x.division, x.sample, x.offset, slot);
+			slot += light_slots(a.lights[l], x.division);
 		}
 		f |= kDirect;
 	}
 	if(sp.flags & kDiffuse) f |= kPathBlock;  // path_flags = Diffuse (no_recursive = false)
 	a.st.flags[i] = f;
+	if(a.recursive) spawn_children(a, x, sp, wo);
+	else a.nd.child[x.n] = make_uint2(0u, 0u);
 }
 
-__global__ __launch_bounds__(kBlock) void k_resolve_camera(PtArgs a)
+__global__ __launch_bounds__(kBlock) void k_resolve_node(PtArgs a)
 {
 	const PtConst &c = *a.c;
 	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -527,18 +1036,21 @@ __global__ __launch_bounds__(kBlock) void k_resolve_camera(PtArgs a)
 		a.st.ext_count[0] = 0u;
 		a.st.ext_count[1] = 0u;
 	}
-	if(i >= a.n_paths) return;
+	if(i >= a.cnt) return;
+	a.st.path_col[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 	if(!(a.st.flags[i] & kDirect)) return;
+	const uint32_t n = a.base + i;
+	const uint32_t division = a.nd.info[n].y >> 8;
 	Rgb col_all = rgb(0.f, 0.f, 0.f);
 	uint32_t slot = i * a.st.slots;
 	for(uint32_t l = 0; l < c.n_lights; ++l)
 	{
-		col_all = cadd(col_all, resolve_light(a, a.lights[l], slot));
-		slot += light_slots(a.lights[l]);
+		col_all = cadd(col_all, resolve_light(a, a.lights[l], division, slot));
+		slot += light_slots(a.lights[l], division);
 	}
-	const float4 c4 = a.st.col[i];
+	const float4 c4 = a.nd.col[n];
 	const Rgb col = cadd(rgb(c4.x, c4.y, c4.z), col_all);
-	a.st.col[i] = make_float4(col.r, col.g, col.b, c4.w);
+	a.nd.col[n] = make_float4(col.r, col.g, col.b, c4.w);
 	a.st.flags[i] &= ~kDirect;
 }
 
@@ -547,21 +1059,32 @@ __global__ __launch_bounds__(kBlock) void k_subpath_init(PtArgs a)
 {
 	const PtConst &c = *a.c;
 	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-	if(i >= a.n_paths) return;
+	if(i >= a.cnt) return;
 	uint32_t f = a.st.flags[i];
 	f &= ~(kActive | kAddEmit | kCaustic | kDirect | kVertex | kSampled);
 	a.st.flags[i] = f;
 	if(!(f & kPathBlock)) return;
+	const NodeCtx x = node_ctx(a, i);
+	if(a.sub >= n_subpaths(c, x.division)) return;
 	const ymx_hit h = a.st.cam_hit[i];
-	const float4 d4 = a.st.cam_dir[i];
-	const V3 dir = v3(d4.x, d4.y, d4.z);
+	V3 from, dir;
+	load_ray(a, x.n, from, dir);
 	Surf sp;
-	make_surf(a, c.cam_from, dir, h, sp);
+	make_surf(a, from, dir, h, sp);
 	const V3 wo = neg(dir);
-	const uint32_t offs = (uint32_t)c.path_samples * sample_of(a, i) + a.st.offset[i] + (uint32_t)a.sub;
+	const uint32_t offs = (uint32_t)c.path_samples * x.sample + x.offset + (uint32_t)a.sub;
 	BSample s;
 	s.s1 = ri_vdc(offs, 0);
 	s.s2 = (float)low_discrepancy(a, 2, offs);
+	if(x.division > 1)
+	{
+		// math::addMod1 with the branch's decorrelation (include/math/math.h:304-309)
+		const float2 dec = a.nd.dec[x.n];
+		float t = s.s1 + dec.x;
+		s.s1 = t > 1 ? t - 1 : t;
+		t = s.s2 + dec.y;
+		s.s2 = t > 1 ? t - 1 : t;
+	}
 	s.flags = kDiffuse | kDiffuse | kReflect | kTransmit;
 	s.pdf = 0.f;
 	s.sampled = kNone;
@@ -569,7 +1092,8 @@ __global__ __launch_bounds__(kBlock) void k_subpath_init(PtArgs a)
 	V3 pdir = v3(0.f, 0.f, 0.f);
 	Rgb scol = bsdf_sample(a, sp, wo, pdir, s, w);
 	scol = cscale(scol, w);
-	a.st.tp_next[i] = make_float4(scol.r, scol.g, scol.b, 0.f);
+	// w rides along in tp_next.w: a later sample call that leaves w unset reuses it
+	a.st.tp_next[i] = make_float4(scol.r, scol.g, scol.b, w);
 	a.st.pwo[i] = make_float4(wo.x, wo.y, wo.z, 0.f);
 	a.st.flags[i] = f | kActive | (s.sampled != kNone ? kSampled : 0u);
 	queue_extension(a, 0, i, sp.p, pdir, 0.00005f);
@@ -601,9 +1125,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_path(PtArgs a)
 	const float4 tpn = a.st.tp_next[i];
 	Rgb tp = rgb(tpn.x, tpn.y, tpn.z);
 	uint32_t first_flags;
-	const uint32_t pixel = pix_of(a, i);  // camera_res_x * y + x
-	const uint32_t sample = sample_of(a, i);
-	const uint32_t offset = a.st.offset[i];
+	const NodeCtx x = node_ctx(a, i);
 	if(d == 0)
 	{
 		const float4 p4 = a.st.pwo[i];
@@ -617,8 +1139,8 @@ __global__ __launch_bounds__(kBlock) void k_shade_path(PtArgs a)
 		first_flags = (f >> 8) & 0x1FFu;
 		if(d > c.rr_min_bounces)
 		{
-			const float random_value = rr_random(pixel, sample * (uint32_t)c.path_samples + (uint32_t)a.sub,
-			                                     (uint32_t)d, c.seed);
+			const float random_value = rr_random(x.pixel, x.sample * (uint32_t)c.path_samples + (uint32_t)a.sub,
+			                                     (uint32_t)d + 256u * x.tag, c.seed);
 			const float probability = cmaximum(tp);
 			if(probability <= 0.f || probability < random_value)
 			{
@@ -628,33 +1150,39 @@ __global__ __launch_bounds__(kBlock) void k_shade_path(PtArgs a)
 			tp = cscale(tp, 1.f / probability);
 		}
 	}
-	a.st.tp_vertex[i] = make_float4(tp.r, tp.g, tp.b, 0.f);
-	a.st.vert_mat[i] = sp.mat;
-	f |= kVertex;
-	const bool need_direct = (d == 0) || (first_flags & kDiffuse);
-	if(need_direct && c.n_lights > 0)
+	// A vertex with zero throughput adds lcol * 0 = 0 to the path colour: its
+	// light estimation and emission are skipped.
+	if(!cblack(tp))
 	{
-		// estimateOneDirectLight (integrator_montecarlo.cc:108-116) with the
-		// deterministic Halton base-2 light-pick index
-		const uint32_t corr = (pixel * (uint32_t)c.aa_samples + sample) *
-		                          ((uint32_t)c.path_samples * (uint32_t)c.bounces) +
-		                      (uint32_t)a.sub * (uint32_t)c.bounces + (uint32_t)d + 1u;
-		const float hv = halton_next(2, corr - 1u);
-		int lnum = (int)(hv * (float)c.n_lights);
-		if(lnum > (int)c.n_lights - 1) lnum = (int)c.n_lights - 1;
-		queue_light(a, sp, pwo, a.lights[lnum], (uint32_t)lnum, sample, offset, i * a.st.slots);
-		a.st.light[i] = lnum;
-		f |= kDirect;
+		a.st.tp_vertex[i] = make_float4(tp.r, tp.g, tp.b, 0.f);
+		f |= kVertex;
+		const bool need_direct = (d == 0) || (first_flags & kDiffuse);
+		if(need_direct && c.n_lights > 0)
+		{
+			// estimateOneDirectLight (integrator_montecarlo.cc:108-116) with the
+			// deterministic Halton base-2 light-pick index
+			const uint32_t corr = (x.pixel * (uint32_t)c.aa_samples + x.sample) *
+			                          ((uint32_t)c.path_samples * (uint32_t)c.bounces) +
+			                      (uint32_t)a.sub * (uint32_t)c.bounces + (uint32_t)d + 1u + x.tag * 0x9E3779B9u;
+			const float hv = halton_next(2, corr - 1u);
+			int lnum = (int)(hv * (float)c.n_lights);
+			if(lnum > (int)c.n_lights - 1) lnum = (int)c.n_lights - 1;
+			queue_light(a, sp, pwo, a.lights[lnum], (uint32_t)lnum, x.division, x.sample, x.offset, i * a.st.slots);
+			a.st.light[i] = lnum;
+			f |= kDirect;
+		}
+		const bool add_emit = (d == 0) ? (first_flags & kEmit) != 0 : ((first_flags & kEmit) && (f & kCaustic));
+		if(add_emit)
+		{
+			const Rgb e = mat_emit(a, sp, pwo);
+			a.st.vert_emit[i] = make_float4(e.r, e.g, e.b, 0.f);
+			f |= kAddEmit;
+		}
 	}
-	if(d == 0)
-	{
-		if(first_flags & kEmit) f |= kAddEmit;
-	}
-	else if((first_flags & kEmit) && (f & kCaustic)) f |= kAddEmit;
 	// next segment (loop iteration depth = d + 1)
 	if(d + 1 < c.bounces)
 	{
-		const uint32_t offs = (uint32_t)c.path_samples * sample + offset + (uint32_t)a.sub;
+		const uint32_t offs = (uint32_t)c.path_samples * x.sample + x.offset + (uint32_t)a.sub;
 		const int d_4 = 4 * (d + 1);
 		BSample s;
 		s.s1 = (float)low_discrepancy(a, d_4 + 3, offs);
@@ -662,14 +1190,17 @@ __global__ __launch_bounds__(kBlock) void k_shade_path(PtArgs a)
 		s.flags = kAll;
 		s.pdf = 0.f;
 		s.sampled = kNone;
-		float w = 0.f;
-		V3 ndir = v3(0.f, 0.f, 0.f);
+		// The reference keeps w and the ray direction in variables of the whole
+		// integrate() call: a sample that leaves them unset (shinydiffuse without a
+		// diffuse component returns white) continues straight on with the last w.
+		float w = tpn.w;
+		V3 ndir = dir;
 		Rgb scol = bsdf_sample(a, sp, pwo, ndir, s, w);
 		scol = cscale(scol, w);
 		if(!cblack(scol))
 		{
 			const Rgb tpn2 = cmul(tp, scol);
-			a.st.tp_next[i] = make_float4(tpn2.r, tpn2.g, tpn2.b, 0.f);
+			a.st.tp_next[i] = make_float4(tpn2.r, tpn2.g, tpn2.b, w);
 			if(s.sampled & (kSpecular | kGlossy | kFilter)) f |= kCaustic;
 			else f &= ~kCaustic;
 			f |= kActive;
@@ -690,16 +1221,21 @@ __global__ __launch_bounds__(kBlock) void k_resolve_path(PtArgs a)
 		*a.st.sh_count = 0u;
 		a.st.ext_count[a.depth & 1] = 0u;  // this segment's queue, consumed by shade_path
 	}
-	if(i >= a.n_paths) return;
+	if(i >= a.cnt) return;
 	const uint32_t f = a.st.flags[i];
 	if(!(f & kVertex)) return;
 	Rgb lcol = rgb(0.f, 0.f, 0.f);
 	if(f & kDirect)
 	{
 		const int lnum = a.st.light[i];
-		lcol = cscale(resolve_light(a, a.lights[lnum], i * a.st.slots), (float)c.n_lights);
+		const uint32_t division = a.nd.info[a.base + i].y >> 8;
+		lcol = cscale(resolve_light(a, a.lights[lnum], division, i * a.st.slots), (float)c.n_lights);
 	}
-	if(f & kAddEmit) lcol = cadd(lcol, a.mats[a.st.vert_mat[i]].emit_color);
+	if(f & kAddEmit)
+	{
+		const float4 e = a.st.vert_emit[i];
+		lcol = cadd(lcol, rgb(e.x, e.y, e.z));
+	}
 	const float4 t4 = a.st.tp_vertex[i];
 	const float4 p4 = a.st.path_col[i];
 	const Rgb pc = cadd(rgb(p4.x, p4.y, p4.z), cmul(lcol, rgb(t4.x, t4.y, t4.z)));
@@ -711,13 +1247,42 @@ __global__ __launch_bounds__(kBlock) void k_finish(PtArgs a)
 {
 	const PtConst &c = *a.c;
 	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-	if(i >= a.n_paths) return;
+	if(i >= a.cnt) return;
 	if(!(a.st.flags[i] & kPathBlock)) return;
-	const int n_samples = c.path_samples > 1 ? c.path_samples : 1;
+	const uint32_t n = a.base + i;
+	const int n_samples = n_subpaths(c, a.nd.info[n].y >> 8);
 	const float4 p4 = a.st.path_col[i];
-	const float4 c4 = a.st.col[i];
+	const float4 c4 = a.nd.col[n];
 	const Rgb col = cadd(rgb(c4.x, c4.y, c4.z), cdiv(rgb(p4.x, p4.y, p4.z), (float)n_samples));
-	a.st.col[i] = make_float4(col.r, col.g, col.b, c4.w);
+	a.nd.col[n] = make_float4(col.r, col.g, col.b, c4.w);
+}
+
+// Adds the children's colours into the nodes of one level, in the order
+// recursiveRaytrace sums them: the glossy branches as (sum of child * weight)
+// / branches; the specular reflection, then the refraction.
+__global__ __launch_bounds__(kBlock) void k_combine(PtArgs a)
+{
+	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if(i >= a.cnt) return;
+	const uint32_t n = a.base + i;
+	const uint2 ch = a.nd.child[n];
+	const uint32_t cnt = ch.y & 0xFFu;
+	if(cnt == 0) return;
+	Rgb sum = rgb(0.f, 0.f, 0.f);
+	for(uint32_t k = 0; k < cnt; ++k)
+	{
+		const float4 cc = a.nd.col[ch.x + k];
+		const float4 w = a.nd.wt[ch.x + k];
+		sum = cadd(sum, cmul(rgb(cc.x, cc.y, cc.z), rgb(w.x, w.y, w.z)));
+	}
+	if(((ch.y >> 8) & 0xFFu) == kChildGlossy)
+	{
+		const uint32_t rsg = ch.y >> 16;
+		sum = cadd(rgb(0.f, 0.f, 0.f), cscale(sum, 1.f / (float)rsg));
+	}
+	const float4 c4 = a.nd.col[n];
+	const Rgb col = cadd(rgb(c4.x, c4.y, c4.z), sum);
+	a.nd.col[n] = make_float4(col.r, col.g, col.b, c4.w);
 }
 
 // roundToIntWithBias, include/render/imagefilm.h:344-348
@@ -761,7 +1326,7 @@ __global__ __launch_bounds__(kBlock) void k_film(PtArgs a)
 			xi = xi > 15 ? 15 : xi;
 			yi = yi > 15 ? 15 : yi;
 			const float wt = c.filter_table[yi * 16 + xi];
-			const float4 col = a.st.col[s];
+			const float4 col = a.nd.col[s];  // camera sample s is node s
 			w = w + wt;
 			acc.x += wt * col.x;
 			acc.y += wt * col.y;
@@ -773,53 +1338,32 @@ __global__ __launch_bounds__(kBlock) void k_film(PtArgs a)
 	a.film[t] = acc;
 }
 
+__global__ void k_count_rays(unsigned long long *total, uint32_t n)
+{
+	if(blockIdx.x == 0 && threadIdx.x == 0) *total += n;
+}
+
 inline unsigned blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
-}  // namespace
-}  // namespace ymx
-
-int ymx_pt_trace_pass(ymx_pt *pt, int32_t first_sample, int32_t count, hipStream_t stream)
+// The stages of one chunk of nodes [a.base, a.base + a.cnt).
+int trace_chunk(ymx_pt *pt, PtArgs &a, hipStream_t stream)
 {
-	using namespace ymx;
 	const PtConst &hc = pt->host_const;
-	const uint32_t npix = (uint32_t)(hc.resx * hc.resy);
-	const uint32_t n = npix * (uint32_t)count;
-	if(count < 1 || n > pt->st.capacity)
-	{
-		ymx_set_error("ymx_pt_trace_pass: %d samples exceed the wavefront capacity", count);
-		return YMX_ERR_INVALID_ARG;
-	}
-	PtArgs a;
-	a.c = pt->d_const;
-	a.mats = pt->d_materials;
-	a.lights = pt->d_lights;
-	a.prim = pt->d_prim;
-	a.faure = pt->d_faure;
-	a.inv_prims = pt->d_inv_prims;
-	a.prims = pt->d_prims;
-	a.st = pt->st;
-	a.film = pt->d_film;
-	a.film_w = pt->d_film_w;
-	a.first_sample = first_sample;
-	a.npix = npix;
-	a.n_paths = n;
-	a.count = count;
-	a.sub = 0;
-	a.depth = 0;
+	const uint32_t n = a.cnt;
 	const unsigned g = blocks_for(n);
 	const uint32_t slot_cap = n * pt->st.slots;
-	k_camera<<<g, kBlock, 0, stream>>>(a);
+	k_count_rays<<<1, 64, 0, stream>>>(pt->st.totals, n);
 	YMX_HIP_CHECK(hipGetLastError());
-	int rc = ymx_launch_closest(pt->accel, pt->st.ext_rays[0], pt->st.ext_hits, n, stream, nullptr);
+	int rc = ymx_launch_closest(pt->accel, pt->nd.ray + a.base, pt->st.ext_hits, n, stream, nullptr);
 	if(rc != YMX_OK) return rc;
-	k_shade_camera<<<g, kBlock, 0, stream>>>(a);
+	k_shade_node<<<g, kBlock, 0, stream>>>(a);
 	YMX_HIP_CHECK(hipGetLastError());
 	if(hc.n_lights > 0)
 	{
 		rc = ymx_launch_shadow_queue(pt->accel, pt->st.sh_rays, pt->st.sh_occ, pt->st.sh_count, slot_cap, stream);
 		if(rc != YMX_OK) return rc;
 	}
-	k_resolve_camera<<<g, kBlock, 0, stream>>>(a);
+	k_resolve_node<<<g, kBlock, 0, stream>>>(a);
 	YMX_HIP_CHECK(hipGetLastError());
 	const int n_sub = hc.path_samples > 1 ? hc.path_samples : 1;
 	for(int sub = 0; sub < n_sub; ++sub)
@@ -847,6 +1391,94 @@ int ymx_pt_trace_pass(ymx_pt *pt, int32_t first_sample, int32_t count, hipStream
 	}
 	k_finish<<<g, kBlock, 0, stream>>>(a);
 	YMX_HIP_CHECK(hipGetLastError());
+	return YMX_OK;
+}
+
+}  // namespace
+}  // namespace ymx
+
+int ymx_pt_trace_pass(ymx_pt *pt, int32_t first_sample, int32_t count, hipStream_t stream)
+{
+	using namespace ymx;
+	const PtConst &hc = pt->host_const;
+	const uint32_t npix = (uint32_t)(hc.resx * hc.resy);
+	const uint32_t n = npix * (uint32_t)count;
+	if(count < 1 || n > pt->st.capacity || n > pt->nd.cap)
+	{
+		ymx_set_error("ymx_pt_trace_pass: %d samples exceed the wavefront capacity", count);
+		return YMX_ERR_INVALID_ARG;
+	}
+	PtArgs a;
+	a.c = pt->d_const;
+	a.mats = pt->d_materials;
+	a.lights = pt->d_lights;
+	a.prim = pt->d_prim;
+	a.faure = pt->d_faure;
+	a.inv_prims = pt->d_inv_prims;
+	a.prims = pt->d_prims;
+	a.st = pt->st;
+	a.nd = pt->nd;
+	a.film = pt->d_film;
+	a.film_w = pt->d_film_w;
+	a.first_sample = first_sample;
+	a.npix = npix;
+	a.n_paths = n;
+	a.count = count;
+	a.sub = 0;
+	a.depth = 0;
+	k_camera<<<blocks_for(n), kBlock, 0, stream>>>(a);
+	YMX_HIP_CHECK(hipGetLastError());
+	// levels of integrate() calls, node ranges [start, start + size)
+	std::vector<uint32_t> lvl_start{0u}, lvl_size{n};
+	const uint32_t cap = pt->st.capacity;
+	for(int level = 0;; ++level)
+	{
+		const uint32_t start = lvl_start[level], size = lvl_size[level];
+		const uint32_t next_start = start + size;
+		const bool spawn = pt->recursive && level + 1 <= hc.raydepth;
+		uint32_t spawned = 0;
+		if(spawn) YMX_HIP_CHECK(hipMemsetAsync(pt->nd.spawn, 0, sizeof(uint32_t), stream));
+		for(uint32_t base = start; base < next_start; base += cap)
+		{
+			a.base = base;
+			a.cnt = (next_start - base) < cap ? (next_start - base) : cap;
+			a.next_start = next_start;
+			a.recursive = spawn ? 1u : 0u;
+			if(spawn)
+			{
+				// room for every child this chunk can spawn, after those spawned so far
+				const uint64_t need = (uint64_t)next_start + spawned + (uint64_t)a.cnt * pt->max_fan;
+				if(need > pt->nd.cap)
+				{
+					const int rc = ymx_pt_grow_nodes(pt, need, next_start + spawned);
+					if(rc != YMX_OK) return rc;
+					a.nd = pt->nd;
+				}
+			}
+			const int rc = trace_chunk(pt, a, stream);
+			if(rc != YMX_OK) return rc;
+			if(spawn)
+			{
+				YMX_HIP_CHECK(hipMemcpyAsync(&spawned, pt->nd.spawn, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+				YMX_HIP_CHECK(hipStreamSynchronize(stream));
+			}
+		}
+		if(spawned == 0) break;
+		lvl_start.push_back(next_start);
+		lvl_size.push_back(spawned);
+	}
+	// children into parents, deepest level first
+	for(int level = (int)lvl_start.size() - 2; level >= 0; --level)
+	{
+		const uint32_t end = lvl_start[level] + lvl_size[level];
+		for(uint32_t base = lvl_start[level]; base < end; base += cap)
+		{
+			a.base = base;
+			a.cnt = (end - base) < cap ? (end - base) : cap;
+			k_combine<<<blocks_for(a.cnt), kBlock, 0, stream>>>(a);
+			YMX_HIP_CHECK(hipGetLastError());
+		}
+	}
 	k_film<<<blocks_for(npix), kBlock, 0, stream>>>(a);
 	YMX_HIP_CHECK(hipGetLastError());
 	return YMX_OK;

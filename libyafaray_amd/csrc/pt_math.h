@@ -48,6 +48,19 @@ YMX_HD V3 normalize(V3 a)
 	return a;
 }
 YMX_HD V3 load3(const float *p) { return V3{p[0], p[1], p[2]}; }
+// Vec3f::reflect in place: vn = (2 v) . n, v = vn n - v (include/geometry/vector.h:357-362)
+YMX_HD V3 reflect_about(V3 v, V3 n)
+{
+	const float vn = dot(mul(2.f, v), n);
+	return V3{vn * n.x - v.x, vn * n.y - v.y, vn * n.z - v.z};
+}
+// Vec3f::reflectDir (include/geometry/vector.h:390-396)
+YMX_HD V3 reflect_dir(V3 normal, V3 vec)
+{
+	const float vn = dot(vec, normal);
+	if(vn < 0.f) return neg(vec);
+	return sub(mul(2.f * vn, normal), vec);
+}
 
 YMX_HD Rgb rgb(float r, float g, float b) { return Rgb{r, g, b}; }
 YMX_HD Rgb cadd(Rgb a, Rgb b) { return Rgb{a.r + b.r, a.g + b.g, a.b + b.b}; }
@@ -89,6 +102,29 @@ YMX_HD float fast_exp2(float x)
 	return e.f * poly_exp(fpart);
 }
 YMX_HD float fast_exp(float a) { return fast_exp2(1.44269504088896340736f * a); }
+YMX_HD float poly_log(float x)
+{
+	return x * (x * (x * (x * (x * -3.4436006e-2f + 3.1821337e-1f) + -1.2315303f) + 2.5988452f) + -3.3241990f) +
+	       3.1157899f;
+}
+// log2 from the exponent bits and a polynomial in the mantissa (the sign bit is ignored)
+YMX_HD float fast_log2(float x)
+{
+	union
+	{
+		int i;
+		float f;
+	} v, m;
+	v.f = x;
+	m.i = (v.i & 0x7FFFFF) | 0x3F800000;  // mantissa with the exponent of 1.0f
+	const float e = (float)(((v.i & 0x7F800000) >> 23) - 127);
+	return poly_log(m.f) * (m.f - 1.f) + e;
+}
+// math::pow = exp2(log2(a) * b) under FAST_MATH (include/math/math.h:176-183)
+YMX_HD float fast_pow(float a, float b) { return fast_exp2(fast_log2(a) * b); }
+
+YMX_HD float std_max(float a, float b) { return (a < b) ? b : a; }
+YMX_HD float std_min(float a, float b) { return (b < a) ? b : a; }
 
 // FAST_TRIG sine (include/math/math.h:228-255): parabola with a 0.225 correction.
 YMX_HD float fast_sin(float x)

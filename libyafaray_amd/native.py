@@ -85,6 +85,13 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not LIB_PATH.exists():
             raise YmxError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+        # torch ships its own libamdhip64.so.7: load it first so that libymx.so
+        # binds to the same HIP runtime (one runtime per process), whichever of
+        # the two the caller touches first
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         handle = C.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(handle, name)

@@ -19,7 +19,16 @@ from .native import check
 
 class YmxMaterial(C.Structure):
     _fields_ = [("type", C.c_uint32), ("color", C.c_float * 3), ("diffuse_reflect", C.c_float),
-                ("emit", C.c_float), ("receive_shadows", C.c_uint32)]
+                ("emit", C.c_float), ("receive_shadows", C.c_uint32), ("diffuse_color", C.c_float * 3),
+                ("glossy_reflect", C.c_float), ("exponent", C.c_float), ("as_diffuse", C.c_uint32),
+                ("reflect", C.c_float), ("ior", C.c_float), ("filter_color", C.c_float * 3),
+                ("transmit_filter", C.c_float), ("mirror_color", C.c_float * 3), ("fake_shadows", C.c_uint32),
+                ("power", C.c_float), ("double_sided", C.c_uint32)]
+
+
+# material "type" param values of the reference (include/material/material.h:151-166), plus short aliases
+MATERIAL_TYPES = {"shinydiffuse": 0, "shinydiffusemat": 0, "glossy": 1, "mirror": 2, "glass": 3,
+                  "light": 4, "light_mat": 4}
 
 
 class YmxLight(C.Structure):
@@ -33,7 +42,7 @@ class YmxPtSettings(C.Structure):
                 ("aspect_factor", C.c_float), ("resx", C.c_int32), ("resy", C.c_int32),
                 ("path_samples", C.c_int32), ("bounces", C.c_int32), ("rr_min_bounces", C.c_int32),
                 ("aa_samples", C.c_int32), ("aa_pixelwidth", C.c_float), ("has_background", C.c_int32),
-                ("background", C.c_float * 3), ("seed", C.c_uint32)]
+                ("background", C.c_float * 3), ("seed", C.c_uint32), ("raydepth", C.c_int32)]
 
 
 YmxPtStats = native.YmxPtStats
@@ -43,14 +52,35 @@ def _f3(v):
     return (C.c_float * 3)(*[float(x) for x in v])
 
 
+def fill_material(dst, m):
+    """Material dict with the reference's param names; absent params take the
+    reference's defaults for that material type."""
+    kind = m.get("type", "shinydiffuse")
+    if kind not in MATERIAL_TYPES:
+        raise ValueError(f"unsupported material type {kind!r}")
+    dst.type = MATERIAL_TYPES[kind]
+    dst.color = _f3(m.get("color", (1, 1, 1)))
+    dst.diffuse_reflect = float(m.get("diffuse_reflect", 0.0 if kind == "glossy" else 1.0))
+    dst.emit = float(m.get("emit", 0.0))
+    dst.receive_shadows = 1 if m.get("receive_shadows", True) else 0
+    dst.diffuse_color = _f3(m.get("diffuse_color", (1, 1, 1)))
+    dst.glossy_reflect = float(m.get("glossy_reflect", 1.0))
+    dst.exponent = float(m.get("exponent", 50.0))
+    dst.as_diffuse = 1 if m.get("as_diffuse", True) else 0
+    dst.reflect = float(m.get("reflect", 1.0))
+    dst.ior = float(m.get("IOR", 1.4))
+    dst.filter_color = _f3(m.get("filter_color", (1, 1, 1)))
+    dst.transmit_filter = float(m.get("transmit_filter", 0.0))
+    dst.mirror_color = _f3(m.get("mirror_color", (1, 1, 1)))
+    dst.fake_shadows = 1 if m.get("fake_shadows", False) else 0
+    dst.power = float(m.get("power", 1.0))
+    dst.double_sided = 1 if m.get("double_sided", False) else 0
+
+
 def build_structs(scene):
     mats = (YmxMaterial * max(1, len(scene["materials"])))()
     for i, m in enumerate(scene["materials"]):
-        mats[i].type = 0
-        mats[i].color = _f3(m["color"])
-        mats[i].diffuse_reflect = float(m.get("diffuse_reflect", 1.0))
-        mats[i].emit = float(m.get("emit", 0.0))
-        mats[i].receive_shadows = 1 if m.get("receive_shadows", True) else 0
+        fill_material(mats[i], m)
     lights = (YmxLight * max(1, len(scene["lights"])))()
     for i, l in enumerate(scene["lights"]):
         lights[i].type = 0 if l["type"] == "area" else 1
@@ -78,6 +108,7 @@ def build_structs(scene):
     st.has_background = 0 if bg is None else 1
     st.background = _f3(bg if bg is not None else (0, 0, 0))
     st.seed = int(s.get("seed", 0))
+    st.raydepth = int(s.get("raydepth", 5))
     return mats, lights, st
 
 

@@ -259,3 +259,77 @@ def cornell_path_scene(n_random: int = 10_000, seed: int = 0, res: int = 256, sp
                     background=None, seed=0)
     return dict(verts=verts, tris=tris, mat_id=mat_id, materials=materials, lights=lights, camera=camera,
                 settings=settings)
+
+
+def quad_tris(c, e1, e2) -> tuple[np.ndarray, np.ndarray]:
+    """Quad c, c + e1, c + e1 + e2, c + e2 as two triangles whose face normal is
+    e1 x e2."""
+    c, e1, e2 = (np.asarray(v, dtype=F32) for v in (c, e1, e2))
+    verts = np.array([c, c + e1, c + e1 + e2, c + e2], dtype=F32)
+    return verts, np.array([[0, 1, 2], [0, 2, 3]], dtype=np.uint32)
+
+
+def box_tris(lo, hi) -> tuple[np.ndarray, np.ndarray]:
+    """Axis-aligned box as 12 triangles with outward face normals."""
+    lo = np.asarray(lo, dtype=F32)
+    s = np.asarray(hi, dtype=F32) - lo
+    x, y, z = (np.array(v, dtype=F32) for v in ((s[0], 0, 0), (0, s[1], 0), (0, 0, s[2])))
+    faces = [(lo, y, x), (lo + z, x, y), (lo, z, y), (lo + x, y, z), (lo, x, z), (lo + y, z, x)]
+    return merge([quad_tris(c, e1, e2) for c, e1, e2 in faces])
+
+
+def cornell_materials_scene(n_random: int = 10_000, seed: int = 0, res: int = 256, spp: int = 16,
+                            glossy_branches: float = 0.0, raydepth: int = 5, point_lights: int = 2):
+    """The Cornell box of cornell_path_scene with every material of the path:
+    random triangles drawn 40 % shinydiffuse / 30 % glossy (roughness r ~ U[0.05, 0.5],
+    Blinn exponent 2 / r^2 - 2) / 15 % mirror / 15 % glass (IOR 1.5); a glass
+    cube, a mirror panel before the back wall, a light_mat quad under the ceiling
+    light, point lights and a constant background of 0.1.  A fraction
+    `glossy_branches` of the glossy triangles has as_diffuse = false, so they
+    recurse through 8 glossy branches instead of being sampled as diffuse."""
+    rng = np.random.default_rng(seed)
+    wv, wt, wall = cornell_walls()
+    tv = random_triangles(rng, n_random, (0.2, 0.2, 0.2), (0.8, 0.8, 0.8))
+    # the cube floats 1/512 above the floor: coplanar faces would make exact
+    # ties in t, which the kd-tree and the BVH break differently
+    cube = box_tris((0.55, 0.3, 1.0 / 512), (0.8, 0.55, 0.25))
+    panel = quad_tris((0.1, 0.95, 0.1), (0.4, 0.0, 0.0), (0.0, 0.0, 0.6))
+    lamp = quad_tris((0.45, 0.45, 0.98), (0.0, 0.1, 0.0), (0.1, 0.0, 0.0))
+    verts, tris = merge([(wv, wt), soup(tv), cube, panel, lamp])
+    materials = [dict(color=(0.7, 0.7, 0.7)), dict(color=(0.7, 0.1, 0.1)), dict(color=(0.1, 0.7, 0.1)),
+                 dict(type="glass", IOR=1.5, filter_color=(0.9, 0.95, 1.0), transmit_filter=0.3),
+                 dict(type="mirror", color=(0.95, 0.95, 0.9), reflect=0.9),
+                 dict(type="light_mat", color=(1.0, 0.9, 0.8), power=3.0)]
+    wall_mat = np.array([0, 0, 0, 1, 2])
+    kind = rng.uniform(0.0, 1.0, n_random)
+    for k in range(n_random):
+        u = kind[k]
+        a = float(rng.uniform(0.2, 0.8))
+        if u < 0.4:
+            materials.append(dict(color=(a, a, a)))
+        elif u < 0.7:
+            r = float(rng.uniform(0.05, 0.5))
+            m = dict(type="glossy", color=(a, 0.9 * a, 0.8 * a), exponent=2.0 / (r * r) - 2.0,
+                     diffuse_reflect=float(rng.uniform(0.0, 0.6)), diffuse_color=(0.8 * a, a, a),
+                     glossy_reflect=float(rng.uniform(0.2, 1.0)))
+            if rng.uniform() < glossy_branches:
+                m["as_diffuse"] = False
+            materials.append(m)
+        elif u < 0.85:
+            materials.append(dict(type="mirror", color=(a, a, 0.9 * a), reflect=float(rng.uniform(0.5, 1.0))))
+        else:
+            materials.append(dict(type="glass", IOR=1.5))
+    mat_id = np.concatenate([wall_mat[wall], 6 + np.arange(n_random), np.full(12, 3), np.full(2, 4),
+                             np.full(2, 5)]).astype(np.uint32)
+    lights = [dict(type="area", corner=(0.4, 0.4, 0.999), point1=(0.4, 0.6, 0.999), point2=(0.6, 0.4, 0.999),
+                   color=(1.0, 1.0, 1.0), power=10.0 / np.pi, samples=4)]
+    for k in range(point_lights):
+        p = rng.uniform(0.15, 0.85, 3)
+        lights.append(dict(type="point", **{"from": (float(p[0]), float(p[1]), float(p[2]))},
+                           color=(1.0, 0.95, 0.9), power=float(np.exp(rng.uniform(np.log(0.01), np.log(0.1))))))
+    camera = dict(from_=(0.5, -1.6, 0.5), to=(0.5, 0.5, 0.5), up=(0.5, -1.6, 1.5),
+                  focal=float(1.0 / (2.0 * np.tan(np.radians(20.0)))), resx=res, resy=res)
+    settings = dict(path_samples=1, bounces=5, rr_min_bounces=2, aa_samples=spp, aa_pixelwidth=1.5,
+                    background=(0.1, 0.1, 0.1), seed=0, raydepth=raydepth)
+    return dict(verts=verts, tris=tris, mat_id=mat_id, materials=materials, lights=lights, camera=camera,
+                settings=settings)

@@ -280,8 +280,11 @@ typedef struct
 enum
 {
 	F_NONE = 0, F_SPECULAR = 1, F_GLOSSY = 2, F_DIFFUSE = 4, F_DISPERSIVE = 8, F_REFLECT = 16, F_TRANSMIT = 32,
-	F_FILTER = 64, F_EMIT = 128, F_VOLUMETRIC = 256, F_ALL = 511
-}; /* include/material/bsdf.h:25-67 */
+	F_FILTER = 64, F_EMIT = 128, F_VOLUMETRIC = 256, F_ALL = 127
+}; /* include/material/bsdf.h:25-67 (All excludes Emit and Volumetric) */
+
+/* RayDivision, include/render/render_data.h:25-31 */
+typedef struct { int division, offset; float dec1, dec2; } ray_div;
 
 typedef struct
 {
@@ -295,8 +298,22 @@ typedef struct
 static uint32_t mat_flags(const ora_material *m)
 {
 	uint32_t f = 0;
-	if(m->diffuse_reflect > 0.00001f) f |= F_DIFFUSE | F_REFLECT;  /* material_shiny_diffuse.cc:175-183 */
-	if(m->emit > 0.f) f |= F_EMIT;                                 /* :134 */
+	switch(m->type)
+	{
+		case ORA_MAT_GLOSSY:  /* GlossyMaterial::GlossyMaterial, material_glossy.cc:130-140 */
+			if(m->diffuse_reflect > 0.f) f = F_DIFFUSE | F_REFLECT;
+			f |= m->as_diffuse ? (F_DIFFUSE | F_REFLECT) : (F_GLOSSY | F_REFLECT);
+			break;
+		case ORA_MAT_MIRROR: f = F_SPECULAR; break;  /* material_mirror.cc:64-68 */
+		case ORA_MAT_GLASS:  /* AllSpecular (+ Filter), material_glass.cc:157-161 */
+			f = F_SPECULAR | F_REFLECT | F_TRANSMIT;
+			if(m->fake_shadows) f |= F_FILTER;
+			break;
+		case ORA_MAT_LIGHT: f = F_EMIT; break;  /* material_light.cc:69-73 */
+		default:
+			if(m->diffuse_reflect > 0.00001f) f |= F_DIFFUSE | F_REFLECT;  /* material_shiny_diffuse.cc:175-183 */
+			if(m->emit > 0.f) f |= F_EMIT;                                 /* :134 */
+	}
 	return f;
 }
 
@@ -352,7 +369,7 @@ static int is_shadowed(const pt_ctx *c, v3 from, v3 dir, float tmin, float tmax)
 static inline v3 face_forward(v3 ng, v3 n, v3 wo) { return vdot(ng, wo) < 0.f ? vneg(n) : n; }
 
 /* ShinyDiffuseMaterial::eval (diffuse only), material_shiny_diffuse.cc:297-340 */
-static rgb bsdf_eval(const surf *sp, v3 wo, v3 wl, uint32_t bsdfs)
+static rgb sd_eval(const surf *sp, v3 wo, v3 wl, uint32_t bsdfs)
 {
 	const float cos_ng_wo = vdot(sp->ng, wo);
 	const float cos_ng_wl = vdot(sp->ng, wl);
@@ -367,7 +384,7 @@ static rgb bsdf_eval(const surf *sp, v3 wo, v3 wl, uint32_t bsdfs)
 }
 
 /* ShinyDiffuseMaterial::pdf, material_shiny_diffuse.cc:441-483 */
-static float bsdf_pdf(const surf *sp, v3 wo, v3 wi, uint32_t bsdfs)
+static float sd_pdf(const surf *sp, v3 wo, v3 wi, uint32_t bsdfs)
 {
 	if(!(bsdfs & F_DIFFUSE)) return 0.f;
 	const v3 n = face_forward(sp->ng, sp->n, wo);
@@ -398,7 +415,7 @@ static v3 cos_hemisphere(v3 n, v3 ru, v3 rv, float s1, float s2)
 typedef struct { float s1, s2; uint32_t flags; uint32_t sampled; float pdf; } bsample;
 
 /* ShinyDiffuseMaterial::sample (diffuse component), material_shiny_diffuse.cc:349-439 */
-static rgb bsdf_sample(const surf *sp, v3 wo, v3 *wi, bsample *s, float *w)
+static rgb sd_sample(const surf *sp, v3 wo, v3 *wi, bsample *s, float *w)
 {
 	const float cos_ng_wo = vdot(sp->ng, wo);
 	const v3 n = face_forward(sp->ng, sp->n, wo);
@@ -435,10 +452,413 @@ static rgb bsdf_sample(const surf *sp, v3 wo, v3 *wi, bsample *s, float *w)
 	return scolor;
 }
 
-static rgb mat_emit(const surf *sp)
+/* ---- microfacet helpers, include/material/material_utils_microfacet.h ---- */
+
+static inline float std_max(float a, float b) { return (a < b) ? b : a; }  /* std::max */
+static inline float std_min(float a, float b) { return (b < a) ? b : a; }  /* std::min */
+
+/* pdfDivisor :26-29 and asDivisor :31-34 */
+static inline float pdf_divisor(float cos) { return 8.f * K_PI * (cos * 0.99f + 0.04f); }
+static inline float as_divisor(float cos_1, float cos_i, float cos_o)
+{
+	return 8.f * K_PI * ((cos_1 * std_max(cos_i, cos_o)) * 0.99f + 0.04f);
+}
+/* blinnD :83-86, blinnPdf :88-91, blinnSample :93-101 */
+static inline float blinn_d(float cos_h, float e) { return (e + 1.f) * ora_fast_pow(cos_h, e); }
+static inline float blinn_pdf(float costheta, float cos_w_h, float e) { return blinn_d(costheta, e) / pdf_divisor(cos_w_h); }
+static v3 blinn_sample(float s_1, float s_2, float exponent)
+{
+	const float cos_theta = ora_fast_pow(1.f - s_2, 1.f / (exponent + 1.f));
+	const float sin_theta = sqrtf(1.f - cos_theta * cos_theta);
+	const float phi = s_1 * K_2PI;
+	return V(sin_theta * ora_fast_cos(phi), sin_theta * ora_fast_sin(phi), cos_theta);
+}
+/* schlickFresnel :181-186 */
+static inline float schlick_fresnel(float costheta, float r)
+{
+	const float c_1 = (1.f - costheta);
+	const float c_2 = c_1 * c_1;
+	return r + ((1.f - r) * c_1 * c_2 * c_2);
+}
+/* diffuseReflect :188-198: the scale is evaluated in double (diffuse_ratio is a
+ * double constant) and rounded to float before multiplying the colour. */
+static rgb diffuse_reflect(float wi_n, float wo_n, float glossy, float diffuse, rgb diff_base)
+{
+	const double diffuse_ratio = 0.387507688;
+	float f_wi = 1.f - (0.5f * wi_n);
+	float temp = f_wi * f_wi;
+	f_wi = temp * temp * f_wi;
+	float f_wo = (1.f - (0.5f * wo_n));
+	temp = f_wo * f_wo;
+	f_wo = temp * temp * f_wo;
+	const float k = (float)(diffuse_ratio * diffuse * (1.f - glossy) * (1.f - f_wi) * (1.f - f_wo));
+	return cscale(diff_base, k);
+}
+
+/* Vec3f::reflect (in place, include/geometry/vector.h:357-362) and
+ * Vec3f::reflectDir (:390-396) */
+static inline v3 reflect_about(v3 v, v3 normal)
+{
+	const float vn = vdot(vmul(2.f, v), normal);
+	return V(vn * normal.x - v.x, vn * normal.y - v.y, vn * normal.z - v.z);
+}
+static inline v3 reflect_dir(v3 normal, v3 vec)
+{
+	const float vn = vdot(vec, normal);
+	if(vn < 0.f) return vneg(vec);
+	return vsub(vmul(2.f * vn, normal), vec);
+}
+
+/* ---- GlossyMaterial (isotropic Blinn, Lambertian diffuse), material_glossy.cc ---- */
+
+static inline rgb mat_rgb(const float c[3]) { return C(c[0], c[1], c[2]); }
+
+/* initBsdf :142-150: p_diffuse = min(0.6, 1 - g / (g + (1 - g) d)) */
+static float glossy_p_diffuse(const ora_material *m)
+{
+	const float m_glossy = m->glossy_reflect, m_diffuse = m->diffuse_reflect;
+	return std_min(0.6f, 1.f - (m_glossy / (m_glossy + (1.f - m_glossy) * m_diffuse)));
+}
+
+/* GlossyMaterial::eval :200-250 */
+static rgb glossy_eval(const surf *sp, v3 wo, v3 wl, uint32_t bsdfs)
 {
 	const ora_material *m = sp->mat;
-	return cscale(C(m->color[0], m->color[1], m->color[2]), m->emit);  /* emit_color_ = emit * color */
+	if(!(bsdfs & F_DIFFUSE) || ((vdot(sp->ng, wl) * vdot(sp->ng, wo)) < 0.f)) return C(0.f, 0.f, 0.f);
+	rgb col = C(0.f, 0.f, 0.f);
+	const int diffuse_flag = (bsdfs & F_DIFFUSE) != 0;
+	const v3 n = face_forward(sp->ng, sp->n, wo);
+	const float wi_n = fabsf(vdot(wl, n));
+	const float wo_n = fabsf(vdot(wo, n));
+	if((m->as_diffuse && diffuse_flag) || (!m->as_diffuse && (bsdfs & F_GLOSSY)))
+	{
+		const v3 h = vnorm(vadd(wo, wl));
+		const float cos_wi_h = std_max(0.f, vdot(wl, h));
+		const float glossy = blinn_d(vdot(h, n), m->exponent) * schlick_fresnel(cos_wi_h, m->glossy_reflect) /
+		                     as_divisor(cos_wi_h, wo_n, wi_n);
+		col = cscale(mat_rgb(m->color), glossy);
+	}
+	if(m->diffuse_reflect > 0.f && diffuse_flag)
+	{
+		const rgb add_col = cscale(mat_rgb(m->diffuse_color), m->diffuse_reflect * (1.f - m->glossy_reflect));
+		col = cadd(col, add_col);
+	}
+	return col;
+}
+
+/* GlossyMaterial::pdf :404-440 */
+static float glossy_pdf(const surf *sp, v3 wo, v3 wi, uint32_t flags)
+{
+	const ora_material *m = sp->mat;
+	if((vdot(sp->ng, wo) * vdot(sp->ng, wi)) < 0.f) return 0.f;
+	const v3 n = face_forward(sp->ng, sp->n, wo);
+	float pdf = 0.f;
+	const float cur_p_diffuse = glossy_p_diffuse(m);
+	const int use_glossy = m->as_diffuse ? (flags & F_DIFFUSE) != 0 : (flags & F_GLOSSY) != 0;
+	const int use_diffuse = m->diffuse_reflect > 0.f && (flags & F_DIFFUSE);
+	if(use_diffuse)
+	{
+		pdf = fabsf(vdot(wi, n));
+		if(use_glossy)
+		{
+			const v3 h = vnorm(vadd(wi, wo));
+			const float cos_wo_h = vdot(wo, h);
+			const float cos_n_h = vdot(n, h);
+			pdf = pdf * cur_p_diffuse + blinn_pdf(cos_n_h, cos_wo_h, m->exponent) * (1.f - cur_p_diffuse);
+		}
+		return pdf;
+	}
+	if(use_glossy)
+	{
+		const v3 h = vnorm(vadd(wi, wo));
+		const float cos_wo_h = vdot(wo, h);
+		const float cos_n_h = vdot(n, h);
+		pdf = blinn_pdf(cos_n_h, cos_wo_h, m->exponent);
+	}
+	return pdf;
+}
+
+/* GlossyMaterial::sample :252-402.  Early returns leave w as the caller had it. */
+static rgb glossy_sample(const surf *sp, v3 wo, v3 *wi, bsample *s, float *w)
+{
+	const ora_material *m = sp->mat;
+	const float cos_ng_wo = vdot(sp->ng, wo);
+	const v3 n = face_forward(sp->ng, sp->n, wo);
+	s->pdf = 0.f;
+	float wi_n = 0.f;
+	const float wo_n = fabsf(vdot(wo, n));
+	rgb scolor = C(0.f, 0.f, 0.f);
+	float s_1 = s->s1;
+	const float cur_p_diffuse = glossy_p_diffuse(m);
+	const int use_glossy = m->as_diffuse ? (s->flags & F_DIFFUSE) != 0 : (s->flags & F_GLOSSY) != 0;
+	const int use_diffuse = m->diffuse_reflect > 0.f && (s->flags & F_DIFFUSE);
+	const rgb gcolor = mat_rgb(m->color), dcolor = mat_rgb(m->diffuse_color);
+	if(use_diffuse)
+	{
+		const float s_p_diffuse = use_glossy ? cur_p_diffuse : 1.f;
+		if(s_1 < s_p_diffuse)
+		{
+			s_1 /= s_p_diffuse;
+			*wi = cos_hemisphere(n, sp->u, sp->v, s_1, s->s2);
+			const float cos_ng_wi = vdot(sp->ng, *wi);
+			if(cos_ng_wi * cos_ng_wo < 0.f) return scolor;
+			wi_n = fabsf(vdot(*wi, n));
+			s->pdf = wi_n;
+			float glossy = 0.f;
+			if(use_glossy)
+			{
+				const v3 h = vnorm(vadd(*wi, wo));
+				const float cos_wo_h = vdot(wo, h);
+				const float cos_wi_h = fabsf(vdot(*wi, h));
+				const float cos_n_h = vdot(n, h);
+				s->pdf = s->pdf * cur_p_diffuse + blinn_pdf(cos_n_h, cos_wo_h, m->exponent) * (1.f - cur_p_diffuse);
+				glossy = blinn_d(cos_n_h, m->exponent) * schlick_fresnel(cos_wi_h, m->glossy_reflect) /
+				         as_divisor(cos_wi_h, wo_n, wi_n);
+			}
+			s->sampled = F_DIFFUSE | F_REFLECT;
+			if(!(s->flags & F_REFLECT)) return C(0.f, 0.f, 0.f);
+			scolor = cscale(gcolor, glossy);
+			scolor = cadd(scolor, diffuse_reflect(wi_n, wo_n, m->glossy_reflect, m->diffuse_reflect, dcolor));
+			*w = wi_n / (s->pdf * 0.99f + 0.01f);
+			return scolor;
+		}
+		s_1 -= cur_p_diffuse;
+		s_1 /= (1.f - cur_p_diffuse);
+	}
+	if(use_glossy)
+	{
+		const v3 hs = blinn_sample(s_1, s->s2, m->exponent);
+		v3 h = vadd(vadd(vmul(hs.x, sp->u), vmul(hs.y, sp->v)), vmul(hs.z, n));
+		float cos_wo_h = vdot(wo, h);
+		if(cos_wo_h < 0.f)
+		{
+			h = reflect_about(h, n);
+			cos_wo_h = vdot(wo, h);
+		}
+		*wi = reflect_dir(h, wo);
+		const float cos_ng_wi = vdot(sp->ng, *wi);
+		if(cos_ng_wo * cos_ng_wi < 0.f) return C(0.f, 0.f, 0.f);
+		wi_n = fabsf(vdot(*wi, n));
+		const float cos_hn = vdot(h, n);
+		s->pdf = blinn_pdf(cos_hn, cos_wo_h, m->exponent);
+		const float glossy = blinn_d(cos_hn, m->exponent) * schlick_fresnel(cos_wo_h, m->glossy_reflect) /
+		                     as_divisor(cos_wo_h, wo_n, wi_n);
+		scolor = cscale(gcolor, glossy);
+		s->sampled = m->as_diffuse ? (F_DIFFUSE | F_REFLECT) : (F_GLOSSY | F_REFLECT);
+	}
+	if(use_diffuse)
+	{
+		const rgb add_col = diffuse_reflect(wi_n, wo_n, m->glossy_reflect, m->diffuse_reflect, dcolor);
+		s->pdf = wi_n * cur_p_diffuse + s->pdf * (1.f - cur_p_diffuse);
+		scolor = cadd(scolor, add_col);
+	}
+	*w = wi_n / (s->pdf * 0.99f + 0.01f);
+	return scolor;
+}
+
+/* ---- MirrorMaterial, material_mirror.cc:64-87, include/material/material_mirror.h:64 ---- */
+
+static rgb mirror_col(const ora_material *m) { return cscale(mat_rgb(m->color), std_min(m->reflect, 1.f)); }
+
+static rgb mirror_sample(const surf *sp, v3 wo, v3 *wi, bsample *s, float *w)
+{
+	*wi = reflect_dir(sp->n, wo);
+	s->sampled = F_SPECULAR | F_REFLECT;
+	*w = 1.f;
+	return cscale(mirror_col(sp->mat), 1.f / fabsf(vdot(sp->n, *wi)));
+}
+
+/* ---- GlassMaterial without dispersion, material_glass.cc:168-378 ---- */
+
+/* Vec3f::refract, include/geometry/vector.h:484-504 */
+static int refract(v3 n, v3 wi, v3 *wo, float ior)
+{
+	float eta;
+	float cos_v_n = vdot(wi, n);
+	if(cos_v_n < 0.f)
+	{
+		eta = ior;
+		n = vneg(n);
+		cos_v_n = -cos_v_n;
+	}
+	else eta = 1.f / ior;
+	const float k = 1.f - eta * eta * (1.f - cos_v_n * cos_v_n);
+	if(k <= 0.f) return 0;
+	*wo = vnorm(vadd(vmul(eta, vneg(wi)), vmul(eta * cos_v_n - sqrtf(k), n)));
+	return 1;
+}
+
+/* Vec3f::fresnel, include/geometry/vector.h:506-518 */
+static void fresnel(v3 i, v3 n, float ior, float *kr_out, float *kt_out)
+{
+	const int negative = vdot(i, n) < 0.f;
+	const float c = vdot(i, negative ? vneg(n) : n);
+	float g = ior * ior + c * c - 1.f;
+	if(g <= 0.f) g = 0.f;
+	else g = sqrtf(g);
+	const float aux = c * (g + c);
+	const float kr = ((0.5f * (g - c) * (g - c)) / ((g + c) * (g + c))) *
+	                 (1.f + ((aux - 1.f) * (aux - 1.f)) / ((aux + 1.f) * (aux + 1.f)));
+	*kr_out = kr;
+	*kt_out = (kr < 1.f) ? 1.f - kr : 0.f;
+}
+
+/* the normal bent towards wo (material_glass.cc:172-174, :307-317) */
+static v3 glass_normal(const surf *sp, v3 wo, int *outside)
+{
+	*outside = vdot(sp->ng, wo) > 0.f;
+	const float cos_wo_n = vdot(sp->n, wo);
+	const int keep = *outside ? cos_wo_n >= 0.f : cos_wo_n <= 0.f;
+	return keep ? sp->n : vnorm(vsub(sp->n, vmul(1.00001f * cos_wo_n, wo)));
+}
+
+/* filter_color_ = transmit_filter * filter_color + (1 - transmit_filter), material_glass.h:92 */
+static rgb glass_filter(const ora_material *m)
+{
+	const float tf = m->transmit_filter;
+	return C(tf * m->filter_color[0] + (1.f - tf), tf * m->filter_color[1] + (1.f - tf),
+	         tf * m->filter_color[2] + (1.f - tf));
+}
+
+static rgb glass_sample(const surf *sp, v3 wo, v3 *wi, bsample *s, float *w)
+{
+	const ora_material *m = sp->mat;
+	if(!(s->flags & F_SPECULAR))  /* no dispersion: the chromatic branch never applies */
+	{
+		s->pdf = 0.f;
+		return C(0.f, 0.f, 0.f);
+	}
+	int outside;
+	const v3 n = glass_normal(sp, wo, &outside);
+	s->pdf = 1.f;
+	const uint32_t transmit_flags = m->fake_shadows ? (F_FILTER | F_TRANSMIT) : (F_SPECULAR | F_TRANSMIT);
+	v3 refdir;
+	if(refract(n, wo, &refdir, m->ior))
+	{
+		float kr, kt;
+		fresnel(wo, n, m->ior, &kr, &kt);
+		const float p_kr = (float)(0.01 + 0.99 * kr), p_kt = (float)(0.01 + 0.99 * kt);
+		if(s->s1 < p_kt && (s->flags & transmit_flags))
+		{
+			*wi = refdir;
+			s->pdf = p_kt;
+			s->sampled = transmit_flags;
+			*w = 1.f;
+			return glass_filter(m);
+		}
+		else if(s->flags & (F_SPECULAR | F_REFLECT))
+		{
+			*wi = reflect_about(wo, n);
+			s->pdf = p_kr;
+			s->sampled = F_SPECULAR | F_REFLECT;
+			*w = 1.f;
+			return mat_rgb(m->mirror_color);
+		}
+	}
+	else if(s->flags & (F_SPECULAR | F_REFLECT))  /* total inner reflection */
+	{
+		*wi = reflect_about(wo, n);
+		s->sampled = F_SPECULAR | F_REFLECT;
+		*w = 1.f;
+		return C(1.f, 1.f, 1.f);
+	}
+	s->pdf = 0.f;
+	return C(0.f, 0.f, 0.f);
+}
+
+/* Material::getSpecular: perfect reflection / refraction directions and colours */
+typedef struct { int has_r, has_t; v3 dir_r, dir_t; rgb col_r, col_t; } specular;
+
+static specular get_specular(int ray_level, const surf *sp, v3 wo)
+{
+	specular sp_out;
+	memset(&sp_out, 0, sizeof sp_out);
+	const ora_material *m = sp->mat;
+	if(m->type == ORA_MAT_MIRROR)  /* MirrorMaterial::getSpecular, material_mirror.cc:78-86 */
+	{
+		sp_out.has_r = 1;
+		sp_out.col_r = mirror_col(m);
+		sp_out.dir_r = reflect_dir(face_forward(sp->ng, sp->n, wo), wo);
+		return sp_out;
+	}
+	if(m->type != ORA_MAT_GLASS) return sp_out;
+	/* GlassMaterial::getSpecular, material_glass.cc:303-366 */
+	int outside;
+	const v3 n = glass_normal(sp, wo, &outside);
+	v3 refdir;
+	if(refract(n, wo, &refdir, m->ior))
+	{
+		float kr, kt;
+		fresnel(wo, n, m->ior, &kr, &kt);
+		sp_out.has_t = 1;
+		sp_out.col_t = cscale(glass_filter(m), kt);
+		sp_out.dir_t = refdir;
+		if(outside || ray_level < 3)
+		{
+			sp_out.has_r = 1;
+			sp_out.dir_r = reflect_about(wo, n);
+			sp_out.col_r = cscale(mat_rgb(m->mirror_color), kr);
+		}
+	}
+	else
+	{
+		sp_out.has_r = 1;
+		sp_out.col_r = mat_rgb(m->mirror_color);
+		sp_out.dir_r = reflect_about(wo, n);
+	}
+	return sp_out;
+}
+
+/* ---- dispatch over the material types ---- */
+
+static rgb bsdf_eval(const surf *sp, v3 wo, v3 wl, uint32_t bsdfs)
+{
+	switch(sp->mat->type)
+	{
+		case ORA_MAT_SHINYDIFFUSE: return sd_eval(sp, wo, wl, bsdfs);
+		case ORA_MAT_GLOSSY: return glossy_eval(sp, wo, wl, bsdfs);
+		default: return C(0.f, 0.f, 0.f);  /* mirror, glass, light: eval is 0 */
+	}
+}
+
+static float bsdf_pdf(const surf *sp, v3 wo, v3 wi, uint32_t bsdfs)
+{
+	switch(sp->mat->type)
+	{
+		case ORA_MAT_SHINYDIFFUSE: return sd_pdf(sp, wo, wi, bsdfs);
+		case ORA_MAT_GLOSSY: return glossy_pdf(sp, wo, wi, bsdfs);
+		default: return 0.f;  /* Material::pdf default */
+	}
+}
+
+static rgb bsdf_sample(const surf *sp, v3 wo, v3 *wi, bsample *s, float *w)
+{
+	switch(sp->mat->type)
+	{
+		case ORA_MAT_SHINYDIFFUSE: return sd_sample(sp, wo, wi, s, w);
+		case ORA_MAT_GLOSSY: return glossy_sample(sp, wo, wi, s, w);
+		case ORA_MAT_MIRROR: return mirror_sample(sp, wo, wi, s, w);
+		case ORA_MAT_GLASS: return glass_sample(sp, wo, wi, s, w);
+		default:  /* LightMaterial::sample, material_light.cc:75-80 */
+			s->pdf = 0.f;
+			*w = 0.f;
+			return C(0.f, 0.f, 0.f);
+	}
+}
+
+/* Material::emit: shinydiffuse emit_color_ = emit * color (material_shiny_diffuse.cc:342-347);
+ * LightMaterial::emit, material_light.cc:82-87 (light_col_ = color * power) */
+static rgb mat_emit(const surf *sp, v3 wo)
+{
+	const ora_material *m = sp->mat;
+	if(m->type == ORA_MAT_LIGHT)
+	{
+		const rgb lc = cscale(mat_rgb(m->color), m->power);
+		if(m->double_sided) return lc;
+		return vdot(wo, sp->n) > 0.f ? lc : C(0.f, 0.f, 0.f);
+	}
+	if(m->type == ORA_MAT_SHINYDIFFUSE) return cscale(mat_rgb(m->color), m->emit);
+	return C(0.f, 0.f, 0.f);
 }
 
 /* ------------------------------------------------------------------ */
@@ -528,7 +948,7 @@ static int area_intersect(const ora_light *l, v3 from, v3 dir, float *t, float *
 /* MonteCarloIntegrator::doLightEstimation and the area / dirac paths,
  * src/integrator/surface/integrator_montecarlo.cc:118-440. */
 static rgb light_estimation(const pt_ctx *c, const ora_light *l, const surf *sp, v3 wo, uint32_t loffs,
-                            uint32_t psd_sample, uint32_t psd_offset)
+                            const ray_div *rd, uint32_t psd_sample, uint32_t psd_offset)
 {
 	rgb col = C(0.f, 0.f, 0.f);
 	const int cast_shadows = l->cast_shadows && sp->mat->receive_shadows;
@@ -555,7 +975,8 @@ static rgb light_estimation(const pt_ctx *c, const ora_light *l, const surf *sp,
 		return col;
 	}
 	const uint32_t l_offs = loffs * 4567u;
-	const int num_samples = (int)ceilf((float)l->samples * 1.f);
+	int num_samples = (int)ceilf((float)l->samples * 1.f);
+	if(rd->division > 1) num_samples = num_samples / rd->division > 1 ? num_samples / rd->division : 1;
 	const float inv_num_samples = 1.f / (float)num_samples;
 	const uint32_t offs = (uint32_t)num_samples * psd_sample + psd_offset + l_offs;
 	const rgb lcolor = light_color(l);
@@ -634,65 +1055,153 @@ static rgb light_estimation(const pt_ctx *c, const ora_light *l, const surf *sp,
 	return col;
 }
 
+/* per camera sample: the PixelSamplingData fields the integrator reads */
+typedef struct { uint32_t pixel, sample, offset; } psd_t;
+
 /* estimateAllDirectLight, integrator_montecarlo.cc:92-106 */
-static rgb direct_all(const pt_ctx *c, const surf *sp, v3 wo, uint32_t psd_sample, uint32_t psd_offset)
+static rgb direct_all(const pt_ctx *c, const surf *sp, v3 wo, const ray_div *rd, const psd_t *psd)
 {
 	rgb col = C(0.f, 0.f, 0.f);
 	for(uint32_t l = 0; l < c->n_lights; ++l)
-		col = cadd(col, light_estimation(c, &c->lights[l], sp, wo, l, psd_sample, psd_offset));
+		col = cadd(col, light_estimation(c, &c->lights[l], sp, wo, l, rd, psd->sample, psd->offset));
 	return col;
 }
 
 /* estimateOneDirectLight, integrator_montecarlo.cc:108-116, with the
  * deterministic light-pick index of ora_pt.h. */
-static rgb direct_one(const pt_ctx *c, const surf *sp, v3 wo, uint32_t pixel, uint32_t psd_sample,
-                      uint32_t psd_offset, uint32_t sub, uint32_t depth)
+static rgb direct_one(const pt_ctx *c, const surf *sp, v3 wo, const ray_div *rd, const psd_t *psd, uint32_t sub,
+                      uint32_t depth, uint32_t tag)
 {
 	const int num_lights = (int)c->n_lights;
 	if(num_lights == 0) return C(0.f, 0.f, 0.f);
 	const uint32_t bounces = (uint32_t)c->st->bounces;
-	const uint32_t corr = (pixel * c->total_samples + psd_sample) * ((uint32_t)c->st->path_samples * bounces) +
-	                      sub * bounces + depth + 1u;
+	const uint32_t corr = (psd->pixel * c->total_samples + psd->sample) * ((uint32_t)c->st->path_samples * bounces) +
+	                      sub * bounces + depth + 1u + tag * 0x9E3779B9u;
 	const float h = ora_halton_next(2, corr - 1u);
 	int lnum = (int)(h * (float)num_lights);
 	if(lnum > num_lights - 1) lnum = num_lights - 1;
-	return cscale(light_estimation(c, &c->lights[lnum], sp, wo, (uint32_t)lnum, psd_sample, psd_offset),
+	return cscale(light_estimation(c, &c->lights[lnum], sp, wo, (uint32_t)lnum, rd, psd->sample, psd->offset),
 	              (float)num_lights);
 }
 
-/* PathIntegrator::integrate, src/integrator/surface/integrator_path_tracer.cc:138-314 */
-static void integrate(const pt_ctx *c, v3 from, v3 dir, uint32_t pixel, uint32_t psd_sample, uint32_t psd_offset,
-                      float out[4])
+/* math::addMod1, include/math/math.h:304-309 */
+static inline float add_mod1(float a, float b)
+{
+	const float s = a + b;
+	return s > 1 ? s - 1 : s;
+}
+
+static void integrate(const pt_ctx *c, v3 from, v3 dir, float tmin, int ray_level, ray_div rd, const psd_t *psd,
+                      uint32_t tag, rgb *out);
+
+/* MonteCarloIntegrator::glossy with glossyReflectNoTransmit (the glossy
+ * material has no transmission), integrator_montecarlo.cc:506-589,623-645 */
+static rgb glossy_branches(const pt_ctx *c, int ray_level, const surf *sp, v3 wo, const ray_div *rd,
+                           const psd_t *psd, uint32_t tag)
+{
+	if(!(sp->flags & F_REFLECT)) return C(0.f, 0.f, 0.f);
+	const int ray_samples_glossy = rd->division > 1 ? (8 / rd->division > 1 ? 8 / rd->division : 1) : 8;
+	ray_div nrd = *rd;
+	nrd.division *= ray_samples_glossy;
+	int branch = nrd.division * rd->offset;
+	uint32_t offs = (uint32_t)ray_samples_glossy * psd->sample + psd->offset;
+	const float inverse_ray_samples_glossy = 1.f / (float)ray_samples_glossy;
+	rgb gcol = C(0.f, 0.f, 0.f);
+	halton h2, h3;
+	halton_start(&h2, 2, offs);
+	halton_start(&h3, 3, offs);
+	for(int ns = 0; ns < ray_samples_glossy; ++ns)
+	{
+		nrd.dec1 = (float)ora_low_discrepancy(2 * ray_level + 1, (uint32_t)branch + psd->offset);
+		nrd.dec2 = (float)ora_low_discrepancy(2 * ray_level + 2, (uint32_t)branch + psd->offset);
+		nrd.offset = branch;
+		++offs;
+		++branch;
+		const float s_1 = halton_next(&h2);
+		const float s_2 = halton_next(&h3);
+		/* glossyReflectNoTransmit */
+		float w = 0.f;
+		bsample s = {s_1, s_2, F_GLOSSY | F_REFLECT, F_NONE, 0.f};
+		v3 wi = V(0.f, 0.f, 0.f);
+		const rgb mcol = bsdf_sample(sp, wo, &wi, &s, &w);
+		rgb integ;
+		integrate(c, sp->p, wi, 0.00005f, ray_level, nrd, psd, tag * 11u + 1u + (uint32_t)ns, &integ);
+		integ = cmul(integ, cscale(mcol, w));
+		gcol = cadd(gcol, integ);
+	}
+	return cscale(gcol, inverse_ray_samples_glossy);
+}
+
+/* MonteCarloIntegrator::recursiveRaytrace with specularReflect / specularRefract,
+ * integrator_montecarlo.cc:647-748 (no dispersion, no volumes, additional depth 0) */
+static rgb recursive_raytrace(const pt_ctx *c, int ray_level, uint32_t bsdfs, const surf *sp, v3 wo,
+                              const ray_div *rd, const psd_t *psd, uint32_t tag)
+{
+	rgb col = C(0.f, 0.f, 0.f);
+	if(ray_level <= c->st->raydepth && ray_level < 20 && (bsdfs & (F_GLOSSY | F_SPECULAR | F_FILTER)))
+	{
+		if(bsdfs & F_GLOSSY) col = cadd(col, glossy_branches(c, ray_level, sp, wo, rd, psd, tag));
+		if(bsdfs & (F_SPECULAR | F_FILTER))
+		{
+			const specular spec = get_specular(ray_level, sp, wo);
+			if(spec.has_r)
+			{
+				rgb integ;
+				integrate(c, sp->p, spec.dir_r, 0.00005f, ray_level, *rd, psd, tag * 11u + 9u, &integ);
+				col = cadd(col, cmul(integ, spec.col_r));
+			}
+			if(spec.has_t)
+			{
+				rgb integ;
+				integrate(c, sp->p, spec.dir_t, 0.00005f, ray_level, *rd, psd, tag * 11u + 10u, &integ);
+				col = cadd(col, cmul(integ, spec.col_t));
+			}
+		}
+	}
+	return col;
+}
+
+/* PathIntegrator::integrate, src/integrator/surface/integrator_path_tracer.cc:138-314.
+ * The alpha it returns is always 1 here (no transparent background). */
+static void integrate(const pt_ctx *c, v3 from, v3 dir, float tmin, int ray_level, ray_div rd, const psd_t *psd,
+                      uint32_t tag, rgb *out)
 {
 	const ora_pt_settings *st = c->st;
 	rgb col = C(0.f, 0.f, 0.f);
-	float alpha = 1.f;
 	float w = 0.f;
 	surf sp;
-	if(!intersect_surface(c, from, dir, 0.f, -1.f, &sp))
+	if(!intersect_surface(c, from, dir, tmin, -1.f, &sp))
 	{
+		/* TiledIntegrator::background, integrator_tiled.cc:693-706 */
 		if(st->has_background) col = C(st->background[0], st->background[1], st->background[2]);
-		out[0] = col.r; out[1] = col.g; out[2] = col.b; out[3] = alpha;
+		*out = col;
 		return;
 	}
 	const uint32_t mat_bsdfs = sp.flags;
 	const v3 wo = vneg(dir);
-	if(mat_bsdfs & F_EMIT) col = cadd(col, mat_emit(&sp));
-	if(mat_bsdfs & F_DIFFUSE) col = cadd(col, direct_all(c, &sp, wo, psd_sample, psd_offset));
+	if(mat_bsdfs & F_EMIT) col = cadd(col, mat_emit(&sp, wo));
+	if(mat_bsdfs & F_DIFFUSE) col = cadd(col, direct_all(c, &sp, wo, &rd, psd));
 	uint32_t path_flags = F_DIFFUSE;
 	if(mat_bsdfs & path_flags)
 	{
 		rgb path_col = C(0.f, 0.f, 0.f);
 		path_flags |= F_DIFFUSE | F_REFLECT | F_TRANSMIT;
-		const int n_samples = st->path_samples > 1 ? st->path_samples : 1;
+		const int n_samples = st->path_samples / rd.division > 1 ? st->path_samples / rd.division : 1;
 		for(int i = 0; i < n_samples; ++i)
 		{
-			const uint32_t offs = (uint32_t)st->path_samples * psd_sample + psd_offset + (uint32_t)i;
+			const uint32_t offs = (uint32_t)st->path_samples * psd->sample + psd->offset + (uint32_t)i;
 			rgb throughput;
 			v3 pwo = wo;
 			bsample s;
-			s.s1 = ora_ri_vdc(offs, 0);
-			s.s2 = (float)ora_low_discrepancy(2, offs);
+			float s_1 = ora_ri_vdc(offs, 0);
+			float s_2 = (float)ora_low_discrepancy(2, offs);
+			if(rd.division > 1)
+			{
+				s_1 = add_mod1(s_1, rd.dec1);
+				s_2 = add_mod1(s_2, rd.dec2);
+			}
+			s.s1 = s_1;
+			s.s2 = s_2;
 			s.flags = path_flags;
 			s.pdf = 0.f;
 			s.sampled = F_NONE;
@@ -703,9 +1212,9 @@ static void integrate(const pt_ctx *c, v3 from, v3 dir, uint32_t pixel, uint32_t
 			surf hit;
 			if(!intersect_surface(c, sp.p, pdir, 0.00005f, -1.f, &hit)) continue;
 			if(s.sampled != F_NONE) pwo = vneg(pdir);
-			rgb lcol = direct_one(c, &hit, pwo, pixel, psd_sample, psd_offset, (uint32_t)i, 0u);
+			rgb lcol = direct_one(c, &hit, pwo, &rd, psd, (uint32_t)i, 0u, tag);
 			const uint32_t mat_bsd_fs = hit.flags;
-			if(mat_bsd_fs & F_EMIT) lcol = cadd(lcol, mat_emit(&hit));
+			if(mat_bsd_fs & F_EMIT) lcol = cadd(lcol, mat_emit(&hit, pwo));
 			path_col = cadd(path_col, cmul(lcol, throughput));
 			int caustic = 0;
 			for(int depth = 1; depth < st->bounces; ++depth)
@@ -723,23 +1232,26 @@ static void integrate(const pt_ctx *c, v3 from, v3 dir, uint32_t pixel, uint32_t
 				if(!intersect_surface(c, hit.p, pdir, 0.00005f, -1.f, &nh)) break;
 				hit = nh;
 				pwo = vneg(pdir);
-				if(mat_bsd_fs & F_DIFFUSE) lcol = direct_one(c, &hit, pwo, pixel, psd_sample, psd_offset, (uint32_t)i, (uint32_t)depth);
+				if(mat_bsd_fs & F_DIFFUSE)
+					lcol = direct_one(c, &hit, pwo, &rd, psd, (uint32_t)i, (uint32_t)depth, tag);
 				else lcol = C(0.f, 0.f, 0.f);
 				if(depth > st->rr_min_bounces)
 				{
-					const float random_value = ora_rr_random(pixel, psd_sample * (uint32_t)st->path_samples + (uint32_t)i, (uint32_t)depth, st->seed);
+					const float random_value =
+						ora_rr_random(psd->pixel, psd->sample * (uint32_t)st->path_samples + (uint32_t)i,
+						              (uint32_t)depth + 256u * tag, st->seed);
 					const float probability = cmax(throughput);
 					if(probability <= 0.f || probability < random_value) break;
 					throughput = cscale(throughput, 1.f / probability);
 				}
-				if((mat_bsd_fs & F_EMIT) && caustic) lcol = cadd(lcol, mat_emit(&hit));
+				if((mat_bsd_fs & F_EMIT) && caustic) lcol = cadd(lcol, mat_emit(&hit, pwo));
 				path_col = cadd(path_col, cmul(lcol, throughput));
 			}
 		}
 		col = cadd(col, cdiv(path_col, (float)n_samples));
 	}
-	/* recursiveRaytrace: no glossy / specular / filter components in this material set */
-	out[0] = col.r; out[1] = col.g; out[2] = col.b; out[3] = alpha;
+	col = cadd(col, recursive_raytrace(c, ray_level + 1, mat_bsdfs, &sp, wo, &rd, psd, tag));
+	*out = col;
 }
 
 static void setup_ctx(pt_ctx *c, const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx,
@@ -786,7 +1298,14 @@ static void pixel_sample(const pt_ctx *c, int x, int y, int sample, float col[4]
 	const float px = (float)x + dx, py = (float)y + dy;
 	v3 dir = vadd(vadd(vmul(px, c->vright), vmul(py, c->vup)), c->vto);
 	dir = vnorm(dir);
-	integrate(c, c->cam_from, dir, pixel, psd_sample, offset, col);
+	const psd_t psd = {pixel, psd_sample, offset};
+	const ray_div rd = {1, 0, 0.f, 0.f};
+	rgb out;
+	integrate(c, c->cam_from, dir, 0.f, 0, rd, &psd, 0u, &out);
+	col[0] = out.r;
+	col[1] = out.g;
+	col[2] = out.b;
+	col[3] = 1.f;
 	*dx_out = dx;
 	*dy_out = dy;
 }
@@ -905,4 +1424,82 @@ void ora_pt_render(const ora_kdtree *kd, const float *verts, const uint32_t *tri
 	}
 	free(cols);
 	free(dxy);
+}
+
+/* ------------------------------------------------------------------ */
+/* single-function access for the oracle's own pinning tests            */
+
+static surf flat_surf(const ora_material *m, const float ng[3])
+{
+	surf sp;
+	memset(&sp, 0, sizeof sp);
+	sp.ng = vload(ng);
+	sp.n = sp.ng;
+	coords_system(sp.n, &sp.u, &sp.v);
+	sp.mat = m;
+	sp.flags = mat_flags(m);
+	return sp;
+}
+
+uint32_t ora_mat_flags(const ora_material *m) { return mat_flags(m); }
+
+void ora_bsdf_eval(const ora_material *m, const float ng[3], const float wo[3], const float wl[3], uint32_t bsdfs,
+                   float out[3])
+{
+	const surf sp = flat_surf(m, ng);
+	const rgb c = bsdf_eval(&sp, vload(wo), vload(wl), bsdfs);
+	out[0] = c.r; out[1] = c.g; out[2] = c.b;
+}
+
+float ora_bsdf_pdf(const ora_material *m, const float ng[3], const float wo[3], const float wi[3], uint32_t bsdfs)
+{
+	const surf sp = flat_surf(m, ng);
+	return bsdf_pdf(&sp, vload(wo), vload(wi), bsdfs);
+}
+
+/* out: wi[3], colour[3], w, pdf, sampled flags (as float) */
+void ora_bsdf_sample(const ora_material *m, const float ng[3], const float wo[3], float s1, float s2, uint32_t flags,
+                     float out[9])
+{
+	const surf sp = flat_surf(m, ng);
+	bsample s = {s1, s2, flags, F_NONE, 0.f};
+	v3 wi = V(0.f, 0.f, 0.f);
+	float w = 0.f;
+	const rgb c = bsdf_sample(&sp, vload(wo), &wi, &s, &w);
+	out[0] = wi.x; out[1] = wi.y; out[2] = wi.z;
+	out[3] = c.r; out[4] = c.g; out[5] = c.b;
+	out[6] = w; out[7] = s.pdf; out[8] = (float)s.sampled;
+}
+
+/* out: has_r, dir_r[3], col_r[3], has_t, dir_t[3], col_t[3] */
+void ora_specular(const ora_material *m, const float ng[3], const float wo[3], int ray_level, float out[14])
+{
+	const surf sp = flat_surf(m, ng);
+	const specular r = get_specular(ray_level, &sp, vload(wo));
+	out[0] = (float)r.has_r;
+	out[1] = r.dir_r.x; out[2] = r.dir_r.y; out[3] = r.dir_r.z;
+	out[4] = r.col_r.r; out[5] = r.col_r.g; out[6] = r.col_r.b;
+	out[7] = (float)r.has_t;
+	out[8] = r.dir_t.x; out[9] = r.dir_t.y; out[10] = r.dir_t.z;
+	out[11] = r.col_t.r; out[12] = r.col_t.g; out[13] = r.col_t.b;
+}
+
+void ora_fresnel(const float i[3], const float n[3], float ior, float out[2])
+{
+	fresnel(vload(i), vload(n), ior, &out[0], &out[1]);
+}
+
+int ora_refract(const float n[3], const float wi[3], float ior, float wo[3])
+{
+	v3 r = V(0.f, 0.f, 0.f);
+	const int ok = refract(vload(n), vload(wi), &r, ior);
+	wo[0] = r.x; wo[1] = r.y; wo[2] = r.z;
+	return ok;
+}
+
+void ora_mat_emit(const ora_material *m, const float ng[3], const float wo[3], float out[3])
+{
+	const surf sp = flat_surf(m, ng);
+	const rgb c = mat_emit(&sp, vload(wo));
+	out[0] = c.r; out[1] = c.g; out[2] = c.b;
 }

@@ -8,6 +8,12 @@
  * (src/integrator/surface/integrator_path_tracer.cc:138-314), the Monte-Carlo
  * direct lighting (src/integrator/surface/integrator_montecarlo.cc:92-440),
  * ShinyDiffuseMaterial (src/material/material_shiny_diffuse.cc:141-485),
+ * GlossyMaterial without anisotropy or Oren-Nayar (src/material/material_glossy.cc:130-430,
+ * include/material/material_utils_microfacet.h), MirrorMaterial
+ * (src/material/material_mirror.cc:64-87), GlassMaterial without dispersion or
+ * absorption (src/material/material_glass.cc:157-378), LightMaterial
+ * (src/material/material_light.cc:69-92), the recursive raytrace with its
+ * glossy ray division (src/integrator/surface/integrator_montecarlo.cc:506-748),
  * AreaLight / PointLight (src/light/light_area.cc:99-163,
  * src/light/light_point.cc:70-97), PerspectiveCamera::shootRay
  * (src/camera/camera_perspective.cc:97-111,165-180), the samplers
@@ -21,9 +27,13 @@
  *     of the tile-seeded multiply-with-carry generator seeded by rand()
  *     (integrator_tiled.cc:256);
  *   - the one-light pick of estimateOneDirectLight uses the Halton base-2
- *     index c = (pixel * total_samples + sample) * bounces + k + 1 (k: the
- *     path vertex) instead of the per-thread correlative counter
- *     (integrator_montecarlo.cc:110-114).
+ *     index c = (pixel * total_samples + sample) * (path_samples * bounces)
+ *     + sub * bounces + k + 1 (k: the path vertex) instead of the per-thread
+ *     correlative counter (integrator_montecarlo.cc:110-114).
+ * Both also take the integrate() call's node tag: 0 for the camera ray, and
+ * tag * 11 + j for its recursive children (j = 1..8 the glossy branches,
+ * 9 the specular reflection, 10 the refraction).  The roulette hashes
+ * depth + 256 * tag; the light pick adds tag * 0x9E3779B9 to c (mod 2^32).
  * Film samples are splatted sample pass by sample pass, pixels in row-major
  * order (the reference's order depends on its thread schedule).
  */
@@ -39,14 +49,32 @@ extern "C" {
 #endif
 
 #define ORA_MAT_SHINYDIFFUSE 0u
+#define ORA_MAT_GLOSSY 1u
+#define ORA_MAT_MIRROR 2u
+#define ORA_MAT_GLASS 3u
+#define ORA_MAT_LIGHT 4u
 
+/* One record for every material type; each type reads its own fields (the
+ * reference's param names and defaults in the comments). */
 typedef struct
 {
-	uint32_t type;          /* ORA_MAT_* */
-	float color[3];         /* "color" */
-	float diffuse_reflect;  /* "diffuse_reflect" (default 1) */
-	float emit;             /* "emit" (default 0) */
-	uint32_t receive_shadows;
+	uint32_t type;            /* ORA_MAT_* */
+	float color[3];           /* shinydiffuse "color" (1), glossy "color" (1), mirror "color" (1), light "color" (1) */
+	float diffuse_reflect;    /* shinydiffuse (1), glossy (0) "diffuse_reflect" */
+	float emit;               /* shinydiffuse "emit" (0) */
+	uint32_t receive_shadows; /* "receive_shadows" (true) */
+	float diffuse_color[3];   /* glossy "diffuse_color" (1) */
+	float glossy_reflect;     /* glossy "glossy_reflect" (1) */
+	float exponent;           /* glossy "exponent" (50) */
+	uint32_t as_diffuse;      /* glossy "as_diffuse" (true) */
+	float reflect;            /* mirror "reflect" (1) */
+	float ior;                /* glass "IOR" (1.4) */
+	float filter_color[3];    /* glass "filter_color" (1) */
+	float transmit_filter;    /* glass "transmit_filter" (0) */
+	float mirror_color[3];    /* glass "mirror_color" (1) */
+	uint32_t fake_shadows;    /* glass "fake_shadows" (false) */
+	float power;              /* light "power" (1) */
+	uint32_t double_sided;    /* light "double_sided" (false) */
 } ora_material;
 
 #define ORA_LIGHT_AREA 0u
@@ -77,6 +105,7 @@ typedef struct
 	int32_t has_background;
 	float background[3];
 	uint32_t seed;
+	int32_t raydepth;        /* MonteCarloIntegrator "raydepth" (5) */
 } ora_pt_settings;
 
 /* Render aa_samples samples per pixel into film sums: film_rgba[4 * (y * resx + x)]
@@ -106,6 +135,21 @@ int ora_faure_perm(int base, int digit);
 float ora_rr_random(uint32_t pixel, uint32_t sample, uint32_t depth, uint32_t seed);
 /* Gauss filter table of ImageFilm (16 x 16 floats) and its scale. */
 void ora_filter_table(float pixelwidth, float table[256], float *table_scale, float *filter_width);
+
+/* Single material functions on a flat surface with normal ng (n = ng), for the
+ * oracle's own tests: eval, pdf, sample (out: wi[3], colour[3], w, pdf,
+ * sampled flags), getSpecular (out: has_r, dir_r[3], col_r[3], has_t,
+ * dir_t[3], col_t[3]), emit; Fresnel (out: kr, kt) and refraction. */
+uint32_t ora_mat_flags(const ora_material *m);
+void ora_bsdf_eval(const ora_material *m, const float ng[3], const float wo[3], const float wl[3], uint32_t bsdfs,
+                   float out[3]);
+float ora_bsdf_pdf(const ora_material *m, const float ng[3], const float wo[3], const float wi[3], uint32_t bsdfs);
+void ora_bsdf_sample(const ora_material *m, const float ng[3], const float wo[3], float s1, float s2, uint32_t flags,
+                     float out[9]);
+void ora_specular(const ora_material *m, const float ng[3], const float wo[3], int ray_level, float out[14]);
+void ora_fresnel(const float i[3], const float n[3], float ior, float out[2]);
+int ora_refract(const float n[3], const float wi[3], float ior, float wo[3]);
+void ora_mat_emit(const ora_material *m, const float ng[3], const float wo[3], float out[3]);
 
 #ifdef __cplusplus
 }

@@ -176,7 +176,37 @@ def fast_pow(a, b) -> float:
 # ---------------------------------------------------------------- path tracer
 class OraMaterial(C.Structure):
     _fields_ = [("type", C.c_uint32), ("color", C.c_float * 3), ("diffuse_reflect", C.c_float),
-                ("emit", C.c_float), ("receive_shadows", C.c_uint32)]
+                ("emit", C.c_float), ("receive_shadows", C.c_uint32), ("diffuse_color", C.c_float * 3),
+                ("glossy_reflect", C.c_float), ("exponent", C.c_float), ("as_diffuse", C.c_uint32),
+                ("reflect", C.c_float), ("ior", C.c_float), ("filter_color", C.c_float * 3),
+                ("transmit_filter", C.c_float), ("mirror_color", C.c_float * 3), ("fake_shadows", C.c_uint32),
+                ("power", C.c_float), ("double_sided", C.c_uint32)]
+
+
+MAT_TYPES = {"shinydiffuse": 0, "shinydiffusemat": 0, "glossy": 1, "mirror": 2, "glass": 3, "light": 4,
+             "light_mat": 4}
+
+
+def fill_material(dst, m):
+    """Material dict (the reference's param names, its defaults) -> record."""
+    kind = m.get("type", "shinydiffuse")
+    dst.type = MAT_TYPES[kind]
+    dst.color = _f3v(m.get("color", (1, 1, 1)))
+    dst.diffuse_reflect = float(m.get("diffuse_reflect", 0.0 if kind == "glossy" else 1.0))
+    dst.emit = float(m.get("emit", 0.0))
+    dst.receive_shadows = int(bool(m.get("receive_shadows", True)))
+    dst.diffuse_color = _f3v(m.get("diffuse_color", (1, 1, 1)))
+    dst.glossy_reflect = float(m.get("glossy_reflect", 1.0))
+    dst.exponent = float(m.get("exponent", 50.0))
+    dst.as_diffuse = int(bool(m.get("as_diffuse", True)))
+    dst.reflect = float(m.get("reflect", 1.0))
+    dst.ior = float(m.get("IOR", 1.4))
+    dst.filter_color = _f3v(m.get("filter_color", (1, 1, 1)))
+    dst.transmit_filter = float(m.get("transmit_filter", 0.0))
+    dst.mirror_color = _f3v(m.get("mirror_color", (1, 1, 1)))
+    dst.fake_shadows = int(bool(m.get("fake_shadows", False)))
+    dst.power = float(m.get("power", 1.0))
+    dst.double_sided = int(bool(m.get("double_sided", False)))
 
 
 class OraLight(C.Structure):
@@ -190,7 +220,7 @@ class OraPtSettings(C.Structure):
                 ("aspect_factor", C.c_float), ("resx", C.c_int32), ("resy", C.c_int32),
                 ("path_samples", C.c_int32), ("bounces", C.c_int32), ("rr_min_bounces", C.c_int32),
                 ("aa_samples", C.c_int32), ("aa_pixelwidth", C.c_float), ("has_background", C.c_int32),
-                ("background", C.c_float * 3), ("seed", C.c_uint32)]
+                ("background", C.c_float * 3), ("seed", C.c_uint32), ("raydepth", C.c_int32)]
 
 
 def _f3v(v):
@@ -200,11 +230,7 @@ def _f3v(v):
 def pt_structs(scene):
     mats = (OraMaterial * len(scene["materials"]))()
     for i, m in enumerate(scene["materials"]):
-        mats[i].type = 0
-        mats[i].color = _f3v(m["color"])
-        mats[i].diffuse_reflect = float(m.get("diffuse_reflect", 1.0))
-        mats[i].emit = float(m.get("emit", 0.0))
-        mats[i].receive_shadows = 1
+        fill_material(mats[i], m)
     lights = (OraLight * max(1, len(scene["lights"])))()
     for i, l in enumerate(scene["lights"]):
         lights[i].type = 0 if l["type"] == "area" else 1
@@ -230,6 +256,7 @@ def pt_structs(scene):
     st.has_background = 0 if bg is None else 1
     st.background = _f3v(bg if bg is not None else (0, 0, 0))
     st.seed = int(s.get("seed", 0))
+    st.raydepth = int(s.get("raydepth", 5))
     return mats, lights, st
 
 
@@ -309,3 +336,93 @@ def filter_table(pixelwidth=1.5):
     scale, width = C.c_float(), C.c_float()
     L.ora_filter_table(C.c_float(pixelwidth), t.ctypes.data, C.byref(scale), C.byref(width))
     return t.reshape(16, 16), scale.value, width.value
+
+
+# ---------------------------------------------------------------- single material functions
+F_SPECULAR, F_GLOSSY, F_DIFFUSE, F_DISPERSIVE, F_REFLECT, F_TRANSMIT, F_FILTER, F_EMIT = 1, 2, 4, 8, 16, 32, 64, 128
+F_ALL = 127
+
+
+def _mat_lib():
+    L = _pt_lib()
+    if not getattr(L, "_mat_ready", False):
+        vp = C.c_void_p
+        L.ora_mat_flags.argtypes = [vp]
+        L.ora_mat_flags.restype = C.c_uint32
+        L.ora_bsdf_eval.argtypes = [vp, vp, vp, vp, C.c_uint32, vp]
+        L.ora_bsdf_pdf.argtypes = [vp, vp, vp, vp, C.c_uint32]
+        L.ora_bsdf_pdf.restype = C.c_float
+        L.ora_bsdf_sample.argtypes = [vp, vp, vp, C.c_float, C.c_float, C.c_uint32, vp]
+        L.ora_specular.argtypes = [vp, vp, vp, C.c_int, vp]
+        L.ora_fresnel.argtypes = [vp, vp, C.c_float, vp]
+        L.ora_refract.argtypes = [vp, vp, C.c_float, vp]
+        L.ora_refract.restype = C.c_int
+        L.ora_mat_emit.argtypes = [vp, vp, vp, vp]
+        L._mat_ready = True
+    return L
+
+
+def _v(x):
+    return np.ascontiguousarray(x, dtype=np.float32)
+
+
+class Material:
+    """One material record, for single-function checks of the oracle."""
+
+    def __init__(self, **params):
+        self.rec = OraMaterial()
+        fill_material(self.rec, params)
+        self.L = _mat_lib()
+
+    @property
+    def flags(self):
+        return self.L.ora_mat_flags(C.byref(self.rec))
+
+    def eval(self, ng, wo, wl, flags=F_ALL):
+        out = np.zeros(3, np.float32)
+        ng, wo, wl = _v(ng), _v(wo), _v(wl)
+        self.L.ora_bsdf_eval(C.byref(self.rec), ng.ctypes.data, wo.ctypes.data, wl.ctypes.data, flags,
+                             out.ctypes.data)
+        return out
+
+    def pdf(self, ng, wo, wi, flags=F_GLOSSY | F_DIFFUSE | F_DISPERSIVE | F_REFLECT | F_TRANSMIT):
+        ng, wo, wi = _v(ng), _v(wo), _v(wi)
+        return self.L.ora_bsdf_pdf(C.byref(self.rec), ng.ctypes.data, wo.ctypes.data, wi.ctypes.data, flags)
+
+    def sample(self, ng, wo, s1, s2, flags=F_ALL):
+        out = np.zeros(9, np.float32)
+        ng, wo = _v(ng), _v(wo)
+        self.L.ora_bsdf_sample(C.byref(self.rec), ng.ctypes.data, wo.ctypes.data, float(s1), float(s2), flags,
+                               out.ctypes.data)
+        return dict(wi=out[0:3].copy(), col=out[3:6].copy(), w=float(out[6]), pdf=float(out[7]),
+                    sampled=int(out[8]))
+
+    def specular(self, ng, wo, ray_level=1):
+        out = np.zeros(14, np.float32)
+        ng, wo = _v(ng), _v(wo)
+        self.L.ora_specular(C.byref(self.rec), ng.ctypes.data, wo.ctypes.data, int(ray_level), out.ctypes.data)
+        r = dict(dir=out[1:4].copy(), col=out[4:7].copy()) if out[0] else None
+        t = dict(dir=out[8:11].copy(), col=out[11:14].copy()) if out[7] else None
+        return r, t
+
+    def emit(self, ng, wo):
+        out = np.zeros(3, np.float32)
+        ng, wo = _v(ng), _v(wo)
+        self.L.ora_mat_emit(C.byref(self.rec), ng.ctypes.data, wo.ctypes.data, out.ctypes.data)
+        return out
+
+
+def fresnel(i, n, ior):
+    L = _mat_lib()
+    out = np.zeros(2, np.float32)
+    i, n = _v(i), _v(n)
+    L.ora_fresnel(i.ctypes.data, n.ctypes.data, float(ior), out.ctypes.data)
+    return float(out[0]), float(out[1])
+
+
+def refract(n, wi, ior):
+    L = _mat_lib()
+    out = np.zeros(3, np.float32)
+    n, wi = _v(n), _v(wi)
+    ok = L.ora_refract(n.ctypes.data, wi.ctypes.data, float(ior), out.ctypes.data)
+    return out if ok else None

@@ -130,3 +130,55 @@ def test_full_cornell_statistics(oracle):
     rel = ((gimg[..., :3] - ref) ** 2 / (ref ** 2 + 1e-2)).mean()
     say(f"  relMSE(16 spp vs 256 spp) = {rel:.3e}")
     assert rel < 0.05
+
+
+# ---------------------------------------------------------------- materials and recursion
+def test_materials_match_oracle(oracle):
+    """Glossy (as_diffuse), mirror, glass and light_mat surfaces, the specular
+    recursion through them, point and area lights, constant background."""
+    sc = scenes.cornell_materials_scene(n_random=1500, res=48, spp=4, seed=4)
+    say("pt materials 48x48x4 (specular recursion)")
+    g = render_gpu(sc)
+    say(f"  stats {g[2]}")
+    compare_films(g[:2], render_cpu(oracle, sc), "materials")
+
+
+def test_glossy_branches_match_oracle(oracle):
+    """Glossy surfaces with as_diffuse = false: 8 glossy branches per hit with
+    the ray division of the reference (fewer sub-paths and light samples below)."""
+    sc = scenes.cornell_materials_scene(n_random=1200, res=40, spp=2, seed=5, glossy_branches=0.6, raydepth=3)
+    sc["settings"]["path_samples"] = 8
+    say("pt glossy branches 40x40x2, path_samples 8, raydepth 3")
+    g = render_gpu(sc)
+    say(f"  stats {g[2]}")
+    compare_films(g[:2], render_cpu(oracle, sc), "glossy branches")
+
+
+def test_specular_depths_match_oracle(oracle):
+    """The glass cube and mirror panel alone, at every ray depth from 0 to 6."""
+    for rd in (0, 1, 2, 3, 6):
+        sc = scenes.cornell_materials_scene(n_random=0, res=48, spp=2, seed=6, raydepth=rd, point_lights=1)
+        say(f"pt specular raydepth {rd}")
+        g = render_gpu(sc)
+        compare_films(g[:2], render_cpu(oracle, sc), f"raydepth {rd}", frac_tol=0.0)
+
+
+def test_glass_plate_closed_form(oracle):
+    """Normal incidence through a glass plate onto an emitter, raydepth 2:
+    pixel = kt^2 * E with the reference's Fresnel term."""
+    def quad(c, e1, e2):
+        return scenes.quad_tris(c, e1, e2)
+    front = quad((-5, -5, 1), (10, 0, 0), (0, 10, 0))
+    back = quad((-5, -5, 0.9), (0, 10, 0), (10, 0, 0))
+    emitter = quad((-5, -5, -1), (10, 0, 0), (0, 10, 0))
+    verts, tris = scenes.merge([front, back, emitter])
+    sc = dict(verts=verts, tris=tris, mat_id=np.array([0, 0, 0, 0, 1, 1], np.uint32),
+              materials=[dict(type="glass", IOR=1.5), dict(color=(1.0, 1.0, 1.0), emit=3.0, diffuse_reflect=0.0)],
+              lights=[], camera=dict(from_=(0, 0, 10), to=(0, 0, 0), up=(0, 1, 10), focal=50.0, resx=1, resy=1),
+              settings=dict(path_samples=1, bounces=3, rr_min_bounces=0, aa_samples=1, aa_pixelwidth=1.5,
+                            background=None, seed=0, raydepth=2))
+    say("pt glass plate")
+    grgba, gw, _ = render_gpu(sc)
+    col = grgba[0, 0, :3] / gw[0, 0]
+    kr, kt = oracle.fresnel(np.float32([0, 0, 1]), np.float32([0, 0, 1]), 1.5)
+    assert np.allclose(col, 3.0 * kt * kt, rtol=2e-6)
