@@ -23,6 +23,7 @@ SOURCES = [
     "accel_api.hip",
     "pt_kernels.hip",
     "pt_api.hip",
+    "guide.hip",
 ]
 
 # -ffp-contract=off / IEEE division / denormals kept: FP32 results must match

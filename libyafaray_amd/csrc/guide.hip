@@ -1,0 +1,777 @@
+// guide.hip -- the path-guiding network on MFMA (include/ymx_guide.h).
+//
+// Every wave works on tiles of 16 samples with v_mfma_f32_16x16x32_bf16:
+// layer l computes H_l^T (64 neurons x 16 samples) = W_l (64 x 64) . H_{l-1}^T.
+// A lane (g = lane / 16, s = lane % 16) holds
+//   B operand (activations): 8 BF16 of sample s at input positions 8g .. 8g+7,
+//   A operand (weights):     8 BF16 of row 16t + s at the same positions,
+//   D (accumulator):         4 FP32 of sample s, neurons 16t + 4g .. 16t + 4g + 3.
+// The accumulators of tiles 2k and 2k+1 are therefore exactly the B operand
+// of K-step k of the next layer once that layer reads its input dimension in
+// the permuted order (4g .. 4g+3, 16 + 4g .. 16 + 4g + 3 within each block of
+// 32) -- the activations never leave registers between layers.  The encoder
+// lays out the first layer's input so that lane group g computes hash-grid
+// levels 4g .. 4g+3 of its sample.
+//
+// Training runs forward (activations stored as BF16 [sample][64]), the loss
+// gradient per sample, backward through the layers (the transposed weights,
+// same permutation trick), the hash-grid gradient by atomics, then the weight
+// gradients as split-K GEMMs dW = dH^T H over the batch, staged through LDS.
+#include "guide_math.h"
+#include "accel_internal.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <new>
+#include <random>
+#include <vector>
+
+using namespace ymx::guide;
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+struct ymx_guide
+{
+	ymx_guide_params p;
+	uint32_t table, n_grid;
+	int32_t res[kLevels];
+	float *grid, *grid_m, *grid_v, *grid_g;      // FP32 master, Adam moments, gradient
+	uint16_t *grid_bf;                           // BF16 copy read by the kernels
+	float *mlp, *mlp_m, *mlp_v, *mlp_g;          // W1 W2 W3 (64 x 64), W4 (48 x 64)
+	uint16_t *w_bf;                              // 4 x [64 out][64 in] (W4 rows 48..63 zero)
+	uint16_t *wt_bf;                             // 4 x [64 in][64 out] (W4^T columns 48..63 zero)
+	uint16_t *X, *H1, *H2, *H3, *dO, *dH3, *dH2, *dH1;  // [max_batch][64]
+	double *loss;
+	int step;
+};
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kMlp = YMX_GUIDE_MLP_WEIGHTS;
+constexpr int kWgradSamples = 64;  // samples per LDS stage of the weight-gradient GEMM
+
+struct GArgs
+{
+	const uint16_t *grid_bf;
+	const uint16_t *w_bf, *wt_bf;
+	float *grid_g, *mlp_g;
+	uint16_t *X, *H1, *H2, *H3, *dO, *dH3, *dH2, *dH1;
+	double *loss;
+	uint32_t table_mask;
+	int32_t res[kLevels];
+	float bmin[3], inv_ext[3];
+};
+
+union Frag
+{
+	bf16x8 v;
+	uint16_t h[8];
+	uint4 u;
+	uint2 d[2];
+};
+
+__device__ __forceinline__ f32x4 mfma(const Frag &a, const Frag &b, f32x4 c)
+{
+	return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ uint32_t count_of(uint32_t n, const uint32_t *n_dev)
+{
+	if(!n_dev) return n;
+	const uint32_t m = *n_dev;
+	return m < n ? m : n;
+}
+
+// Position of a sample on the unit cube of the scene box.
+__device__ __forceinline__ void unit_pos(const GArgs &g, const ymx_guide_sample &r, float p[3])
+{
+	for(int a = 0; a < 3; ++a)
+	{
+		float v = (r.pos[a] - g.bmin[a]) * g.inv_ext[a];
+		p[a] = v < 0.f ? 0.f : (v > 1.f ? 1.f : v);
+	}
+}
+
+// Trilinear corners of one level: entry index and weight of corner c (bit 0 x, 1 y, 2 z).
+struct Corners
+{
+	uint32_t idx[8];
+	float w[8];
+};
+__device__ __forceinline__ void level_corners(const GArgs &g, int lev, const float p[3], Corners &c)
+{
+	const int r = g.res[lev];
+	uint32_t b[3];
+	float f[3];
+	for(int a = 0; a < 3; ++a)
+	{
+		const float s = p[a] * (float)r;
+		const float fl = floorf(s);
+		b[a] = (uint32_t)fl;
+		f[a] = s - fl;
+	}
+	for(int k = 0; k < 8; ++k)
+	{
+		const uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = (k >> 2) & 1;
+		const float wx = dx ? f[0] : 1.f - f[0];
+		const float wy = dy ? f[1] : 1.f - f[1];
+		const float wz = dz ? f[2] : 1.f - f[2];
+		c.w[k] = (wx * wy) * wz;
+		c.idx[k] = grid_index(r, g.table_mask, b[0] + dx, b[1] + dy, b[2] + dz);
+	}
+}
+
+// First-layer input of the lane: K-block 0 holds levels 4g .. 4g+3 (2 features
+// each), K-block 1 the normal, wo and a constant 1 (lane group 0 only).
+__device__ __forceinline__ void encode(const GArgs &g, const ymx_guide_sample &r, int grp, Frag &x0, Frag &x1)
+{
+	float p[3];
+	unit_pos(g, r, p);
+	const uint32_t table = g.table_mask + 1u;
+	for(int q = 0; q < 4; ++q)
+	{
+		const int lev = 4 * grp + q;
+		Corners c;
+		level_corners(g, lev, p, c);
+		float f0 = 0.f, f1 = 0.f;
+		const uint32_t *lv = reinterpret_cast<const uint32_t *>(g.grid_bf) + (size_t)lev * table;
+		for(int k = 0; k < 8; ++k)
+		{
+			const uint32_t pair = lv[c.idx[k]];
+			f0 = f0 + c.w[k] * bf2f((uint16_t)(pair & 0xFFFFu));
+			f1 = f1 + c.w[k] * bf2f((uint16_t)(pair >> 16));
+		}
+		x0.h[2 * q] = f2bf(f0);
+		x0.h[2 * q + 1] = f2bf(f1);
+	}
+	for(int j = 0; j < 8; ++j) x1.h[j] = 0;
+	if(grp == 0)
+	{
+		x1.h[0] = f2bf(r.normal[0]);
+		x1.h[1] = f2bf(r.normal[1]);
+		x1.h[2] = f2bf(r.normal[2]);
+		x1.h[3] = f2bf(r.wo[0]);
+		x1.h[4] = f2bf(r.wo[1]);
+		x1.h[5] = f2bf(r.wo[2]);
+		x1.h[6] = f2bf(1.f);
+	}
+}
+
+// A operand of a layer whose input arrives in the permuted order (see the
+// file comment): row `row` of the [64][64] matrix m, K-step ks, lane group grp.
+__device__ __forceinline__ Frag load_a_perm(const uint16_t *m, int row, int ks, int grp)
+{
+	Frag a;
+	const uint16_t *r = m + row * 64 + 32 * ks;
+	a.d[0] = *reinterpret_cast<const uint2 *>(r + 4 * grp);
+	a.d[1] = *reinterpret_cast<const uint2 *>(r + 16 + 4 * grp);
+	return a;
+}
+__device__ __forceinline__ Frag load_a(const uint16_t *m, int row, int ks, int grp)
+{
+	Frag a;
+	a.u = *reinterpret_cast<const uint4 *>(m + row * 64 + 32 * ks + 8 * grp);
+	return a;
+}
+
+// ReLU'd accumulators of tiles 2ks, 2ks+1 as the next B operand (BF16).
+__device__ __forceinline__ Frag acc_to_b(const f32x4 *acc, int ks, bool relu)
+{
+	Frag b;
+	for(int i = 0; i < 4; ++i)
+	{
+		float v0 = acc[2 * ks][i], v1 = acc[2 * ks + 1][i];
+		if(relu)
+		{
+			v0 = v0 > 0.f ? v0 : 0.f;
+			v1 = v1 > 0.f ? v1 : 0.f;
+		}
+		b.h[i] = f2bf(v0);
+		b.h[4 + i] = f2bf(v1);
+	}
+	return b;
+}
+
+// Store the lane's part of a [sample][64] activation row (positions of acc_to_b).
+__device__ __forceinline__ void store_act(uint16_t *buf, uint32_t s, int grp, const Frag *b)
+{
+	uint16_t *row = buf + (size_t)s * 64;
+	for(int ks = 0; ks < 2; ++ks)
+	{
+		*reinterpret_cast<uint2 *>(row + 32 * ks + 4 * grp) = b[ks].d[0];
+		*reinterpret_cast<uint2 *>(row + 32 * ks + 16 + 4 * grp) = b[ks].d[1];
+	}
+}
+
+// One hidden layer: 4 output tiles, 2 K-steps.
+__device__ __forceinline__ void layer(const uint16_t *w, const Frag *in, bool perm, int sl, int grp, f32x4 *acc,
+                                      int tiles)
+{
+	for(int t = 0; t < tiles; ++t)
+	{
+		acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+		for(int ks = 0; ks < 2; ++ks)
+		{
+			const Frag a = perm ? load_a_perm(w, 16 * t + sl, ks, grp) : load_a(w, 16 * t + sl, ks, grp);
+			acc[t] = mfma(a, in[ks], acc[t]);
+		}
+	}
+}
+
+// Mixture decoding of 40 raw outputs (unit mu, clamped log kappa, softmax weights).
+__device__ void decode(const float *raw, float *mix)
+{
+	float mx = -INFINITY;
+	for(int k = 0; k < kLobes; ++k) mx = fmaxf(mx, raw[5 * k + 4]);
+	float e[kLobes], sum = 0.f;
+	for(int k = 0; k < kLobes; ++k)
+	{
+		e[k] = expf(raw[5 * k + 4] - mx);
+		sum += e[k];
+	}
+	for(int k = 0; k < kLobes; ++k)
+	{
+		const float *l = raw + 5 * k;
+		float n = sqrtf(l[0] * l[0] + l[1] * l[1] + l[2] * l[2]);
+		const float inv = 1.f / fmaxf(n, 1e-8f);
+		const float s = fminf(fmaxf(l[3], kKappaLogMin), kKappaLogMax);
+		const float kappa = expf(s);
+		float *o = mix + 6 * k;
+		o[0] = l[0] * inv;
+		o[1] = l[1] * inv;
+		o[2] = l[2] * inv;
+		o[3] = kappa;
+		o[4] = e[k] / sum;
+		o[5] = kappa / (6.283185307179586f * (-expm1f(-2.f * kappa)));
+	}
+}
+
+// d(-c log q)/d raw for one training sample (oracle/ora_guide.py dloss_dout).
+__device__ float loss_grad(const float *raw, const ymx_guide_sample &r, float *d)
+{
+	float mix[kMix];
+	decode(raw, mix);
+	const float wx = r.w[0], wy = r.w[1], wz = r.w[2];
+	float v[kLobes], cosv[kLobes], q = 0.f;
+	for(int k = 0; k < kLobes; ++k)
+	{
+		const float *l = mix + 6 * k;
+		cosv[k] = l[0] * wx + l[1] * wy + l[2] * wz;
+		v[k] = l[4] * l[5] * expf(l[3] * (cosv[k] - 1.f));
+		q += v[k];
+	}
+	const float c = r.c;
+	const float qs = fmaxf(q, 1e-30f);
+	for(int k = 0; k < kLobes; ++k)
+	{
+		const float *l = mix + 6 * k;
+		const float *rk = raw + 5 * k;
+		const float resp = v[k] / qs;
+		const float kappa = l[3];
+		d[5 * k + 4] = -c * (resp - l[4]);
+		const float e2 = expf(-2.f * kappa);
+		const float dk = 1.f / kappa - 2.f * e2 / (-expm1f(-2.f * kappa)) + cosv[k] - 1.f;
+		const bool inside = rk[3] > kKappaLogMin && rk[3] < kKappaLogMax;
+		d[5 * k + 3] = inside ? -c * resp * kappa * dk : 0.f;
+		const float n = sqrtf(rk[0] * rk[0] + rk[1] * rk[1] + rk[2] * rk[2]);
+		const float sc = -c * resp * kappa / fmaxf(n, 1e-8f);
+		d[5 * k + 0] = sc * (wx - cosv[k] * l[0]);
+		d[5 * k + 1] = sc * (wy - cosv[k] * l[1]);
+		d[5 * k + 2] = sc * (wz - cosv[k] * l[2]);
+	}
+	for(int k = 5 * kLobes; k < 64; ++k) d[k] = 0.f;
+	return -c * logf(qs);
+}
+
+// Forward: inference (decoded mixtures) or training (activations, loss gradient).
+template <bool kTrain>
+__global__ __launch_bounds__(kBlock) void k_forward(GArgs g, const ymx_guide_sample *smp, uint32_t cap,
+                                                     const uint32_t *n_dev, float *mix_out)
+{
+	__shared__ float outs[kWaves][16][kOut];
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int sl = lane & 15, grp = lane >> 4;
+	const uint32_t n = count_of(cap, n_dev);
+	const uint32_t tile0 = (blockIdx.x * kWaves + wave) * 16;
+	const bool wave_active = tile0 < n;  // uniform over the wave: MFMAs run with all lanes
+	const uint32_t s = tile0 + sl;
+	const bool valid = s < n;
+	if(wave_active)
+	{
+		ymx_guide_sample r;
+		if(valid) r = smp[s];
+		else memset(&r, 0, sizeof r);
+		Frag in[2];
+		encode(g, r, grp, in[0], in[1]);
+		if(kTrain && valid)
+		{
+			uint16_t *row = g.X + (size_t)s * 64;
+			*reinterpret_cast<uint4 *>(row + 8 * grp) = in[0].u;
+			*reinterpret_cast<uint4 *>(row + 32 + 8 * grp) = in[1].u;
+		}
+		f32x4 acc[4];
+		layer(g.w_bf, in, false, sl, grp, acc, 4);
+		uint16_t *hbuf[3] = {g.H1, g.H2, g.H3};
+		for(int l = 1; l < 4; ++l)
+		{
+			in[0] = acc_to_b(acc, 0, true);
+			in[1] = acc_to_b(acc, 1, true);
+			if(kTrain && valid) store_act(hbuf[l - 1], s, grp, in);
+			layer(g.w_bf + l * 4096, in, true, sl, grp, acc, l < 3 ? 4 : 3);
+		}
+		for(int t = 0; t < 3; ++t)
+			for(int i = 0; i < 4; ++i) outs[wave][sl][16 * t + 4 * grp + i] = acc[t][i];
+	}
+	__syncthreads();
+	if(!wave_active || grp != 0 || !valid) return;
+	const ymx_guide_sample r = smp[s];
+	const float *raw = outs[wave][sl];
+	if(!kTrain)
+	{
+		float mix[kMix];
+		decode(raw, mix);
+		float4 *o = reinterpret_cast<float4 *>(mix_out + (size_t)s * kMix);
+		for(int k = 0; k < kMix / 4; ++k) o[k] = make_float4(mix[4 * k], mix[4 * k + 1], mix[4 * k + 2], mix[4 * k + 3]);
+		return;
+	}
+	float d[64];
+	const float l = loss_grad(raw, r, d);
+	atomicAdd(g.loss, (double)l);
+	uint16_t *row = g.dO + (size_t)s * 64;
+	for(int k = 0; k < 64; k += 8)
+	{
+		Frag f;
+		for(int j = 0; j < 8; ++j) f.h[j] = f2bf(d[k + j]);
+		*reinterpret_cast<uint4 *>(row + k) = f.u;
+	}
+}
+
+// Backward through the layers: dH (BF16, ReLU mask applied) of layers 3, 2, 1
+// and the hash-grid gradient.
+__global__ __launch_bounds__(kBlock) void k_backward(GArgs g, const ymx_guide_sample *smp, uint32_t cap,
+                                                      const uint32_t *n_dev)
+{
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int sl = lane & 15, grp = lane >> 4;
+	const uint32_t n = count_of(cap, n_dev);
+	const uint32_t tile0 = (blockIdx.x * kWaves + wave) * 16;
+	if(tile0 >= n) return;
+	const uint32_t s = tile0 + sl;
+	const bool valid = s < n;
+	const uint32_t sr = valid ? s : tile0;  // rows of an idle lane are read but not used
+	Frag in[2];
+	in[0].u = *reinterpret_cast<const uint4 *>(g.dO + (size_t)sr * 64 + 8 * grp);
+	in[1].u = *reinterpret_cast<const uint4 *>(g.dO + (size_t)sr * 64 + 32 + 8 * grp);
+	if(!valid)
+	{
+		in[0].u = make_uint4(0, 0, 0, 0);
+		in[1].u = make_uint4(0, 0, 0, 0);
+	}
+	f32x4 acc[4];
+	// dH3 = W4^T dO (input positions of dO in natural order)
+	layer(g.wt_bf + 3 * 4096, in, false, sl, grp, acc, 4);
+	const uint16_t *hact[3] = {g.H3, g.H2, g.H1};
+	uint16_t *dbuf[3] = {g.dH3, g.dH2, g.dH1};
+	for(int l = 0; l < 3; ++l)
+	{
+		// ReLU mask from the stored activation at the accumulator positions
+		const uint16_t *hrow = hact[l] + (size_t)sr * 64;
+		for(int t = 0; t < 4; ++t)
+		{
+			const uint2 hv = *reinterpret_cast<const uint2 *>(hrow + 16 * t + 4 * grp);
+			const uint16_t hh[4] = {(uint16_t)(hv.x & 0xFFFFu), (uint16_t)(hv.x >> 16), (uint16_t)(hv.y & 0xFFFFu),
+			                        (uint16_t)(hv.y >> 16)};
+			for(int i = 0; i < 4; ++i)
+				if(!(bf2f(hh[i]) > 0.f)) acc[t][i] = 0.f;
+		}
+		in[0] = acc_to_b(acc, 0, false);
+		in[1] = acc_to_b(acc, 1, false);
+		if(valid) store_act(dbuf[l], s, grp, in);
+		// next: dH_{l-1} = W_l^T dH_l (layer 3 - l), input positions permuted
+		layer(g.wt_bf + (2 - l) * 4096, in, true, sl, grp, acc, l < 2 ? 4 : 2);
+	}
+	if(!valid) return;
+	// acc[t][i] = dX[16 t + 4 grp + i], t = 0, 1: the grid features of levels
+	// 8t + 2 grp and 8t + 2 grp + 1
+	const ymx_guide_sample r = smp[s];
+	float p[3];
+	unit_pos(g, r, p);
+	const uint32_t table = g.table_mask + 1u;
+	for(int t = 0; t < 2; ++t)
+		for(int q = 0; q < 2; ++q)
+		{
+			const int lev = 8 * t + 2 * grp + q;
+			const float d0 = acc[t][2 * q], d1 = acc[t][2 * q + 1];
+			Corners c;
+			level_corners(g, lev, p, c);
+			float *lv = g.grid_g + (size_t)lev * table * kFeat;
+			for(int k = 0; k < 8; ++k)
+			{
+				atomicAdd(lv + 2 * (size_t)c.idx[k], c.w[k] * d0);
+				atomicAdd(lv + 2 * (size_t)c.idx[k] + 1, c.w[k] * d1);
+			}
+		}
+}
+
+// dW (64 x 64 FP32) += sum over samples of dH[s][o] * H[s][i], split over
+// blocks along the samples.  Wave w computes output rows 16w .. 16w+15.
+__global__ __launch_bounds__(kBlock) void k_wgrad(const uint16_t *dH, const uint16_t *H, uint32_t cap,
+                                                   const uint32_t *n_dev, float *dW, int rows)
+{
+	// [feature][sample], rows padded to 144 B (a multiple of 16 for 128-bit reads)
+	__shared__ __attribute__((aligned(16))) uint16_t sd[64][kWgradSamples + 8];
+	__shared__ __attribute__((aligned(16))) uint16_t sh[64][kWgradSamples + 8];
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int sl = lane & 15, grp = lane >> 4;
+	const uint32_t n = count_of(cap, n_dev);
+	const uint32_t per = ((n + gridDim.x - 1) / gridDim.x + kWgradSamples - 1) / kWgradSamples * kWgradSamples;
+	const uint32_t b0 = blockIdx.x * per;
+	const uint32_t b1 = b0 + per < n ? b0 + per : n;
+	f32x4 acc[4] = {};
+	for(uint32_t base = b0; base < b1; base += kWgradSamples)
+	{
+		// stage 64 samples x 64 features of both operands, transposed
+		for(int e = threadIdx.x; e < kWgradSamples * 8; e += kBlock)
+		{
+			const int sm = e >> 3, part = e & 7;
+			const uint32_t s = base + sm;
+			Frag a, h;
+			if(s < b1)
+			{
+				a.u = *reinterpret_cast<const uint4 *>(dH + (size_t)s * 64 + 8 * part);
+				h.u = *reinterpret_cast<const uint4 *>(H + (size_t)s * 64 + 8 * part);
+			}
+			else
+			{
+				a.u = make_uint4(0, 0, 0, 0);
+				h.u = make_uint4(0, 0, 0, 0);
+			}
+			for(int j = 0; j < 8; ++j)
+			{
+				sd[8 * part + j][sm] = a.h[j];
+				sh[8 * part + j][sm] = h.h[j];
+			}
+		}
+		__syncthreads();
+		for(int kb = 0; kb < kWgradSamples / 32; ++kb)
+		{
+			Frag a, b;
+			a.u = *reinterpret_cast<const uint4 *>(&sd[16 * wave + sl][32 * kb + 8 * grp]);
+			for(int t = 0; t < 4; ++t)
+			{
+				b.u = *reinterpret_cast<const uint4 *>(&sh[16 * t + sl][32 * kb + 8 * grp]);
+				acc[t] = mfma(a, b, acc[t]);
+			}
+		}
+		__syncthreads();
+	}
+	// acc[t][i] = dW[16 wave + 4 grp + i][16 t + sl]
+	for(int t = 0; t < 4; ++t)
+		for(int i = 0; i < 4; ++i)
+		{
+			const int o = 16 * wave + 4 * grp + i;
+			if(o < rows) atomicAdd(dW + o * 64 + 16 * t + sl, acc[t][i]);
+		}
+}
+
+// Adam on FP32 master weights (Kingma & Ba 2015, bias-corrected), then the
+// BF16 copy.
+__global__ void k_adam(float *p, float *m, float *v, float *gr, uint16_t *pb, uint32_t n, float lr, float b1, float b2,
+                       float eps, float c1, float c2)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= n) return;
+	const float gi = gr[i];
+	const float mi = b1 * m[i] + (1.f - b1) * gi;
+	const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+	m[i] = mi;
+	v[i] = vi;
+	const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+	p[i] = pi;
+	gr[i] = 0.f;
+	if(pb) pb[i] = f2bf(pi);
+}
+
+// BF16 weight layouts from the FP32 master: W (rows padded to 64) and W^T.
+__global__ void k_weights_bf(const float *mlp, uint16_t *w_bf, uint16_t *wt_bf)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= 4 * 4096) return;
+	const int l = i / 4096, o = (i % 4096) / 64, c = i % 64;
+	const int rows = l < 3 ? 64 : 48;
+	const float v = o < rows ? mlp[l * 4096 + o * 64 + c] : 0.f;
+	w_bf[l * 4096 + o * 64 + c] = f2bf(v);
+	wt_bf[l * 4096 + c * 64 + o] = f2bf(v);
+}
+
+GArgs args_of(const ymx_guide *g)
+{
+	GArgs a;
+	a.grid_bf = g->grid_bf;
+	a.w_bf = g->w_bf;
+	a.wt_bf = g->wt_bf;
+	a.grid_g = g->grid_g;
+	a.mlp_g = g->mlp_g;
+	a.X = g->X;
+	a.H1 = g->H1;
+	a.H2 = g->H2;
+	a.H3 = g->H3;
+	a.dO = g->dO;
+	a.dH3 = g->dH3;
+	a.dH2 = g->dH2;
+	a.dH1 = g->dH1;
+	a.loss = g->loss;
+	a.table_mask = g->table - 1u;
+	for(int l = 0; l < kLevels; ++l) a.res[l] = g->res[l];
+	for(int k = 0; k < 3; ++k)
+	{
+		a.bmin[k] = g->p.bmin[k];
+		a.inv_ext[k] = 1.f / (g->p.bmax[k] - g->p.bmin[k]);
+	}
+	return a;
+}
+
+void free_guide(ymx_guide *g)
+{
+	if(!g) return;
+	void *ptrs[] = {g->grid, g->grid_m, g->grid_v, g->grid_g, g->grid_bf, g->mlp, g->mlp_m, g->mlp_v, g->mlp_g,
+	                g->w_bf, g->wt_bf, g->X, g->H1, g->H2, g->H3, g->dO, g->dH3, g->dH2, g->dH1, g->loss};
+	for(void *p : ptrs)
+		if(p) (void)hipFree(p);
+	delete g;
+}
+
+template <typename T> hipError_t galloc(T *&p, size_t count) { return hipMalloc((void **)&p, sizeof(T) * (count ? count : 1)); }
+
+int refresh_bf16(ymx_guide *g, hipStream_t s)
+{
+	k_weights_bf<<<(4 * 4096 + 255) / 256, 256, 0, s>>>(g->mlp, g->w_bf, g->wt_bf);
+	YMX_HIP_CHECK(hipGetLastError());
+	return YMX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ymx_guide_create(const ymx_guide_params *p, ymx_guide **out)
+{
+	if(!p || !out || p->struct_size != sizeof(ymx_guide_params))
+	{
+		ymx_set_error("ymx_guide_create: NULL argument or struct_size mismatch");
+		return YMX_ERR_INVALID_ARG;
+	}
+	*out = nullptr;
+	if(p->log2_table_size < 8 || p->log2_table_size > 22 || p->n_min < 2 || p->n_max < p->n_min ||
+	   p->n_max > (1 << 20) || p->max_batch == 0 || p->max_batch > (1u << 24) ||
+	   !(p->bmax[0] > p->bmin[0] && p->bmax[1] > p->bmin[1] && p->bmax[2] > p->bmin[2]))
+	{
+		ymx_set_error("ymx_guide_create: unsupported parameters");
+		return YMX_ERR_UNSUPPORTED;
+	}
+	ymx_guide *g = new(std::nothrow) ymx_guide();
+	if(!g)
+	{
+		ymx_set_error("ymx_guide_create: host allocation failed");
+		return YMX_ERR_OUT_OF_MEMORY;
+	}
+	memset(g, 0, sizeof *g);
+	g->p = *p;
+	g->table = 1u << p->log2_table_size;
+	g->n_grid = kLevels * g->table * kFeat;
+	// N_l = floor(N_min b^l), b = exp((ln N_max - ln N_min) / (L - 1))
+	const double b = exp((log((double)p->n_max) - log((double)p->n_min)) / (kLevels - 1));
+	for(int l = 0; l < kLevels; ++l) g->res[l] = (int32_t)floor((double)p->n_min * pow(b, (double)l));
+#define G_TRY(expr)                                                            \
+	do                                                                         \
+	{                                                                          \
+		hipError_t e_ = (expr);                                                \
+		if(e_ != hipSuccess)                                                   \
+		{                                                                      \
+			ymx_set_error("%s: %s", #expr, hipGetErrorString(e_));             \
+			free_guide(g);                                                     \
+			return e_ == hipErrorOutOfMemory ? YMX_ERR_OUT_OF_MEMORY : YMX_ERR_HIP; \
+		}                                                                      \
+	} while(0)
+	G_TRY(galloc(g->grid, g->n_grid));
+	G_TRY(galloc(g->grid_m, g->n_grid));
+	G_TRY(galloc(g->grid_v, g->n_grid));
+	G_TRY(galloc(g->grid_g, g->n_grid));
+	G_TRY(galloc(g->grid_bf, g->n_grid));
+	G_TRY(galloc(g->mlp, kMlp));
+	G_TRY(galloc(g->mlp_m, kMlp));
+	G_TRY(galloc(g->mlp_v, kMlp));
+	G_TRY(galloc(g->mlp_g, kMlp));
+	G_TRY(galloc(g->w_bf, 4 * 4096));
+	G_TRY(galloc(g->wt_bf, 4 * 4096));
+	uint16_t **acts[] = {&g->X, &g->H1, &g->H2, &g->H3, &g->dO, &g->dH3, &g->dH2, &g->dH1};
+	for(uint16_t **a : acts) G_TRY(galloc(*a, (size_t)p->max_batch * 64));
+	G_TRY(galloc(g->loss, 1));
+	// initial weights: grid U(-1e-4, 1e-4), W U(-s, s) with s = sqrt(6 / fan_in)
+	std::mt19937 rng(p->seed);
+	std::uniform_real_distribution<float> u(-1.f, 1.f);
+	std::vector<float> grid(g->n_grid), mlp(kMlp, 0.f);
+	for(float &x : grid) x = 1e-4f * u(rng);
+	for(int l = 0; l < 4; ++l)
+	{
+		const int rows = l < 3 ? 64 : 48;
+		const float s = sqrtf(6.f / 64.f);
+		for(int i = 0; i < rows * 64; ++i) mlp[l * 4096 + i] = s * u(rng);
+	}
+	for(int i = 5 * kLobes * 64; i < 48 * 64; ++i) mlp[3 * 4096 + i] = 0.f;
+	*out = g;
+	const int rc = ymx_guide_set_weights(g, grid.data(), mlp.data());
+	if(rc != YMX_OK)
+	{
+		free_guide(g);
+		*out = nullptr;
+	}
+	return rc;
+#undef G_TRY
+}
+
+int ymx_guide_destroy(ymx_guide *g)
+{
+	if(!g) return YMX_OK;
+	(void)hipDeviceSynchronize();
+	free_guide(g);
+	return YMX_OK;
+}
+
+int ymx_guide_set_weights(ymx_guide *g, const float *grid, const float *mlp)
+{
+	if(!g || !grid || !mlp)
+	{
+		ymx_set_error("ymx_guide_set_weights: NULL argument");
+		return YMX_ERR_INVALID_ARG;
+	}
+	YMX_HIP_CHECK(hipMemcpy(g->grid, grid, sizeof(float) * g->n_grid, hipMemcpyHostToDevice));
+	YMX_HIP_CHECK(hipMemcpy(g->mlp, mlp, sizeof(float) * kMlp, hipMemcpyHostToDevice));
+	std::vector<uint16_t> gb(g->n_grid);
+	for(uint32_t i = 0; i < g->n_grid; ++i) gb[i] = f2bf(grid[i]);
+	YMX_HIP_CHECK(hipMemcpy(g->grid_bf, gb.data(), sizeof(uint16_t) * g->n_grid, hipMemcpyHostToDevice));
+	YMX_HIP_CHECK(hipMemset(g->grid_m, 0, sizeof(float) * g->n_grid));
+	YMX_HIP_CHECK(hipMemset(g->grid_v, 0, sizeof(float) * g->n_grid));
+	YMX_HIP_CHECK(hipMemset(g->grid_g, 0, sizeof(float) * g->n_grid));
+	YMX_HIP_CHECK(hipMemset(g->mlp_m, 0, sizeof(float) * kMlp));
+	YMX_HIP_CHECK(hipMemset(g->mlp_v, 0, sizeof(float) * kMlp));
+	YMX_HIP_CHECK(hipMemset(g->mlp_g, 0, sizeof(float) * kMlp));
+	g->step = 0;
+	const int rc = refresh_bf16(g, nullptr);
+	if(rc != YMX_OK) return rc;
+	YMX_HIP_CHECK(hipDeviceSynchronize());
+	return YMX_OK;
+}
+
+int ymx_guide_get_weights(ymx_guide *g, float *grid, float *mlp)
+{
+	if(!g || !grid || !mlp)
+	{
+		ymx_set_error("ymx_guide_get_weights: NULL argument");
+		return YMX_ERR_INVALID_ARG;
+	}
+	YMX_HIP_CHECK(hipDeviceSynchronize());
+	YMX_HIP_CHECK(hipMemcpy(grid, g->grid, sizeof(float) * g->n_grid, hipMemcpyDeviceToHost));
+	YMX_HIP_CHECK(hipMemcpy(mlp, g->mlp, sizeof(float) * kMlp, hipMemcpyDeviceToHost));
+	return YMX_OK;
+}
+
+int ymx_guide_infer(ymx_guide *g, const ymx_guide_sample *queries, float *mix, uint32_t n, const uint32_t *n_dev,
+                    void *stream)
+{
+	if(!g || (n && (!queries || !mix)))
+	{
+		ymx_set_error("ymx_guide_infer: NULL argument");
+		return YMX_ERR_INVALID_ARG;
+	}
+	if(n == 0) return YMX_OK;
+	const unsigned blocks = (n + 16 * kWaves - 1) / (16 * kWaves);
+	k_forward<false><<<blocks, kBlock, 0, (hipStream_t)stream>>>(args_of(g), queries, n, n_dev, mix);
+	YMX_HIP_CHECK(hipGetLastError());
+	return YMX_OK;
+}
+
+int ymx_guide_backward(ymx_guide *g, const ymx_guide_sample *samples, uint32_t n, const uint32_t *n_dev,
+                       void *stream)
+{
+	if(!g || (n && !samples))
+	{
+		ymx_set_error("ymx_guide_backward: NULL argument");
+		return YMX_ERR_INVALID_ARG;
+	}
+	if(n > g->p.max_batch)
+	{
+		ymx_set_error("ymx_guide_backward: %u samples exceed max_batch %u", n, g->p.max_batch);
+		return YMX_ERR_INVALID_ARG;
+	}
+	hipStream_t s = (hipStream_t)stream;
+	YMX_HIP_CHECK(hipMemsetAsync(g->grid_g, 0, sizeof(float) * g->n_grid, s));
+	YMX_HIP_CHECK(hipMemsetAsync(g->mlp_g, 0, sizeof(float) * kMlp, s));
+	YMX_HIP_CHECK(hipMemsetAsync(g->loss, 0, sizeof(double), s));
+	if(n == 0) return YMX_OK;
+	const GArgs a = args_of(g);
+	const unsigned blocks = (n + 16 * kWaves - 1) / (16 * kWaves);
+	k_forward<true><<<blocks, kBlock, 0, s>>>(a, samples, n, n_dev, nullptr);
+	YMX_HIP_CHECK(hipGetLastError());
+	k_backward<<<blocks, kBlock, 0, s>>>(a, samples, n, n_dev);
+	YMX_HIP_CHECK(hipGetLastError());
+	const uint16_t *dh[4] = {g->dH1, g->dH2, g->dH3, g->dO};
+	const uint16_t *hh[4] = {g->X, g->H1, g->H2, g->H3};
+	unsigned wblocks = (n + 4095) / 4096;
+	if(wblocks < 1) wblocks = 1;
+	if(wblocks > 512) wblocks = 512;
+	for(int l = 0; l < 4; ++l)
+	{
+		k_wgrad<<<wblocks, kBlock, 0, s>>>(dh[l], hh[l], n, n_dev, g->mlp_g + l * 4096, l < 3 ? 64 : 48);
+		YMX_HIP_CHECK(hipGetLastError());
+	}
+	return YMX_OK;
+}
+
+int ymx_guide_get_gradients(ymx_guide *g, float *grid, float *mlp, double *loss)
+{
+	if(!g || !grid || !mlp || !loss)
+	{
+		ymx_set_error("ymx_guide_get_gradients: NULL argument");
+		return YMX_ERR_INVALID_ARG;
+	}
+	YMX_HIP_CHECK(hipDeviceSynchronize());
+	YMX_HIP_CHECK(hipMemcpy(grid, g->grid_g, sizeof(float) * g->n_grid, hipMemcpyDeviceToHost));
+	YMX_HIP_CHECK(hipMemcpy(mlp, g->mlp_g, sizeof(float) * kMlp, hipMemcpyDeviceToHost));
+	YMX_HIP_CHECK(hipMemcpy(loss, g->loss, sizeof(double), hipMemcpyDeviceToHost));
+	return YMX_OK;
+}
+
+int ymx_guide_adam_step(ymx_guide *g, void *stream)
+{
+	if(!g)
+	{
+		ymx_set_error("ymx_guide_adam_step: NULL handle");
+		return YMX_ERR_INVALID_ARG;
+	}
+	hipStream_t s = (hipStream_t)stream;
+	g->step += 1;
+	const float c1 = 1.f - powf(g->p.beta1, (float)g->step);
+	const float c2 = 1.f - powf(g->p.beta2, (float)g->step);
+	k_adam<<<(g->n_grid + 255) / 256, 256, 0, s>>>(g->grid, g->grid_m, g->grid_v, g->grid_g, g->grid_bf, g->n_grid,
+	                                              g->p.lr, g->p.beta1, g->p.beta2, g->p.eps, c1, c2);
+	YMX_HIP_CHECK(hipGetLastError());
+	k_adam<<<(kMlp + 255) / 256, 256, 0, s>>>(g->mlp, g->mlp_m, g->mlp_v, g->mlp_g, nullptr, kMlp, g->p.lr,
+	                                         g->p.beta1, g->p.beta2, g->p.eps, c1, c2);
+	YMX_HIP_CHECK(hipGetLastError());
+	return refresh_bf16(g, s);
+}
+
+int ymx_guide_train_step(ymx_guide *g, const ymx_guide_sample *samples, uint32_t n, const uint32_t *n_dev,
+                         void *stream)
+{
+	const int rc = ymx_guide_backward(g, samples, n, n_dev, stream);
+	if(rc != YMX_OK) return rc;
+	return ymx_guide_adam_step(g, stream);
+}
+
+}  // extern "C"

@@ -70,7 +70,24 @@ SIGNATURES = {
     "ymx_accel_traversal_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int,
                                             C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_void_p]),
     "ymx_last_error": (C.c_char_p, []),
+    # include/ymx_guide.h
+    "ymx_guide_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "ymx_guide_destroy": (C.c_int, [C.c_void_p]),
+    "ymx_guide_set_weights": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ymx_guide_get_weights": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ymx_guide_infer": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "ymx_guide_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "ymx_guide_get_gradients": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]),
+    "ymx_guide_adam_step": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "ymx_guide_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
 }
+
+
+class YmxGuideParams(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("log2_table_size", C.c_uint32), ("n_min", C.c_int32),
+                ("n_max", C.c_int32), ("bmin", C.c_float * 3), ("bmax", C.c_float * 3), ("max_batch", C.c_uint32),
+                ("seed", C.c_uint32), ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
+                ("eps", C.c_float)]
 
 _lib = None
 
