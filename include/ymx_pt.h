@@ -38,6 +38,7 @@
 #include <stdint.h>
 
 #include "ymx_accel.h"
+#include "ymx_guide.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -125,6 +126,19 @@ int ymx_pt_render(ymx_pt *pt, int32_t first_sample, int32_t n_samples, void *str
  * filter weights; the pixel value is rgba / weight (ImageFilm::flush). */
 int ymx_pt_read_film(ymx_pt *pt, float *rgba, float *weight);
 
+/* Path guiding.  No counterpart in the reference, whose path tracer samples
+ * the BSDF only (src/integrator/surface/integrator_path_tracer.cc:190-240).
+ * With a guide (include/ymx_guide.h) set, the next direction at shinydiffuse
+ * path vertices is drawn by one-sample MIS: the BSDF with probability
+ * bsdf_prob (clamped to [0.2, 1]), the guide's mixture q otherwise, with the
+ * weight F / (bsdf_prob p + (1 - bsdf_prob) q), where p is the BSDF sampler's
+ * density and F / p its plain weight -- the image converges to the same
+ * expectation as without a guide.  train != 0 records (position, normal, wo,
+ * direction, radiance weight) samples during renders and runs train_steps Adam
+ * steps on them after each wavefront.  guide NULL switches guiding off.  The
+ * guide must outlive its use by this handle. */
+int ymx_pt_set_guide(ymx_pt *pt, ymx_guide *guide, float bsdf_prob, int32_t train, uint32_t train_steps);
+
 /* Counters of the last ymx_pt_render: rays cast, and its duration in ms on
  * the stream (events around the whole render). */
 typedef struct ymx_pt_stats
@@ -134,6 +148,8 @@ typedef struct ymx_pt_stats
 	uint64_t extension_rays;
 	uint64_t shadow_rays;
 	float render_ms;
+	uint64_t guide_samples;       /* training samples the paths produced (before subsampling) */
+	uint64_t guide_steps;         /* guide training steps taken */
 } ymx_pt_stats;
 
 int ymx_pt_get_stats(const ymx_pt *pt, ymx_pt_stats *stats);

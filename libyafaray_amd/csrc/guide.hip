@@ -556,6 +556,8 @@ int refresh_bf16(ymx_guide *g, hipStream_t s)
 
 }  // namespace
 
+uint32_t ymx_guide_batch_limit(const ymx_guide *g) { return g ? g->p.max_batch : 0u; }
+
 extern "C" {
 
 int ymx_guide_create(const ymx_guide_params *p, ymx_guide **out)

@@ -70,9 +70,19 @@ void free_nodes(PtNodes &nd)
 	nd.cap = 0;
 }
 
+void free_guide_buffers(PtGuide &g)
+{
+	void *ptrs[] = {g.query, g.mix_node, g.mix_path, g.rec, g.base, g.mask, g.train_buf, g.train_count,
+	                g.query_count};
+	for(void *p : ptrs)
+		if(p) (void)hipFree(p);
+	memset(&g, 0, sizeof g);
+}
+
 void free_pt(ymx_pt *pt)
 {
 	if(!pt) return;
+	free_guide_buffers(pt->g);
 	free_nodes(pt->nd);
 	void *ptrs[] = {pt->d_prim, pt->d_materials, pt->d_lights, pt->d_const, pt->d_faure, pt->d_inv_prims, pt->d_prims,
 	                pt->d_film, pt->d_film_w, pt->st.path_col, pt->st.tp_next, pt->st.tp_vertex, pt->st.pwo,
@@ -480,6 +490,64 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 	return YMX_OK;
 }
 
+int ymx_pt_set_guide(ymx_pt *pt, ymx_guide *guide, float bsdf_prob, int32_t train, uint32_t train_steps)
+{
+	if(!pt)
+	{
+		ymx_set_error("ymx_pt_set_guide: NULL handle");
+		return YMX_ERR_INVALID_ARG;
+	}
+	YMX_HIP_CHECK(hipDeviceSynchronize());
+	free_guide_buffers(pt->g);
+	pt->guide = guide;
+	pt->train_steps = 0;
+	if(!guide) return YMX_OK;
+	PtGuide &g = pt->g;
+	const uint32_t cap = pt->st.capacity < (1u << 21) ? pt->st.capacity : (1u << 21);
+	pt->guide_cap = cap;
+	const uint32_t bounces = (uint32_t)pt->host_const.bounces;
+	hipError_t e = hipSuccess;
+#define GB_TRY(expr)                                                          \
+	do                                                                        \
+	{                                                                         \
+		if((e = (expr)) != hipSuccess)                                        \
+		{                                                                     \
+			ymx_set_error("ymx_pt_set_guide: %s: %s", #expr, hipGetErrorString(e)); \
+			free_guide_buffers(g);                                            \
+			pt->guide = nullptr;                                              \
+			return e == hipErrorOutOfMemory ? YMX_ERR_OUT_OF_MEMORY : YMX_ERR_HIP; \
+		}                                                                     \
+	} while(0)
+	GB_TRY(dalloc(g.query, cap));
+	GB_TRY(dalloc(g.mix_node, (size_t)cap * YMX_GUIDE_MIX_FLOATS));
+	GB_TRY(dalloc(g.mix_path, (size_t)cap * YMX_GUIDE_MIX_FLOATS));
+	GB_TRY(dalloc(g.query_count, 1));
+	g.bsdf_prob = bsdf_prob < 0.2f ? 0.2f : (bsdf_prob > 1.f ? 1.f : bsdf_prob);
+	g.on = 1u;
+	if(train)
+	{
+		// ring of recent samples: a power of two, at least 4 mini-batches, 1M by default
+		uint64_t tcap = 1ull << 20;
+		while(tcap < 4ull * ymx_guide_batch_limit(guide) && tcap < (1ull << 24)) tcap <<= 1;
+		GB_TRY(dalloc(g.rec, (size_t)bounces * cap));
+		GB_TRY(dalloc(g.base, (size_t)bounces * cap));
+		GB_TRY(dalloc(g.mask, cap));
+		GB_TRY(hipMemset(g.mask, 0, sizeof(uint32_t) * cap));
+		GB_TRY(dalloc(g.train_buf, (size_t)tcap));
+		GB_TRY(dalloc(g.train_count, 1));
+		GB_TRY(hipMemset(g.train_count, 0, sizeof(uint32_t)));
+		g.train_cap = (uint32_t)tcap;
+		g.accept = 1.f;
+		g.train = 1u;
+		pt->train_steps = train_steps;
+		pt->g_seen = 0;
+		pt->g_filled = 0;
+		pt->g_rng = 0x853C49E6748FEA9Bull;
+	}
+#undef GB_TRY
+	return YMX_OK;
+}
+
 int ymx_pt_destroy(ymx_pt *pt)
 {
 	if(!pt) return YMX_OK;
@@ -511,6 +579,8 @@ int ymx_pt_render(ymx_pt *pt, int32_t first_sample, int32_t n_samples, void *str
 	}
 	hipStream_t s = (hipStream_t)stream;
 	YMX_HIP_CHECK(hipMemsetAsync(pt->st.totals, 0, 3 * sizeof(unsigned long long), s));
+	pt->stats.guide_samples = 0;
+	pt->stats.guide_steps = 0;
 	YMX_HIP_CHECK(hipEventRecord(pt->ev0, s));
 	for(int32_t k = 0; k < n_samples; k += (int32_t)pt->batch)
 	{
@@ -553,6 +623,8 @@ int ymx_pt_get_stats(const ymx_pt *pt, ymx_pt_stats *stats)
 	stats->extension_rays = t[1];
 	stats->shadow_rays = t[2];
 	stats->render_ms = ms;
+	stats->guide_samples = pt->stats.guide_samples;
+	stats->guide_steps = pt->stats.guide_steps;
 	return YMX_OK;
 }
 

@@ -1,6 +1,7 @@
 // pt_internal.h -- device data of the wavefront path tracer.
 #pragma once
 
+#include "../../include/ymx_guide.h"
 #include "../../include/ymx_pt.h"
 #include "accel_internal.h"
 #include "pt_math.h"
@@ -123,6 +124,27 @@ struct PtState
 	unsigned long long *totals;  // rays cast: [0] camera + recursive, [1] extension, [2] shadow
 };
 
+// Path guiding inside the wavefront (see ymx_pt_set_guide): mixtures of the
+// node vertices (by slot) and of the current path vertices (by queue entry),
+// and the training samples the paths produce.
+struct PtGuide
+{
+	uint32_t on;               // guided sampling at shinydiffuse vertices
+	uint32_t train;            // record training samples
+	float bsdf_prob;           // one-sample MIS: probability of sampling the BSDF
+	float accept;              // probability that a finished record is kept for training
+	ymx_guide_sample *query;   // [capacity] queries of one inference batch
+	float *mix_node;           // [capacity][48]
+	float *mix_path;           // [capacity][48]
+	ymx_guide_sample *rec;     // [bounces][capacity] pending records of the current sub-path
+	float4 *base;              // [bounces][capacity] path colour before the record's segment, w: lum(throughput)
+	uint32_t *mask;            // [capacity] bit r: record r is pending
+	ymx_guide_sample *train_buf;  // collected training samples
+	uint32_t *train_count;
+	uint32_t train_cap;
+	uint32_t *query_count;     // queries of the current path-vertex batch
+};
+
 }  // namespace ymx
 
 struct ymx_pt
@@ -143,6 +165,12 @@ struct ymx_pt
 	ymx::PtNodes nd;
 	bool recursive;                  // some material spawns recursive rays and raydepth >= 1
 	uint32_t max_fan;                // most children one node can spawn (8 glossy branches, 2 specular)
+	ymx_guide *guide;                // path guiding (NULL: BSDF sampling only)
+	ymx::PtGuide g;
+	uint32_t guide_cap;              // slots the guiding buffers hold
+	uint32_t train_steps;            // guide training steps after each wavefront
+	uint32_t g_seen, g_filled;       // training samples written so far (counter value), valid in the ring
+	uint64_t g_rng;                  // mini-batch offsets
 	uint32_t batch;                  // AA samples per wavefront
 	ymx::PtConst host_const;
 	ymx_pt_stats stats;
@@ -156,3 +184,9 @@ int ymx_pt_trace_pass(ymx_pt *pt, int32_t first_sample, int32_t count, hipStream
 // Grows the node pool to hold at least `need` nodes, keeping the first `keep`
 // (pt_api.hip).  Synchronous.
 int ymx_pt_grow_nodes(ymx_pt *pt, uint64_t need, uint32_t keep);
+
+// Guide training on the samples of the last wavefront (pt_kernels.hip).
+int ymx_pt_train_guide(ymx_pt *pt, hipStream_t stream);
+
+// Most samples one training step of the guide takes (guide.hip).
+uint32_t ymx_guide_batch_limit(const ymx_guide *g);

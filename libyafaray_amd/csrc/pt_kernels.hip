@@ -22,6 +22,7 @@
 // samples with the Gauss filter table in the order a row-major scatter would
 // add them (ImageFilm::addSample).  Compiled with -ffp-contract=off and IEEE
 // division: the shading keeps the reference's operation order (pt_math.h).
+#include "guide_math.h"
 #include "pt_internal.h"
 
 #include <float.h>
@@ -61,6 +62,7 @@ struct PtArgs
 	uint32_t recursive;    // children may be spawned from this chunk
 	int32_t sub;
 	int32_t depth;
+	PtGuide g;             // path guiding (g.on == 0: BSDF sampling only)
 };
 
 // What a kernel needs to know of the integrate() call behind slot i.
@@ -536,6 +538,100 @@ __device__ Rgb mat_emit(const PtArgs &a, const Surf &sp, V3 wo)
 	const PtMaterial &m = a.mats[sp.mat];
 	if(m.type == YMX_MAT_LIGHT && !m.double_sided && !(dot(wo, sp.n) > 0.f)) return rgb(0.f, 0.f, 0.f);
 	return m.emit_color;
+}
+
+// ---- path guiding (one-sample MIS of the BSDF sampler and the guide's mixture) ----
+
+__device__ __forceinline__ bool guided_material(const PtArgs &a, const Surf &sp)
+{
+	return a.g.on && a.mats[sp.mat].type == YMX_MAT_SHINYDIFFUSE && (sp.flags & kDiffuse);
+}
+
+// What the shinydiffuse sampler (sd_sample) returns as colour * weight had it
+// drawn wi, and the density with which it draws wi: the cosine lobe around the
+// face-forwarded normal, cos / pi.
+__device__ void sd_weight_density(const PtMaterial &m, const Surf &sp, V3 wo, V3 wi, Rgb &sw, float &ptrue)
+{
+	const V3 n = face_forward(sp.ng, sp.n, wo);
+	const float cos_n = dot(wi, n);
+	const Rgb scolor = (dot(sp.ng, wo) * dot(sp.ng, wi) > 0) ? cscale(m.color, m.diffuse_reflect) : rgb(0.f, 0.f, 0.f);
+	const float pdf = fabsf(cos_n);  // width 1: the diffuse component is the only one
+	const float w = fabsf(dot(wi, sp.n)) / (pdf * 0.99f + 0.01f);
+	sw = cscale(scolor, w);
+	ptrue = cos_n > 0.f ? cos_n * k1Pi : 0.f;
+}
+
+// Counter-based uniforms of the guided sampling, distinct from the roulette stream.
+__device__ __forceinline__ float guide_rand(const NodeCtx &x, uint32_t ps, uint32_t seg, uint32_t seed, uint32_t k)
+{
+	return rr_random(x.pixel ^ (0x632BE59Bu * (k + 1u)), ps, seg + 256u * x.tag + 0x10000u * (k + 1u),
+	                 seed ^ 0xA511E9B3u);
+}
+
+// Next direction at a guided vertex: the BSDF's own sample with probability
+// alpha, otherwise the mixture's; weight = F / (alpha p + (1 - alpha) q) with
+// F = sw * p the plain sampler's integrand.  Returns false when the weight is 0.
+__device__ bool guided_sample(const PtArgs &a, const Surf &sp, V3 wo, const float *mix, BSample &s, float u_sel,
+                              float u0, float u1, float u2, V3 &wi, Rgb &weight)
+{
+	const PtMaterial &m = a.mats[sp.mat];
+	const float alpha = a.g.bsdf_prob;
+	if(u_sel < alpha)
+	{
+		float w = 0.f;
+		(void)sd_sample(m, sp, wo, wi, s, w);
+	}
+	else
+	{
+		float x, y, z;
+		ymx::guide::mix_sample(mix, u0, u1, u2, x, y, z);
+		wi = normalize(v3(x, y, z));
+		s.sampled = kDiffuse | kReflect;
+		s.pdf = 0.f;
+	}
+	Rgb sw;
+	float p;
+	sd_weight_density(m, sp, wo, wi, sw, p);
+	const float q = ymx::guide::mix_pdf(mix, wi.x, wi.y, wi.z);
+	const float denom = alpha * p + (1.f - alpha) * q;
+	if(!(p > 0.f) || !(denom > 0.f))
+	{
+		weight = rgb(0.f, 0.f, 0.f);
+		return false;
+	}
+	weight = cscale(sw, p / denom);
+	return true;
+}
+
+__device__ __forceinline__ float luminance(Rgb c) { return 0.2126f * c.r + 0.7152f * c.g + 0.0722f * c.b; }
+
+// A pending training sample of segment r (position, normal, wo, direction) and
+// the throughput before it; the path colour before it is set in base.xyz.
+__device__ __forceinline__ void record_segment(const PtArgs &a, uint32_t slot, uint32_t r, const Surf &sp, V3 wo, V3 wi,
+                                               float tp_lum)
+{
+	ymx_guide_sample rec;
+	const V3 n = face_forward(sp.ng, sp.n, wo);
+	rec.pos[0] = sp.p.x;
+	rec.pos[1] = sp.p.y;
+	rec.pos[2] = sp.p.z;
+	rec.c = 0.f;
+	rec.normal[0] = n.x;
+	rec.normal[1] = n.y;
+	rec.normal[2] = n.z;
+	rec.pad0 = 0.f;
+	rec.wo[0] = wo.x;
+	rec.wo[1] = wo.y;
+	rec.wo[2] = wo.z;
+	rec.pad1 = 0.f;
+	rec.w[0] = wi.x;
+	rec.w[1] = wi.y;
+	rec.w[2] = wi.z;
+	rec.pad2 = 0.f;
+	const size_t k = (size_t)r * a.cnt + slot;
+	a.g.rec[k] = rec;
+	a.g.base[k].w = tp_lum;
+	a.g.mask[slot] |= 1u << r;
 }
 
 // ShapePolygon<float, 4>::intersect, include/geometry/shape/shape_polygon.h:126-176
@@ -1063,6 +1159,7 @@ __global__ __launch_bounds__(kBlock) void k_subpath_init(PtArgs a)
 	uint32_t f = a.st.flags[i];
 	f &= ~(kActive | kAddEmit | kCaustic | kDirect | kVertex | kSampled);
 	a.st.flags[i] = f;
+	if(a.g.train) a.g.mask[i] = 0u;
 	if(!(f & kPathBlock)) return;
 	const NodeCtx x = node_ctx(a, i);
 	if(a.sub >= n_subpaths(c, x.division)) return;
@@ -1090,8 +1187,29 @@ __global__ __launch_bounds__(kBlock) void k_subpath_init(PtArgs a)
 	s.sampled = kNone;
 	float w = 0.f;
 	V3 pdir = v3(0.f, 0.f, 0.f);
-	Rgb scol = bsdf_sample(a, sp, wo, pdir, s, w);
-	scol = cscale(scol, w);
+	Rgb scol;
+	if(guided_material(a, sp))
+	{
+		const uint32_t ps = x.sample * (uint32_t)c.path_samples + (uint32_t)a.sub;
+		guided_sample(a, sp, wo, a.g.mix_node + (size_t)i * ymx::guide::kMix, s, guide_rand(x, ps, 0, c.seed, 0),
+		              guide_rand(x, ps, 0, c.seed, 1), guide_rand(x, ps, 0, c.seed, 2), guide_rand(x, ps, 0, c.seed, 3),
+		              pdir, scol);
+		w = 1.f;
+		if(a.g.train)
+		{
+			// the path colour before this sub-path, throughput 1
+			record_segment(a, i, 0, sp, wo, pdir, 1.f);
+			const float4 pc = a.st.path_col[i];
+			a.g.base[i].x = pc.x;
+			a.g.base[i].y = pc.y;
+			a.g.base[i].z = pc.z;
+		}
+	}
+	else
+	{
+		scol = bsdf_sample(a, sp, wo, pdir, s, w);
+		scol = cscale(scol, w);
+	}
 	// w rides along in tp_next.w: a later sample call that leaves w unset reuses it
 	a.st.tp_next[i] = make_float4(scol.r, scol.g, scol.b, w);
 	a.st.pwo[i] = make_float4(wo.x, wo.y, wo.z, 0.f);
@@ -1195,10 +1313,25 @@ __global__ __launch_bounds__(kBlock) void k_shade_path(PtArgs a)
 		// diffuse component returns white) continues straight on with the last w.
 		float w = tpn.w;
 		V3 ndir = dir;
-		Rgb scol = bsdf_sample(a, sp, pwo, ndir, s, w);
-		scol = cscale(scol, w);
+		Rgb scol;
+		const bool guided = guided_material(a, sp);
+		if(guided)
+		{
+			const uint32_t ps = x.sample * (uint32_t)c.path_samples + (uint32_t)a.sub;
+			const uint32_t seg = (uint32_t)d + 1u;
+			guided_sample(a, sp, pwo, a.g.mix_path + (size_t)q * ymx::guide::kMix, s, guide_rand(x, ps, seg, c.seed, 0),
+			              guide_rand(x, ps, seg, c.seed, 1), guide_rand(x, ps, seg, c.seed, 2),
+			              guide_rand(x, ps, seg, c.seed, 3), ndir, scol);
+			w = 1.f;
+		}
+		else
+		{
+			scol = bsdf_sample(a, sp, pwo, ndir, s, w);
+			scol = cscale(scol, w);
+		}
 		if(!cblack(scol))
 		{
+			if(guided && a.g.train && !cblack(tp)) record_segment(a, i, (uint32_t)d + 1u, sp, pwo, ndir, luminance(tp));
 			const Rgb tpn2 = cmul(tp, scol);
 			a.st.tp_next[i] = make_float4(tpn2.r, tpn2.g, tpn2.b, w);
 			if(s.sampled & (kSpecular | kGlossy | kFilter)) f |= kCaustic;
@@ -1241,6 +1374,111 @@ __global__ __launch_bounds__(kBlock) void k_resolve_path(PtArgs a)
 	const Rgb pc = cadd(rgb(p4.x, p4.y, p4.z), cmul(lcol, rgb(t4.x, t4.y, t4.z)));
 	a.st.path_col[i] = make_float4(pc.r, pc.g, pc.b, 0.f);
 	a.st.flags[i] = f & ~(kDirect | kVertex | kAddEmit);
+	if(a.g.train && ((a.g.mask[i] >> (a.depth + 1)) & 1u))
+	{
+		// the segment sampled at this vertex starts after the vertex's own light
+		float4 *b = a.g.base + (size_t)(a.depth + 1) * a.cnt + i;
+		b->x = pc.r;
+		b->y = pc.g;
+		b->z = pc.b;
+	}
+}
+
+// After a sub-path: each recorded segment's training weight is the radiance
+// that reached the path through it, relative to the throughput before it
+// (an estimate of f(w) / p(w) for the guide's loss).  A fraction a.g.accept
+// of them is kept.
+__global__ __launch_bounds__(kBlock) void k_guide_collect(PtArgs a)
+{
+	const PtConst &c = *a.c;
+	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if(i >= a.cnt) return;
+	const uint32_t mask = a.g.mask[i];
+	if(!mask) return;
+	a.g.mask[i] = 0u;
+	const NodeCtx x = node_ctx(a, i);
+	const float4 p4 = a.st.path_col[i];
+	for(int r = 0; r < c.bounces; ++r)
+	{
+		if(!((mask >> r) & 1u)) continue;
+		const size_t k = (size_t)r * a.cnt + i;
+		const float4 b = a.g.base[k];
+		const float cw = luminance(rgb(p4.x - b.x, p4.y - b.y, p4.z - b.z)) / b.w;
+		if(!(cw > 0.f) || !(cw < 1e30f)) continue;
+		const uint32_t ps = x.sample * (uint32_t)c.path_samples + (uint32_t)a.sub;
+		if(guide_rand(x, ps, (uint32_t)r, c.seed, 7) >= a.g.accept) continue;
+		// ring buffer of the most recent samples (train_cap is a power of two)
+		const uint32_t slot = atomicAdd(a.g.train_count, 1u) & (a.g.train_cap - 1u);
+		ymx_guide_sample rec = a.g.rec[k];
+		rec.c = cw;
+		a.g.train_buf[slot] = rec;
+	}
+}
+
+// Queries of the guide at the node vertices that start sub-paths.
+__global__ __launch_bounds__(kBlock) void k_guide_query_node(PtArgs a)
+{
+	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if(i >= a.cnt) return;
+	ymx_guide_sample qr;
+	memset(&qr, 0, sizeof qr);
+	if(a.st.flags[i] & kPathBlock)
+	{
+		const ymx_hit h = a.st.cam_hit[i];
+		V3 from, dir;
+		load_ray(a, a.base + i, from, dir);
+		Surf sp;
+		make_surf(a, from, dir, h, sp);
+		if(guided_material(a, sp))
+		{
+			const V3 wo = neg(dir);
+			const V3 n = face_forward(sp.ng, sp.n, wo);
+			qr.pos[0] = sp.p.x;
+			qr.pos[1] = sp.p.y;
+			qr.pos[2] = sp.p.z;
+			qr.normal[0] = n.x;
+			qr.normal[1] = n.y;
+			qr.normal[2] = n.z;
+			qr.wo[0] = wo.x;
+			qr.wo[1] = wo.y;
+			qr.wo[2] = wo.z;
+		}
+	}
+	a.g.query[i] = qr;
+}
+
+// Queries of the guide at the path vertices of segment a.depth (queue order).
+__global__ __launch_bounds__(kBlock) void k_guide_query_path(PtArgs a)
+{
+	const int qsel = a.depth & 1;
+	const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+	if(q >= a.st.ext_count[qsel]) return;
+	ymx_guide_sample qr;
+	memset(&qr, 0, sizeof qr);
+	const ymx_hit h = a.st.ext_hits[q];
+	if(h.prim != YMX_NO_HIT)
+	{
+		const float4 *rr = reinterpret_cast<const float4 *>(a.st.ext_rays[qsel] + q);
+		const float4 ro = rr[0], rd = rr[1];
+		const V3 from = v3(ro.x, ro.y, ro.z), dir = v3(rd.x, rd.y, rd.z);
+		Surf sp;
+		make_surf(a, from, dir, h, sp);
+		if(guided_material(a, sp))
+		{
+			const V3 wo = neg(dir);
+			const V3 n = face_forward(sp.ng, sp.n, wo);
+			qr.pos[0] = sp.p.x;
+			qr.pos[1] = sp.p.y;
+			qr.pos[2] = sp.p.z;
+			qr.normal[0] = n.x;
+			qr.normal[1] = n.y;
+			qr.normal[2] = n.z;
+			qr.wo[0] = wo.x;
+			qr.wo[1] = wo.y;
+			qr.wo[2] = wo.z;
+		}
+	}
+	a.g.query[q] = qr;
 }
 
 __global__ __launch_bounds__(kBlock) void k_finish(PtArgs a)
@@ -1365,6 +1603,13 @@ int trace_chunk(ymx_pt *pt, PtArgs &a, hipStream_t stream)
 	}
 	k_resolve_node<<<g, kBlock, 0, stream>>>(a);
 	YMX_HIP_CHECK(hipGetLastError());
+	if(a.g.on)
+	{
+		k_guide_query_node<<<g, kBlock, 0, stream>>>(a);
+		YMX_HIP_CHECK(hipGetLastError());
+		rc = ymx_guide_infer(pt->guide, a.g.query, a.g.mix_node, n, nullptr, stream);
+		if(rc != YMX_OK) return rc;
+	}
 	const int n_sub = hc.path_samples > 1 ? hc.path_samples : 1;
 	for(int sub = 0; sub < n_sub; ++sub)
 	{
@@ -1378,6 +1623,13 @@ int trace_chunk(ymx_pt *pt, PtArgs &a, hipStream_t stream)
 			rc = ymx_launch_closest_queue(pt->accel, pt->st.ext_rays[qsel], pt->st.ext_hits, pt->st.ext_count + qsel, n,
 			                              stream);
 			if(rc != YMX_OK) return rc;
+			if(a.g.on && d + 1 < hc.bounces)
+			{
+				k_guide_query_path<<<g, kBlock, 0, stream>>>(a);
+				YMX_HIP_CHECK(hipGetLastError());
+				rc = ymx_guide_infer(pt->guide, a.g.query, a.g.mix_path, n, pt->st.ext_count + qsel, stream);
+				if(rc != YMX_OK) return rc;
+			}
 			k_shade_path<<<g, kBlock, 0, stream>>>(a);
 			YMX_HIP_CHECK(hipGetLastError());
 			if(hc.n_lights > 0)
@@ -1386,6 +1638,11 @@ int trace_chunk(ymx_pt *pt, PtArgs &a, hipStream_t stream)
 				if(rc != YMX_OK) return rc;
 			}
 			k_resolve_path<<<g, kBlock, 0, stream>>>(a);
+			YMX_HIP_CHECK(hipGetLastError());
+		}
+		if(a.g.train)
+		{
+			k_guide_collect<<<g, kBlock, 0, stream>>>(a);
 			YMX_HIP_CHECK(hipGetLastError());
 		}
 	}
@@ -1426,11 +1683,12 @@ int ymx_pt_trace_pass(ymx_pt *pt, int32_t first_sample, int32_t count, hipStream
 	a.count = count;
 	a.sub = 0;
 	a.depth = 0;
+	a.g = pt->g;
 	k_camera<<<blocks_for(n), kBlock, 0, stream>>>(a);
 	YMX_HIP_CHECK(hipGetLastError());
 	// levels of integrate() calls, node ranges [start, start + size)
 	std::vector<uint32_t> lvl_start{0u}, lvl_size{n};
-	const uint32_t cap = pt->st.capacity;
+	const uint32_t cap = pt->g.on && pt->guide_cap < pt->st.capacity ? pt->guide_cap : pt->st.capacity;
 	for(int level = 0;; ++level)
 	{
 		const uint32_t start = lvl_start[level], size = lvl_size[level];
@@ -1481,5 +1739,40 @@ int ymx_pt_trace_pass(ymx_pt *pt, int32_t first_sample, int32_t count, hipStream
 	}
 	k_film<<<blocks_for(npix), kBlock, 0, stream>>>(a);
 	YMX_HIP_CHECK(hipGetLastError());
+	if(pt->g.train) return ymx_pt_train_guide(pt, stream);
+	return YMX_OK;
+}
+
+// Adam steps of the guide on mini-batches drawn from the ring buffer of recent
+// samples, then the acceptance probability that lets one wavefront fill about a
+// quarter of the ring (so it holds the last few wavefronts).
+int ymx_pt_train_guide(ymx_pt *pt, hipStream_t stream)
+{
+	uint32_t total = 0;
+	YMX_HIP_CHECK(hipMemcpyAsync(&total, pt->g.train_count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+	YMX_HIP_CHECK(hipStreamSynchronize(stream));
+	const uint32_t kept = total - pt->g_seen;  // this wavefront's samples (mod 2^32)
+	pt->g_seen = total;
+	pt->g_filled = pt->g_filled + kept < pt->g.train_cap ? pt->g_filled + kept : pt->g.train_cap;
+	const uint32_t n = pt->g_filled;
+	const uint32_t max_batch = ymx_guide_batch_limit(pt->guide);
+	pt->stats.guide_samples += (uint64_t)(kept / (double)(pt->g.accept > 0.f ? pt->g.accept : 1.f));
+	if(n > 0 && pt->train_steps > 0)
+	{
+		pt->stats.guide_steps += pt->train_steps;
+		const uint32_t batch = n < max_batch ? n : max_batch;
+		for(uint32_t k = 0; k < pt->train_steps; ++k)
+		{
+			// a contiguous slice at a pseudo-random offset (the ring's order is the GPU's)
+			pt->g_rng = pt->g_rng * 6364136223846793005ull + 1442695040888963407ull;
+			const uint32_t off = n > batch ? (uint32_t)((pt->g_rng >> 33) % (uint64_t)(n - batch + 1)) : 0u;
+			const int rc = ymx_guide_train_step(pt->guide, pt->g.train_buf + off, batch, nullptr, stream);
+			if(rc != YMX_OK) return rc;
+		}
+	}
+	const double records = kept / (double)(pt->g.accept > 0.f ? pt->g.accept : 1.f);
+	double acc = records > 0 ? 0.25 * pt->g.train_cap / records : 1.0;
+	acc = acc > 1.0 ? 1.0 : (acc < 1e-4 ? 1e-4 : acc);
+	pt->g.accept = (float)acc;
 	return YMX_OK;
 }

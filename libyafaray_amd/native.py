@@ -46,7 +46,8 @@ class YmxAccelInfo(C.Structure):
 
 class YmxPtStats(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("camera_rays", C.c_uint64), ("extension_rays", C.c_uint64),
-                ("shadow_rays", C.c_uint64), ("render_ms", C.c_float)]
+                ("shadow_rays", C.c_uint64), ("render_ms", C.c_float), ("guide_samples", C.c_uint64),
+                ("guide_steps", C.c_uint64)]
 
 
 # Every symbol include/ymx_accel.h and include/ymx_pt.h declare, with its ctypes signature.
@@ -59,6 +60,7 @@ SIGNATURES = {
     "ymx_pt_render": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
     "ymx_pt_read_film": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "ymx_pt_get_stats": (C.c_int, [C.c_void_p, C.POINTER(YmxPtStats)]),
+    "ymx_pt_set_guide": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_int32, C.c_uint32]),
     "ymx_accel_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
                                    C.POINTER(YmxAccelParams), C.POINTER(C.c_void_p)]),
     "ymx_accel_destroy": (C.c_int, [C.c_void_p]),

@@ -150,6 +150,14 @@ class GpuPathTracer:
     def clear(self):
         check(self._lib.ymx_pt_clear_film(self._h, self._stream()), "ymx_pt_clear_film")
 
+    def set_guide(self, guide, bsdf_prob: float = 0.5, train: bool = False, train_steps: int = 4):
+        """Mix a GpuGuide into the shinydiffuse next-direction sampling (one-sample
+        MIS); with train, each wavefront's paths train it.  guide None: off."""
+        h = guide.handle if guide is not None else None
+        check(self._lib.ymx_pt_set_guide(self._h, h, float(bsdf_prob), int(bool(train)), int(train_steps)),
+              "ymx_pt_set_guide")
+        self._guide = guide  # keep it alive while set
+
     def render(self, first_sample: int = 0, n_samples: int | None = None):
         n = self.st.aa_samples if n_samples is None else n_samples
         check(self._lib.ymx_pt_render(self._h, int(first_sample), int(n), self._stream()), "ymx_pt_render")
@@ -165,4 +173,4 @@ class GpuPathTracer:
         s.struct_size = C.sizeof(YmxPtStats)
         check(self._lib.ymx_pt_get_stats(self._h, C.byref(s)), "ymx_pt_get_stats")
         return {"camera_rays": s.camera_rays, "extension_rays": s.extension_rays, "shadow_rays": s.shadow_rays,
-                "render_ms": s.render_ms}
+                "render_ms": s.render_ms, "guide_samples": s.guide_samples, "guide_steps": s.guide_steps}

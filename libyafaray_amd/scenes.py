@@ -333,3 +333,86 @@ def cornell_materials_scene(n_random: int = 10_000, seed: int = 0, res: int = 25
                     background=(0.1, 0.1, 0.1), seed=0, raydepth=raydepth)
     return dict(verts=verts, tris=tris, mat_id=mat_id, materials=materials, lights=lights, camera=camera,
                 settings=settings)
+
+
+def grid_quad_tris(c, e1, e2, n1: int, n2: int, skip=None) -> tuple[np.ndarray, np.ndarray]:
+    """Quad c + [0,1] e1 + [0,1] e2 tessellated into n1 x n2 cells (2 triangles
+    each, face normal e1 x e2); skip(i, j) -> True leaves cell (i, j) out."""
+    c, e1, e2 = (np.asarray(v, dtype=np.float64) for v in (c, e1, e2))
+    i = np.arange(n1 + 1)[:, None, None] / n1
+    j = np.arange(n2 + 1)[None, :, None] / n2
+    verts = (c + i * e1 + j * e2).reshape(-1, 3).astype(F32)
+    cells = [(a, b) for a in range(n1) for b in range(n2) if skip is None or not skip(a, b)]
+    ii = np.array([a for a, _ in cells], dtype=np.int64)
+    jj = np.array([b for _, b in cells], dtype=np.int64)
+    v00 = ii * (n2 + 1) + jj
+    v10 = (ii + 1) * (n2 + 1) + jj
+    v11 = (ii + 1) * (n2 + 1) + jj + 1
+    v01 = ii * (n2 + 1) + jj + 1
+    tris = np.concatenate([np.stack([v00, v10, v11], -1), np.stack([v00, v11, v01], -1)]).astype(np.uint32)
+    return verts, tris
+
+
+def occluded_room_scene(res=(1280, 720), spp: int = 16, seed: int = 0, wall_cells: int = 180,
+                        n_boxes: int = 150, bounces: int = 10, slit: float = 0.1):
+    """Learned-guiding workload: a 4 x 3 x 3 closed room (walls tessellated, plus
+    seeded box furniture, about 500k triangles) lit only through a `slit`-wide
+    door slit from an adjacent 2 x 3 x 3 room holding a 1 x 1 ceiling area light
+    of radiance 50.  All surfaces Lambertian; camera in the main room."""
+    rng = np.random.default_rng(seed)
+    parts, mats = [], []
+    W, D, H = 4.0, 3.0, 3.0          # main room: x in [0, 4], y in [0, 3], z in [0, 3]
+    n = wall_cells
+    inward = [  # (corner, e1, e2) with e1 x e2 pointing into the room
+        ((0, 0, 0), (0, D, 0), (W, 0, 0)),        # floor (normal +z)
+        ((0, 0, H), (W, 0, 0), (0, D, 0)),        # ceiling (-z)
+        ((0, 0, 0), (W, 0, 0), (0, 0, H)),        # y = 0 wall (+y)
+        ((0, D, 0), (0, 0, H), (W, 0, 0)),        # y = D wall (-y)
+        ((0, 0, 0), (0, 0, H), (0, D, 0)),        # x = 0 wall (+x)
+    ]
+    for c, e1, e2 in inward:
+        parts.append(grid_quad_tris(c, e1, e2, n, n))
+        mats.append(0)
+    # dividing wall x = W with a door slit: y in [1.45, 1.45 + slit], z in [0, 2]
+    y0, y1 = 1.5 - slit / 2, 1.5 + slit / 2
+    nd = n
+    def in_slit(i, j):
+        ya, yb = i * D / nd, (i + 1) * D / nd
+        za = j * H / nd
+        return yb > y0 and ya < y1 and za < 2.0
+    # the slit edges must fall on cell boundaries: cells of D / nd wide
+    parts.append(grid_quad_tris((W, 0, 0), (0, D, 0), (0, 0, H), nd, nd, skip=in_slit))
+    mats.append(0)
+    # adjacent room x in [W, W + 2]
+    W2 = 2.0
+    m = max(8, (2 * n) // 3)
+    adj = [
+        ((W, 0, 0), (0, D, 0), (W2, 0, 0)),
+        ((W, 0, H), (W2, 0, 0), (0, D, 0)),
+        ((W, 0, 0), (W2, 0, 0), (0, 0, H)),
+        ((W, D, 0), (0, 0, H), (W2, 0, 0)),
+        ((W + W2, 0, 0), (0, D, 0), (0, 0, H)),
+    ]
+    for c, e1, e2 in adj:
+        parts.append(grid_quad_tris(c, e1, e2, m, m))
+        mats.append(0)
+    # box furniture in the main room, clear of the camera: most on the floor, some shelves
+    for k in range(n_boxes):
+        sx, sy, sz = rng.uniform(0.1, 0.6, 3)
+        x0 = rng.uniform(1.0, W - 0.3 - sx)
+        y_ = rng.uniform(0.1, D - 0.1 - sy)
+        z0 = 0.0 if rng.uniform() < 0.8 else rng.uniform(0.5, 2.0)
+        parts.append(box_tris((x0, y_, z0 + 1.0 / 512), (x0 + sx, y_ + sy, z0 + sz)))
+        mats.append(1 + k)
+    verts, tris = merge(parts)
+    mat_id = np.concatenate([np.full(len(t), mm, np.uint32) for (_, t), mm in zip(parts, mats)])
+    albedo = rng.uniform(0.3, 0.8, n_boxes)
+    materials = [dict(color=(0.7, 0.7, 0.7))] + [dict(color=(a, 0.9 * a, 0.8 * a)) for a in albedo]
+    lights = [dict(type="area", corner=(W + 0.5, 1.0, H - 0.001), point1=(W + 0.5, 2.0, H - 0.001),
+                   point2=(W + 1.5, 1.0, H - 0.001), color=(1.0, 1.0, 1.0), power=50.0 / np.pi, samples=1)]
+    camera = dict(from_=(0.4, 0.6, 1.7), to=(4.0, 1.8, 0.9), up=(0.4, 0.6, 2.7),
+                  focal=float(1.0 / (2.0 * np.tan(np.radians(35.0)))), resx=int(res[0]), resy=int(res[1]))
+    settings = dict(path_samples=1, bounces=bounces, rr_min_bounces=3, aa_samples=spp, aa_pixelwidth=1.5,
+                    background=None, seed=seed, raydepth=0)
+    return dict(verts=verts, tris=tris, mat_id=mat_id, materials=materials, lights=lights, camera=camera,
+                settings=settings)
