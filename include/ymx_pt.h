@@ -23,12 +23,14 @@
  * glass (no dispersion or absorption) and light_mat; specular and glossy
  * (as_diffuse = 0) surfaces recurse as the reference's recursiveRaytrace does,
  * up to `raydepth`.  Lights: area (corner / point1 / point2, colour, power,
- * samples) and point (from, colour, power).  Param meanings and defaults follow
+ * samples), point (from, colour, power) and object (a mesh of the scene: a
+ * triangle range, colour, power, samples, double_sided).  Param meanings and defaults follow
  * the reference's ParamMaps (include/material/material_shiny_diffuse.h:85-96,
  * include/material/material_glossy.h:68-80, include/material/material_mirror.h:57-58,
  * include/material/material_glass.h:68-75, include/material/material_light.h:59-61,
  * include/integrator/surface/integrator_montecarlo.h:47,
  * include/light/light_area.h:50-57, include/light/light_point.h:52-54,
+ * include/light/light_object_light.h:53-61,
  * include/integrator/surface/integrator_path_tracer.h:55-58,
  * include/render/imagefilm.h:196-225, include/camera/camera.h:82-89).
  */
@@ -51,6 +53,7 @@ extern "C" {
 #define YMX_MAT_LIGHT 4u        /* "light_mat" */
 #define YMX_LIGHT_AREA 0u
 #define YMX_LIGHT_POINT 1u
+#define YMX_LIGHT_OBJECT 2u     /* "objectlight" */
 
 /* One record for every material type; each type reads its own fields, named
  * after the reference's params (defaults in brackets). */
@@ -82,8 +85,12 @@ typedef struct ymx_light
 	float from[3];                          /* point light position */
 	float color[3];
 	float power;
-	int32_t samples;                        /* area light samples, default 4 */
+	int32_t samples;                        /* area / object light samples, default 4 */
 	uint32_t cast_shadows;                  /* default 1 */
+	/* object light: its object ("object_name") is the triangle range
+	 * [first_tri, first_tri + n_tris) of the path tracer's triangles */
+	uint32_t first_tri, n_tris;
+	uint32_t double_sided;                  /* object light "double_sided", default 0 */
 } ymx_light;
 
 typedef struct ymx_pt_settings
@@ -107,9 +114,11 @@ typedef struct ymx_pt ymx_pt;
 
 /* Create a path tracer over an accelerator built from the same triangles.
  * verts/tri_idx: host arrays as given to ymx_accel_create (for face normals);
- * mat_id: host, one material index per triangle. */
+ * mat_id: host, one material index per triangle; vis: host, the visibility
+ * masks given to ymx_accel_create (NULL: all visible and casting shadows), read by object
+ * lights, whose own intersection test sees their YMX_VIS_VISIBLE triangles only. */
 int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, const uint32_t *tri_idx,
-                  const uint32_t *mat_id, uint32_t n_tris, const ymx_material *materials, uint32_t n_materials,
+                  const uint32_t *mat_id, const uint8_t *vis, uint32_t n_tris, const ymx_material *materials, uint32_t n_materials,
                   const ymx_light *lights, uint32_t n_lights, const ymx_pt_settings *settings, ymx_pt **out);
 
 int ymx_pt_destroy(ymx_pt *pt);

@@ -84,7 +84,7 @@ void free_pt(ymx_pt *pt)
 	if(!pt) return;
 	free_guide_buffers(pt->g);
 	free_nodes(pt->nd);
-	void *ptrs[] = {pt->d_prim, pt->d_materials, pt->d_lights, pt->d_const, pt->d_faure, pt->d_inv_prims, pt->d_prims,
+	void *ptrs[] = {pt->d_prim, pt->d_materials, pt->d_lights, pt->d_ltris, pt->d_const, pt->d_faure, pt->d_inv_prims, pt->d_prims,
 	                pt->d_film, pt->d_film_w, pt->st.path_col, pt->st.tp_next, pt->st.tp_vertex, pt->st.pwo,
 	                pt->st.cam_hit, pt->st.dxy, pt->st.flags, pt->st.light, pt->st.vert_emit, pt->st.ext_rays[0],
 	                pt->st.ext_rays[1], pt->st.ext_path[0], pt->st.ext_path[1], pt->st.ext_count, pt->st.ext_hits,
@@ -143,6 +143,70 @@ void filter_table(float pixelwidth, PtConst &c)
 	c.filter_width = fw;
 }
 
+// ObjectLight::initIs (src/light/light_object_light.cc:87-104) for the
+// triangles idx[0 .. 3 n): each triangle's area 0.5 |e1 x e2|
+// (include/geometry/shape/shape_polygon.h:64-76), the Pdf1D cdf of the areas
+// (include/sampler/sample_pdf1d.h:60-81: a double running sum of area / n,
+// rounded to float, divided by the float total) and the tree bound of the
+// light's own kd-tree (src/accelerator/accelerator_kdtree_original.cc:89-103:
+// the vertex bound grown by 0.001 of its extent on each axis).  Appends the
+// triangles to `out`; returns area_, the double sum of the float areas.
+float object_light_tables(const float *verts, const uint32_t *idx, uint32_t n, const uint8_t *vis,
+                          const float4 *face_normals, std::vector<PtLightTri> &out, float bound_a[3], float bound_g[3])
+{
+	std::vector<float> areas(n);
+	double total = 0.0;
+	float a[3] = {0.f, 0.f, 0.f}, g[3] = {0.f, 0.f, 0.f};
+	for(uint32_t i = 0; i < n; ++i)
+	{
+		const V3 v[3] = {load3(verts + 3 * (size_t)idx[3 * (size_t)i]), load3(verts + 3 * (size_t)idx[3 * (size_t)i + 1]),
+		                 load3(verts + 3 * (size_t)idx[3 * (size_t)i + 2])};
+		const V3 cr = cross(sub(v[1], v[0]), sub(v[2], v[0]));
+		areas[i] = 0.5f * sqrtf(dot(cr, cr));
+		total += (double)areas[i];
+		for(int k = 0; k < 3; ++k)
+		{
+			const float p[3] = {v[k].x, v[k].y, v[k].z};
+			for(int ax = 0; ax < 3; ++ax)
+			{
+				if(i == 0 && k == 0) a[ax] = g[ax] = p[ax];
+				else
+				{
+					if(p[ax] < a[ax]) a[ax] = p[ax];
+					if(p[ax] > g[ax]) g[ax] = p[ax];
+				}
+			}
+		}
+		PtLightTri t;
+		const uint32_t vm = vis ? (uint32_t)vis[i] : (uint32_t)(YMX_VIS_VISIBLE | YMX_VIS_CASTS_SHADOWS);
+		float vf;
+		memcpy(&vf, &vm, 4);
+		t.v0_cdf = make_float4(v[0].x, v[0].y, v[0].z, 0.f);
+		t.v1_vis = make_float4(v[1].x, v[1].y, v[1].z, vf);
+		t.v2 = make_float4(v[2].x, v[2].y, v[2].z, 0.f);
+		t.n = make_float4(face_normals[i].x, face_normals[i].y, face_normals[i].z, 0.f);
+		out.push_back(t);
+	}
+	const size_t first = out.size() - n;
+	const double delta = 1.0 / (double)n;
+	double acc = 0.0;
+	std::vector<float> cdf(n);
+	for(uint32_t i = 0; i < n; ++i)
+	{
+		acc += (double)areas[i] * delta;
+		cdf[i] = (float)acc;
+	}
+	const float integral = (float)acc;
+	for(uint32_t i = 0; i < n; ++i) out[first + i].v0_cdf.w = cdf[i] / integral;
+	for(int ax = 0; ax < 3; ++ax)
+	{
+		const double offset = (double)(g[ax] - a[ax]) * 0.001;
+		bound_a[ax] = a[ax] - (float)offset;
+		bound_g[ax] = g[ax] + (float)offset;
+	}
+	return (float)total;
+}
+
 }  // namespace
 
 int ymx_pt_grow_nodes(ymx_pt *pt, uint64_t need, uint32_t keep)
@@ -181,7 +245,8 @@ int ymx_pt_grow_nodes(ymx_pt *pt, uint64_t need, uint32_t keep)
 extern "C" {
 
 int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, const uint32_t *tri_idx,
-                  const uint32_t *mat_id, uint32_t n_tris, const ymx_material *materials, uint32_t n_materials,
+                  const uint32_t *mat_id, const uint8_t *vis, uint32_t n_tris, const ymx_material *materials,
+                  uint32_t n_materials,
                   const ymx_light *lights, uint32_t n_lights, const ymx_pt_settings *st, ymx_pt **out)
 {
 	if(!out || !accel || !st || (n_tris && (!verts || !tri_idx || !mat_id)) || (n_materials && !materials) ||
@@ -225,13 +290,20 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 			return YMX_ERR_UNSUPPORTED;
 		}
 	for(uint32_t l = 0; l < n_lights; ++l)
-		if((lights[l].type != YMX_LIGHT_AREA && lights[l].type != YMX_LIGHT_POINT) ||
-		   (lights[l].type == YMX_LIGHT_AREA && (lights[l].samples < 1 || lights[l].samples > 64)))
+	{
+		const ymx_light &s = lights[l];
+		if(s.type > YMX_LIGHT_OBJECT || (s.type != YMX_LIGHT_POINT && (s.samples < 1 || s.samples > 64)))
 		{
-			ymx_set_error("ymx_pt_create: light %u unsupported (type %u, samples %d)", l, lights[l].type,
-			              lights[l].samples);
+			ymx_set_error("ymx_pt_create: light %u unsupported (type %u, samples %d)", l, s.type, s.samples);
 			return YMX_ERR_UNSUPPORTED;
 		}
+		if(s.type == YMX_LIGHT_OBJECT && (s.n_tris == 0 || s.first_tri > n_tris || s.n_tris > n_tris - s.first_tri))
+		{
+			ymx_set_error("ymx_pt_create: object light %u covers triangles [%u, %u + %u) of %u", l, s.first_tri,
+			              s.first_tri, s.n_tris, n_tris);
+			return YMX_ERR_INVALID_ARG;
+		}
+	}
 
 	ymx_pt *pt = new(std::nothrow) ymx_pt();
 	if(!pt)
@@ -332,8 +404,10 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 	PT_TRY(dalloc(pt->d_materials, mats.size()));
 	PT_TRY(hipMemcpy(pt->d_materials, mats.data(), sizeof(PtMaterial) * mats.size(), hipMemcpyHostToDevice));
 
-	// lights (src/light/light_area.cc, include/light/light_area.h:70-80, include/light/light_point.h:64)
+	// lights (src/light/light_area.cc, include/light/light_area.h:70-80, include/light/light_point.h:64,
+	// src/light/light_object_light.cc:87-104)
 	std::vector<PtLight> lts(n_lights ? n_lights : 1);
+	std::vector<PtLightTri> ltris;
 	uint32_t slots_camera = 0, slots_path = 0;
 	for(uint32_t l = 0; l < n_lights; ++l)
 	{
@@ -359,17 +433,30 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 			d.quad[2] = sub(add(p1, p2), c0);
 			d.quad[3] = p2;
 		}
+		else if(s.type == YMX_LIGHT_OBJECT)
+		{
+			d.color = cscale(cp, kPi);
+			d.double_sided = s.double_sided ? 1u : 0u;
+			d.first = (uint32_t)ltris.size();
+			d.count = s.n_tris;
+			d.area = object_light_tables(verts, tri_idx + 3 * (size_t)s.first_tri, s.n_tris,
+			                             vis ? vis + s.first_tri : nullptr, prim.data() + s.first_tri, ltris,
+			                             d.bound_a, d.bound_g);
+		}
 		else
 		{
 			d.color = cp;
 			d.from = load3(s.from);
 		}
-		const uint32_t k = s.type == YMX_LIGHT_AREA ? 2u * (uint32_t)s.samples : 1u;
+		const uint32_t k = s.type == YMX_LIGHT_POINT ? 1u : 2u * (uint32_t)s.samples;
 		slots_camera += k;
 		slots_path = k > slots_path ? k : slots_path;
 	}
 	PT_TRY(dalloc(pt->d_lights, lts.size()));
 	PT_TRY(hipMemcpy(pt->d_lights, lts.data(), sizeof(PtLight) * lts.size(), hipMemcpyHostToDevice));
+	if(ltris.empty()) ltris.resize(1);
+	PT_TRY(dalloc(pt->d_ltris, ltris.size()));
+	PT_TRY(hipMemcpy(pt->d_ltris, ltris.data(), sizeof(PtLightTri) * ltris.size(), hipMemcpyHostToDevice));
 
 	// constants: camera (src/camera/camera.cc:89-104, camera_perspective.cc:97-111)
 	PtConst &c = pt->host_const;

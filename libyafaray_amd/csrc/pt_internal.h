@@ -50,11 +50,24 @@ struct PtLight
 	uint32_t type;
 	uint32_t cast_shadows;
 	int32_t samples;
-	float area;           // normal_flipped_.normalized().length()
+	float area;           // area light: normal_flipped_.normalized().length(); object light: its surface area
 	V3 corner, to_x, to_y, normal_flipped;
 	V3 quad[4];           // corner, point1, point1 + point2 - corner, point2
 	V3 from;              // point light
-	Rgb color;            // colour * power (* pi for area lights)
+	Rgb color;            // colour * power (* pi for area and object lights)
+	// object light: its triangles are ltris[first, first + count), and the
+	// tree bound of its own kd-tree (include/accelerator/accelerator_kdtree_original.cc)
+	uint32_t first, count;
+	uint32_t double_sided;
+	float bound_a[3], bound_g[3];
+};
+
+// One triangle of an object light (64 B): v0 and its cumulative area
+// fraction (the light's Pdf1D cdf), v1 and its visibility mask, v2, and the
+// face normal.
+struct PtLightTri
+{
+	float4 v0_cdf, v1_vis, v2, n;
 };
 
 constexpr int kMaxLights = 512;
@@ -155,6 +168,7 @@ struct ymx_pt
 	float4 *d_prim;                  // (face normal xyz, material id bits) per primitive
 	ymx::PtMaterial *d_materials;
 	ymx::PtLight *d_lights;
+	ymx::PtLightTri *d_ltris;        // triangles of the object lights
 	ymx::PtConst *d_const;
 	int32_t *d_faure;                // [kLdsDims][kLdsMaxBase] scrambling permutations
 	double *d_inv_prims;             // [kLdsDims]

@@ -24,6 +24,7 @@
 // division: the shading keeps the reference's operation order (pt_math.h).
 #include "guide_math.h"
 #include "pt_internal.h"
+#include "ray_tri.h"
 
 #include <float.h>
 
@@ -44,6 +45,7 @@ struct PtArgs
 	const PtConst *c;
 	const PtMaterial *mats;
 	const PtLight *lights;
+	const PtLightTri *ltris;
 	const float4 *prim;
 	const int32_t *faure;
 	const double *inv_prims;
@@ -692,7 +694,7 @@ __device__ __forceinline__ int light_samples(const PtLight &l, uint32_t division
 
 __device__ __forceinline__ uint32_t light_slots(const PtLight &l, uint32_t division)
 {
-	return l.type == YMX_LIGHT_AREA ? 2u * (uint32_t)light_samples(l, division) : 1u;
+	return l.type == YMX_LIGHT_POINT ? 1u : 2u * (uint32_t)light_samples(l, division);
 }
 
 // Accelerator::isShadowed (include/accelerator/accelerator.h:103-111): origin
@@ -700,7 +702,7 @@ __device__ __forceinline__ uint32_t light_slots(const PtLight &l, uint32_t divis
 // test is needed (t_max <= 0 can never be occluded).
 __device__ int32_t queue_shadow(const PtArgs &a, V3 from, V3 dir, float tmin, float tmax)
 {
-	const float t_max = tmax - 2 * tmin;
+	const float t_max = (tmax >= 0.f) ? tmax - 2 * tmin : FLT_MAX;
 	if(!(t_max > 0.f)) return -1;
 	const V3 f = add(from, mul(tmin, dir));
 	const uint32_t q = atomicAdd(a.st.sh_count, 1u);
@@ -728,6 +730,132 @@ __device__ __forceinline__ void set_slot(const PtArgs &a, uint32_t slot, Rgb c, 
 	}
 	a.st.slot_contrib[slot] = make_float4(c.r, c.g, c.b, 1.f);
 	a.st.slot_queue[slot] = cast ? queue_shadow(a, from, dir, tmin, tmax) : -1;
+}
+
+// ObjectLight::sampleSurface (src/light/light_object_light.cc:124-139): the
+// triangle from the light's area cdf by Pdf1D::dSample (std::lower_bound,
+// include/sampler/sample_pdf1d.h:83-95), the first coordinate rescaled to that
+// triangle's cdf interval, then ShapePolygon::sample's uniform triangle point
+// (include/geometry/shape/shape_polygon.h:78-90).
+__device__ void obj_sample_surface(const PtArgs &a, const PtLight &l, float s1, float s2, V3 &p, V3 &n)
+{
+	const PtLightTri *lt = a.ltris + l.first;
+	uint32_t idx;
+	if(s1 <= 0.f) idx = 0;
+	else if(s1 >= 1.f) idx = l.count - 1;
+	else
+	{
+		uint32_t lo = 0, hi = l.count;
+		while(lo < hi)
+		{
+			const uint32_t mid = lo + (hi - lo) / 2;
+			if(lt[mid].v0_cdf.w < s1) lo = mid + 1;
+			else hi = mid;
+		}
+		idx = lo;
+	}
+	const float cdf = lt[idx].v0_cdf.w;
+	float delta = cdf, ss1;
+	if(idx > 0)
+	{
+		const float prev = lt[idx - 1].v0_cdf.w;
+		delta -= prev;
+		ss1 = (s1 - prev) / delta;
+	}
+	else ss1 = s1 / delta;
+	const PtLightTri t = lt[idx];
+	const float su1 = sqrtf(ss1);
+	const float u = 1.f - su1;
+	const float v = s2 * su1;
+	const float w = 1.f - u - v;
+	p = add(add(v3(u * t.v0_cdf.x, u * t.v0_cdf.y, u * t.v0_cdf.z), v3(v * t.v1_vis.x, v * t.v1_vis.y, v * t.v1_vis.z)),
+	        v3(w * t.v2.x, w * t.v2.y, w * t.v2.z));
+	n = v3(t.n.x, t.n.y, t.n.z);
+}
+
+// Light::illumSample of the area and object lights (src/light/light_area.cc:99-124,
+// src/light/light_object_light.cc:141-171).  False when the light gives no sample.
+__device__ bool illum_sample(const PtArgs &a, const PtLight &l, V3 sp_p, float s1, float s2, V3 &ldir_out,
+                             float &dist_out, float &pdf_out)
+{
+	V3 ldir;
+	float dist_sqr, cos_angle, area_cos;
+	if(l.type == YMX_LIGHT_OBJECT)
+	{
+		V3 p, n;
+		obj_sample_surface(a, l, s1, s2, p, n);
+		ldir = sub(p, sp_p);
+		dist_sqr = dot(ldir, ldir);
+		const float dist = sqrtf(dist_sqr);
+		if(dist <= 0.f) return false;
+		ldir = mul(1.f / dist, ldir);
+		cos_angle = -dot(ldir, n);
+		if(cos_angle <= 0.f)
+		{
+			if(l.double_sided) cos_angle = -cos_angle;
+			else return false;
+		}
+		area_cos = l.area * cos_angle;
+		pdf_out = dist_sqr * kPi / ((area_cos == 0.f) ? 1e-8f : area_cos);
+		dist_out = dist;
+	}
+	else
+	{
+		const V3 p = add(add(l.corner, mul(s1, l.to_x)), mul(s2, l.to_y));
+		ldir = sub(p, sp_p);
+		dist_sqr = dot(ldir, ldir);
+		const float dist = sqrtf(dist_sqr);
+		if(dist <= 0.f) return false;
+		ldir = mul(1.f / dist, ldir);
+		cos_angle = dot(ldir, l.normal_flipped);
+		if(cos_angle <= 0.f) return false;
+		pdf_out = dist_sqr * kPi / (l.area * cos_angle);
+		dist_out = dist;
+	}
+	ldir_out = ldir;
+	return true;
+}
+
+// ObjectLight::intersect (src/light/light_object_light.cc:211-229): the closest
+// hit among the light's visible triangles past max(tmin, dynamic bias of the
+// light's tree bound) (include/accelerator/accelerator.h:64,122-131,
+// include/accelerator/accelerator_kdtree_common.h:139), by testing every
+// triangle of the light.  The reference leaves its `t` argument, the ray's
+// tmax_ = -1, unchanged, so the 1 / (t * t) factor of the inverse pdf is 1.
+__device__ bool obj_intersect(const PtArgs &a, const PtLight &l, V3 from, V3 dir, float tmin, float &ipdf)
+{
+	const float f[3] = {from.x, from.y, from.z}, d[3] = {dir.x, dir.y, dir.z};
+	float enter, leave;
+	if(!ymx::tree_cross(l.bound_a, l.bound_g, f, d, FLT_MAX, enter, leave)) return false;
+	const float bias = (0.1f * 0.00005f) * fabsf(leave - enter);
+	const float t_min = (tmin < bias) ? bias : tmin;
+	float best = FLT_MAX;
+	int32_t hit = -1;
+	const PtLightTri *lt = a.ltris + l.first;
+	for(uint32_t i = 0; i < l.count; ++i)
+	{
+		const PtLightTri t = lt[i];
+		if(!(__float_as_uint(t.v1_vis.w) & YMX_VIS_VISIBLE)) continue;
+		const float4 e1 = make_float4(t.v1_vis.x - t.v0_cdf.x, t.v1_vis.y - t.v0_cdf.y, t.v1_vis.z - t.v0_cdf.z, 0.f);
+		const float4 e2 = make_float4(t.v2.x - t.v0_cdf.x, t.v2.y - t.v0_cdf.y, t.v2.z - t.v0_cdf.z, 0.f);
+		float u, v;
+		const float th = ymx::tri_test(t.v0_cdf, e1, e2, f, d, u, v);
+		if(th <= 0.f || th < t_min || th >= best) continue;
+		best = th;
+		hit = (int32_t)i;
+	}
+	if(hit < 0) return false;
+	const float4 n = lt[hit].n;
+	float cos_angle = dot(dir, v3(-n.x, -n.y, -n.z));
+	if(cos_angle <= 0.f)
+	{
+		if(l.double_sided) cos_angle = fabsf(cos_angle);
+		else return false;
+	}
+	const float t_arg = -1.f;
+	const float idist_sqr = 1.f / (t_arg * t_arg);
+	ipdf = idist_sqr * l.area * cos_angle * k1Pi;
+	return true;
 }
 
 // MonteCarloIntegrator::doLightEstimation with its area / dirac paths
@@ -764,7 +892,7 @@ __device__ void queue_light(const PtArgs &a, const Surf &sp, V3 wo, const PtLigh
 	const uint32_t l_offs = loffs * 4567u;
 	const int num_samples = light_samples(l, division);
 	const uint32_t offs = (uint32_t)num_samples * psd_sample + psd_offset + l_offs;
-	// areaLightSampleLight (:192-316) with AreaLight::illumSample (src/light/light_area.cc:99-124)
+	// areaLightSampleLight (:192-316) with Light::illumSample
 	Halton h2, h3;
 	h2.start(2, offs - 1);
 	h3.start(3, offs - 1);
@@ -773,24 +901,9 @@ __device__ void queue_light(const PtArgs &a, const Surf &sp, V3 wo, const PtLigh
 		const float s1 = h2.next();
 		const float s2 = h3.next();
 		const uint32_t slot = slot0 + (uint32_t)i;
-		const V3 p = add(add(l.corner, mul(s1, l.to_x)), mul(s2, l.to_y));
-		V3 ldir = sub(p, sp.p);
-		const float dist_sqr = dot(ldir, ldir);
-		const float dist = sqrtf(dist_sqr);
-		if(dist <= 0.f)
-		{
-			set_slot_empty(a, slot);
-			continue;
-		}
-		ldir = mul(1.f / dist, ldir);
-		const float cos_angle = dot(ldir, l.normal_flipped);
-		if(cos_angle <= 0.f)
-		{
-			set_slot_empty(a, slot);
-			continue;
-		}
-		const float lpdf = dist_sqr * kPi / (l.area * cos_angle);
-		if(!(lpdf > 1e-6f))
+		V3 ldir;
+		float dist, lpdf;
+		if(!illum_sample(a, l, sp.p, s1, s2, ldir, dist, lpdf) || !(lpdf > 1e-6f))
 		{
 			set_slot_empty(a, slot);
 			continue;
@@ -808,7 +921,9 @@ __device__ void queue_light(const PtArgs &a, const Surf &sp, V3 wo, const PtLigh
 		const Rgb c = cdiv(cscale(cscale(cmul(surf_col, l.color), angle), w), lpdf);
 		set_slot(a, slot, c, cast_shadows, sp.p, ldir, shadow_tmin, dist);
 	}
-	// areaLightSampleMaterial (:318-415) with AreaLight::intersect (src/light/light_area.cc:146-155)
+	// areaLightSampleMaterial (:318-415) with Light::intersect.  The shadow ray
+	// ends at the ray's tmax_, which only AreaLight::intersect sets (to the hit
+	// distance); it stays -1 (no end) for an object light.
 	const float btmin = 0.00005f * pscale;
 	h2.start(2, offs - 1);
 	h3.start(3, offs - 1);
@@ -824,27 +939,33 @@ __device__ void queue_light(const PtArgs &a, const Surf &sp, V3 wo, const PtLigh
 		float W = 0.f;
 		V3 dir = v3(0.f, 0.f, 0.f);
 		const Rgb surf_col = bsdf_sample(a, sp, wo, dir, s, W);
-		const float cos_l = dot(dir, l.normal_flipped);
-		bool done = false;
-		if(cos_l > 0.f)
+		float t = -1.f, ipdf = 0.f;
+		bool hit = false;
+		if(l.type == YMX_LIGHT_OBJECT) hit = obj_intersect(a, l, sp.p, dir, btmin, ipdf);
+		else
 		{
-			const float t = quad_intersect(l.quad, sp.p, dir);
-			if(t > 0.f && s.pdf > 1e-6f)
+			// AreaLight::intersect (src/light/light_area.cc:146-155)
+			const float cos_l = dot(dir, l.normal_flipped);
+			if(cos_l > 0.f)
 			{
-				const float ipdf = 1.f / (t * t) * l.area * cos_l * k1Pi;
-				if(ipdf > 1e-6f)
+				t = quad_intersect(l.quad, sp.p, dir);
+				if(t > 0.f)
 				{
-					const float l_pdf = 1.f / ipdf;
-					const float l_2 = l_pdf * l_pdf;
-					const float m_2 = s.pdf * s.pdf;
-					const float w = m_2 / (l_2 + m_2);
-					const Rgb c = cscale(cscale(cmul(surf_col, l.color), w), W);
-					set_slot(a, slot, c, cast_shadows, sp.p, dir, btmin, t);
-					done = true;
+					hit = true;
+					ipdf = 1.f / (t * t) * l.area * cos_l * k1Pi;
 				}
 			}
 		}
-		if(!done) set_slot_empty(a, slot);
+		if(hit && s.pdf > 1e-6f && ipdf > 1e-6f)
+		{
+			const float l_pdf = 1.f / ipdf;
+			const float l_2 = l_pdf * l_pdf;
+			const float m_2 = s.pdf * s.pdf;
+			const float w = m_2 / (l_2 + m_2);
+			const Rgb c = cscale(cscale(cmul(surf_col, l.color), w), W);
+			set_slot(a, slot, c, cast_shadows, sp.p, dir, btmin, t);
+		}
+		else set_slot_empty(a, slot);
 	}
 }
 
@@ -1669,6 +1790,7 @@ int ymx_pt_trace_pass(ymx_pt *pt, int32_t first_sample, int32_t count, hipStream
 	a.c = pt->d_const;
 	a.mats = pt->d_materials;
 	a.lights = pt->d_lights;
+	a.ltris = pt->d_ltris;
 	a.prim = pt->d_prim;
 	a.faure = pt->d_faure;
 	a.inv_prims = pt->d_inv_prims;

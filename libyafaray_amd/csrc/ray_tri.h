@@ -1,0 +1,89 @@
+// ray_tri.h -- the reference's tree-bound crossing and triangle test as
+// device functions, shared by the traversal kernels (accel_trace.hip) and the
+// object-light intersection of the path tracer (pt_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <float.h>
+
+namespace ymx {
+
+// include/geometry/bound.h:157-198 on the tree bound, exactly as the reference
+// evaluates it.
+__device__ __forceinline__ bool tree_cross(const float a[3], const float g[3], const float from[3],
+                                           const float dir[3], float t_max, float &enter, float &leave)
+{
+	float p[3];
+	for(int k = 0; k < 3; ++k) p[k] = from[k] - a[k];
+	float lmin = -FLT_MAX, lmax = FLT_MAX;
+#pragma unroll
+	for(int axis = 0; axis < 3; ++axis)
+	{
+		if(dir[axis] != 0.f)
+		{
+			const float inv_dir = 1.f / dir[axis];
+			float ltmin, ltmax;
+			if(inv_dir > 0.f)
+			{
+				ltmin = -p[axis] * inv_dir;
+				ltmax = ((g[axis] - a[axis]) - p[axis]) * inv_dir;
+			}
+			else
+			{
+				ltmin = ((g[axis] - a[axis]) - p[axis]) * inv_dir;
+				ltmax = -p[axis] * inv_dir;
+			}
+			if(axis == 0)
+			{
+				lmin = ltmin;
+				lmax = ltmax;
+			}
+			else
+			{
+				lmin = (lmin < ltmin) ? ltmin : lmin;
+				lmax = (lmax < ltmax) ? lmax : ltmax;
+			}
+			if((lmax < 0.f) || (lmin > t_max)) return false;
+		}
+	}
+	if((lmin <= lmax) && (lmax >= 0.f) && (lmin <= t_max))
+	{
+		enter = lmin;
+		leave = lmax;
+		return true;
+	}
+	return false;
+}
+
+// include/geometry/shape/shape_polygon.h:126-176 (N == 3) on a pre-formed
+// record (v0, e1, e2).  Returns t > 0 on a hit, 0 otherwise.
+__device__ __forceinline__ float tri_test(const float4 t0, const float4 t1, const float4 t2, const float o[3],
+                                          const float d[3], float &u_out, float &v_out)
+{
+	const float e1x = t1.x, e1y = t1.y, e1z = t1.z;
+	const float e2x = t2.x, e2y = t2.y, e2z = t2.z;
+	// pvec = dir ^ edge_2
+	const float px = d[1] * e2z - d[2] * e2y;
+	const float py = d[2] * e2x - d[0] * e2z;
+	const float pz = d[0] * e2y - d[1] * e2x;
+	const float det = e1x * px + e1y * py + e1z * pz;
+	if(det == 0.f) return 0.f;
+	const float inv_det = 1.f / det;
+	const float tx = o[0] - t0.x, ty = o[1] - t0.y, tz = o[2] - t0.z;
+	const float u = (tx * px + ty * py + tz * pz) * inv_det;
+	if(!(u >= 0.f && u <= 1.f)) return 0.f;
+	// qvec = tvec ^ edge_1
+	const float qx = ty * e1z - tz * e1y;
+	const float qy = tz * e1x - tx * e1z;
+	const float qz = tx * e1y - ty * e1x;
+	const float v = (d[0] * qx + d[1] * qy + d[2] * qz) * inv_det;
+	if(!(v >= 0.f && (u + v) <= 1.f)) return 0.f;
+	const float t = (e2x * qx + e2y * qy + e2z * qz) * inv_det;
+	if(!(t > 0.f)) return 0.f;
+	u_out = u;
+	v_out = v;
+	return t;
+}
+
+}  // namespace ymx

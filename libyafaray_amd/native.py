@@ -52,8 +52,8 @@ class YmxPtStats(C.Structure):
 
 # Every symbol include/ymx_accel.h and include/ymx_pt.h declare, with its ctypes signature.
 SIGNATURES = {
-    "ymx_pt_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
-                                C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+    "ymx_pt_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
                                 C.POINTER(C.c_void_p)]),
     "ymx_pt_destroy": (C.c_int, [C.c_void_p]),
     "ymx_pt_clear_film": (C.c_int, [C.c_void_p, C.c_void_p]),

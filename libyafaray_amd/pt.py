@@ -34,7 +34,15 @@ MATERIAL_TYPES = {"shinydiffuse": 0, "shinydiffusemat": 0, "glossy": 1, "mirror"
 class YmxLight(C.Structure):
     _fields_ = [("type", C.c_uint32), ("corner", C.c_float * 3), ("point1", C.c_float * 3),
                 ("point2", C.c_float * 3), ("from_", C.c_float * 3), ("color", C.c_float * 3),
-                ("power", C.c_float), ("samples", C.c_int32), ("cast_shadows", C.c_uint32)]
+                ("power", C.c_float), ("samples", C.c_int32), ("cast_shadows", C.c_uint32),
+                ("first_tri", C.c_uint32), ("n_tris", C.c_uint32), ("double_sided", C.c_uint32)]
+
+
+# light "type" param values of the reference (include/light/light.h:117-128), plus short aliases
+LIGHT_TYPES = {"area": 0, "arealight": 0, "point": 1, "pointlight": 1, "object": 2, "objectlight": 2}
+
+# object "visibility" param values (include/common/visibility.h:31-37) as YMX_VIS_* masks
+VISIBILITY = {"normal": 3, "no_shadows": 1, "shadow_only": 2, "invisible": 0}
 
 
 class YmxPtSettings(C.Structure):
@@ -83,7 +91,9 @@ def build_structs(scene):
         fill_material(mats[i], m)
     lights = (YmxLight * max(1, len(scene["lights"])))()
     for i, l in enumerate(scene["lights"]):
-        lights[i].type = 0 if l["type"] == "area" else 1
+        if l["type"] not in LIGHT_TYPES:
+            raise ValueError(f"unsupported light type {l['type']!r}")
+        lights[i].type = LIGHT_TYPES[l["type"]]
         for k in ("corner", "point1", "point2"):
             setattr(lights[i], k, _f3(l.get(k, (0, 0, 0))))
         lights[i].from_ = _f3(l.get("from", (0, 0, 0)))
@@ -91,6 +101,9 @@ def build_structs(scene):
         lights[i].power = float(l.get("power", 1.0))
         lights[i].samples = int(l.get("samples", 4))
         lights[i].cast_shadows = 1 if l.get("cast_shadows", True) else 0
+        lights[i].first_tri = int(l.get("first_tri", 0))
+        lights[i].n_tris = int(l.get("n_tris", 0))
+        lights[i].double_sided = 1 if l.get("double_sided", False) else 0
     cam, s = scene["camera"], scene["settings"]
     st = YmxPtSettings()
     st.from_ = _f3(cam["from_"])
@@ -120,12 +133,15 @@ class GpuPathTracer:
         self.verts = np.ascontiguousarray(scene["verts"], dtype=np.float32)
         self.tris = np.ascontiguousarray(scene["tris"], dtype=np.uint32)
         self.mat_id = np.ascontiguousarray(scene["mat_id"], dtype=np.uint32)
-        self.accel = accel or GpuAccelerator(self.verts, self.tris)
+        vis = scene.get("vis")
+        self.vis = None if vis is None else np.ascontiguousarray(vis, dtype=np.uint8)
+        self.accel = accel or GpuAccelerator(self.verts, self.tris, self.vis)
         self.mats, self.lights, self.st = build_structs(scene)
         self.resx, self.resy = self.st.resx, self.st.resy
         h = C.c_void_p()
         rc = self._lib.ymx_pt_create(self.accel.handle, self.verts.ctypes.data, self.verts.shape[0],
-                                     self.tris.ctypes.data, self.mat_id.ctypes.data, self.tris.shape[0],
+                                     self.tris.ctypes.data, self.mat_id.ctypes.data,
+                                     None if self.vis is None else self.vis.ctypes.data, self.tris.shape[0],
                                      C.addressof(self.mats), len(scene["materials"]), C.addressof(self.lights),
                                      len(scene["lights"]), C.addressof(self.st), C.byref(h))
         check(rc, "ymx_pt_create")

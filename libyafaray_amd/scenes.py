@@ -416,3 +416,75 @@ def occluded_room_scene(res=(1280, 720), spp: int = 16, seed: int = 0, wall_cell
                     background=None, seed=seed, raydepth=0)
     return dict(verts=verts, tris=tris, mat_id=mat_id, materials=materials, lights=lights, camera=camera,
                 settings=settings)
+
+
+def icosphere_tris(center, radius: float, subdiv: int = 2) -> tuple[np.ndarray, np.ndarray]:
+    """Sphere mesh: an octahedron with each face split into 4 `subdiv` times and
+    its vertices pushed to the sphere (8 * 4^subdiv triangles, outward face
+    normals)."""
+    v = [np.array(p, np.float64) for p in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1))]
+    f = [(0, 2, 4), (2, 1, 4), (1, 3, 4), (3, 0, 4), (2, 0, 5), (1, 2, 5), (3, 1, 5), (0, 3, 5)]
+    for _ in range(subdiv):
+        mid, nf = {}, []
+
+        def m(a, b):
+            k = (min(a, b), max(a, b))
+            if k not in mid:
+                p = v[a] + v[b]
+                v.append(p / np.linalg.norm(p))
+                mid[k] = len(v) - 1
+            return mid[k]
+        for a, b, c in f:
+            ab, bc, ca = m(a, b), m(b, c), m(c, a)
+            nf += [(a, ab, ca), (ab, b, bc), (ca, bc, c), (ab, bc, ca)]
+        f = nf
+    verts = (np.array(v) * radius + np.asarray(center, np.float64)).astype(F32)
+    return verts, np.array(f, np.uint32)
+
+
+def object_light_scene(n_random: int = 2000, seed: int = 0, res: int = 64, spp: int = 4,
+                       visibility: str = "no_shadows", double_sided: bool = False, subdiv: int = 2):
+    """Cornell box lit by two object lights (include/light/light_object_light.h):
+    a light_mat sphere mesh (8 * 4^subdiv triangles) hanging in the box, and a
+    two-triangle ceiling panel facing down; no area light.  Both light meshes
+    carry the object `visibility` ("normal", "no_shadows", "shadow_only",
+    "invisible"); the rest of the scene is "normal".  Random triangles are
+    shinydiffuse and glossy, and one point light keeps the shadows of the
+    panel's back side lit."""
+    from .pt import VISIBILITY
+    rng = np.random.default_rng(seed)
+    wv, wt, wall = cornell_walls()
+    tv = random_triangles(rng, n_random, (0.15, 0.15, 0.05), (0.85, 0.85, 0.6))
+    sphere = icosphere_tris((0.35, 0.6, 0.75), 0.08, subdiv)
+    panel = quad_tris((0.55, 0.3, 0.97), (0.0, 0.2, 0.0), (0.2, 0.0, 0.0))  # normal -z
+    verts, tris = merge([(wv, wt), soup(tv), sphere, panel])
+    n_wall, n_sph = len(wt), len(sphere[1])
+    materials = [dict(color=(0.7, 0.7, 0.7)), dict(color=(0.7, 0.1, 0.1)), dict(color=(0.1, 0.7, 0.1)),
+                 dict(type="light_mat", color=(1.0, 0.85, 0.6), power=4.0),
+                 dict(type="light_mat", color=(0.7, 0.8, 1.0), power=2.0, double_sided=double_sided)]
+    wall_mat = np.array([0, 0, 0, 1, 2])
+    for k in range(n_random):
+        a = float(rng.uniform(0.2, 0.8))
+        if rng.uniform() < 0.6:
+            materials.append(dict(color=(a, a, a)))
+        else:
+            r = float(rng.uniform(0.1, 0.5))
+            materials.append(dict(type="glossy", color=(a, 0.9 * a, 0.8 * a), exponent=2.0 / (r * r) - 2.0,
+                                  diffuse_reflect=float(rng.uniform(0.0, 0.6)), diffuse_color=(0.8 * a, a, a),
+                                  glossy_reflect=float(rng.uniform(0.2, 1.0))))
+    mat_id = np.concatenate([wall_mat[wall], 5 + np.arange(n_random), np.full(n_sph, 3),
+                             np.full(2, 4)]).astype(np.uint32)
+    first_sph = n_wall + n_random
+    vis = np.full(len(tris), 3, np.uint8)
+    vis[first_sph:] = VISIBILITY[visibility]
+    lights = [dict(type="objectlight", first_tri=first_sph, n_tris=n_sph, color=(1.0, 0.85, 0.6), power=4.0,
+                   samples=4),
+              dict(type="objectlight", first_tri=first_sph + n_sph, n_tris=2, color=(0.7, 0.8, 1.0), power=2.0,
+                   samples=2, double_sided=double_sided),
+              dict(type="point", **{"from": (0.8, 0.2, 0.3)}, color=(1.0, 1.0, 1.0), power=0.02)]
+    camera = dict(from_=(0.5, -1.6, 0.5), to=(0.5, 0.5, 0.5), up=(0.5, -1.6, 1.5),
+                  focal=float(1.0 / (2.0 * np.tan(np.radians(20.0)))), resx=res, resy=res)
+    settings = dict(path_samples=1, bounces=4, rr_min_bounces=2, aa_samples=spp, aa_pixelwidth=1.5,
+                    background=None, seed=seed, raydepth=2)
+    return dict(verts=verts, tris=tris, mat_id=mat_id, vis=vis, materials=materials, lights=lights,
+                camera=camera, settings=settings)

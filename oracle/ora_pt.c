@@ -269,8 +269,10 @@ typedef struct
 	const uint32_t *idx;
 	const uint32_t *mat_id;
 	const ora_material *mats;
+	const uint8_t *vis;
 	const ora_light *lights;
 	uint32_t n_lights;
+	struct obj_light *objs;   /* [n_lights], set up for ORA_LIGHT_OBJECT lights */
 	const ora_pt_settings *st;
 	/* camera (src/camera/camera.cc:89-110, camera_perspective.cc:97-111) */
 	v3 cam_from, vright, vup, vto;
@@ -867,7 +869,8 @@ static rgb mat_emit(const surf *sp, v3 wo)
 static rgb light_color(const ora_light *l)
 {
 	const rgb c = cscale(C(l->color[0], l->color[1], l->color[2]), l->power);
-	return l->type == ORA_LIGHT_AREA ? cscale(c, K_PI) : c;  /* light_area.h:78, light_point.h:64 */
+	/* light_area.h:78, light_object_light.h:82, light_point.h:64 */
+	return l->type == ORA_LIGHT_POINT ? c : cscale(c, K_PI);
 }
 
 static v3 area_normal_flipped(const ora_light *l)
@@ -945,6 +948,177 @@ static int area_intersect(const ora_light *l, v3 from, v3 dir, float *t, float *
 	return 1;
 }
 
+/* ObjectLight::initIs (src/light/light_object_light.cc:87-104): the
+ * triangle areas, their Pdf1D (include/sampler/sample_pdf1d.h:60-81: a
+ * double running sum of area / n, each entry rounded to float and divided by
+ * the float total), area_ = the double sum of the float areas, and the tree
+ * bound of the light's own kd-tree. */
+struct obj_light
+{
+	float *cdf;
+	float area;
+	float a[3], g[3];
+};
+
+static v3 tri_vertex(const pt_ctx *c, uint32_t prim, int k)
+{
+	return vload(c->verts + 3 * (size_t)c->idx[3 * (size_t)prim + k]);
+}
+
+static int obj_light_init(const pt_ctx *c, const ora_light *l, struct obj_light *o)
+{
+	const uint32_t n = l->n_tris;
+	o->cdf = (float *)malloc(sizeof(float) * (n ? n : 1));
+	float *areas = (float *)malloc(sizeof(float) * (n ? n : 1));
+	if(!o->cdf || !areas) { free(areas); return 0; }
+	double total = 0.0;
+	for(uint32_t i = 0; i < n; ++i)
+	{
+		const uint32_t p = l->first_tri + i;
+		const v3 v0 = tri_vertex(c, p, 0);
+		const v3 cr = vcross(vsub(tri_vertex(c, p, 1), v0), vsub(tri_vertex(c, p, 2), v0));
+		areas[i] = 0.5f * sqrtf(vdot(cr, cr));   /* ShapePolygon::surfaceArea, shape_polygon.h:64-76 */
+		total += (double)areas[i];
+	}
+	const double delta = 1.0 / (double)n;
+	double acc = 0.0;
+	for(uint32_t i = 0; i < n; ++i)
+	{
+		acc += (double)areas[i] * delta;
+		o->cdf[i] = (float)acc;
+	}
+	const float integral = (float)acc;
+	for(uint32_t i = 0; i < n; ++i) o->cdf[i] /= integral;
+	o->area = (float)total;
+	ora_tree_bound(c->verts, c->idx + 3 * (size_t)l->first_tri, n, o->a, o->g);
+	free(areas);
+	return 1;
+}
+
+/* ObjectLight::sampleSurface (src/light/light_object_light.cc:124-139) with
+ * Pdf1D::dSample (sample_pdf1d.h:83-95, std::lower_bound) and the triangle
+ * sample of ShapePolygon::sample (shape_polygon.h:78-90). */
+static void obj_sample_surface(const pt_ctx *c, const ora_light *l, const struct obj_light *o, float s1, float s2,
+                               v3 *p, v3 *n)
+{
+	const uint32_t cnt = l->n_tris;
+	uint32_t idx;
+	if(s1 <= 0.f) idx = 0;
+	else if(s1 >= 1.f) idx = cnt - 1;
+	else
+	{
+		uint32_t lo = 0, hi = cnt;   /* first entry not less than s1 */
+		while(lo < hi)
+		{
+			const uint32_t mid = lo + (hi - lo) / 2;
+			if(o->cdf[mid] < s1) lo = mid + 1;
+			else hi = mid;
+		}
+		idx = lo;
+	}
+	float delta = o->cdf[idx];
+	float ss1;
+	if(idx > 0)
+	{
+		delta -= o->cdf[idx - 1];
+		ss1 = (s1 - o->cdf[idx - 1]) / delta;
+	}
+	else ss1 = s1 / delta;
+	const uint32_t prim = l->first_tri + idx;
+	const v3 v0 = tri_vertex(c, prim, 0), v1 = tri_vertex(c, prim, 1), v2 = tri_vertex(c, prim, 2);
+	const float su1 = sqrtf(ss1);
+	const float u = 1.f - su1;
+	const float v = s2 * su1;
+	*p = vadd(vadd(vmul(u, v0), vmul(v, v1)), vmul(1.f - u - v, v2));
+	*n = vnorm(vcross(vsub(v1, v0), vsub(v2, v0)));   /* face_normal_geometric_ */
+}
+
+/* Light::illumSample of the area and object lights (src/light/light_area.cc:99-124,
+ * src/light/light_object_light.cc:141-171): the direction and distance to a
+ * point on the light and the sample's pdf; 0 when the light gives no sample. */
+static int illum_sample(const pt_ctx *c, const ora_light *l, const struct obj_light *o, v3 sp_p, float s1, float s2,
+                        v3 *ldir_out, float *dist_out, float *pdf_out)
+{
+	if(l->type == ORA_LIGHT_OBJECT)
+	{
+		v3 p, n;
+		obj_sample_surface(c, l, o, s1, s2, &p, &n);
+		v3 ldir = vsub(p, sp_p);
+		const float dist_sqr = vdot(ldir, ldir);
+		const float dist = sqrtf(dist_sqr);
+		if(dist <= 0.f) return 0;
+		ldir = vmul(1.f / dist, ldir);
+		float cos_angle = -vdot(ldir, n);
+		if(cos_angle <= 0.f)
+		{
+			if(l->double_sided) cos_angle = -cos_angle;
+			else return 0;
+		}
+		const float area_mul_cosangle = o->area * cos_angle;
+		*pdf_out = dist_sqr * K_PI / ((area_mul_cosangle == 0.f) ? 1e-8f : area_mul_cosangle);
+		*ldir_out = ldir;
+		*dist_out = dist;
+		return 1;
+	}
+	const v3 c0 = vload(l->corner);
+	const v3 to_x = vsub(vload(l->point1), c0), to_y = vsub(vload(l->point2), c0);
+	const v3 p = vadd(vadd(c0, vmul(s1, to_x)), vmul(s2, to_y));
+	v3 ldir = vsub(p, sp_p);
+	const float dist_sqr = vdot(ldir, ldir);
+	const float dist = sqrtf(dist_sqr);
+	if(dist <= 0.f) return 0;
+	ldir = vmul(1.f / dist, ldir);
+	const float cos_angle = vdot(ldir, area_normal_flipped(l));
+	if(cos_angle <= 0.f) return 0;
+	*pdf_out = dist_sqr * K_PI / (area_area(l) * cos_angle);
+	*ldir_out = ldir;
+	*dist_out = dist;
+	return 1;
+}
+
+/* ObjectLight::intersect (src/light/light_object_light.cc:211-229): the
+ * closest hit of the light's own kd-tree (t_min = max(ray tmin, dynamic
+ * bias), visible triangles only, include/accelerator/accelerator.h:122-131).
+ * The reference never writes the hit distance back into its `t` argument,
+ * which areaLightSampleMaterial passes as the ray's tmax_ = -1, so 1 / (t * t)
+ * is 1: the returned inverse pdf ignores the distance. */
+static int obj_intersect(const pt_ctx *c, const ora_light *l, const struct obj_light *o, v3 from, v3 dir, float tmin,
+                         float *ipdf)
+{
+	const float f[3] = {from.x, from.y, from.z}, d[3] = {dir.x, dir.y, dir.z};
+	float enter, leave;
+	if(!ora_bound_cross(o->a, o->g, f, d, FLT_MAX, &enter, &leave)) return 0;
+	const float bias = ora_dynamic_bias(enter, leave);
+	const float t_min = (tmin < bias) ? bias : tmin;
+	float best = FLT_MAX;
+	uint32_t hit = 0xFFFFFFFFu;
+	for(uint32_t i = 0; i < l->n_tris; ++i)
+	{
+		const uint32_t p = l->first_tri + i;
+		if(c->vis && !(c->vis[p] & ORA_VIS_VISIBLE)) continue;
+		const v3 v0 = tri_vertex(c, p, 0), v1 = tri_vertex(c, p, 1), v2 = tri_vertex(c, p, 2);
+		const float a0[3] = {v0.x, v0.y, v0.z}, a1[3] = {v1.x, v1.y, v1.z}, a2[3] = {v2.x, v2.y, v2.z};
+		float t, u, v;
+		if(!ora_triangle_intersect(a0, a1, a2, f, d, &t, &u, &v)) continue;
+		if(t <= 0.f || t < t_min || t >= best) continue;
+		best = t;
+		hit = p;
+	}
+	if(hit == 0xFFFFFFFFu) return 0;
+	const v3 v0 = tri_vertex(c, hit, 0);
+	const v3 n = vnorm(vcross(vsub(tri_vertex(c, hit, 1), v0), vsub(tri_vertex(c, hit, 2), v0)));
+	float cos_angle = vdot(dir, vneg(n));
+	if(cos_angle <= 0.f)
+	{
+		if(l->double_sided) cos_angle = fabsf(cos_angle);
+		else return 0;
+	}
+	const float t_arg = -1.f;
+	const float idist_sqr = 1.f / (t_arg * t_arg);
+	*ipdf = idist_sqr * o->area * cos_angle * K_1_PI;
+	return 1;
+}
+
 /* MonteCarloIntegrator::doLightEstimation and the area / dirac paths,
  * src/integrator/surface/integrator_montecarlo.cc:118-440. */
 static rgb light_estimation(const pt_ctx *c, const ora_light *l, const surf *sp, v3 wo, uint32_t loffs,
@@ -980,10 +1154,7 @@ static rgb light_estimation(const pt_ctx *c, const ora_light *l, const surf *sp,
 	const float inv_num_samples = 1.f / (float)num_samples;
 	const uint32_t offs = (uint32_t)num_samples * psd_sample + psd_offset + l_offs;
 	const rgb lcolor = light_color(l);
-	const v3 nf = area_normal_flipped(l);
-	const float area = area_area(l);
-	const v3 c0 = vload(l->corner);
-	const v3 to_x = vsub(vload(l->point1), c0), to_y = vsub(vload(l->point2), c0);
+	const struct obj_light *o = (l->type == ORA_LIGHT_OBJECT) ? &c->objs[l - c->lights] : NULL;
 	halton h2, h3;
 	/* areaLightSampleLight, :192-316 */
 	halton_start(&h2, 2, offs - 1);
@@ -993,16 +1164,9 @@ static rgb light_estimation(const pt_ctx *c, const ora_light *l, const surf *sp,
 	{
 		const float s1 = halton_next(&h2);
 		const float s2 = halton_next(&h3);
-		/* AreaLight::illumSample, src/light/light_area.cc:99-124 */
-		const v3 p = vadd(vadd(c0, vmul(s1, to_x)), vmul(s2, to_y));
-		v3 ldir = vsub(p, sp->p);
-		const float dist_sqr = vdot(ldir, ldir);
-		const float dist = sqrtf(dist_sqr);
-		if(dist <= 0.f) continue;
-		ldir = vmul(1.f / dist, ldir);
-		const float cos_angle = vdot(ldir, nf);
-		if(cos_angle <= 0.f) continue;
-		const float lpdf = dist_sqr * K_PI / (area * cos_angle);
+		v3 ldir;
+		float dist, lpdf;
+		if(!illum_sample(c, l, o, sp->p, s1, s2, &ldir, &dist, &lpdf)) continue;
 		int shadowed = 0;
 		if(cast_shadows) shadowed = is_shadowed(c, sp->p, ldir, shadow_tmin, dist);
 		if(!shadowed && lpdf > 1e-6f)
@@ -1036,8 +1200,12 @@ static rgb light_estimation(const pt_ctx *c, const ora_light *l, const surf *sp,
 		float W = 0.f;
 		v3 dir = V(0.f, 0.f, 0.f);
 		const rgb surf_col = bsdf_sample(sp, wo, &dir, &s, &W);
-		float t, ipdf;
-		if(area_intersect(l, sp->p, dir, &t, &ipdf) && s.pdf > 1e-6f)
+		/* Light::intersect; the shadow ray ends at the ray's tmax_, which only
+		 * AreaLight::intersect sets (to the hit distance; -1 otherwise: no end) */
+		float t = -1.f, ipdf = 0.f;
+		const int light_hit = o ? obj_intersect(c, l, o, sp->p, dir, btmin, &ipdf)
+		                        : area_intersect(l, sp->p, dir, &t, &ipdf);
+		if(light_hit && s.pdf > 1e-6f)
 		{
 			int shadowed = 0;
 			if(cast_shadows) shadowed = is_shadowed(c, sp->p, dir, btmin, t);
@@ -1254,12 +1422,32 @@ static void integrate(const pt_ctx *c, v3 from, v3 dir, float tmin, int ray_leve
 	*out = col;
 }
 
-static void setup_ctx(pt_ctx *c, const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx,
-                      const uint32_t *mat_id, const ora_material *mats, const ora_light *lights, uint32_t n_lights,
-                      const ora_pt_settings *st)
+static void free_ctx(pt_ctx *c)
 {
-	c->kd = kd; c->verts = verts; c->idx = tri_idx; c->mat_id = mat_id; c->mats = mats;
+	if(!c->objs) return;
+	for(uint32_t l = 0; l < c->n_lights; ++l) free(c->objs[l].cdf);
+	free(c->objs);
+	c->objs = NULL;
+}
+
+/* Returns 0 when an object light's tables cannot be allocated. */
+static int setup_ctx(pt_ctx *c, const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx,
+                     const uint32_t *mat_id, const uint8_t *vis, const ora_material *mats, const ora_light *lights,
+                     uint32_t n_lights, const ora_pt_settings *st)
+{
+	c->kd = kd; c->verts = verts; c->idx = tri_idx; c->mat_id = mat_id; c->vis = vis; c->mats = mats;
 	c->lights = lights; c->n_lights = n_lights; c->st = st;
+	c->objs = NULL;
+	for(uint32_t l = 0; l < n_lights; ++l)
+	{
+		if(lights[l].type != ORA_LIGHT_OBJECT) continue;
+		if(!c->objs && !(c->objs = (struct obj_light *)calloc(n_lights, sizeof(struct obj_light)))) return 0;
+		if(lights[l].n_tris == 0 || !obj_light_init(c, &lights[l], &c->objs[l]))
+		{
+			free_ctx(c);
+			return 0;
+		}
+	}
 	c->total_samples = (uint32_t)st->aa_samples;
 	/* Camera::Camera, src/camera/camera.cc:89-104 */
 	const v3 from = vload(st->from);
@@ -1278,6 +1466,7 @@ static void setup_ctx(pt_ctx *c, const ora_kdtree *kd, const float *verts, const
 	c->vup = V(c->vup.x / (float)st->resy, c->vup.y / (float)st->resy, c->vup.z / (float)st->resy);
 	c->vright = V(c->vright.x / (float)st->resx, c->vright.y / (float)st->resx, c->vright.z / (float)st->resx);
 	c->cam_from = from;
+	return 1;
 }
 
 /* TiledIntegrator::renderTile per-sample set-up, integrator_tiled.cc:290-335 (one pass) */
@@ -1314,7 +1503,7 @@ void ora_pt_camera_dir(const ora_pt_settings *st, float px, float py, float dir[
 {
 	pt_ctx c;
 	memset(&c, 0, sizeof c);
-	setup_ctx(&c, NULL, NULL, NULL, NULL, NULL, NULL, 0, st);
+	setup_ctx(&c, NULL, NULL, NULL, NULL, NULL, NULL, NULL, 0, st);
 	const v3 d = vnorm(vadd(vadd(vmul(px, c.vright), vmul(py, c.vup)), c.vto));
 	dir[0] = d.x;
 	dir[1] = d.y;
@@ -1322,12 +1511,13 @@ void ora_pt_camera_dir(const ora_pt_settings *st, float px, float py, float dir[
 }
 
 void ora_pt_pixel_sample(const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx, const uint32_t *mat_id,
-                         const ora_material *mats, const ora_light *lights, uint32_t n_lights,
+                         const uint8_t *vis, const ora_material *mats, const ora_light *lights, uint32_t n_lights,
                          const ora_pt_settings *st, int x, int y, int sample, float col_alpha[4], float dxdy[2])
 {
 	pt_ctx c;
-	setup_ctx(&c, kd, verts, tri_idx, mat_id, mats, lights, n_lights, st);
+	if(!setup_ctx(&c, kd, verts, tri_idx, mat_id, vis, mats, lights, n_lights, st)) return;
 	pixel_sample(&c, x, y, sample, col_alpha, &dxdy[0], &dxdy[1]);
+	free_ctx(&c);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1386,17 +1576,17 @@ static void add_sample(int resx, int resy, const float table[256], float table_s
 }
 
 void ora_pt_render(const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx, const uint32_t *mat_id,
-                   const ora_material *mats, const ora_light *lights, uint32_t n_lights,
+                   const uint8_t *vis, const ora_material *mats, const ora_light *lights, uint32_t n_lights,
                    const ora_pt_settings *st, float *film_rgba, float *film_w, int threads)
 {
 	pt_ctx c;
-	setup_ctx(&c, kd, verts, tri_idx, mat_id, mats, lights, n_lights, st);
+	if(!setup_ctx(&c, kd, verts, tri_idx, mat_id, vis, mats, lights, n_lights, st)) return;
 	float table[256], table_scale, filter_width;
 	ora_filter_table(st->aa_pixelwidth, table, &table_scale, &filter_width);
 	const size_t npix = (size_t)st->resx * st->resy;
 	float *cols = (float *)malloc(npix * 4 * sizeof(float));
 	float *dxy = (float *)malloc(npix * 2 * sizeof(float));
-	if(!cols || !dxy) { free(cols); free(dxy); return; }
+	if(!cols || !dxy) { free(cols); free(dxy); free_ctx(&c); return; }
 	if(threads < 1) threads = 1;
 	if(threads > 256) threads = 256;
 	pass_job jobs[256];
@@ -1424,6 +1614,7 @@ void ora_pt_render(const ora_kdtree *kd, const float *verts, const uint32_t *tri
 	}
 	free(cols);
 	free(dxy);
+	free_ctx(&c);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1502,4 +1693,50 @@ void ora_mat_emit(const ora_material *m, const float ng[3], const float wo[3], f
 	const surf sp = flat_surf(m, ng);
 	const rgb c = mat_emit(&sp, vload(wo));
 	out[0] = c.r; out[1] = c.g; out[2] = c.b;
+}
+
+/* ObjectLight pieces on their own: set-up (area_ and the area cdf),
+ * illumSample from a point, and intersect along a direction. */
+static int light_ctx(pt_ctx *c, const float *verts, const uint32_t *tri_idx, const uint8_t *vis, const ora_light *l,
+                     struct obj_light *o)
+{
+	memset(c, 0, sizeof *c);
+	c->verts = verts; c->idx = tri_idx; c->vis = vis;
+	return l->type == ORA_LIGHT_OBJECT && l->n_tris > 0 && obj_light_init(c, l, o);
+}
+
+float ora_object_light_setup(const float *verts, const uint32_t *tri_idx, const ora_light *l, float *cdf)
+{
+	pt_ctx c;
+	struct obj_light o;
+	if(!light_ctx(&c, verts, tri_idx, NULL, l, &o)) return -1.f;
+	memcpy(cdf, o.cdf, sizeof(float) * l->n_tris);
+	free(o.cdf);
+	return o.area;
+}
+
+int ora_object_light_sample(const float *verts, const uint32_t *tri_idx, const ora_light *l, const float p[3],
+                            float s1, float s2, float out[5])
+{
+	pt_ctx c;
+	struct obj_light o;
+	if(!light_ctx(&c, verts, tri_idx, NULL, l, &o)) return -1;
+	v3 ldir = V(0.f, 0.f, 0.f);
+	float dist = 0.f, pdf = 0.f;
+	const int hit = illum_sample(&c, l, &o, vload(p), s1, s2, &ldir, &dist, &pdf);
+	out[0] = ldir.x; out[1] = ldir.y; out[2] = ldir.z; out[3] = dist; out[4] = pdf;
+	free(o.cdf);
+	return hit;
+}
+
+int ora_object_light_intersect(const float *verts, const uint32_t *tri_idx, const uint8_t *vis, const ora_light *l,
+                               const float from[3], const float dir[3], float tmin, float *ipdf)
+{
+	pt_ctx c;
+	struct obj_light o;
+	if(!light_ctx(&c, verts, tri_idx, vis, l, &o)) return -1;
+	*ipdf = 0.f;
+	const int hit = obj_intersect(&c, l, &o, vload(from), vload(dir), tmin, ipdf);
+	free(o.cdf);
+	return hit;
 }

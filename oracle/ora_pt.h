@@ -14,8 +14,9 @@
  * absorption (src/material/material_glass.cc:157-378), LightMaterial
  * (src/material/material_light.cc:69-92), the recursive raytrace with its
  * glossy ray division (src/integrator/surface/integrator_montecarlo.cc:506-748),
- * AreaLight / PointLight (src/light/light_area.cc:99-163,
- * src/light/light_point.cc:70-97), PerspectiveCamera::shootRay
+ * AreaLight / PointLight / ObjectLight (src/light/light_area.cc:99-163,
+ * src/light/light_point.cc:70-97, src/light/light_object_light.cc:87-237 with
+ * Pdf1D, include/sampler/sample_pdf1d.h:60-92), PerspectiveCamera::shootRay
  * (src/camera/camera_perspective.cc:97-111,165-180), the samplers
  * (include/sampler/sample.h:43-148, include/sampler/halton.h:30-81,
  * src/sampler/halton.cc:419-440) and ImageFilm::addSample with the Gauss
@@ -79,6 +80,7 @@ typedef struct
 
 #define ORA_LIGHT_AREA 0u
 #define ORA_LIGHT_POINT 1u
+#define ORA_LIGHT_OBJECT 2u
 
 typedef struct
 {
@@ -87,8 +89,11 @@ typedef struct
 	float from[3];                          /* point light */
 	float color[3];
 	float power;
-	int32_t samples;                        /* area light samples (default 4) */
+	int32_t samples;                        /* area / object light samples (default 4) */
 	uint32_t cast_shadows;
+	/* object light: the object is the triangle range [first_tri, first_tri + n_tris) */
+	uint32_t first_tri, n_tris;
+	uint32_t double_sided;                  /* object light "double_sided" (false) */
 } ora_light;
 
 typedef struct
@@ -110,15 +115,17 @@ typedef struct
 
 /* Render aa_samples samples per pixel into film sums: film_rgba[4 * (y * resx + x)]
  * (colour * weight, alpha * weight) and film_w[y * resx + x] (weights).
- * Buffers are accumulated into (zero them first). */
+ * Buffers are accumulated into (zero them first).  vis: the visibility mask of
+ * every triangle (the one kd was built with; object lights read it), NULL: all
+ * ORA_VIS_VISIBLE | ORA_VIS_SHADOWS. */
 void ora_pt_render(const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx, const uint32_t *mat_id,
-                   const ora_material *mats, const ora_light *lights, uint32_t n_lights,
+                   const uint8_t *vis, const ora_material *mats, const ora_light *lights, uint32_t n_lights,
                    const ora_pt_settings *st, float *film_rgba, float *film_w, int threads);
 
 /* Same, but only the sample values of one pixel sample (no film), for
  * sample-level checks: col_alpha[4], and the sub-pixel offsets dx, dy. */
 void ora_pt_pixel_sample(const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx, const uint32_t *mat_id,
-                         const ora_material *mats, const ora_light *lights, uint32_t n_lights,
+                         const uint8_t *vis, const ora_material *mats, const ora_light *lights, uint32_t n_lights,
                          const ora_pt_settings *st, int x, int y, int sample, float col_alpha[4], float dxdy[2]);
 
 /* PerspectiveCamera::shootRay direction for film coordinates (px, py). */
@@ -150,6 +157,17 @@ void ora_specular(const ora_material *m, const float ng[3], const float wo[3], i
 void ora_fresnel(const float i[3], const float n[3], float ior, float out[2]);
 int ora_refract(const float n[3], const float wi[3], float ior, float wo[3]);
 void ora_mat_emit(const ora_material *m, const float ng[3], const float wo[3], float out[3]);
+
+/* ObjectLight (src/light/light_object_light.cc:87-237) on its own, for the
+ * oracle's pinning tests.  setup: returns area_ and writes the area cdf
+ * (n_tris floats).  sample: illumSample from p; out = (ldir, dist, pdf),
+ * returns 1 on a sample.  intersect: Light::intersect along dir; returns 1 on
+ * a hit with its inverse pdf.  -1: not an object light. */
+float ora_object_light_setup(const float *verts, const uint32_t *tri_idx, const ora_light *l, float *cdf);
+int ora_object_light_sample(const float *verts, const uint32_t *tri_idx, const ora_light *l, const float p[3],
+                            float s1, float s2, float out[5]);
+int ora_object_light_intersect(const float *verts, const uint32_t *tri_idx, const uint8_t *vis, const ora_light *l,
+                               const float from[3], const float dir[3], float tmin, float *ipdf);
 
 #ifdef __cplusplus
 }

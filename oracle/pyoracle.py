@@ -212,7 +212,28 @@ def fill_material(dst, m):
 class OraLight(C.Structure):
     _fields_ = [("type", C.c_uint32), ("corner", C.c_float * 3), ("point1", C.c_float * 3),
                 ("point2", C.c_float * 3), ("from_", C.c_float * 3), ("color", C.c_float * 3),
-                ("power", C.c_float), ("samples", C.c_int32), ("cast_shadows", C.c_uint32)]
+                ("power", C.c_float), ("samples", C.c_int32), ("cast_shadows", C.c_uint32),
+                ("first_tri", C.c_uint32), ("n_tris", C.c_uint32), ("double_sided", C.c_uint32)]
+
+
+# light "type" param values of the reference (src/light/light.cc factory), as ORA_LIGHT_*
+LIGHT_TYPES = {"area": 0, "arealight": 0, "point": 1, "pointlight": 1, "object": 2, "objectlight": 2}
+
+
+def fill_light(dst, l):
+    if l["type"] not in LIGHT_TYPES:
+        raise ValueError(f"unsupported light type {l['type']!r}")
+    dst.type = LIGHT_TYPES[l["type"]]
+    for k in ("corner", "point1", "point2"):
+        setattr(dst, k, _f3v(l.get(k, (0, 0, 0))))
+    dst.from_ = _f3v(l.get("from", (0, 0, 0)))
+    dst.color = _f3v(l.get("color", (1, 1, 1)))
+    dst.power = float(l.get("power", 1.0))
+    dst.samples = int(l.get("samples", 4))
+    dst.cast_shadows = int(bool(l.get("cast_shadows", True)))
+    dst.first_tri = int(l.get("first_tri", 0))
+    dst.n_tris = int(l.get("n_tris", 0))
+    dst.double_sided = int(bool(l.get("double_sided", False)))
 
 
 class OraPtSettings(C.Structure):
@@ -233,14 +254,7 @@ def pt_structs(scene):
         fill_material(mats[i], m)
     lights = (OraLight * max(1, len(scene["lights"])))()
     for i, l in enumerate(scene["lights"]):
-        lights[i].type = 0 if l["type"] == "area" else 1
-        for k in ("corner", "point1", "point2"):
-            setattr(lights[i], k, _f3v(l.get(k, (0, 0, 0))))
-        lights[i].from_ = _f3v(l.get("from", (0, 0, 0)))
-        lights[i].color = _f3v(l.get("color", (1, 1, 1)))
-        lights[i].power = float(l.get("power", 1.0))
-        lights[i].samples = int(l.get("samples", 4))
-        lights[i].cast_shadows = 1
+        fill_light(lights[i], l)
     cam, s = scene["camera"], scene["settings"]
     st = OraPtSettings()
     st.from_ = _f3v(cam["from_"])
@@ -264,8 +278,15 @@ def _pt_lib():
     L = lib()
     if not getattr(L, "_pt_ready", False):
         vp = C.c_void_p
-        L.ora_pt_render.argtypes = [vp, vp, vp, vp, vp, vp, C.c_uint32, vp, vp, vp, C.c_int]
-        L.ora_pt_pixel_sample.argtypes = [vp, vp, vp, vp, vp, vp, C.c_uint32, vp, C.c_int, C.c_int, C.c_int, vp, vp]
+        L.ora_pt_render.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_uint32, vp, vp, vp, C.c_int]
+        L.ora_pt_pixel_sample.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_uint32, vp, C.c_int, C.c_int, C.c_int,
+                                          vp, vp]
+        L.ora_object_light_setup.argtypes = [vp, vp, vp, vp]
+        L.ora_object_light_setup.restype = C.c_float
+        L.ora_object_light_sample.argtypes = [vp, vp, vp, vp, C.c_float, C.c_float, vp]
+        L.ora_object_light_sample.restype = C.c_int
+        L.ora_object_light_intersect.argtypes = [vp, vp, vp, vp, vp, vp, C.c_float, C.POINTER(C.c_float)]
+        L.ora_object_light_intersect.restype = C.c_int
         L.ora_ri_vdc.argtypes = [C.c_uint32, C.c_uint32]
         L.ora_ri_vdc.restype = C.c_float
         L.ora_ri_s.argtypes = [C.c_uint32, C.c_uint32]
@@ -295,7 +316,8 @@ class PathTracer:
         self.verts = np.ascontiguousarray(scene["verts"], dtype=np.float32)
         self.tris = np.ascontiguousarray(scene["tris"], dtype=np.uint32)
         self.mat_id = np.ascontiguousarray(scene["mat_id"], dtype=np.uint32)
-        self.kd = kd or KdTree(self.verts, self.tris)
+        self.kd = kd or KdTree(self.verts, self.tris, scene.get("vis"))
+        self.vis = self.kd.vis
         self.mats, self.lights, self.st = pt_structs(scene)
         self.n_lights = len(scene["lights"])
 
@@ -307,7 +329,7 @@ class PathTracer:
         rgba = np.zeros((st.resy, st.resx, 4), dtype=np.float32)
         w = np.zeros((st.resy, st.resx), dtype=np.float32)
         L.ora_pt_render(self.kd._h, self.verts.ctypes.data, self.tris.ctypes.data, self.mat_id.ctypes.data,
-                        C.addressof(self.mats), C.addressof(self.lights), self.n_lights, C.addressof(st),
+                        self.vis.ctypes.data, C.addressof(self.mats), C.addressof(self.lights), self.n_lights, C.addressof(st),
                         rgba.ctypes.data, w.ctypes.data, threads or default_threads())
         return rgba, w
 
@@ -316,7 +338,7 @@ class PathTracer:
         out = np.zeros(4, dtype=np.float32)
         dxdy = np.zeros(2, dtype=np.float32)
         L.ora_pt_pixel_sample(self.kd._h, self.verts.ctypes.data, self.tris.ctypes.data, self.mat_id.ctypes.data,
-                              C.addressof(self.mats), C.addressof(self.lights), self.n_lights, C.addressof(self.st),
+                              self.vis.ctypes.data, C.addressof(self.mats), C.addressof(self.lights), self.n_lights, C.addressof(self.st),
                               int(x), int(y), int(sample), out.ctypes.data, dxdy.ctypes.data)
         return out, dxdy
 
@@ -426,3 +448,35 @@ def refract(n, wi, ior):
     n, wi = _v(n), _v(wi)
     ok = L.ora_refract(n.ctypes.data, wi.ctypes.data, float(ior), out.ctypes.data)
     return out if ok else None
+
+
+class ObjectLight:
+    """ObjectLight pieces of the oracle on one light dict (type "object") over a
+    triangle soup: area_, the area cdf, illumSample and intersect."""
+
+    def __init__(self, verts, tris, light, vis=None):
+        self.verts, self.tris, self.vis = _prep(verts, tris, vis)
+        self.l = OraLight()
+        fill_light(self.l, light)
+
+    def setup(self):
+        cdf = np.zeros(max(1, self.l.n_tris), dtype=np.float32)
+        area = _pt_lib().ora_object_light_setup(self.verts.ctypes.data, self.tris.ctypes.data, C.addressof(self.l),
+                                                cdf.ctypes.data)
+        return float(area), cdf[:self.l.n_tris]
+
+    def sample(self, p, s1, s2):
+        out = np.zeros(5, dtype=np.float32)
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        hit = _pt_lib().ora_object_light_sample(self.verts.ctypes.data, self.tris.ctypes.data, C.addressof(self.l),
+                                                p.ctypes.data, float(s1), float(s2), out.ctypes.data)
+        return hit, out[:3], float(out[3]), float(out[4])
+
+    def intersect(self, frm, d, tmin):
+        ipdf = C.c_float(0.0)
+        frm = np.ascontiguousarray(frm, dtype=np.float32)
+        d = np.ascontiguousarray(d, dtype=np.float32)
+        hit = _pt_lib().ora_object_light_intersect(self.verts.ctypes.data, self.tris.ctypes.data,
+                                                   self.vis.ctypes.data, C.addressof(self.l), frm.ctypes.data,
+                                                   d.ctypes.data, float(tmin), C.byref(ipdf))
+        return hit, float(ipdf.value)

@@ -182,3 +182,16 @@ def test_glass_plate_closed_form(oracle):
     col = grgba[0, 0, :3] / gw[0, 0]
     kr, kt = oracle.fresnel(np.float32([0, 0, 1]), np.float32([0, 0, 1]), 1.5)
     assert np.allclose(col, 3.0 * kt * kt, rtol=2e-6)
+
+
+@pytest.mark.parametrize("visibility,double_sided", [("no_shadows", False), ("normal", True), ("invisible", False)])
+def test_object_lights_match_oracle(oracle, visibility, double_sided):
+    """Two object lights (a 128-triangle sphere and a two-triangle panel) with
+    their area-cdf sampling, their own closest-hit test in the BSDF half of the
+    MIS, the unbounded shadow ray of that half, sidedness and the object's
+    visibility mask."""
+    sc = scenes.object_light_scene(n_random=2000, res=48, spp=4, visibility=visibility, double_sided=double_sided)
+    say(f"pt object lights 48x48x4 ({visibility}, double_sided={double_sided})")
+    g = render_gpu(sc)
+    say(f"  stats {g[2]}")
+    compare_films(g[:2], render_cpu(oracle, sc), f"object lights {visibility}")
