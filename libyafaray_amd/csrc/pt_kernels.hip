@@ -549,18 +549,15 @@ __device__ __forceinline__ bool guided_material(const PtArgs &a, const Surf &sp)
 	return a.g.on && a.mats[sp.mat].type == YMX_MAT_SHINYDIFFUSE && (sp.flags & kDiffuse);
 }
 
-// What the shinydiffuse sampler (sd_sample) returns as colour * weight had it
-// drawn wi, and the density with which it draws wi: the cosine lobe around the
-// face-forwarded normal, cos / pi.
-__device__ void sd_weight_density(const PtMaterial &m, const Surf &sp, V3 wo, V3 wi, Rgb &sw, float &ptrue)
+// What the shinydiffuse sampler (sd_sample) returns as colour and weight w had
+// it drawn the unit direction wi.
+__device__ Rgb sd_weight(const PtMaterial &m, const Surf &sp, V3 wo, V3 wi, float &w)
 {
 	const V3 n = face_forward(sp.ng, sp.n, wo);
-	const float cos_n = dot(wi, n);
 	const Rgb scolor = (dot(sp.ng, wo) * dot(sp.ng, wi) > 0) ? cscale(m.color, m.diffuse_reflect) : rgb(0.f, 0.f, 0.f);
-	const float pdf = fabsf(cos_n);  // width 1: the diffuse component is the only one
-	const float w = fabsf(dot(wi, sp.n)) / (pdf * 0.99f + 0.01f);
-	sw = cscale(scolor, w);
-	ptrue = cos_n > 0.f ? cos_n * k1Pi : 0.f;
+	const float pdf = fabsf(dot(wi, n));  // width 1: the diffuse component is the only one
+	w = fabsf(dot(wi, sp.n)) / (pdf * 0.99f + 0.01f);
+	return scolor;
 }
 
 // Counter-based uniforms of the guided sampling, distinct from the roulette stream.
@@ -572,16 +569,23 @@ __device__ __forceinline__ float guide_rand(const NodeCtx &x, uint32_t ps, uint3
 
 // Next direction at a guided vertex: the BSDF's own sample with probability
 // alpha, otherwise the mixture's; weight = F / (alpha p + (1 - alpha) q) with
-// F = sw * p the plain sampler's integrand.  Returns false when the weight is 0.
+// F = sw * p the plain sampler's integrand, p = cos / pi the density of its
+// cosine lobe and q the mixture's.  The BSDF branch keeps the sampler's own
+// direction and weight (its direction is unit only to the accuracy of the
+// fast sine and cosine); both densities are evaluated at the unit direction,
+// since a sharp lobe's exp(kappa (mu . w - 1)) amplifies any length error by
+// kappa.  w_plain: the sampler's w for wi (what a later vertex that leaves w
+// unset reuses).  Returns false when the weight is 0.
 __device__ bool guided_sample(const PtArgs &a, const Surf &sp, V3 wo, const float *mix, BSample &s, float u_sel,
-                              float u0, float u1, float u2, V3 &wi, Rgb &weight)
+                              float u0, float u1, float u2, V3 &wi, Rgb &weight, float &w_plain)
 {
 	const PtMaterial &m = a.mats[sp.mat];
 	const float alpha = a.g.bsdf_prob;
+	Rgb sw;
 	if(u_sel < alpha)
 	{
-		float w = 0.f;
-		(void)sd_sample(m, sp, wo, wi, s, w);
+		w_plain = 0.f;
+		sw = cscale(sd_sample(m, sp, wo, wi, s, w_plain), w_plain);
 	}
 	else
 	{
@@ -590,11 +594,12 @@ __device__ bool guided_sample(const PtArgs &a, const Surf &sp, V3 wo, const floa
 		wi = normalize(v3(x, y, z));
 		s.sampled = kDiffuse | kReflect;
 		s.pdf = 0.f;
+		sw = cscale(sd_weight(m, sp, wo, wi, w_plain), w_plain);
 	}
-	Rgb sw;
-	float p;
-	sd_weight_density(m, sp, wo, wi, sw, p);
-	const float q = ymx::guide::mix_pdf(mix, wi.x, wi.y, wi.z);
+	const V3 wn = normalize(wi);
+	const float cos_n = dot(wn, face_forward(sp.ng, sp.n, wo));
+	const float p = cos_n > 0.f ? cos_n * k1Pi : 0.f;
+	const float q = ymx::guide::mix_pdf(mix, wn.x, wn.y, wn.z);
 	const float denom = alpha * p + (1.f - alpha) * q;
 	if(!(p > 0.f) || !(denom > 0.f))
 	{
@@ -1314,8 +1319,7 @@ __global__ __launch_bounds__(kBlock) void k_subpath_init(PtArgs a)
 		const uint32_t ps = x.sample * (uint32_t)c.path_samples + (uint32_t)a.sub;
 		guided_sample(a, sp, wo, a.g.mix_node + (size_t)i * ymx::guide::kMix, s, guide_rand(x, ps, 0, c.seed, 0),
 		              guide_rand(x, ps, 0, c.seed, 1), guide_rand(x, ps, 0, c.seed, 2), guide_rand(x, ps, 0, c.seed, 3),
-		              pdir, scol);
-		w = 1.f;
+		              pdir, scol, w);
 		if(a.g.train)
 		{
 			// the path colour before this sub-path, throughput 1
@@ -1380,13 +1384,18 @@ __global__ __launch_bounds__(kBlock) void k_shade_path(PtArgs a)
 		{
 			const float random_value = rr_random(x.pixel, x.sample * (uint32_t)c.path_samples + (uint32_t)a.sub,
 			                                     (uint32_t)d + 256u * x.tag, c.seed);
-			const float probability = cmaximum(tp);
+			// The reference divides by the throughput's maximum even above 1, which
+			// scales such paths down; BSDF-sampled Lambertian paths never get there,
+			// guided ones do (weights up to albedo / alpha), so with guiding on a
+			// probability of 1 keeps the path unscaled.
+			float probability = cmaximum(tp);
+			if(a.g.on && probability > 1.f) probability = 1.f;
 			if(probability <= 0.f || probability < random_value)
 			{
 				a.st.flags[i] = f;
 				return;
 			}
-			tp = cscale(tp, 1.f / probability);
+			if(!(a.g.on && probability == 1.f)) tp = cscale(tp, 1.f / probability);
 		}
 	}
 	// A vertex with zero throughput adds lcol * 0 = 0 to the path colour: its
@@ -1442,8 +1451,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_path(PtArgs a)
 			const uint32_t seg = (uint32_t)d + 1u;
 			guided_sample(a, sp, pwo, a.g.mix_path + (size_t)q * ymx::guide::kMix, s, guide_rand(x, ps, seg, c.seed, 0),
 			              guide_rand(x, ps, seg, c.seed, 1), guide_rand(x, ps, seg, c.seed, 2),
-			              guide_rand(x, ps, seg, c.seed, 3), ndir, scol);
-			w = 1.f;
+			              guide_rand(x, ps, seg, c.seed, 3), ndir, scol, w);
 		}
 		else
 		{

@@ -141,43 +141,57 @@ def test_training_learns_a_position_dependent_lobe():
 
 
 # ---------------------------------------------------------------- guided path tracing
-def _render(sc, spp, guide=None, train_passes=0, seed=0):
+def _render(sc, spp, guide=None, train_passes=0, seed=0, seeds=(0,)):
+    """Film images (one per seed) of a plain or guided render; with
+    train_passes the guide first trains on that many 1-spp passes."""
     from libyafaray_amd.pt import GpuPathTracer
     s = dict(sc)
     s["settings"] = dict(sc["settings"], aa_samples=spp, seed=seed)
     pt = GpuPathTracer(s)
-    if guide is not None:
-        if train_passes:
-            pt.set_guide(guide, bsdf_prob=0.3, train=True, train_steps=8)
-            pt.clear()
-            for k in range(train_passes):
-                pt.render(k, 1)
-        pt.set_guide(guide, bsdf_prob=0.3, train=False)
-    pt.clear()
-    pt.render(0, spp)
-    rgba, w = pt.film()
+    if guide is not None and train_passes:
+        pt.set_guide(guide, bsdf_prob=0.3, train=True, train_steps=8)
+        pt.clear()
+        for k in range(train_passes):
+            pt.render(k, 1)
     pt.close()
-    return rgba[..., :3] / w[..., None]
+    imgs = []
+    for sd in seeds:
+        s["settings"] = dict(sc["settings"], aa_samples=spp, seed=sd)
+        pt = GpuPathTracer(s)
+        if guide is not None:
+            pt.set_guide(guide, bsdf_prob=0.3, train=False)
+        pt.clear()
+        pt.render(0, spp)
+        rgba, w = pt.film()
+        pt.close()
+        imgs.append(rgba[..., :3] / w[..., None])
+    return imgs
 
 
 def test_guided_render_converges_to_the_plain_image():
     """One-sample MIS keeps the estimator unbiased: with an untrained and with a
-    trained guide the image converges to the plain (BSDF-sampling) image -- the
-    image means agree within 0.5 % at 1024 spp (a wrong MIS weight is off by
-    several per cent) and the per-pixel relMSE against a 4096-spp plain render is
-    as small as the plain 1024-spp render's."""
+    trained guide, the image mean of independent guided renders (seeds 1..8,
+    256 spp each: the guided directions and the roulette draw from the seed)
+    agrees with a 4096-spp plain render's within four standard errors of
+    their spread (a wrong MIS weight or a biased roulette is off by per cents),
+    and the guided images are finite and of the same order of error."""
     from libyafaray_amd import scenes
     from libyafaray_amd.guide import GpuGuide
     sc = scenes.cornell_path_scene(n_random=2000, res=48, spp=16)
     bmin, bmax = sc["verts"].min(0) - 1e-3, sc["verts"].max(0) + 1e-3
-    ref = _render(sc, 4096, seed=99)
+    ref = _render(sc, 4096, seeds=(99,))[0]
     rel = lambda x: float(((x - ref) ** 2 / (ref ** 2 + 1e-2)).mean())
-    plain = _render(sc, 1024)
-    say(f"  plain 1024 spp: mean {plain.mean():.5f} (reference {ref.mean():.5f}), relMSE {rel(plain):.2e}")
+    plain = _render(sc, 256)[0]
+    say(f"  plain 256 spp: mean {plain.mean():.5f} (reference {ref.mean():.5f}), relMSE {rel(plain):.2e}")
     for train in (0, 8):
         g = GpuGuide(bmin, bmax, log2_table_size=16, max_batch=1 << 14, seed=3)
-        img = _render(sc, 1024, guide=g, train_passes=train)
+        imgs = _render(sc, 256, guide=g, train_passes=train, seeds=tuple(range(1, 9)))
         g.close()
-        say(f"  guided ({train} training passes) 1024 spp: mean {img.mean():.5f}, relMSE {rel(img):.2e}")
-        assert abs(img.mean() / ref.mean() - 1) < 5e-3
-        assert rel(img) < 3 * rel(plain) + 1e-5
+        means = np.array([im.mean() for im in imgs], np.float64)
+        m, se = means.mean(), means.std(ddof=1) / np.sqrt(len(means))
+        errs = [rel(im) for im in imgs]
+        say(f"  guided ({train} training passes) 8 x 256 spp: mean {m:.5f} +- {se:.5f}, "
+            f"ratio {m / ref.mean():.4f}, relMSE median {np.median(errs):.2e}")
+        assert all(np.isfinite(im).all() for im in imgs)
+        assert abs(m - ref.mean()) < 4 * se + 1e-3 * ref.mean()
+        assert np.median(errs) < 20 * rel(plain)
