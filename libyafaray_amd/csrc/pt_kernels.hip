@@ -584,6 +584,13 @@ __device__ bool guided_sample(const PtArgs &a, const Surf &sp, V3 wo, const floa
 	Rgb sw;
 	if(u_sel < alpha)
 	{
+		// The BSDF branch draws its direction from the guide's own uniforms, not
+		// from the sample's low-discrepancy dimensions: those are correlated with
+		// the light samples of the next vertex, and weighting them by the MIS
+		// factor p / (alpha p + (1 - alpha) q) would no longer average to the
+		// plain estimate (it leaves the camera-vertex bounce about 1 % high).
+		s.s1 = u1;
+		s.s2 = u2;
 		w_plain = 0.f;
 		sw = cscale(sd_sample(m, sp, wo, wi, s, w_plain), w_plain);
 	}

@@ -445,13 +445,19 @@ void ora_kd_bound(const ora_kdtree *kd, float a[3], float g[3])
 }
 
 uint32_t ora_kd_num_nodes(const ora_kdtree *kd) { return kd->n_nodes; }
+uint32_t ora_kd_num_prims(const ora_kdtree *kd) { return kd->n_tris; }
 
 /* ------------------------------------------------------------------ */
 /* traversal: kdtree::intersect<Node, Stack, T>                         */
 
 typedef struct { int32_t node; float t; float point[3]; int prev; } kstack;
 
-enum { TEST_NEAREST = 0, TEST_SHADOW = 1 };
+enum { TEST_NEAREST = 0, TEST_SHADOW = 1, TEST_TRANSPARENT = 2 };
+
+/* TransparentShadow state: the per-primitive transparency flags, the depth
+ * limit and the transparent primitives met so far (the reference's `filtered`
+ * set and depth counter). */
+typedef struct { const uint8_t *transparent; int max_depth; uint32_t *prims; int count; } transp_state;
 
 static inline int prim_test(const ora_kdtree *kd, uint32_t p, const float from[3], const float dir[3],
                             float *t, float *u, float *v)
@@ -462,8 +468,9 @@ static inline int prim_test(const ora_kdtree *kd, uint32_t p, const float from[3
 }
 
 /* accelerator_kdtree_common.h:107-255.  Returns 1 for a hit (closest: *h set;
- * shadow: *h->prim is the occluder). */
-static int kd_intersect(const ora_kdtree *kd, const ora_ray *ray, float t_max, int test, ora_hit *h)
+ * shadow and transparent shadow: *h->prim is the occluder). */
+static int kd_intersect(const ora_kdtree *kd, const ora_ray *ray, float t_max, int test, ora_hit *h,
+                        transp_state *ts)
 {
 	h->prim = ORA_NO_HIT; h->t = 0.f; h->u = 0.f; h->v = 0.f;
 	float enter, leave;
@@ -535,7 +542,7 @@ static int kd_intersect(const ora_kdtree *kd, const ora_ray *ray, float t_max, i
 				h->prim = p; h->t = t; h->u = u; h->v = v;
 				hit = 1;
 			}
-			else
+			else if(test == TEST_SHADOW)
 			{
 				/* Accelerator::primitiveIntersectionShadow, accelerator.h:135-145 */
 				if(!prim_test(kd, p, ray->org, ray->dir, &t, &u, &v)) continue;
@@ -543,6 +550,29 @@ static int kd_intersect(const ora_kdtree *kd, const ora_ray *ray, float t_max, i
 				if(!(kd->vis[p] & ORA_VIS_SHADOWS)) continue;
 				h->prim = p; h->t = t; h->u = u; h->v = v;
 				return 1;
+			}
+			else
+			{
+				/* Accelerator::primitiveIntersectionTransparentShadow, accelerator.h:147-168:
+				 * an opaque caster occludes; a transparent one met for the first time
+				 * is kept, and occludes once max_depth of them are kept */
+				if(!prim_test(kd, p, ray->org, ray->dir, &t, &u, &v)) continue;
+				if(t <= 0.f || t < t_min || t >= t_max) continue;
+				if(!(kd->vis[p] & ORA_VIS_SHADOWS)) continue;
+				if(!ts->transparent[p])
+				{
+					h->prim = p; h->t = t; h->u = u; h->v = v;
+					return 1;
+				}
+				int seen = 0;
+				for(int k = 0; k < ts->count; ++k) seen |= (ts->prims[k] == p);
+				if(seen) continue;
+				if(ts->count >= ts->max_depth)
+				{
+					h->prim = p; h->t = t; h->u = u; h->v = v;
+					return 1;
+				}
+				ts->prims[ts->count++] = p;
 			}
 		}
 		if(test == TEST_NEAREST && hit && cur_tmax <= stack[exit_id].t) return 1;
@@ -557,7 +587,17 @@ static int kd_intersect(const ora_kdtree *kd, const ora_ray *ray, float t_max, i
 
 int ora_kd_intersect1(const ora_kdtree *kd, const ora_ray *ray, float t_max, int shadow, ora_hit *hit)
 {
-	return kd_intersect(kd, ray, t_max, shadow ? TEST_SHADOW : TEST_NEAREST, hit);
+	return kd_intersect(kd, ray, t_max, shadow ? TEST_SHADOW : TEST_NEAREST, hit, NULL);
+}
+
+int ora_kd_transparent1(const ora_kdtree *kd, const ora_ray *ray, float t_max, const uint8_t *transparent,
+                        int max_depth, uint32_t *prims, int *count)
+{
+	transp_state ts = {transparent, max_depth, prims, 0};
+	ora_hit h;
+	const int occ = kd_intersect(kd, ray, t_max, TEST_TRANSPARENT, &h, &ts);
+	*count = ts.count;
+	return occ;
 }
 
 typedef struct
@@ -579,7 +619,7 @@ static void *kd_worker(void *arg)
 		const ora_ray *ray = &j->rays[r];
 		const float t_max = (ray->tmax >= 0.f) ? ray->tmax : FLT_MAX;
 		ora_hit h;
-		const int hit = kd_intersect(j->kd, ray, t_max, j->test, &h);
+		const int hit = kd_intersect(j->kd, ray, t_max, j->test, &h, NULL);
 		if(j->test == TEST_NEAREST) j->hits[r] = h;
 		else
 		{

@@ -273,6 +273,7 @@ typedef struct
 	const ora_light *lights;
 	uint32_t n_lights;
 	struct obj_light *objs;   /* [n_lights], set up for ORA_LIGHT_OBJECT lights */
+	uint8_t *transparent;     /* per triangle, with transparent shadows: its material isTransparent() */
 	const ora_pt_settings *st;
 	/* camera (src/camera/camera.cc:89-110, camera_perspective.cc:97-111) */
 	v3 cam_from, vright, vup, vto;
@@ -863,6 +864,41 @@ static rgb mat_emit(const surf *sp, v3 wo)
 	return C(0.f, 0.f, 0.f);
 }
 
+/* GlassMaterial::getTransparency (src/material/material_glass.cc:297-304) at
+ * a triangle met by a shadow ray along dir: kt of the Fresnel terms about the
+ * face normal turned towards dir, times the filter colour. */
+static rgb transparency(const pt_ctx *c, uint32_t prim, v3 dir)
+{
+	const uint32_t *ix = c->idx + 3 * (size_t)prim;
+	const v3 p0 = vload(c->verts + 3 * (size_t)ix[0]);
+	const v3 ng = vnorm(vcross(vsub(vload(c->verts + 3 * (size_t)ix[1]), p0), vsub(vload(c->verts + 3 * (size_t)ix[2]), p0)));
+	const ora_material *m = &c->mats[c->mat_id[prim]];
+	float kr, kt;
+	fresnel(dir, face_forward(ng, ng, dir), m->ior, &kr, &kt);
+	return cscale(glass_filter(m), kt);
+}
+
+/* Accelerator::isShadowed, or with the integrator's "transpShad"
+ * isShadowedTransparentShadow (include/accelerator/accelerator.h:103-119):
+ * returns 1 when shadowed; otherwise *scol is the product, in the order the
+ * kd-tree meets them, of the transparencies of the transparent shadow casters
+ * on the way (IntersectData::color_, starting at 1). */
+static int shadow_ray(const pt_ctx *c, v3 from, v3 dir, float tmin, float tmax, rgb *scol)
+{
+	*scol = C(1.f, 1.f, 1.f);
+	if(!c->st->transparent_shadows) return is_shadowed(c, from, dir, tmin, tmax);
+	const v3 f = vadd(from, vmul(tmin, dir));
+	const float t_max = (tmax >= 0.f) ? tmax - 2 * tmin : FLT_MAX;
+	ora_ray r = {{f.x, f.y, f.z}, tmin, {dir.x, dir.y, dir.z}, tmax};
+	uint32_t prims[ORA_MAX_SHADOW_DEPTH];
+	int cnt = 0;
+	if(ora_kd_transparent1(c->kd, &r, t_max, c->transparent, c->st->shadow_depth, prims, &cnt)) return 1;
+	rgb col = C(1.f, 1.f, 1.f);
+	for(int k = 0; k < cnt; ++k) col = cmul(col, transparency(c, prims[k], dir));
+	*scol = col;
+	return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* lights                                                                */
 
@@ -1139,12 +1175,14 @@ static rgb light_estimation(const pt_ctx *c, const ora_light *l, const surf *sp,
 		ldir = vmul(1.f / dist, ldir);
 		const rgb lcol = cscale(light_color(l), idist_sqr);
 		int shadowed = 0;
-		if(cast_shadows) shadowed = is_shadowed(c, sp->p, ldir, shadow_tmin, dist);
+		rgb scol = C(1.f, 1.f, 1.f);
+		if(cast_shadows) shadowed = shadow_ray(c, sp->p, ldir, shadow_tmin, dist, &scol);
 		const float angle = fabsf(vdot(sp->n, ldir));
 		if(!shadowed)
 		{
 			const rgb surf_col = bsdf_eval(sp, wo, ldir, F_ALL);
-			col = cadd(col, cmul(cscale(cmul(surf_col, lcol), angle), C(1.f, 1.f, 1.f)));
+			const rgb lc = (c->st->transparent_shadows && cast_shadows) ? cmul(lcol, scol) : lcol;
+			col = cadd(col, cmul(cscale(cmul(surf_col, lc), angle), C(1.f, 1.f, 1.f)));
 		}
 		return col;
 	}
@@ -1168,9 +1206,11 @@ static rgb light_estimation(const pt_ctx *c, const ora_light *l, const surf *sp,
 		float dist, lpdf;
 		if(!illum_sample(c, l, o, sp->p, s1, s2, &ldir, &dist, &lpdf)) continue;
 		int shadowed = 0;
-		if(cast_shadows) shadowed = is_shadowed(c, sp->p, ldir, shadow_tmin, dist);
+		rgb scol = C(1.f, 1.f, 1.f);
+		if(cast_shadows) shadowed = shadow_ray(c, sp->p, ldir, shadow_tmin, dist, &scol);
 		if(!shadowed && lpdf > 1e-6f)
 		{
+			const rgb lc = (c->st->transparent_shadows && cast_shadows) ? cmul(lcolor, scol) : lcolor;
 			const rgb surf_col = bsdf_eval(sp, wo, ldir, F_ALL);
 			const float angle = fabsf(vdot(sp->n, ldir));
 			float w = 1.f;
@@ -1181,7 +1221,7 @@ static rgb light_estimation(const pt_ctx *c, const ora_light *l, const surf *sp,
 				const float m_2 = m_pdf * m_pdf;
 				w = l_2 / (l_2 + m_2);
 			}
-			acc = cadd(acc, cdiv(cscale(cscale(cmul(surf_col, lcolor), angle), w), lpdf));
+			acc = cadd(acc, cdiv(cscale(cscale(cmul(surf_col, lc), angle), w), lpdf));
 		}
 	}
 	col = cadd(col, cscale(acc, inv_num_samples));
@@ -1208,14 +1248,16 @@ static rgb light_estimation(const pt_ctx *c, const ora_light *l, const surf *sp,
 		if(light_hit && s.pdf > 1e-6f)
 		{
 			int shadowed = 0;
-			if(cast_shadows) shadowed = is_shadowed(c, sp->p, dir, btmin, t);
+			rgb scol = C(1.f, 1.f, 1.f);
+			if(cast_shadows) shadowed = shadow_ray(c, sp->p, dir, btmin, t, &scol);
 			if(!shadowed && ipdf > 1e-6f)
 			{
+				const rgb lc = (c->st->transparent_shadows && cast_shadows) ? cmul(lcolor, scol) : lcolor;
 				const float l_pdf = 1.f / ipdf;
 				const float l_2 = l_pdf * l_pdf;
 				const float m_2 = s.pdf * s.pdf;
 				const float w = m_2 / (l_2 + m_2);
-				acc = cadd(acc, cscale(cscale(cmul(surf_col, lcolor), w), W));
+				acc = cadd(acc, cscale(cscale(cmul(surf_col, lc), w), W));
 			}
 		}
 	}
@@ -1424,6 +1466,8 @@ static void integrate(const pt_ctx *c, v3 from, v3 dir, float tmin, int ray_leve
 
 static void free_ctx(pt_ctx *c)
 {
+	free(c->transparent);
+	c->transparent = NULL;
 	if(!c->objs) return;
 	for(uint32_t l = 0; l < c->n_lights; ++l) free(c->objs[l].cdf);
 	free(c->objs);
@@ -1438,10 +1482,28 @@ static int setup_ctx(pt_ctx *c, const ora_kdtree *kd, const float *verts, const 
 	c->kd = kd; c->verts = verts; c->idx = tri_idx; c->mat_id = mat_id; c->vis = vis; c->mats = mats;
 	c->lights = lights; c->n_lights = n_lights; c->st = st;
 	c->objs = NULL;
+	c->transparent = NULL;
+	if(st->transparent_shadows && kd)
+	{
+		/* isTransparent(): glass with fake_shadows (include/material/material_glass.h:82) */
+		const uint32_t n = ora_kd_num_prims(kd);
+		if(st->shadow_depth < 0 || st->shadow_depth > ORA_MAX_SHADOW_DEPTH ||
+		   !(c->transparent = (uint8_t *)calloc(n ? n : 1, 1)))
+			return 0;
+		for(uint32_t p = 0; p < n; ++p)
+		{
+			const ora_material *m = &mats[mat_id[p]];
+			c->transparent[p] = (m->type == ORA_MAT_GLASS && m->fake_shadows) ? 1 : 0;
+		}
+	}
 	for(uint32_t l = 0; l < n_lights; ++l)
 	{
 		if(lights[l].type != ORA_LIGHT_OBJECT) continue;
-		if(!c->objs && !(c->objs = (struct obj_light *)calloc(n_lights, sizeof(struct obj_light)))) return 0;
+		if(!c->objs && !(c->objs = (struct obj_light *)calloc(n_lights, sizeof(struct obj_light))))
+		{
+			free_ctx(c);
+			return 0;
+		}
 		if(lights[l].n_tris == 0 || !obj_light_init(c, &lights[l], &c->objs[l]))
 		{
 			free_ctx(c);

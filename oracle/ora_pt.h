@@ -111,7 +111,11 @@ typedef struct
 	float background[3];
 	uint32_t seed;
 	int32_t raydepth;        /* MonteCarloIntegrator "raydepth" (5) */
+	int32_t transparent_shadows;  /* "transpShad" (false) */
+	int32_t shadow_depth;    /* "shadowDepth" (4), at most ORA_MAX_SHADOW_DEPTH */
 } ora_pt_settings;
+
+#define ORA_MAX_SHADOW_DEPTH 64
 
 /* Render aa_samples samples per pixel into film sums: film_rgba[4 * (y * resx + x)]
  * (colour * weight, alpha * weight) and film_w[y * resx + x] (weights).

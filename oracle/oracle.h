@@ -90,10 +90,19 @@ ora_kdtree *ora_kd_build(const float *verts, const uint32_t *tri_idx, const uint
 void ora_kd_free(ora_kdtree *kd);
 void ora_kd_bound(const ora_kdtree *kd, float a[3], float g[3]);
 uint32_t ora_kd_num_nodes(const ora_kdtree *kd);
+uint32_t ora_kd_num_prims(const ora_kdtree *kd);
 
 /* One ray through the kd-tree port: shadow = 0 closest (t_max rule of
  * Accelerator::intersect applied by the caller), 1 any hit.  Returns 1 on hit. */
 int ora_kd_intersect1(const ora_kdtree *kd, const ora_ray *ray, float t_max, int shadow, ora_hit *hit);
+
+/* accelerator_kdtree_common.h:107-255, TransparentShadow, with
+ * Accelerator::primitiveIntersectionTransparentShadow (accelerator.h:147-168):
+ * t_min = max(ray tmin, dynamic bias); returns 1 when occluded (an opaque
+ * shadow caster, or a transparent one beyond max_depth of them), else 0 with
+ * the transparent[] shadow casters met, in traversal order, in prims[0 .. *count). */
+int ora_kd_transparent1(const ora_kdtree *kd, const ora_ray *ray, float t_max, const uint8_t *transparent,
+                        int max_depth, uint32_t *prims, int *count);
 
 /* include/accelerator/accelerator_kdtree_common.h:107-255, Nearest. */
 void ora_kd_closest(const ora_kdtree *kd, const ora_ray *rays, uint32_t n_rays,
