@@ -153,6 +153,7 @@ struct PtGuide
 	float4 *base;              // [bounces][capacity] path colour before the record's segment, w: lum(throughput)
 	uint32_t *mask;            // [capacity] bit r: record r is pending
 	ymx_guide_sample *train_buf;  // collected training samples
+	ymx_guide_sample *batch_buf;  // one mini-batch gathered from the ring
 	uint32_t *train_count;
 	uint32_t train_cap;
 	uint32_t *query_count;     // queries of the current path-vertex batch

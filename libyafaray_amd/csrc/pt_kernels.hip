@@ -1551,6 +1551,22 @@ __global__ __launch_bounds__(kBlock) void k_guide_collect(PtArgs a)
 	}
 }
 
+// One training mini-batch: dst[k] = ring[h(k) mod n] for a hashed index h.
+__global__ __launch_bounds__(kBlock) void k_guide_gather(ymx_guide_sample *dst, const ymx_guide_sample *ring, uint32_t n,
+                                                         uint32_t batch, uint32_t seed)
+{
+	const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+	if(k >= batch) return;
+	uint32_t h = (k + 0x9E3779B9u * seed) * 0x85EBCA6Bu;
+	h ^= h >> 13;
+	h *= 0xC2B2AE35u;
+	h ^= h >> 16;
+	h ^= seed;
+	h *= 0x27D4EB2Fu;
+	h ^= h >> 15;
+	dst[k] = ring[(uint32_t)(((uint64_t)h * n) >> 32)];
+}
+
 // Queries of the guide at the node vertices that start sub-paths.
 __global__ __launch_bounds__(kBlock) void k_guide_query_node(PtArgs a)
 {
@@ -1885,6 +1901,7 @@ int ymx_pt_trace_pass(ymx_pt *pt, int32_t first_sample, int32_t count, hipStream
 // quarter of the ring (so it holds the last few wavefronts).
 int ymx_pt_train_guide(ymx_pt *pt, hipStream_t stream)
 {
+	using namespace ymx;
 	uint32_t total = 0;
 	YMX_HIP_CHECK(hipMemcpyAsync(&total, pt->g.train_count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
 	YMX_HIP_CHECK(hipStreamSynchronize(stream));
@@ -1900,10 +1917,14 @@ int ymx_pt_train_guide(ymx_pt *pt, hipStream_t stream)
 		const uint32_t batch = n < max_batch ? n : max_batch;
 		for(uint32_t k = 0; k < pt->train_steps; ++k)
 		{
-			// a contiguous slice at a pseudo-random offset (the ring's order is the GPU's)
+			// samples drawn uniformly (with replacement) from the whole ring: its
+			// order follows the wavefront's slots, so a contiguous slice would be
+			// one band of the image
 			pt->g_rng = pt->g_rng * 6364136223846793005ull + 1442695040888963407ull;
-			const uint32_t off = n > batch ? (uint32_t)((pt->g_rng >> 33) % (uint64_t)(n - batch + 1)) : 0u;
-			const int rc = ymx_guide_train_step(pt->guide, pt->g.train_buf + off, batch, nullptr, stream);
+			k_guide_gather<<<blocks_for(batch), kBlock, 0, stream>>>(pt->g.batch_buf, pt->g.train_buf, n, batch,
+			                                                          (uint32_t)(pt->g_rng >> 32));
+			YMX_HIP_CHECK(hipGetLastError());
+			const int rc = ymx_guide_train_step(pt->guide, pt->g.batch_buf, batch, nullptr, stream);
 			if(rc != YMX_OK) return rc;
 		}
 	}
