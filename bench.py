@@ -49,6 +49,11 @@ def parse():
                    help="run only a plain 1 GiB device-to-device copy (HBM counter calibration)")
     p.add_argument("--no-pt", action="store_true", help="skip the path-tracing measurement")
     p.add_argument("--pt-only", action="store_true", help="only the path-tracing measurement")
+    p.add_argument("--no-guided", action="store_true", help="skip the guided path-tracing measurement")
+    p.add_argument("--guided-only", action="store_true", help="only the guided path-tracing measurement")
+    p.add_argument("--guided-res", type=str, default="1280x720")
+    p.add_argument("--guided-ref-spp", type=int, default=4096)
+    p.add_argument("--guided-spp", type=int, default=64)
     return p.parse_args()
 
 
@@ -90,6 +95,93 @@ def measure_path_tracer(args, torch):
         out["cpu_port_threads"] = threads
         out["speedup_vs_cpu_port"] = round(cpu_s / best, 1)
     log(args, f"path tracer: {out}")
+    return out
+
+
+def measure_guiding(args, torch):
+    """Occluded room (4 x 3 x 3 m, about 533k triangles, lit through a 0.1 m
+    slit from an adjacent room; max depth 10): per-pixel relMSE
+    (I - R)^2 / (R^2 + 0.01) of plain and guided renders of `guided_spp`
+    samples against a `guided_ref_spp` plain reference (two independent
+    halves, whose own error is subtracted), render times, and the time to
+    relMSE 1e-3 extrapolated with relMSE proportional to 1 / samples (both
+    estimators are unbiased).  The guide (16-level hash grid of 2^19 entries x
+    2 features, 3 x 64 MLP, 8 vMF lobes, BF16 MFMA, Adam lr 1e-2) trains on the
+    radiance samples of the first 32 spp; its training time is charged to the
+    guided render.  Guided sampling mixes the BSDF sampler (probability
+    bsdf_prob) and the learned mixture by one-sample MIS."""
+    from libyafaray_amd import scenes
+    from libyafaray_amd.guide import GpuGuide
+    from libyafaray_amd.pt import GpuPathTracer
+    resx, resy = (int(v) for v in args.guided_res.split("x"))
+    sc = scenes.occluded_room_scene(res=(resx, resy), spp=16)
+    spp, ref_spp = args.guided_spp, args.guided_ref_spp
+
+    def tracer(aa, seed):
+        s = dict(sc)
+        s["settings"] = dict(sc["settings"], aa_samples=aa, seed=seed)
+        return GpuPathTracer(s)
+
+    def image(pt):
+        rgba, w = pt.film()
+        return rgba[..., :3] / w[..., None]
+
+    def relmse(a, b):
+        return float(((a.astype(np.float64) - b) ** 2 / (b.astype(np.float64) ** 2 + 1e-2)).mean())
+
+    pt = tracer(ref_spp // 2, 777)
+    pt.clear()
+    pt.render(0, ref_spp // 2)
+    r1 = image(pt)
+    pt.clear()
+    pt.render(ref_spp // 2, ref_spp // 2)
+    r2 = image(pt)
+    pt.close()
+    ref = (r1 + r2) / 2
+    ref_err = relmse(r1, r2) / 4
+
+    def timed(pt, first, n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pt.clear()
+        pt.render(first, n)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    p = tracer(spp, 0)
+    timed(p, 0, 1)
+    t_plain = timed(p, 0, spp)
+    e_plain = relmse(image(p), ref) - ref_err
+    bmin, bmax = sc["verts"].min(0) - 1e-3, sc["verts"].max(0) + 1e-3
+    alpha, steps, passes = 0.5, 4, 32
+    g = GpuGuide(bmin, bmax, log2_table_size=19, n_min=4, n_max=32, max_batch=1 << 16, lr=1e-2, seed=1)
+    p.set_guide(g, bsdf_prob=alpha, train=True, train_steps=steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    p.clear()
+    for k in range(passes):
+        p.render(1_000_000 + k, 1)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    p.set_guide(g, bsdf_prob=alpha, train=False)
+    t_guided = timed(p, 0, spp)
+    e_guided = relmse(image(p), ref) - ref_err
+    p.set_guide(None)
+    g.close()
+    p.close()
+    target = 1e-3
+    tt_plain = t_plain * e_plain / target
+    tt_guided = t_train + t_guided * e_guided / target
+    out = {"workload": f"occluded room, {len(sc['tris'])} tris, {resx}x{resy}, depth 10, slit 0.1 m",
+           "reference_spp": ref_spp, "reference_relmse": ref_err, "spp": spp,
+           "plain_ms": round(t_plain * 1e3, 1), "guided_ms": round(t_guided * 1e3, 1),
+           "train_ms": round(t_train * 1e3, 1), "train": f"{passes} passes of 1 spp, {steps} Adam steps each",
+           "bsdf_prob": alpha, "grid_levels": "16 (resolution 4..32)",
+           "relmse_plain": e_plain, "relmse_guided": e_guided,
+           "equal_spp_gain": round(e_plain / e_guided, 3) if e_guided > 0 else None,
+           "time_to_relmse_1e-3_plain_s": round(tt_plain, 2), "time_to_relmse_1e-3_guided_s": round(tt_guided, 2),
+           "guided_over_plain_time": round(tt_guided / tt_plain, 3) if tt_plain > 0 else None}
+    log(args, f"guided path tracing: {out}")
     return out
 
 
@@ -147,6 +239,9 @@ def main():
         return
     if args.pt_only:
         print(json.dumps({"path_tracing": measure_path_tracer(args, torch)}), flush=True)
+        return
+    if args.guided_only:
+        print(json.dumps({"guided_path_tracing": measure_guiding(args, torch)}), flush=True)
         return
 
     t0 = time.perf_counter()
@@ -250,6 +345,8 @@ def main():
     }
     if not args.no_pt:
         out["extra"]["path_tracing"] = measure_path_tracer(args, torch)
+    if not args.no_guided:
+        out["extra"]["guided_path_tracing"] = measure_guiding(args, torch)
     print(json.dumps(out), flush=True)
 
 

@@ -129,6 +129,23 @@ int ymx_accel_intersect_shadow(const ymx_accel *accel, const ymx_ray *d_rays,
                                uint32_t *d_occluded, uint32_t *d_prim, uint32_t n_rays,
                                void *stream);
 
+/* Transparent-shadow query for n rays: Accelerator::intersectTransparentShadow
+ * (include/accelerator/accelerator.h:113-119,147-168,
+ * include/accelerator/accelerator_kdtree_common.h:107-255, TransparentShadow),
+ * called by isShadowedTransparentShadow when the integrator's "transpShad" is
+ * set.  t_min = max(ray.tmin, dynamic bias), t_max as in the any-hit query.
+ * d_transparent: device bitmask, bit p % 32 of word p / 32 set when primitive
+ * p's material is transparent (Material::isTransparent).  A ray is occluded
+ * by an opaque shadow caster, or by more than max_depth transparent ones.
+ * d_occluded as in ymx_accel_intersect_shadow; d_counts[i]: the transparent
+ * casters kept for an unoccluded ray (0 when occluded); d_hits[i * max_depth +
+ * k] for k < d_counts[i]: (primitive id, t as float bits), sorted by t (the
+ * caller multiplies their Material::getTransparency, as the reference's
+ * IntersectData::color_ does in the order its kd-tree meets them). */
+int ymx_accel_intersect_transparent(const ymx_accel *accel, const ymx_ray *d_rays, const uint32_t *d_transparent,
+                                    uint32_t max_depth, uint32_t *d_occluded, uint32_t *d_counts, uint32_t *d_hits,
+                                    uint32_t n_rays, void *stream);
+
 /* Counting replay of the query kernels: total 64-byte nodes fetched and
  * triangles tested over the batch.  query: 0 closest, 1 shadow.  Synchronous;
  * the counts are written to host memory. */

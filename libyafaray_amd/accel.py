@@ -130,6 +130,25 @@ class GpuAccelerator:
         torch.cuda.synchronize()
         return unpack_hits(h.cpu().numpy())
 
+    def intersect_transparent(self, rays: np.ndarray, transparent: np.ndarray, max_depth: int):
+        """Transparent-shadow query for host rays; transparent: one flag per
+        primitive.  Returns (occluded bool[n], counts[n], hits (n, max_depth, 2)
+        of (primitive, t))."""
+        torch = _torch()
+        r = torch.from_numpy(np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, RAY_FLOATS)).cuda()
+        n = r.shape[0]
+        mask = torch.from_numpy(transparent_mask(transparent).view(np.int32)).cuda()
+        occ = torch.empty(((n + 31) // 32,), dtype=torch.int32, device=r.device)
+        counts = torch.empty((n,), dtype=torch.int32, device=r.device)
+        hits = torch.zeros((max(1, n * max_depth), 2), dtype=torch.int32, device=r.device)
+        rc = self._lib.ymx_accel_intersect_transparent(self._h, r.data_ptr(), mask.data_ptr(), int(max_depth),
+                                                       occ.data_ptr(), counts.data_ptr(), hits.data_ptr(), n,
+                                                       self._stream_ptr(None))
+        check(rc, "ymx_accel_intersect_transparent")
+        torch.cuda.synchronize()
+        h = hits.cpu().numpy().view(np.uint32)[: n * max_depth].reshape(n, max_depth, 2)
+        return unpack_bits(occ.cpu().numpy(), n), counts.cpu().numpy().view(np.uint32), h
+
     def intersect_shadow(self, rays: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
         torch = _torch()
         r = torch.from_numpy(np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, RAY_FLOATS)).cuda()
@@ -138,6 +157,15 @@ class GpuAccelerator:
         occ = self.intersect_shadow_device(r, prim=prim)
         torch.cuda.synchronize()
         return unpack_bits(occ.cpu().numpy(), n), prim.cpu().numpy().view(np.uint32)
+
+
+def transparent_mask(flags: np.ndarray) -> np.ndarray:
+    """Per-primitive transparency flags -> the query's bitmask words (uint32)."""
+    f = np.ascontiguousarray(flags, dtype=bool).reshape(-1)
+    words = np.zeros((len(f) + 31) // 32 + 1, dtype=np.uint32)
+    idx = np.nonzero(f)[0]
+    np.bitwise_or.at(words, idx >> 5, (np.uint32(1) << (idx & 31).astype(np.uint32)))
+    return words
 
 
 def unpack_hits(h: np.ndarray) -> dict:

@@ -197,6 +197,23 @@ int ymx_accel_intersect_shadow(const ymx_accel *accel, const ymx_ray *d_rays, ui
 	return ymx_launch_shadow(accel, d_rays, d_occluded, d_prim, n_rays, (hipStream_t)stream, nullptr);
 }
 
+int ymx_accel_intersect_transparent(const ymx_accel *accel, const ymx_ray *d_rays, const uint32_t *d_transparent,
+                                    uint32_t max_depth, uint32_t *d_occluded, uint32_t *d_counts, uint32_t *d_hits,
+                                    uint32_t n_rays, void *stream)
+{
+	if(n_rays == 0) return YMX_OK;
+	int rc = check_count(n_rays, "ymx_accel_intersect_transparent");
+	if(rc == YMX_OK) rc = check_query(accel, d_rays, d_occluded, "ymx_accel_intersect_transparent");
+	if(rc != YMX_OK) return rc;
+	if(!d_transparent || !d_counts || (max_depth > 0 && !d_hits) || max_depth > 64)
+	{
+		ymx_set_error("ymx_accel_intersect_transparent: NULL buffer or max_depth %u above 64", max_depth);
+		return YMX_ERR_INVALID_ARG;
+	}
+	return ymx_launch_transparent(accel, d_rays, d_transparent, max_depth, d_occluded, d_counts,
+	                              reinterpret_cast<uint2 *>(d_hits), n_rays, nullptr, (hipStream_t)stream);
+}
+
 int ymx_accel_traversal_stats(const ymx_accel *accel, const ymx_ray *d_rays, uint32_t n_rays, int query,
                               uint64_t *node_fetches, uint64_t *tri_tests, void *stream)
 {

@@ -70,6 +70,12 @@ struct ymx_trace_args
 	uint32_t *occluded;
 	uint32_t *occ_prim;
 	unsigned long long *stats;
+	// transparent shadows: bit p of transp set when primitive p's material is
+	// transparent; per ray up to max_depth (prim, t bits) kept, sorted by t
+	const uint32_t *transp;
+	uint32_t max_depth;
+	uint32_t *t_count;
+	uint2 *t_hits;
 };
 
 // Build entry (accel_build.hip) and launch entries (accel_trace.hip).
@@ -80,6 +86,10 @@ int ymx_launch_closest(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits,
 int ymx_launch_shadow(const ymx_accel *acc, const ymx_ray *rays, uint32_t *occ, uint32_t *prim,
                       uint32_t n, hipStream_t stream, unsigned long long *stats);
 int ymx_trace_setup(ymx_accel *acc);
+// Transparent-shadow any-hit of n rays (or *n_dev when given) (accel_trace.hip).
+int ymx_launch_transparent(const ymx_accel *acc, const ymx_ray *rays, const uint32_t *transp, uint32_t max_depth,
+                           uint32_t *occ, uint32_t *counts, uint2 *hits, uint32_t n, const uint32_t *n_dev,
+                           hipStream_t stream);
 // Queue variants: the ray count is read on the device from *n_dev (<= capacity).
 int ymx_launch_closest_queue(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, const uint32_t *n_dev,
                              uint32_t capacity, hipStream_t stream);

@@ -130,9 +130,14 @@ __device__ __forceinline__ uint32_t wave_fetch(uint32_t *counter)
 	return __shfl(base, 0);
 }
 
-template <bool kShadow, bool kStats>
+// kMode: kClosest (Accelerator::intersect), kShadowAny (intersectShadow) or
+// kTransparent (intersectTransparentShadow).
+enum : int { kClosest = 0, kShadowAny = 1, kTransparent = 2 };
+
+template <int kMode, bool kStats>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(ymx_trace_args args)
 {
+	constexpr bool kShadow = kMode != kClosest;
 	__shared__ int32_t lds_stack[kStack * kTraceBlock];
 	const uint32_t gtid = blockIdx.x * kTraceBlock + threadIdx.x;
 	LaneStack st;
@@ -160,12 +165,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(ymx_trace_args args)
 		bool live = valid && tree_cross(args.bmin, args.bmax, ray.org, ray.dir, t_max, enter, leave);
 		// include/accelerator/accelerator_kdtree_common.h:139
 		const float bias = kBiasScale * fabsf(leave - enter);
-		const float t_min = kShadow ? bias : ((ray.tmin < bias) ? bias : ray.tmin);
+		const float t_min = (kMode == kShadowAny) ? bias : ((ray.tmin < bias) ? bias : ray.tmin);
 		r.tlo = t_min - t_min * kWindow;
 		float best_t = t_max;
 		uint32_t best_prim = YMX_NO_HIT;
 		float best_u = 0.f, best_v = 0.f;
 		bool occluded = false;
+		uint32_t n_transp = 0;  // transparent shadow casters kept (kTransparent)
 
 		st.sp = 0;
 		int32_t ref = live ? 0 : YMX_REF_EMPTY;
@@ -209,13 +215,35 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(ymx_trace_args args)
 				if(t <= 0.f || t < t_min) continue;
 				const uint32_t prim = __float_as_uint(t0.w);
 				const uint32_t vmask = __float_as_uint(t1.w);
-				if(kShadow)
+				if(kMode == kShadowAny)
 				{
 					// include/accelerator/accelerator.h:135-145
 					if(t >= t_max || !(vmask & YMX_VIS_CASTS_SHADOWS)) continue;
 					occluded = true;
 					best_prim = prim;
 					break;
+				}
+				else if(kMode == kTransparent)
+				{
+					// include/accelerator/accelerator.h:147-168: an opaque caster
+					// occludes; transparent ones are kept (each primitive sits in
+					// one leaf, so none is met twice) until more than max_depth
+					// of them occlude.  The kept ones are stored sorted by t.
+					if(t >= t_max || !(vmask & YMX_VIS_CASTS_SHADOWS)) continue;
+					if(!((args.transp[prim >> 5] >> (prim & 31u)) & 1u) || n_transp >= args.max_depth)
+					{
+						occluded = true;
+						best_prim = prim;
+						break;
+					}
+					uint2 *list = args.t_hits + (size_t)rid * args.max_depth;
+					uint32_t k = n_transp++;
+					while(k > 0 && __uint_as_float(list[k - 1].y) > t)
+					{
+						list[k] = list[k - 1];
+						--k;
+					}
+					list[k] = make_uint2(prim, __float_as_uint(t));
 				}
 				else
 				{
@@ -244,6 +272,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(ymx_trace_args args)
 				if(w + 1 < n_words) args.occluded[w + 1] = (uint32_t)(mask >> 32);
 			}
 			if(valid && args.occ_prim) args.occ_prim[rid] = occluded ? best_prim : YMX_NO_HIT;
+			if(kMode == kTransparent && valid) args.t_count[rid] = occluded ? 0u : n_transp;
 		}
 		else if(valid)
 		{
@@ -284,10 +313,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(ymx_trace_args args)
 
 __global__ void k_reset_work(uint32_t *work) { work[0] = 0u; }
 
-template <bool kShadow, bool kStats>
+template <int kMode, bool kStats>
 int launch(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t *occ, uint32_t *prim, uint32_t n,
-           hipStream_t stream, unsigned long long *stats, const uint32_t *n_dev = nullptr)
+           hipStream_t stream, unsigned long long *stats, const uint32_t *n_dev = nullptr,
+           const uint32_t *transp = nullptr, uint32_t max_depth = 0, uint32_t *t_count = nullptr,
+           uint2 *t_hits = nullptr)
 {
+	constexpr bool kShadow = kMode != kClosest;
 	if(n == 0) return YMX_OK;
 	ymx_trace_args a;
 	a.nodes = acc->d_nodes;
@@ -308,11 +340,15 @@ int launch(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t *o
 	a.occluded = occ;
 	a.occ_prim = prim;
 	a.stats = stats;
+	a.transp = transp;
+	a.max_depth = max_depth;
+	a.t_count = t_count;
+	a.t_hits = t_hits;
 	const uint32_t needed = ((n + 63u) / 64u) * 64u;
 	uint32_t threads = a.grid_threads < needed ? a.grid_threads : needed;
 	const uint32_t blocks = (threads + kTraceBlock - 1) / kTraceBlock;
 	k_reset_work<<<1, 1, 0, stream>>>(acc->d_work);
-	k_trace<kShadow, kStats><<<blocks, kTraceBlock, 0, stream>>>(a);
+	k_trace<kMode, kStats><<<blocks, kTraceBlock, 0, stream>>>(a);
 	YMX_HIP_CHECK(hipGetLastError());
 	return YMX_OK;
 }
@@ -326,8 +362,11 @@ int ymx_trace_setup(ymx_accel *acc)
 	hipDeviceProp_t prop;
 	YMX_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
 	int per_cu_c = 0, per_cu_s = 0;
-	YMX_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_c, k_trace<false, false>, kTraceBlock, 0));
-	YMX_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s, k_trace<true, false>, kTraceBlock, 0));
+	YMX_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_c, k_trace<kClosest, false>, kTraceBlock, 0));
+	YMX_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s, k_trace<kShadowAny, false>, kTraceBlock, 0));
+	int per_cu_t = 0;
+	YMX_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_t, k_trace<kTransparent, false>, kTraceBlock, 0));
+	if(per_cu_t < per_cu_s) per_cu_s = per_cu_t;  // the any-hit grid serves both
 	if(per_cu_c < 1) per_cu_c = 1;
 	if(per_cu_s < 1) per_cu_s = 1;
 	acc->grid_threads_closest = (uint32_t)(prop.multiProcessorCount * per_cu_c * kTraceBlock);
@@ -345,25 +384,33 @@ int ymx_trace_setup(ymx_accel *acc)
 int ymx_launch_closest(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t n, hipStream_t stream,
                        unsigned long long *stats)
 {
-	if(stats) return launch<false, true>(acc, rays, hits, nullptr, nullptr, n, stream, stats);
-	return launch<false, false>(acc, rays, hits, nullptr, nullptr, n, stream, nullptr);
+	if(stats) return launch<kClosest, true>(acc, rays, hits, nullptr, nullptr, n, stream, stats);
+	return launch<kClosest, false>(acc, rays, hits, nullptr, nullptr, n, stream, nullptr);
 }
 
 int ymx_launch_closest_queue(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, const uint32_t *n_dev,
                              uint32_t capacity, hipStream_t stream)
 {
-	return launch<false, false>(acc, rays, hits, nullptr, nullptr, capacity, stream, nullptr, n_dev);
+	return launch<kClosest, false>(acc, rays, hits, nullptr, nullptr, capacity, stream, nullptr, n_dev);
 }
 
 int ymx_launch_shadow_queue(const ymx_accel *acc, const ymx_ray *rays, uint32_t *occ, const uint32_t *n_dev,
                             uint32_t capacity, hipStream_t stream)
 {
-	return launch<true, false>(acc, rays, nullptr, occ, nullptr, capacity, stream, nullptr, n_dev);
+	return launch<kShadowAny, false>(acc, rays, nullptr, occ, nullptr, capacity, stream, nullptr, n_dev);
 }
 
 int ymx_launch_shadow(const ymx_accel *acc, const ymx_ray *rays, uint32_t *occ, uint32_t *prim, uint32_t n,
                       hipStream_t stream, unsigned long long *stats)
 {
-	if(stats) return launch<true, true>(acc, rays, nullptr, occ, prim, n, stream, stats);
-	return launch<true, false>(acc, rays, nullptr, occ, prim, n, stream, nullptr);
+	if(stats) return launch<kShadowAny, true>(acc, rays, nullptr, occ, prim, n, stream, stats);
+	return launch<kShadowAny, false>(acc, rays, nullptr, occ, prim, n, stream, nullptr);
+}
+
+int ymx_launch_transparent(const ymx_accel *acc, const ymx_ray *rays, const uint32_t *transp, uint32_t max_depth,
+                           uint32_t *occ, uint32_t *counts, uint2 *hits, uint32_t n, const uint32_t *n_dev,
+                           hipStream_t stream)
+{
+	return launch<kTransparent, false>(acc, rays, nullptr, occ, nullptr, n, stream, nullptr, n_dev, transp, max_depth,
+	                                   counts, hits);
 }

@@ -89,7 +89,7 @@ void free_pt(ymx_pt *pt)
 	                pt->st.cam_hit, pt->st.dxy, pt->st.flags, pt->st.light, pt->st.vert_emit, pt->st.ext_rays[0],
 	                pt->st.ext_rays[1], pt->st.ext_path[0], pt->st.ext_path[1], pt->st.ext_count, pt->st.ext_hits,
 	                pt->st.slot_contrib, pt->st.slot_queue, pt->st.sh_rays, pt->st.sh_occ, pt->st.sh_count,
-	                pt->st.totals, pt->nd.spawn};
+	                pt->st.sh_tcount, pt->st.sh_thits, pt->d_transp, pt->st.totals, pt->nd.spawn};
 	for(void *p : ptrs)
 		if(p) (void)hipFree(p);
 	if(pt->ev0) (void)hipEventDestroy(pt->ev0);
@@ -265,7 +265,8 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 	}
 	if(st->resx <= 0 || st->resy <= 0 || (int64_t)st->resx * st->resy > (1 << 26) || st->bounces < 1 ||
 	   st->bounces > 12 || st->aa_samples < 1 || st->path_samples < 0 || n_lights > (uint32_t)kMaxLights ||
-	   st->raydepth < 0 || st->raydepth > kMaxRayDepth)
+	   st->raydepth < 0 || st->raydepth > kMaxRayDepth ||
+	   (st->transparent_shadows && (st->shadow_depth < 0 || st->shadow_depth > 64)))
 	{
 		ymx_set_error("ymx_pt_create: unsupported settings (res %dx%d, bounces %d, AA %d, lights %u, raydepth %d)",
 		              st->resx, st->resy, st->bounces, st->aa_samples, n_lights, st->raydepth);
@@ -485,6 +486,8 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 	c.rr_min_bounces = st->rr_min_bounces;
 	c.aa_samples = st->aa_samples;
 	c.raydepth = st->raydepth;
+	c.transparent_shadows = st->transparent_shadows ? 1u : 0u;
+	c.shadow_depth = st->transparent_shadows ? (uint32_t)st->shadow_depth : 0u;
 	c.n_lights = n_lights;
 	c.slots_camera = slots_camera;
 	c.slots_path = slots_path;
@@ -568,6 +571,20 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 	PT_TRY(dalloc(s.sh_occ, (slot_cap + 31) / 32 + 2));
 	PT_TRY(dalloc(s.sh_count, 1));
 	PT_TRY(hipMemset(s.sh_count, 0, sizeof(uint32_t)));
+	if(c.transparent_shadows)
+	{
+		// Material::isTransparent: glass with fake shadows (include/material/material_glass.h:82)
+		std::vector<uint32_t> bits(((size_t)n_tris + 31) / 32 + 1, 0u);
+		for(uint32_t i = 0; i < n_tris; ++i)
+		{
+			const ymx_material &m = materials[mat_id[i]];
+			if(m.type == YMX_MAT_GLASS && m.fake_shadows) bits[i >> 5] |= 1u << (i & 31u);
+		}
+		PT_TRY(dalloc(pt->d_transp, bits.size()));
+		PT_TRY(hipMemcpy(pt->d_transp, bits.data(), sizeof(uint32_t) * bits.size(), hipMemcpyHostToDevice));
+		PT_TRY(dalloc(s.sh_tcount, slot_cap));
+		PT_TRY(dalloc(s.sh_thits, slot_cap * (size_t)(c.shadow_depth ? c.shadow_depth : 1)));
+	}
 	PT_TRY(dalloc(s.totals, 3));
 	PT_TRY(hipMemset(s.totals, 0, 3 * sizeof(unsigned long long)));
 	PT_TRY(hipEventCreate(&pt->ev0));

@@ -81,6 +81,8 @@ struct PtConst
 	int32_t resx, resy;
 	int32_t path_samples, bounces, rr_min_bounces, aa_samples;
 	int32_t raydepth;
+	uint32_t transparent_shadows;  // "transpShad"
+	uint32_t shadow_depth;         // "shadowDepth"
 	uint32_t n_lights;
 	uint32_t slots_camera;   // shadow slots of a camera vertex (all lights)
 	uint32_t slots_path;     // shadow slots of a path vertex (one light)
@@ -132,6 +134,8 @@ struct PtState
 	ymx_ray *sh_rays;
 	uint32_t *sh_occ;      // bit per queued shadow ray
 	uint32_t *sh_count;
+	uint32_t *sh_tcount;   // transparent shadows: casters kept per queued shadow ray
+	uint2 *sh_thits;       // ... and their (primitive, t bits), shadow_depth per ray
 	uint32_t capacity;     // slots (nodes per chunk); also the camera paths per wavefront
 	uint32_t slots;        // shadow slots per path
 	unsigned long long *totals;  // rays cast: [0] camera + recursive, [1] extension, [2] shadow
@@ -170,6 +174,7 @@ struct ymx_pt
 	ymx::PtMaterial *d_materials;
 	ymx::PtLight *d_lights;
 	ymx::PtLightTri *d_ltris;        // triangles of the object lights
+	uint32_t *d_transp;              // bit per primitive: its material is transparent (transparent shadows)
 	ymx::PtConst *d_const;
 	int32_t *d_faure;                // [kLdsDims][kLdsMaxBase] scrambling permutations
 	double *d_inv_prims;             // [kLdsDims]

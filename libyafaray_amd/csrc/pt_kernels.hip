@@ -981,6 +981,24 @@ __device__ void queue_light(const PtArgs &a, const Surf &sp, V3 wo, const PtLigh
 	}
 }
 
+// GlassMaterial::getTransparency (src/material/material_glass.cc:297-304) at a
+// triangle a shadow ray along dir passes: kt of the Fresnel terms about the
+// face normal turned towards dir, times the filter colour.
+__device__ Rgb transparency(const PtArgs &a, uint32_t prim, V3 dir)
+{
+	const float4 pr = a.prim[prim];
+	const V3 ng = v3(pr.x, pr.y, pr.z);
+	const PtMaterial &m = a.mats[__float_as_uint(pr.w)];
+	float kr, kt;
+	fresnel(dir, face_forward(ng, ng, dir), m.ior, kr, kt);
+	return cscale(m.filter_color, kt);
+}
+
+// A slot's contribution when its shadow ray is unoccluded.  With transparent
+// shadows it is scaled by the product of the transparencies passed, front to
+// back (the reference scales the light colour by that product before the
+// other factors, and multiplies in its kd-tree's order: the result agrees to
+// float rounding).
 __device__ __forceinline__ bool slot_lit(const PtArgs &a, uint32_t slot, Rgb &c)
 {
 	const float4 v = a.st.slot_contrib[slot];
@@ -988,6 +1006,19 @@ __device__ __forceinline__ bool slot_lit(const PtArgs &a, uint32_t slot, Rgb &c)
 	const int32_t q = a.st.slot_queue[slot];
 	if(q >= 0 && ((a.st.sh_occ[(uint32_t)q >> 5] >> ((uint32_t)q & 31u)) & 1u)) return false;
 	c = rgb(v.x, v.y, v.z);
+	if(q >= 0 && a.c->transparent_shadows)
+	{
+		const uint32_t n = a.st.sh_tcount[q];
+		if(n > 0)
+		{
+			const float4 d = reinterpret_cast<const float4 *>(a.st.sh_rays + q)[1];
+			const V3 dir = v3(d.x, d.y, d.z);
+			Rgb s = rgb(1.f, 1.f, 1.f);
+			for(uint32_t k = 0; k < n; ++k)
+				s = cmul(s, transparency(a, a.st.sh_thits[(size_t)q * a.c->shadow_depth + k].x, dir));
+			c = cmul(c, s);
+		}
+	}
 	return true;
 }
 
@@ -1735,6 +1766,17 @@ __global__ void k_count_rays(unsigned long long *total, uint32_t n)
 
 inline unsigned blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
+// The queued shadow rays: any-hit, or with transparent shadows the
+// transparent-shadow query (Accelerator::isShadowed / isShadowedTransparentShadow).
+int shadow_pass(ymx_pt *pt, uint32_t slot_cap, hipStream_t stream)
+{
+	const PtConst &hc = pt->host_const;
+	if(hc.transparent_shadows)
+		return ymx_launch_transparent(pt->accel, pt->st.sh_rays, pt->d_transp, hc.shadow_depth, pt->st.sh_occ,
+		                              pt->st.sh_tcount, pt->st.sh_thits, slot_cap, pt->st.sh_count, stream);
+	return ymx_launch_shadow_queue(pt->accel, pt->st.sh_rays, pt->st.sh_occ, pt->st.sh_count, slot_cap, stream);
+}
+
 // The stages of one chunk of nodes [a.base, a.base + a.cnt).
 int trace_chunk(ymx_pt *pt, PtArgs &a, hipStream_t stream)
 {
@@ -1750,7 +1792,7 @@ int trace_chunk(ymx_pt *pt, PtArgs &a, hipStream_t stream)
 	YMX_HIP_CHECK(hipGetLastError());
 	if(hc.n_lights > 0)
 	{
-		rc = ymx_launch_shadow_queue(pt->accel, pt->st.sh_rays, pt->st.sh_occ, pt->st.sh_count, slot_cap, stream);
+		rc = shadow_pass(pt, slot_cap, stream);
 		if(rc != YMX_OK) return rc;
 	}
 	k_resolve_node<<<g, kBlock, 0, stream>>>(a);
@@ -1786,7 +1828,7 @@ int trace_chunk(ymx_pt *pt, PtArgs &a, hipStream_t stream)
 			YMX_HIP_CHECK(hipGetLastError());
 			if(hc.n_lights > 0)
 			{
-				rc = ymx_launch_shadow_queue(pt->accel, pt->st.sh_rays, pt->st.sh_occ, pt->st.sh_count, slot_cap, stream);
+				rc = shadow_pass(pt, slot_cap, stream);
 				if(rc != YMX_OK) return rc;
 			}
 			k_resolve_path<<<g, kBlock, 0, stream>>>(a);

@@ -50,7 +50,8 @@ class YmxPtSettings(C.Structure):
                 ("aspect_factor", C.c_float), ("resx", C.c_int32), ("resy", C.c_int32),
                 ("path_samples", C.c_int32), ("bounces", C.c_int32), ("rr_min_bounces", C.c_int32),
                 ("aa_samples", C.c_int32), ("aa_pixelwidth", C.c_float), ("has_background", C.c_int32),
-                ("background", C.c_float * 3), ("seed", C.c_uint32), ("raydepth", C.c_int32)]
+                ("background", C.c_float * 3), ("seed", C.c_uint32), ("raydepth", C.c_int32),
+                ("transparent_shadows", C.c_int32), ("shadow_depth", C.c_int32)]
 
 
 YmxPtStats = native.YmxPtStats
@@ -122,6 +123,8 @@ def build_structs(scene):
     st.background = _f3(bg if bg is not None else (0, 0, 0))
     st.seed = int(s.get("seed", 0))
     st.raydepth = int(s.get("raydepth", 5))
+    st.transparent_shadows = 1 if s.get("transpShad", False) else 0
+    st.shadow_depth = int(s.get("shadowDepth", 4))
     return mats, lights, st
 
 

@@ -160,6 +160,31 @@ class KdTree:
                              threads or default_threads())
         return _hits(out)
 
+    def transparent(self, rays, transparent, max_depth):
+        """TransparentShadow query per ray (accelerator.h:147-168): occluded flags,
+        counts and the transparent casters met, in the kd-tree's order."""
+        L = lib()
+        if not getattr(L, "_tr_ready", False):
+            L.ora_kd_transparent1.argtypes = [C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_int, C.c_void_p,
+                                              C.POINTER(C.c_int)]
+            L.ora_kd_transparent1.restype = C.c_int
+            L._tr_ready = True
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        flags = np.ascontiguousarray(transparent, dtype=np.uint8)
+        n = rays.shape[0]
+        occ = np.zeros(n, bool)
+        counts = np.zeros(n, np.uint32)
+        prims = np.zeros((n, max(1, max_depth)), np.uint32)
+        buf = np.zeros(max(1, max_depth), np.uint32)
+        cnt = C.c_int(0)
+        for i in range(n):
+            t_max = float(rays[i, 7]) if rays[i, 7] >= 0 else float(np.finfo(np.float32).max)
+            occ[i] = bool(L.ora_kd_transparent1(self._h, rays[i].ctypes.data, t_max, flags.ctypes.data, int(max_depth),
+                                                buf.ctypes.data, C.byref(cnt)))
+            counts[i] = 0 if occ[i] else cnt.value
+            prims[i, :cnt.value] = buf[:cnt.value]
+        return occ, counts, prims
+
     def shadow(self, rays, threads=None):
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
         occ = np.zeros(rays.shape[0], dtype=np.uint8)

@@ -261,3 +261,33 @@ def test_full_raycast_workload_properties(oracle):
     occ_g, _ = acc.intersect_shadow(shadow[sub])
     occ_k, _ = kd.shadow(shadow[sub])
     assert np.array_equal(occ_g, occ_k)
+
+
+@pytest.mark.parametrize("max_depth", [0, 1, 4])
+def test_transparent_shadow_query(oracle, max_depth):
+    """intersectTransparentShadow against the kd-tree port's restatement: the
+    same occlusion decision (an opaque caster, or more than max_depth
+    transparent ones), and for unoccluded rays the same set of transparent
+    casters (the GPU lists them by t, the kd-tree in its own order)."""
+    verts, tris, rays, rng = small_scene(3, res=24, n_random=3000)
+    rays = rays[:6000].copy()
+    rays[:, 3] = rng.uniform(0.0, 0.01, len(rays)).astype(np.float32)   # tmin: max(tmin, bias) rule
+    rays[:, 7] = scenes.shadow_tmax(rng, len(rays))
+    transparent = rng.uniform(size=len(tris)) < 0.7
+    say(f"transparent shadows depth={max_depth} rays={len(rays)} tris={len(tris)}")
+    acc = GpuAccelerator(verts, tris)
+    occ, counts, hits = acc.intersect_transparent(rays, transparent, max_depth)
+    kd = oracle.KdTree(verts, tris)
+    eocc, ecounts, eprims = kd.transparent(rays, transparent, max_depth)
+    say(f"  occluded {eocc.mean():.3f}, unoccluded with casters {((~eocc) & (ecounts > 0)).mean():.3f}")
+    assert 0.05 < eocc.mean() < 0.95
+    assert np.array_equal(occ.astype(bool), eocc)
+    assert np.array_equal(counts, ecounts)
+    with_casters = np.nonzero((~eocc) & (ecounts > 0))[0]
+    if max_depth > 0:
+        assert len(with_casters) > 100
+    for i in with_casters:
+        k = counts[i]
+        assert set(hits[i, :k, 0].tolist()) == set(eprims[i, :k].tolist())
+        t = hits[i, :k, 1].view(np.float32)
+        assert np.all(np.diff(t) >= 0) and np.all(t > 0) and np.all(t < rays[i, 7])

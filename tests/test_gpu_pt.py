@@ -195,3 +195,25 @@ def test_object_lights_match_oracle(oracle, visibility, double_sided):
     g = render_gpu(sc)
     say(f"  stats {g[2]}")
     compare_films(g[:2], render_cpu(oracle, sc), f"object lights {visibility}")
+
+
+def test_transparent_shadows_match_oracle(oracle):
+    """"transpShad" with fake-shadow glass: shadow rays pass up to "shadowDepth"
+    glass surfaces, scaled by their kt * filter colour.  The product is taken
+    front to back on the GPU and in kd-tree order by the oracle, and applied to
+    the finished contribution rather than to the light colour first, so films
+    agree to float rounding (the usual tolerance), not bit for bit."""
+    sc = scenes.cornell_materials_scene(n_random=1500, res=48, spp=4, seed=5)
+    for m in sc["materials"]:
+        if m.get("type") == "glass":
+            m["fake_shadows"] = True
+    sc["settings"] = dict(sc["settings"], transpShad=True, shadowDepth=3)
+    say("pt transparent shadows 48x48x4 (fake-shadow glass, depth 3)")
+    g = render_gpu(sc)
+    o = render_cpu(oracle, sc)
+    gimg, oimg = compare_films(g[:2], o, "transparent shadows")
+    # the glass now lets light through: brighter than with opaque shadows
+    sc2 = dict(sc)
+    sc2["settings"] = dict(sc["settings"], transpShad=False)
+    g2 = render_gpu(sc2)
+    assert (g[0][..., :3] / g[1][..., None]).mean() > (g2[0][..., :3] / g2[1][..., None]).mean()
