@@ -185,6 +185,48 @@ def measure_guiding(args, torch):
     return out
 
 
+BF16_PEAK_TFLOPS = 2500.0
+MLP_FLOPS = 2 * (3 * 64 * 64 + 48 * 64)   # one query through the 3 x 64 hidden layers and the 48-wide output
+
+
+def measure_guide_kernels(args, torch):
+    """The guiding network's kernels on their own: inference over 2^21 queries
+    and one training step (forward, backward, weight gradients, Adam) over a
+    65,536-sample batch, timed with HIP events on the stream they run on.
+    MFMA utilisation = MLP FLOPs (inference: 30,720 per query; training: three
+    times that, forward plus the input- and weight-gradient products) over
+    kernel time, against the dense BF16 peak.  The hash-grid gathers and
+    scatters are not counted as FLOPs."""
+    from libyafaray_amd.guide import GpuGuide, pack_samples
+    rng = np.random.default_rng(11)
+
+    def unit(n):
+        v = rng.normal(size=(n, 3))
+        return (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+
+    n_inf, n_train = 1 << 21, 1 << 16
+    g = GpuGuide((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), log2_table_size=19, max_batch=n_train, seed=2)
+    q = torch.from_numpy(pack_samples(rng.uniform(0, 1, (n_inf, 3)).astype(np.float32), unit(n_inf),
+                                      unit(n_inf))).cuda()
+    smp = torch.from_numpy(pack_samples(rng.uniform(0, 1, (n_train, 3)).astype(np.float32), unit(n_train),
+                                        unit(n_train), unit(n_train),
+                                        rng.uniform(0.1, 2.0, n_train).astype(np.float32))).cuda()
+    mix = torch.empty((n_inf, 48), dtype=torch.float32, device="cuda")
+    _, inf_ms = timed_passes(lambda: g.infer(q, mix), 10, 2, torch)
+    _, train_ms = timed_passes(lambda: g.train_step(smp), 10, 2, torch)
+    g.close()
+    inf_tflops = MLP_FLOPS * n_inf / (inf_ms * 1e-3) / 1e12
+    train_tflops = 3 * MLP_FLOPS * n_train / (train_ms * 1e-3) / 1e12
+    out = {"inference_queries": n_inf, "inference_ms": round(inf_ms, 3),
+           "inference_mqueries_per_s": round(n_inf / (inf_ms * 1e-3) / 1e6, 1),
+           "inference_mlp_tflops": round(inf_tflops, 2), "inference_mfma_frac": round(inf_tflops / BF16_PEAK_TFLOPS, 5),
+           "train_batch": n_train, "train_step_ms": round(train_ms, 3),
+           "train_mlp_tflops": round(train_tflops, 2), "train_mfma_frac": round(train_tflops / BF16_PEAK_TFLOPS, 5),
+           "peak_bf16_tflops": BF16_PEAK_TFLOPS}
+    log(args, f"guide kernels: {out}")
+    return out
+
+
 def calibrate_copy():
     """A plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters give the
     scale between the counters and bytes moved."""
@@ -241,7 +283,8 @@ def main():
         print(json.dumps({"path_tracing": measure_path_tracer(args, torch)}), flush=True)
         return
     if args.guided_only:
-        print(json.dumps({"guided_path_tracing": measure_guiding(args, torch)}), flush=True)
+        print(json.dumps({"guide_kernels": measure_guide_kernels(args, torch),
+                          "guided_path_tracing": measure_guiding(args, torch)}), flush=True)
         return
 
     t0 = time.perf_counter()
@@ -347,6 +390,7 @@ def main():
         out["extra"]["path_tracing"] = measure_path_tracer(args, torch)
     if not args.no_guided:
         out["extra"]["guided_path_tracing"] = measure_guiding(args, torch)
+        out["extra"]["guide_kernels"] = measure_guide_kernels(args, torch)
     print(json.dumps(out), flush=True)
 
 

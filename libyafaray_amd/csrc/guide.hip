@@ -558,6 +558,15 @@ int refresh_bf16(ymx_guide *g, hipStream_t s)
 
 uint32_t ymx_guide_batch_limit(const ymx_guide *g) { return g ? g->p.max_batch : 0u; }
 
+void ymx_guide_bounds(const ymx_guide *g, float bmin[3], float bmax[3])
+{
+	for(int k = 0; k < 3; ++k)
+	{
+		bmin[k] = g->p.bmin[k];
+		bmax[k] = g->p.bmax[k];
+	}
+}
+
 extern "C" {
 
 int ymx_guide_create(const ymx_guide_params *p, ymx_guide **out)

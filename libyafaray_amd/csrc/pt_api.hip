@@ -72,7 +72,7 @@ void free_nodes(PtNodes &nd)
 
 void free_guide_buffers(PtGuide &g)
 {
-	void *ptrs[] = {g.query, g.mix_node, g.mix_path, g.rec, g.base, g.mask, g.train_buf, g.batch_buf, g.train_count,
+	void *ptrs[] = {g.query, g.mix_node, g.mix_path, g.rec, g.base, g.mask, g.train_buf, g.batch_buf, g.cell_sum, g.train_count,
 	                g.query_count};
 	for(void *p : ptrs)
 		if(p) (void)hipFree(p);
@@ -639,6 +639,12 @@ int ymx_pt_set_guide(ymx_pt *pt, ymx_guide *guide, float bsdf_prob, int32_t trai
 		GB_TRY(hipMemset(g.mask, 0, sizeof(uint32_t) * cap));
 		GB_TRY(dalloc(g.train_buf, (size_t)tcap));
 		GB_TRY(dalloc(g.batch_buf, (size_t)ymx_guide_batch_limit(guide)));
+		const size_t cells = (size_t)kGuideCells * kGuideCells * kGuideCells;
+		GB_TRY(dalloc(g.cell_sum, cells));
+		GB_TRY(hipMemset(g.cell_sum, 0, sizeof(float2) * cells));
+		float bmax[3];
+		ymx_guide_bounds(guide, g.bmin, bmax);
+		for(int k = 0; k < 3; ++k) g.scale[k] = (float)kGuideCells / (bmax[k] - g.bmin[k]);
 		GB_TRY(dalloc(g.train_count, 1));
 		GB_TRY(hipMemset(g.train_count, 0, sizeof(uint32_t)));
 		g.train_cap = (uint32_t)tcap;

@@ -43,6 +43,7 @@ struct PtMaterial
 YMX_HD bool spawns_children(uint32_t flags) { return (flags & (kGlossy | kSpecular | kFilter)) != 0; }
 
 constexpr int kMaxRayDepth = 10;
+constexpr int kGuideCells = 16;  // regions per axis over which training weights are normalised
 constexpr uint32_t kGlossyBranches = 8;  // MonteCarloIntegrator::initial_ray_samples_glossy_
 
 struct PtLight
@@ -158,6 +159,8 @@ struct PtGuide
 	uint32_t *mask;            // [capacity] bit r: record r is pending
 	ymx_guide_sample *train_buf;  // collected training samples
 	ymx_guide_sample *batch_buf;  // one mini-batch gathered from the ring
+	float2 *cell_sum;             // [kGuideCells^3] running (sum of weights, count) of kept samples per region
+	float bmin[3], scale[3];      // region of a position: floor((p - bmin) * scale) per axis
 	uint32_t *train_count;
 	uint32_t train_cap;
 	uint32_t *query_count;     // queries of the current path-vertex batch
@@ -208,5 +211,6 @@ int ymx_pt_grow_nodes(ymx_pt *pt, uint64_t need, uint32_t keep);
 // Guide training on the samples of the last wavefront (pt_kernels.hip).
 int ymx_pt_train_guide(ymx_pt *pt, hipStream_t stream);
 
-// Most samples one training step of the guide takes (guide.hip).
+// Most samples one training step of the guide takes, and its bounding box (guide.hip).
 uint32_t ymx_guide_batch_limit(const ymx_guide *g);
+void ymx_guide_bounds(const ymx_guide *g, float bmin[3], float bmax[3]);

@@ -617,6 +617,18 @@ __device__ bool guided_sample(const PtArgs &a, const Surf &sp, V3 wo, const floa
 	return true;
 }
 
+// Region of a guide sample's position for the training-weight normalisation.
+__device__ __forceinline__ uint32_t guide_cell(const PtGuide &g, const float p[3])
+{
+	uint32_t c[3];
+	for(int k = 0; k < 3; ++k)
+	{
+		const float f = (p[k] - g.bmin[k]) * g.scale[k];
+		c[k] = !(f > 0.f) ? 0u : (f >= (float)(kGuideCells - 1) ? (uint32_t)(kGuideCells - 1) : (uint32_t)f);
+	}
+	return (c[2] * kGuideCells + c[1]) * kGuideCells + c[0];
+}
+
 __device__ __forceinline__ float luminance(Rgb c) { return 0.2126f * c.r + 0.7152f * c.g + 0.0722f * c.b; }
 
 // A pending training sample of segment r (position, normal, wo, direction) and
@@ -1579,12 +1591,18 @@ __global__ __launch_bounds__(kBlock) void k_guide_collect(PtArgs a)
 		ymx_guide_sample rec = a.g.rec[k];
 		rec.c = cw;
 		a.g.train_buf[slot] = rec;
+		float2 *cs = a.g.cell_sum + guide_cell(a.g, rec.pos);
+		atomicAdd(&cs->x, cw);
+		atomicAdd(&cs->y, 1.f);
 	}
 }
 
-// One training mini-batch: dst[k] = ring[h(k) mod n] for a hashed index h.
+// One training mini-batch: dst[k] = ring[h(k) mod n] for a hashed index h,
+// its weight divided by the mean weight of its region: the loss -c log q then
+// fits the shape of every region's directional distribution with the same
+// emphasis, instead of spending the network on the brightest regions.
 __global__ __launch_bounds__(kBlock) void k_guide_gather(ymx_guide_sample *dst, const ymx_guide_sample *ring, uint32_t n,
-                                                         uint32_t batch, uint32_t seed)
+                                                         uint32_t batch, uint32_t seed, PtGuide g)
 {
 	const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
 	if(k >= batch) return;
@@ -1595,7 +1613,10 @@ __global__ __launch_bounds__(kBlock) void k_guide_gather(ymx_guide_sample *dst, 
 	h ^= seed;
 	h *= 0x27D4EB2Fu;
 	h ^= h >> 15;
-	dst[k] = ring[(uint32_t)(((uint64_t)h * n) >> 32)];
+	ymx_guide_sample s = ring[(uint32_t)(((uint64_t)h * n) >> 32)];
+	const float2 cs = g.cell_sum[guide_cell(g, s.pos)];
+	s.c = cs.x > 0.f ? s.c * (cs.y / cs.x) : s.c;
+	dst[k] = s;
 }
 
 // Queries of the guide at the node vertices that start sub-paths.
@@ -1964,7 +1985,7 @@ int ymx_pt_train_guide(ymx_pt *pt, hipStream_t stream)
 			// one band of the image
 			pt->g_rng = pt->g_rng * 6364136223846793005ull + 1442695040888963407ull;
 			k_guide_gather<<<blocks_for(batch), kBlock, 0, stream>>>(pt->g.batch_buf, pt->g.train_buf, n, batch,
-			                                                          (uint32_t)(pt->g_rng >> 32));
+			                                                          (uint32_t)(pt->g_rng >> 32), pt->g);
 			YMX_HIP_CHECK(hipGetLastError());
 			const int rc = ymx_guide_train_step(pt->guide, pt->g.batch_buf, batch, nullptr, stream);
 			if(rc != YMX_OK) return rc;
