@@ -146,6 +146,19 @@ int ymx_accel_intersect_transparent(const ymx_accel *accel, const ymx_ray *d_ray
                                     uint32_t max_depth, uint32_t *d_occluded, uint32_t *d_counts, uint32_t *d_hits,
                                     uint32_t n_rays, void *stream);
 
+/* Copy the built tree to host memory (no counterpart in the reference: it
+ * serves the per-ray queries of the C++ host layer, which traverse the same
+ * tree on the CPU; libyafaray_amd/host).  nodes: n_nodes x 64 bytes, node i =
+ * 4 x float4: (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y), (c1.lo.x, c1.hi.x, c1.lo.y,
+ * c1.hi.y), (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z), (ref0, ref1, -, -) as int32
+ * bits; the root is node 0; a ref >= 0 is a node, 0xFFFFFFFF no child, and
+ * any other negative ref a leaf 0x80000000 | first << 4 | (count - 1) over
+ * triangle records [first, first + count).  tris: n_tris x 48 bytes (one
+ * record when n_tris is 0), 3 x float4: (v0, primitive id bits), (v1 - v0,
+ * visibility bits), (v2 - v0, 0).  The byte counts must equal those of
+ * ymx_accel_get_info. */
+int ymx_accel_export(const ymx_accel *accel, void *nodes, uint64_t node_bytes, void *tris, uint64_t tri_bytes);
+
 /* Counting replay of the query kernels: total 64-byte nodes fetched and
  * triangles tested over the batch.  query: 0 closest, 1 shadow.  Synchronous;
  * the counts are written to host memory. */

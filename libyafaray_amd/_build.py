@@ -75,6 +75,49 @@ def build_library(verbose: bool = False) -> Path:
     return LIB
 
 
+HOST_SRC = PKG / "host" / "ymx_host.cpp"
+HOST_LIB = PKG / "libymx_host.so"
+CPP_TEST_SRC = ROOT / "tests" / "cpp" / "test_host.cpp"
+CPP_TEST_BIN = ROOT / "tests" / "cpp" / "bin" / "test_host"
+
+# The host layer keeps the kernels' float arithmetic (single-ray traversal of
+# the exported tree), so it is compiled without contraction or fast math too.
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+              "-Wall", "-Wno-unknown-pragmas"]
+
+
+def build_host(verbose: bool = False) -> Path:
+    """The C++ host layer (libymx_host.so) over libymx.so's C ABI."""
+    tmp = HOST_LIB.with_suffix(".so.tmp")
+    cmd = [shutil.which("g++") or "g++", *HOST_FLAGS, "-shared", "-I", str(ROOT / "include"),
+           "-o", str(tmp), str(HOST_SRC), "-L", str(PKG), "-lymx", "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"host layer build failed:\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, HOST_LIB)
+    if verbose:
+        print(f"built {HOST_LIB}")
+    return HOST_LIB
+
+
+def build_cpp_tests(verbose: bool = False) -> Path:
+    """The C++ test driver of the host layer (links the oracle as its checker)."""
+    CPP_TEST_BIN.parent.mkdir(exist_ok=True)
+    tmp = CPP_TEST_BIN.with_suffix(".tmp")
+    cmd = [hipcc(), "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
+           "-Wno-unused-command-line-argument", "-Wno-unknown-pragmas",
+           "-I", str(ROOT / "include"), "-o", str(tmp), str(CPP_TEST_SRC),
+           "-L", str(PKG), "-lymx_host", "-lymx", "-L", str(ROOT / "oracle"), "-loracle",
+           "-Wl,-rpath,$ORIGIN/../../../libyafaray_amd", "-Wl,-rpath,$ORIGIN/../../../oracle"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"C++ test build failed:\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, CPP_TEST_BIN)
+    if verbose:
+        print(f"built {CPP_TEST_BIN}")
+    return CPP_TEST_BIN
+
+
 def build_oracle(verbose: bool = False) -> Path:
     """Compile the CPU oracle (test infrastructure) with its own Makefile."""
     res = subprocess.run(["make", "-C", str(ROOT / "oracle"), "-s"], capture_output=True, text=True)
@@ -87,4 +130,6 @@ def build_oracle(verbose: bool = False) -> Path:
 
 if __name__ == "__main__":
     build_library(verbose=True)
+    build_host(verbose=True)
     build_oracle(verbose=True)
+    build_cpp_tests(verbose=True)

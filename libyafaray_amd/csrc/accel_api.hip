@@ -123,6 +123,19 @@ int ymx_accel_bound(const ymx_accel *accel, float bound_min[3], float bound_max[
 	return YMX_OK;
 }
 
+int ymx_accel_export(const ymx_accel *accel, void *nodes, uint64_t node_bytes, void *tris, uint64_t tri_bytes)
+{
+	if(!accel || !nodes || !tris || node_bytes != 64ull * accel->n_nodes ||
+	   tri_bytes != 48ull * (accel->n_tris ? accel->n_tris : 1))
+	{
+		ymx_set_error("ymx_accel_export: NULL buffer or byte counts other than ymx_accel_get_info's");
+		return YMX_ERR_INVALID_ARG;
+	}
+	YMX_HIP_CHECK(hipMemcpy(nodes, accel->d_nodes, node_bytes, hipMemcpyDeviceToHost));
+	YMX_HIP_CHECK(hipMemcpy(tris, accel->d_tris, tri_bytes, hipMemcpyDeviceToHost));
+	return YMX_OK;
+}
+
 int ymx_accel_get_info(const ymx_accel *accel, ymx_accel_info *info)
 {
 	if(!accel || !info || info->struct_size != sizeof(ymx_accel_info))

@@ -226,17 +226,22 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(ymx_trace_args args)
 				else if(kMode == kTransparent)
 				{
 					// include/accelerator/accelerator.h:147-168: an opaque caster
-					// occludes; transparent ones are kept (each primitive sits in
-					// one leaf, so none is met twice) until more than max_depth
-					// of them occlude.  The kept ones are stored sorted by t.
+					// occludes; transparent ones are kept, once each (the
+					// reference's `filtered` set: a one-triangle tree reaches its
+					// leaf through both children), until more than max_depth of
+					// them occlude.  The kept ones are stored sorted by t.
 					if(t >= t_max || !(vmask & YMX_VIS_CASTS_SHADOWS)) continue;
-					if(!((args.transp[prim >> 5] >> (prim & 31u)) & 1u) || n_transp >= args.max_depth)
+					const bool transparent = (args.transp[prim >> 5] >> (prim & 31u)) & 1u;
+					uint2 *list = args.t_hits + (size_t)rid * args.max_depth;
+					bool seen = false;
+					for(uint32_t j = 0; transparent && j < n_transp; ++j) seen |= list[j].x == prim;
+					if(seen) continue;
+					if(!transparent || n_transp >= args.max_depth)
 					{
 						occluded = true;
 						best_prim = prim;
 						break;
 					}
-					uint2 *list = args.t_hits + (size_t)rid * args.max_depth;
 					uint32_t k = n_transp++;
 					while(k > 0 && __uint_as_float(list[k - 1].y) > t)
 					{

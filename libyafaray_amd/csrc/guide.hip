@@ -43,6 +43,8 @@ struct ymx_guide
 	uint16_t *w_bf;                              // 4 x [64 out][64 in] (W4 rows 48..63 zero)
 	uint16_t *wt_bf;                             // 4 x [64 in][64 out] (W4^T columns 48..63 zero)
 	uint16_t *X, *H1, *H2, *H3, *dO, *dH3, *dH2, *dH1;  // [max_batch][64]
+	float *dXg;                                  // [max_batch][32] grid-feature gradients of the small levels
+	uint32_t small_mask;                         // levels whose gradient is summed in LDS (k_grid_small)
 	double *loss;
 	int step;
 };
@@ -53,6 +55,8 @@ constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 constexpr int kMlp = YMX_GUIDE_MLP_WEIGHTS;
 constexpr int kWgradSamples = 64;  // samples per LDS stage of the weight-gradient GEMM
+constexpr uint32_t kSmallEntries = 6144;  // dense levels up to this many entries sum their gradient in LDS
+constexpr int kSmallChunks = 32;          // blocks per small level, each over a slice of the batch
 
 struct GArgs
 {
@@ -60,6 +64,8 @@ struct GArgs
 	const uint16_t *w_bf, *wt_bf;
 	float *grid_g, *mlp_g;
 	uint16_t *X, *H1, *H2, *H3, *dO, *dH3, *dH2, *dH1;
+	float *dXg;
+	uint32_t small_mask;
 	double *loss;
 	uint32_t table_mask;
 	int32_t res[kLevels];
@@ -406,6 +412,12 @@ __global__ __launch_bounds__(kBlock) void k_backward(GArgs g, const ymx_guide_sa
 		{
 			const int lev = 8 * t + 2 * grp + q;
 			const float d0 = acc[t][2 * q], d1 = acc[t][2 * q + 1];
+			if((g.small_mask >> lev) & 1u)
+			{
+				// coarse dense level: summed per block in LDS by k_grid_small
+				*reinterpret_cast<float2 *>(g.dXg + (size_t)s * 32 + 2 * lev) = make_float2(d0, d1);
+				continue;
+			}
 			Corners c;
 			level_corners(g, lev, p, c);
 			float *lv = g.grid_g + (size_t)lev * table * kFeat;
@@ -415,6 +427,51 @@ __global__ __launch_bounds__(kBlock) void k_backward(GArgs g, const ymx_guide_sa
 				atomicAdd(lv + 2 * (size_t)c.idx[k] + 1, c.w[k] * d1);
 			}
 		}
+}
+
+// Grid gradient of the small dense levels: block (level k of the list,
+// chunk) adds the corners of its slice of the batch into an LDS copy of the
+// level, then adds the non-zero entries to the gradient with one global atomic
+// each (on the coarse levels every sample hits the same few entries, and
+// per-sample global atomics serialise).
+struct SmallLevels
+{
+	int32_t lev[kLevels];
+	int32_t count;
+};
+__global__ __launch_bounds__(kBlock) void k_grid_small(GArgs g, SmallLevels sl, const ymx_guide_sample *smp,
+                                                        uint32_t cap, const uint32_t *n_dev)
+{
+	extern __shared__ float lds_grid[];
+	const int lev = sl.lev[blockIdx.x / kSmallChunks];
+	const uint32_t chunk = blockIdx.x % kSmallChunks;
+	const uint32_t r1 = (uint32_t)g.res[lev] + 1u;
+	const uint32_t entries = r1 * r1 * r1;  // <= kSmallEntries, checked on the host
+	for(uint32_t e = threadIdx.x; e < 2 * entries; e += kBlock) lds_grid[e] = 0.f;
+	__syncthreads();
+	const uint32_t n = count_of(cap, n_dev);
+	const uint32_t per = (n + kSmallChunks - 1) / kSmallChunks;
+	const uint32_t s0 = chunk * per, s1 = s0 + per < n ? s0 + per : n;
+	for(uint32_t s = s0 + threadIdx.x; s < s1; s += kBlock)
+	{
+		float p[3];
+		unit_pos(g, smp[s], p);
+		const float2 d = *reinterpret_cast<const float2 *>(g.dXg + (size_t)s * 32 + 2 * lev);
+		Corners c;
+		level_corners(g, lev, p, c);
+		for(int k = 0; k < 8; ++k)
+		{
+			atomicAdd(&lds_grid[2 * c.idx[k]], c.w[k] * d.x);
+			atomicAdd(&lds_grid[2 * c.idx[k] + 1], c.w[k] * d.y);
+		}
+	}
+	__syncthreads();
+	float *lv = g.grid_g + (size_t)lev * (g.table_mask + 1u) * kFeat;
+	for(uint32_t e = threadIdx.x; e < 2 * entries; e += kBlock)
+	{
+		const float v = lds_grid[e];
+		if(v != 0.f) atomicAdd(lv + e, v);
+	}
 }
 
 // dW (64 x 64 FP32) += sum over samples of dH[s][o] * H[s][i], split over
@@ -524,6 +581,8 @@ GArgs args_of(const ymx_guide *g)
 	a.dH3 = g->dH3;
 	a.dH2 = g->dH2;
 	a.dH1 = g->dH1;
+	a.dXg = g->dXg;
+	a.small_mask = g->small_mask;
 	a.loss = g->loss;
 	a.table_mask = g->table - 1u;
 	for(int l = 0; l < kLevels; ++l) a.res[l] = g->res[l];
@@ -539,7 +598,7 @@ void free_guide(ymx_guide *g)
 {
 	if(!g) return;
 	void *ptrs[] = {g->grid, g->grid_m, g->grid_v, g->grid_g, g->grid_bf, g->mlp, g->mlp_m, g->mlp_v, g->mlp_g,
-	                g->w_bf, g->wt_bf, g->X, g->H1, g->H2, g->H3, g->dO, g->dH3, g->dH2, g->dH1, g->loss};
+	                g->w_bf, g->wt_bf, g->X, g->H1, g->H2, g->H3, g->dO, g->dH3, g->dH2, g->dH1, g->dXg, g->loss};
 	for(void *p : ptrs)
 		if(p) (void)hipFree(p);
 	delete g;
@@ -621,6 +680,13 @@ int ymx_guide_create(const ymx_guide_params *p, ymx_guide **out)
 	G_TRY(galloc(g->wt_bf, 4 * 4096));
 	uint16_t **acts[] = {&g->X, &g->H1, &g->H2, &g->H3, &g->dO, &g->dH3, &g->dH2, &g->dH1};
 	for(uint16_t **a : acts) G_TRY(galloc(*a, (size_t)p->max_batch * 64));
+	g->small_mask = 0;
+	for(int l = 0; l < kLevels; ++l)
+	{
+		const uint64_t r1 = (uint64_t)g->res[l] + 1u;
+		if(r1 * r1 * r1 <= kSmallEntries && r1 * r1 * r1 <= (uint64_t)g->table) g->small_mask |= 1u << l;
+	}
+	if(g->small_mask) G_TRY(galloc(g->dXg, (size_t)p->max_batch * 32));
 	G_TRY(galloc(g->loss, 1));
 	// initial weights: grid U(-1e-4, 1e-4), W U(-s, s) with s = sqrt(6 / fan_in)
 	std::mt19937 rng(p->seed);
@@ -730,6 +796,21 @@ int ymx_guide_backward(ymx_guide *g, const ymx_guide_sample *samples, uint32_t n
 	YMX_HIP_CHECK(hipGetLastError());
 	k_backward<<<blocks, kBlock, 0, s>>>(a, samples, n, n_dev);
 	YMX_HIP_CHECK(hipGetLastError());
+	if(g->small_mask)
+	{
+		SmallLevels sl;
+		sl.count = 0;
+		uint32_t max_entries = 0;
+		for(int l = 0; l < kLevels; ++l)
+			if((g->small_mask >> l) & 1u)
+			{
+				sl.lev[sl.count++] = l;
+				const uint32_t r1 = (uint32_t)g->res[l] + 1u;
+				max_entries = r1 * r1 * r1 > max_entries ? r1 * r1 * r1 : max_entries;
+			}
+		k_grid_small<<<sl.count * kSmallChunks, kBlock, 2 * sizeof(float) * max_entries, s>>>(a, sl, samples, n, n_dev);
+		YMX_HIP_CHECK(hipGetLastError());
+	}
 	const uint16_t *dh[4] = {g->dH1, g->dH2, g->dH3, g->dO};
 	const uint16_t *hh[4] = {g->X, g->H1, g->H2, g->H3};
 	unsigned wblocks = (n + 4095) / 4096;

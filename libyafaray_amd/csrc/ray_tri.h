@@ -1,17 +1,28 @@
-// ray_tri.h -- the reference's tree-bound crossing and triangle test as
-// device functions, shared by the traversal kernels (accel_trace.hip) and the
-// object-light intersection of the path tracer (pt_kernels.hip).
+// ray_tri.h -- the reference's tree-bound crossing and triangle test, shared
+// by the traversal kernels (accel_trace.hip), the object-light intersection of
+// the path tracer (pt_kernels.hip) and the host traversal of the C++ layer
+// (libyafaray_amd/host), which compiles them as plain C++.
 #pragma once
 
-#include <hip/hip_runtime.h>
-
 #include <float.h>
+#include <math.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define YMX_RT_FN __host__ __device__ __forceinline__
+#else
+#define YMX_RT_FN inline
+struct float4
+{
+	float x, y, z, w;
+};
+#endif
 
 namespace ymx {
 
 // include/geometry/bound.h:157-198 on the tree bound, exactly as the reference
 // evaluates it.
-__device__ __forceinline__ bool tree_cross(const float a[3], const float g[3], const float from[3],
+YMX_RT_FN bool tree_cross(const float a[3], const float g[3], const float from[3],
                                            const float dir[3], float t_max, float &enter, float &leave)
 {
 	float p[3];
@@ -58,7 +69,7 @@ __device__ __forceinline__ bool tree_cross(const float a[3], const float g[3], c
 
 // include/geometry/shape/shape_polygon.h:126-176 (N == 3) on a pre-formed
 // record (v0, e1, e2).  Returns t > 0 on a hit, 0 otherwise.
-__device__ __forceinline__ float tri_test(const float4 t0, const float4 t1, const float4 t2, const float o[3],
+YMX_RT_FN float tri_test(const float4 t0, const float4 t1, const float4 t2, const float o[3],
                                           const float d[3], float &u_out, float &v_out)
 {
 	const float e1x = t1.x, e1y = t1.y, e1z = t1.z;

@@ -1,0 +1,508 @@
+// ymx_host.cpp -- the C++ host layer (ymx_host.hpp) over libymx.so's C ABI.
+#include "ymx_host.hpp"
+
+#include <cmath>
+#include <cstring>
+
+#include "../csrc/ray_tri.h"
+
+namespace ymx_host {
+
+// ---------------------------------------------------------------- ParamMap
+
+namespace {
+
+template <typename T>
+T get_as(const std::map<std::string, Param> &m, const std::string &name, const T &def, bool *ok)
+{
+	const auto it = m.find(name);
+	if(it == m.end()) return def;
+	if(const T *v = std::get_if<T>(&it->second)) return *v;
+	if(ok) *ok = false;
+	return def;
+}
+
+}  // namespace
+
+bool ParamMap::getBool(const std::string &name, bool def, bool *ok) const { return get_as(map_, name, def, ok); }
+int ParamMap::getInt(const std::string &name, int def, bool *ok) const { return get_as(map_, name, def, ok); }
+double ParamMap::getFloat(const std::string &name, double def, bool *ok) const
+{
+	// an int param where a float is expected is read as a float, as the reference does
+	const auto it = map_.find(name);
+	if(it != map_.end())
+		if(const int *i = std::get_if<int>(&it->second)) return (double)*i;
+	return get_as(map_, name, def, ok);
+}
+std::string ParamMap::getString(const std::string &name, const std::string &def, bool *ok) const
+{
+	return get_as(map_, name, def, ok);
+}
+Color ParamMap::getColor(const std::string &name, const Color &def, bool *ok) const { return get_as(map_, name, def, ok); }
+Vector ParamMap::getVector(const std::string &name, const Vector &def, bool *ok) const
+{
+	return get_as(map_, name, def, ok);
+}
+
+// ---------------------------------------------------------------- Accelerator
+
+namespace {
+
+constexpr float kBiasScale = 0.1f * 0.00005f;  // include/accelerator/accelerator.h:64,85
+constexpr float kSlabWiden = 1.0000004f;       // the traversal kernels' conservative slab widening
+constexpr float kWindow = 0x1p-12f;
+constexpr int32_t kRefEmpty = (int32_t)0xFFFFFFFF;
+
+inline float bits_f(int32_t i)
+{
+	float f;
+	memcpy(&f, &i, 4);
+	return f;
+}
+inline int32_t f_bits(float f)
+{
+	int32_t i;
+	memcpy(&i, &f, 4);
+	return i;
+}
+
+struct HostRay
+{
+	float o[3], d[3], idir[3];
+	float tlo;
+};
+
+// The traversal kernels' two-child slab test (accel_trace.hip, box2), evaluated
+// the same way so that the host visits every leaf the device visits.
+inline int box2(const float *n, const HostRay &r, float thi, float &tn0, float &tn1)
+{
+	const float a0x = (n[0] - r.o[0]) * r.idir[0], b0x = (n[1] - r.o[0]) * r.idir[0];
+	const float a0y = (n[2] - r.o[1]) * r.idir[1], b0y = (n[3] - r.o[1]) * r.idir[1];
+	const float a0z = (n[8] - r.o[2]) * r.idir[2], b0z = (n[9] - r.o[2]) * r.idir[2];
+	const float a1x = (n[4] - r.o[0]) * r.idir[0], b1x = (n[5] - r.o[0]) * r.idir[0];
+	const float a1y = (n[6] - r.o[1]) * r.idir[1], b1y = (n[7] - r.o[1]) * r.idir[1];
+	const float a1z = (n[10] - r.o[2]) * r.idir[2], b1z = (n[11] - r.o[2]) * r.idir[2];
+	const float n0 = fmaxf(fmaxf(fminf(a0x, b0x), fminf(a0y, b0y)), fmaxf(fminf(a0z, b0z), r.tlo));
+	const float x0 = fminf(fminf(fmaxf(a0x, b0x), fmaxf(a0y, b0y)), fmaxf(a0z, b0z)) * kSlabWiden;
+	const float n1 = fmaxf(fmaxf(fminf(a1x, b1x), fminf(a1y, b1y)), fmaxf(fminf(a1z, b1z), r.tlo));
+	const float x1 = fminf(fminf(fmaxf(a1x, b1x), fmaxf(a1y, b1y)), fmaxf(a1z, b1z)) * kSlabWiden;
+	tn0 = n0;
+	tn1 = n1;
+	int m = 0;
+	if(n0 <= fminf(x0, thi)) m |= 1;
+	if(n1 <= fminf(x1, thi)) m |= 2;
+	return m;
+}
+
+}  // namespace
+
+std::unique_ptr<Accelerator> Accelerator::factory(const ParamMap &params, const float *verts, uint32_t n_verts,
+                                                  const uint32_t *tri_idx, const uint8_t *vis, uint32_t n_tris,
+                                                  std::string *error)
+{
+	const std::string type = params.getString("type", "");
+	if(type != "mi355x-lbvh")
+	{
+		if(error) *error = "Accelerator: type '" + type + "' is not built here (only 'mi355x-lbvh')";
+		return nullptr;
+	}
+	ymx_accel_params p{};
+	p.struct_size = sizeof p;
+	p.max_leaf_size = params.getInt("max_leaf_size", 0);
+	std::unique_ptr<Accelerator> a(new Accelerator());
+	if(ymx_accel_create(verts, n_verts, tri_idx, vis, n_tris, &p, &a->acc_) != YMX_OK)
+	{
+		if(error) *error = std::string("Accelerator: ") + ymx_last_error();
+		return nullptr;
+	}
+	ymx_accel_info info{};
+	info.struct_size = sizeof info;
+	if(ymx_accel_get_info(a->acc_, &info) != YMX_OK)
+	{
+		if(error) *error = std::string("Accelerator: ") + ymx_last_error();
+		return nullptr;
+	}
+	a->n_tris_ = n_tris;
+	a->nodes_.resize(info.node_bytes / 4);
+	a->tris_.resize(info.tri_bytes / 4);
+	if(ymx_accel_export(a->acc_, a->nodes_.data(), info.node_bytes, a->tris_.data(), info.tri_bytes) != YMX_OK)
+	{
+		if(error) *error = std::string("Accelerator: ") + ymx_last_error();
+		return nullptr;
+	}
+	for(int k = 0; k < 3; ++k)
+	{
+		a->bound_.a[k] = info.bound_min[k];
+		a->bound_.g[k] = info.bound_max[k];
+	}
+	return a;
+}
+
+Accelerator::~Accelerator()
+{
+	if(acc_) ymx_accel_destroy(acc_);
+}
+
+IntersectData Accelerator::traverse(const Ray &ray, float t_max, Mode mode,
+                                    const std::function<bool(uint32_t, float, float, float)> &any) const
+{
+	IntersectData out;
+	float enter = 0.f, leave = 0.f;
+	if(!ymx::tree_cross(bound_.a, bound_.g, ray.from, ray.dir, t_max, enter, leave)) return out;
+	// include/accelerator/accelerator_kdtree_common.h:139
+	const float bias = kBiasScale * fabsf(leave - enter);
+	const float t_min = (mode == Mode::Shadow) ? bias : ((ray.tmin < bias) ? bias : ray.tmin);
+	HostRay r;
+	for(int k = 0; k < 3; ++k)
+	{
+		r.o[k] = ray.from[k];
+		r.d[k] = ray.dir[k];
+		const float dk = (fabsf(ray.dir[k]) > 1e-30f) ? ray.dir[k] : copysignf(1e-30f, ray.dir[k]);
+		r.idir[k] = 1.f / dk;
+	}
+	r.tlo = t_min - t_min * kWindow;
+	float best_t = t_max;
+	uint32_t best_prim = kNoHit;
+	float best_u = 0.f, best_v = 0.f;
+	std::vector<int32_t> stack;
+	stack.reserve(64);
+	int32_t ref = 0;
+	while(ref != kRefEmpty)
+	{
+		while(ref >= 0)
+		{
+			const float *n = nodes_.data() + 16 * (size_t)ref;
+			const float thi = best_t + best_t * kWindow;
+			float tn0, tn1;
+			const int m = box2(n, r, thi, tn0, tn1);
+			int32_t c0 = f_bits(n[12]), c1 = f_bits(n[13]);
+			if(m == 3)
+			{
+				if(tn1 < tn0) std::swap(c0, c1);
+				stack.push_back(c1);
+				ref = c0;
+			}
+			else if(m == 1) ref = c0;
+			else if(m == 2) ref = c1;
+			else
+			{
+				ref = stack.empty() ? kRefEmpty : stack.back();
+				if(!stack.empty()) stack.pop_back();
+			}
+		}
+		if(ref == kRefEmpty) break;
+		const uint32_t first = ((uint32_t)ref & 0x7FFFFFFFu) >> 4, count = ((uint32_t)ref & 15u) + 1u;
+		for(uint32_t k = 0; k < count; ++k)
+		{
+			const float *t = tris_.data() + 12 * (size_t)(first + k);
+			const float4 t0{t[0], t[1], t[2], t[3]}, t1{t[4], t[5], t[6], t[7]}, t2{t[8], t[9], t[10], t[11]};
+			float u, v;
+			const float th = ymx::tri_test(t0, t1, t2, r.o, r.d, u, v);
+			if(th <= 0.f || th < t_min) continue;
+			const uint32_t prim = (uint32_t)f_bits(t0.w);
+			const uint32_t vmask = (uint32_t)f_bits(t1.w);
+			if(mode == Mode::Closest)
+			{
+				// include/accelerator/accelerator.h:122-133; exact ties keep the lowest id
+				if(!(vmask & YMX_VIS_VISIBLE)) continue;
+				const bool closer = (th < best_t) || (th == best_t && best_prim != kNoHit && prim < best_prim);
+				if(!closer) continue;
+				best_t = th;
+				best_prim = prim;
+				best_u = u;
+				best_v = v;
+			}
+			else
+			{
+				// include/accelerator/accelerator.h:135-168
+				if(th >= t_max || !(vmask & YMX_VIS_CASTS_SHADOWS)) continue;
+				if(any(prim, th, u, v))
+				{
+					out.t_hit = out.t_max = th;
+					out.u = u;
+					out.v = v;
+					out.primitive = prim;
+					return out;
+				}
+			}
+		}
+		ref = stack.empty() ? kRefEmpty : stack.back();
+		if(!stack.empty()) stack.pop_back();
+	}
+	if(mode == Mode::Closest && best_prim != kNoHit)
+	{
+		out.t_hit = out.t_max = best_t;
+		out.u = best_u;
+		out.v = best_v;
+		out.primitive = best_prim;
+	}
+	return out;
+}
+
+IntersectData Accelerator::intersect(const Ray &ray, float t_max) const
+{
+	return traverse(ray, t_max, Mode::Closest, nullptr);
+}
+
+IntersectData Accelerator::intersectShadow(const Ray &ray, float t_max) const
+{
+	return traverse(ray, t_max, Mode::Shadow, [](uint32_t, float, float, float) { return true; });
+}
+
+int Accelerator::intersect(const ymx_ray *d_rays, ymx_hit *d_hits, uint32_t n, void *stream) const
+{
+	return ymx_accel_intersect(acc_, d_rays, d_hits, n, stream);
+}
+
+int Accelerator::intersectShadow(const ymx_ray *d_rays, uint32_t *d_occluded, uint32_t n, void *stream) const
+{
+	return ymx_accel_intersect_shadow(acc_, d_rays, d_occluded, nullptr, n, stream);
+}
+
+// ---------------------------------------------------------------- Scene
+
+namespace {
+
+void set3(float dst[3], const Color &c)
+{
+	for(int k = 0; k < 3; ++k) dst[k] = (float)c[k];
+}
+void set3(float dst[3], const Vector &v)
+{
+	for(int k = 0; k < 3; ++k) dst[k] = (float)v[k];
+}
+
+}  // namespace
+
+size_t Scene::createObject(const std::string &name, const ParamMap &params)
+{
+	Object o;
+	o.name = name;
+	const std::string vis = params.getString("visibility", "normal");
+	// include/common/visibility.h:31-37
+	if(vis == "invisible") o.visibility = 0;
+	else if(vis == "shadow_only") o.visibility = YMX_VIS_CASTS_SHADOWS;
+	else if(vis == "no_shadows") o.visibility = YMX_VIS_VISIBLE;
+	objects_.push_back(o);
+	return objects_.size() - 1;
+}
+
+size_t Scene::addVertex(size_t object_id, double x, double y, double z)
+{
+	if(object_id >= objects_.size()) return (size_t)-1;
+	std::vector<float> &v = objects_[object_id].verts;
+	v.push_back((float)x);
+	v.push_back((float)y);
+	v.push_back((float)z);
+	return v.size() / 3 - 1;
+}
+
+bool Scene::addTriangle(size_t object_id, size_t a, size_t b, size_t c, size_t material_id)
+{
+	if(object_id >= objects_.size() || material_id >= materials_.size()) return false;
+	Object &o = objects_[object_id];
+	const size_t nv = o.verts.size() / 3;
+	if(a >= nv || b >= nv || c >= nv) return false;
+	o.tris.insert(o.tris.end(), {(uint32_t)a, (uint32_t)b, (uint32_t)c});
+	o.mats.push_back((uint32_t)material_id);
+	return true;
+}
+
+bool Scene::createMaterial(size_t *id, const std::string &name, const ParamMap &p, std::string *error)
+{
+	// include/material/material.h:151-166 type names, each type's param defaults
+	const std::string type = p.getString("type", "");
+	ymx_material m{};
+	if(type == "shinydiffusemat") m.type = YMX_MAT_SHINYDIFFUSE;
+	else if(type == "glossy") m.type = YMX_MAT_GLOSSY;
+	else if(type == "mirror") m.type = YMX_MAT_MIRROR;
+	else if(type == "glass") m.type = YMX_MAT_GLASS;
+	else if(type == "light_mat") m.type = YMX_MAT_LIGHT;
+	else
+	{
+		if(error) *error = "Scene: material '" + name + "' has unsupported type '" + type + "'";
+		return false;
+	}
+	bool ok = true;
+	set3(m.color, p.getColor("color", {1, 1, 1, 1}, &ok));
+	m.diffuse_reflect = (float)p.getFloat("diffuse_reflect", type == "glossy" ? 0.0 : 1.0, &ok);
+	m.emit = (float)p.getFloat("emit", 0.0, &ok);
+	m.receive_shadows = p.getBool("receive_shadows", true, &ok) ? 1u : 0u;
+	set3(m.diffuse_color, p.getColor("diffuse_color", {1, 1, 1, 1}, &ok));
+	m.glossy_reflect = (float)p.getFloat("glossy_reflect", 1.0, &ok);
+	m.exponent = (float)p.getFloat("exponent", 50.0, &ok);
+	m.as_diffuse = p.getBool("as_diffuse", true, &ok) ? 1u : 0u;
+	m.reflect = (float)p.getFloat("reflect", 1.0, &ok);
+	m.ior = (float)p.getFloat("IOR", 1.4, &ok);
+	set3(m.filter_color, p.getColor("filter_color", {1, 1, 1, 1}, &ok));
+	m.transmit_filter = (float)p.getFloat("transmit_filter", 0.0, &ok);
+	set3(m.mirror_color, p.getColor("mirror_color", {1, 1, 1, 1}, &ok));
+	m.fake_shadows = p.getBool("fake_shadows", false, &ok) ? 1u : 0u;
+	m.power = (float)p.getFloat("power", 1.0, &ok);
+	m.double_sided = p.getBool("double_sided", false, &ok) ? 1u : 0u;
+	if(!ok)
+	{
+		if(error) *error = "Scene: material '" + name + "' has a param of the wrong type";
+		return false;
+	}
+	materials_.push_back(m);
+	material_names_.push_back(name);
+	if(id) *id = materials_.size() - 1;
+	return true;
+}
+
+bool Scene::createLight(const std::string &name, const ParamMap &p, std::string *error)
+{
+	// include/light/light.h:117-128 type names
+	const std::string type = p.getString("type", "");
+	ymx_light l{};
+	bool ok = true;
+	if(type == "arealight")
+	{
+		l.type = YMX_LIGHT_AREA;
+		set3(l.corner, p.getVector("corner", {0, 0, 0}, &ok));
+		set3(l.point1, p.getVector("point1", {0, 0, 0}, &ok));
+		set3(l.point2, p.getVector("point2", {0, 0, 0}, &ok));
+	}
+	else if(type == "pointlight")
+	{
+		l.type = YMX_LIGHT_POINT;
+		set3(l.from, p.getVector("from", {0, 0, 0}, &ok));
+	}
+	else if(type == "objectlight")
+	{
+		l.type = YMX_LIGHT_OBJECT;
+		l.double_sided = p.getBool("double_sided", false, &ok) ? 1u : 0u;
+	}
+	else
+	{
+		if(error) *error = "Scene: light '" + name + "' has unsupported type '" + type + "'";
+		return false;
+	}
+	set3(l.color, p.getColor("color", {1, 1, 1, 1}, &ok));
+	l.power = (float)p.getFloat("power", 1.0, &ok);
+	l.samples = p.getInt("samples", 4, &ok);
+	l.cast_shadows = p.getBool("cast_shadows", true, &ok) ? 1u : 0u;
+	const std::string object = p.getString("object_name", "", &ok);
+	if(!ok || (type == "objectlight" && object.empty()))
+	{
+		if(error) *error = "Scene: light '" + name + "' has a missing or mistyped param";
+		return false;
+	}
+	lights_.emplace_back(l, object);
+	return true;
+}
+
+// ---------------------------------------------------------------- PathIntegrator
+
+std::unique_ptr<PathIntegrator> PathIntegrator::create(const Scene &scene, const ParamMap &integrator,
+                                                       const ParamMap &camera, const ParamMap &film,
+                                                       const ParamMap *background, std::string *error)
+{
+	auto fail = [&](const std::string &msg) -> std::unique_ptr<PathIntegrator> {
+		if(error) *error = "PathIntegrator: " + msg;
+		return nullptr;
+	};
+	if(integrator.getString("type", "pathtracing") != "pathtracing") return fail("only the 'pathtracing' type is built");
+	if(camera.getString("type", "perspective") != "perspective") return fail("only the 'perspective' camera is built");
+	// flatten the objects; an object light covers its object's triangle range
+	std::vector<float> verts;
+	std::vector<uint32_t> tris, mats;
+	std::vector<uint8_t> vis;
+	std::map<std::string, std::pair<uint32_t, uint32_t>> ranges;
+	for(const Scene::Object &o : scene.objects_)
+	{
+		const uint32_t v0 = (uint32_t)(verts.size() / 3), t0 = (uint32_t)mats.size();
+		verts.insert(verts.end(), o.verts.begin(), o.verts.end());
+		for(uint32_t i : o.tris) tris.push_back(v0 + i);
+		mats.insert(mats.end(), o.mats.begin(), o.mats.end());
+		vis.insert(vis.end(), o.mats.size(), o.visibility);
+		ranges[o.name] = {t0, (uint32_t)o.mats.size()};
+	}
+	std::vector<ymx_light> lights;
+	for(const auto &[l, object] : scene.lights_)
+	{
+		ymx_light x = l;
+		if(x.type == YMX_LIGHT_OBJECT)
+		{
+			const auto it = ranges.find(object);
+			if(it == ranges.end() || it->second.second == 0) return fail("object light's object '" + object + "' not found");
+			x.first_tri = it->second.first;
+			x.n_tris = it->second.second;
+		}
+		lights.push_back(x);
+	}
+	std::unique_ptr<PathIntegrator> pi(new PathIntegrator());
+	ParamMap accel_params = scene.accel_params_;
+	if(!accel_params.has("type")) accel_params.setString("type", "mi355x-lbvh");
+	std::string aerr;
+	pi->accel_ = Accelerator::factory(accel_params, verts.data(), (uint32_t)(verts.size() / 3), tris.data(), vis.data(),
+	                                  (uint32_t)mats.size(), &aerr);
+	if(!pi->accel_) return fail(aerr);
+	bool ok = true;
+	ymx_pt_settings &s = pi->settings_;
+	set3(s.from, camera.getVector("from", {0, 1, 0}, &ok));
+	set3(s.to, camera.getVector("to", {0, 0, 0}, &ok));
+	set3(s.up, camera.getVector("up", {0, 1, 1}, &ok));
+	s.focal = (float)camera.getFloat("focal", 1.0, &ok);
+	s.aspect_factor = (float)camera.getFloat("aspect_ratio_factor", 1.0, &ok);
+	s.resx = camera.getInt("resx", 320, &ok);
+	s.resy = camera.getInt("resy", 200, &ok);
+	s.path_samples = integrator.getInt("path_samples", 32, &ok);
+	s.bounces = integrator.getInt("bounces", 3, &ok);
+	s.rr_min_bounces = integrator.getInt("russian_roulette_min_bounces", 0, &ok);
+	s.raydepth = integrator.getInt("raydepth", 5, &ok);
+	s.transparent_shadows = integrator.getBool("transpShad", false, &ok) ? 1 : 0;
+	s.shadow_depth = integrator.getInt("shadowDepth", 4, &ok);
+	s.seed = (uint32_t)integrator.getInt("seed", 0, &ok);
+	s.aa_samples = film.getInt("AA_minsamples", 1, &ok);
+	s.aa_pixelwidth = (float)film.getFloat("AA_pixelwidth", 1.5, &ok);
+	if(background)
+	{
+		s.has_background = 1;
+		set3(s.background, background->getColor("color", {0, 0, 0, 1}, &ok));
+	}
+	if(!ok) return fail("a param has the wrong type");
+	if(ymx_pt_create(pi->accel_->handle(), verts.data(), (uint32_t)(verts.size() / 3), tris.data(), mats.data(),
+	                 vis.data(), (uint32_t)mats.size(), scene.materials_.data(), (uint32_t)scene.materials_.size(),
+	                 lights.data(), (uint32_t)lights.size(), &s, &pi->pt_) != YMX_OK)
+		return fail(ymx_last_error());
+	return pi;
+}
+
+PathIntegrator::~PathIntegrator()
+{
+	if(pt_) ymx_pt_destroy(pt_);
+}
+
+int PathIntegrator::render(int first, int n)
+{
+	const int rc = ymx_pt_render(pt_, first, n > 0 ? n : settings_.aa_samples, nullptr);
+	if(rc != YMX_OK) error_ = ymx_last_error();
+	return rc;
+}
+
+int PathIntegrator::clearFilm()
+{
+	const int rc = ymx_pt_clear_film(pt_, nullptr);
+	if(rc != YMX_OK) error_ = ymx_last_error();
+	return rc;
+}
+
+int PathIntegrator::film(std::vector<float> &rgba, std::vector<float> &weight) const
+{
+	const size_t n = (size_t)settings_.resx * settings_.resy;
+	rgba.assign(4 * n, 0.f);
+	weight.assign(n, 0.f);
+	return ymx_pt_read_film(pt_, rgba.data(), weight.data());
+}
+
+ymx_pt_stats PathIntegrator::stats() const
+{
+	ymx_pt_stats st{};
+	st.struct_size = sizeof st;
+	(void)ymx_pt_get_stats(pt_, &st);
+	return st;
+}
+
+}  // namespace ymx_host

@@ -69,6 +69,7 @@ SIGNATURES = {
     "ymx_accel_intersect": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "ymx_accel_intersect_shadow": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                                              C.c_void_p]),
+    "ymx_accel_export": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64]),
     "ymx_accel_intersect_transparent": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
                                                   C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "ymx_accel_traversal_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int,

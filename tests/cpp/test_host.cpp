@@ -1,0 +1,499 @@
+// test_host.cpp -- the C++ host layer (libyafaray_amd/host/ymx_host.hpp)
+// driven the way the reference's clients drive libYafaRay: param maps for the
+// accelerator, materials, lights, camera and integrator.  Checks against the
+// CPU oracle (oracle/), which it links as the checker only:
+//   accel_closest / accel_shadow: single-ray host queries, the GPU batch
+//     queries and the oracle's brute force agree bit for bit;
+//   accel_transparent: host, GPU and the kd-tree port agree on occlusion and
+//     on the transparent casters passed;
+//   pathtracer: a scene built object by object renders the oracle's film;
+//   errors: unknown types and missing objects are refused with a message.
+// Usage: test_host [--cpu]   (--cpu: only the checks that need no GPU)
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "../../libyafaray_amd/host/ymx_host.hpp"
+#include "../../oracle/ora_pt.h"
+#include "../../oracle/oracle.h"
+
+using namespace ymx_host;
+
+static int g_fail = 0;
+#define CHECK(cond, ...)                   \
+	do                                     \
+	{                                      \
+		if(!(cond))                        \
+		{                                  \
+			printf("  FAIL: " __VA_ARGS__); \
+			printf("\n");                  \
+			++g_fail;                      \
+		}                                  \
+	} while(0)
+
+struct Soup
+{
+	std::vector<float> verts;
+	std::vector<uint32_t> tris;
+};
+
+static Soup random_soup(uint32_t n, uint32_t seed)
+{
+	std::mt19937 rng(seed);
+	std::uniform_real_distribution<float> u(0.f, 1.f);
+	Soup s;
+	for(uint32_t i = 0; i < n; ++i)
+	{
+		const float c[3] = {u(rng), u(rng), u(rng)};
+		const float e = 0.02f + 0.1f * u(rng);
+		for(int k = 0; k < 3; ++k)
+		{
+			for(int a = 0; a < 3; ++a) s.verts.push_back(c[a] + e * (u(rng) - 0.5f));
+			s.tris.push_back(3 * i + k);
+		}
+	}
+	return s;
+}
+
+static std::vector<Ray> random_rays(uint32_t n, uint32_t seed, bool shadow)
+{
+	std::mt19937 rng(seed);
+	std::uniform_real_distribution<float> u(0.f, 1.f);
+	std::normal_distribution<float> g;
+	std::vector<Ray> rays(n);
+	for(Ray &r : rays)
+	{
+		float d[3] = {g(rng), g(rng), g(rng)};
+		const float l = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+		for(int k = 0; k < 3; ++k)
+		{
+			r.from[k] = -0.2f + 1.4f * u(rng);
+			r.dir[k] = d[k] / l;
+		}
+		r.tmin = 1e-3f * u(rng);
+		r.tmax = shadow ? 0.05f + u(rng) : (u(rng) < 0.5f ? -1.f : 0.1f + 2.f * u(rng));
+	}
+	return rays;
+}
+
+static float tmax_of(const Ray &r) { return r.tmax >= 0.f ? r.tmax : FLT_MAX; }
+
+static void to_device(const std::vector<Ray> &rays, ymx_ray **d)
+{
+	std::vector<ymx_ray> h(rays.size());
+	for(size_t i = 0; i < rays.size(); ++i)
+	{
+		memcpy(h[i].org, rays[i].from, 12);
+		memcpy(h[i].dir, rays[i].dir, 12);
+		h[i].tmin = rays[i].tmin;
+		h[i].tmax = rays[i].tmax;
+	}
+	(void)hipMalloc(d, sizeof(ymx_ray) * h.size());
+	(void)hipMemcpy(*d, h.data(), sizeof(ymx_ray) * h.size(), hipMemcpyHostToDevice);
+}
+
+static std::vector<ora_ray> to_oracle(const std::vector<Ray> &rays)
+{
+	std::vector<ora_ray> o(rays.size());
+	for(size_t i = 0; i < rays.size(); ++i)
+	{
+		memcpy(o[i].org, rays[i].from, 12);
+		memcpy(o[i].dir, rays[i].dir, 12);
+		o[i].tmin = rays[i].tmin;
+		o[i].tmax = rays[i].tmax;
+	}
+	return o;
+}
+
+static std::unique_ptr<Accelerator> build(const Soup &s, const std::vector<uint8_t> &vis)
+{
+	ParamMap p;
+	p.setString("type", "mi355x-lbvh");
+	std::string err;
+	auto a = Accelerator::factory(p, s.verts.data(), (uint32_t)(s.verts.size() / 3), s.tris.data(), vis.data(),
+	                              (uint32_t)(s.tris.size() / 3), &err);
+	CHECK(a != nullptr, "factory: %s", err.c_str());
+	return a;
+}
+
+static void test_accel_closest()
+{
+	const Soup s = random_soup(4000, 1);
+	const uint32_t nt = (uint32_t)(s.tris.size() / 3);
+	std::vector<uint8_t> vis(nt, 3);
+	for(uint32_t i = 0; i < nt; i += 7) vis[i] = 2;  // shadow_only: not visible to closest hit
+	auto a = build(s, vis);
+	if(!a) return;
+	const std::vector<Ray> rays = random_rays(20000, 2, false);
+	std::vector<ymx_hit> gpu(rays.size());
+	ymx_ray *d_rays;
+	ymx_hit *d_hits;
+	to_device(rays, &d_rays);
+	(void)hipMalloc(&d_hits, sizeof(ymx_hit) * rays.size());
+	CHECK(a->intersect(d_rays, d_hits, (uint32_t)rays.size(), nullptr) == YMX_OK, "batch intersect");
+	(void)hipMemcpy(gpu.data(), d_hits, sizeof(ymx_hit) * rays.size(), hipMemcpyDeviceToHost);
+	std::vector<ora_hit> ora(rays.size());
+	const std::vector<ora_ray> orays = to_oracle(rays);
+	ora_closest_brute(s.verts.data(), s.tris.data(), vis.data(), nt, orays.data(), (uint32_t)rays.size(), ora.data(), 8);
+	size_t hits = 0, bad_host = 0, bad_ora = 0;
+	for(size_t i = 0; i < rays.size(); ++i)
+	{
+		const IntersectData h = a->intersect(rays[i], tmax_of(rays[i]));
+		hits += h.isHit();
+		if(h.primitive != gpu[i].prim || (h.isHit() && (h.t_hit != gpu[i].t || h.u != gpu[i].u || h.v != gpu[i].v)))
+			++bad_host;
+		if(h.primitive != ora[i].prim || (h.isHit() && h.t_hit != ora[i].t)) ++bad_ora;
+	}
+	printf("  closest: %zu rays, %zu hits, host != gpu: %zu, host != oracle: %zu\n", rays.size(), hits, bad_host, bad_ora);
+	CHECK(hits > rays.size() / 10 && bad_host == 0 && bad_ora == 0, "closest-hit mismatch");
+	(void)hipFree(d_rays);
+	(void)hipFree(d_hits);
+}
+
+static void test_accel_shadow()
+{
+	const Soup s = random_soup(4000, 3);
+	const uint32_t nt = (uint32_t)(s.tris.size() / 3);
+	std::vector<uint8_t> vis(nt, 3);
+	for(uint32_t i = 0; i < nt; i += 5) vis[i] = 1;  // no_shadows
+	auto a = build(s, vis);
+	if(!a) return;
+	const std::vector<Ray> rays = random_rays(20000, 4, true);
+	ymx_ray *d_rays;
+	uint32_t *d_occ;
+	to_device(rays, &d_rays);
+	const size_t words = (rays.size() + 31) / 32;
+	(void)hipMalloc(&d_occ, 4 * words);
+	CHECK(a->intersectShadow(d_rays, d_occ, (uint32_t)rays.size(), nullptr) == YMX_OK, "batch shadow");
+	std::vector<uint32_t> occ(words);
+	(void)hipMemcpy(occ.data(), d_occ, 4 * words, hipMemcpyDeviceToHost);
+	std::vector<uint8_t> ora(rays.size());
+	const std::vector<ora_ray> orays = to_oracle(rays);
+	ora_shadow_brute(s.verts.data(), s.tris.data(), vis.data(), nt, orays.data(), (uint32_t)rays.size(), ora.data(), 8);
+	size_t n_occ = 0, bad = 0;
+	for(size_t i = 0; i < rays.size(); ++i)
+	{
+		const bool h = a->intersectShadow(rays[i], tmax_of(rays[i])).isHit();
+		const bool g = (occ[i >> 5] >> (i & 31)) & 1u;
+		n_occ += h;
+		bad += (h != g) || (h != (bool)ora[i]);
+	}
+	printf("  shadow: %zu rays, %zu occluded, mismatches %zu\n", rays.size(), n_occ, bad);
+	CHECK(n_occ > rays.size() / 20 && n_occ < rays.size() && bad == 0, "shadow mismatch");
+	(void)hipFree(d_rays);
+	(void)hipFree(d_occ);
+}
+
+static void test_accel_transparent()
+{
+	const Soup s = random_soup(3000, 5);
+	const uint32_t nt = (uint32_t)(s.tris.size() / 3);
+	std::vector<uint8_t> vis(nt, 3), transp(nt, 0);
+	std::mt19937 rng(6);
+	for(uint32_t i = 0; i < nt; ++i) transp[i] = (rng() % 10) < 7;
+	auto a = build(s, vis);
+	if(!a) return;
+	const int depth = 3;
+	const std::vector<Ray> rays = random_rays(8000, 7, true);
+	// GPU query
+	ymx_ray *d_rays;
+	to_device(rays, &d_rays);
+	std::vector<uint32_t> mask((nt + 31) / 32 + 1, 0u);
+	for(uint32_t i = 0; i < nt; ++i)
+		if(transp[i]) mask[i >> 5] |= 1u << (i & 31);
+	uint32_t *d_mask, *d_occ, *d_cnt, *d_hits;
+	(void)hipMalloc(&d_mask, 4 * mask.size());
+	(void)hipMemcpy(d_mask, mask.data(), 4 * mask.size(), hipMemcpyHostToDevice);
+	const size_t words = (rays.size() + 31) / 32;
+	(void)hipMalloc(&d_occ, 4 * words);
+	(void)hipMalloc(&d_cnt, 4 * rays.size());
+	(void)hipMalloc(&d_hits, 8 * depth * rays.size());
+	CHECK(ymx_accel_intersect_transparent(a->handle(), d_rays, d_mask, depth, d_occ, d_cnt, d_hits,
+	                                      (uint32_t)rays.size(), nullptr) == YMX_OK, "batch transparent");
+	std::vector<uint32_t> occ(words), cnt(rays.size()), hits(2 * depth * rays.size());
+	(void)hipMemcpy(occ.data(), d_occ, 4 * words, hipMemcpyDeviceToHost);
+	(void)hipMemcpy(cnt.data(), d_cnt, 4 * rays.size(), hipMemcpyDeviceToHost);
+	(void)hipMemcpy(hits.data(), d_hits, 8 * depth * rays.size(), hipMemcpyDeviceToHost);
+	// kd-tree port
+	ora_kdtree *kd = ora_kd_build(s.verts.data(), s.tris.data(), vis.data(), nt);
+	const std::vector<ora_ray> orays = to_oracle(rays);
+	size_t bad = 0, passed = 0, n_occ = 0;
+	for(size_t i = 0; i < rays.size(); ++i)
+	{
+		std::set<uint32_t> met;
+		auto tr = [&](uint32_t p, const float *) {
+			met.insert(p);
+			return std::array<float, 3>{0.5f, 0.75f, 1.f};
+		};
+		const IntersectData h = a->intersectTransparentShadow(rays[i], depth, tmax_of(rays[i]), transp, tr);
+		const bool g = (occ[i >> 5] >> (i & 31)) & 1u;
+		uint32_t prims[64];
+		int pc = 0;
+		const int o = ora_kd_transparent1(kd, &orays[i], tmax_of(rays[i]), transp.data(), depth, prims, &pc);
+		n_occ += h.isHit();
+		if(h.isHit() != g || h.isHit() != (bool)o)
+		{
+			++bad;
+			continue;
+		}
+		if(h.isHit()) continue;
+		std::set<uint32_t> gset, oset(prims, prims + pc);
+		for(uint32_t k = 0; k < cnt[i]; ++k) gset.insert(hits[2 * (depth * i + k)]);
+		if(met != gset || met != oset) ++bad;
+		if(!met.empty())
+		{
+			++passed;
+			const float expect = powf(0.5f, (float)met.size());
+			if(fabsf(h.color[0] - expect) > 1e-6f * expect) ++bad;
+		}
+	}
+	ora_kd_free(kd);
+	printf("  transparent: %zu rays, %zu occluded, %zu pass casters, mismatches %zu\n", rays.size(), n_occ, passed, bad);
+	CHECK(bad == 0 && passed > 50 && n_occ > 50, "transparent-shadow mismatch");
+	(void)hipFree(d_rays);
+	(void)hipFree(d_mask);
+	(void)hipFree(d_occ);
+	(void)hipFree(d_cnt);
+	(void)hipFree(d_hits);
+}
+
+// A Cornell-style box built through the Scene API, with the oracle fed the
+// same flattened arrays.
+static void quad(Scene &sc, size_t obj, size_t mat, const float c[3], const float e1[3], const float e2[3],
+                 std::vector<float> &verts, std::vector<uint32_t> &tris, std::vector<uint32_t> &mats)
+{
+	const float p[4][3] = {{c[0], c[1], c[2]},
+	                       {c[0] + e1[0], c[1] + e1[1], c[2] + e1[2]},
+	                       {c[0] + e1[0] + e2[0], c[1] + e1[1] + e2[1], c[2] + e1[2] + e2[2]},
+	                       {c[0] + e2[0], c[1] + e2[1], c[2] + e2[2]}};
+	size_t id[4];
+	const uint32_t base = (uint32_t)(verts.size() / 3);
+	for(int k = 0; k < 4; ++k)
+	{
+		id[k] = sc.addVertex(obj, p[k][0], p[k][1], p[k][2]);
+		verts.insert(verts.end(), p[k], p[k] + 3);
+	}
+	(void)id;
+	const size_t nv = verts.size() / 3;
+	const size_t local0 = id[0];
+	sc.addTriangle(obj, local0, local0 + 1, local0 + 2, mat);
+	sc.addTriangle(obj, local0, local0 + 2, local0 + 3, mat);
+	tris.insert(tris.end(), {base, base + 1, base + 2, base, base + 2, base + 3});
+	mats.insert(mats.end(), {(uint32_t)mat, (uint32_t)mat});
+	(void)nv;
+}
+
+static void test_pathtracer()
+{
+	Scene sc;
+	std::string err;
+	std::vector<ora_material> omats;
+	auto material = [&](const std::string &name, double r, double g, double b) {
+		ParamMap p;
+		p.setString("type", "shinydiffusemat");
+		p.setColor("color", r, g, b);
+		size_t id = 0;
+		CHECK(sc.createMaterial(&id, name, p, &err), "material: %s", err.c_str());
+		ora_material m{};
+		m.type = ORA_MAT_SHINYDIFFUSE;
+		m.color[0] = (float)r;
+		m.color[1] = (float)g;
+		m.color[2] = (float)b;
+		m.diffuse_reflect = 1.f;
+		m.receive_shadows = 1;
+		m.diffuse_color[0] = m.diffuse_color[1] = m.diffuse_color[2] = 1.f;
+		m.glossy_reflect = 1.f;
+		m.exponent = 50.f;
+		m.as_diffuse = 1;
+		m.reflect = 1.f;
+		m.ior = 1.4f;
+		m.filter_color[0] = m.filter_color[1] = m.filter_color[2] = 1.f;
+		m.mirror_color[0] = m.mirror_color[1] = m.mirror_color[2] = 1.f;
+		m.power = 1.f;
+		omats.push_back(m);
+		return id;
+	};
+	const size_t white = material("white", 0.7, 0.7, 0.7), red = material("red", 0.7, 0.1, 0.1),
+	             green = material("green", 0.1, 0.7, 0.1), grey = material("grey", 0.4, 0.4, 0.4);
+	std::vector<float> verts;
+	std::vector<uint32_t> tris, mats;
+	ParamMap normal;
+	const size_t walls = sc.createObject("walls", normal);
+	const float o[3] = {0, 0, 0}, x[3] = {1, 0, 0}, y[3] = {0, 1, 0}, z[3] = {0, 0, 1};
+	const float top[3] = {0, 0, 1}, back[3] = {0, 1, 0}, right[3] = {1, 0, 0};
+	quad(sc, walls, white, o, y, x, verts, tris, mats);       // floor
+	quad(sc, walls, white, top, x, y, verts, tris, mats);     // ceiling
+	quad(sc, walls, white, back, z, x, verts, tris, mats);    // back wall
+	quad(sc, walls, red, o, z, y, verts, tris, mats);         // left wall
+	quad(sc, walls, green, right, y, z, verts, tris, mats);   // right wall
+	const size_t box = sc.createObject("box", normal);
+	const float b0[3] = {0.3f, 0.4f, 1.f / 512}, bx[3] = {0.3f, 0, 0}, by[3] = {0, 0.3f, 0}, bz[3] = {0, 0, 0.4f};
+	const float b1[3] = {0.3f, 0.4f, 0.4f + 1.f / 512};
+	quad(sc, box, grey, b1, bx, by, verts, tris, mats);  // box top
+	quad(sc, box, grey, b0, bz, bx, verts, tris, mats);  // box front
+	quad(sc, box, grey, b0, by, bz, verts, tris, mats);  // box left
+	ParamMap al;
+	al.setString("type", "arealight");
+	al.setVector("corner", 0.4, 0.4, 0.999);
+	al.setVector("point1", 0.4, 0.6, 0.999);
+	al.setVector("point2", 0.6, 0.4, 0.999);
+	al.setColor("color", 1, 1, 1);
+	al.setFloat("power", 10.0 / M_PI);
+	al.setInt("samples", 4);
+	CHECK(sc.createLight("area", al, &err), "light: %s", err.c_str());
+	ParamMap pl;
+	pl.setString("type", "pointlight");
+	pl.setVector("from", 0.8, 0.2, 0.6);
+	pl.setColor("color", 1, 0.9, 0.8);
+	pl.setFloat("power", 0.05);
+	CHECK(sc.createLight("point", pl, &err), "light: %s", err.c_str());
+	ParamMap integ, cam, film;
+	integ.setString("type", "pathtracing");
+	integ.setInt("bounces", 4);
+	integ.setInt("path_samples", 1);
+	integ.setInt("russian_roulette_min_bounces", 2);
+	cam.setString("type", "perspective");
+	cam.setVector("from", 0.5, -1.6, 0.5);
+	cam.setVector("to", 0.5, 0.5, 0.5);
+	cam.setVector("up", 0.5, -1.6, 1.5);
+	cam.setFloat("focal", 1.0 / (2.0 * tan(20.0 * M_PI / 180.0)));
+	cam.setInt("resx", 40);
+	cam.setInt("resy", 32);
+	film.setInt("AA_minsamples", 4);
+	auto pi = PathIntegrator::create(sc, integ, cam, film, nullptr, &err);
+	CHECK(pi != nullptr, "create: %s", err.c_str());
+	if(!pi) return;
+	CHECK(pi->clearFilm() == YMX_OK && pi->render() == YMX_OK, "render: %s", pi->lastError().c_str());
+	std::vector<float> rgba, w;
+	CHECK(pi->film(rgba, w) == YMX_OK, "film");
+	// the oracle on the same arrays
+	ora_light lights[2] = {};
+	const float corner[3] = {0.4f, 0.4f, 0.999f}, p1[3] = {0.4f, 0.6f, 0.999f}, p2[3] = {0.6f, 0.4f, 0.999f};
+	lights[0].type = ORA_LIGHT_AREA;
+	memcpy(lights[0].corner, corner, 12);
+	memcpy(lights[0].point1, p1, 12);
+	memcpy(lights[0].point2, p2, 12);
+	lights[0].color[0] = lights[0].color[1] = lights[0].color[2] = 1.f;
+	lights[0].power = (float)(10.0 / M_PI);
+	lights[0].samples = 4;
+	lights[0].cast_shadows = 1;
+	lights[1].type = ORA_LIGHT_POINT;
+	lights[1].from[0] = 0.8f;
+	lights[1].from[1] = 0.2f;
+	lights[1].from[2] = 0.6f;
+	lights[1].color[0] = 1.f;
+	lights[1].color[1] = 0.9f;
+	lights[1].color[2] = 0.8f;
+	lights[1].power = 0.05f;
+	lights[1].samples = 4;
+	lights[1].cast_shadows = 1;
+	ora_pt_settings st{};
+	const float from[3] = {0.5f, -1.6f, 0.5f}, to[3] = {0.5f, 0.5f, 0.5f}, up[3] = {0.5f, -1.6f, 1.5f};
+	memcpy(st.from, from, 12);
+	memcpy(st.to, to, 12);
+	memcpy(st.up, up, 12);
+	st.focal = (float)(1.0 / (2.0 * tan(20.0 * M_PI / 180.0)));
+	st.aspect_factor = 1.f;
+	st.resx = 40;
+	st.resy = 32;
+	st.path_samples = 1;
+	st.bounces = 4;
+	st.rr_min_bounces = 2;
+	st.aa_samples = 4;
+	st.aa_pixelwidth = 1.5f;
+	st.raydepth = 5;
+	st.shadow_depth = 4;
+	std::vector<uint8_t> vis(mats.size(), 3);
+	ora_kdtree *kd = ora_kd_build(verts.data(), tris.data(), vis.data(), (uint32_t)mats.size());
+	std::vector<float> orgba(4 * 40 * 32, 0.f), ow(40 * 32, 0.f);
+	ora_pt_render(kd, verts.data(), tris.data(), mats.data(), vis.data(), omats.data(), lights, 2, &st, orgba.data(),
+	              ow.data(), 8);
+	ora_kd_free(kd);
+	size_t bad_w = 0, bad_c = 0;
+	double mean = 0.0;
+	for(size_t i = 0; i < w.size(); ++i)
+	{
+		bad_w += w[i] != ow[i];
+		for(int k = 0; k < 3; ++k)
+		{
+			const float g = rgba[4 * i + k] / w[i], r = orgba[4 * i + k] / ow[i];
+			mean += r;
+			if(fabsf(g - r) > 1e-4f * fabsf(r) + 1e-6f) ++bad_c;
+		}
+	}
+	printf("  pathtracer: 40x32x4, mean %.5f, weight mismatches %zu, colour values off %zu\n", mean / (3.0 * w.size()),
+	       bad_w, bad_c);
+	CHECK(bad_w == 0 && bad_c <= 3 && mean > 0.0, "film differs from the oracle");
+}
+
+static void test_errors()
+{
+	std::string err;
+	ParamMap p;
+	p.setString("type", "yafaray-kdtree-original");
+	const float v[9] = {0, 0, 0, 1, 0, 0, 0, 1, 0};
+	const uint32_t t[3] = {0, 1, 2};
+	CHECK(Accelerator::factory(p, v, 3, t, nullptr, 1, &err) == nullptr && !err.empty(), "other accelerator types refused");
+	Scene sc;
+	ParamMap m;
+	m.setString("type", "roughglass");
+	size_t id;
+	err.clear();
+	CHECK(!sc.createMaterial(&id, "x", m, &err) && !err.empty(), "unsupported material refused");
+	m.setString("type", "glossy");
+	m.setString("exponent", "fifty");
+	err.clear();
+	CHECK(!sc.createMaterial(&id, "x", m, &err) && !err.empty(), "mistyped param refused");
+	ParamMap l;
+	l.setString("type", "objectlight");
+	err.clear();
+	CHECK(!sc.createLight("l", l, &err) && !err.empty(), "object light without object_name refused");
+	l.setString("object_name", "nowhere");
+	CHECK(sc.createLight("l", l, &err), "object light accepted at creation: %s", err.c_str());
+	ParamMap integ, cam, film;
+	err.clear();
+	CHECK(PathIntegrator::create(sc, integ, cam, film, nullptr, &err) == nullptr && !err.empty(),
+	      "object light with a missing object refused at create");
+	integ.setString("type", "photonmapping");
+	err.clear();
+	CHECK(PathIntegrator::create(sc, integ, cam, film, nullptr, &err) == nullptr && !err.empty(),
+	      "other integrators refused");
+	ParamMap q;
+	q.setInt("n", 3);
+	bool ok = true;
+	CHECK(q.getFloat("n", 0.0, &ok) == 3.0 && ok, "int read as float");
+	CHECK(q.getString("n", "d", &ok) == "d" && !ok, "mistyped read flagged");
+}
+
+int main(int argc, char **argv)
+{
+	const bool cpu_only = argc > 1 && strcmp(argv[1], "--cpu") == 0;
+	struct
+	{
+		const char *name;
+		void (*fn)();
+		bool gpu;
+	} tests[] = {{"errors", test_errors, false},
+	             {"accel_closest", test_accel_closest, true},
+	             {"accel_shadow", test_accel_shadow, true},
+	             {"accel_transparent", test_accel_transparent, true},
+	             {"pathtracer", test_pathtracer, true}};
+	for(const auto &t : tests)
+	{
+		if(cpu_only && t.gpu) continue;
+		printf("%s\n", t.name);
+		fflush(stdout);
+		const int before = g_fail;
+		t.fn();
+		printf("%s %s\n", g_fail == before ? "PASS" : "FAIL", t.name);
+		fflush(stdout);
+	}
+	printf("%s\n", g_fail ? "FAILED" : "ALL PASSED");
+	return g_fail ? 1 : 0;
+}
