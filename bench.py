@@ -98,36 +98,55 @@ def measure_path_tracer(args, torch):
     return out
 
 
+GUIDE_N_MIN, GUIDE_N_MAX = 4, 32   # hash-grid resolutions of the guided occluded-room run
+GUIDE_TRAIN_SPP = 32               # the guide trains on the first 32 spp of the render
+
+
 def measure_guiding(args, torch):
     """Occluded room (4 x 3 x 3 m, about 533k triangles, lit through a 0.1 m
     slit from an adjacent room; max depth 10): per-pixel relMSE
-    (I - R)^2 / (R^2 + 0.01) of plain and guided renders of `guided_spp`
-    samples against a `guided_ref_spp` plain reference (two independent
-    halves, whose own error is subtracted), render times, and the time to
-    relMSE 1e-3 extrapolated with relMSE proportional to 1 / samples (both
-    estimators are unbiased).  The guide (16-level hash grid of 2^19 entries x
-    2 features, 3 x 64 MLP, 8 vMF lobes, BF16 MFMA, Adam lr 1e-2) trains on the
-    radiance samples of the first 32 spp; its training time is charged to the
-    guided render.  Guided sampling mixes the BSDF sampler (probability
-    bsdf_prob) and the learned mixture by one-sample MIS."""
+    (I - R)^2 / (R^2 + 0.01) against a `guided_ref_spp` plain reference (two
+    independent halves, whose own error is subtracted), and the time each
+    tracer needs to reach relMSE 1e-3.
+
+    Plain: `guided_spp` samples; relMSE falls as 1 / samples, so the time to
+    1e-3 is t_plain * relMSE / 1e-3.
+
+    Guided: the guide (16-level hash grid of 2^19 entries x 2 features, 3 x 64
+    MLP, 8 vMF lobes, BF16 MFMA, Adam lr 1e-2) trains online on the first 32
+    spp of the render (32 passes of 1 spp, 4 Adam steps after each), then the
+    render continues with the trained guide up to `guided_spp`.  Every sample,
+    training passes included, is a sample of the image (all are unbiased);
+    guided sampling mixes the BSDF sampler (probability bsdf_prob) and the
+    mixture by one-sample MIS.  With v_t and v_g the per-sample error of the
+    training and post-training samples (relMSE x samples of each part, the
+    post-training part taken from the film sums after minus before), an image of
+    32 + m samples has relMSE (32 v_t + m v_g) / (32 + m)^2; the time to 1e-3 is
+    the training time plus m times the post-training time per sample, for the
+    smallest such m."""
     from libyafaray_amd import scenes
     from libyafaray_amd.guide import GpuGuide
     from libyafaray_amd.pt import GpuPathTracer
     resx, resy = (int(v) for v in args.guided_res.split("x"))
     sc = scenes.occluded_room_scene(res=(resx, resy), spp=16)
     spp, ref_spp = args.guided_spp, args.guided_ref_spp
+    n0 = GUIDE_TRAIN_SPP
+    if spp <= n0:
+        raise SystemExit(f"--guided-spp must exceed the {n0} training samples")
 
     def tracer(aa, seed):
         s = dict(sc)
         s["settings"] = dict(sc["settings"], aa_samples=aa, seed=seed)
         return GpuPathTracer(s)
 
+    def ratio(rgba, w):
+        return rgba[..., :3].astype(np.float64) / w[..., None]
+
     def image(pt):
-        rgba, w = pt.film()
-        return rgba[..., :3] / w[..., None]
+        return ratio(*pt.film())
 
     def relmse(a, b):
-        return float(((a.astype(np.float64) - b) ** 2 / (b.astype(np.float64) ** 2 + 1e-2)).mean())
+        return float(((a - b) ** 2 / (b.astype(np.float64) ** 2 + 1e-2)).mean())
 
     pt = tracer(ref_spp // 2, 777)
     pt.clear()
@@ -140,45 +159,61 @@ def measure_guiding(args, torch):
     ref = (r1 + r2) / 2
     ref_err = relmse(r1, r2) / 4
 
-    def timed(pt, first, n):
+    def timed(fn):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        pt.clear()
-        pt.render(first, n)
+        fn()
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
     p = tracer(spp, 0)
-    timed(p, 0, 1)
-    t_plain = timed(p, 0, spp)
+    p.clear()
+    p.render(0, 1)  # warm-up
+    p.clear()
+    t_plain = timed(lambda: p.render(0, spp))
     e_plain = relmse(image(p), ref) - ref_err
     bmin, bmax = sc["verts"].min(0) - 1e-3, sc["verts"].max(0) + 1e-3
-    alpha, steps, passes = 0.5, 4, 32
-    g = GpuGuide(bmin, bmax, log2_table_size=19, n_min=4, n_max=32, max_batch=1 << 16, lr=1e-2, seed=1)
+    alpha, steps = 0.5, 4
+    g = GpuGuide(bmin, bmax, log2_table_size=19, n_min=GUIDE_N_MIN, n_max=GUIDE_N_MAX, max_batch=1 << 16,
+                 lr=1e-2, seed=1)
     p.set_guide(g, bsdf_prob=alpha, train=True, train_steps=steps)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
     p.clear()
-    for k in range(passes):
-        p.render(1_000_000 + k, 1)
-    torch.cuda.synchronize()
-    t_train = time.perf_counter() - t0
+
+    def train():
+        for k in range(n0):
+            p.render(k, 1)
+
+    t_train = timed(train)
+    rgba_t, w_t = p.film()
+    e_train = relmse(ratio(rgba_t, w_t), ref) - ref_err
     p.set_guide(g, bsdf_prob=alpha, train=False)
-    t_guided = timed(p, 0, spp)
-    e_guided = relmse(image(p), ref) - ref_err
+    m = spp - n0
+    t_rest = timed(lambda: p.render(n0, m))
+    rgba_c, w_c = p.film()
+    e_all = relmse(ratio(rgba_c, w_c), ref) - ref_err
+    e_rest = relmse(ratio(rgba_c.astype(np.float64) - rgba_t, w_c.astype(np.float64) - w_t), ref) - ref_err
     p.set_guide(None)
     g.close()
     p.close()
     target = 1e-3
     tt_plain = t_plain * e_plain / target
-    tt_guided = t_train + t_guided * e_guided / target
+    v_t, v_g = e_train * n0, e_rest * m
+    # smallest m with (n0 v_t + m v_g) / (n0 + m)^2 <= target (a quadratic in m)
+    a_, b_, c_ = target, 2 * target * n0 - v_g, target * n0 * n0 - n0 * v_t
+    m_need = 0.0 if c_ >= 0 else (-b_ + (b_ * b_ - 4 * a_ * c_) ** 0.5) / (2 * a_)
+    tt_guided = t_train + m_need * t_rest / m
     out = {"workload": f"occluded room, {len(sc['tris'])} tris, {resx}x{resy}, depth 10, slit 0.1 m",
            "reference_spp": ref_spp, "reference_relmse": ref_err, "spp": spp,
-           "plain_ms": round(t_plain * 1e3, 1), "guided_ms": round(t_guided * 1e3, 1),
-           "train_ms": round(t_train * 1e3, 1), "train": f"{passes} passes of 1 spp, {steps} Adam steps each",
-           "bsdf_prob": alpha, "grid_levels": "16 (resolution 4..32)",
-           "relmse_plain": e_plain, "relmse_guided": e_guided,
-           "equal_spp_gain": round(e_plain / e_guided, 3) if e_guided > 0 else None,
+           "plain_ms": round(t_plain * 1e3, 1),
+           "guided_ms": round((t_train + t_rest) * 1e3, 1),
+           "train_ms": round(t_train * 1e3, 1),
+           "train": f"first {n0} spp of the render, 1 spp per pass, {steps} Adam steps after each",
+           "post_train_ms_per_spp": round(t_rest / m * 1e3, 2), "plain_ms_per_spp": round(t_plain / spp * 1e3, 2),
+           "bsdf_prob": alpha, "grid_levels": f"16 (resolution {GUIDE_N_MIN}..{GUIDE_N_MAX})",
+           "relmse_plain": e_plain, "relmse_guided": e_all,
+           "relmse_training_part": e_train, "relmse_post_training_part": e_rest,
+           "equal_spp_gain": round(e_plain / e_all, 3) if e_all > 0 else None,
+           "per_sample_gain_after_training": round((e_plain * spp) / v_g, 3) if v_g > 0 else None,
            "time_to_relmse_1e-3_plain_s": round(tt_plain, 2), "time_to_relmse_1e-3_guided_s": round(tt_guided, 2),
            "guided_over_plain_time": round(tt_guided / tt_plain, 3) if tt_plain > 0 else None}
     log(args, f"guided path tracing: {out}")
@@ -205,7 +240,9 @@ def measure_guide_kernels(args, torch):
         return (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
 
     n_inf, n_train = 1 << 21, 1 << 16
-    g = GpuGuide((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), log2_table_size=19, max_batch=n_train, seed=2)
+    # the grid of the guided occluded-room run (16 levels, resolution 4 .. 32)
+    g = GpuGuide((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), log2_table_size=19, n_min=GUIDE_N_MIN, n_max=GUIDE_N_MAX,
+                 max_batch=n_train, seed=2)
     q = torch.from_numpy(pack_samples(rng.uniform(0, 1, (n_inf, 3)).astype(np.float32), unit(n_inf),
                                       unit(n_inf))).cuda()
     smp = torch.from_numpy(pack_samples(rng.uniform(0, 1, (n_train, 3)).astype(np.float32), unit(n_train),

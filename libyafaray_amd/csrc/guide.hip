@@ -131,26 +131,78 @@ __device__ __forceinline__ void level_corners(const GArgs &g, int lev, const flo
 	}
 }
 
-// First-layer input of the lane: K-block 0 holds levels 4g .. 4g+3 (2 features
-// each), K-block 1 the normal, wo and a constant 1 (lane group 0 only).
-__device__ __forceinline__ void encode(const GArgs &g, const ymx_guide_sample &r, int grp, Frag &x0, Frag &x1)
+// Two consecutive lattice entries (2 BF16 features each) in one 8-byte load;
+// entries are only 4-byte aligned, which global loads accept.
+typedef unsigned long long __attribute__((aligned(4))) entry_pair_t;
+
+// Interpolated features of one level.  On a dense level the corners (x, y, z)
+// and (x + 1, y, z) are adjacent entries, so the 8 corners take 4 paired loads;
+// at x == res (only on the upper face, where the x weight of the upper corner
+// is 0) the pair starts at res - 1 with weights (0, 1), which sums the same
+// terms.  The products and the order of the sum are those of level_corners.
+__device__ __forceinline__ void level_features(const GArgs &g, int lev, const float p[3], float &f0, float &f1)
 {
-	float p[3];
-	unit_pos(g, r, p);
+	const int r = g.res[lev];
 	const uint32_t table = g.table_mask + 1u;
-	for(int q = 0; q < 4; ++q)
+	const uint32_t *lv = reinterpret_cast<const uint32_t *>(g.grid_bf) + (size_t)lev * table;
+	const uint32_t r1 = (uint32_t)r + 1u;
+	f0 = 0.f;
+	f1 = 0.f;
+	if((uint64_t)r1 * r1 * r1 > (uint64_t)table)
 	{
-		const int lev = 4 * grp + q;
 		Corners c;
 		level_corners(g, lev, p, c);
-		float f0 = 0.f, f1 = 0.f;
-		const uint32_t *lv = reinterpret_cast<const uint32_t *>(g.grid_bf) + (size_t)lev * table;
 		for(int k = 0; k < 8; ++k)
 		{
 			const uint32_t pair = lv[c.idx[k]];
 			f0 = f0 + c.w[k] * bf2f((uint16_t)(pair & 0xFFFFu));
 			f1 = f1 + c.w[k] * bf2f((uint16_t)(pair >> 16));
 		}
+		return;
+	}
+	uint32_t b[3];
+	float f[3];
+	for(int a = 0; a < 3; ++a)
+	{
+		const float sc = p[a] * (float)r;
+		const float fl = floorf(sc);
+		b[a] = (uint32_t)fl;
+		f[a] = sc - fl;
+	}
+	const uint32_t ur = (uint32_t)r;
+	if(b[0] >= ur)
+	{
+		b[0] = ur - 1u;  // res >= 2
+		f[0] = 1.f;
+	}
+	for(int k = 0; k < 8; k += 2)
+	{
+		const uint32_t dy = (k >> 1) & 1, dz = (k >> 2) & 1;
+		const uint32_t y = b[1] + dy < ur ? b[1] + dy : ur;
+		const uint32_t z = b[2] + dz < ur ? b[2] + dz : ur;
+		const float wy = dy ? f[1] : 1.f - f[1];
+		const float wz = dz ? f[2] : 1.f - f[2];
+		const float w0 = ((1.f - f[0]) * wy) * wz;
+		const float w1 = (f[0] * wy) * wz;
+		const unsigned long long e2 = *reinterpret_cast<const entry_pair_t *>(lv + b[0] + y * r1 + z * r1 * r1);
+		const uint32_t e0 = (uint32_t)e2, e1 = (uint32_t)(e2 >> 32);
+		f0 = f0 + w0 * bf2f((uint16_t)(e0 & 0xFFFFu));
+		f1 = f1 + w0 * bf2f((uint16_t)(e0 >> 16));
+		f0 = f0 + w1 * bf2f((uint16_t)(e1 & 0xFFFFu));
+		f1 = f1 + w1 * bf2f((uint16_t)(e1 >> 16));
+	}
+}
+
+// First-layer input of the lane: K-block 0 holds levels 4g .. 4g+3 (2 features
+// each), K-block 1 the normal, wo and a constant 1 (lane group 0 only).
+__device__ __forceinline__ void encode(const GArgs &g, const ymx_guide_sample &r, int grp, Frag &x0, Frag &x1)
+{
+	float p[3];
+	unit_pos(g, r, p);
+	for(int q = 0; q < 4; ++q)
+	{
+		float f0, f1;
+		level_features(g, 4 * grp + q, p, f0, f1);
 		x0.h[2 * q] = f2bf(f0);
 		x0.h[2 * q + 1] = f2bf(f1);
 	}
@@ -256,6 +308,35 @@ __device__ void decode(const float *raw, float *mix)
 	}
 }
 
+// decode() of lobes 2 grp and 2 grp + 1 only (12 floats to out), with the
+// softmax max and sum taken over all 8 lobes in decode()'s order.
+__device__ __forceinline__ void decode_pair(const float *raw, int grp, float *out)
+{
+	float mx = -INFINITY;
+	for(int k = 0; k < kLobes; ++k) mx = fmaxf(mx, raw[5 * k + 4]);
+	float sum = 0.f;
+	for(int k = 0; k < kLobes; ++k) sum += expf(raw[5 * k + 4] - mx);
+	float o[12];
+	for(int j = 0; j < 2; ++j)
+	{
+		const int k = 2 * grp + j;
+		const float *l = raw + 5 * k;
+		const float n = sqrtf(l[0] * l[0] + l[1] * l[1] + l[2] * l[2]);
+		const float inv = 1.f / fmaxf(n, 1e-8f);
+		const float sc = fminf(fmaxf(l[3], kKappaLogMin), kKappaLogMax);
+		const float kappa = expf(sc);
+		float *q = o + 6 * j;
+		q[0] = l[0] * inv;
+		q[1] = l[1] * inv;
+		q[2] = l[2] * inv;
+		q[3] = kappa;
+		q[4] = expf(l[4] - mx) / sum;
+		q[5] = kappa / (6.283185307179586f * (-expm1f(-2.f * kappa)));
+	}
+	float4 *dst = reinterpret_cast<float4 *>(out);
+	for(int k = 0; k < 3; ++k) dst[k] = make_float4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+}
+
 // d(-c log q)/d raw for one training sample (oracle/ora_guide.py dloss_dout).
 __device__ float loss_grad(const float *raw, const ymx_guide_sample &r, float *d)
 {
@@ -293,10 +374,9 @@ __device__ float loss_grad(const float *raw, const ymx_guide_sample &r, float *d
 	return -c * logf(qs);
 }
 
-// Forward: inference (decoded mixtures) or training (activations, loss gradient).
-template <bool kTrain>
+// Training forward: the BF16 activations of every layer and the loss gradient.
 __global__ __launch_bounds__(kBlock) void k_forward(GArgs g, const ymx_guide_sample *smp, uint32_t cap,
-                                                     const uint32_t *n_dev, float *mix_out)
+                                                     const uint32_t *n_dev)
 {
 	__shared__ float outs[kWaves][16][kOut];
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -313,7 +393,7 @@ __global__ __launch_bounds__(kBlock) void k_forward(GArgs g, const ymx_guide_sam
 		else memset(&r, 0, sizeof r);
 		Frag in[2];
 		encode(g, r, grp, in[0], in[1]);
-		if(kTrain && valid)
+		if(valid)
 		{
 			uint16_t *row = g.X + (size_t)s * 64;
 			*reinterpret_cast<uint4 *>(row + 8 * grp) = in[0].u;
@@ -326,7 +406,7 @@ __global__ __launch_bounds__(kBlock) void k_forward(GArgs g, const ymx_guide_sam
 		{
 			in[0] = acc_to_b(acc, 0, true);
 			in[1] = acc_to_b(acc, 1, true);
-			if(kTrain && valid) store_act(hbuf[l - 1], s, grp, in);
+			if(valid) store_act(hbuf[l - 1], s, grp, in);
 			layer(g.w_bf + l * 4096, in, true, sl, grp, acc, l < 3 ? 4 : 3);
 		}
 		for(int t = 0; t < 3; ++t)
@@ -336,14 +416,6 @@ __global__ __launch_bounds__(kBlock) void k_forward(GArgs g, const ymx_guide_sam
 	if(!wave_active || grp != 0 || !valid) return;
 	const ymx_guide_sample r = smp[s];
 	const float *raw = outs[wave][sl];
-	if(!kTrain)
-	{
-		float mix[kMix];
-		decode(raw, mix);
-		float4 *o = reinterpret_cast<float4 *>(mix_out + (size_t)s * kMix);
-		for(int k = 0; k < kMix / 4; ++k) o[k] = make_float4(mix[4 * k], mix[4 * k + 1], mix[4 * k + 2], mix[4 * k + 3]);
-		return;
-	}
 	float d[64];
 	const float l = loss_grad(raw, r, d);
 	atomicAdd(g.loss, (double)l);
@@ -353,6 +425,89 @@ __global__ __launch_bounds__(kBlock) void k_forward(GArgs g, const ymx_guide_sam
 		Frag f;
 		for(int j = 0; j < 8; ++j) f.h[j] = f2bf(d[k + j]);
 		*reinterpret_cast<uint4 *>(row + k) = f.u;
+	}
+}
+
+// Inference, persistent: each block stages the four weight matrices in LDS
+// once (32 KiB) and its waves then take 16-query tiles until the batch is
+// done.  Every A operand is one 16-byte LDS read: row r of layer l is 8 chunks
+// of 8 BF16, chunk c = 4 ks + grp holding the elements lane group grp needs
+// at K-step ks (natural order for layer 1, the permuted order of the file
+// comment for the others), stored at position c ^ (r & 7) so that the 16 rows
+// a wave reads together fall on distinct banks.
+constexpr int kInferWaves = 12;
+constexpr int kInferBlock = 64 * kInferWaves;
+constexpr int kInferBlocksPerCu = 2;
+
+__device__ __forceinline__ void layer_lds(const uint4 *wl, int l, const Frag *in, int sl, int grp, f32x4 *acc, int tiles)
+{
+	for(int t = 0; t < tiles; ++t)
+	{
+		acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+		const int row = 16 * t + sl;
+		for(int ks = 0; ks < 2; ++ks)
+		{
+			Frag a;
+			a.u = wl[(l * 64 + row) * 8 + ((4 * ks + grp) ^ (row & 7))];
+			acc[t] = mfma(a, in[ks], acc[t]);
+		}
+	}
+}
+
+__device__ __forceinline__ void wave_sync()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(kInferBlock) void k_infer(GArgs g, const ymx_guide_sample *smp, uint32_t cap,
+                                                        const uint32_t *n_dev, float *mix_out)
+{
+	__shared__ uint4 wl[4 * 64 * 8];
+	__shared__ float outs[kInferWaves][16][kOut];
+	for(int e = threadIdx.x; e < 4 * 64 * 8; e += kInferBlock)
+	{
+		const int l = e >> 9, row = (e >> 3) & 63, c = e & 7;
+		const uint16_t *src = g.w_bf + l * 4096 + row * 64;
+		Frag f;
+		if(l == 0)
+			f.u = *reinterpret_cast<const uint4 *>(src + 8 * c);
+		else
+		{
+			const int ks = c >> 2, grp = c & 3;
+			f.d[0] = *reinterpret_cast<const uint2 *>(src + 32 * ks + 4 * grp);
+			f.d[1] = *reinterpret_cast<const uint2 *>(src + 32 * ks + 16 + 4 * grp);
+		}
+		wl[(l * 64 + row) * 8 + (c ^ (row & 7))] = f.u;
+	}
+	__syncthreads();
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int sl = lane & 15, grp = lane >> 4;
+	const uint32_t n = count_of(cap, n_dev);
+	const uint32_t tiles = (n + 15u) / 16u;
+	for(uint32_t tile = blockIdx.x * kInferWaves + wave; tile < tiles; tile += gridDim.x * kInferWaves)
+	{
+		const uint32_t s = tile * 16u + sl;
+		const bool valid = s < n;
+		ymx_guide_sample r;
+		if(valid) r = smp[s];
+		else memset(&r, 0, sizeof r);
+		Frag in[2];
+		encode(g, r, grp, in[0], in[1]);
+		f32x4 acc[4];
+		layer_lds(wl, 0, in, sl, grp, acc, 4);
+		for(int l = 1; l < 4; ++l)
+		{
+			in[0] = acc_to_b(acc, 0, true);
+			in[1] = acc_to_b(acc, 1, true);
+			layer_lds(wl, l, in, sl, grp, acc, l < 3 ? 4 : 3);
+		}
+		for(int t = 0; t < 3; ++t)
+			for(int i = 0; i < 4; ++i) outs[wave][sl][16 * t + 4 * grp + i] = acc[t][i];
+		wave_sync();
+		if(valid) decode_pair(outs[wave][sl], grp, mix_out + (size_t)s * kMix + 12 * grp);
+		wave_sync();
 	}
 }
 
@@ -766,8 +921,18 @@ int ymx_guide_infer(ymx_guide *g, const ymx_guide_sample *queries, float *mix, u
 		return YMX_ERR_INVALID_ARG;
 	}
 	if(n == 0) return YMX_OK;
-	const unsigned blocks = (n + 16 * kWaves - 1) / (16 * kWaves);
-	k_forward<false><<<blocks, kBlock, 0, (hipStream_t)stream>>>(args_of(g), queries, n, n_dev, mix);
+	static int cus = 0;
+	if(cus == 0)
+	{
+		int dev = 0, c = 0;
+		if(hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0)
+			c = 256;
+		cus = c;
+	}
+	const uint32_t tiles = (n + 15u) / 16u;
+	const uint32_t want = (tiles + kInferWaves - 1) / kInferWaves;
+	const uint32_t blocks = want < (uint32_t)(kInferBlocksPerCu * cus) ? want : (uint32_t)(kInferBlocksPerCu * cus);
+	k_infer<<<blocks, kInferBlock, 0, (hipStream_t)stream>>>(args_of(g), queries, n, n_dev, mix);
 	YMX_HIP_CHECK(hipGetLastError());
 	return YMX_OK;
 }
@@ -792,7 +957,7 @@ int ymx_guide_backward(ymx_guide *g, const ymx_guide_sample *samples, uint32_t n
 	if(n == 0) return YMX_OK;
 	const GArgs a = args_of(g);
 	const unsigned blocks = (n + 16 * kWaves - 1) / (16 * kWaves);
-	k_forward<true><<<blocks, kBlock, 0, s>>>(a, samples, n, n_dev, nullptr);
+	k_forward<<<blocks, kBlock, 0, s>>>(a, samples, n, n_dev);
 	YMX_HIP_CHECK(hipGetLastError());
 	k_backward<<<blocks, kBlock, 0, s>>>(a, samples, n, n_dev);
 	YMX_HIP_CHECK(hipGetLastError());

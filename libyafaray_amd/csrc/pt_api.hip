@@ -607,7 +607,9 @@ int ymx_pt_set_guide(ymx_pt *pt, ymx_guide *guide, float bsdf_prob, int32_t trai
 	pt->train_steps = 0;
 	if(!guide) return YMX_OK;
 	PtGuide &g = pt->g;
-	const uint32_t cap = pt->st.capacity < (1u << 21) ? pt->st.capacity : (1u << 21);
+	// the guide buffers cover the whole wavefront (2^24 paths at most: 7.5 GB
+	// of queries and mixtures, 13 GB more of pending training records)
+	const uint32_t cap = pt->st.capacity;
 	pt->guide_cap = cap;
 	const uint32_t bounces = (uint32_t)pt->host_const.bounces;
 	hipError_t e = hipSuccess;
