@@ -43,8 +43,12 @@ struct ymx_guide
 	uint16_t *w_bf;                              // 4 x [64 out][64 in] (W4 rows 48..63 zero)
 	uint16_t *wt_bf;                             // 4 x [64 in][64 out] (W4^T columns 48..63 zero)
 	uint16_t *X, *H1, *H2, *H3, *dO, *dH3, *dH2, *dH1;  // [max_batch][64]
-	float *dXg;                                  // [max_batch][32] grid-feature gradients of the small levels
-	uint32_t small_mask;                         // levels whose gradient is summed in LDS (k_grid_small)
+	float *dXg;                                  // [level][max_batch] float2 grid-feature gradients (dense levels)
+	uint32_t slab_mask;                          // dense levels whose gradient is summed per slab (k_grid_dense)
+	uint32_t dense_off[kLevels + 1];             // prefix of the slab levels' entry counts
+	uint32_t used_off[kLevels + 1];              // prefix of the used entries (dense: (res + 1)^3, hashed: table)
+	float *dgrad_part;                           // [kGridParts][dense entries] float2 partial sums
+	float *wgrad_part;                           // [kWgradBlocks][4][64 x 64] partial weight gradients
 	double *loss;
 	int step;
 };
@@ -55,8 +59,10 @@ constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 constexpr int kMlp = YMX_GUIDE_MLP_WEIGHTS;
 constexpr int kWgradSamples = 64;  // samples per LDS stage of the weight-gradient GEMM
-constexpr uint32_t kSmallEntries = 6144;  // dense levels up to this many entries sum their gradient in LDS
-constexpr int kSmallChunks = 32;          // blocks per small level, each over a slice of the batch
+constexpr uint32_t kSlabEntries = 8192;  // lattice entries per block of the dense-level gradient (64 KiB of LDS)
+constexpr int kMaxSlabs = 96;
+constexpr int kGridParts = 8;              // slices of the batch per slab, summed by k_grid_reduce
+constexpr int kWgradBlocks = 128;          // slices of the batch of the weight-gradient GEMMs
 
 struct GArgs
 {
@@ -65,7 +71,7 @@ struct GArgs
 	float *grid_g, *mlp_g;
 	uint16_t *X, *H1, *H2, *H3, *dO, *dH3, *dH2, *dH1;
 	float *dXg;
-	uint32_t small_mask;
+	uint32_t slab_mask, dxg_stride;
 	double *loss;
 	uint32_t table_mask;
 	int32_t res[kLevels];
@@ -567,10 +573,10 @@ __global__ __launch_bounds__(kBlock) void k_backward(GArgs g, const ymx_guide_sa
 		{
 			const int lev = 8 * t + 2 * grp + q;
 			const float d0 = acc[t][2 * q], d1 = acc[t][2 * q + 1];
-			if((g.small_mask >> lev) & 1u)
+			if((g.slab_mask >> lev) & 1u)
 			{
-				// coarse dense level: summed per block in LDS by k_grid_small
-				*reinterpret_cast<float2 *>(g.dXg + (size_t)s * 32 + 2 * lev) = make_float2(d0, d1);
+				// dense level: summed per lattice slab in LDS by k_grid_dense
+				*reinterpret_cast<float2 *>(g.dXg + 2 * ((size_t)lev * g.dxg_stride + s)) = make_float2(d0, d1);
 				continue;
 			}
 			Corners c;
@@ -584,56 +590,108 @@ __global__ __launch_bounds__(kBlock) void k_backward(GArgs g, const ymx_guide_sa
 		}
 }
 
-// Grid gradient of the small dense levels: block (level k of the list,
-// chunk) adds the corners of its slice of the batch into an LDS copy of the
-// level, then adds the non-zero entries to the gradient with one global atomic
-// each (on the coarse levels every sample hits the same few entries, and
-// per-sample global atomics serialise).
-struct SmallLevels
+// Grid gradient of the dense levels without global atomics.  A slab is a run
+// of whole z-planes of one level (at most kSlabEntries entries); block (slab,
+// part) adds the corner contributions of its slice of the batch that fall in
+// the slab into LDS (corner weights and the sum order within a sample as in
+// level_corners) and stores the slab to its part of dgrad_part; k_grid_reduce
+// then sums the parts of every entry into the gradient.
+struct Slab
 {
-	int32_t lev[kLevels];
+	int32_t lev;
+	uint32_t z0, z1, off;  // planes [z0, z1); off: first entry in the dense entry space
+};
+struct SlabList
+{
+	Slab s[kMaxSlabs];
 	int32_t count;
 };
-__global__ __launch_bounds__(kBlock) void k_grid_small(GArgs g, SmallLevels sl, const ymx_guide_sample *smp,
-                                                        uint32_t cap, const uint32_t *n_dev)
+__global__ __launch_bounds__(kBlock) void k_grid_dense(GArgs g, SlabList sl, const ymx_guide_sample *smp,
+                                                        uint32_t cap, const uint32_t *n_dev, float *part_out,
+                                                        uint32_t dense_total)
 {
-	extern __shared__ float lds_grid[];
-	const int lev = sl.lev[blockIdx.x / kSmallChunks];
-	const uint32_t chunk = blockIdx.x % kSmallChunks;
-	const uint32_t r1 = (uint32_t)g.res[lev] + 1u;
-	const uint32_t entries = r1 * r1 * r1;  // <= kSmallEntries, checked on the host
-	for(uint32_t e = threadIdx.x; e < 2 * entries; e += kBlock) lds_grid[e] = 0.f;
+	extern __shared__ float lds_slab[];
+	const Slab sb = sl.s[blockIdx.x];
+	const uint32_t part = blockIdx.y;
+	const uint32_t r = (uint32_t)g.res[sb.lev], r1 = r + 1u, plane = r1 * r1;
+	const uint32_t entries = (sb.z1 - sb.z0) * plane;  // <= kSlabEntries, checked on the host
+	for(uint32_t e = threadIdx.x; e < 2 * entries; e += kBlock) lds_slab[e] = 0.f;
 	__syncthreads();
 	const uint32_t n = count_of(cap, n_dev);
-	const uint32_t per = (n + kSmallChunks - 1) / kSmallChunks;
-	const uint32_t s0 = chunk * per, s1 = s0 + per < n ? s0 + per : n;
+	const uint32_t per = (n + kGridParts - 1) / kGridParts;
+	const uint32_t s0 = part * per, s1 = s0 + per < n ? s0 + per : n;
+	const float2 *dxg = reinterpret_cast<const float2 *>(g.dXg) + (size_t)sb.lev * g.dxg_stride;
 	for(uint32_t s = s0 + threadIdx.x; s < s1; s += kBlock)
 	{
 		float p[3];
 		unit_pos(g, smp[s], p);
-		const float2 d = *reinterpret_cast<const float2 *>(g.dXg + (size_t)s * 32 + 2 * lev);
-		Corners c;
-		level_corners(g, lev, p, c);
+		uint32_t b[3];
+		float f[3];
+		for(int a = 0; a < 3; ++a)
+		{
+			const float sc = p[a] * (float)r;
+			const float fl = floorf(sc);
+			b[a] = (uint32_t)fl;
+			f[a] = sc - fl;
+		}
+		if(b[2] + 1u < sb.z0 || b[2] >= sb.z1) continue;
+		const float2 d = dxg[s];
 		for(int k = 0; k < 8; ++k)
 		{
-			atomicAdd(&lds_grid[2 * c.idx[k]], c.w[k] * d.x);
-			atomicAdd(&lds_grid[2 * c.idx[k] + 1], c.w[k] * d.y);
+			const uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = (k >> 2) & 1;
+			const uint32_t z = b[2] + dz < r ? b[2] + dz : r;
+			if(z < sb.z0 || z >= sb.z1) continue;
+			const uint32_t x = b[0] + dx < r ? b[0] + dx : r;
+			const uint32_t y = b[1] + dy < r ? b[1] + dy : r;
+			const float wx = dx ? f[0] : 1.f - f[0];
+			const float wy = dy ? f[1] : 1.f - f[1];
+			const float wz = dz ? f[2] : 1.f - f[2];
+			const float w = (wx * wy) * wz;
+			const uint32_t e = x + y * r1 + (z - sb.z0) * plane;
+			atomicAdd(&lds_slab[2 * e], w * d.x);
+			atomicAdd(&lds_slab[2 * e + 1], w * d.y);
 		}
 	}
 	__syncthreads();
-	float *lv = g.grid_g + (size_t)lev * (g.table_mask + 1u) * kFeat;
-	for(uint32_t e = threadIdx.x; e < 2 * entries; e += kBlock)
-	{
-		const float v = lds_grid[e];
-		if(v != 0.f) atomicAdd(lv + e, v);
-	}
+	float *dst = part_out + 2 * ((size_t)part * dense_total + sb.off);
+	for(uint32_t e = threadIdx.x; e < 2 * entries; e += kBlock) dst[e] = lds_slab[e];
+}
+
+struct LevelOffsets
+{
+	uint32_t off[kLevels + 1];  // prefix of per-level entry counts in a compact entry space
+	uint32_t mask;              // levels present in that space
+};
+__device__ __forceinline__ int level_of(const LevelOffsets &lo, uint32_t e)
+{
+	int l = 0;
+	while(l < kLevels - 1 && e >= lo.off[l + 1]) ++l;
+	return l;
+}
+
+__global__ void k_grid_reduce(const float *part, uint32_t dense_total, LevelOffsets lo, float *grid_g, uint32_t table)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= 2 * dense_total) return;
+	float acc = 0.f;
+	for(int k = 0; k < kGridParts; ++k) acc += part[(size_t)k * 2 * dense_total + i];
+	const uint32_t e = i >> 1;
+	const int l = level_of(lo, e);
+	grid_g[2 * ((size_t)l * table + (e - lo.off[l])) + (i & 1u)] = acc;
 }
 
 // dW (64 x 64 FP32) += sum over samples of dH[s][o] * H[s][i], split over
 // blocks along the samples.  Wave w computes output rows 16w .. 16w+15.
-__global__ __launch_bounds__(kBlock) void k_wgrad(const uint16_t *dH, const uint16_t *H, uint32_t cap,
-                                                   const uint32_t *n_dev, float *dW, int rows)
+struct WgradLayers
 {
+	const uint16_t *dH[4], *H[4];
+};
+__global__ __launch_bounds__(kBlock) void k_wgrad(WgradLayers wl, uint32_t cap, const uint32_t *n_dev, float *part)
+{
+	const int layer = blockIdx.y;
+	const uint16_t *dH = wl.dH[layer], *H = wl.H[layer];
+	const int rows = layer < 3 ? 64 : 48;
+	float *dW = part + ((size_t)blockIdx.x * 4 + layer) * 4096;
 	// [feature][sample], rows padded to 144 B (a multiple of 16 for 128-bit reads)
 	__shared__ __attribute__((aligned(16))) uint16_t sd[64][kWgradSamples + 8];
 	__shared__ __attribute__((aligned(16))) uint16_t sh[64][kWgradSamples + 8];
@@ -686,8 +744,18 @@ __global__ __launch_bounds__(kBlock) void k_wgrad(const uint16_t *dH, const uint
 		for(int i = 0; i < 4; ++i)
 		{
 			const int o = 16 * wave + 4 * grp + i;
-			if(o < rows) atomicAdd(dW + o * 64 + 16 * t + sl, acc[t][i]);
+			dW[o * 64 + 16 * t + sl] = o < rows ? acc[t][i] : 0.f;
 		}
+}
+
+// mlp_g[i] = sum over the blocks of k_wgrad of their partial gradients.
+__global__ void k_wgrad_reduce(const float *part, uint32_t blocks, float *mlp_g)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= 4 * 4096) return;
+	float acc = 0.f;
+	for(uint32_t b = 0; b < blocks; ++b) acc += part[(size_t)b * 4 * 4096 + i];
+	mlp_g[i] = acc;
 }
 
 // Adam on FP32 master weights (Kingma & Ba 2015, bias-corrected), then the
@@ -706,6 +774,28 @@ __global__ void k_adam(float *p, float *m, float *v, float *gr, uint16_t *pb, ui
 	p[i] = pi;
 	gr[i] = 0.f;
 	if(pb) pb[i] = f2bf(pi);
+}
+
+// Adam over the lattice entries in use (a dense level uses its first
+// (res + 1)^3 entries; an entry no sample reaches keeps a zero gradient and
+// zero moments, so Adam would leave it unchanged).
+__global__ void k_adam_grid(float *p, float *m, float *v, float *gr, uint16_t *pb, LevelOffsets lo, uint32_t table,
+                            float lr, float b1, float b2, float eps, float c1, float c2)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= 2 * lo.off[kLevels]) return;
+	const uint32_t e = i >> 1;
+	const int l = level_of(lo, e);
+	const size_t k = 2 * ((size_t)l * table + (e - lo.off[l])) + (i & 1u);
+	const float gi = gr[k];
+	const float mi = b1 * m[k] + (1.f - b1) * gi;
+	const float vi = b2 * v[k] + (1.f - b2) * gi * gi;
+	m[k] = mi;
+	v[k] = vi;
+	const float pi = p[k] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+	p[k] = pi;
+	gr[k] = 0.f;
+	pb[k] = f2bf(pi);
 }
 
 // BF16 weight layouts from the FP32 master: W (rows padded to 64) and W^T.
@@ -737,7 +827,8 @@ GArgs args_of(const ymx_guide *g)
 	a.dH2 = g->dH2;
 	a.dH1 = g->dH1;
 	a.dXg = g->dXg;
-	a.small_mask = g->small_mask;
+	a.slab_mask = g->slab_mask;
+	a.dxg_stride = g->p.max_batch;
 	a.loss = g->loss;
 	a.table_mask = g->table - 1u;
 	for(int l = 0; l < kLevels; ++l) a.res[l] = g->res[l];
@@ -753,7 +844,7 @@ void free_guide(ymx_guide *g)
 {
 	if(!g) return;
 	void *ptrs[] = {g->grid, g->grid_m, g->grid_v, g->grid_g, g->grid_bf, g->mlp, g->mlp_m, g->mlp_v, g->mlp_g,
-	                g->w_bf, g->wt_bf, g->X, g->H1, g->H2, g->H3, g->dO, g->dH3, g->dH2, g->dH1, g->dXg, g->loss};
+	                g->w_bf, g->wt_bf, g->X, g->H1, g->H2, g->H3, g->dO, g->dH3, g->dH2, g->dH1, g->dXg, g->dgrad_part, g->wgrad_part, g->loss};
 	for(void *p : ptrs)
 		if(p) (void)hipFree(p);
 	delete g;
@@ -835,13 +926,36 @@ int ymx_guide_create(const ymx_guide_params *p, ymx_guide **out)
 	G_TRY(galloc(g->wt_bf, 4 * 4096));
 	uint16_t **acts[] = {&g->X, &g->H1, &g->H2, &g->H3, &g->dO, &g->dH3, &g->dH2, &g->dH1};
 	for(uint16_t **a : acts) G_TRY(galloc(*a, (size_t)p->max_batch * 64));
-	g->small_mask = 0;
+	// Dense levels whose z-planes fit a slab, coarse first while the slab list
+	// has room, sum their gradient in LDS; the others use global atomics.
+	g->slab_mask = 0;
+	g->dense_off[0] = g->used_off[0] = 0;
+	int slabs = 0;
 	for(int l = 0; l < kLevels; ++l)
 	{
 		const uint64_t r1 = (uint64_t)g->res[l] + 1u;
-		if(r1 * r1 * r1 <= kSmallEntries && r1 * r1 * r1 <= (uint64_t)g->table) g->small_mask |= 1u << l;
+		const bool dense = r1 * r1 * r1 <= (uint64_t)g->table;
+		bool slab = false;
+		if(dense && r1 * r1 <= kSlabEntries)
+		{
+			const uint64_t per = kSlabEntries / (r1 * r1);
+			const int need = (int)((r1 + per - 1) / per);
+			if(slabs + need <= kMaxSlabs)
+			{
+				slab = true;
+				slabs += need;
+				g->slab_mask |= 1u << l;
+			}
+		}
+		g->dense_off[l + 1] = g->dense_off[l] + (slab ? (uint32_t)(r1 * r1 * r1) : 0u);
+		g->used_off[l + 1] = g->used_off[l] + (dense ? (uint32_t)(r1 * r1 * r1) : g->table);
 	}
-	if(g->small_mask) G_TRY(galloc(g->dXg, (size_t)p->max_batch * 32));
+	if(g->slab_mask)
+	{
+		G_TRY(galloc(g->dXg, (size_t)p->max_batch * 2 * kLevels));
+		G_TRY(galloc(g->dgrad_part, (size_t)kGridParts * 2 * g->dense_off[kLevels]));
+	}
+	G_TRY(galloc(g->wgrad_part, (size_t)kWgradBlocks * 4 * 4096));
 	G_TRY(galloc(g->loss, 1));
 	// initial weights: grid U(-1e-4, 1e-4), W U(-s, s) with s = sqrt(6 / fan_in)
 	std::mt19937 rng(p->seed);
@@ -951,41 +1065,64 @@ int ymx_guide_backward(ymx_guide *g, const ymx_guide_sample *samples, uint32_t n
 		return YMX_ERR_INVALID_ARG;
 	}
 	hipStream_t s = (hipStream_t)stream;
-	YMX_HIP_CHECK(hipMemsetAsync(g->grid_g, 0, sizeof(float) * g->n_grid, s));
-	YMX_HIP_CHECK(hipMemsetAsync(g->mlp_g, 0, sizeof(float) * kMlp, s));
+	// dense levels: k_grid_reduce stores their gradient; hashed ones are summed by atomics
+	for(int l = 0; l < kLevels; ++l)
+		if(!((g->slab_mask >> l) & 1u))
+			YMX_HIP_CHECK(hipMemsetAsync(g->grid_g + (size_t)l * g->table * kFeat, 0, sizeof(float) * g->table * kFeat, s));
 	YMX_HIP_CHECK(hipMemsetAsync(g->loss, 0, sizeof(double), s));
-	if(n == 0) return YMX_OK;
 	const GArgs a = args_of(g);
+	if(n == 0)
+	{
+		for(int l = 0; l < kLevels; ++l)
+			if((g->slab_mask >> l) & 1u)
+				YMX_HIP_CHECK(hipMemsetAsync(g->grid_g + (size_t)l * g->table * kFeat, 0,
+				                             sizeof(float) * (g->dense_off[l + 1] - g->dense_off[l]) * kFeat, s));
+		YMX_HIP_CHECK(hipMemsetAsync(g->mlp_g, 0, sizeof(float) * kMlp, s));
+		return YMX_OK;
+	}
 	const unsigned blocks = (n + 16 * kWaves - 1) / (16 * kWaves);
 	k_forward<<<blocks, kBlock, 0, s>>>(a, samples, n, n_dev);
 	YMX_HIP_CHECK(hipGetLastError());
 	k_backward<<<blocks, kBlock, 0, s>>>(a, samples, n, n_dev);
 	YMX_HIP_CHECK(hipGetLastError());
-	if(g->small_mask)
+	if(g->slab_mask)
 	{
-		SmallLevels sl;
+		SlabList sl;
 		sl.count = 0;
-		uint32_t max_entries = 0;
 		for(int l = 0; l < kLevels; ++l)
-			if((g->small_mask >> l) & 1u)
+		{
+			if(!((g->slab_mask >> l) & 1u)) continue;
+			const uint32_t r1 = (uint32_t)g->res[l] + 1u, plane = r1 * r1;
+			const uint32_t per = kSlabEntries / plane;  // planes per slab (>= 1 for a slab level)
+			for(uint32_t z0 = 0; z0 < r1; z0 += per)
 			{
-				sl.lev[sl.count++] = l;
-				const uint32_t r1 = (uint32_t)g->res[l] + 1u;
-				max_entries = r1 * r1 * r1 > max_entries ? r1 * r1 * r1 : max_entries;
+				if(sl.count == kMaxSlabs)
+				{
+					ymx_set_error("ymx_guide_backward: more than %d lattice slabs", kMaxSlabs);
+					return YMX_ERR_UNSUPPORTED;
+				}
+				Slab &sb = sl.s[sl.count++];
+				sb.lev = l;
+				sb.z0 = z0;
+				sb.z1 = z0 + per < r1 ? z0 + per : r1;
+				sb.off = g->dense_off[l] + z0 * plane;
 			}
-		k_grid_small<<<sl.count * kSmallChunks, kBlock, 2 * sizeof(float) * max_entries, s>>>(a, sl, samples, n, n_dev);
+		}
+		const uint32_t dense_total = g->dense_off[kLevels];
+		k_grid_dense<<<dim3(sl.count, kGridParts), kBlock, 2 * sizeof(float) * kSlabEntries, s>>>(
+			a, sl, samples, n, n_dev, g->dgrad_part, dense_total);
+		YMX_HIP_CHECK(hipGetLastError());
+		LevelOffsets lo;
+		for(int l = 0; l <= kLevels; ++l) lo.off[l] = g->dense_off[l];
+		lo.mask = g->slab_mask;
+		k_grid_reduce<<<(2 * dense_total + 255) / 256, 256, 0, s>>>(g->dgrad_part, dense_total, lo, g->grid_g, g->table);
 		YMX_HIP_CHECK(hipGetLastError());
 	}
-	const uint16_t *dh[4] = {g->dH1, g->dH2, g->dH3, g->dO};
-	const uint16_t *hh[4] = {g->X, g->H1, g->H2, g->H3};
-	unsigned wblocks = (n + 4095) / 4096;
-	if(wblocks < 1) wblocks = 1;
-	if(wblocks > 512) wblocks = 512;
-	for(int l = 0; l < 4; ++l)
-	{
-		k_wgrad<<<wblocks, kBlock, 0, s>>>(dh[l], hh[l], n, n_dev, g->mlp_g + l * 4096, l < 3 ? 64 : 48);
-		YMX_HIP_CHECK(hipGetLastError());
-	}
+	WgradLayers wl = {{g->dH1, g->dH2, g->dH3, g->dO}, {g->X, g->H1, g->H2, g->H3}};
+	k_wgrad<<<dim3(kWgradBlocks, 4), kBlock, 0, s>>>(wl, n, n_dev, g->wgrad_part);
+	YMX_HIP_CHECK(hipGetLastError());
+	k_wgrad_reduce<<<(4 * 4096 + 255) / 256, 256, 0, s>>>(g->wgrad_part, kWgradBlocks, g->mlp_g);
+	YMX_HIP_CHECK(hipGetLastError());
 	return YMX_OK;
 }
 
@@ -1014,8 +1151,12 @@ int ymx_guide_adam_step(ymx_guide *g, void *stream)
 	g->step += 1;
 	const float c1 = 1.f - powf(g->p.beta1, (float)g->step);
 	const float c2 = 1.f - powf(g->p.beta2, (float)g->step);
-	k_adam<<<(g->n_grid + 255) / 256, 256, 0, s>>>(g->grid, g->grid_m, g->grid_v, g->grid_g, g->grid_bf, g->n_grid,
-	                                              g->p.lr, g->p.beta1, g->p.beta2, g->p.eps, c1, c2);
+	LevelOffsets lo;
+	for(int l = 0; l <= kLevels; ++l) lo.off[l] = g->used_off[l];
+	lo.mask = (1u << kLevels) - 1u;
+	k_adam_grid<<<(2 * g->used_off[kLevels] + 255) / 256, 256, 0, s>>>(g->grid, g->grid_m, g->grid_v, g->grid_g,
+	                                                                   g->grid_bf, lo, g->table, g->p.lr, g->p.beta1,
+	                                                                   g->p.beta2, g->p.eps, c1, c2);
 	YMX_HIP_CHECK(hipGetLastError());
 	k_adam<<<(kMlp + 255) / 256, 256, 0, s>>>(g->mlp, g->mlp_m, g->mlp_v, g->mlp_g, nullptr, kMlp, g->p.lr,
 	                                         g->p.beta1, g->p.beta2, g->p.eps, c1, c2);
