@@ -53,7 +53,14 @@ def parse():
     p.add_argument("--guided-only", action="store_true", help="only the guided path-tracing measurement")
     p.add_argument("--guided-res", type=str, default="1280x720")
     p.add_argument("--guided-ref-spp", type=int, default=4096)
-    p.add_argument("--guided-spp", type=int, default=64)
+    p.add_argument("--guided-spp", type=int, default=256)
+    p.add_argument("--guided-bsdf-prob", type=float, default=0.5)
+    p.add_argument("--guided-steps", type=int, default=16, help="Adam steps after each training pass")
+    p.add_argument("--guided-lr", type=float, default=1e-2)
+    p.add_argument("--guided-depth", type=int, default=GUIDE_DEPTH,
+                   help="path vertices after the camera vertex that are guided (-1: all)")
+    p.add_argument("--guided-train-passes", type=int, default=8,
+                   help="training passes over the first 32 spp (32 / passes spp each)")
     return p.parse_args()
 
 
@@ -100,6 +107,8 @@ def measure_path_tracer(args, torch):
 
 GUIDE_N_MIN, GUIDE_N_MAX = 4, 32   # hash-grid resolutions of the guided occluded-room run
 GUIDE_TRAIN_SPP = 32               # the guide trains on the first 32 spp of the render
+GUIDE_DEPTH = 1                    # guided: the camera vertex and the next path vertex
+_GUIDING_REFERENCE = {}            # (resx, resy, spp) -> (reference image, its own relMSE)
 
 
 def measure_guiding(args, torch):
@@ -114,8 +123,10 @@ def measure_guiding(args, torch):
 
     Guided: the guide (16-level hash grid of 2^19 entries x 2 features, 3 x 64
     MLP, 8 vMF lobes, BF16 MFMA, Adam lr 1e-2) trains online on the first 32
-    spp of the render (32 passes of 1 spp, 4 Adam steps after each), then the
-    render continues with the trained guide up to `guided_spp`.  Every sample,
+    spp of the render (8 passes of 4 spp, 16 Adam steps after each), then the
+    render continues with the trained guide up to `guided_spp`.  The camera
+    vertex and the next path vertex are guided (deeper vertices add cost and
+    no measurable variance reduction on this scene).  Every sample,
     training passes included, is a sample of the image (all are unbiased);
     guided sampling mixes the BSDF sampler (probability bsdf_prob) and the
     mixture by one-sample MIS.  With v_t and v_g the per-sample error of the
@@ -148,16 +159,18 @@ def measure_guiding(args, torch):
     def relmse(a, b):
         return float(((a - b) ** 2 / (b.astype(np.float64) ** 2 + 1e-2)).mean())
 
-    pt = tracer(ref_spp // 2, 777)
-    pt.clear()
-    pt.render(0, ref_spp // 2)
-    r1 = image(pt)
-    pt.clear()
-    pt.render(ref_spp // 2, ref_spp // 2)
-    r2 = image(pt)
-    pt.close()
-    ref = (r1 + r2) / 2
-    ref_err = relmse(r1, r2) / 4
+    key = (resx, resy, ref_spp)
+    if key not in _GUIDING_REFERENCE:
+        pt = tracer(ref_spp // 2, 777)
+        pt.clear()
+        pt.render(0, ref_spp // 2)
+        r1 = image(pt)
+        pt.clear()
+        pt.render(ref_spp // 2, ref_spp // 2)
+        r2 = image(pt)
+        pt.close()
+        _GUIDING_REFERENCE[key] = ((r1 + r2) / 2, relmse(r1, r2) / 4)
+    ref, ref_err = _GUIDING_REFERENCE[key]
 
     def timed(fn):
         torch.cuda.synchronize()
@@ -173,15 +186,21 @@ def measure_guiding(args, torch):
     t_plain = timed(lambda: p.render(0, spp))
     e_plain = relmse(image(p), ref) - ref_err
     bmin, bmax = sc["verts"].min(0) - 1e-3, sc["verts"].max(0) + 1e-3
-    alpha, steps = 0.5, 4
-    g = GpuGuide(bmin, bmax, log2_table_size=19, n_min=GUIDE_N_MIN, n_max=GUIDE_N_MAX, max_batch=1 << 16,
-                 lr=1e-2, seed=1)
+    alpha, steps = args.guided_bsdf_prob, args.guided_steps
+    n_min, n_max = getattr(args, "guided_n_min", GUIDE_N_MIN), getattr(args, "guided_n_max", GUIDE_N_MAX)
+    g = GpuGuide(bmin, bmax, log2_table_size=19, n_min=n_min, n_max=n_max, max_batch=1 << 16,
+                 lr=args.guided_lr, seed=1)
+    depth = getattr(args, "guided_depth", GUIDE_DEPTH)
+    passes = getattr(args, "guided_train_passes", 8)
+    if passes < 1 or n0 % passes:
+        raise SystemExit(f"--guided-train-passes must divide {n0}")
+    p.set_guide_depth(None if depth < 0 else depth)
     p.set_guide(g, bsdf_prob=alpha, train=True, train_steps=steps)
     p.clear()
 
     def train():
-        for k in range(n0):
-            p.render(k, 1)
+        for k in range(passes):
+            p.render(k * (n0 // passes), n0 // passes)
 
     t_train = timed(train)
     rgba_t, w_t = p.film()
@@ -207,9 +226,10 @@ def measure_guiding(args, torch):
            "plain_ms": round(t_plain * 1e3, 1),
            "guided_ms": round((t_train + t_rest) * 1e3, 1),
            "train_ms": round(t_train * 1e3, 1),
-           "train": f"first {n0} spp of the render, 1 spp per pass, {steps} Adam steps after each",
+           "train": f"first {n0} spp of the render, {passes} passes of {n0 // passes} spp, {steps} Adam steps after each",
+           "guided_vertices": "all" if depth < 0 else f"camera vertex and the next {depth}",
            "post_train_ms_per_spp": round(t_rest / m * 1e3, 2), "plain_ms_per_spp": round(t_plain / spp * 1e3, 2),
-           "bsdf_prob": alpha, "grid_levels": f"16 (resolution {GUIDE_N_MIN}..{GUIDE_N_MAX})",
+           "bsdf_prob": alpha, "lr": args.guided_lr, "grid_levels": f"16 (resolution {n_min}..{n_max})",
            "relmse_plain": e_plain, "relmse_guided": e_all,
            "relmse_training_part": e_train, "relmse_post_training_part": e_rest,
            "equal_spp_gain": round(e_plain / e_all, 3) if e_all > 0 else None,

@@ -150,6 +150,12 @@ int ymx_pt_read_film(ymx_pt *pt, float *rgba, float *weight);
  * guide must outlive its use by this handle. */
 int ymx_pt_set_guide(ymx_pt *pt, ymx_guide *guide, float bsdf_prob, int32_t train, uint32_t train_steps);
 
+/* Limit guiding to the sub-path's start vertex and its next max_depth path
+ * vertices (deeper vertices sample the BSDF alone and cost no guide query);
+ * max_depth < 0: every vertex (the default).  No counterpart in the
+ * reference.  Applies to the guide set now and to later ones. */
+int ymx_pt_set_guide_depth(ymx_pt *pt, int32_t max_depth);
+
 /* Counters of the last ymx_pt_render: rays cast, and its duration in ms on
  * the stream (events around the whole render). */
 typedef struct ymx_pt_stats

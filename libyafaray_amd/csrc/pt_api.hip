@@ -313,6 +313,7 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 		return YMX_ERR_OUT_OF_MEMORY;
 	}
 	memset(pt, 0, sizeof *pt);
+	pt->guide_depth = -1;
 	pt->accel = accel;
 	pt->n_tris = n_tris;
 	pt->n_materials = n_materials;
@@ -594,6 +595,18 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 	return YMX_OK;
 }
 
+int ymx_pt_set_guide_depth(ymx_pt *pt, int32_t max_depth)
+{
+	if(!pt)
+	{
+		ymx_set_error("ymx_pt_set_guide_depth: NULL handle");
+		return YMX_ERR_INVALID_ARG;
+	}
+	pt->guide_depth = max_depth < 0 ? -1 : max_depth;
+	pt->g.max_depth = max_depth < 0 ? INT32_MAX : max_depth;
+	return YMX_OK;
+}
+
 int ymx_pt_set_guide(ymx_pt *pt, ymx_guide *guide, float bsdf_prob, int32_t train, uint32_t train_steps)
 {
 	if(!pt)
@@ -629,6 +642,7 @@ int ymx_pt_set_guide(ymx_pt *pt, ymx_guide *guide, float bsdf_prob, int32_t trai
 	GB_TRY(dalloc(g.mix_path, (size_t)cap * YMX_GUIDE_MIX_FLOATS));
 	GB_TRY(dalloc(g.query_count, 1));
 	g.bsdf_prob = bsdf_prob < 0.2f ? 0.2f : (bsdf_prob > 1.f ? 1.f : bsdf_prob);
+	g.max_depth = pt->guide_depth < 0 ? INT32_MAX : pt->guide_depth;
 	g.on = 1u;
 	if(train)
 	{

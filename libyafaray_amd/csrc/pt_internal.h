@@ -150,6 +150,7 @@ struct PtGuide
 	uint32_t on;               // guided sampling at shinydiffuse vertices
 	uint32_t train;            // record training samples
 	float bsdf_prob;           // one-sample MIS: probability of sampling the BSDF
+	int32_t max_depth;         // path vertices past the sub-path start that are guided (segments d < max_depth)
 	float accept;              // probability that a finished record is kept for training
 	ymx_guide_sample *query;   // [capacity] queries of one inference batch
 	float *mix_node;           // [capacity][48]
@@ -192,6 +193,7 @@ struct ymx_pt
 	ymx::PtGuide g;
 	uint32_t guide_cap;              // slots the guiding buffers hold
 	uint32_t train_steps;            // guide training steps after each wavefront
+	int32_t guide_depth;             // ymx_pt_set_guide_depth (-1: every vertex)
 	uint32_t g_seen, g_filled;       // training samples written so far (counter value), valid in the ring
 	uint64_t g_rng;                  // mini-batch offsets
 	uint32_t batch;                  // AA samples per wavefront

@@ -1494,7 +1494,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_path(PtArgs a)
 		float w = tpn.w;
 		V3 ndir = dir;
 		Rgb scol;
-		const bool guided = guided_material(a, sp);
+		const bool guided = d < a.g.max_depth && guided_material(a, sp);
 		if(guided)
 		{
 			const uint32_t ps = x.sample * (uint32_t)c.path_samples + (uint32_t)a.sub;
@@ -1838,7 +1838,7 @@ int trace_chunk(ymx_pt *pt, PtArgs &a, hipStream_t stream)
 			rc = ymx_launch_closest_queue(pt->accel, pt->st.ext_rays[qsel], pt->st.ext_hits, pt->st.ext_count + qsel, n,
 			                              stream);
 			if(rc != YMX_OK) return rc;
-			if(a.g.on && d + 1 < hc.bounces)
+			if(a.g.on && d + 1 < hc.bounces && d < a.g.max_depth)
 			{
 				k_guide_query_path<<<g, kBlock, 0, stream>>>(a);
 				YMX_HIP_CHECK(hipGetLastError());

@@ -177,6 +177,12 @@ class GpuPathTracer:
               "ymx_pt_set_guide")
         self._guide = guide  # keep it alive while set
 
+    def set_guide_depth(self, max_depth: int | None):
+        """Guide only the sub-path start vertex and the next `max_depth` path
+        vertices (None: every vertex)."""
+        check(self._lib.ymx_pt_set_guide_depth(self._h, -1 if max_depth is None else int(max_depth)),
+              "ymx_pt_set_guide_depth")
+
     def render(self, first_sample: int = 0, n_samples: int | None = None):
         n = self.st.aa_samples if n_samples is None else n_samples
         check(self._lib.ymx_pt_render(self._h, int(first_sample), int(n), self._stream()), "ymx_pt_render")
