@@ -61,7 +61,9 @@ constexpr int kMlp = YMX_GUIDE_MLP_WEIGHTS;
 constexpr int kWgradSamples = 64;  // samples per LDS stage of the weight-gradient GEMM
 constexpr uint32_t kSlabEntries = 8192;  // lattice entries per block of the dense-level gradient (64 KiB of LDS)
 constexpr int kMaxSlabs = 96;
-constexpr int kGridParts = 8;              // slices of the batch per slab, summed by k_grid_reduce
+constexpr int kGridParts = 16;             // slices of the batch per slab, summed by k_grid_reduce
+constexpr int kGridBlock = 1024;           // lanes of a k_grid_dense block (2 blocks of 64 KiB LDS per CU)
+constexpr int kWgradSlices = 8;            // k_wgrad_reduce: slices of the partials, summed by atomics
 constexpr int kWgradBlocks = 128;          // slices of the batch of the weight-gradient GEMMs
 
 struct GArgs
@@ -606,7 +608,7 @@ struct SlabList
 	Slab s[kMaxSlabs];
 	int32_t count;
 };
-__global__ __launch_bounds__(kBlock) void k_grid_dense(GArgs g, SlabList sl, const ymx_guide_sample *smp,
+__global__ __launch_bounds__(kGridBlock) void k_grid_dense(GArgs g, SlabList sl, const ymx_guide_sample *smp,
                                                         uint32_t cap, const uint32_t *n_dev, float *part_out,
                                                         uint32_t dense_total)
 {
@@ -615,13 +617,13 @@ __global__ __launch_bounds__(kBlock) void k_grid_dense(GArgs g, SlabList sl, con
 	const uint32_t part = blockIdx.y;
 	const uint32_t r = (uint32_t)g.res[sb.lev], r1 = r + 1u, plane = r1 * r1;
 	const uint32_t entries = (sb.z1 - sb.z0) * plane;  // <= kSlabEntries, checked on the host
-	for(uint32_t e = threadIdx.x; e < 2 * entries; e += kBlock) lds_slab[e] = 0.f;
+	for(uint32_t e = threadIdx.x; e < 2 * entries; e += kGridBlock) lds_slab[e] = 0.f;
 	__syncthreads();
 	const uint32_t n = count_of(cap, n_dev);
 	const uint32_t per = (n + kGridParts - 1) / kGridParts;
 	const uint32_t s0 = part * per, s1 = s0 + per < n ? s0 + per : n;
 	const float2 *dxg = reinterpret_cast<const float2 *>(g.dXg) + (size_t)sb.lev * g.dxg_stride;
-	for(uint32_t s = s0 + threadIdx.x; s < s1; s += kBlock)
+	for(uint32_t s = s0 + threadIdx.x; s < s1; s += kGridBlock)
 	{
 		float p[3];
 		unit_pos(g, smp[s], p);
@@ -654,7 +656,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_dense(GArgs g, SlabList sl, con
 	}
 	__syncthreads();
 	float *dst = part_out + 2 * ((size_t)part * dense_total + sb.off);
-	for(uint32_t e = threadIdx.x; e < 2 * entries; e += kBlock) dst[e] = lds_slab[e];
+	for(uint32_t e = threadIdx.x; e < 2 * entries; e += kGridBlock) dst[e] = lds_slab[e];
 }
 
 struct LevelOffsets
@@ -748,14 +750,17 @@ __global__ __launch_bounds__(kBlock) void k_wgrad(WgradLayers wl, uint32_t cap, 
 		}
 }
 
-// mlp_g[i] = sum over the blocks of k_wgrad of their partial gradients.
+// mlp_g[i] += sum over a slice (blockIdx.y) of the k_wgrad blocks' partial
+// gradients; mlp_g is cleared before.
 __global__ void k_wgrad_reduce(const float *part, uint32_t blocks, float *mlp_g)
 {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if(i >= 4 * 4096) return;
+	const uint32_t per = (blocks + kWgradSlices - 1) / kWgradSlices;
+	const uint32_t b0 = blockIdx.y * per, b1 = b0 + per < blocks ? b0 + per : blocks;
 	float acc = 0.f;
-	for(uint32_t b = 0; b < blocks; ++b) acc += part[(size_t)b * 4 * 4096 + i];
-	mlp_g[i] = acc;
+	for(uint32_t b = b0; b < b1; ++b) acc += part[(size_t)b * 4 * 4096 + i];
+	atomicAdd(mlp_g + i, acc);
 }
 
 // Adam on FP32 master weights (Kingma & Ba 2015, bias-corrected), then the
@@ -1109,7 +1114,7 @@ int ymx_guide_backward(ymx_guide *g, const ymx_guide_sample *samples, uint32_t n
 			}
 		}
 		const uint32_t dense_total = g->dense_off[kLevels];
-		k_grid_dense<<<dim3(sl.count, kGridParts), kBlock, 2 * sizeof(float) * kSlabEntries, s>>>(
+		k_grid_dense<<<dim3(sl.count, kGridParts), kGridBlock, 2 * sizeof(float) * kSlabEntries, s>>>(
 			a, sl, samples, n, n_dev, g->dgrad_part, dense_total);
 		YMX_HIP_CHECK(hipGetLastError());
 		LevelOffsets lo;
@@ -1121,7 +1126,8 @@ int ymx_guide_backward(ymx_guide *g, const ymx_guide_sample *samples, uint32_t n
 	WgradLayers wl = {{g->dH1, g->dH2, g->dH3, g->dO}, {g->X, g->H1, g->H2, g->H3}};
 	k_wgrad<<<dim3(kWgradBlocks, 4), kBlock, 0, s>>>(wl, n, n_dev, g->wgrad_part);
 	YMX_HIP_CHECK(hipGetLastError());
-	k_wgrad_reduce<<<(4 * 4096 + 255) / 256, 256, 0, s>>>(g->wgrad_part, kWgradBlocks, g->mlp_g);
+	YMX_HIP_CHECK(hipMemsetAsync(g->mlp_g, 0, sizeof(float) * kMlp, s));
+	k_wgrad_reduce<<<dim3((4 * 4096 + 255) / 256, kWgradSlices), 256, 0, s>>>(g->wgrad_part, kWgradBlocks, g->mlp_g);
 	YMX_HIP_CHECK(hipGetLastError());
 	return YMX_OK;
 }
