@@ -137,6 +137,26 @@ int ymx_pt_render(ymx_pt *pt, int32_t first_sample, int32_t n_samples, void *str
  * filter weights; the pixel value is rgba / weight (ImageFilm::flush). */
 int ymx_pt_read_film(ymx_pt *pt, float *rgba, float *weight);
 
+/* Save the film as ImageFilm::imageFilmSave writes it
+ * (src/render/imagefilm.cc:1099-1176): the NUL-terminated tag
+ * "YAF_FILMv4_0_0", ten int32 (computer node, base sampling offset, sampling
+ * offset, width, height, x0, x1, y0, y1, image layers = 1), the filter
+ * weights row by row, then the combined layer's weighted RGBA sums row by
+ * row, all in host byte order.  The film covers the whole image (x0 = y0 =
+ * 0).  Synchronous. */
+int ymx_pt_save_film(ymx_pt *pt, const char *path, int32_t computer_node, int32_t base_sampling_offset,
+                     int32_t sampling_offset);
+
+/* Load a film written in that format (ImageFilm::imageFilmLoad,
+ * src/render/imagefilm.cc:910-1018): the tag and the size, borders and layer
+ * count must match this handle's film (YMX_ERR_INVALID_ARG otherwise, the
+ * film unchanged).  add = 0 replaces the film; add != 0 adds the loaded sums
+ * to it, as imageFilmLoadAllInFolder merges the films of several render
+ * nodes (:1072-1094).  The three offsets read are returned where the pointers
+ * are not NULL.  Synchronous. */
+int ymx_pt_load_film(ymx_pt *pt, const char *path, int32_t add, int32_t *computer_node,
+                     int32_t *base_sampling_offset, int32_t *sampling_offset);
+
 /* Path guiding.  No counterpart in the reference, whose path tracer samples
  * the BSDF only (src/integrator/surface/integrator_path_tracer.cc:190-240).
  * With a guide (include/ymx_guide.h) set, the next direction at shinydiffuse

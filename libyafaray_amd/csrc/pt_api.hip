@@ -3,6 +3,7 @@
 #include "pt_internal.h"
 
 #include <float.h>
+#include <stdio.h>
 #include <math.h>
 #include <string.h>
 
@@ -731,6 +732,88 @@ int ymx_pt_read_film(ymx_pt *pt, float *rgba, float *weight)
 	const uint32_t n = (uint32_t)(pt->host_const.resx * pt->host_const.resy);  // film pixels
 	YMX_HIP_CHECK(hipMemcpy(rgba, pt->d_film, sizeof(float4) * n, hipMemcpyDeviceToHost));
 	YMX_HIP_CHECK(hipMemcpy(weight, pt->d_film_w, sizeof(float) * n, hipMemcpyDeviceToHost));
+	return YMX_OK;
+}
+
+int ymx_pt_save_film(ymx_pt *pt, const char *path, int32_t computer_node, int32_t base_sampling_offset,
+                     int32_t sampling_offset)
+{
+	if(!pt || !path)
+	{
+		ymx_set_error("ymx_pt_save_film: NULL argument");
+		return YMX_ERR_INVALID_ARG;
+	}
+	const int32_t w = pt->host_const.resx, h = pt->host_const.resy;
+	const size_t n = (size_t)w * (size_t)h;
+	std::vector<float> rgba(4 * n), weight(n);
+	const int rc = ymx_pt_read_film(pt, rgba.data(), weight.data());
+	if(rc != YMX_OK) return rc;
+	FILE *f = fopen(path, "wb");
+	if(!f)
+	{
+		ymx_set_error("ymx_pt_save_film: cannot open '%s' for writing", path);
+		return YMX_ERR_INVALID_ARG;
+	}
+	static const char kTag[] = "YAF_FILMv4_0_0";  // written with its NUL
+	const int32_t head[10] = {computer_node, base_sampling_offset, sampling_offset, w, h, 0, w - 1, 0, h - 1, 1};
+	bool ok = fwrite(kTag, 1, sizeof kTag, f) == sizeof kTag;
+	ok = ok && fwrite(head, sizeof(int32_t), 10, f) == 10;
+	ok = ok && fwrite(weight.data(), sizeof(float), n, f) == n;
+	ok = ok && fwrite(rgba.data(), sizeof(float), 4 * n, f) == 4 * n;
+	ok = (fclose(f) == 0) && ok;
+	if(!ok)
+	{
+		ymx_set_error("ymx_pt_save_film: write to '%s' failed", path);
+		return YMX_ERR_HIP;
+	}
+	return YMX_OK;
+}
+
+int ymx_pt_load_film(ymx_pt *pt, const char *path, int32_t add, int32_t *computer_node,
+                     int32_t *base_sampling_offset, int32_t *sampling_offset)
+{
+	if(!pt || !path)
+	{
+		ymx_set_error("ymx_pt_load_film: NULL argument");
+		return YMX_ERR_INVALID_ARG;
+	}
+	const int32_t w = pt->host_const.resx, h = pt->host_const.resy;
+	const size_t n = (size_t)w * (size_t)h;
+	FILE *f = fopen(path, "rb");
+	if(!f)
+	{
+		ymx_set_error("ymx_pt_load_film: cannot open '%s'", path);
+		return YMX_ERR_INVALID_ARG;
+	}
+	static const char kTag[] = "YAF_FILMv4_0_0";
+	char tag[sizeof kTag];
+	int32_t head[10];
+	std::vector<float> rgba(4 * n), weight(n);
+	bool ok = fread(tag, 1, sizeof tag, f) == sizeof tag && memcmp(tag, kTag, sizeof kTag) == 0;
+	ok = ok && fread(head, sizeof(int32_t), 10, f) == 10;
+	const bool shape = ok && head[3] == w && head[4] == h && head[5] == 0 && head[6] == w - 1 && head[7] == 0 &&
+	                   head[8] == h - 1 && head[9] == 1;
+	ok = shape && fread(weight.data(), sizeof(float), n, f) == n && fread(rgba.data(), sizeof(float), 4 * n, f) == 4 * n;
+	fclose(f);
+	if(!ok)
+	{
+		ymx_set_error("ymx_pt_load_film: '%s' is not a %dx%d single-layer YAF_FILMv4_0_0 film", path, w, h);
+		return YMX_ERR_INVALID_ARG;
+	}
+	if(add)
+	{
+		std::vector<float> cur_rgba(4 * n), cur_w(n);
+		const int rc = ymx_pt_read_film(pt, cur_rgba.data(), cur_w.data());
+		if(rc != YMX_OK) return rc;
+		for(size_t i = 0; i < n; ++i) weight[i] = cur_w[i] + weight[i];
+		for(size_t i = 0; i < 4 * n; ++i) rgba[i] = cur_rgba[i] + rgba[i];
+	}
+	YMX_HIP_CHECK(hipDeviceSynchronize());
+	YMX_HIP_CHECK(hipMemcpy(pt->d_film, rgba.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+	YMX_HIP_CHECK(hipMemcpy(pt->d_film_w, weight.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+	if(computer_node) *computer_node = head[0];
+	if(base_sampling_offset) *base_sampling_offset = head[1];
+	if(sampling_offset) *sampling_offset = head[2];
 	return YMX_OK;
 }
 

@@ -62,6 +62,9 @@ SIGNATURES = {
     "ymx_pt_get_stats": (C.c_int, [C.c_void_p, C.POINTER(YmxPtStats)]),
     "ymx_pt_set_guide": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_int32, C.c_uint32]),
     "ymx_pt_set_guide_depth": (C.c_int, [C.c_void_p, C.c_int32]),
+    "ymx_pt_save_film": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32, C.c_int32]),
+    "ymx_pt_load_film": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_int32)]),
     "ymx_accel_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
                                    C.POINTER(YmxAccelParams), C.POINTER(C.c_void_p)]),
     "ymx_accel_destroy": (C.c_int, [C.c_void_p]),

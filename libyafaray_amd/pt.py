@@ -177,6 +177,19 @@ class GpuPathTracer:
               "ymx_pt_set_guide")
         self._guide = guide  # keep it alive while set
 
+    def save_film(self, path, computer_node: int = 0, base_sampling_offset: int = 0, sampling_offset: int = 0):
+        """Write the film in ImageFilm's YAF_FILMv4_0_0 format."""
+        check(self._lib.ymx_pt_save_film(self._h, str(path).encode(), int(computer_node), int(base_sampling_offset),
+                                         int(sampling_offset)), "ymx_pt_save_film")
+
+    def load_film(self, path, add: bool = False) -> dict:
+        """Read a YAF_FILMv4_0_0 film into this tracer's film (replace, or add
+        its sums); returns the offsets stored in the file."""
+        node, base, off = C.c_int32(), C.c_int32(), C.c_int32()
+        check(self._lib.ymx_pt_load_film(self._h, str(path).encode(), int(bool(add)), C.byref(node), C.byref(base),
+                                         C.byref(off)), "ymx_pt_load_film")
+        return {"computer_node": node.value, "base_sampling_offset": base.value, "sampling_offset": off.value}
+
     def set_guide_depth(self, max_depth: int | None):
         """Guide only the sub-path start vertex and the next `max_depth` path
         vertices (None: every vertex)."""

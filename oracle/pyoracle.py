@@ -508,3 +508,58 @@ class ObjectLight:
                                                    self.vis.ctypes.data, C.addressof(self.l), frm.ctypes.data,
                                                    d.ctypes.data, float(tmin), C.byref(ipdf))
         return hit, float(ipdf.value)
+
+
+# ---------------------------------------------------------------- film file
+
+FILM_TAG = b"YAF_FILMv4_0_0"
+
+
+def write_film(path, rgba, weight, computer_node=0, base_sampling_offset=0, sampling_offset=0):
+    """ImageFilm::imageFilmSave (src/render/imagefilm.cc:1099-1176) for a film
+    of one image layer covering the whole image: the tag through
+    File::append(std::string), which writes its NUL (src/common/file.cc:187-191);
+    the node, the two sampling offsets, width, height, x0, x1 = x0 + w - 1,
+    y0, y1 = y0 + h - 1 and the layer count as int; the weights for y, then x;
+    each layer's r, g, b, a for y, then x.  File::append<T> writes host-order
+    bytes (include/common/file.h:90-94)."""
+    import struct
+    weight = np.asarray(weight, np.float32)
+    rgba = np.asarray(rgba, np.float32)
+    h, w = weight.shape
+    with open(path, "wb") as f:
+        f.write(FILM_TAG + b"\0")
+        f.write(struct.pack("=10i", computer_node, base_sampling_offset, sampling_offset, w, h, 0, w - 1, 0, h - 1, 1))
+        for y in range(h):
+            for x in range(w):
+                f.write(struct.pack("=f", float(weight[y, x])))
+        for y in range(h):
+            for x in range(w):
+                f.write(struct.pack("=4f", *(float(v) for v in rgba[y, x, :4])))
+
+
+def read_film(path, w, h):
+    """ImageFilm::imageFilmLoad (src/render/imagefilm.cc:910-1018) for a
+    one-layer film of w x h: None when the tag, size, borders or layer count
+    differ, else (rgba [h][w][4], weight [h][w], (node, base offset, offset))."""
+    import struct
+    data = Path(path).read_bytes()
+    end = data.find(b"\0")
+    if end < 0 or data[:end] != FILM_TAG:
+        return None
+    pos = end + 1
+    head = struct.unpack_from("=10i", data, pos)
+    pos += 40
+    if head[3:] != (w, h, 0, w - 1, 0, h - 1, 1):
+        return None
+    weight = np.empty((h, w), np.float32)
+    for y in range(h):
+        for x in range(w):
+            weight[y, x] = struct.unpack_from("=f", data, pos)[0]
+            pos += 4
+    rgba = np.empty((h, w, 4), np.float32)
+    for y in range(h):
+        for x in range(w):
+            rgba[y, x] = struct.unpack_from("=4f", data, pos)
+            pos += 16
+    return rgba, weight, head[:3]

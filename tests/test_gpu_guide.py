@@ -141,13 +141,15 @@ def test_training_learns_a_position_dependent_lobe():
 
 
 # ---------------------------------------------------------------- guided path tracing
-def _render(sc, spp, guide=None, train_passes=0, seed=0, seeds=(0,)):
+def _render(sc, spp, guide=None, train_passes=0, seed=0, seeds=(0,), depth=None):
     """Film images (one per seed) of a plain or guided render; with
-    train_passes the guide first trains on that many 1-spp passes."""
+    train_passes the guide first trains on that many 1-spp passes; depth: the
+    path vertices after the sub-path start that are guided (None: all)."""
     from libyafaray_amd.pt import GpuPathTracer
     s = dict(sc)
     s["settings"] = dict(sc["settings"], aa_samples=spp, seed=seed)
     pt = GpuPathTracer(s)
+    pt.set_guide_depth(depth)
     if guide is not None and train_passes:
         pt.set_guide(guide, bsdf_prob=0.3, train=True, train_steps=8)
         pt.clear()
@@ -158,6 +160,7 @@ def _render(sc, spp, guide=None, train_passes=0, seed=0, seeds=(0,)):
     for sd in seeds:
         s["settings"] = dict(sc["settings"], aa_samples=spp, seed=sd)
         pt = GpuPathTracer(s)
+        pt.set_guide_depth(depth)
         if guide is not None:
             pt.set_guide(guide, bsdf_prob=0.3, train=False)
         pt.clear()
@@ -168,13 +171,15 @@ def _render(sc, spp, guide=None, train_passes=0, seed=0, seeds=(0,)):
     return imgs
 
 
-def test_guided_render_converges_to_the_plain_image():
+@pytest.mark.parametrize("depth", [None, 1])
+def test_guided_render_converges_to_the_plain_image(depth):
     """One-sample MIS keeps the estimator unbiased: with an untrained and with a
     trained guide, the image mean of independent guided renders (seeds 1..8,
     256 spp each: the guided directions and the roulette draw from the seed)
     agrees with a 4096-spp plain render's within four standard errors of
     their spread (a wrong MIS weight or a biased roulette is off by per cents),
-    and the guided images are finite and of the same order of error."""
+    and the guided images are finite and of the same order of error.  Guiding
+    every vertex, and only the camera vertex and the next one."""
     from libyafaray_amd import scenes
     from libyafaray_amd.guide import GpuGuide
     sc = scenes.cornell_path_scene(n_random=2000, res=48, spp=16)
@@ -185,7 +190,7 @@ def test_guided_render_converges_to_the_plain_image():
     say(f"  plain 256 spp: mean {plain.mean():.5f} (reference {ref.mean():.5f}), relMSE {rel(plain):.2e}")
     for train in (0, 8):
         g = GpuGuide(bmin, bmax, log2_table_size=16, max_batch=1 << 14, seed=3)
-        imgs = _render(sc, 256, guide=g, train_passes=train, seeds=tuple(range(1, 9)))
+        imgs = _render(sc, 256, guide=g, train_passes=train, seeds=tuple(range(1, 9)), depth=depth)
         g.close()
         means = np.array([im.mean() for im in imgs], np.float64)
         m, se = means.mean(), means.std(ddof=1) / np.sqrt(len(means))

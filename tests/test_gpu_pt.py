@@ -217,3 +217,38 @@ def test_transparent_shadows_match_oracle(oracle):
     sc2["settings"] = dict(sc["settings"], transpShad=False)
     g2 = render_gpu(sc2)
     assert (g[0][..., :3] / g[1][..., None]).mean() > (g2[0][..., :3] / g2[1][..., None]).mean()
+
+
+def test_film_save_load_in_reference_format(oracle, tmp_path):
+    """ymx_pt_save_film writes the bytes of the reference's film file for the
+    same sums (oracle.write_film); loading replaces or adds the sums, as a
+    resumed render or a render-farm merge does; a film of another size is
+    refused and leaves the film as it was."""
+    from libyafaray_amd.pt import GpuPathTracer
+    from libyafaray_amd.native import YmxError
+    sc = small_scene(n_random=500, res=40, spp=2)
+    pt = GpuPathTracer(sc)
+    pt.clear()
+    pt.render(0, 2)
+    rgba, w = pt.film()
+    f_gpu, f_ora = tmp_path / "gpu.film", tmp_path / "ora.film"
+    pt.save_film(f_gpu, computer_node=2, base_sampling_offset=5, sampling_offset=9)
+    oracle.write_film(f_ora, rgba, w, computer_node=2, base_sampling_offset=5, sampling_offset=9)
+    assert f_gpu.read_bytes() == f_ora.read_bytes()
+    back = oracle.read_film(f_gpu, sc["camera"]["resx"], sc["camera"]["resy"])
+    assert back is not None and np.array_equal(back[0], rgba) and np.array_equal(back[1], w)
+    other = GpuPathTracer(sc)
+    other.clear()
+    assert other.load_film(f_gpu) == {"computer_node": 2, "base_sampling_offset": 5, "sampling_offset": 9}
+    r2, w2 = other.film()
+    assert np.array_equal(r2, rgba) and np.array_equal(w2, w)
+    other.load_film(f_gpu, add=True)
+    r3, w3 = other.film()
+    assert np.array_equal(r3, rgba + rgba) and np.array_equal(w3, w + w)
+    oracle.write_film(tmp_path / "small.film", np.ones((20, 20, 4), np.float32), np.ones((20, 20), np.float32))
+    with pytest.raises(YmxError):
+        other.load_film(tmp_path / "small.film")
+    r4, w4 = other.film()
+    assert np.array_equal(r4, r3) and np.array_equal(w4, w3)
+    other.close()
+    pt.close()
