@@ -175,9 +175,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(ymx_trace_args args)
 
 		st.sp = 0;
 		int32_t ref = live ? 0 : YMX_REF_EMPTY;
-		while(ref != YMX_REF_EMPTY)
+		for(;;)
 		{
-			// descend internal nodes until this lane holds a leaf (or nothing)
+			// Descend internal nodes.  The first leaf met is postponed and the
+			// descent goes on until every active lane of the wave has a leaf
+			// to test (or no internal node left), so the lanes run the
+			// triangle tests together (Aila & Laine 2009, speculative
+			// traversal).  Every leaf reached is still tested exactly once.
+			int32_t leaf = YMX_REF_EMPTY;
 			while(ref >= 0)
 			{
 				const float4 *np = args.nodes + 4 * (size_t)ref;
@@ -201,10 +206,24 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(ymx_trace_args args)
 				else if(m == 1) ref = c0;
 				else if(m == 2) ref = c1;
 				else ref = st.pop();
+				if(ref < 0 && ref != YMX_REF_EMPTY && leaf == YMX_REF_EMPTY)
+				{
+					leaf = ref;
+					ref = st.pop();
+				}
+				if(__all(leaf != YMX_REF_EMPTY || ref < 0)) break;
 			}
-			if(ref == YMX_REF_EMPTY) break;
-			// leaf
-			const uint32_t first = ymx_leaf_first(ref), count = ymx_leaf_count(ref);
+			if(leaf == YMX_REF_EMPTY)
+			{
+				if(ref == YMX_REF_EMPTY) break;
+				if(ref < 0)
+				{
+					leaf = ref;
+					ref = st.pop();
+				}
+				else continue;  // the wave stopped while this lane still descends
+			}
+			const uint32_t first = ymx_leaf_first(leaf), count = ymx_leaf_count(leaf);
 			for(uint32_t k = 0; k < count; ++k)
 			{
 				const float4 *tp = args.tris + 3 * (size_t)(first + k);
@@ -263,7 +282,6 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(ymx_trace_args args)
 				}
 			}
 			if(kShadow && occluded) break;
-			ref = st.pop();
 		}
 
 		if(kShadow)
