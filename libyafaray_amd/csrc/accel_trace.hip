@@ -334,20 +334,23 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(ymx_trace_args args)
 	}
 }
 
-// Closest hit with per-lane ray replacement (Aila & Laine 2009, persistent
-// threads with dynamic ray fetch).  A lane whose ray is done takes the next
+// Closest hit and any hit with per-lane ray replacement (Aila & Laine 2009,
+// persistent threads with dynamic ray fetch).  A lane whose ray is done takes the next
 // unclaimed ray while the rest of the wave keeps its traversal state; the
 // wave leaves its traversal loop to refill once fewer than kRefill lanes are
-// still traversing.  Each ray is traversed exactly as by k_trace<kClosest>,
-// so the hit is the same.  On the incoherent 8.4M-ray batch this is 17 %
-// faster than k_trace; on coherent batches and on the path tracer's queues,
-// where the lanes of a wave finish together, mixing rays of different
-// progress in a wave costs more than it gains, so those use k_trace.
+// still traversing.  Each ray is traversed exactly as by k_trace, so the
+// result is the same; the any-hit occlusion bits are set by atomicOr into a
+// cleared mask, since the rays of a wave are no longer one 64-ray group.  On
+// the incoherent 8.4M-ray batch this is 17 % faster than k_trace for closest
+// hit; on the path tracer's queues, where the lanes of a wave finish
+// together, mixing rays of different progress costs more than it gains, so
+// those use k_trace.
 constexpr int kRefill = 24;
 
-template <bool kStats>
-__global__ __launch_bounds__(kTraceBlock) void k_closest_dyn(ymx_trace_args args)
+template <int kMode, bool kStats>
+__global__ __launch_bounds__(kTraceBlock) void k_dyn(ymx_trace_args args)
 {
+	static_assert(kMode == kClosest || kMode == kShadowAny, "transparent shadows use k_trace");
 	__shared__ int32_t lds_stack[kStack * kTraceBlock];
 	const uint32_t gtid = blockIdx.x * kTraceBlock + threadIdx.x;
 	const int lane = (int)__lane_id();
@@ -364,7 +367,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_closest_dyn(ymx_trace_args args
 	bool alive = false, exhausted = false;
 	uint32_t rid = 0;
 	RayState r;
-	float t_min = 0.f, best_t = 0.f, best_u = 0.f, best_v = 0.f;
+	float t_min = 0.f, t_max = 0.f, best_t = 0.f, best_u = 0.f, best_v = 0.f;
 	uint32_t best_prim = YMX_NO_HIT;
 	int32_t ref = YMX_REF_EMPTY;
 	for(;;)
@@ -386,12 +389,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_closest_dyn(ymx_trace_args args
 				{
 					const ymx_ray ray = load_ray(args.rays, rid);
 					setup_ray(ray, r);
-					const float t_max = (ray.tmax >= 0.f) ? ray.tmax : FLT_MAX;
+					t_max = (ray.tmax >= 0.f) ? ray.tmax : FLT_MAX;
 					float enter = 0.f, leave = 0.f;
 					const bool live = tree_cross(args.bmin, args.bmax, ray.org, ray.dir, t_max, enter, leave);
 					// include/accelerator/accelerator_kdtree_common.h:139
 					const float bias = kBiasScale * fabsf(leave - enter);
-					t_min = (ray.tmin < bias) ? bias : ray.tmin;
+					t_min = (kMode == kShadowAny) ? bias : ((ray.tmin < bias) ? bias : ray.tmin);
 					r.tlo = t_min - t_min * kWindow;
 					best_t = t_max;
 					best_prim = YMX_NO_HIT;
@@ -457,6 +460,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_closest_dyn(ymx_trace_args args
 						if(t <= 0.f || t < t_min) continue;
 						const uint32_t prim = __float_as_uint(t0.w);
 						const uint32_t vmask = __float_as_uint(t1.w);
+						if(kMode == kShadowAny)
+						{
+							// include/accelerator/accelerator.h:135-145
+							if(t >= t_max || !(vmask & YMX_VIS_CASTS_SHADOWS)) continue;
+							best_prim = prim;
+							ref = YMX_REF_EMPTY;  // occluded: the ray is done
+							break;
+						}
 						// include/accelerator/accelerator.h:122-133; exact ties keep the lowest id
 						if(!(vmask & YMX_VIS_VISIBLE)) continue;
 						const bool closer =
@@ -472,8 +483,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_closest_dyn(ymx_trace_args args
 				{
 					// ray done
 					const bool hit = best_prim != YMX_NO_HIT;
-					reinterpret_cast<float4 *>(args.hits)[rid] =
-					    make_float4(__uint_as_float(best_prim), hit ? best_t : 0.f, hit ? best_u : 0.f, hit ? best_v : 0.f);
+					if(kMode == kShadowAny)
+					{
+						if(hit) atomicOr(args.occluded + (rid >> 5), 1u << (rid & 31u));
+						if(args.occ_prim) args.occ_prim[rid] = best_prim;
+					}
+					else
+						reinterpret_cast<float4 *>(args.hits)[rid] = make_float4(
+						    __uint_as_float(best_prim), hit ? best_t : 0.f, hit ? best_u : 0.f, hit ? best_v : 0.f);
 					alive = false;
 					break;
 				}
@@ -534,7 +551,11 @@ int launch(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t *o
 	uint32_t threads = a.grid_threads < needed ? a.grid_threads : needed;
 	const uint32_t blocks = (threads + kTraceBlock - 1) / kTraceBlock;
 	k_reset_work<<<1, 1, 0, stream>>>(acc->d_work);
-	if constexpr(kMode == kClosest && kDynamic) k_closest_dyn<kStats><<<blocks, kTraceBlock, 0, stream>>>(a);
+	if constexpr(kDynamic && kMode != kTransparent)
+	{
+		if(kShadow) YMX_HIP_CHECK(hipMemsetAsync(occ, 0, sizeof(uint32_t) * (((size_t)n + 31u) / 32u), stream));
+		k_dyn<kMode, kStats><<<blocks, kTraceBlock, 0, stream>>>(a);
+	}
 	else k_trace<kMode, kStats><<<blocks, kTraceBlock, 0, stream>>>(a);
 	YMX_HIP_CHECK(hipGetLastError());
 	return YMX_OK;
@@ -551,12 +572,15 @@ int ymx_trace_setup(ymx_accel *acc)
 	int per_cu_c = 0, per_cu_s = 0;
 	YMX_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_c, k_trace<kClosest, false>, kTraceBlock, 0));
 	int per_cu_d = 0;
-	YMX_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_d, k_closest_dyn<false>, kTraceBlock, 0));
+	YMX_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_d, k_dyn<kClosest, false>, kTraceBlock, 0));
 	if(per_cu_d < per_cu_c) per_cu_c = per_cu_d;  // the closest-hit grid serves both
 	YMX_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s, k_trace<kShadowAny, false>, kTraceBlock, 0));
 	int per_cu_t = 0;
 	YMX_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_t, k_trace<kTransparent, false>, kTraceBlock, 0));
-	if(per_cu_t < per_cu_s) per_cu_s = per_cu_t;  // the any-hit grid serves both
+	if(per_cu_t < per_cu_s) per_cu_s = per_cu_t;  // the any-hit grid serves all three
+	int per_cu_ds = 0;
+	YMX_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_ds, k_dyn<kShadowAny, false>, kTraceBlock, 0));
+	if(per_cu_ds < per_cu_s) per_cu_s = per_cu_ds;
 	if(per_cu_c < 1) per_cu_c = 1;
 	if(per_cu_s < 1) per_cu_s = 1;
 	acc->grid_threads_closest = (uint32_t)(prop.multiProcessorCount * per_cu_c * kTraceBlock);
@@ -593,8 +617,8 @@ int ymx_launch_shadow_queue(const ymx_accel *acc, const ymx_ray *rays, uint32_t 
 int ymx_launch_shadow(const ymx_accel *acc, const ymx_ray *rays, uint32_t *occ, uint32_t *prim, uint32_t n,
                       hipStream_t stream, unsigned long long *stats)
 {
-	if(stats) return launch<kShadowAny, true>(acc, rays, nullptr, occ, prim, n, stream, stats);
-	return launch<kShadowAny, false>(acc, rays, nullptr, occ, prim, n, stream, nullptr);
+	if(stats) return launch<kShadowAny, true, true>(acc, rays, nullptr, occ, prim, n, stream, stats);
+	return launch<kShadowAny, false, true>(acc, rays, nullptr, occ, prim, n, stream, nullptr);
 }
 
 int ymx_launch_transparent(const ymx_accel *acc, const ymx_ray *rays, const uint32_t *transp, uint32_t max_depth,

@@ -497,6 +497,29 @@ int PathIntegrator::film(std::vector<float> &rgba, std::vector<float> &weight) c
 	return ymx_pt_read_film(pt_, rgba.data(), weight.data());
 }
 
+int PathIntegrator::saveFilm(const std::string &path, int computer_node, int base_sampling_offset, int sampling_offset)
+{
+	const int rc = ymx_pt_save_film(pt_, path.c_str(), computer_node, base_sampling_offset, sampling_offset);
+	if(rc != YMX_OK) error_ = ymx_last_error();
+	return rc;
+}
+
+int PathIntegrator::loadFilm(const std::string &path, bool add, int *computer_node, int *base_sampling_offset,
+                             int *sampling_offset)
+{
+	int32_t node = 0, base = 0, off = 0;
+	const int rc = ymx_pt_load_film(pt_, path.c_str(), add ? 1 : 0, &node, &base, &off);
+	if(rc != YMX_OK)
+	{
+		error_ = ymx_last_error();
+		return rc;
+	}
+	if(computer_node) *computer_node = node;
+	if(base_sampling_offset) *base_sampling_offset = base;
+	if(sampling_offset) *sampling_offset = off;
+	return YMX_OK;
+}
+
 ymx_pt_stats PathIntegrator::stats() const
 {
 	ymx_pt_stats st{};

@@ -204,6 +204,12 @@ class PathIntegrator
 		// The film as ImageFilm holds it: colour and alpha sums weighted by the
 		// filter, and the weight sums; pixel (x, y) at index y * resx + x.
 		int film(std::vector<float> &rgba, std::vector<float> &weight) const;
+		// ImageFilm::imageFilmSave / imageFilmLoad (src/render/imagefilm.cc:910-1176):
+		// the YAF_FILMv4_0_0 film file; load with add merges a render node's film.
+		int saveFilm(const std::string &path, int computer_node = 0, int base_sampling_offset = 0,
+		             int sampling_offset = 0);
+		int loadFilm(const std::string &path, bool add, int *computer_node = nullptr,
+		             int *base_sampling_offset = nullptr, int *sampling_offset = nullptr);
 		int resx() const { return settings_.resx; }
 		int resy() const { return settings_.resy; }
 		ymx_pt_stats stats() const;

@@ -10,6 +10,9 @@
 //   errors: unknown types and missing objects are refused with a message.
 // Usage: test_host [--cpu]   (--cpu: only the checks that need no GPU)
 #include <hip/hip_runtime.h>
+#include <unistd.h>
+
+#include <cstdlib>
 
 #include <cmath>
 #include <cstdio>
@@ -430,6 +433,29 @@ static void test_pathtracer()
 	printf("  pathtracer: 40x32x4, mean %.5f, weight mismatches %zu, colour values off %zu\n", mean / (3.0 * w.size()),
 	       bad_w, bad_c);
 	CHECK(bad_w == 0 && bad_c <= 3 && mean > 0.0, "film differs from the oracle");
+	// the film file: save, then load into a second integrator (replace, then merge)
+	char path[] = "/tmp/ymx_host_film_XXXXXX";
+	const int fd = mkstemp(path);
+	CHECK(fd >= 0, "temporary film file");
+	if(fd < 0) return;
+	close(fd);
+	CHECK(pi->saveFilm(path, 1, 2, 4) == YMX_OK, "saveFilm: %s", pi->lastError().c_str());
+	auto pi2 = PathIntegrator::create(sc, integ, cam, film, nullptr, &err);
+	CHECK(pi2 != nullptr, "second integrator: %s", err.c_str());
+	if(pi2)
+	{
+		int node = -1, base = -1, off = -1;
+		CHECK(pi2->clearFilm() == YMX_OK && pi2->loadFilm(path, false, &node, &base, &off) == YMX_OK,
+		      "loadFilm: %s", pi2->lastError().c_str());
+		std::vector<float> rgba2, w2;
+		CHECK(pi2->film(rgba2, w2) == YMX_OK && rgba2 == rgba && w2 == w && node == 1 && base == 2 && off == 4,
+		      "loaded film differs");
+		CHECK(pi2->loadFilm(path, true) == YMX_OK && pi2->film(rgba2, w2) == YMX_OK, "merge");
+		bool doubled = true;
+		for(size_t i = 0; i < w.size(); ++i) doubled = doubled && w2[i] == w[i] + w[i];
+		CHECK(doubled, "merged weights are not the sum");
+	}
+	unlink(path);
 }
 
 static void test_errors()
