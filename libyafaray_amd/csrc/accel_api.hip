@@ -54,7 +54,7 @@ int ymx_accel_create(const float *verts, uint32_t n_verts, const uint32_t *tri_i
 		ymx_set_error("ymx_accel_create: %u triangles exceed the limit of %u", n_tris, YMX_LEAF_MAX_FIRST);
 		return YMX_ERR_UNSUPPORTED;
 	}
-	int max_leaf = 1;
+	int max_leaf = 1, builder = YMX_BUILDER_LBVH;
 	if(params)
 	{
 		if(params->struct_size != sizeof(ymx_accel_params))
@@ -69,6 +69,12 @@ int ymx_accel_create(const float *verts, uint32_t n_verts, const uint32_t *tri_i
 			return YMX_ERR_INVALID_ARG;
 		}
 		if(params->max_leaf_size > 0) max_leaf = params->max_leaf_size;
+		if(params->builder != YMX_BUILDER_LBVH && params->builder != YMX_BUILDER_PLOC)
+		{
+			ymx_set_error("ymx_accel_create: unknown builder %d", params->builder);
+			return YMX_ERR_INVALID_ARG;
+		}
+		builder = params->builder;
 	}
 	for(size_t i = 0; i < 3 * (size_t)n_tris; ++i)
 	{
@@ -92,7 +98,7 @@ int ymx_accel_create(const float *verts, uint32_t n_verts, const uint32_t *tri_i
 		delete acc;
 		return YMX_ERR_HIP;
 	}
-	int rc = ymx_build_device(acc, verts, n_verts, tri_idx, vis, n_tris, max_leaf);
+	int rc = ymx_build_device(acc, verts, n_verts, tri_idx, vis, n_tris, max_leaf, builder);
 	if(rc == YMX_OK) rc = ymx_trace_setup(acc);
 	if(rc != YMX_OK)
 	{

@@ -8,13 +8,15 @@
 //            centroid bounds folded in with ordered-int atomics.
 //   morton   63-bit Morton code of the centroid (21 bits per axis).
 //   sort     radix sort of (code, triangle) pairs.
-//   karras   one thread per internal node finds its key range and split
-//            (T. Karras, "Maximizing Parallelism in the Construction of BVHs,
-//            Octrees, and k-d Trees", HPG 2012), writing child refs and
-//            parent links.
 //   reorder  triangle records into leaf order.
-//   refit    one thread per leaf climbs to the root; the second arrival at a
-//            node unions both child boxes (atomic counter per node).
+//   topology, by the builder chosen at create (ymx_accel_params.builder):
+//     LBVH (default): one thread per internal node finds its key range and
+//            split (T. Karras, "Maximizing Parallelism in the Construction of
+//            BVHs, Octrees, and k-d Trees", HPG 2012), writing child refs and
+//            parent links; then one thread per leaf climbs to the root and
+//            the second arrival at a node unions both child boxes (refit).
+//     PLOC:  agglomerative clustering in Morton order (see k_ploc_nn),
+//            writing nodes, boxes and parent links as pairs merge.
 //   depth    longest root-to-leaf chain, sizing the traversal stack.
 //
 // The reference builds a SAH kd-tree on the CPU instead
@@ -291,6 +293,144 @@ __global__ __launch_bounds__(kBuildBlock) void k_depth(const int32_t *__restrict
 	if((threadIdx.x & 63) == 0) atomicMax(max_depth, d);
 }
 
+// ---- PLOC: parallel locally-ordered clustering (D. Meister, J. Bittner,
+// "Parallel Locally-Ordered Clustering for Bounding Volume Hierarchy
+// Construction", IEEE TVCG 2018).  Clusters stay in Morton order; each
+// iteration every cluster finds the neighbour within kPlocRadius positions
+// whose union box has the smallest surface area, mutual nearest neighbours
+// merge into a new internal node, and the survivors are compacted.  Node
+// indices are handed out downwards from n - 2, so the last merge, the root,
+// is node 0.
+constexpr int kPlocRadius = 16;
+
+struct Cluster
+{
+	float4 lo, hi;  // xyz: box; lo.w: node ref bits
+};
+
+__device__ __forceinline__ float union_area(const Cluster &a, const Cluster &b)
+{
+	const float dx = fmaxf(a.hi.x, b.hi.x) - fminf(a.lo.x, b.lo.x);
+	const float dy = fmaxf(a.hi.y, b.hi.y) - fminf(a.lo.y, b.lo.y);
+	const float dz = fmaxf(a.hi.z, b.hi.z) - fminf(a.lo.z, b.lo.z);
+	return dx * dy + dy * dz + dz * dx;
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_ploc_init(const float *__restrict__ prim_box,
+                                                           const uint32_t *__restrict__ order, uint32_t n, float pad,
+                                                           Cluster *__restrict__ c)
+{
+	const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+	if(j >= n) return;
+	const uint32_t p = order[j];
+	const float *b = prim_box + 6 * (size_t)p;
+	Cluster k;
+	k.lo = make_float4(b[0] - pad, b[1] - pad, b[2] - pad, __int_as_float(ymx_leaf_ref(j, 1u)));
+	k.hi = make_float4(b[3] + pad, b[4] + pad, b[5] + pad, 0.f);
+	c[j] = k;
+}
+
+// Nearest neighbour of each cluster (smallest union area; ties to the lower index).
+__global__ __launch_bounds__(kBuildBlock) void k_ploc_nn(const Cluster *__restrict__ c, uint32_t count,
+                                                         int32_t *__restrict__ nn)
+{
+	__shared__ Cluster tile[kBuildBlock + 2 * kPlocRadius];
+	const int base = (int)(blockIdx.x * blockDim.x) - kPlocRadius;
+	for(int t = threadIdx.x; t < kBuildBlock + 2 * kPlocRadius; t += kBuildBlock)
+	{
+		const int g = base + t;
+		if(g >= 0 && g < (int)count) tile[t] = c[g];
+	}
+	__syncthreads();
+	const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+	if(i >= (int)count) return;
+	const Cluster me = tile[i - base];
+	float best = INFINITY;
+	int best_j = -1;
+	const int j0 = i - kPlocRadius < 0 ? 0 : i - kPlocRadius;
+	const int j1 = i + kPlocRadius >= (int)count ? (int)count - 1 : i + kPlocRadius;
+	for(int j = j0; j <= j1; ++j)
+	{
+		if(j == i) continue;
+		const float a = union_area(me, tile[j - base]);
+		if(a < best)  // ascending j: ties keep the lower index
+		{
+			best = a;
+			best_j = j;
+		}
+	}
+	nn[i] = best_j;
+}
+
+// keep[i]: cluster i survives (merged or not); merge[i]: i starts a merge.
+__global__ __launch_bounds__(kBuildBlock) void k_ploc_flags(const int32_t *__restrict__ nn, uint32_t count,
+                                                            uint32_t *__restrict__ keep, uint32_t *__restrict__ merge)
+{
+	const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+	if(i >= (int)count) return;
+	const int j = nn[i];
+	const bool mutual = j >= 0 && nn[j] == i;
+	keep[i] = (mutual && j < i) ? 0u : 1u;
+	merge[i] = (mutual && i < j) ? 1u : 0u;
+}
+
+// Every internal node must hold two child refs in range (d_nodes starts as
+// 0xFF bytes, so a node no merge wrote has YMX_REF_EMPTY refs).
+__global__ __launch_bounds__(kBuildBlock) void k_check_tree(const float4 *__restrict__ nodes, uint32_t n_tris,
+                                                            uint32_t *__restrict__ bad)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i + 1 >= n_tris) return;
+	const float4 f3 = nodes[4 * (size_t)i + 3];
+	const int32_t r[2] = {__float_as_int(f3.x), __float_as_int(f3.y)};
+	for(int k = 0; k < 2; ++k)
+	{
+		const bool ok = r[k] >= 0 ? (uint32_t)r[k] < n_tris - 1u && (uint32_t)r[k] != i
+		                          : r[k] != YMX_REF_EMPTY && ymx_leaf_first(r[k]) < n_tris && ymx_leaf_count(r[k]) == 1u;
+		if(!ok) atomicOr(bad, 1u);
+	}
+}
+
+__global__ void k_ploc_totals(const uint32_t *keep, const uint32_t *keep_pos, const uint32_t *merge,
+                              const uint32_t *merge_pos, uint32_t count, uint32_t *totals)
+{
+	totals[0] = keep_pos[count - 1] + keep[count - 1];
+	totals[1] = merge_pos[count - 1] + merge[count - 1];
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_ploc_merge(const Cluster *__restrict__ c, const int32_t *__restrict__ nn,
+                                                            const uint32_t *__restrict__ keep,
+                                                            const uint32_t *__restrict__ keep_pos,
+                                                            const uint32_t *__restrict__ merge,
+                                                            const uint32_t *__restrict__ merge_pos, uint32_t count,
+                                                            int32_t next_node, float4 *__restrict__ nodes,
+                                                            int32_t *__restrict__ parent_internal,
+                                                            int32_t *__restrict__ parent_leaf, Cluster *__restrict__ out)
+{
+	const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+	if(i >= (int)count || !keep[i]) return;
+	Cluster a = c[i];
+	if(merge[i])
+	{
+		const Cluster b = c[nn[i]];
+		const int32_t node = next_node - (int32_t)merge_pos[i];
+		const int32_t refs[2] = {__float_as_int(a.lo.w), __float_as_int(b.lo.w)};
+		const Cluster *kids[2] = {&a, &b};
+		for(int s = 0; s < 2; ++s)
+		{
+			const float lo[3] = {kids[s]->lo.x, kids[s]->lo.y, kids[s]->lo.z};
+			const float hi[3] = {kids[s]->hi.x, kids[s]->hi.y, kids[s]->hi.z};
+			store_slot(nodes, node, s, lo, hi);
+			if(refs[s] >= 0) parent_internal[refs[s]] = node;
+			else parent_leaf[ymx_leaf_first(refs[s])] = node;
+		}
+		nodes[4 * (size_t)node + 3] = make_float4(__int_as_float(refs[0]), __int_as_float(refs[1]), 0.f, 0.f);
+		a.lo = make_float4(fminf(a.lo.x, b.lo.x), fminf(a.lo.y, b.lo.y), fminf(a.lo.z, b.lo.z), __int_as_float(node));
+		a.hi = make_float4(fmaxf(a.hi.x, b.hi.x), fmaxf(a.hi.y, b.hi.y), fmaxf(a.hi.z, b.hi.z), 0.f);
+	}
+	out[keep_pos[i]] = a;
+}
+
 inline unsigned grid_for(uint64_t n) { return (unsigned)((n + kBuildBlock - 1) / kBuildBlock); }
 
 struct DevBuf
@@ -340,7 +480,7 @@ void reference_tree_bound(const float lo_in[3], const float hi_in[3], uint32_t n
 }  // namespace
 
 int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const uint32_t *idx,
-                     const uint8_t *vis, uint32_t n_tris, int max_leaf)
+                     const uint8_t *vis, uint32_t n_tris, int max_leaf, int builder)
 {
 	(void)max_leaf;
 	acc->n_tris = n_tris;
@@ -416,7 +556,7 @@ int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const
 	}
 
 	DevBuf d_verts, d_idx, d_vis, d_tris_unsorted, d_prim_box, d_centroid, d_bounds, d_keys, d_keys_alt,
-	    d_vals, d_vals_alt, d_nodes, d_tris, d_parent_int, d_parent_leaf, d_flags, d_depth, d_temp;
+	    d_vals, d_vals_alt, d_nodes, d_tris, d_parent_int, d_parent_leaf, d_depth, d_temp;
 	YMX_HIP_CHECK(d_verts.alloc(sizeof(float) * 3 * (size_t)n_verts));
 	YMX_HIP_CHECK(d_idx.alloc(sizeof(uint32_t) * 3 * (size_t)n_tris));
 	if(vis) YMX_HIP_CHECK(d_vis.alloc(n_tris));
@@ -432,7 +572,6 @@ int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const
 	YMX_HIP_CHECK(d_tris.alloc(sizeof(float4) * 3 * (size_t)n_tris));
 	YMX_HIP_CHECK(d_parent_int.alloc(sizeof(int32_t) * (size_t)(n_tris - 1)));
 	YMX_HIP_CHECK(d_parent_leaf.alloc(sizeof(int32_t) * (size_t)n_tris));
-	YMX_HIP_CHECK(d_flags.alloc(sizeof(uint32_t) * (size_t)(n_tris - 1)));
 	YMX_HIP_CHECK(d_depth.alloc(sizeof(uint32_t)));
 
 	YMX_HIP_CHECK(hipMemcpy(d_verts.p, verts, sizeof(float) * 3 * (size_t)n_verts, hipMemcpyHostToDevice));
@@ -481,22 +620,103 @@ int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const
 	const uint64_t *keys = kb.Current();
 	const uint32_t *order = vb.Current();
 
-	k_karras<<<grid_for(n_tris - 1), kBuildBlock, 0, stream>>>(keys, (int)n_tris, d_nodes.as<float4>(),
-	                                                           d_parent_int.as<int32_t>(),
-	                                                           d_parent_leaf.as<int32_t>());
-	YMX_HIP_CHECK(hipGetLastError());
 	k_reorder<<<grid_for(n_tris), kBuildBlock, 0, stream>>>(d_tris_unsorted.as<float4>(), order, n_tris,
 	                                                        d_tris.as<float4>());
 	YMX_HIP_CHECK(hipGetLastError());
-	YMX_HIP_CHECK(hipMemsetAsync(d_flags.p, 0, sizeof(uint32_t) * (size_t)(n_tris - 1), stream));
 	YMX_HIP_CHECK(hipMemsetAsync(d_depth.p, 0, sizeof(uint32_t), stream));
 	float ext = 0.f;
 	for(int k = 0; k < 3; ++k) ext = fmaxf(ext, scene_hi[k] - scene_lo[k]);
 	const float pad = ext * 0x1p-20f + 1e-30f;
-	k_refit<<<grid_for(n_tris), kBuildBlock, 0, stream>>>(d_nodes.as<float4>(), d_prim_box.as<float>(), order,
-	                                                      n_tris, d_parent_int.as<int32_t>(),
-	                                                      d_parent_leaf.as<int32_t>(), d_flags.as<uint32_t>(), pad);
-	YMX_HIP_CHECK(hipGetLastError());
+	if(builder != YMX_BUILDER_PLOC)
+	{
+		DevBuf d_flags;
+		YMX_HIP_CHECK(d_flags.alloc(sizeof(uint32_t) * (size_t)(n_tris - 1)));
+		k_karras<<<grid_for(n_tris - 1), kBuildBlock, 0, stream>>>(keys, (int)n_tris, d_nodes.as<float4>(),
+		                                                           d_parent_int.as<int32_t>(),
+		                                                           d_parent_leaf.as<int32_t>());
+		YMX_HIP_CHECK(hipGetLastError());
+		YMX_HIP_CHECK(hipMemsetAsync(d_flags.p, 0, sizeof(uint32_t) * (size_t)(n_tris - 1), stream));
+		k_refit<<<grid_for(n_tris), kBuildBlock, 0, stream>>>(d_nodes.as<float4>(), d_prim_box.as<float>(), order,
+		                                                      n_tris, d_parent_int.as<int32_t>(),
+		                                                      d_parent_leaf.as<int32_t>(), d_flags.as<uint32_t>(), pad);
+		YMX_HIP_CHECK(hipGetLastError());
+	}
+	else
+	{
+		YMX_HIP_CHECK(hipMemsetAsync(d_nodes.p, 0xFF, sizeof(float4) * 4 * (size_t)(n_tris - 1), stream));
+		DevBuf d_ca, d_cb, d_nn, d_keep, d_merge, d_kpos, d_mpos, d_tot, d_scan;
+		YMX_HIP_CHECK(d_ca.alloc(sizeof(Cluster) * (size_t)n_tris));
+		YMX_HIP_CHECK(d_cb.alloc(sizeof(Cluster) * (size_t)n_tris));
+		YMX_HIP_CHECK(d_nn.alloc(sizeof(int32_t) * (size_t)n_tris));
+		YMX_HIP_CHECK(d_keep.alloc(sizeof(uint32_t) * (size_t)n_tris));
+		YMX_HIP_CHECK(d_merge.alloc(sizeof(uint32_t) * (size_t)n_tris));
+		YMX_HIP_CHECK(d_kpos.alloc(sizeof(uint32_t) * (size_t)n_tris));
+		YMX_HIP_CHECK(d_mpos.alloc(sizeof(uint32_t) * (size_t)n_tris));
+		YMX_HIP_CHECK(d_tot.alloc(sizeof(uint32_t) * 2));
+		size_t scan_bytes = 0;
+		YMX_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, d_keep.as<uint32_t>(), d_kpos.as<uint32_t>(),
+		                                               (int)n_tris, stream));
+		YMX_HIP_CHECK(d_scan.alloc(scan_bytes));
+		k_ploc_init<<<grid_for(n_tris), kBuildBlock, 0, stream>>>(d_prim_box.as<float>(), order, n_tris, pad,
+		                                                          d_ca.as<Cluster>());
+		YMX_HIP_CHECK(hipGetLastError());
+		Cluster *cin = d_ca.as<Cluster>(), *cout = d_cb.as<Cluster>();
+		uint32_t count = n_tris, iterations = 0;
+		int32_t next_node = (int32_t)n_tris - 2;
+		while(count > 1)
+		{
+			if(++iterations > 1000000u)
+			{
+				ymx_set_error("ymx_accel_create: BVH clustering does not converge");
+				return YMX_ERR_HIP;
+			}
+			k_ploc_nn<<<grid_for(count), kBuildBlock, 0, stream>>>(cin, count, d_nn.as<int32_t>());
+			YMX_HIP_CHECK(hipGetLastError());
+			k_ploc_flags<<<grid_for(count), kBuildBlock, 0, stream>>>(d_nn.as<int32_t>(), count, d_keep.as<uint32_t>(),
+			                                                          d_merge.as<uint32_t>());
+			YMX_HIP_CHECK(hipGetLastError());
+			size_t sb = scan_bytes;
+			YMX_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(d_scan.p, sb, d_keep.as<uint32_t>(), d_kpos.as<uint32_t>(),
+			                                               (int)count, stream));
+			sb = scan_bytes;
+			YMX_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(d_scan.p, sb, d_merge.as<uint32_t>(), d_mpos.as<uint32_t>(),
+			                                               (int)count, stream));
+			k_ploc_totals<<<1, 1, 0, stream>>>(d_keep.as<uint32_t>(), d_kpos.as<uint32_t>(), d_merge.as<uint32_t>(),
+			                                   d_mpos.as<uint32_t>(), count, d_tot.as<uint32_t>());
+			YMX_HIP_CHECK(hipGetLastError());
+			k_ploc_merge<<<grid_for(count), kBuildBlock, 0, stream>>>(
+			    cin, d_nn.as<int32_t>(), d_keep.as<uint32_t>(), d_kpos.as<uint32_t>(), d_merge.as<uint32_t>(),
+			    d_mpos.as<uint32_t>(), count, next_node, d_nodes.as<float4>(), d_parent_int.as<int32_t>(),
+			    d_parent_leaf.as<int32_t>(), cout);
+			YMX_HIP_CHECK(hipGetLastError());
+			uint32_t tot[2] = {0u, 0u};
+			YMX_HIP_CHECK(hipMemcpyAsync(tot, d_tot.p, sizeof tot, hipMemcpyDeviceToHost, stream));
+			YMX_HIP_CHECK(hipStreamSynchronize(stream));
+			if(tot[1] == 0u || tot[1] > (uint32_t)(next_node + 1) || tot[0] + tot[1] != count)
+			{
+				ymx_set_error("ymx_accel_create: BVH clustering made no valid progress (%u clusters, %u merges)",
+				              count, tot[1]);
+				return YMX_ERR_HIP;
+			}
+			next_node -= (int32_t)tot[1];
+			count = tot[0];
+			Cluster *t = cin;
+			cin = cout;
+			cout = t;
+		}
+		// the tree must be complete before any traversal reads it
+		YMX_HIP_CHECK(hipMemsetAsync(d_tot.p, 0, sizeof(uint32_t), stream));
+		k_check_tree<<<grid_for(n_tris), kBuildBlock, 0, stream>>>(d_nodes.as<float4>(), n_tris, d_tot.as<uint32_t>());
+		YMX_HIP_CHECK(hipGetLastError());
+		uint32_t bad = 0;
+		YMX_HIP_CHECK(hipMemcpyAsync(&bad, d_tot.p, sizeof bad, hipMemcpyDeviceToHost, stream));
+		YMX_HIP_CHECK(hipStreamSynchronize(stream));
+		if(next_node != -1 || bad)
+		{
+			ymx_set_error("ymx_accel_create: BVH clustering left an incomplete tree");
+			return YMX_ERR_HIP;
+		}
+	}
 	k_depth<<<grid_for(n_tris), kBuildBlock, 0, stream>>>(d_parent_int.as<int32_t>(), d_parent_leaf.as<int32_t>(),
 	                                                      n_tris, d_depth.as<uint32_t>());
 	YMX_HIP_CHECK(hipGetLastError());

@@ -80,7 +80,7 @@ struct ymx_trace_args
 
 // Build entry (accel_build.hip) and launch entries (accel_trace.hip).
 int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const uint32_t *idx,
-                     const uint8_t *vis, uint32_t n_tris, int max_leaf);
+                     const uint8_t *vis, uint32_t n_tris, int max_leaf, int builder);
 int ymx_launch_closest(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t n,
                        hipStream_t stream, unsigned long long *stats);
 int ymx_launch_shadow(const ymx_accel *acc, const ymx_ray *rays, uint32_t *occ, uint32_t *prim,

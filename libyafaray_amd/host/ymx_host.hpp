@@ -100,7 +100,8 @@ class Accelerator
 {
 	public:
 		// Accelerator::factory (src/accelerator/accelerator.cc:44-55) for type
-		// "mi355x-lbvh": builds the tree on the current GPU over n_tris
+		// "mi355x-lbvh" (params "builder": "lbvh" default or "ploc",
+		// "max_leaf_size"): builds the tree on the current GPU over n_tris
 		// triangles (verts: 3 floats each; tri_idx: 3 indices each; vis: one
 		// visibility mask each or nullptr).  nullptr and `error` on failure.
 		static std::unique_ptr<Accelerator> factory(const ParamMap &params, const float *verts, uint32_t n_verts,

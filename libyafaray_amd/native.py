@@ -27,7 +27,11 @@ class YmxHit(C.Structure):
 
 
 class YmxAccelParams(C.Structure):
-    _fields_ = [("struct_size", C.c_uint32), ("max_leaf_size", C.c_int32), ("reserved", C.c_int32 * 6)]
+    _fields_ = [("struct_size", C.c_uint32), ("max_leaf_size", C.c_int32), ("builder", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
+
+
+BUILDERS = {"lbvh": 0, "ploc": 1}
 
 
 class YmxAccelInfo(C.Structure):

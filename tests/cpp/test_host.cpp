@@ -114,10 +114,13 @@ static std::vector<ora_ray> to_oracle(const std::vector<Ray> &rays)
 	return o;
 }
 
+static const char *g_builder = "lbvh";
+
 static std::unique_ptr<Accelerator> build(const Soup &s, const std::vector<uint8_t> &vis)
 {
 	ParamMap p;
 	p.setString("type", "mi355x-lbvh");
+	p.setString("builder", g_builder);
 	std::string err;
 	auto a = Accelerator::factory(p, s.verts.data(), (uint32_t)(s.verts.size() / 3), s.tris.data(), vis.data(),
 	                              (uint32_t)(s.tris.size() / 3), &err);
@@ -466,6 +469,10 @@ static void test_errors()
 	const float v[9] = {0, 0, 0, 1, 0, 0, 0, 1, 0};
 	const uint32_t t[3] = {0, 1, 2};
 	CHECK(Accelerator::factory(p, v, 3, t, nullptr, 1, &err) == nullptr && !err.empty(), "other accelerator types refused");
+	p.setString("type", "mi355x-lbvh");
+	p.setString("builder", "sah-sweep");
+	err.clear();
+	CHECK(Accelerator::factory(p, v, 3, t, nullptr, 1, &err) == nullptr && !err.empty(), "unknown builder refused");
 	Scene sc;
 	ParamMap m;
 	m.setString("type", "roughglass");
@@ -507,6 +514,8 @@ int main(int argc, char **argv)
 		bool gpu;
 	} tests[] = {{"errors", test_errors, false},
 	             {"accel_closest", test_accel_closest, true},
+	             {"accel_closest_ploc", [] { g_builder = "ploc"; test_accel_closest(); g_builder = "lbvh"; }, true},
+	             {"accel_shadow_ploc", [] { g_builder = "ploc"; test_accel_shadow(); g_builder = "lbvh"; }, true},
 	             {"accel_shadow", test_accel_shadow, true},
 	             {"accel_transparent", test_accel_transparent, true},
 	             {"pathtracer", test_pathtracer, true}};

@@ -51,11 +51,12 @@ def small_scene(seed=0, res=64, n_random=2000):
     return verts, tris, rays, rng
 
 
+@pytest.mark.parametrize("builder", ["lbvh", "ploc"])
 @pytest.mark.parametrize("seed", [0, 1])
-def test_closest_bit_exact_small(oracle, seed):
+def test_closest_bit_exact_small(oracle, seed, builder):
     verts, tris, rays, _ = small_scene(seed)
     say(f"closest small seed={seed} tris={len(tris)} rays={len(rays)}")
-    acc = GpuAccelerator(verts, tris)
+    acc = GpuAccelerator(verts, tris, builder=builder)
     a, g = acc.bound()
     ea, eg = oracle.tree_bound(verts, tris)
     assert np.array_equal(a, ea) and np.array_equal(g, eg)
@@ -65,13 +66,14 @@ def test_closest_bit_exact_small(oracle, seed):
     assert_hits_equal(got, exp, "closest")
 
 
+@pytest.mark.parametrize("builder", ["lbvh", "ploc"])
 @pytest.mark.parametrize("seed", [0, 1])
-def test_shadow_bit_exact_small(oracle, seed):
+def test_shadow_bit_exact_small(oracle, seed, builder):
     verts, tris, rays, rng = small_scene(seed)
     rays = rays.copy()
     rays[:, 7] = scenes.shadow_tmax(rng, len(rays))
     say(f"shadow small seed={seed} rays={len(rays)}")
-    acc = GpuAccelerator(verts, tris)
+    acc = GpuAccelerator(verts, tris, builder=builder)
     occ, prim = acc.intersect_shadow(rays)
     exp = oracle.shadow_brute(verts, tris, rays)
     assert 0.05 < exp.mean() < 0.95
@@ -96,7 +98,8 @@ def test_visibility_masks(oracle):
     assert np.array_equal(occ, oracle.shadow_brute(verts, tris, srays, vis=vis))
 
 
-def test_ties_on_grid_vertices(oracle):
+@pytest.mark.parametrize("builder", ["lbvh", "ploc"])
+def test_ties_on_grid_vertices(oracle, builder):
     """Vertical rays through heightfield vertices and edges meet several
     triangles at the same t; the lowest id must win, as in the oracle."""
     verts, tris = scenes.heightfield_scene(res=33, n_random=0, seed=3)
@@ -108,20 +111,21 @@ def test_ties_on_grid_vertices(oracle):
     rays[:, 6] = -1.0
     rays[:, 7] = -1.0
     say(f"ties rays={len(rays)}")
-    acc = GpuAccelerator(verts, tris)
+    acc = GpuAccelerator(verts, tris, builder=builder)
     got = acc.intersect(rays)
     exp = oracle.closest_brute(verts, tris, rays)
     assert_hits_equal(got, exp, "ties")
 
 
+@pytest.mark.parametrize("builder", ["lbvh", "ploc"])
 @pytest.mark.parametrize("n_tris", [0, 1, 2, 3])
-def test_tiny_scenes(oracle, n_tris):
+def test_tiny_scenes(oracle, n_tris, builder):
     rng = np.random.default_rng(n_tris)
     tv = scenes.random_triangles(rng, max(n_tris, 1), (0.3, 0.3, 0.3), (0.7, 0.7, 0.7), edge=(0.1, 0.3))[:n_tris]
     verts, tris = scenes.soup(tv) if n_tris else (np.zeros((3, 3), np.float32), np.zeros((0, 3), np.uint32))
     rays = scenes.incoherent_rays(rng, 5000, (0.2, 0.2, 0.2), (0.8, 0.8, 0.8))
     say(f"tiny n_tris={n_tris}")
-    acc = GpuAccelerator(verts, tris)
+    acc = GpuAccelerator(verts, tris, builder=builder)
     got = acc.intersect(rays)
     if n_tris == 0:
         assert np.all(got["prim"] == NO_HIT)
@@ -186,7 +190,8 @@ def test_deterministic_repeat():
         assert np.array_equal(bits(a[k]) if k != "prim" else a[k], bits(b[k]) if k != "prim" else b[k])
 
 
-def test_medium_scene_against_kdtree_port(oracle):
+@pytest.mark.parametrize("builder", ["lbvh", "ploc"])
+def test_medium_scene_against_kdtree_port(oracle, builder):
     """~140k triangles: GPU vs the reference-style kd-tree (ties listed)."""
     verts, tris = scenes.heightfield_scene(res=256, n_random=10_000, seed=7)
     rng = np.random.default_rng(70)
@@ -194,7 +199,7 @@ def test_medium_scene_against_kdtree_port(oracle):
     rays = np.concatenate([scenes.incoherent_rays(rng, 150_000, lo, hi),
                            scenes.camera_rays(512, 256, **scenes.RAYCAST_CAMERA)])
     say(f"medium tris={len(tris)} rays={len(rays)}")
-    acc = GpuAccelerator(verts, tris)
+    acc = GpuAccelerator(verts, tris, builder=builder)
     got = acc.intersect(rays)
     kd = oracle.KdTree(verts, tris)
     exp = kd.closest(rays)
@@ -213,12 +218,13 @@ def test_medium_scene_against_kdtree_port(oracle):
 
 
 @pytest.mark.timeout(600)
-def test_full_raycast_workload_properties(oracle):
+@pytest.mark.parametrize("builder", ["lbvh", "ploc"])
+def test_full_raycast_workload_properties(oracle, builder):
     """The benchmark scene (~2.19M triangles) with all 16.8M rays."""
     import torch
     verts, tris, coherent, incoherent, shadow = scenes.raycast_workload()
     say(f"full tris={len(tris)} rays={len(coherent) + len(incoherent)}")
-    acc = GpuAccelerator(verts, tris)
+    acc = GpuAccelerator(verts, tris, builder=builder)
     info = acc.info()
     say(f"  nodes={info['n_nodes']} depth={info['max_depth']} build_ms={info['build_ms']:.1f}")
     for name, rays in (("coherent", coherent), ("incoherent", incoherent)):
