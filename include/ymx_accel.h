@@ -72,7 +72,7 @@ typedef struct ymx_accel ymx_accel;
 typedef struct ymx_accel_params
 {
 	uint32_t struct_size;   /* sizeof(ymx_accel_params) */
-	int32_t max_leaf_size;  /* triangles per leaf after collapse (1..8), 0 = default */
+	int32_t max_leaf_size;  /* LBVH: at most this many triangles per leaf (1..8), 0 = default (4) */
 	int32_t builder;        /* YMX_BUILDER_LBVH (0, default) or YMX_BUILDER_PLOC */
 	int32_t reserved[5];
 } ymx_accel_params;

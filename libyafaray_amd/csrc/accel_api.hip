@@ -54,7 +54,9 @@ int ymx_accel_create(const float *verts, uint32_t n_verts, const uint32_t *tri_i
 		ymx_set_error("ymx_accel_create: %u triangles exceed the limit of %u", n_tris, YMX_LEAF_MAX_FIRST);
 		return YMX_ERR_UNSUPPORTED;
 	}
-	int max_leaf = 1, builder = YMX_BUILDER_LBVH;
+	// 4 triangles per leaf: on the 2.19M-triangle heightfield +2 % incoherent
+	// and +14 % coherent closest hit over 1 (LBVH; PLOC keeps 1)
+	int max_leaf = 4, builder = YMX_BUILDER_LBVH;
 	if(params)
 	{
 		if(params->struct_size != sizeof(ymx_accel_params))

@@ -155,7 +155,7 @@ __device__ __forceinline__ int karras_delta(const uint64_t *__restrict__ keys, i
 __global__ __launch_bounds__(kBuildBlock) void k_karras(const uint64_t *__restrict__ keys, int n,
                                                         float4 *__restrict__ nodes,
                                                         int32_t *__restrict__ parent_internal,
-                                                        int32_t *__restrict__ parent_leaf)
+                                                        int32_t *__restrict__ parent_leaf, uint2 *__restrict__ range)
 {
 	const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
 	if(i >= n - 1) return;
@@ -178,6 +178,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_karras(const uint64_t *__restri
 	const int gamma = i + s * d + (d < 0 ? -1 : 0);
 	const int lo = i < j ? i : j;
 	const int hi = i < j ? j : i;
+	range[i] = make_uint2((uint32_t)lo, (uint32_t)hi);
 	int32_t c0, c1;
 	if(lo == gamma)
 	{
@@ -272,6 +273,30 @@ __global__ __launch_bounds__(kBuildBlock) void k_refit(float4 *nodes, const floa
 		child_ref = node;
 		node = (node == 0) ? -1 : parent_internal[node];
 	}
+}
+
+// Leaves of up to max_leaf triangles: an LBVH node covers the contiguous leaf
+// range [lo, hi], so a child whose range is that small becomes one leaf over
+// those triangles (its box, set by refit, is already their union).  The
+// nodes below it are no longer reached.
+__global__ __launch_bounds__(kBuildBlock) void k_collapse_leaves(float4 *__restrict__ nodes,
+                                                                 const uint2 *__restrict__ range, uint32_t n,
+                                                                 uint32_t max_leaf)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i + 1 >= n) return;
+	float4 f3 = nodes[4 * (size_t)i + 3];
+	int32_t c[2] = {__float_as_int(f3.x), __float_as_int(f3.y)};
+	for(int k = 0; k < 2; ++k)
+	{
+		if(c[k] < 0) continue;
+		const uint2 r = range[c[k]];
+		const uint32_t size = r.y - r.x + 1u;
+		if(size <= max_leaf) c[k] = ymx_leaf_ref(r.x, size);
+	}
+	f3.x = __int_as_float(c[0]);
+	f3.y = __int_as_float(c[1]);
+	nodes[4 * (size_t)i + 3] = f3;
 }
 
 __global__ __launch_bounds__(kBuildBlock) void k_depth(const int32_t *__restrict__ parent_internal,
@@ -482,7 +507,6 @@ void reference_tree_bound(const float lo_in[3], const float hi_in[3], uint32_t n
 int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const uint32_t *idx,
                      const uint8_t *vis, uint32_t n_tris, int max_leaf, int builder)
 {
-	(void)max_leaf;
 	acc->n_tris = n_tris;
 	acc->build_ms = 0.f;
 	hipStream_t stream = nullptr;
@@ -629,17 +653,24 @@ int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const
 	const float pad = ext * 0x1p-20f + 1e-30f;
 	if(builder != YMX_BUILDER_PLOC)
 	{
-		DevBuf d_flags;
+		DevBuf d_flags, d_range;
 		YMX_HIP_CHECK(d_flags.alloc(sizeof(uint32_t) * (size_t)(n_tris - 1)));
+		YMX_HIP_CHECK(d_range.alloc(sizeof(uint2) * (size_t)(n_tris - 1)));
 		k_karras<<<grid_for(n_tris - 1), kBuildBlock, 0, stream>>>(keys, (int)n_tris, d_nodes.as<float4>(),
 		                                                           d_parent_int.as<int32_t>(),
-		                                                           d_parent_leaf.as<int32_t>());
+		                                                           d_parent_leaf.as<int32_t>(), d_range.as<uint2>());
 		YMX_HIP_CHECK(hipGetLastError());
 		YMX_HIP_CHECK(hipMemsetAsync(d_flags.p, 0, sizeof(uint32_t) * (size_t)(n_tris - 1), stream));
 		k_refit<<<grid_for(n_tris), kBuildBlock, 0, stream>>>(d_nodes.as<float4>(), d_prim_box.as<float>(), order,
 		                                                      n_tris, d_parent_int.as<int32_t>(),
 		                                                      d_parent_leaf.as<int32_t>(), d_flags.as<uint32_t>(), pad);
 		YMX_HIP_CHECK(hipGetLastError());
+		if(max_leaf > 1)
+		{
+			k_collapse_leaves<<<grid_for(n_tris - 1), kBuildBlock, 0, stream>>>(d_nodes.as<float4>(), d_range.as<uint2>(),
+			                                                                    n_tris, (uint32_t)max_leaf);
+			YMX_HIP_CHECK(hipGetLastError());
+		}
 	}
 	else
 	{
