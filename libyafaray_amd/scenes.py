@@ -488,3 +488,48 @@ def object_light_scene(n_random: int = 2000, seed: int = 0, res: int = 64, spp: 
                     background=None, seed=seed, raydepth=2)
     return dict(verts=verts, tris=tris, mat_id=mat_id, vis=vis, materials=materials, lights=lights,
                 camera=camera, settings=settings)
+
+
+def terrain_spheres_scene(res=(1920, 1080), spp: int = 64, seed: int = 0, hf_res: int = 1024, n_spheres: int = 500,
+                          subdiv: int = 3, n_point_lights: int = 64, depth: int = 8):
+    """Mixed-material path-tracing workload: the ray-cast heightfield
+    (`hf_res`^2 vertices) with `n_spheres` icospheres above it (8 * 4^subdiv
+    triangles each), sphere materials drawn 40 % shinydiffuse / 30 % glossy
+    (roughness r ~ U[0.05, 0.5], Blinn exponent 2 / r^2 - 2) / 15 % mirror /
+    15 % glass (IOR 1.5), `n_point_lights` point lights of power log-U[1, 100],
+    one area light and a constant background of 0.1; path depth `depth`.  The
+    defaults are the full frame; tests use a small one."""
+    rng = np.random.default_rng(seed)
+    hf = grid_mesh(fractal_heights(hf_res, seed=seed))
+    parts, mats = [hf], [0]
+    materials = [dict(color=(0.5, 0.45, 0.4))]
+    for k in range(n_spheres):
+        c = (float(rng.uniform(0.1, 0.9)), float(rng.uniform(0.1, 0.9)), float(rng.uniform(0.3, 0.5)))
+        parts.append(icosphere_tris(c, float(rng.uniform(0.02, 0.05)), subdiv))
+        mats.append(len(materials))
+        u, a = rng.uniform(), float(rng.uniform(0.2, 0.8))
+        if u < 0.4:
+            materials.append(dict(color=(a, a, a)))
+        elif u < 0.7:
+            r = float(rng.uniform(0.05, 0.5))
+            materials.append(dict(type="glossy", color=(a, 0.9 * a, 0.8 * a), exponent=2.0 / (r * r) - 2.0,
+                                  diffuse_reflect=float(rng.uniform(0.0, 0.6)), diffuse_color=(0.8 * a, a, a),
+                                  glossy_reflect=float(rng.uniform(0.2, 1.0))))
+        elif u < 0.85:
+            materials.append(dict(type="mirror", color=(a, a, 0.9 * a), reflect=float(rng.uniform(0.5, 1.0))))
+        else:
+            materials.append(dict(type="glass", IOR=1.5))
+    verts, tris = merge(parts)
+    mat_id = np.concatenate([np.full(len(t), m, np.uint32) for (_, t), m in zip(parts, mats)])
+    lights = [dict(type="area", corner=(0.4, 0.4, 1.2), point1=(0.4, 0.6, 1.2), point2=(0.6, 0.4, 1.2),
+                   color=(1.0, 1.0, 1.0), power=20.0 / np.pi, samples=4)]
+    for k in range(n_point_lights):
+        p = (float(rng.uniform(0.0, 1.0)), float(rng.uniform(0.0, 1.0)), float(rng.uniform(0.6, 1.0)))
+        lights.append(dict(type="point", **{"from": p}, color=(1.0, 0.95, 0.9),
+                           power=float(np.exp(rng.uniform(0.0, np.log(100.0))))))
+    camera = dict(from_=(0.5, -0.6, 0.9), to=(0.5, 0.55, 0.15), up=(0.5, -0.6, 1.9),
+                  focal=float(1.0 / (2.0 * np.tan(np.radians(30.0)))), resx=int(res[0]), resy=int(res[1]))
+    settings = dict(path_samples=1, bounces=depth, rr_min_bounces=3, aa_samples=spp, aa_pixelwidth=1.5,
+                    background=(0.1, 0.1, 0.1), seed=seed, raydepth=depth)
+    return dict(verts=verts, tris=tris, mat_id=mat_id, materials=materials, lights=lights, camera=camera,
+                settings=settings)

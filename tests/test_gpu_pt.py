@@ -252,3 +252,15 @@ def test_film_save_load_in_reference_format(oracle, tmp_path):
     assert np.array_equal(r4, r3) and np.array_equal(w4, w3)
     other.close()
     pt.close()
+
+
+def test_terrain_spheres_matches_oracle(oracle):
+    """The mixed-material terrain workload at a small size: heightfield plus
+    icospheres with the shinydiffuse / glossy / mirror / glass mix, 16 point
+    lights, an area light, background 0.1, depth 8."""
+    sc = scenes.terrain_spheres_scene(res=(96, 54), spp=4, hf_res=64, n_spheres=40, subdiv=1, n_point_lights=16,
+                                      seed=3)
+    say(f"pt terrain + spheres 96x54x4, {len(sc['tris'])} tris, {len(sc['lights'])} lights")
+    g = render_gpu(sc)
+    say(f"  stats {g[2]}")
+    compare_films(g[:2], render_cpu(oracle, sc), "terrain + spheres")
