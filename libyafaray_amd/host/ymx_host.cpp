@@ -291,6 +291,7 @@ size_t Scene::createObject(const std::string &name, const ParamMap &params)
 	else if(vis == "shadow_only") o.visibility = YMX_VIS_CASTS_SHADOWS;
 	else if(vis == "no_shadows") o.visibility = YMX_VIS_VISIBLE;
 	objects_.push_back(o);
+	modified_ |= ModifiedObjects;
 	return objects_.size() - 1;
 }
 
@@ -298,6 +299,7 @@ size_t Scene::addVertex(size_t object_id, double x, double y, double z)
 {
 	if(object_id >= objects_.size()) return (size_t)-1;
 	std::vector<float> &v = objects_[object_id].verts;
+	modified_ |= ModifiedObjects;
 	v.push_back((float)x);
 	v.push_back((float)y);
 	v.push_back((float)z);
@@ -312,6 +314,7 @@ bool Scene::addTriangle(size_t object_id, size_t a, size_t b, size_t c, size_t m
 	if(a >= nv || b >= nv || c >= nv) return false;
 	o.tris.insert(o.tris.end(), {(uint32_t)a, (uint32_t)b, (uint32_t)c});
 	o.mats.push_back((uint32_t)material_id);
+	modified_ |= ModifiedObjects;
 	return true;
 }
 
@@ -355,6 +358,7 @@ bool Scene::createMaterial(size_t *id, const std::string &name, const ParamMap &
 	materials_.push_back(m);
 	material_names_.push_back(name);
 	if(id) *id = materials_.size() - 1;
+	modified_ |= ModifiedMaterials;
 	return true;
 }
 
@@ -397,12 +401,47 @@ bool Scene::createLight(const std::string &name, const ParamMap &p, std::string 
 		return false;
 	}
 	lights_.emplace_back(l, object);
+	modified_ |= ModifiedLights;
 	return true;
 }
 
 // ---------------------------------------------------------------- PathIntegrator
 
-std::unique_ptr<PathIntegrator> PathIntegrator::create(const Scene &scene, const ParamMap &integrator,
+bool Scene::preprocess(uint32_t flags, std::string *error)
+{
+	if(accel_ && !(flags & (ModifiedObjects | ModifiedAcceleratorParams))) return true;
+	flat_verts_.clear();
+	flat_tris_.clear();
+	flat_mats_.clear();
+	flat_vis_.clear();
+	ranges_.clear();
+	for(const Object &o : objects_)
+	{
+		const uint32_t v0 = (uint32_t)(flat_verts_.size() / 3), t0 = (uint32_t)flat_mats_.size();
+		flat_verts_.insert(flat_verts_.end(), o.verts.begin(), o.verts.end());
+		for(uint32_t i : o.tris) flat_tris_.push_back(v0 + i);
+		flat_mats_.insert(flat_mats_.end(), o.mats.begin(), o.mats.end());
+		flat_vis_.insert(flat_vis_.end(), o.mats.size(), o.visibility);
+		ranges_[o.name] = {t0, (uint32_t)o.mats.size()};
+	}
+	ParamMap p = accel_params_;
+	if(!p.has("type")) p.setString("type", "mi355x-lbvh");
+	// the clustered tree traces the path tracer's rays 4-7 % faster
+	if(!p.has("builder")) p.setString("builder", "ploc");
+	std::string aerr;
+	std::shared_ptr<const Accelerator> a = Accelerator::factory(p, flat_verts_.data(), (uint32_t)(flat_verts_.size() / 3),
+	                                                            flat_tris_.data(), flat_vis_.data(),
+	                                                            (uint32_t)flat_mats_.size(), &aerr);
+	if(!a)
+	{
+		if(error) *error = "Scene: " + aerr;
+		return false;
+	}
+	accel_ = std::move(a);
+	return true;
+}
+
+std::unique_ptr<PathIntegrator> PathIntegrator::create(Scene &scene, const ParamMap &integrator,
                                                        const ParamMap &camera, const ParamMap &film,
                                                        const ParamMap *background, std::string *error)
 {
@@ -412,20 +451,13 @@ std::unique_ptr<PathIntegrator> PathIntegrator::create(const Scene &scene, const
 	};
 	if(integrator.getString("type", "pathtracing") != "pathtracing") return fail("only the 'pathtracing' type is built");
 	if(camera.getString("type", "perspective") != "perspective") return fail("only the 'perspective' camera is built");
-	// flatten the objects; an object light covers its object's triangle range
-	std::vector<float> verts;
-	std::vector<uint32_t> tris, mats;
-	std::vector<uint8_t> vis;
-	std::map<std::string, std::pair<uint32_t, uint32_t>> ranges;
-	for(const Scene::Object &o : scene.objects_)
-	{
-		const uint32_t v0 = (uint32_t)(verts.size() / 3), t0 = (uint32_t)mats.size();
-		verts.insert(verts.end(), o.verts.begin(), o.verts.end());
-		for(uint32_t i : o.tris) tris.push_back(v0 + i);
-		mats.insert(mats.end(), o.mats.begin(), o.mats.end());
-		vis.insert(vis.end(), o.mats.size(), o.visibility);
-		ranges[o.name] = {t0, (uint32_t)o.mats.size()};
-	}
+	std::string perr;
+	if(!scene.preprocess(scene.checkAndClearModifiedFlags(), &perr)) return fail(perr);
+	const std::vector<float> &verts = scene.flat_verts_;
+	const std::vector<uint32_t> &tris = scene.flat_tris_, &mats = scene.flat_mats_;
+	const std::vector<uint8_t> &vis = scene.flat_vis_;
+	const auto &ranges = scene.ranges_;
+	// an object light covers its object's triangle range
 	std::vector<ymx_light> lights;
 	for(const auto &[l, object] : scene.lights_)
 	{
@@ -440,14 +472,7 @@ std::unique_ptr<PathIntegrator> PathIntegrator::create(const Scene &scene, const
 		lights.push_back(x);
 	}
 	std::unique_ptr<PathIntegrator> pi(new PathIntegrator());
-	ParamMap accel_params = scene.accel_params_;
-	if(!accel_params.has("type")) accel_params.setString("type", "mi355x-lbvh");
-	// the clustered tree traces the path tracer's rays 4-7 % faster
-	if(!accel_params.has("builder")) accel_params.setString("builder", "ploc");
-	std::string aerr;
-	pi->accel_ = Accelerator::factory(accel_params, verts.data(), (uint32_t)(verts.size() / 3), tris.data(), vis.data(),
-	                                  (uint32_t)mats.size(), &aerr);
-	if(!pi->accel_) return fail(aerr);
+	pi->accel_ = scene.accelerator();
 	bool ok = true;
 	ymx_pt_settings &s = pi->settings_;
 	set3(s.from, camera.getVector("from", {0, 1, 0}, &ok));

@@ -164,7 +164,36 @@ class Scene
 		bool createMaterial(size_t *id, const std::string &name, const ParamMap &params, std::string *error);
 		bool createLight(const std::string &name, const ParamMap &params, std::string *error);
 		// "type" "mi355x-lbvh" (the only GPU type; others are refused).
-		void setAcceleratorParams(const ParamMap &params) { accel_params_ = params; }
+		void setAcceleratorParams(const ParamMap &params)
+		{
+			accel_params_ = params;
+			modified_ |= ModifiedAcceleratorParams;
+		}
+
+		// Modified flags, as yafaray_SceneModifiedFlags
+		// (include/public_api/yafaray_c_api.h:47-57): the kinds of change since
+		// the last checkAndClearModifiedFlags (Scene::checkAndClearSceneModifiedFlags).
+		enum : uint32_t
+		{
+			ModifiedNothing = 0,
+			ModifiedObjects = 1u << 0,
+			ModifiedLights = 1u << 1,
+			ModifiedMaterials = 1u << 2,
+			ModifiedAcceleratorParams = 1u << 6,
+			ModifiedAll = ModifiedObjects | ModifiedLights | ModifiedMaterials | ModifiedAcceleratorParams
+		};
+		uint32_t checkAndClearModifiedFlags()
+		{
+			const uint32_t m = modified_;
+			modified_ = ModifiedNothing;
+			return m;
+		}
+		// Scene::preprocess (src/scene/scene.cc:314-362): with modified objects or
+		// accelerator params (or no accelerator yet), flatten the objects into the
+		// device arrays and rebuild the accelerator (builder "ploc" unless the
+		// params name one); otherwise keep both.  false and `error` on failure.
+		bool preprocess(uint32_t flags, std::string *error);
+		std::shared_ptr<const Accelerator> accelerator() const { return accel_; }
 
 	private:
 		friend class PathIntegrator;
@@ -180,6 +209,13 @@ class Scene
 		std::vector<std::string> material_names_;
 		std::vector<std::pair<ymx_light, std::string>> lights_;  // object lights keep their object name
 		ParamMap accel_params_;
+		uint32_t modified_ = ModifiedAll;
+		// state of the last preprocess
+		std::shared_ptr<const Accelerator> accel_;
+		std::vector<float> flat_verts_;
+		std::vector<uint32_t> flat_tris_, flat_mats_;
+		std::vector<uint8_t> flat_vis_;
+		std::map<std::string, std::pair<uint32_t, uint32_t>> ranges_;  // object name -> (first triangle, count)
 };
 
 class PathIntegrator
@@ -191,7 +227,9 @@ class PathIntegrator
 		// "up", "resx", "resy", "focal", "aspect_ratio_factor"), film params
 		// ("AA_minsamples", "AA_pixelwidth") and an optional constant
 		// background "color".  Builds the accelerator and the wavefront state.
-		static std::unique_ptr<PathIntegrator> create(const Scene &scene, const ParamMap &integrator,
+		// Preprocesses the scene with its modified flags first (the accelerator is
+		// shared with the scene and rebuilt only when its objects changed).
+		static std::unique_ptr<PathIntegrator> create(Scene &scene, const ParamMap &integrator,
 		                                              const ParamMap &camera, const ParamMap &film,
 		                                              const ParamMap *background, std::string *error);
 		~PathIntegrator();
@@ -218,7 +256,7 @@ class PathIntegrator
 
 	private:
 		PathIntegrator() = default;
-		std::unique_ptr<Accelerator> accel_;
+		std::shared_ptr<const Accelerator> accel_;
 		ymx_pt *pt_ = nullptr;
 		ymx_pt_settings settings_{};
 		std::string error_;

@@ -461,6 +461,57 @@ static void test_pathtracer()
 	unlink(path);
 }
 
+// Scene::preprocess keeps the accelerator unless objects or accelerator
+// params changed (src/scene/scene.cc:314-362).
+static void test_scene_preprocess()
+{
+	Scene sc;
+	std::string err;
+	ParamMap m;
+	m.setString("type", "shinydiffusemat");
+	size_t mat = 0;
+	CHECK(sc.createMaterial(&mat, "grey", m, &err), "material: %s", err.c_str());
+	ParamMap normal;
+	const size_t obj = sc.createObject("quad", normal);
+	const double q[4][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}};
+	for(const auto &v : q) sc.addVertex(obj, v[0], v[1], v[2]);
+	sc.addTriangle(obj, 0, 1, 2, mat);
+	sc.addTriangle(obj, 0, 2, 3, mat);
+	uint32_t flags = sc.checkAndClearModifiedFlags();
+	CHECK((flags & Scene::ModifiedObjects) && (flags & Scene::ModifiedMaterials), "creation flags %x", flags);
+	CHECK(sc.checkAndClearModifiedFlags() == Scene::ModifiedNothing, "flags cleared");
+	CHECK(sc.preprocess(flags, &err), "preprocess: %s", err.c_str());
+	const auto a0 = sc.accelerator();
+	CHECK(a0 && a0->numPrimitives() == 2, "two primitives");
+	size_t mat2 = 0;
+	CHECK(sc.createMaterial(&mat2, "red", m, &err), "material: %s", err.c_str());
+	flags = sc.checkAndClearModifiedFlags();
+	CHECK(flags == Scene::ModifiedMaterials, "material flag %x", flags);
+	CHECK(sc.preprocess(flags, &err) && sc.accelerator() == a0, "a material change keeps the accelerator");
+	sc.addVertex(obj, 0.5, 0.5, 1.0);
+	sc.addTriangle(obj, 0, 1, 4, mat2);
+	flags = sc.checkAndClearModifiedFlags();
+	CHECK(flags == Scene::ModifiedObjects, "object flag %x", flags);
+	CHECK(sc.preprocess(flags, &err), "preprocess: %s", err.c_str());
+	const auto a1 = sc.accelerator();
+	CHECK(a1 && a1 != a0 && a1->numPrimitives() == 3, "an object change rebuilds the accelerator");
+	ParamMap ap;
+	ap.setString("type", "mi355x-lbvh");
+	ap.setString("builder", "lbvh");
+	sc.setAcceleratorParams(ap);
+	flags = sc.checkAndClearModifiedFlags();
+	CHECK(flags == Scene::ModifiedAcceleratorParams && sc.preprocess(flags, &err) && sc.accelerator() != a1,
+	      "new accelerator params rebuild it");
+	Ray r;
+	r.from[0] = 0.5f;
+	r.from[1] = 0.25f;
+	r.from[2] = 2.f;  // the new triangle lies on z = 2y, above the quad at z = 0
+	r.dir[2] = -1.f;
+	const IntersectData h = sc.accelerator()->intersect(r, FLT_MAX);
+	CHECK(h.isHit() && h.primitive == 2 && fabsf(h.t_hit - 1.5f) < 1e-6f, "the added triangle is hit first (%u, %g)",
+	      h.primitive, h.t_hit);
+}
+
 static void test_errors()
 {
 	std::string err;
@@ -518,6 +569,7 @@ int main(int argc, char **argv)
 	             {"accel_shadow_ploc", [] { g_builder = "ploc"; test_accel_shadow(); g_builder = "lbvh"; }, true},
 	             {"accel_shadow", test_accel_shadow, true},
 	             {"accel_transparent", test_accel_transparent, true},
+	             {"scene_preprocess", test_scene_preprocess, true},
 	             {"pathtracer", test_pathtracer, true}};
 	for(const auto &t : tests)
 	{
