@@ -90,6 +90,9 @@ int ymx_trace_setup(ymx_accel *acc);
 int ymx_launch_transparent(const ymx_accel *acc, const ymx_ray *rays, const uint32_t *transp, uint32_t max_depth,
                            uint32_t *occ, uint32_t *counts, uint2 *hits, uint32_t n, const uint32_t *n_dev,
                            hipStream_t stream);
+// Closest hit of n coherent rays with the wave-ordered kernel (no ray replacement).
+int ymx_launch_closest_ordered(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t n,
+                               hipStream_t stream);
 // Queue variants: the ray count is read on the device from *n_dev (<= capacity).
 int ymx_launch_closest_queue(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, const uint32_t *n_dev,
                              uint32_t capacity, hipStream_t stream);

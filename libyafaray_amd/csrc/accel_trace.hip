@@ -602,6 +602,12 @@ int ymx_launch_closest(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits,
 	return launch<kClosest, false, true>(acc, rays, hits, nullptr, nullptr, n, stream, nullptr);
 }
 
+int ymx_launch_closest_ordered(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t n,
+                               hipStream_t stream)
+{
+	return launch<kClosest, false>(acc, rays, hits, nullptr, nullptr, n, stream, nullptr);
+}
+
 int ymx_launch_closest_queue(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, const uint32_t *n_dev,
                              uint32_t capacity, hipStream_t stream)
 {

@@ -1807,7 +1807,8 @@ int trace_chunk(ymx_pt *pt, PtArgs &a, hipStream_t stream)
 	const uint32_t slot_cap = n * pt->st.slots;
 	k_count_rays<<<1, 64, 0, stream>>>(pt->st.totals, n);
 	YMX_HIP_CHECK(hipGetLastError());
-	int rc = ymx_launch_closest(pt->accel, pt->nd.ray + a.base, pt->st.ext_hits, n, stream, nullptr);
+	// camera rays are coherent: the wave-ordered kernel (ray replacement costs there)
+	int rc = ymx_launch_closest_ordered(pt->accel, pt->nd.ray + a.base, pt->st.ext_hits, n, stream);
 	if(rc != YMX_OK) return rc;
 	k_shade_node<<<g, kBlock, 0, stream>>>(a);
 	YMX_HIP_CHECK(hipGetLastError());
