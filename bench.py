@@ -53,12 +53,15 @@ def parse():
     p.add_argument("--guided-only", action="store_true", help="only the guided path-tracing measurement")
     p.add_argument("--guided-res", type=str, default="1280x720")
     p.add_argument("--guided-ref-spp", type=int, default=4096)
-    p.add_argument("--guided-spp", type=int, default=256)
+    p.add_argument("--guided-spp", type=int, default=640,
+                   help="samples of each tracer; near the 1e-3 target so that little is extrapolated")
     p.add_argument("--guided-bsdf-prob", type=float, default=0.5)
     p.add_argument("--guided-steps", type=int, default=16, help="Adam steps after each training pass")
     p.add_argument("--guided-lr", type=float, default=1e-2)
     p.add_argument("--guided-depth", type=int, default=GUIDE_DEPTH,
                    help="path vertices after the camera vertex that are guided (-1: all)")
+    p.add_argument("--guided-verify", action="store_true",
+                   help="also render both tracers to the predicted sample counts and measure the relMSE reached")
     p.add_argument("--guided-train-passes", type=int, default=8,
                    help="training passes over the first 32 spp (32 / passes spp each)")
     return p.parse_args()
@@ -211,10 +214,43 @@ def measure_guiding(args, torch):
     rgba_c, w_c = p.film()
     e_all = relmse(ratio(rgba_c, w_c), ref) - ref_err
     e_rest = relmse(ratio(rgba_c.astype(np.float64) - rgba_t, w_c.astype(np.float64) - w_t), ref) - ref_err
+    target = 1e-3
+    verify = None
+    if getattr(args, "guided_verify", False):
+        # render each tracer to its predicted sample count and measure the relMSE it reaches
+        v_t0, v_g0 = e_train * n0, e_rest * m
+        a0, b0, c0 = target, 2 * target * n0 - v_g0, target * n0 * n0 - n0 * v_t0
+        m_v = 0 if c0 >= 0 else int(np.ceil((-b0 + (b0 * b0 - 4 * a0 * c0) ** 0.5) / (2 * a0)))
+        n_plain = int(np.ceil(spp * e_plain / target))
+        pv = tracer(n_plain, 0)
+        pv.clear()
+        t_pv = timed(lambda: pv.render(0, n_plain))
+        e_pv = relmse(image(pv), ref) - ref_err
+        pv.close()
+        g2 = GpuGuide(bmin, bmax, log2_table_size=19, n_min=n_min, n_max=n_max, max_batch=1 << 16,
+                      lr=args.guided_lr, seed=1)
+        pg = tracer(n0 + m_v, 0)  # a tracer whose AA sample count covers every sample rendered
+        pg.set_guide_depth(None if depth < 0 else depth)
+        pg.set_guide(g2, bsdf_prob=alpha, train=True, train_steps=steps)
+        pg.clear()
+
+        def train_v():
+            for k in range(passes):
+                pg.render(k * (n0 // passes), n0 // passes)
+
+        t_gv = timed(train_v)
+        pg.set_guide(g2, bsdf_prob=alpha, train=False)
+        t_gv += timed(lambda: pg.render(n0, m_v) if m_v > 0 else None)
+        e_gv = relmse(image(pg), ref) - ref_err
+        pg.set_guide(None)
+        pg.close()
+        g2.close()
+        verify = {"plain_spp": n_plain, "plain_s": round(t_pv, 3), "plain_relmse": e_pv,
+                  "guided_spp": n0 + m_v, "guided_s": round(t_gv, 3), "guided_relmse": e_gv,
+                  "guided_over_plain_time": round(t_gv / t_pv, 3) if t_pv > 0 else None}
     p.set_guide(None)
     g.close()
     p.close()
-    target = 1e-3
     tt_plain = t_plain * e_plain / target
     v_t, v_g = e_train * n0, e_rest * m
     # smallest m with (n0 v_t + m v_g) / (n0 + m)^2 <= target (a quadratic in m)
@@ -236,6 +272,8 @@ def measure_guiding(args, torch):
            "per_sample_gain_after_training": round((e_plain * spp) / v_g, 3) if v_g > 0 else None,
            "time_to_relmse_1e-3_plain_s": round(tt_plain, 2), "time_to_relmse_1e-3_guided_s": round(tt_guided, 2),
            "guided_over_plain_time": round(tt_guided / tt_plain, 3) if tt_plain > 0 else None}
+    if verify is not None:
+        out["verify_at_predicted_spp"] = verify
     log(args, f"guided path tracing: {out}")
     return out
 
