@@ -9,7 +9,11 @@ rays with tmax uniform in (0, 1].
 
 A step is one closest-hit pass over the 8,388,608 incoherent rays, with rays
 and BVH already resident in HBM.  The coherent closest-hit and incoherent
-any-hit rates are reported alongside.  Prints one JSON line.
+any-hit rates are reported alongside.  The same JSON line carries, under
+`extra`, the path-traced Cornell frame (256 x 256, 16 spp) beside the CPU
+restatement of the reference's integrator, the guided vs plain time to
+relMSE 1e-3 on the occluded room (1280 x 720), and the guiding network's
+kernels on their own.  Prints one JSON line.
 """
 from __future__ import annotations
 
