@@ -290,6 +290,7 @@ size_t Scene::createObject(const std::string &name, const ParamMap &params)
 	if(vis == "invisible") o.visibility = 0;
 	else if(vis == "shadow_only") o.visibility = YMX_VIS_CASTS_SHADOWS;
 	else if(vis == "no_shadows") o.visibility = YMX_VIS_VISIBLE;
+	o.base = params.getBool("is_base_object", false);
 	objects_.push_back(o);
 	modified_ |= ModifiedObjects;
 	return objects_.size() - 1;
@@ -407,6 +408,32 @@ bool Scene::createLight(const std::string &name, const ParamMap &p, std::string 
 
 // ---------------------------------------------------------------- PathIntegrator
 
+size_t Scene::createInstance()
+{
+	instances_.emplace_back();
+	modified_ |= ModifiedObjects;
+	return instances_.size() - 1;
+}
+
+bool Scene::addInstanceObject(size_t instance_id, size_t base_object_id)
+{
+	if(instance_id >= instances_.size() || base_object_id >= objects_.size()) return false;
+	instances_[instance_id].objects.push_back(base_object_id);
+	modified_ |= ModifiedObjects;
+	return true;
+}
+
+bool Scene::addInstanceMatrix(size_t instance_id, const double m[16], float time)
+{
+	(void)time;
+	if(instance_id >= instances_.size() || !m) return false;
+	std::array<float, 16> f;
+	for(int k = 0; k < 16; ++k) f[k] = (float)m[k];  // Matrix4f of the reference
+	instances_[instance_id].matrices.push_back(f);
+	modified_ |= ModifiedObjects;
+	return true;
+}
+
 bool Scene::preprocess(uint32_t flags, std::string *error)
 {
 	if(accel_ && !(flags & (ModifiedObjects | ModifiedAcceleratorParams))) return true;
@@ -415,14 +442,41 @@ bool Scene::preprocess(uint32_t flags, std::string *error)
 	flat_mats_.clear();
 	flat_vis_.clear();
 	ranges_.clear();
-	for(const Object &o : objects_)
-	{
-		const uint32_t v0 = (uint32_t)(flat_verts_.size() / 3), t0 = (uint32_t)flat_mats_.size();
-		flat_verts_.insert(flat_verts_.end(), o.verts.begin(), o.verts.end());
+	// Scene::preprocess skips base objects (src/scene/scene.cc:322-326); they
+	// enter through their instances
+	auto append = [&](const Object &o, const float *m) {
+		const uint32_t v0 = (uint32_t)(flat_verts_.size() / 3);
+		if(!m) flat_verts_.insert(flat_verts_.end(), o.verts.begin(), o.verts.end());
+		else
+			for(size_t v = 0; v + 2 < o.verts.size(); v += 3)
+				for(int i = 0; i < 3; ++i)
+				{
+					// include/geometry/matrix.h:131-143: sum_j m[i][j] p[j], then + m[i][3]
+					float a = 0.f;
+					for(int j = 0; j < 3; ++j) a += m[4 * i + j] * o.verts[v + j];
+					a += m[4 * i + 3];
+					flat_verts_.push_back(a);
+				}
 		for(uint32_t i : o.tris) flat_tris_.push_back(v0 + i);
 		flat_mats_.insert(flat_mats_.end(), o.mats.begin(), o.mats.end());
 		flat_vis_.insert(flat_vis_.end(), o.mats.size(), o.visibility);
+	};
+	for(const Object &o : objects_)
+	{
+		if(o.base) continue;
+		const uint32_t t0 = (uint32_t)flat_mats_.size();
+		append(o, nullptr);
 		ranges_[o.name] = {t0, (uint32_t)o.mats.size()};
+	}
+	for(const Instance &in : instances_)
+	{
+		if(in.matrices.size() > 1)
+		{
+			if(error) *error = "Scene: an instance with several matrices (motion blur) is not built";
+			return false;
+		}
+		if(in.matrices.empty()) continue;
+		for(size_t id : in.objects) append(objects_[id], in.matrices[0].data());
 	}
 	ParamMap p = accel_params_;
 	if(!p.has("type")) p.setString("type", "mi355x-lbvh");

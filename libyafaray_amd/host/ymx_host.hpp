@@ -154,8 +154,19 @@ class Scene
 {
 	public:
 		// Returns the object id; params: "visibility" ("normal", "no_shadows",
-		// "shadow_only", "invisible"), as src/geometry/object/object.cc.
+		// "shadow_only", "invisible"), "is_base_object" (only rendered through
+		// instances), as src/geometry/object/object.cc.
 		size_t createObject(const std::string &name, const ParamMap &params);
+		// Instances (yafaray_createInstance / yafaray_addInstanceObject /
+		// yafaray_addInstanceMatrix, include/public_api/yafaray_c_api.h:227-231):
+		// copies of base objects under an object-to-world matrix (row-major m[4][4]).
+		// preprocess flattens each copy into the triangle list, transforming its
+		// vertices as Matrix4f * Point3f does (src/geometry/primitive/primitive_face.h,
+		// include/geometry/matrix.h:131-143), so hits equal the reference's.  Not
+		// built: instances of instances and motion blur (several matrices).
+		size_t createInstance();
+		bool addInstanceObject(size_t instance_id, size_t base_object_id);
+		bool addInstanceMatrix(size_t instance_id, const double m[16], float time);
 		size_t addVertex(size_t object_id, double x, double y, double z);
 		bool addTriangle(size_t object_id, size_t a, size_t b, size_t c, size_t material_id);
 		// Material params (type "shinydiffusemat", "glossy", "mirror", "glass",
@@ -201,10 +212,17 @@ class Scene
 		{
 			std::string name;
 			uint8_t visibility = YMX_VIS_VISIBLE | YMX_VIS_CASTS_SHADOWS;
+			bool base = false;
 			std::vector<float> verts;
 			std::vector<uint32_t> tris, mats;  // local vertex indices
 		};
 		std::vector<Object> objects_;
+		struct Instance
+		{
+			std::vector<size_t> objects;
+			std::vector<std::array<float, 16>> matrices;
+		};
+		std::vector<Instance> instances_;
 		std::vector<ymx_material> materials_;
 		std::vector<std::string> material_names_;
 		std::vector<std::pair<ymx_light, std::string>> lights_;  // object lights keep their object name

@@ -512,6 +512,52 @@ static void test_scene_preprocess()
 	      h.primitive, h.t_hit);
 }
 
+// Instances: base objects rendered only through instances, under their
+// object-to-world matrices.
+static void test_instances()
+{
+	Scene sc;
+	std::string err;
+	ParamMap m;
+	m.setString("type", "shinydiffusemat");
+	size_t mat = 0;
+	CHECK(sc.createMaterial(&mat, "grey", m, &err), "material: %s", err.c_str());
+	ParamMap base;
+	base.setBool("is_base_object", true);
+	const size_t obj = sc.createObject("tri", base);
+	sc.addVertex(obj, 0, 0, 0);
+	sc.addVertex(obj, 1, 0, 0);
+	sc.addVertex(obj, 0, 1, 0);
+	sc.addTriangle(obj, 0, 1, 2, mat);
+	const double up[16] = {2, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 1, 0, 0, 0, 1};  // x scaled by 2, lifted to z = 1
+	const double side[16] = {1, 0, 0, 5, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};  // moved to x = 5
+	const size_t i0 = sc.createInstance(), i1 = sc.createInstance();
+	CHECK(sc.addInstanceObject(i0, obj) && sc.addInstanceMatrix(i0, up, 0.f), "instance 0");
+	CHECK(sc.addInstanceObject(i1, obj) && sc.addInstanceMatrix(i1, side, 0.f), "instance 1");
+	CHECK(!sc.addInstanceObject(9, obj) && !sc.addInstanceMatrix(9, up, 0.f), "unknown instance refused");
+	CHECK(sc.preprocess(sc.checkAndClearModifiedFlags(), &err), "preprocess: %s", err.c_str());
+	const auto a = sc.accelerator();
+	CHECK(a && a->numPrimitives() == 2, "two instanced triangles, the base object left out");
+	if(!a) return;
+	auto down = [&](float x, float y) {
+		Ray r;
+		r.from[0] = x;
+		r.from[1] = y;
+		r.from[2] = 3.f;
+		r.dir[2] = -1.f;
+		return a->intersect(r, FLT_MAX);
+	};
+	const IntersectData h0 = down(1.5f, 0.2f), h1 = down(5.2f, 0.2f), h2 = down(0.2f, 0.9f), h3 = down(2.5f, 0.2f);
+	CHECK(h0.isHit() && h0.primitive == 0 && h0.t_hit == 2.f, "scaled instance hit at z = 1 (%u, %g)", h0.primitive, h0.t_hit);
+	CHECK(h1.isHit() && h1.primitive == 1 && h1.t_hit == 3.f, "moved instance hit at z = 0 (%u, %g)", h1.primitive, h1.t_hit);
+	CHECK(h2.isHit() && h2.primitive == 0, "inside the scaled copy");
+	CHECK(!h3.isHit(), "outside both copies");
+	const double two[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+	sc.addInstanceMatrix(i1, two, 1.f);
+	err.clear();
+	CHECK(!sc.preprocess(sc.checkAndClearModifiedFlags(), &err) && !err.empty(), "motion blur refused");
+}
+
 static void test_errors()
 {
 	std::string err;
@@ -570,6 +616,7 @@ int main(int argc, char **argv)
 	             {"accel_shadow", test_accel_shadow, true},
 	             {"accel_transparent", test_accel_transparent, true},
 	             {"scene_preprocess", test_scene_preprocess, true},
+	             {"instances", test_instances, true},
 	             {"pathtracer", test_pathtracer, true}};
 	for(const auto &t : tests)
 	{
