@@ -345,7 +345,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(ymx_trace_args args)
 // hit; on the path tracer's queues, where the lanes of a wave finish
 // together, mixing rays of different progress costs more than it gains, so
 // those use k_trace.
-constexpr int kRefill = 24;
+constexpr int kRefill = 12;
 
 template <int kMode, bool kStats>
 __global__ __launch_bounds__(kTraceBlock) void k_dyn(ymx_trace_args args)
