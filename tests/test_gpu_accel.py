@@ -297,3 +297,44 @@ def test_transparent_shadow_query(oracle, max_depth):
         assert set(hits[i, :k, 0].tolist()) == set(eprims[i, :k].tolist())
         t = hits[i, :k, 1].view(np.float32)
         assert np.all(np.diff(t) >= 0) and np.all(t > 0) and np.all(t < rays[i, 7])
+
+
+@pytest.mark.parametrize("builder", ["lbvh", "ploc"])
+def test_exported_tree_structure(builder):
+    """The exported tree is a tree over every triangle exactly once: node 0 is
+    the root, every other node has exactly one parent, refs stay in range.
+    LBVH nodes are in depth-first order with a node's internal children in
+    the next free adjacent slots (no node under a collapsed leaf is kept)."""
+    import ctypes as C
+    from libyafaray_amd import native
+    verts, tris = scenes.heightfield_scene(res=128, n_random=3000, seed=11)
+    acc = GpuAccelerator(verts, tris, builder=builder)
+    info = acc.info()
+    n_nodes, n_tris = info["n_nodes"], len(tris)
+    nodes = np.zeros((n_nodes, 16), dtype=np.float32)
+    tri_rec = np.zeros((max(1, n_tris), 12), dtype=np.float32)
+    native.check(native.lib().ymx_accel_export(acc.handle, nodes.ctypes.data, C.c_uint64(nodes.nbytes),
+                                               tri_rec.ctypes.data, C.c_uint64(tri_rec.nbytes)), "ymx_accel_export")
+    refs = nodes[:, 12:14].copy().view(np.int32)
+    internal = refs >= 0
+    assert np.all(refs[internal] < n_nodes)
+    parents = np.bincount(refs[internal], minlength=n_nodes)
+    assert parents[0] == 0 and np.all(parents[1:] == 1)
+    leaf = refs[~internal].view(np.uint32)
+    first, count = (leaf & 0x7FFFFFFF) >> 4, (leaf & 15) + 1
+    covered = np.zeros(n_tris, dtype=np.int64)
+    for f, c in zip(first, count):
+        covered[f:f + c] += 1
+    assert np.all(covered == 1)
+    ids = tri_rec[:, 3].copy().view(np.uint32)
+    assert np.array_equal(np.sort(ids), np.arange(n_tris, dtype=np.uint32))
+    if builder == "lbvh":
+        assert n_nodes < n_tris - 1  # collapsed subtrees dropped
+        nxt, stack = 1, [0]
+        while stack:
+            n = stack.pop()
+            ch = [int(r) for r in refs[n] if r >= 0]
+            assert ch == list(range(nxt, nxt + len(ch))), (n, ch, nxt)
+            nxt += len(ch)
+            stack.extend(reversed(ch))
+        assert nxt == n_nodes
