@@ -445,16 +445,35 @@ __global__ __launch_bounds__(kBuildBlock) void k_lay_parents(const float4 *__res
 	nint[i] = k;
 }
 
-// Internal nodes per subtree, bottom-up: the last child to finish sums its
-// parent (the arrival count of refit, over internal children only).
+__device__ __forceinline__ void lay_sums(const float4 *nodes, uint32_t n, uint32_t *size, uint32_t *tcount)
+{
+	const float4 f3 = nodes[4 * (size_t)n + 3];
+	const int32_t r[2] = {__float_as_int(f3.x), __float_as_int(f3.y)};
+	uint32_t s = 1u, t = 0u;
+	for(int k = 0; k < 2; ++k)
+	{
+		if(r[k] >= 0)
+		{
+			s += load_u32_coherent(&size[r[k]]);
+			if(tcount) t += load_u32_coherent(&tcount[r[k]]);
+		}
+		else if(r[k] != YMX_REF_EMPTY) t += ymx_leaf_count(r[k]);
+	}
+	__hip_atomic_store(&size[n], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	if(tcount) __hip_atomic_store(&tcount[n], t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Internal nodes (size) and, when tcount is given, triangles per subtree,
+// bottom-up: the last child to finish sums its parent (the arrival count of
+// refit, over internal children only).
 __global__ __launch_bounds__(kBuildBlock) void k_lay_size(const float4 *__restrict__ nodes, uint32_t n_int,
                                                           const int32_t *__restrict__ par,
                                                           const uint32_t *__restrict__ nint, uint32_t *arrive,
-                                                          uint32_t *size, uint32_t *bad)
+                                                          uint32_t *size, uint32_t *tcount, uint32_t *bad)
 {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if(i >= n_int || nint[i] != 0u) return;
-	__hip_atomic_store(&size[i], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	lay_sums(nodes, i, size, tcount);
 	uint32_t node = i;
 	for(uint32_t guard = 0; node != 0u; ++guard)
 	{
@@ -469,12 +488,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_lay_size(const float4 *__restri
 		const uint32_t old = atomicAdd(&arrive[p], 1u);
 		if(old + 1u < nint[p]) return;
 		__threadfence();
-		const float4 f3 = nodes[4 * (size_t)p + 3];
-		const int32_t r[2] = {__float_as_int(f3.x), __float_as_int(f3.y)};
-		uint32_t s = 1u;
-		for(int k = 0; k < 2; ++k)
-			if(r[k] >= 0) s += load_u32_coherent(&size[r[k]]);
-		__hip_atomic_store(&size[p], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		lay_sums(nodes, (uint32_t)p, size, tcount);
 		node = (uint32_t)p;
 	}
 }
@@ -555,6 +569,83 @@ __global__ __launch_bounds__(kBuildBlock) void k_lay_write(const float4 *__restr
 	dst[4 * (size_t)d + 1] = src[4 * (size_t)i + 1];
 	dst[4 * (size_t)d + 2] = src[4 * (size_t)i + 2];
 	dst[4 * (size_t)d + 3] = f3;
+}
+
+// PLOC leaves: triangles are rewritten in depth-first leaf order, so that
+// every subtree covers a contiguous run of records, and a child subtree of at
+// most max_leaf triangles becomes one leaf over its run.  With T(n) the first
+// record of n's subtree (T(root) = 0), a node's second child starts after the
+// triangles of its first: T(c1) = T(n) + tcount(c0).  New refs go to
+// new_refs; the tree itself is read only.
+__device__ __forceinline__ uint32_t ref_tris(int32_t r, const uint32_t *tcount)
+{
+	return r >= 0 ? tcount[r] : ymx_leaf_count(r);
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_ploc_leaves(const float4 *__restrict__ nodes, uint32_t n_int,
+                                                             const int32_t *__restrict__ par,
+                                                             const uint32_t *__restrict__ tcount,
+                                                             const float4 *__restrict__ tris_in, uint32_t n_tris,
+                                                             uint32_t max_leaf, float4 *__restrict__ tris_out,
+                                                             int2 *__restrict__ new_refs, uint32_t *bad)
+{
+	const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+	if(n >= n_int) return;
+	uint32_t first = 0u, c = n;
+	for(uint32_t guard = 0; c != 0u; ++guard)
+	{
+		const int32_t p = par[c];
+		if(p < 0 || guard > n_int)
+		{
+			atomicOr(bad, 1u);
+			return;
+		}
+		const float4 f3 = nodes[4 * (size_t)p + 3];
+		const int32_t r0 = __float_as_int(f3.x), r1 = __float_as_int(f3.y);
+		if(r1 == (int32_t)c && r0 != (int32_t)c) first += ref_tris(r0, tcount);
+		c = (uint32_t)p;
+	}
+	const float4 f3 = nodes[4 * (size_t)n + 3];
+	int32_t r[2] = {__float_as_int(f3.x), __float_as_int(f3.y)};
+	uint32_t off = first;
+	for(int k = 0; k < 2; ++k)
+	{
+		const uint32_t cnt = ref_tris(r[k], tcount);
+		if(off + cnt > n_tris)
+		{
+			atomicOr(bad, 1u);
+			return;
+		}
+		if(r[k] < 0)
+		{
+			// a one-triangle leaf of the clustering: its record moves to slot off
+			const uint32_t j = ymx_leaf_first(r[k]);
+			if(j >= n_tris || cnt != 1u)
+			{
+				atomicOr(bad, 1u);
+				return;
+			}
+			tris_out[3 * (size_t)off + 0] = tris_in[3 * (size_t)j + 0];
+			tris_out[3 * (size_t)off + 1] = tris_in[3 * (size_t)j + 1];
+			tris_out[3 * (size_t)off + 2] = tris_in[3 * (size_t)j + 2];
+			r[k] = ymx_leaf_ref(off, 1u);
+		}
+		else if(cnt <= max_leaf) r[k] = ymx_leaf_ref(off, cnt);
+		off += cnt;
+	}
+	new_refs[n] = make_int2(r[0], r[1]);
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_set_refs(float4 *__restrict__ nodes, uint32_t n_int,
+                                                          const int2 *__restrict__ new_refs)
+{
+	const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+	if(n >= n_int) return;
+	const int2 r = new_refs[n];
+	float4 f3 = nodes[4 * (size_t)n + 3];
+	f3.x = __int_as_float(r.x);
+	f3.y = __int_as_float(r.y);
+	nodes[4 * (size_t)n + 3] = f3;
 }
 
 __global__ void k_ploc_totals(const uint32_t *keep, const uint32_t *keep_pos, const uint32_t *merge,
@@ -641,6 +732,91 @@ void reference_tree_bound(const float lo_in[3], const float hi_in[3], uint32_t n
 		bmin[k] = a;
 		bmax[k] = g;
 	}
+}
+
+// Depth-first relayout of a complete tree of n_tris - 1 internal nodes (root
+// 0) into a new node buffer; with ploc_leaves, first the depth-first triangle
+// order and the leaves of up to max_leaf triangles (k_ploc_leaves).
+int relayout_tree(DevBuf &d_nodes, DevBuf &d_tris, uint32_t n_tris, bool ploc_leaves, uint32_t max_leaf,
+                  hipStream_t stream, uint32_t &n_nodes)
+{
+	const uint32_t n_int = n_tris - 1;
+	DevBuf d_par, d_nint, d_arrive, d_size, d_pos, d_bad, d_out;
+	YMX_HIP_CHECK(d_par.alloc(sizeof(int32_t) * (size_t)n_int));
+	YMX_HIP_CHECK(d_nint.alloc(sizeof(uint32_t) * (size_t)n_int));
+	YMX_HIP_CHECK(d_arrive.alloc(sizeof(uint32_t) * (size_t)n_int));
+	YMX_HIP_CHECK(d_size.alloc(sizeof(uint32_t) * (size_t)n_int));
+	YMX_HIP_CHECK(d_pos.alloc(sizeof(uint32_t) * (size_t)n_int));
+	YMX_HIP_CHECK(d_bad.alloc(sizeof(uint32_t)));
+	auto sizes = [&](uint32_t *tcount) -> int {
+		YMX_HIP_CHECK(hipMemsetAsync(d_par.p, 0xFF, sizeof(int32_t) * (size_t)n_int, stream));
+		YMX_HIP_CHECK(hipMemsetAsync(d_arrive.p, 0, sizeof(uint32_t) * (size_t)n_int, stream));
+		YMX_HIP_CHECK(hipMemsetAsync(d_size.p, 0, sizeof(uint32_t) * (size_t)n_int, stream));
+		YMX_HIP_CHECK(hipMemsetAsync(d_bad.p, 0, sizeof(uint32_t), stream));
+		k_lay_parents<<<grid_for(n_int), kBuildBlock, 0, stream>>>(d_nodes.as<float4>(), n_int, d_par.as<int32_t>(),
+		                                                           d_nint.as<uint32_t>());
+		YMX_HIP_CHECK(hipGetLastError());
+		k_lay_size<<<grid_for(n_int), kBuildBlock, 0, stream>>>(d_nodes.as<float4>(), n_int, d_par.as<int32_t>(),
+		                                                        d_nint.as<uint32_t>(), d_arrive.as<uint32_t>(),
+		                                                        d_size.as<uint32_t>(), tcount, d_bad.as<uint32_t>());
+		YMX_HIP_CHECK(hipGetLastError());
+		return YMX_OK;
+	};
+	uint32_t hs[2] = {0u, 0u};
+	if(ploc_leaves)
+	{
+		DevBuf d_tcount, d_tris_out, d_refs;
+		YMX_HIP_CHECK(d_tcount.alloc(sizeof(uint32_t) * (size_t)n_int));
+		YMX_HIP_CHECK(d_tris_out.alloc(sizeof(float4) * 3 * (size_t)n_tris));
+		YMX_HIP_CHECK(d_refs.alloc(sizeof(int2) * (size_t)n_int));
+		YMX_HIP_CHECK(hipMemsetAsync(d_tcount.p, 0, sizeof(uint32_t) * (size_t)n_int, stream));
+		int rc = sizes(d_tcount.as<uint32_t>());
+		if(rc != YMX_OK) return rc;
+		k_ploc_leaves<<<grid_for(n_int), kBuildBlock, 0, stream>>>(
+		    d_nodes.as<float4>(), n_int, d_par.as<int32_t>(), d_tcount.as<uint32_t>(), d_tris.as<float4>(), n_tris,
+		    max_leaf, d_tris_out.as<float4>(), d_refs.as<int2>(), d_bad.as<uint32_t>());
+		YMX_HIP_CHECK(hipGetLastError());
+		YMX_HIP_CHECK(hipMemcpyAsync(&hs[0], d_tcount.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+		YMX_HIP_CHECK(hipMemcpyAsync(&hs[1], d_bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+		YMX_HIP_CHECK(hipStreamSynchronize(stream));
+		if(hs[1] || hs[0] != n_tris)
+		{
+			ymx_set_error("ymx_accel_create: leaf collapse found an inconsistent tree (%u of %u triangles)", hs[0],
+			              n_tris);
+			return YMX_ERR_HIP;
+		}
+		k_set_refs<<<grid_for(n_int), kBuildBlock, 0, stream>>>(d_nodes.as<float4>(), n_int, d_refs.as<int2>());
+		YMX_HIP_CHECK(hipGetLastError());
+		std::swap(d_tris.p, d_tris_out.p);
+	}
+	int rc = sizes(nullptr);
+	if(rc != YMX_OK) return rc;
+	k_lay_pos<<<grid_for(n_int), kBuildBlock, 0, stream>>>(d_nodes.as<float4>(), n_int, d_par.as<int32_t>(),
+	                                                       d_nint.as<uint32_t>(), d_size.as<uint32_t>(),
+	                                                       d_pos.as<uint32_t>(), d_bad.as<uint32_t>());
+	YMX_HIP_CHECK(hipGetLastError());
+	YMX_HIP_CHECK(hipMemcpyAsync(&hs[0], d_size.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+	YMX_HIP_CHECK(hipMemcpyAsync(&hs[1], d_bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+	YMX_HIP_CHECK(hipStreamSynchronize(stream));
+	if(hs[1] || hs[0] == 0u || hs[0] > n_int)
+	{
+		ymx_set_error("ymx_accel_create: node relayout found an inconsistent tree (%u of %u nodes)", hs[0], n_int);
+		return YMX_ERR_HIP;
+	}
+	YMX_HIP_CHECK(d_out.alloc(sizeof(float4) * 4 * (size_t)hs[0]));
+	k_lay_write<<<grid_for(n_int), kBuildBlock, 0, stream>>>(d_nodes.as<float4>(), n_int, d_pos.as<uint32_t>(), hs[0],
+	                                                         d_out.as<float4>(), d_bad.as<uint32_t>());
+	YMX_HIP_CHECK(hipGetLastError());
+	YMX_HIP_CHECK(hipMemcpyAsync(&hs[1], d_bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+	YMX_HIP_CHECK(hipStreamSynchronize(stream));
+	if(hs[1])
+	{
+		ymx_set_error("ymx_accel_create: node relayout wrote out of range");
+		return YMX_ERR_HIP;
+	}
+	std::swap(d_nodes.p, d_out.p);
+	n_nodes = hs[0];
+	return YMX_OK;
 }
 
 }  // namespace
@@ -896,55 +1072,14 @@ int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const
 	// LBVH: the Karras node order puts a parent far from its children near the
 	// root and leaves the nodes under collapsed leaves in place; the depth-first
 	// relayout drops those (2.19M -> 0.70M nodes on the heightfield workload)
-	// and traces 2 % faster.  PLOC trees are left as built: the relayout gains
-	// nothing on them.
-	if(builder != YMX_BUILDER_PLOC)
+	// and traces 2 % faster.  PLOC: leaves of up to max_leaf triangles need the
+	// triangles in depth-first order first; with one triangle per leaf the tree
+	// is left as built (the relayout alone gains nothing on it).
+	if(builder != YMX_BUILDER_PLOC || max_leaf > 1)
 	{
-		const uint32_t n_int = n_tris - 1;
-		DevBuf d_par, d_nint, d_arrive, d_size, d_pos, d_bad, d_out;
-		YMX_HIP_CHECK(d_par.alloc(sizeof(int32_t) * (size_t)n_int));
-		YMX_HIP_CHECK(d_nint.alloc(sizeof(uint32_t) * (size_t)n_int));
-		YMX_HIP_CHECK(d_arrive.alloc(sizeof(uint32_t) * (size_t)n_int));
-		YMX_HIP_CHECK(d_size.alloc(sizeof(uint32_t) * (size_t)n_int));
-		YMX_HIP_CHECK(d_pos.alloc(sizeof(uint32_t) * (size_t)n_int));
-		YMX_HIP_CHECK(d_bad.alloc(sizeof(uint32_t)));
-		YMX_HIP_CHECK(hipMemsetAsync(d_par.p, 0xFF, sizeof(int32_t) * (size_t)n_int, stream));
-		YMX_HIP_CHECK(hipMemsetAsync(d_arrive.p, 0, sizeof(uint32_t) * (size_t)n_int, stream));
-		YMX_HIP_CHECK(hipMemsetAsync(d_size.p, 0, sizeof(uint32_t) * (size_t)n_int, stream));
-		YMX_HIP_CHECK(hipMemsetAsync(d_bad.p, 0, sizeof(uint32_t), stream));
-		k_lay_parents<<<grid_for(n_int), kBuildBlock, 0, stream>>>(d_nodes.as<float4>(), n_int, d_par.as<int32_t>(),
-		                                                           d_nint.as<uint32_t>());
-		YMX_HIP_CHECK(hipGetLastError());
-		k_lay_size<<<grid_for(n_int), kBuildBlock, 0, stream>>>(d_nodes.as<float4>(), n_int, d_par.as<int32_t>(),
-		                                                        d_nint.as<uint32_t>(), d_arrive.as<uint32_t>(),
-		                                                        d_size.as<uint32_t>(), d_bad.as<uint32_t>());
-		YMX_HIP_CHECK(hipGetLastError());
-		k_lay_pos<<<grid_for(n_int), kBuildBlock, 0, stream>>>(d_nodes.as<float4>(), n_int, d_par.as<int32_t>(),
-		                                                       d_nint.as<uint32_t>(), d_size.as<uint32_t>(),
-		                                                       d_pos.as<uint32_t>(), d_bad.as<uint32_t>());
-		YMX_HIP_CHECK(hipGetLastError());
-		uint32_t hs[2] = {0u, 0u};
-		YMX_HIP_CHECK(hipMemcpyAsync(&hs[0], d_size.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-		YMX_HIP_CHECK(hipMemcpyAsync(&hs[1], d_bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-		YMX_HIP_CHECK(hipStreamSynchronize(stream));
-		if(hs[1] || hs[0] == 0u || hs[0] > n_int)
-		{
-			ymx_set_error("ymx_accel_create: node relayout found an inconsistent tree (%u of %u nodes)", hs[0], n_int);
-			return YMX_ERR_HIP;
-		}
-		YMX_HIP_CHECK(d_out.alloc(sizeof(float4) * 4 * (size_t)hs[0]));
-		k_lay_write<<<grid_for(n_int), kBuildBlock, 0, stream>>>(d_nodes.as<float4>(), n_int, d_pos.as<uint32_t>(),
-		                                                         hs[0], d_out.as<float4>(), d_bad.as<uint32_t>());
-		YMX_HIP_CHECK(hipGetLastError());
-		YMX_HIP_CHECK(hipMemcpyAsync(&hs[1], d_bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-		YMX_HIP_CHECK(hipStreamSynchronize(stream));
-		if(hs[1])
-		{
-			ymx_set_error("ymx_accel_create: node relayout wrote out of range");
-			return YMX_ERR_HIP;
-		}
-		std::swap(d_nodes.p, d_out.p);
-		n_nodes = hs[0];
+		const int rc = relayout_tree(d_nodes, d_tris, n_tris, builder == YMX_BUILDER_PLOC, (uint32_t)max_leaf, stream,
+		                             n_nodes);
+		if(rc != YMX_OK) return rc;
 	}
 	YMX_HIP_CHECK(hipEventRecord(ev.b, stream));
 	uint32_t depth = 0;

@@ -303,8 +303,9 @@ def test_transparent_shadow_query(oracle, max_depth):
 def test_exported_tree_structure(builder):
     """The exported tree is a tree over every triangle exactly once: node 0 is
     the root, every other node has exactly one parent, refs stay in range.
-    LBVH nodes are in depth-first order with a node's internal children in
-    the next free adjacent slots (no node under a collapsed leaf is kept)."""
+    With leaves of up to 4 triangles (the default) both builders store the
+    nodes in depth-first order with a node's internal children in the next
+    free adjacent slots, and keep no node under a collapsed leaf."""
     import ctypes as C
     from libyafaray_amd import native
     verts, tris = scenes.heightfield_scene(res=128, n_random=3000, seed=11)
@@ -328,13 +329,28 @@ def test_exported_tree_structure(builder):
     assert np.all(covered == 1)
     ids = tri_rec[:, 3].copy().view(np.uint32)
     assert np.array_equal(np.sort(ids), np.arange(n_tris, dtype=np.uint32))
-    if builder == "lbvh":
-        assert n_nodes < n_tris - 1  # collapsed subtrees dropped
-        nxt, stack = 1, [0]
-        while stack:
-            n = stack.pop()
-            ch = [int(r) for r in refs[n] if r >= 0]
-            assert ch == list(range(nxt, nxt + len(ch))), (n, ch, nxt)
-            nxt += len(ch)
-            stack.extend(reversed(ch))
-        assert nxt == n_nodes
+    assert n_nodes < n_tris - 1  # collapsed subtrees dropped
+    assert count.max() <= 4 and count.max() > 1
+    nxt, stack = 1, [0]
+    while stack:
+        n = stack.pop()
+        ch = [int(r) for r in refs[n] if r >= 0]
+        assert ch == list(range(nxt, nxt + len(ch))), (n, ch, nxt)
+        nxt += len(ch)
+        stack.extend(reversed(ch))
+    assert nxt == n_nodes
+
+
+@pytest.mark.parametrize("builder", ["lbvh", "ploc"])
+@pytest.mark.parametrize("max_leaf", [1, 2, 3, 8])
+def test_leaf_sizes_bit_exact(oracle, builder, max_leaf):
+    """Every leaf size gives the brute-force closest hits and occlusion bits
+    (leaf collapse, triangle reordering and node relayout change no result)."""
+    verts, tris, rays, rng = small_scene(4, res=48, n_random=1500)
+    say(f"leaf sizes builder={builder} max_leaf={max_leaf} tris={len(tris)} rays={len(rays)}")
+    acc = GpuAccelerator(verts, tris, builder=builder, max_leaf_size=max_leaf)
+    assert_hits_equal(acc.intersect(rays), oracle.closest_brute(verts, tris, rays), f"closest leaf {max_leaf}")
+    srays = rays.copy()
+    srays[:, 7] = scenes.shadow_tmax(rng, len(srays))
+    occ, _ = acc.intersect_shadow(srays)
+    assert np.array_equal(occ, oracle.shadow_brute(verts, tris, srays))
