@@ -13,10 +13,15 @@
  *   - Ray and hit buffers passed to the query calls are DEVICE pointers owned
  *     by the caller.  `stream` is a hipStream_t (NULL: the default stream);
  *     queries are asynchronous on it.
- *   - A handle is immutable after ymx_accel_create.  Queries on one handle
- *     share a small per-handle work area (a work counter and a stack spill
- *     area), so queries on the SAME handle must not overlap in time: issue
- *     them on one stream, or synchronise between streams.
+ *   - A handle is immutable after ymx_accel_create, and its queries may be
+ *     issued concurrently, from several host threads and on several streams,
+ *     as the reference's const queries are.  Each stream's queries use a work
+ *     area of their own (a ray counter and a traversal-stack spill area,
+ *     allocated on the stream's first query and kept until
+ *     ymx_accel_destroy; a stream whose queries have all completed hands its
+ *     area to the next new stream).
+ *   - The traversal stack holds max_depth + 1 entries (ymx_accel_info) and a
+ *     traversal never needs more than max_depth, so it cannot overflow.
  *   - FP32 throughout.  Closest-hit primitive ids are bit-exact with the
  *     reference's kd-tree except for exact ties in t, which this library
  *     resolves to the lowest primitive id.

@@ -15,9 +15,7 @@ void free_accel(ymx_accel *acc)
 	if(!acc) return;
 	if(acc->d_nodes) (void)hipFree(acc->d_nodes);
 	if(acc->d_tris) (void)hipFree(acc->d_tris);
-	if(acc->d_spill) (void)hipFree(acc->d_spill);
-	if(acc->d_work) (void)hipFree(acc->d_work);
-	if(acc->d_stats) (void)hipFree(acc->d_stats);
+	ymx_free_work_areas(acc);
 	delete acc;
 }
 }  // namespace
@@ -93,7 +91,6 @@ int ymx_accel_create(const float *verts, uint32_t n_verts, const uint32_t *tri_i
 		ymx_set_error("ymx_accel_create: host allocation failed");
 		return YMX_ERR_OUT_OF_MEMORY;
 	}
-	memset(acc, 0, sizeof *acc);
 	if(hipGetDevice(&acc->device) != hipSuccess)
 	{
 		ymx_set_error("ymx_accel_create: no HIP device");
@@ -246,31 +243,27 @@ int ymx_accel_traversal_stats(const ymx_accel *accel, const ymx_ray *d_rays, uin
 	*node_fetches = 0;
 	*tri_tests = 0;
 	if(n_rays == 0) return YMX_OK;
-	int rc = check_query(accel, d_rays, accel ? (const void *)accel->d_stats : nullptr, "ymx_accel_traversal_stats");
+	int rc = check_query(accel, d_rays, node_fetches, "ymx_accel_traversal_stats");
 	if(rc != YMX_OK) return rc;
 	hipStream_t s = (hipStream_t)stream;
 	void *scratch_out = nullptr;
 	const size_t out_bytes = query == 0 ? sizeof(ymx_hit) * (size_t)n_rays : sizeof(uint32_t) * (((size_t)n_rays + 31) / 32);
-	YMX_HIP_CHECK(hipMalloc(&scratch_out, out_bytes));
+	// the counters live behind the results, in this call's own allocation
+	const size_t stats_off = (out_bytes + 15u) & ~(size_t)15u;
+	YMX_HIP_CHECK(hipMalloc(&scratch_out, stats_off + 2 * sizeof(unsigned long long)));
+	unsigned long long *d_stats = reinterpret_cast<unsigned long long *>((char *)scratch_out + stats_off);
 	unsigned long long host[2] = {0, 0};
-	hipError_t e = hipMemsetAsync(accel->d_stats, 0, sizeof host, s);
+	hipError_t e = hipMemsetAsync(d_stats, 0, sizeof host, s);
 	if(e == hipSuccess)
 	{
-		rc = query == 0 ? ymx_launch_closest(accel, d_rays, (ymx_hit *)scratch_out, n_rays, s, accel->d_stats)
-		                : ymx_launch_shadow(accel, d_rays, (uint32_t *)scratch_out, nullptr, n_rays, s, accel->d_stats);
-		if(rc == YMX_OK) e = hipMemcpyAsync(host, accel->d_stats, sizeof host, hipMemcpyDeviceToHost, s);
+		rc = query == 0 ? ymx_launch_closest(accel, d_rays, (ymx_hit *)scratch_out, n_rays, s, d_stats)
+		                : ymx_launch_shadow(accel, d_rays, (uint32_t *)scratch_out, nullptr, n_rays, s, d_stats);
+		if(rc == YMX_OK) e = hipMemcpyAsync(host, d_stats, sizeof host, hipMemcpyDeviceToHost, s);
 		if(rc == YMX_OK && e == hipSuccess) e = hipStreamSynchronize(s);
 	}
 	(void)hipFree(scratch_out);
 	if(rc != YMX_OK) return rc;
 	YMX_HIP_CHECK(e);
-	uint32_t work[2] = {0, 0};
-	YMX_HIP_CHECK(hipMemcpy(work, accel->d_work, sizeof work, hipMemcpyDeviceToHost));
-	if(work[1])
-	{
-		ymx_set_error("ymx_accel_traversal_stats: traversal stack overflow (depth %u)", accel->max_depth);
-		return YMX_ERR_UNSUPPORTED;
-	}
 	*node_fetches = host[0];
 	*tri_tests = host[1];
 	return YMX_OK;

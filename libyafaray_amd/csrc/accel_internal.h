@@ -21,6 +21,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <deque>
+#include <mutex>
+
 #include "../../include/ymx_accel.h"
 
 #define YMX_REF_EMPTY ((int32_t)0xFFFFFFFF)
@@ -35,6 +38,21 @@ static __host__ __device__ inline int32_t ymx_leaf_ref(uint32_t first, uint32_t 
 static __host__ __device__ inline uint32_t ymx_leaf_first(int32_t ref) { return ((uint32_t)ref & 0x7FFFFFFFu) >> 4; }
 static __host__ __device__ inline uint32_t ymx_leaf_count(int32_t ref) { return ((uint32_t)ref & 15u) + 1u; }
 
+// Per-stream work area of the persistent query kernels: a ray counter and a
+// finished-wave counter (both left at zero by the last wave of each launch,
+// so a launch needs no reset before it) and the stack spill area, one column
+// of spill_depth entries per persistent lane.  Launches on one stream are
+// ordered, so they share one area; a launch on another stream takes an area
+// whose last launch has completed (its event) or a new one, so queries on one
+// handle may overlap across streams.
+struct ymx_work_area
+{
+	hipStream_t stream;
+	hipEvent_t done;    // recorded after each launch on the area
+	uint32_t *d_work;   // [0] ray counter, [1] finished waves
+	int32_t *d_spill;
+};
+
 struct ymx_accel
 {
 	int device;
@@ -45,14 +63,26 @@ struct ymx_accel
 	float build_ms;
 	float4 *d_nodes;
 	float4 *d_tris;
-	// query work area (shared by the queries of this handle)
-	uint32_t *d_work;        // [0] ray counter, [1] overflow flag
-	int32_t *d_spill;        // stack spill: spill_depth entries per persistent lane
+	// Traversal stack: YMX_LDS_STACK entries in LDS plus spill_depth in HBM,
+	// max_depth + 1 in all.  A lane at an internal node of depth D holds at
+	// most D deferred siblings (one per level of its root path), and internal
+	// nodes lie at depth <= max_depth - 1, so the stack never holds more than
+	// max_depth entries and cannot overflow.
 	uint32_t spill_depth;
 	uint32_t grid_threads_closest;
 	uint32_t grid_threads_shadow;
-	unsigned long long *d_stats;  // [0] node fetches, [1] triangle tests
+	uint32_t grid_threads_max;
+	// query work areas; work_mutex is held while a launch picks its area and
+	// enqueues its kernels and event
+	mutable std::mutex work_mutex;
+	mutable std::deque<ymx_work_area> work_areas;
 };
+
+// The work area for a launch on `stream` (work_mutex held by the caller): the
+// stream's own, else one whose last launch has completed, else a new one.
+// Returns NULL and sets the error when a new area cannot be allocated.
+ymx_work_area *ymx_work_area_for(const ymx_accel *acc, hipStream_t stream);
+void ymx_free_work_areas(ymx_accel *acc);
 
 struct ymx_trace_args
 {

@@ -67,19 +67,17 @@ struct LaneStack
 	uint32_t stride;  // grid_threads
 	uint32_t cap;     // spill capacity (entries)
 	uint32_t sp;
-	bool overflow;
 
+	// The stack holds at most max_depth entries and kStack + cap is
+	// max_depth + 1 (ymx_accel), so the capacity test never fails; it only
+	// keeps every store inside the spill area.
 	__device__ __forceinline__ void push(int32_t ref)
 	{
 		const uint32_t slot = sp & (kStack - 1);
 		if(sp >= (uint32_t)kStack)
 		{
 			const uint32_t k = sp - kStack;
-			if(k >= cap)
-			{
-				overflow = true;
-				return;
-			}
+			if(k >= cap) return;
 			spill[(size_t)k * stride] = lds[slot * kTraceBlock];
 		}
 		lds[slot * kTraceBlock] = ref;
@@ -95,6 +93,24 @@ struct LaneStack
 		return ref;
 	}
 };
+
+// End of a persistent wave: the last wave of the launch to finish resets the
+// ray counter and the finished-wave counter, so the next launch on the same
+// work area starts from zero without a reset of its own.  Every other wave
+// has made its last ray fetch (an atomic whose result it waited for) before
+// its increment here, so none fetches after the reset.
+__device__ __forceinline__ void finish_wave(uint32_t *work)
+{
+	if(__lane_id() == 0)
+	{
+		const uint32_t waves = gridDim.x * (uint32_t)(kTraceBlock / 64);
+		if(atomicAdd(work + 1, 1u) == waves - 1u)
+		{
+			atomicExch(work, 0u);
+			atomicExch(work + 1, 0u);
+		}
+	}
+}
 
 __device__ __forceinline__ void setup_ray(const ymx_ray &ray, RayState &r)
 {
@@ -145,7 +161,6 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(ymx_trace_args args)
 	st.spill = args.spill + gtid;
 	st.stride = args.grid_threads;
 	st.cap = args.spill_depth;
-	st.overflow = false;
 	unsigned long long n_fetch = 0, n_tests = 0;
 	const uint32_t n_rays = args.n_dev ? *args.n_dev : args.n_rays;
 
@@ -318,7 +333,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(ymx_trace_args args)
 			    make_float4(__uint_as_float(h.prim), h.t, h.u, h.v);
 		}
 	}
-	if(st.overflow) atomicOr(args.work + 1, 1u);
+	finish_wave(args.work);
 	if(kStats)
 	{
 		for(int off = 32; off > 0; off >>= 1)
@@ -359,7 +374,6 @@ __global__ __launch_bounds__(kTraceBlock) void k_dyn(ymx_trace_args args)
 	st.spill = args.spill + gtid;
 	st.stride = args.grid_threads;
 	st.cap = args.spill_depth;
-	st.overflow = false;
 	st.sp = 0;
 	unsigned long long n_fetch = 0, n_tests = 0;
 	const uint32_t n_rays = args.n_dev ? *args.n_dev : args.n_rays;
@@ -498,7 +512,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_dyn(ymx_trace_args args)
 			}
 		}
 	}
-	if(st.overflow) atomicOr(args.work + 1, 1u);
+	finish_wave(args.work);
 	if(kStats)
 	{
 		for(int off = 32; off > 0; off >>= 1)
@@ -514,7 +528,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_dyn(ymx_trace_args args)
 	}
 }
 
-__global__ void k_reset_work(uint32_t *work) { work[0] = 0u; }
+__global__ void k_clear_words(uint32_t *p, uint32_t n)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i < n) p[i] = 0u;
+}
 
 template <int kMode, bool kStats, bool kDynamic = false>
 int launch(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t *occ, uint32_t *prim, uint32_t n,
@@ -535,8 +553,11 @@ int launch(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t *o
 		a.bmin[k] = acc->bmin[k];
 		a.bmax[k] = acc->bmax[k];
 	}
-	a.work = acc->d_work;
-	a.spill = acc->d_spill;
+	std::lock_guard<std::mutex> lock(acc->work_mutex);
+	ymx_work_area *wa = ymx_work_area_for(acc, stream);
+	if(!wa) return YMX_ERR_OUT_OF_MEMORY;
+	a.work = wa->d_work;
+	a.spill = wa->d_spill;
 	a.spill_depth = acc->spill_depth;
 	a.grid_threads = kShadow ? acc->grid_threads_shadow : acc->grid_threads_closest;
 	a.hits = hits;
@@ -550,14 +571,20 @@ int launch(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t *o
 	const uint32_t needed = ((n + 63u) / 64u) * 64u;
 	uint32_t threads = a.grid_threads < needed ? a.grid_threads : needed;
 	const uint32_t blocks = (threads + kTraceBlock - 1) / kTraceBlock;
-	k_reset_work<<<1, 1, 0, stream>>>(acc->d_work);
 	if constexpr(kDynamic && kMode != kTransparent)
 	{
-		if(kShadow) YMX_HIP_CHECK(hipMemsetAsync(occ, 0, sizeof(uint32_t) * (((size_t)n + 31u) / 32u), stream));
+		if(kShadow)
+		{
+			// occlusion bits are OR-ed in: clear the mask (a kernel, not a
+			// memset node, so the launch sequence can be graph-captured)
+			const uint32_t words = (n + 31u) / 32u;
+			k_clear_words<<<(words + 255u) / 256u, 256, 0, stream>>>(occ, words);
+		}
 		k_dyn<kMode, kStats><<<blocks, kTraceBlock, 0, stream>>>(a);
 	}
 	else k_trace<kMode, kStats><<<blocks, kTraceBlock, 0, stream>>>(a);
 	YMX_HIP_CHECK(hipGetLastError());
+	YMX_HIP_CHECK(hipEventRecord(wa->done, stream));
 	return YMX_OK;
 }
 
@@ -587,12 +614,50 @@ int ymx_trace_setup(ymx_accel *acc)
 	acc->grid_threads_shadow = (uint32_t)(prop.multiProcessorCount * per_cu_s * kTraceBlock);
 	const uint32_t gt = acc->grid_threads_closest > acc->grid_threads_shadow ? acc->grid_threads_closest
 	                                                                         : acc->grid_threads_shadow;
+	acc->grid_threads_max = gt;
 	acc->spill_depth = acc->max_depth > (uint32_t)kStack ? acc->max_depth - (uint32_t)kStack + 1u : 1u;
-	YMX_HIP_CHECK(hipMalloc(&acc->d_spill, sizeof(int32_t) * (size_t)acc->spill_depth * gt));
-	YMX_HIP_CHECK(hipMalloc(&acc->d_work, 64));
-	YMX_HIP_CHECK(hipMemset(acc->d_work, 0, 64));
-	YMX_HIP_CHECK(hipMalloc(&acc->d_stats, 2 * sizeof(unsigned long long)));
-	return YMX_OK;
+	// the work area of the default stream, so its first query allocates nothing
+	std::lock_guard<std::mutex> lock(acc->work_mutex);
+	return ymx_work_area_for(acc, nullptr) ? YMX_OK : YMX_ERR_OUT_OF_MEMORY;
+}
+
+ymx_work_area *ymx_work_area_for(const ymx_accel *acc, hipStream_t stream)
+{
+	for(ymx_work_area &w : acc->work_areas)
+		if(w.stream == stream) return &w;
+	for(ymx_work_area &w : acc->work_areas)
+	{
+		if(hipEventQuery(w.done) == hipSuccess)
+		{
+			w.stream = stream;
+			return &w;
+		}
+	}
+	ymx_work_area w = {};
+	w.stream = stream;
+	const size_t spill_bytes = sizeof(int32_t) * (size_t)acc->spill_depth * acc->grid_threads_max;
+	if(hipMalloc(&w.d_work, 64) != hipSuccess || hipMemset(w.d_work, 0, 64) != hipSuccess ||
+	   hipMalloc(&w.d_spill, spill_bytes) != hipSuccess ||
+	   hipEventCreateWithFlags(&w.done, hipEventDisableTiming) != hipSuccess)
+	{
+		if(w.d_work) (void)hipFree(w.d_work);
+		if(w.d_spill) (void)hipFree(w.d_spill);
+		ymx_set_error("query work area: device allocation of %zu bytes failed", spill_bytes + 64);
+		return nullptr;
+	}
+	acc->work_areas.push_back(w);
+	return &acc->work_areas.back();
+}
+
+void ymx_free_work_areas(ymx_accel *acc)
+{
+	for(ymx_work_area &w : acc->work_areas)
+	{
+		(void)hipFree(w.d_work);
+		(void)hipFree(w.d_spill);
+		(void)hipEventDestroy(w.done);
+	}
+	acc->work_areas.clear();
 }
 
 int ymx_launch_closest(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t n, hipStream_t stream,

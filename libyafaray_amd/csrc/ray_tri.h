@@ -52,7 +52,9 @@ YMX_RT_FN bool tree_cross(const float a[3], const float g[3], const float from[3
 			}
 			else
 			{
-				lmin = (lmin < ltmin) ? ltmin : lmin;
+				// std::max(ltmin, lmin) and std::min(ltmax, lmax) with their NaN
+				// behaviour: a NaN ltmin / ltmax is kept, a NaN lmin / lmax replaced
+				lmin = (ltmin < lmin) ? lmin : ltmin;
 				lmax = (lmax < ltmax) ? lmax : ltmax;
 			}
 			if((lmax < 0.f) || (lmin > t_max)) return false;

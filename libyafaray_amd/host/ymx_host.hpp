@@ -225,6 +225,7 @@ class Scene
 		std::vector<Instance> instances_;
 		std::vector<ymx_material> materials_;
 		std::vector<std::string> material_names_;
+		std::vector<uint8_t> material_vis_;  // "visibility" of each material
 		std::vector<std::pair<ymx_light, std::string>> lights_;  // object lights keep their object name
 		ParamMap accel_params_;
 		uint32_t modified_ = ModifiedAll;

@@ -106,8 +106,10 @@ int ora_bound_cross(const float a[3], const float g[3], const float from[3], con
 			}
 			else
 			{
-				/* std::max(ltmin, lmin) returns lmin unless lmin < ltmin. */
-				lmin = (lmin < ltmin) ? ltmin : lmin;
+				/* std::max(ltmin, lmin) is (ltmin < lmin) ? lmin : ltmin, so a NaN
+				 * ltmin (0 * inf for an origin on a face and a denormal direction
+				 * component) is kept, and a NaN lmin is replaced by ltmin. */
+				lmin = (ltmin < lmin) ? lmin : ltmin;
 				/* std::min(ltmax, lmax) returns ltmax unless lmax < ltmax. */
 				lmax = (lmax < ltmax) ? lmax : ltmax;
 			}

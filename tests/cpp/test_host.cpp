@@ -22,6 +22,7 @@
 #include <string>
 #include <vector>
 
+#include "../../libyafaray_amd/csrc/ray_tri.h"
 #include "../../libyafaray_amd/host/ymx_host.hpp"
 #include "../../oracle/ora_pt.h"
 #include "../../oracle/oracle.h"
@@ -512,6 +513,77 @@ static void test_scene_preprocess()
 	      h.primitive, h.t_hit);
 }
 
+// Visibility: an object of visibility "invisible" is left out of the
+// primitive list (src/scene/scene.cc:323), so the next object's triangles keep
+// the reference's ids; a material's "visibility" is ANDed with the object's in
+// every query (include/accelerator/accelerator.h:126-127,138-139).
+static void test_visibility_order()
+{
+	Scene sc;
+	std::string err;
+	ParamMap m;
+	m.setString("type", "shinydiffusemat");
+	size_t grey = 0, shadow_only = 0, ghost = 0;
+	CHECK(sc.createMaterial(&grey, "grey", m, &err), "material: %s", err.c_str());
+	m.setString("visibility", "shadow_only");
+	CHECK(sc.createMaterial(&shadow_only, "shadow_only", m, &err), "material: %s", err.c_str());
+	m.setString("visibility", "invisible");
+	CHECK(sc.createMaterial(&ghost, "ghost", m, &err), "material: %s", err.c_str());
+	ParamMap inv;
+	inv.setString("visibility", "invisible");
+	const size_t hidden = sc.createObject("hidden", inv);  // above everything, created first
+	sc.addVertex(hidden, 0, 0, 0.5);
+	sc.addVertex(hidden, 4, 0, 0.5);
+	sc.addVertex(hidden, 0, 4, 0.5);
+	sc.addTriangle(hidden, 0, 1, 2, grey);
+	ParamMap normal;
+	const size_t obj = sc.createObject("shown", normal);
+	auto tri = [&](double x, double y, double z, size_t mat) {
+		const size_t v = sc.addVertex(obj, x, y, z);
+		sc.addVertex(obj, x + 1, y, z);
+		sc.addVertex(obj, x, y + 1, z);
+		sc.addTriangle(obj, v, v + 1, v + 2, mat);
+	};
+	tri(0, 0, 0.0, grey);         // id 0
+	tri(2, 0, 0.3, shadow_only);  // id 1
+	tri(0, 2, 0.2, ghost);        // id 2
+	CHECK(sc.preprocess(sc.checkAndClearModifiedFlags(), &err), "preprocess: %s", err.c_str());
+	const auto a = sc.accelerator();
+	if(!a) return;
+	auto down = [](float x, float y) {
+		Ray r;
+		r.from[0] = x;
+		r.from[1] = y;
+		r.from[2] = 3.f;
+		r.dir[2] = -1.f;
+		return r;
+	};
+	const IntersectData c0 = a->intersect(down(0.2f, 0.2f), FLT_MAX);
+	CHECK(c0.isHit() && c0.primitive == 0 && c0.t_hit == 3.f,
+	      "the invisible object is skipped and the next object's first triangle is id 0 (%u, %g)", c0.primitive,
+	      c0.t_hit);
+	const IntersectData s0 = a->intersectShadow(down(0.2f, 0.2f), 10.f);
+	CHECK(s0.isHit() && s0.primitive == 0, "the invisible object casts no shadow (%u)", s0.primitive);
+	CHECK(!a->intersect(down(2.2f, 0.2f), FLT_MAX).isHit(), "a shadow_only material is not seen");
+	const IntersectData s1 = a->intersectShadow(down(2.2f, 0.2f), 10.f);
+	CHECK(s1.isHit() && s1.primitive == 1, "a shadow_only material casts shadows (%u)", s1.primitive);
+	CHECK(!a->intersect(down(0.2f, 2.2f), FLT_MAX).isHit(), "an invisible material is not seen");
+	CHECK(!a->intersectShadow(down(0.2f, 2.2f), 10.f).isHit(), "an invisible material casts no shadow");
+	// the batched GPU query agrees with the single-ray host walk
+	std::vector<Ray> rays = {down(0.2f, 0.2f), down(2.2f, 0.2f), down(0.2f, 2.2f), down(1.5f, 1.5f)};
+	ymx_ray *d_rays = nullptr;
+	to_device(rays, &d_rays);
+	ymx_hit *d_hits = nullptr;
+	CHECK(hipMalloc(&d_hits, sizeof(ymx_hit) * rays.size()) == hipSuccess, "hipMalloc");
+	CHECK(a->intersect(d_rays, d_hits, (uint32_t)rays.size(), nullptr) == YMX_OK, "batched query");
+	std::vector<ymx_hit> hits(rays.size());
+	CHECK(hipMemcpy(hits.data(), d_hits, sizeof(ymx_hit) * hits.size(), hipMemcpyDeviceToHost) == hipSuccess, "copy");
+	CHECK(hits[0].prim == 0 && hits[1].prim == YMX_NO_HIT && hits[2].prim == YMX_NO_HIT && hits[3].prim == YMX_NO_HIT,
+	      "batched ids %u %u %u %u", hits[0].prim, hits[1].prim, hits[2].prim, hits[3].prim);
+	(void)hipFree(d_rays);
+	(void)hipFree(d_hits);
+}
+
 // Instances: base objects rendered only through instances, under their
 // object-to-world matrices.
 static void test_instances()
@@ -601,6 +673,35 @@ static void test_errors()
 	CHECK(q.getString("n", "d", &ok) == "d" && !ok, "mistyped read flagged");
 }
 
+// The tree-bound slab test the kernels and the host traversal share
+// (ray_tri.h), on the NaN cases of include/geometry/bound.h:183-188: an origin
+// on a face with a denormal direction component gives -0 * inf = NaN, which
+// std::max(ltmin, lmin) overwrites when it comes from the first axis and keeps
+// when it comes from a later one.
+static void test_slab_nan_order()
+{
+	const float a[3] = {0.f, 0.f, 0.f}, g[3] = {1.f, 1.f, 1.f};
+	float e = 0.f, l = 0.f;
+	{
+		const float o[3] = {0.f, 0.5f, 0.5f}, d[3] = {1e-40f, 1.f, 0.f};
+		const bool hit = ymx::tree_cross(a, g, o, d, FLT_MAX, e, l);
+		CHECK(hit && e == -0.5f && l == 0.5f, "NaN on x replaced by the y slab (hit %d, %g, %g)", hit, e, l);
+	}
+	{
+		const float o[3] = {0.5f, 0.f, 0.5f}, d[3] = {1.f, 1e-40f, 0.f};
+		CHECK(!ymx::tree_cross(a, g, o, d, FLT_MAX, e, l), "NaN on y kept: no crossing");
+	}
+	{
+		const float o[3] = {0.5f, 0.5f, 1.f}, d[3] = {1.f, 0.f, -1e-40f};
+		CHECK(!ymx::tree_cross(a, g, o, d, FLT_MAX, e, l), "NaN on z kept: no crossing");
+	}
+	{
+		const float o[3] = {-1.f, 0.5f, 0.5f}, d[3] = {1.f, 0.f, 0.f};
+		const bool hit = ymx::tree_cross(a, g, o, d, FLT_MAX, e, l);
+		CHECK(hit && e == 1.f && l == 2.f, "axis ray enters at 1 and leaves at 2");
+	}
+}
+
 int main(int argc, char **argv)
 {
 	const bool cpu_only = argc > 1 && strcmp(argv[1], "--cpu") == 0;
@@ -610,6 +711,7 @@ int main(int argc, char **argv)
 		void (*fn)();
 		bool gpu;
 	} tests[] = {{"errors", test_errors, false},
+	             {"slab_nan_order", test_slab_nan_order, false},
 	             {"accel_closest", test_accel_closest, true},
 	             {"accel_closest_ploc", [] { g_builder = "ploc"; test_accel_closest(); g_builder = "lbvh"; }, true},
 	             {"accel_shadow_ploc", [] { g_builder = "ploc"; test_accel_shadow(); g_builder = "lbvh"; }, true},
@@ -617,6 +719,7 @@ int main(int argc, char **argv)
 	             {"accel_transparent", test_accel_transparent, true},
 	             {"scene_preprocess", test_scene_preprocess, true},
 	             {"instances", test_instances, true},
+	             {"visibility_order", test_visibility_order, true},
 	             {"pathtracer", test_pathtracer, true}};
 	for(const auto &t : tests)
 	{

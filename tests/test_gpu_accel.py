@@ -181,6 +181,99 @@ def test_axis_aligned_and_degenerate_rays(oracle):
     assert_hits_equal(acc.intersect(rays), oracle.closest_brute(verts, tris, rays), "axis")
 
 
+def test_face_origin_denormal_direction(oracle):
+    """Origins on a face of the tree bound with a denormal direction component
+    along that face's axis: the slab test meets -0 * inf = NaN, which
+    std::max(ltmin, lmin) drops when it comes from the first axis and keeps
+    when it comes from a later one (include/geometry/bound.h:183-188).  Such a
+    ray stays in the face plane, which no triangle reaches (the bound is grown
+    by 0.1 %), so the order decides only whether the bound is crossed (pinned
+    on the CPU in the oracle and the shared header); here every ray must come
+    back as the oracle's miss, with no NaN-driven stray hit."""
+    verts, tris, _, rng = small_scene(8, res=32, n_random=500)
+    acc = GpuAccelerator(verts, tris)
+    a, g = acc.bound()
+    n = 3000
+    den = np.float32(1e-40)
+    assert den != 0 and np.isinf(np.float32(1) / den)
+
+    def face_rays(axis, lo_face):
+        r = np.zeros((n, 8), np.float32)
+        r[:, 0:3] = rng.uniform(a, g, size=(n, 3))
+        r[:, axis] = a[axis] if lo_face else g[axis]
+        d = rng.normal(size=(n, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        d[:, axis] = den if lo_face else -den
+        r[:, 4:7] = d
+        r[:, 7] = -1.0
+        return r
+
+    rays = np.concatenate([face_rays(0, True), face_rays(1, True), face_rays(2, False)])
+    say("face-origin rays with denormal direction components")
+    exp = oracle.closest_brute(verts, tris, rays)
+    assert_hits_equal(acc.intersect(rays), exp, "face-origin rays")
+    occ, _ = acc.intersect_shadow(rays)
+    assert np.array_equal(occ, oracle.shadow_brute(verts, tris, rays))
+    acc.close()
+
+
+def test_queries_on_several_streams(oracle):
+    """Queries on one handle overlap across streams and host threads: each
+    stream has its own work area (ray counter and stack spill), and every
+    launch leaves its counters at zero for the next one on that stream."""
+    import threading
+
+    import torch
+    verts, tris, rays, rng = small_scene(9, res=48, n_random=1500)
+    srays = rays.copy()
+    srays[:, 7] = scenes.shadow_tmax(rng, len(rays))
+    exp_c = oracle.closest_brute(verts, tris, rays)
+    exp_s = oracle.shadow_brute(verts, tris, srays)
+    acc = GpuAccelerator(verts, tris)
+    d_rays = torch.from_numpy(rays).cuda()
+    d_srays = torch.from_numpy(srays).cuda()
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream() for _ in range(3)]
+    say("interleaved launches on 3 streams")
+    outs = []
+    for rep in range(4):
+        for k, s in enumerate(streams):
+            if (rep + k) % 2 == 0:
+                outs.append(("c", acc.intersect_device(d_rays, stream=s)))
+            else:
+                outs.append(("s", acc.intersect_shadow_device(d_srays, stream=s)))
+    torch.cuda.synchronize()
+    from libyafaray_amd.accel import unpack_bits, unpack_hits
+    for kind, o in outs:
+        if kind == "c":
+            assert_hits_equal(unpack_hits(o.cpu().numpy()), exp_c, "closest on a side stream")
+        else:
+            assert np.array_equal(unpack_bits(o.cpu().numpy(), len(srays)), exp_s)
+    say("two host threads, one stream each")
+    results, errors = {}, []
+
+    def worker(i):
+        try:
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                got = [acc.intersect_device(d_rays, stream=s) for _ in range(5)]
+            s.synchronize()
+            results[i] = [g.cpu().numpy() for g in got]
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=worker, args=(i,)) for i in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for i in range(2):
+        for h in results[i]:
+            assert_hits_equal(unpack_hits(h), exp_c, f"thread {i}")
+    acc.close()
+
+
 def test_deterministic_repeat():
     verts, tris, rays, _ = small_scene(6)
     acc = GpuAccelerator(verts, tris)

@@ -105,6 +105,14 @@ UNIT = ((0, 0, 0), (1, 1, 1))
     ((2, 0.5, 0.5), (1, 0, 0), FMAX, (False, None, None)),   # lmax < 0
     ((-1, 5.0, 0.5), (1, 0, 0), FMAX, (True, 1.0, 2.0)),     # zero dir.y: no y rejection (quirk)
     ((-1, 5.0, 0.5), (1, 1e-30, 0), FMAX, (False, None, None)),
+    # origin on a face with a denormal direction component: 1/dir is inf and
+    # -p * inv_dir is 0 * inf = NaN.  std::max(ltmin, lmin) keeps a NaN ltmin
+    # and replaces a NaN lmin (include/geometry/bound.h:187), so a NaN on the
+    # first axis is overwritten by the next axis ...
+    ((0, 0.5, 0.5), (1e-40, 1, 0), FMAX, (True, -0.5, 0.5)),
+    # ... and a NaN on a later axis survives and fails lmin <= lmax
+    ((0.5, 0, 0.5), (1, 1e-40, 0), FMAX, (False, None, None)),
+    ((0.5, 0.5, 1), (1, 0, -1e-40), FMAX, (False, None, None)),
 ])
 def test_bound_cross_closed_form(oracle, org, dir, tmax, expect):
     hit, e, l = oracle.bound_cross(*UNIT, org, dir, tmax)
