@@ -43,6 +43,7 @@ extern "C" {
 #define YMX_ERR_UNSUPPORTED (-4)
 
 #define YMX_NO_HIT 0xFFFFFFFFu
+#define YMX_NO_VERTEX 0xFFFFFFFFu  /* 4th index of a triangle in a face list */
 
 /* Primitive visibility bits, as include/common/visibility.h:28-33 (Visible,
  * CastsShadows).  The mask passed per primitive is the AND of the primitive's
@@ -102,6 +103,7 @@ typedef struct ymx_accel_info
 	float build_ms;          /* device time of the GPU build kernels */
 	float bound_min[3];      /* tree bound (getBound()) */
 	float bound_max[3];
+	uint32_t n_quads;        /* quad primitives (ymx_accel_create_faces) */
 } ymx_accel_info;
 
 /* Build the accelerator over n_tris triangles on the current HIP device.
@@ -114,6 +116,21 @@ typedef struct ymx_accel_info
 int ymx_accel_create(const float *verts, uint32_t n_verts, const uint32_t *tri_idx,
                      const uint8_t *vis, uint32_t n_tris, const ymx_accel_params *params,
                      ymx_accel **out);
+
+/* The same over a face list of triangles and quads, as the reference's meshes
+ * hold them (yafaray_addTriangle / yafaray_addQuad,
+ * include/public_api/yafaray_c_api.h:219-222, one PrimitivePolygon<float, 3 or
+ * 4> per face, src/geometry/object/object_mesh.cc:96-112).  face_idx: host, 4
+ * vertex indices per face; face_idx[4 i + 3] == YMX_NO_VERTEX makes face i a
+ * triangle, otherwise it is the quad (v0, v1, v2, v3).  A quad is ONE
+ * primitive: its id is its face index, and a hit on it is the reference's quad
+ * test (include/geometry/shape/shape_polygon.h:126-176): the second triangle
+ * (v0, v2, v3) is tried only when the first triangle's u is outside [0, 1],
+ * and uv comes back as {u + v, v} from the first triangle and {u, u + v} from
+ * the second.  Every query below accepts handles of either kind. */
+int ymx_accel_create_faces(const float *verts, uint32_t n_verts, const uint32_t *face_idx,
+                           const uint8_t *vis, uint32_t n_faces, const ymx_accel_params *params,
+                           ymx_accel **out);
 
 /* Release the handle and its device memory (blocks until its work is done). */
 int ymx_accel_destroy(ymx_accel *accel);
@@ -170,9 +187,16 @@ int ymx_accel_intersect_transparent(const ymx_accel *accel, const ymx_ray *d_ray
  * any other negative ref a leaf 0x80000000 | first << 4 | (count - 1) over
  * triangle records [first, first + count).  tris: n_tris x 48 bytes (one
  * record when n_tris is 0), 3 x float4: (v0, primitive id bits), (v1 - v0,
- * visibility bits), (v2 - v0, 0).  The byte counts must equal those of
+ * visibility bits), (v2 - v0, quad index bits: 0 for a triangle).  The byte counts must equal those of
  * ymx_accel_get_info. */
 int ymx_accel_export(const ymx_accel *accel, void *nodes, uint64_t node_bytes, void *tris, uint64_t tri_bytes);
+
+/* The quad records a handle from ymx_accel_create_faces adds to ymx_accel_export's
+ * (no counterpart in the reference; for the same host walk): a primitive
+ * record whose third float4 has a non-zero w (as uint32 bits) q is a quad, and
+ * quads[q - 1] (16 bytes: e3 = v3 - v0, 0) completes it.  quad_bytes must be
+ * 16 x n_quads of ymx_accel_get_info. */
+int ymx_accel_export_quads(const ymx_accel *accel, void *quads, uint64_t quad_bytes);
 
 /* Counting replay of the query kernels: total 64-byte nodes fetched and
  * triangles tested over the batch.  query: 0 closest, 1 shadow.  Synchronous;

@@ -27,7 +27,11 @@ def _torch():
 
 
 class GpuAccelerator:
-    """Immutable LBVH over a triangle list; queries are batched on the device."""
+    """Immutable BVH over a primitive list; queries are batched on the device.
+
+    tris: (n, 3) triangles, or (n, 4) faces whose 4th index is YMX_NO_VERTEX
+    for a triangle and a vertex for a quad (one primitive per face, tested as
+    the reference's ShapePolygon<float, 4>)."""
 
     type_name = "mi355x-lbvh"
 
@@ -35,7 +39,9 @@ class GpuAccelerator:
                  max_leaf_size: int = 0, builder: str = "lbvh"):
         self._lib = native.lib()
         verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
-        tris = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1, 3)
+        tris = np.ascontiguousarray(tris, dtype=np.uint32)
+        faces = tris.ndim == 2 and tris.shape[1] == 4
+        tris = tris.reshape(-1, 4 if faces else 3)
         if vis is not None:
             vis = np.ascontiguousarray(vis, dtype=np.uint8).reshape(-1)
             if vis.shape[0] != tris.shape[0]:
@@ -44,7 +50,8 @@ class GpuAccelerator:
             raise ValueError(f"unknown BVH builder {builder!r} (one of {sorted(native.BUILDERS)})")
         params = YmxAccelParams(C.sizeof(YmxAccelParams), int(max_leaf_size), native.BUILDERS[builder])
         handle = C.c_void_p()
-        rc = self._lib.ymx_accel_create(
+        create = self._lib.ymx_accel_create_faces if faces else self._lib.ymx_accel_create
+        rc = create(
             verts.ctypes.data_as(C.c_void_p), verts.shape[0],
             tris.ctypes.data_as(C.c_void_p), None if vis is None else vis.ctypes.data_as(C.c_void_p),
             tris.shape[0], C.byref(params), C.byref(handle))
@@ -80,7 +87,7 @@ class GpuAccelerator:
         return {
             "n_tris": inf.n_tris, "n_nodes": inf.n_nodes, "max_depth": inf.max_depth,
             "node_bytes": inf.node_bytes, "tri_bytes": inf.tri_bytes, "build_ms": inf.build_ms,
-            "bound_min": list(inf.bound_min), "bound_max": list(inf.bound_max),
+            "bound_min": list(inf.bound_min), "bound_max": list(inf.bound_max), "n_quads": inf.n_quads,
         }
 
     # --- batched queries on device tensors -------------------------------

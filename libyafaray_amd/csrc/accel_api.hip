@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <new>
+#include <vector>
 
 namespace {
 thread_local char g_last_error[512] = "";
@@ -15,6 +16,7 @@ void free_accel(ymx_accel *acc)
 	if(!acc) return;
 	if(acc->d_nodes) (void)hipFree(acc->d_nodes);
 	if(acc->d_tris) (void)hipFree(acc->d_tris);
+	if(acc->d_quads) (void)hipFree(acc->d_quads);
 	ymx_free_work_areas(acc);
 	delete acc;
 }
@@ -32,24 +34,24 @@ extern "C" {
 
 const char *ymx_last_error(void) { return g_last_error; }
 
-int ymx_accel_create(const float *verts, uint32_t n_verts, const uint32_t *tri_idx, const uint8_t *vis,
-                     uint32_t n_tris, const ymx_accel_params *params, ymx_accel **out)
+static int create_accel(const char *fn, const float *verts, uint32_t n_verts, const uint32_t *idx, uint32_t stride,
+                        const uint8_t *vis, uint32_t n_tris, const ymx_accel_params *params, ymx_accel **out)
 {
 	g_last_error[0] = '\0';
 	if(!out)
 	{
-		ymx_set_error("ymx_accel_create: out is NULL");
+		ymx_set_error("%s: out is NULL", fn);
 		return YMX_ERR_INVALID_ARG;
 	}
 	*out = nullptr;
-	if(n_tris > 0 && (!verts || !tri_idx || n_verts == 0))
+	if(n_tris > 0 && (!verts || !idx || n_verts == 0))
 	{
-		ymx_set_error("ymx_accel_create: NULL geometry with %u triangles", n_tris);
+		ymx_set_error("%s: NULL geometry with %u primitives", fn, n_tris);
 		return YMX_ERR_INVALID_ARG;
 	}
 	if(n_tris > YMX_LEAF_MAX_FIRST)
 	{
-		ymx_set_error("ymx_accel_create: %u triangles exceed the limit of %u", n_tris, YMX_LEAF_MAX_FIRST);
+		ymx_set_error("%s: %u primitives exceed the limit of %u", fn, n_tris, YMX_LEAF_MAX_FIRST);
 		return YMX_ERR_UNSUPPORTED;
 	}
 	// 4 triangles per leaf: on the 2.19M-triangle heightfield +2 % incoherent
@@ -59,45 +61,58 @@ int ymx_accel_create(const float *verts, uint32_t n_verts, const uint32_t *tri_i
 	{
 		if(params->struct_size != sizeof(ymx_accel_params))
 		{
-			ymx_set_error("ymx_accel_create: params.struct_size %u != %zu", params->struct_size,
-			              sizeof(ymx_accel_params));
+			ymx_set_error("%s: params.struct_size %u != %zu", fn, params->struct_size, sizeof(ymx_accel_params));
 			return YMX_ERR_INVALID_ARG;
 		}
 		if(params->max_leaf_size < 0 || params->max_leaf_size > 8)
 		{
-			ymx_set_error("ymx_accel_create: max_leaf_size %d outside 0..8", params->max_leaf_size);
+			ymx_set_error("%s: max_leaf_size %d outside 0..8", fn, params->max_leaf_size);
 			return YMX_ERR_INVALID_ARG;
 		}
 		if(params->max_leaf_size > 0) max_leaf = params->max_leaf_size;
 		if(params->builder != YMX_BUILDER_LBVH && params->builder != YMX_BUILDER_PLOC)
 		{
-			ymx_set_error("ymx_accel_create: unknown builder %d", params->builder);
+			ymx_set_error("%s: unknown builder %d", fn, params->builder);
 			return YMX_ERR_INVALID_ARG;
 		}
 		builder = params->builder;
 	}
-	for(size_t i = 0; i < 3 * (size_t)n_tris; ++i)
+	// a face's 4th index is YMX_NO_VERTEX for a triangle; quads get e3 slots
+	std::vector<uint32_t> slot;
+	uint32_t n_quads = 0;
+	for(size_t i = 0; i < (size_t)n_tris; ++i)
 	{
-		if(tri_idx[i] >= n_verts)
+		const bool quad = stride == 4 && idx[4 * i + 3] != YMX_NO_VERTEX;
+		for(uint32_t k = 0; k < (quad ? 4u : 3u); ++k)
 		{
-			ymx_set_error("ymx_accel_create: triangle %zu references vertex %u >= n_verts %u", i / 3, tri_idx[i],
-			              n_verts);
-			return YMX_ERR_INVALID_ARG;
+			if(idx[stride * i + k] >= n_verts)
+			{
+				ymx_set_error("%s: primitive %zu references vertex %u >= n_verts %u", fn, i, idx[stride * i + k],
+				              n_verts);
+				return YMX_ERR_INVALID_ARG;
+			}
+		}
+		if(quad)
+		{
+			if(slot.empty()) slot.assign(n_tris, 0u);
+			slot[i] = ++n_quads;
 		}
 	}
 	ymx_accel *acc = new(std::nothrow) ymx_accel();
 	if(!acc)
 	{
-		ymx_set_error("ymx_accel_create: host allocation failed");
+		ymx_set_error("%s: host allocation failed", fn);
 		return YMX_ERR_OUT_OF_MEMORY;
 	}
 	if(hipGetDevice(&acc->device) != hipSuccess)
 	{
-		ymx_set_error("ymx_accel_create: no HIP device");
+		ymx_set_error("%s: no HIP device", fn);
 		delete acc;
 		return YMX_ERR_HIP;
 	}
-	int rc = ymx_build_device(acc, verts, n_verts, tri_idx, vis, n_tris, max_leaf, builder);
+	acc->n_quads = n_quads;
+	int rc = ymx_build_device(acc, verts, n_verts, idx, stride, n_quads ? slot.data() : nullptr, vis, n_tris, max_leaf,
+	                          builder);
 	if(rc == YMX_OK) rc = ymx_trace_setup(acc);
 	if(rc != YMX_OK)
 	{
@@ -106,6 +121,18 @@ int ymx_accel_create(const float *verts, uint32_t n_verts, const uint32_t *tri_i
 	}
 	*out = acc;
 	return YMX_OK;
+}
+
+int ymx_accel_create(const float *verts, uint32_t n_verts, const uint32_t *tri_idx, const uint8_t *vis,
+                     uint32_t n_tris, const ymx_accel_params *params, ymx_accel **out)
+{
+	return create_accel("ymx_accel_create", verts, n_verts, tri_idx, 3, vis, n_tris, params, out);
+}
+
+int ymx_accel_create_faces(const float *verts, uint32_t n_verts, const uint32_t *face_idx, const uint8_t *vis,
+                           uint32_t n_faces, const ymx_accel_params *params, ymx_accel **out)
+{
+	return create_accel("ymx_accel_create_faces", verts, n_verts, face_idx, 4, vis, n_faces, params, out);
 }
 
 int ymx_accel_destroy(ymx_accel *accel)
@@ -141,6 +168,17 @@ int ymx_accel_export(const ymx_accel *accel, void *nodes, uint64_t node_bytes, v
 	return YMX_OK;
 }
 
+int ymx_accel_export_quads(const ymx_accel *accel, void *quads, uint64_t quad_bytes)
+{
+	if(!accel || quad_bytes != 16ull * accel->n_quads || (accel->n_quads && !quads))
+	{
+		ymx_set_error("ymx_accel_export_quads: NULL buffer or a byte count other than 16 x n_quads");
+		return YMX_ERR_INVALID_ARG;
+	}
+	if(accel->n_quads) YMX_HIP_CHECK(hipMemcpy(quads, accel->d_quads, quad_bytes, hipMemcpyDeviceToHost));
+	return YMX_OK;
+}
+
 int ymx_accel_get_info(const ymx_accel *accel, ymx_accel_info *info)
 {
 	if(!accel || !info || info->struct_size != sizeof(ymx_accel_info))
@@ -154,6 +192,7 @@ int ymx_accel_get_info(const ymx_accel *accel, ymx_accel_info *info)
 	info->node_bytes = 64ull * accel->n_nodes;
 	info->tri_bytes = 48ull * (accel->n_tris ? accel->n_tris : 1);
 	info->build_ms = accel->build_ms;
+	info->n_quads = accel->n_quads;
 	memcpy(info->bound_min, accel->bmin, sizeof info->bound_min);
 	memcpy(info->bound_max, accel->bmax, sizeof info->bound_max);
 	return YMX_OK;

@@ -59,9 +59,11 @@ __global__ void k_init_bounds(int *b)
 }
 
 __global__ __launch_bounds__(kBuildBlock) void k_prep(const float *__restrict__ verts,
-                                                      const uint32_t *__restrict__ idx,
+                                                      const uint32_t *__restrict__ idx, uint32_t stride,
+                                                      const uint32_t *__restrict__ quad_slot,
                                                       const uint8_t *__restrict__ vis, uint32_t n,
                                                       float4 *__restrict__ tris_unsorted,
+                                                      float4 *__restrict__ quads,
                                                       float *__restrict__ prim_box,
                                                       float *__restrict__ centroid, int *bounds)
 {
@@ -70,8 +72,9 @@ __global__ __launch_bounds__(kBuildBlock) void k_prep(const float *__restrict__ 
 	float clo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, chi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
 	if(i < n)
 	{
-		const uint32_t a = idx[3 * (size_t)i], b = idx[3 * (size_t)i + 1], c = idx[3 * (size_t)i + 2];
-		float v[3][3];
+		const uint32_t a = idx[stride * (size_t)i], b = idx[stride * (size_t)i + 1], c = idx[stride * (size_t)i + 2];
+		const uint32_t slot = quad_slot ? quad_slot[i] : 0u;
+		float v[4][3];
 		for(int k = 0; k < 3; ++k)
 		{
 			v[0][k] = verts[3 * (size_t)a + k];
@@ -82,6 +85,21 @@ __global__ __launch_bounds__(kBuildBlock) void k_prep(const float *__restrict__ 
 		{
 			lo[k] = fminf(v[0][k], fminf(v[1][k], v[2][k]));
 			hi[k] = fmaxf(v[0][k], fmaxf(v[1][k], v[2][k]));
+		}
+		if(slot)
+		{
+			// FacePrimitive::getBound over all four vertices
+			const uint32_t d = idx[stride * (size_t)i + 3];
+			for(int k = 0; k < 3; ++k)
+			{
+				v[3][k] = verts[3 * (size_t)d + k];
+				lo[k] = fminf(lo[k], v[3][k]);
+				hi[k] = fmaxf(hi[k], v[3][k]);
+			}
+			quads[slot - 1u] = make_float4(v[3][0] - v[0][0], v[3][1] - v[0][1], v[3][2] - v[0][2], 0.f);
+		}
+		for(int k = 0; k < 3; ++k)
+		{
 			clo[k] = chi[k] = 0.5f * (lo[k] + hi[k]);
 			prim_box[6 * (size_t)i + k] = lo[k];
 			prim_box[6 * (size_t)i + 3 + k] = hi[k];
@@ -92,7 +110,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_prep(const float *__restrict__ 
 		tris_unsorted[3 * (size_t)i + 1] = make_float4(v[1][0] - v[0][0], v[1][1] - v[0][1],
 		                                               v[1][2] - v[0][2], __uint_as_float(vmask));
 		tris_unsorted[3 * (size_t)i + 2] = make_float4(v[2][0] - v[0][0], v[2][1] - v[0][1],
-		                                               v[2][2] - v[0][2], 0.f);
+		                                               v[2][2] - v[0][2], __uint_as_float(slot));
 	}
 	// wave64 reduction, then one atomic per wave
 	for(int off = 32; off > 0; off >>= 1)
@@ -821,8 +839,8 @@ int relayout_tree(DevBuf &d_nodes, DevBuf &d_tris, uint32_t n_tris, bool ploc_le
 
 }  // namespace
 
-int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const uint32_t *idx,
-                     const uint8_t *vis, uint32_t n_tris, int max_leaf, int builder)
+int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const uint32_t *idx, uint32_t stride,
+                     const uint32_t *quad_slot, const uint8_t *vis, uint32_t n_tris, int max_leaf, int builder)
 {
 	acc->n_tris = n_tris;
 	acc->build_ms = 0.f;
@@ -839,23 +857,33 @@ int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const
 		float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {0.f, 0.f, 0.f};
 		float4 tri[3] = {};
 		const int32_t c0 = ymx_leaf_ref(0, 1);
+		const bool quad = n_tris == 1 && quad_slot && quad_slot[0];
+		float4 e3 = {};
 		if(n_tris == 1)
 		{
-			float v[3][3];
-			for(int c = 0; c < 3; ++c)
+			float v[4][3];
+			const int nv = quad ? 4 : 3;
+			for(int c = 0; c < nv; ++c)
 				for(int k = 0; k < 3; ++k) v[c][k] = verts[3 * (size_t)idx[c] + k];
 			for(int k = 0; k < 3; ++k)
 			{
 				lo[k] = fminf(v[0][k], fminf(v[1][k], v[2][k]));
 				hi[k] = fmaxf(v[0][k], fmaxf(v[1][k], v[2][k]));
+				if(quad)
+				{
+					lo[k] = fminf(lo[k], v[3][k]);
+					hi[k] = fmaxf(hi[k], v[3][k]);
+				}
 			}
-			uint32_t zero = 0, vm = vis ? vis[0] : 3u;
-			float fz, fv;
+			uint32_t zero = 0, vm = vis ? vis[0] : 3u, one = quad ? 1u : 0u;
+			float fz, fv, fq;
 			memcpy(&fz, &zero, 4);
 			memcpy(&fv, &vm, 4);
+			memcpy(&fq, &one, 4);
 			tri[0] = make_float4(v[0][0], v[0][1], v[0][2], fz);
 			tri[1] = make_float4(v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2], fv);
-			tri[2] = make_float4(v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2], 0.f);
+			tri[2] = make_float4(v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2], fq);
+			if(quad) e3 = make_float4(v[3][0] - v[0][0], v[3][1] - v[0][1], v[3][2] - v[0][2], 0.f);
 		}
 		else
 		{
@@ -886,6 +914,15 @@ int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const
 		YMX_HIP_CHECK(dt.alloc(sizeof tri));
 		YMX_HIP_CHECK(hipMemcpy(dn.p, node, sizeof node, hipMemcpyHostToDevice));
 		YMX_HIP_CHECK(hipMemcpy(dt.p, tri, sizeof tri, hipMemcpyHostToDevice));
+		if(quad)
+		{
+			DevBuf dq;
+			YMX_HIP_CHECK(dq.alloc(sizeof e3));
+			YMX_HIP_CHECK(hipMemcpy(dq.p, &e3, sizeof e3, hipMemcpyHostToDevice));
+			acc->d_quads = dq.as<float4>();
+			acc->n_quads = 1;
+			dq.release();
+		}
 		acc->d_nodes = dn.as<float4>();
 		acc->d_tris = dt.as<float4>();
 		dn.release();
@@ -897,9 +934,15 @@ int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const
 	}
 
 	DevBuf d_verts, d_idx, d_vis, d_tris_unsorted, d_prim_box, d_centroid, d_bounds, d_keys, d_keys_alt,
-	    d_vals, d_vals_alt, d_nodes, d_tris, d_parent_int, d_parent_leaf, d_depth, d_temp;
+	    d_vals, d_vals_alt, d_nodes, d_tris, d_parent_int, d_parent_leaf, d_depth, d_temp, d_slot, d_quads;
 	YMX_HIP_CHECK(d_verts.alloc(sizeof(float) * 3 * (size_t)n_verts));
-	YMX_HIP_CHECK(d_idx.alloc(sizeof(uint32_t) * 3 * (size_t)n_tris));
+	YMX_HIP_CHECK(d_idx.alloc(sizeof(uint32_t) * stride * (size_t)n_tris));
+	if(quad_slot)
+	{
+		YMX_HIP_CHECK(d_slot.alloc(sizeof(uint32_t) * (size_t)n_tris));
+		YMX_HIP_CHECK(d_quads.alloc(sizeof(float4) * (size_t)(acc->n_quads ? acc->n_quads : 1)));
+		YMX_HIP_CHECK(hipMemcpy(d_slot.p, quad_slot, sizeof(uint32_t) * (size_t)n_tris, hipMemcpyHostToDevice));
+	}
 	if(vis) YMX_HIP_CHECK(d_vis.alloc(n_tris));
 	YMX_HIP_CHECK(d_tris_unsorted.alloc(sizeof(float4) * 3 * (size_t)n_tris));
 	YMX_HIP_CHECK(d_prim_box.alloc(sizeof(float) * 6 * (size_t)n_tris));
@@ -916,7 +959,7 @@ int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const
 	YMX_HIP_CHECK(d_depth.alloc(sizeof(uint32_t)));
 
 	YMX_HIP_CHECK(hipMemcpy(d_verts.p, verts, sizeof(float) * 3 * (size_t)n_verts, hipMemcpyHostToDevice));
-	YMX_HIP_CHECK(hipMemcpy(d_idx.p, idx, sizeof(uint32_t) * 3 * (size_t)n_tris, hipMemcpyHostToDevice));
+	YMX_HIP_CHECK(hipMemcpy(d_idx.p, idx, sizeof(uint32_t) * stride * (size_t)n_tris, hipMemcpyHostToDevice));
 	if(vis) YMX_HIP_CHECK(hipMemcpy(d_vis.p, vis, n_tris, hipMemcpyHostToDevice));
 
 	EventPair ev;
@@ -925,10 +968,10 @@ int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const
 	YMX_HIP_CHECK(hipEventRecord(ev.a, stream));
 
 	k_init_bounds<<<1, 64, 0, stream>>>(d_bounds.as<int>());
-	k_prep<<<grid_for(n_tris), kBuildBlock, 0, stream>>>(d_verts.as<float>(), d_idx.as<uint32_t>(),
-	                                                     vis ? d_vis.as<uint8_t>() : nullptr, n_tris,
-	                                                     d_tris_unsorted.as<float4>(), d_prim_box.as<float>(),
-	                                                     d_centroid.as<float>(), d_bounds.as<int>());
+	k_prep<<<grid_for(n_tris), kBuildBlock, 0, stream>>>(
+	    d_verts.as<float>(), d_idx.as<uint32_t>(), stride, quad_slot ? d_slot.as<uint32_t>() : nullptr,
+	    vis ? d_vis.as<uint8_t>() : nullptr, n_tris, d_tris_unsorted.as<float4>(),
+	    quad_slot ? d_quads.as<float4>() : nullptr, d_prim_box.as<float>(), d_centroid.as<float>(), d_bounds.as<int>());
 	YMX_HIP_CHECK(hipGetLastError());
 	int hb[12];
 	YMX_HIP_CHECK(hipMemcpyAsync(hb, d_bounds.p, sizeof hb, hipMemcpyDeviceToHost, stream));
@@ -1095,6 +1138,11 @@ int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const
 	acc->d_tris = d_tris.as<float4>();
 	d_nodes.release();
 	d_tris.release();
+	if(quad_slot)
+	{
+		acc->d_quads = d_quads.as<float4>();
+		d_quads.release();
+	}
 	reference_tree_bound(scene_lo, scene_hi, n_tris, acc->bmin, acc->bmax);
 	return YMX_OK;
 }

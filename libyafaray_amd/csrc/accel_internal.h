@@ -13,7 +13,9 @@
 //   child ref == YMX_REF_EMPTY : no child (its box is empty)
 //   triangle j (48 B, 3 x float4), in leaf order:
 //     t0 = (v0.xyz, prim id bits)   t1 = (e1 = v1 - v0, visibility bits)
-//     t2 = (e2 = v2 - v0, 0)
+//     t2 = (e2 = v2 - v0, quad)
+//   quad = 0 for a triangle; for a quad primitive (v0, v1, v2, v3) the
+//   1-based index of its e3 = v3 - v0 in the quad array (float4 each)
 //   e1/e2 are the same FP32 differences the reference forms in
 //   ShapePolygon::intersect (include/geometry/shape/shape_polygon.h:130-131).
 #pragma once
@@ -63,6 +65,8 @@ struct ymx_accel
 	float build_ms;
 	float4 *d_nodes;
 	float4 *d_tris;
+	float4 *d_quads;         // e3 of each quad primitive (NULL without quads)
+	uint32_t n_quads;
 	// Traversal stack: YMX_LDS_STACK entries in LDS plus spill_depth in HBM,
 	// max_depth + 1 in all.  A lane at an internal node of depth D holds at
 	// most D deferred siblings (one per level of its root path), and internal
@@ -88,6 +92,7 @@ struct ymx_trace_args
 {
 	const float4 *nodes;
 	const float4 *tris;
+	const float4 *quads;
 	const ymx_ray *rays;
 	uint32_t n_rays;          // ray count, or the capacity when n_dev is set
 	const uint32_t *n_dev;    // optional device-side ray count (queues)
@@ -109,8 +114,11 @@ struct ymx_trace_args
 };
 
 // Build entry (accel_build.hip) and launch entries (accel_trace.hip).
-int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const uint32_t *idx,
-                     const uint8_t *vis, uint32_t n_tris, int max_leaf, int builder);
+// idx: `stride` vertex indices per primitive (3, or 4 with quad_slot);
+// quad_slot (NULL: no quads): 0 for a triangle, else the 1-based slot of the
+// quad's e3 in acc->d_quads (acc->n_quads slots).
+int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const uint32_t *idx, uint32_t stride,
+                     const uint32_t *quad_slot, const uint8_t *vis, uint32_t n_tris, int max_leaf, int builder);
 int ymx_launch_closest(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t n,
                        hipStream_t stream, unsigned long long *stats);
 int ymx_launch_shadow(const ymx_accel *acc, const ymx_ray *rays, uint32_t *occ, uint32_t *prim,

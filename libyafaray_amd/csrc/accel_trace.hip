@@ -35,7 +35,7 @@ struct RayState
 };
 
 using ymx::tree_cross;
-using ymx::tri_test;
+using ymx::prim_test;
 
 // Slab test of both children of one node.  Returns bit 0 / bit 1 for child
 // 0 / 1 and their entry distances.
@@ -245,7 +245,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(ymx_trace_args args)
 				const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
 				if(kStats) ++n_tests;
 				float u, v;
-				const float t = tri_test(t0, t1, t2, r.o, r.d, u, v);
+				const float t = prim_test(t0, t1, t2, args.quads, r.o, r.d, u, v);
 				if(t <= 0.f || t < t_min) continue;
 				const uint32_t prim = __float_as_uint(t0.w);
 				const uint32_t vmask = __float_as_uint(t1.w);
@@ -470,7 +470,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_dyn(ymx_trace_args args)
 						const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
 						if(kStats) ++n_tests;
 						float u, v;
-						const float t = tri_test(t0, t1, t2, r.o, r.d, u, v);
+						const float t = prim_test(t0, t1, t2, args.quads, r.o, r.d, u, v);
 						if(t <= 0.f || t < t_min) continue;
 						const uint32_t prim = __float_as_uint(t0.w);
 						const uint32_t vmask = __float_as_uint(t1.w);
@@ -545,6 +545,7 @@ int launch(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t *o
 	ymx_trace_args a;
 	a.nodes = acc->d_nodes;
 	a.tris = acc->d_tris;
+	a.quads = acc->d_quads;
 	a.rays = rays;
 	a.n_rays = n;
 	a.n_dev = n_dev;

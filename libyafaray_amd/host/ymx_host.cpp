@@ -100,6 +100,20 @@ std::unique_ptr<Accelerator> Accelerator::factory(const ParamMap &params, const 
                                                   const uint32_t *tri_idx, const uint8_t *vis, uint32_t n_tris,
                                                   std::string *error)
 {
+	return create(params, verts, n_verts, tri_idx, false, vis, n_tris, error);
+}
+
+std::unique_ptr<Accelerator> Accelerator::factoryFaces(const ParamMap &params, const float *verts, uint32_t n_verts,
+                                                       const uint32_t *face_idx, const uint8_t *vis, uint32_t n_faces,
+                                                       std::string *error)
+{
+	return create(params, verts, n_verts, face_idx, true, vis, n_faces, error);
+}
+
+std::unique_ptr<Accelerator> Accelerator::create(const ParamMap &params, const float *verts, uint32_t n_verts,
+                                                 const uint32_t *tri_idx, bool faces, const uint8_t *vis,
+                                                 uint32_t n_tris, std::string *error)
+{
 	const std::string type = params.getString("type", "");
 	if(type != "mi355x-lbvh")
 	{
@@ -117,7 +131,9 @@ std::unique_ptr<Accelerator> Accelerator::factory(const ParamMap &params, const 
 	}
 	p.builder = builder == "ploc" ? YMX_BUILDER_PLOC : YMX_BUILDER_LBVH;
 	std::unique_ptr<Accelerator> a(new Accelerator());
-	if(ymx_accel_create(verts, n_verts, tri_idx, vis, n_tris, &p, &a->acc_) != YMX_OK)
+	const int rc = faces ? ymx_accel_create_faces(verts, n_verts, tri_idx, vis, n_tris, &p, &a->acc_)
+	                     : ymx_accel_create(verts, n_verts, tri_idx, vis, n_tris, &p, &a->acc_);
+	if(rc != YMX_OK)
 	{
 		if(error) *error = std::string("Accelerator: ") + ymx_last_error();
 		return nullptr;
@@ -132,7 +148,9 @@ std::unique_ptr<Accelerator> Accelerator::factory(const ParamMap &params, const 
 	a->n_tris_ = n_tris;
 	a->nodes_.resize(info.node_bytes / 4);
 	a->tris_.resize(info.tri_bytes / 4);
-	if(ymx_accel_export(a->acc_, a->nodes_.data(), info.node_bytes, a->tris_.data(), info.tri_bytes) != YMX_OK)
+	a->quads_.resize(4 * (size_t)info.n_quads);
+	if(ymx_accel_export(a->acc_, a->nodes_.data(), info.node_bytes, a->tris_.data(), info.tri_bytes) != YMX_OK ||
+	   ymx_accel_export_quads(a->acc_, a->quads_.data(), 16ull * info.n_quads) != YMX_OK)
 	{
 		if(error) *error = std::string("Accelerator: ") + ymx_last_error();
 		return nullptr;
@@ -142,6 +160,7 @@ std::unique_ptr<Accelerator> Accelerator::factory(const ParamMap &params, const 
 		a->bound_.a[k] = info.bound_min[k];
 		a->bound_.g[k] = info.bound_max[k];
 	}
+	a->max_depth_ = info.max_depth;
 	return a;
 }
 
@@ -187,6 +206,11 @@ IntersectData Accelerator::traverse(const Ray &ray, float t_max, Mode mode,
 			{
 				if(tn1 < tn0) std::swap(c0, c1);
 				stack.push_back(c1);
+				const uint32_t used = (uint32_t)stack.size();
+				uint32_t seen = max_stack_.load(std::memory_order_relaxed);
+				while(used > seen && !max_stack_.compare_exchange_weak(seen, used, std::memory_order_relaxed))
+				{
+				}
 				ref = c0;
 			}
 			else if(m == 1) ref = c0;
@@ -204,7 +228,7 @@ IntersectData Accelerator::traverse(const Ray &ray, float t_max, Mode mode,
 			const float *t = tris_.data() + 12 * (size_t)(first + k);
 			const float4 t0{t[0], t[1], t[2], t[3]}, t1{t[4], t[5], t[6], t[7]}, t2{t[8], t[9], t[10], t[11]};
 			float u, v;
-			const float th = ymx::tri_test(t0, t1, t2, r.o, r.d, u, v);
+			const float th = ymx::prim_test(t0, t1, t2, reinterpret_cast<const float4 *>(quads_.data()), r.o, r.d, u, v);
 			if(th <= 0.f || th < t_min) continue;
 			const uint32_t prim = (uint32_t)f_bits(t0.w);
 			const uint32_t vmask = (uint32_t)f_bits(t1.w);

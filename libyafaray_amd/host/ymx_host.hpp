@@ -23,6 +23,7 @@
 #include <array>
 #include <cfloat>
 #include <cstdint>
+#include <atomic>
 #include <functional>
 #include <map>
 #include <memory>
@@ -107,6 +108,12 @@ class Accelerator
 		static std::unique_ptr<Accelerator> factory(const ParamMap &params, const float *verts, uint32_t n_verts,
 		                                            const uint32_t *tri_idx, const uint8_t *vis, uint32_t n_tris,
 		                                            std::string *error);
+		// The same over faces of 4 indices each, face_idx[4 i + 3] ==
+		// YMX_NO_VERTEX for a triangle, else a quad (one primitive per face;
+		// ymx_accel_create_faces).
+		static std::unique_ptr<Accelerator> factoryFaces(const ParamMap &params, const float *verts, uint32_t n_verts,
+		                                                 const uint32_t *face_idx, const uint8_t *vis,
+		                                                 uint32_t n_faces, std::string *error);
 		~Accelerator();
 		Accelerator(const Accelerator &) = delete;
 		Accelerator &operator=(const Accelerator &) = delete;
@@ -128,9 +135,16 @@ class Accelerator
 
 		const ymx_accel *handle() const { return acc_; }
 		uint32_t numPrimitives() const { return n_tris_; }
+		uint32_t maxDepth() const { return max_depth_; }
+		// the deepest traversal stack the host queries have used so far (never
+		// above maxDepth(), which sizes the GPU kernels' stacks)
+		uint32_t maxStackUsed() const { return max_stack_.load(); }
 
 	private:
 		Accelerator() = default;
+		static std::unique_ptr<Accelerator> create(const ParamMap &params, const float *verts, uint32_t n_verts,
+		                                           const uint32_t *idx, bool faces, const uint8_t *vis, uint32_t n,
+		                                           std::string *error);
 		enum class Mode { Closest, Shadow, Transparent };
 		// Traversal of the host copy with the GPU kernels' arithmetic and
 		// culling.  Closest: returns the nearest visible hit in [t_min, t_max)
@@ -142,7 +156,10 @@ class Accelerator
 		uint32_t n_tris_ = 0;
 		std::vector<float> nodes_;  // 16 floats per node
 		std::vector<float> tris_;   // 12 floats per triangle record
+		std::vector<float> quads_;  // 4 floats (e3, 0) per quad record
 		Bound bound_{};
+		uint32_t max_depth_ = 0;
+		mutable std::atomic<uint32_t> max_stack_{0};
 };
 
 // ---------------------------------------------------------------- path tracer

@@ -14,6 +14,7 @@ LIB_PATH = PKG / "libymx.so"
 
 YMX_OK = 0
 YMX_NO_HIT = 0xFFFFFFFF
+YMX_NO_VERTEX = 0xFFFFFFFF
 YMX_VIS_VISIBLE = 1
 YMX_VIS_CASTS_SHADOWS = 2
 
@@ -45,6 +46,7 @@ class YmxAccelInfo(C.Structure):
         ("build_ms", C.c_float),
         ("bound_min", C.c_float * 3),
         ("bound_max", C.c_float * 3),
+        ("n_quads", C.c_uint32),
     ]
 
 
@@ -71,6 +73,9 @@ SIGNATURES = {
                                    C.POINTER(C.c_int32)]),
     "ymx_accel_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
                                    C.POINTER(YmxAccelParams), C.POINTER(C.c_void_p)]),
+    "ymx_accel_create_faces": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
+                                         C.POINTER(YmxAccelParams), C.POINTER(C.c_void_p)]),
+    "ymx_accel_export_quads": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     "ymx_accel_destroy": (C.c_int, [C.c_void_p]),
     "ymx_accel_bound": (C.c_int, [C.c_void_p, C.c_float * 3, C.c_float * 3]),
     "ymx_accel_get_info": (C.c_int, [C.c_void_p, C.POINTER(YmxAccelInfo)]),

@@ -22,6 +22,7 @@ from __future__ import annotations
 import numpy as np
 
 F32 = np.float32
+NO_VERTEX = 0xFFFFFFFF  # 4th index of a triangle in a face list
 
 
 def _hash2(ix: np.ndarray, iy: np.ndarray, seed: int) -> np.ndarray:
@@ -85,6 +86,37 @@ def grid_mesh(heights: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     tris[0::2] = np.stack([v00, v01, v11], axis=-1)
     tris[1::2] = np.stack([v00, v11, v10], axis=-1)
     return verts, tris
+
+
+def grid_quads(heights: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """The heightfield of grid_mesh with one quad face per cell, (v00, v01,
+    v11, v10) (non-planar where the cell's corners are), as faces of 4 indices."""
+    res = heights.shape[0]
+    verts, _ = grid_mesh(heights)
+    i, j = np.meshgrid(np.arange(res - 1), np.arange(res - 1), indexing="ij")
+    v00 = (i * res + j).reshape(-1)
+    faces = np.stack([v00, v00 + 1, v00 + res + 1, v00 + res], axis=-1).astype(np.uint32)
+    return verts, faces
+
+
+def quad_face_scene(res: int = 64, n_random: int = 2000, seed: int = 0):
+    """Faces of 4 indices (4th NO_VERTEX for a triangle): the heightfield as
+    quads, random planar quads (parallelograms) and random slightly bent quads
+    above it, and random triangles."""
+    rng = np.random.default_rng(seed)
+    hv, hf = grid_quads(fractal_heights(res, seed=seed))
+    n = n_random
+    c = rng.uniform((0.0, 0.0, 0.22), (1.0, 1.0, 0.5), size=(n, 3))
+    a = unit_vectors(rng, n) * rng.uniform(0.01, 0.03, (n, 1))
+    b = unit_vectors(rng, n) * rng.uniform(0.01, 0.03, (n, 1))
+    q = np.stack([c - a - b, c + a - b, c + a + b, c - a + b], 1)
+    q[1::2, 3] += unit_vectors(rng, (n + 1) // 2)[: q[1::2].shape[0]] * 0.004
+    qv = q.reshape(-1, 3).astype(F32)
+    qf = np.arange(4 * n, dtype=np.uint32).reshape(n, 4)
+    qf[::4, 3] = NO_VERTEX  # a quarter of them triangles (v0, v1, v2)
+    verts = np.concatenate([hv, qv]).astype(F32)
+    faces = np.concatenate([hf, qf + np.where(qf == NO_VERTEX, 0, hv.shape[0]).astype(np.uint32)])
+    return verts, faces.astype(np.uint32)
 
 
 def unit_vectors(rng: np.random.Generator, n: int) -> np.ndarray:

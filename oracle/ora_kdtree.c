@@ -40,6 +40,7 @@ struct ora_kdtree
 {
 	const float *verts;
 	const uint32_t *idx;
+	uint32_t stride;  /* 3: triangles; 4: faces with idx[3] == ORA_NO_VERTEX for triangles */
 	const uint8_t *vis;
 	uint32_t n_tris;
 	kbound tree_bound;
@@ -379,11 +380,12 @@ static void build_tree(ora_kdtree *kd, uint32_t n_prims, const kbound *node_boun
 	free(right);
 }
 
-ora_kdtree *ora_kd_build(const float *verts, const uint32_t *tri_idx, const uint8_t *vis, uint32_t n_tris)
+ora_kdtree *ora_kd_build_faces(const float *verts, const uint32_t *tri_idx, uint32_t stride, const uint8_t *vis,
+                               uint32_t n_tris)
 {
 	ora_kdtree *kd = (ora_kdtree *)calloc(1, sizeof(ora_kdtree));
 	if(!kd) return NULL;
-	kd->verts = verts; kd->idx = tri_idx; kd->vis = vis; kd->n_tris = n_tris;
+	kd->verts = verts; kd->idx = tri_idx; kd->stride = stride; kd->vis = vis; kd->n_tris = n_tris;
 	/* accelerator_kdtree_original.cc:75-86 */
 	kd->cost_ratio = 0.8f;
 	kd->e_bonus = 0.33f;
@@ -406,9 +408,10 @@ ora_kdtree *ora_kd_build(const float *verts, const uint32_t *tri_idx, const uint
 	for(uint32_t i = 0; i < n_tris; ++i)
 	{
 		kbound *b = &kd->prim_bounds[i];
-		for(int k = 0; k < 3; ++k)
+		const int nv = ora_face_nverts(tri_idx, stride, i);
+		for(int k = 0; k < nv; ++k)
 		{
-			const float *p = verts + 3 * (size_t)tri_idx[3 * (size_t)i + k];
+			const float *p = verts + 3 * (size_t)tri_idx[(size_t)stride * i + k];
 			for(int ax = 0; ax < 3; ++ax)
 			{
 				if(k == 0) { b->a[ax] = p[ax]; b->g[ax] = p[ax]; }
@@ -421,12 +424,17 @@ ora_kdtree *ora_kd_build(const float *verts, const uint32_t *tri_idx, const uint
 		}
 		prims[i] = i;
 	}
-	ora_tree_bound(verts, tri_idx, n_tris, kd->tree_bound.a, kd->tree_bound.g);
+	ora_tree_bound_faces(verts, tri_idx, stride, n_tris, kd->tree_bound.a, kd->tree_bound.g);
 	build_tree(kd, n_tris, &kd->tree_bound, prims, 0, 0, edges);
 	free(prims);
 	for(int i = 0; i < 3; ++i) free(edges[i]);
 	if(kd->failed) { ora_kd_free(kd); return NULL; }
 	return kd;
+}
+
+ora_kdtree *ora_kd_build(const float *verts, const uint32_t *tri_idx, const uint8_t *vis, uint32_t n_tris)
+{
+	return ora_kd_build_faces(verts, tri_idx, 3, vis, n_tris);
 }
 
 void ora_kd_free(ora_kdtree *kd)
@@ -462,9 +470,7 @@ typedef struct { const uint8_t *transparent; int max_depth; uint32_t *prims; int
 static inline int prim_test(const ora_kdtree *kd, uint32_t p, const float from[3], const float dir[3],
                             float *t, float *u, float *v)
 {
-	const uint32_t *ix = kd->idx + 3 * (size_t)p;
-	return ora_triangle_intersect(kd->verts + 3 * (size_t)ix[0], kd->verts + 3 * (size_t)ix[1],
-	                              kd->verts + 3 * (size_t)ix[2], from, dir, t, u, v);
+	return ora_face_intersect(kd->verts, kd->idx, kd->stride, p, from, dir, t, u, v);
 }
 
 /* accelerator_kdtree_common.h:107-255.  Returns 1 for a hit (closest: *h set;

@@ -73,6 +73,78 @@ int ora_triangle_intersect(const float v0[3], const float v1[3], const float v2[
 	return 0;
 }
 
+/* include/geometry/shape/shape_polygon.h:126-176, N == 4.  The first
+ * triangle (v0, v1, v2) decides alone unless its u leaves [0, 1]: det == 0, or
+ * u inside [0, 1] with v or t failing, is a miss.  Only for u outside [0, 1]
+ * (or NaN) is the second triangle (v0, v2, v3) tried.  The uv returned is
+ * {u + v, v} from the first triangle and {u, u + v} from the second. */
+int ora_quad_intersect(const float v0[3], const float v1[3], const float v2[3], const float v3[3],
+                       const float from[3], const float dir[3], float *t_out, float *u_out, float *v_out)
+{
+	float edge_1[3], edge_2[3], pvec_2[3], tvec[3], qvec[3];
+	sub3(v1, v0, edge_1);
+	sub3(v2, v0, edge_2);
+	cross3(dir, edge_2, pvec_2);
+	const float det_1_2 = dot3(edge_1, pvec_2);
+	*t_out = 0.f;
+	*u_out = 0.f;
+	*v_out = 0.f;
+	if(det_1_2 == 0.f) return 0;
+	const float inv_det_1_2 = 1.f / det_1_2;
+	sub3(from, v0, tvec);
+	float u = dot3(tvec, pvec_2) * inv_det_1_2;
+	if(u >= 0.f && u <= 1.f)
+	{
+		cross3(tvec, edge_1, qvec);
+		const float v = dot3(dir, qvec) * inv_det_1_2;
+		if(v >= 0.f && (u + v) <= 1.f)
+		{
+			const float t = dot3(edge_2, qvec) * inv_det_1_2;
+			if(t > 0.f)
+			{
+				*t_out = t;
+				*u_out = u + v;
+				*v_out = v;
+				return 1;
+			}
+		}
+		return 0;
+	}
+	float edge_3[3], pvec_3[3];
+	sub3(v3, v0, edge_3);
+	cross3(dir, edge_3, pvec_3);
+	const float det_2_3 = dot3(edge_2, pvec_3);
+	if(det_2_3 == 0.f) return 0;
+	const float inv_det_2_3 = 1.f / det_2_3;
+	u = dot3(tvec, pvec_3) * inv_det_2_3;
+	if(!(u >= 0.f && u <= 1.f)) return 0;
+	cross3(tvec, edge_2, qvec);
+	const float v = dot3(dir, qvec) * inv_det_2_3;
+	if(!(v >= 0.f && (u + v) <= 1.f)) return 0;
+	const float t = dot3(edge_3, qvec) * inv_det_2_3;
+	if(!(t > 0.f)) return 0;
+	*t_out = t;
+	*u_out = u;
+	*v_out = u + v;
+	return 1;
+}
+
+int ora_face_nverts(const uint32_t *idx, uint32_t stride, uint32_t p)
+{
+	return (stride == 4 && idx[4 * (size_t)p + 3] != ORA_NO_VERTEX) ? 4 : 3;
+}
+
+int ora_face_intersect(const float *verts, const uint32_t *idx, uint32_t stride, uint32_t p, const float from[3],
+                       const float dir[3], float *t, float *u, float *v)
+{
+	const uint32_t *ix = idx + (size_t)stride * p;
+	if(ora_face_nverts(idx, stride, p) == 4)
+		return ora_quad_intersect(verts + 3 * (size_t)ix[0], verts + 3 * (size_t)ix[1], verts + 3 * (size_t)ix[2],
+		                          verts + 3 * (size_t)ix[3], from, dir, t, u, v);
+	return ora_triangle_intersect(verts + 3 * (size_t)ix[0], verts + 3 * (size_t)ix[1], verts + 3 * (size_t)ix[2],
+	                              from, dir, t, u, v);
+}
+
 /* include/geometry/bound.h:157-198 (Smits' method).  Axes whose direction
  * component is exactly zero are skipped, so no rejection happens for them even
  * when the origin lies outside that slab. */
@@ -129,15 +201,16 @@ int ora_bound_cross(const float a[3], const float g[3], const float from[3], con
  * bound of include/geometry/primitive/primitive_face.h:139-155: union of the
  * triangle vertex bounds, then each axis grown by 0.001 * extent, where the
  * extent is a float difference widened to double. */
-void ora_tree_bound(const float *verts, const uint32_t *tri_idx, uint32_t n_tris,
-                    float a_out[3], float g_out[3])
+void ora_tree_bound_faces(const float *verts, const uint32_t *idx, uint32_t stride, uint32_t n_tris,
+                          float a_out[3], float g_out[3])
 {
 	float a[3] = {0.f, 0.f, 0.f}, g[3] = {0.f, 0.f, 0.f};
 	for(uint32_t i = 0; i < n_tris; ++i)
 	{
-		for(int k = 0; k < 3; ++k)
+		const int nv = ora_face_nverts(idx, stride, i);
+		for(int k = 0; k < nv; ++k)
 		{
-			const float *p = verts + 3 * (size_t)tri_idx[3 * (size_t)i + k];
+			const float *p = verts + 3 * (size_t)idx[(size_t)stride * i + k];
 			for(int ax = 0; ax < 3; ++ax)
 			{
 				if(i == 0 && k == 0) { a[ax] = p[ax]; g[ax] = p[ax]; }
@@ -159,6 +232,11 @@ void ora_tree_bound(const float *verts, const uint32_t *tri_idx, uint32_t n_tris
 	memcpy(g_out, g, sizeof g);
 }
 
+void ora_tree_bound(const float *verts, const uint32_t *tri_idx, uint32_t n_tris, float a_out[3], float g_out[3])
+{
+	ora_tree_bound_faces(verts, tri_idx, 3, n_tris, a_out, g_out);
+}
+
 /* include/accelerator/accelerator.h:64, with min_raydist_ = 5e-5f (:85). */
 float ora_dynamic_bias(float enter, float leave)
 {
@@ -173,6 +251,7 @@ typedef struct
 {
 	const float *verts;
 	const uint32_t *idx;
+	uint32_t stride;
 	const uint8_t *vis;
 	uint32_t n_tris;
 	const ora_ray *rays;
@@ -200,11 +279,8 @@ static void brute_one_closest(const brute_job *j, uint32_t r)
 		for(uint32_t p = 0; p < j->n_tris; ++p)
 		{
 			if(!(j->vis[p] & ORA_VIS_VISIBLE)) continue;
-			const uint32_t *ix = j->idx + 3 * (size_t)p;
 			float t, u, v;
-			if(!ora_triangle_intersect(j->verts + 3 * (size_t)ix[0], j->verts + 3 * (size_t)ix[1],
-			                           j->verts + 3 * (size_t)ix[2], ray->org, ray->dir, &t, &u, &v))
-				continue;
+			if(!ora_face_intersect(j->verts, j->idx, j->stride, p, ray->org, ray->dir, &t, &u, &v)) continue;
 			/* include/accelerator/accelerator.h:125 */
 			if(t <= 0.f || t < t_min || t >= best) continue;
 			best = t;
@@ -226,11 +302,8 @@ static void brute_one_shadow(const brute_job *j, uint32_t r)
 		for(uint32_t p = 0; p < j->n_tris && !occ; ++p)
 		{
 			if(!(j->vis[p] & ORA_VIS_SHADOWS)) continue;
-			const uint32_t *ix = j->idx + 3 * (size_t)p;
 			float t, u, v;
-			if(!ora_triangle_intersect(j->verts + 3 * (size_t)ix[0], j->verts + 3 * (size_t)ix[1],
-			                           j->verts + 3 * (size_t)ix[2], ray->org, ray->dir, &t, &u, &v))
-				continue;
+			if(!ora_face_intersect(j->verts, j->idx, j->stride, p, ray->org, ray->dir, &t, &u, &v)) continue;
 			/* include/accelerator/accelerator.h:137 */
 			if(t <= 0.f || t < t_min || t >= t_max) continue;
 			occ = 1;
@@ -254,7 +327,7 @@ static void brute_run(brute_job *proto, int threads)
 {
 	if(threads < 1) threads = 1;
 	if(threads > 256) threads = 256;
-	ora_tree_bound(proto->verts, proto->idx, proto->n_tris, proto->a, proto->g);
+	ora_tree_bound_faces(proto->verts, proto->idx, proto->stride, proto->n_tris, proto->a, proto->g);
 	brute_job jobs[256];
 	pthread_t th[256];
 	for(int i = 0; i < threads; ++i)
@@ -268,46 +341,59 @@ static void brute_run(brute_job *proto, int threads)
 	for(int i = 0; i < threads; ++i) pthread_join(th[i], NULL);
 }
 
+void ora_closest_brute_faces(const float *verts, const uint32_t *idx, uint32_t stride, const uint8_t *vis,
+                             uint32_t n_tris, const ora_ray *rays, uint32_t n_rays, ora_hit *hits, int threads)
+{
+	brute_job j;
+	memset(&j, 0, sizeof j);
+	j.verts = verts; j.idx = idx; j.stride = stride; j.vis = vis; j.n_tris = n_tris;
+	j.rays = rays; j.n_rays = n_rays; j.hits = hits; j.shadow = 0;
+	brute_run(&j, threads);
+}
+
+void ora_shadow_brute_faces(const float *verts, const uint32_t *idx, uint32_t stride, const uint8_t *vis,
+                            uint32_t n_tris, const ora_ray *rays, uint32_t n_rays, uint8_t *occluded, int threads)
+{
+	brute_job j;
+	memset(&j, 0, sizeof j);
+	j.verts = verts; j.idx = idx; j.stride = stride; j.vis = vis; j.n_tris = n_tris;
+	j.rays = rays; j.n_rays = n_rays; j.occluded = occluded; j.shadow = 1;
+	brute_run(&j, threads);
+}
+
 void ora_closest_brute(const float *verts, const uint32_t *tri_idx, const uint8_t *vis,
                        uint32_t n_tris, const ora_ray *rays, uint32_t n_rays,
                        ora_hit *hits, int threads)
 {
-	brute_job j;
-	memset(&j, 0, sizeof j);
-	j.verts = verts; j.idx = tri_idx; j.vis = vis; j.n_tris = n_tris;
-	j.rays = rays; j.n_rays = n_rays; j.hits = hits; j.shadow = 0;
-	brute_run(&j, threads);
+	ora_closest_brute_faces(verts, tri_idx, 3, vis, n_tris, rays, n_rays, hits, threads);
 }
 
 void ora_shadow_brute(const float *verts, const uint32_t *tri_idx, const uint8_t *vis,
                       uint32_t n_tris, const ora_ray *rays, uint32_t n_rays,
                       uint8_t *occluded, int threads)
 {
-	brute_job j;
-	memset(&j, 0, sizeof j);
-	j.verts = verts; j.idx = tri_idx; j.vis = vis; j.n_tris = n_tris;
-	j.rays = rays; j.n_rays = n_rays; j.occluded = occluded; j.shadow = 1;
-	brute_run(&j, threads);
+	ora_shadow_brute_faces(verts, tri_idx, 3, vis, n_tris, rays, n_rays, occluded, threads);
+}
+
+void ora_retest_faces(const float *verts, const uint32_t *idx, uint32_t stride, const ora_ray *rays,
+                      const uint32_t *prim, uint32_t n_rays, ora_hit *out)
+{
+	for(uint32_t i = 0; i < n_rays; ++i)
+	{
+		ora_hit h = {ORA_NO_HIT, 0.f, 0.f, 0.f};
+		float t, u, v;
+		if(prim[i] != ORA_NO_HIT && ora_face_intersect(verts, idx, stride, prim[i], rays[i].org, rays[i].dir, &t, &u, &v))
+		{
+			h.prim = prim[i]; h.t = t; h.u = u; h.v = v;
+		}
+		out[i] = h;
+	}
 }
 
 void ora_retest(const float *verts, const uint32_t *tri_idx, const ora_ray *rays, const uint32_t *prim,
                 uint32_t n_rays, ora_hit *out)
 {
-	for(uint32_t i = 0; i < n_rays; ++i)
-	{
-		ora_hit h = {ORA_NO_HIT, 0.f, 0.f, 0.f};
-		if(prim[i] != ORA_NO_HIT)
-		{
-			const uint32_t *ix = tri_idx + 3 * (size_t)prim[i];
-			float t, u, v;
-			if(ora_triangle_intersect(verts + 3 * (size_t)ix[0], verts + 3 * (size_t)ix[1], verts + 3 * (size_t)ix[2],
-			                          rays[i].org, rays[i].dir, &t, &u, &v))
-			{
-				h.prim = prim[i]; h.t = t; h.u = u; h.v = v;
-			}
-		}
-		out[i] = h;
-	}
+	ora_retest_faces(verts, tri_idx, 3, rays, prim, n_rays, out);
 }
 
 /* ------------------------------------------------------------------ */

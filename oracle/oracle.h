@@ -14,6 +14,10 @@
  *   ora_triangle_intersect  -- closed-form hits (axis-aligned, exactly
  *                              representable), double-precision re-derivation,
  *                              barycentric reconstruction, mutation cases.
+ *   ora_quad_intersect      -- closed-form hits on both triangles with their
+ *                              uv mappings, the second triangle tried only for
+ *                              u outside [0, 1] (a first-triangle v or t miss
+ *                              is final), det == 0, non-planar quads.
  *   ora_bound_cross         -- closed-form slab intervals, the zero-direction
  *                              quirk, early-out cases.
  *   ora_tree_bound          -- closed form of the 0.1% enlargement.
@@ -38,6 +42,7 @@ typedef struct { float org[3]; float tmin; float dir[3]; float tmax; } ora_ray;
 typedef struct { uint32_t prim; float t; float u; float v; } ora_hit;
 
 #define ORA_NO_HIT 0xFFFFFFFFu
+#define ORA_NO_VERTEX 0xFFFFFFFFu  /* 4th index of a triangle in a face list */
 #define ORA_VIS_VISIBLE 1u   /* include/common/visibility.h:28-33 */
 #define ORA_VIS_SHADOWS 2u
 
@@ -46,6 +51,19 @@ int ora_triangle_intersect(const float v0[3], const float v1[3], const float v2[
                            const float from[3], const float dir[3],
                            float *t, float *u, float *v);
 
+/* include/geometry/shape/shape_polygon.h:126-176 (N == 4): the second
+ * triangle (v0, v2, v3) only when the first triangle's u is outside [0, 1];
+ * uv {u + v, v} or {u, u + v}.  Returns 1 on hit. */
+int ora_quad_intersect(const float v0[3], const float v1[3], const float v2[3], const float v3[3],
+                       const float from[3], const float dir[3], float *t, float *u, float *v);
+
+/* Face lists: `stride` 3 (triangles) or 4 (idx[4 p + 3] == ORA_NO_VERTEX for
+ * a triangle, else a quad v0 v1 v2 v3).  A face is one primitive, as a
+ * PrimitivePolygon<float, 3 or 4> is (src/geometry/object/object_mesh.cc:96-112). */
+int ora_face_nverts(const uint32_t *idx, uint32_t stride, uint32_t p);
+int ora_face_intersect(const float *verts, const uint32_t *idx, uint32_t stride, uint32_t p, const float from[3],
+                       const float dir[3], float *t, float *u, float *v);
+
 /* include/geometry/bound.h:157-198. Returns 1 when crossed. */
 int ora_bound_cross(const float a[3], const float g[3], const float from[3], const float dir[3],
                     float t_max, float *enter, float *leave);
@@ -53,6 +71,9 @@ int ora_bound_cross(const float a[3], const float g[3], const float from[3], con
 /* src/accelerator/accelerator_kdtree_original.cc:89-103 */
 void ora_tree_bound(const float *verts, const uint32_t *tri_idx, uint32_t n_tris,
                     float a_out[3], float g_out[3]);
+
+void ora_tree_bound_faces(const float *verts, const uint32_t *idx, uint32_t stride, uint32_t n_faces,
+                          float a_out[3], float g_out[3]);
 
 /* include/accelerator/accelerator.h:64 */
 float ora_dynamic_bias(float enter, float leave);
@@ -71,6 +92,14 @@ void ora_shadow_brute(const float *verts, const uint32_t *tri_idx, const uint8_t
                       uint32_t n_tris, const ora_ray *rays, uint32_t n_rays,
                       uint8_t *occluded, int threads);
 
+/* The same over a face list (triangles and quads). */
+void ora_closest_brute_faces(const float *verts, const uint32_t *idx, uint32_t stride, const uint8_t *vis,
+                             uint32_t n_faces, const ora_ray *rays, uint32_t n_rays, ora_hit *hits, int threads);
+void ora_shadow_brute_faces(const float *verts, const uint32_t *idx, uint32_t stride, const uint8_t *vis,
+                            uint32_t n_faces, const ora_ray *rays, uint32_t n_rays, uint8_t *occluded, int threads);
+void ora_retest_faces(const float *verts, const uint32_t *idx, uint32_t stride, const ora_ray *rays,
+                      const uint32_t *prim, uint32_t n_rays, ora_hit *out);
+
 /* Re-run the triangle test for given (ray, primitive) pairs: out[i] is the
  * ora_hit of ray i against primitive prim[i] (prim[i] == ORA_NO_HIT: skipped,
  * out[i] = {ORA_NO_HIT, 0, 0, 0}). */
@@ -87,6 +116,8 @@ typedef struct ora_kdtree ora_kdtree;
  * not query results.  Returns NULL on allocation failure. */
 ora_kdtree *ora_kd_build(const float *verts, const uint32_t *tri_idx, const uint8_t *vis,
                          uint32_t n_tris);
+ora_kdtree *ora_kd_build_faces(const float *verts, const uint32_t *idx, uint32_t stride, const uint8_t *vis,
+                               uint32_t n_faces);
 void ora_kd_free(ora_kdtree *kd);
 void ora_kd_bound(const ora_kdtree *kd, float a[3], float g[3]);
 uint32_t ora_kd_num_nodes(const ora_kdtree *kd);

@@ -29,6 +29,14 @@ def lib() -> C.CDLL:
         vp = C.c_void_p
         L.ora_triangle_intersect.argtypes = [fp, fp, fp, fp, fp, fp, fp, fp]
         L.ora_triangle_intersect.restype = C.c_int
+        L.ora_quad_intersect.argtypes = [fp, fp, fp, fp, fp, fp, fp, fp, fp]
+        L.ora_quad_intersect.restype = C.c_int
+        L.ora_tree_bound_faces.argtypes = [vp, vp, C.c_uint32, C.c_uint32, fp, fp]
+        L.ora_closest_brute_faces.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32, vp, C.c_uint32, vp, C.c_int]
+        L.ora_shadow_brute_faces.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32, vp, C.c_uint32, vp, C.c_int]
+        L.ora_retest_faces.argtypes = [vp, vp, C.c_uint32, vp, vp, C.c_uint32, vp]
+        L.ora_kd_build_faces.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32]
+        L.ora_kd_build_faces.restype = vp
         L.ora_bound_cross.argtypes = [fp, fp, fp, fp, C.c_float, fp, fp]
         L.ora_bound_cross.restype = C.c_int
         L.ora_tree_bound.argtypes = [vp, vp, C.c_uint32, fp, fp]
@@ -69,6 +77,13 @@ def triangle_intersect(v0, v1, v2, org, dir):
     return bool(hit), t.value, u.value, v.value
 
 
+def quad_intersect(v0, v1, v2, v3, org, dir):
+    t, u, v = C.c_float(), C.c_float(), C.c_float()
+    hit = lib().ora_quad_intersect(_f3(v0), _f3(v1), _f3(v2), _f3(v3), _f3(org), _f3(dir),
+                                   C.byref(t), C.byref(u), C.byref(v))
+    return bool(hit), t.value, u.value, v.value
+
+
 def bound_cross(a, g, org, dir, t_max):
     e, l = C.c_float(), C.c_float()
     hit = lib().ora_bound_cross(_f3(a), _f3(g), _f3(org), _f3(dir), C.c_float(t_max), C.byref(e), C.byref(l))
@@ -76,10 +91,12 @@ def bound_cross(a, g, org, dir, t_max):
 
 
 def tree_bound(verts, tris):
+    """tris: (n, 3) triangles, or (n, 4) faces whose 4th index is NO_HIT for a
+    triangle (every function here takes either)."""
     verts = np.ascontiguousarray(verts, dtype=np.float32)
     tris = np.ascontiguousarray(tris, dtype=np.uint32)
     a, g = (C.c_float * 3)(), (C.c_float * 3)()
-    lib().ora_tree_bound(verts.ctypes.data, tris.ctypes.data, tris.shape[0], a, g)
+    lib().ora_tree_bound_faces(verts.ctypes.data, tris.ctypes.data, tris.shape[1], tris.shape[0], a, g)
     return np.array(a[:], dtype=np.float32), np.array(g[:], dtype=np.float32)
 
 
@@ -90,6 +107,8 @@ def dynamic_bias(enter, leave) -> float:
 def _prep(verts, tris, vis):
     verts = np.ascontiguousarray(verts, dtype=np.float32)
     tris = np.ascontiguousarray(tris, dtype=np.uint32)
+    if tris.ndim != 2 or tris.shape[1] not in (3, 4):
+        raise ValueError("faces must be (n, 3) or (n, 4) vertex indices")
     if vis is None:
         vis = np.full(tris.shape[0], 3, dtype=np.uint8)
     vis = np.ascontiguousarray(vis, dtype=np.uint8)
@@ -100,8 +119,9 @@ def closest_brute(verts, tris, rays, vis=None, threads=None) -> dict:
     verts, tris, vis = _prep(verts, tris, vis)
     rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
     out = np.zeros((rays.shape[0], 4), dtype=np.uint32)
-    lib().ora_closest_brute(verts.ctypes.data, tris.ctypes.data, vis.ctypes.data, tris.shape[0],
-                            rays.ctypes.data, rays.shape[0], out.ctypes.data, threads or default_threads())
+    lib().ora_closest_brute_faces(verts.ctypes.data, tris.ctypes.data, tris.shape[1], vis.ctypes.data,
+                                  tris.shape[0], rays.ctypes.data, rays.shape[0], out.ctypes.data,
+                                  threads or default_threads())
     return _hits(out)
 
 
@@ -109,8 +129,9 @@ def shadow_brute(verts, tris, rays, vis=None, threads=None) -> np.ndarray:
     verts, tris, vis = _prep(verts, tris, vis)
     rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
     occ = np.zeros(rays.shape[0], dtype=np.uint8)
-    lib().ora_shadow_brute(verts.ctypes.data, tris.ctypes.data, vis.ctypes.data, tris.shape[0],
-                           rays.ctypes.data, rays.shape[0], occ.ctypes.data, threads or default_threads())
+    lib().ora_shadow_brute_faces(verts.ctypes.data, tris.ctypes.data, tris.shape[1], vis.ctypes.data,
+                                 tris.shape[0], rays.ctypes.data, rays.shape[0], occ.ctypes.data,
+                                 threads or default_threads())
     return occ
 
 
@@ -120,8 +141,8 @@ def retest(verts, tris, rays, prim) -> dict:
     rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
     prim = np.ascontiguousarray(prim, dtype=np.uint32)
     out = np.zeros((rays.shape[0], 4), dtype=np.uint32)
-    lib().ora_retest(verts.ctypes.data, tris.ctypes.data, rays.ctypes.data, prim.ctypes.data, rays.shape[0],
-                     out.ctypes.data)
+    lib().ora_retest_faces(verts.ctypes.data, tris.ctypes.data, tris.shape[1], rays.ctypes.data, prim.ctypes.data,
+                           rays.shape[0], out.ctypes.data)
     return _hits(out)
 
 
@@ -135,8 +156,8 @@ class KdTree:
 
     def __init__(self, verts, tris, vis=None):
         self.verts, self.tris, self.vis = _prep(verts, tris, vis)
-        self._h = lib().ora_kd_build(self.verts.ctypes.data, self.tris.ctypes.data, self.vis.ctypes.data,
-                                     self.tris.shape[0])
+        self._h = lib().ora_kd_build_faces(self.verts.ctypes.data, self.tris.ctypes.data, self.tris.shape[1],
+                                           self.vis.ctypes.data, self.tris.shape[0])
         if not self._h:
             raise MemoryError("ora_kd_build failed")
 

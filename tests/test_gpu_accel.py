@@ -447,3 +447,56 @@ def test_leaf_sizes_bit_exact(oracle, builder, max_leaf):
     srays[:, 7] = scenes.shadow_tmax(rng, len(srays))
     occ, _ = acc.intersect_shadow(srays)
     assert np.array_equal(occ, oracle.shadow_brute(verts, tris, srays))
+
+
+@pytest.mark.parametrize("builder", ["lbvh", "ploc"])
+@pytest.mark.parametrize("max_leaf", [1, 4])
+def test_quad_faces_bit_exact(oracle, builder, max_leaf):
+    """Faces of triangles and quads (ymx_accel_create_faces): the heightfield
+    as bent quads, planar and bent random quads, and triangles.  A quad is one
+    primitive, tested as include/geometry/shape/shape_polygon.h:126-176 does;
+    ids, t and both uv mappings bit-exact against the oracle."""
+    verts, faces = scenes.quad_face_scene(res=64, n_random=3000, seed=12)
+    rng = np.random.default_rng(12)
+    lo, hi = verts.min(0), verts.max(0)
+    g = np.linspace(0.0, 1.0, 127, dtype=np.float32)  # cell corners, edges and centres
+    xy = np.stack(np.meshgrid(g, g, indexing="xy"), -1).reshape(-1, 2)
+    vertical = np.zeros((len(xy), 8), np.float32)
+    vertical[:, 0:2] = xy
+    vertical[:, 2] = 1.0
+    vertical[:, 6] = -1.0
+    vertical[:, 7] = -1.0
+    rays = np.concatenate([scenes.incoherent_rays(rng, 20000, lo, hi),
+                           scenes.camera_rays(160, 80, **scenes.RAYCAST_CAMERA), vertical])
+    say(f"quad faces builder={builder} max_leaf={max_leaf} faces={len(faces)} rays={len(rays)}")
+    acc = GpuAccelerator(verts, faces, builder=builder, max_leaf_size=max_leaf)
+    assert acc.info()["n_quads"] == int((faces[:, 3] != 0xFFFFFFFF).sum())
+    a, gb = acc.bound()
+    ea, eg = oracle.tree_bound(verts, faces)
+    assert np.array_equal(a, ea) and np.array_equal(gb, eg)
+    exp = oracle.closest_brute(verts, faces, rays)
+    hit_quads = exp["prim"][exp["prim"] != NO_HIT]
+    assert (faces[hit_quads, 3] != 0xFFFFFFFF).mean() > 0.5
+    # both of the reference's uv mappings occur: second-triangle hits have v >= u
+    assert ((exp["v"] > exp["u"]) & (exp["prim"] != NO_HIT)).any()
+    assert_hits_equal(acc.intersect(rays), exp, "quad closest")
+    srays = rays.copy()
+    srays[:, 7] = scenes.shadow_tmax(rng, len(rays))
+    occ, prim = acc.intersect_shadow(srays)
+    assert np.array_equal(occ, oracle.shadow_brute(verts, faces, srays))
+    r = oracle.retest(verts, faces, srays, np.where(occ == 1, prim, NO_HIT).astype(np.uint32))
+    assert np.all(r["prim"][occ == 1] == prim[occ == 1])
+    acc.close()
+
+
+def test_single_quad_and_mixed_tiny(oracle):
+    verts = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0.2], [0, 1, 0], [0.5, 0.5, 0.5], [2, 0, 0.5], [2, 1, 0.5]],
+                     np.float32)
+    rng = np.random.default_rng(3)
+    rays = scenes.incoherent_rays(rng, 4000, (-0.2, -0.2, -0.5), (1.2, 1.2, 1.0))
+    for faces in (np.array([[0, 1, 2, 3]], np.uint32),
+                  np.array([[0, 1, 2, 3], [4, 5, 6, 0xFFFFFFFF]], np.uint32)):
+        say(f"tiny faces n={len(faces)}")
+        acc = GpuAccelerator(verts, faces)
+        assert_hits_equal(acc.intersect(rays), oracle.closest_brute(verts, faces, rays), f"{len(faces)} faces")
+        acc.close()

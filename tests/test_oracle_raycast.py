@@ -331,3 +331,93 @@ def test_fast_math_error_bounds(oracle):
     xs = np.linspace(-20, 20, 4001).astype(F32)
     assert max(abs(L.ora_fast_sin(float(x)) - np.sin(float(x))) for x in xs) < 1.2e-3
     assert max(abs(L.ora_fast_cos(float(x)) - np.cos(float(x))) for x in xs) < 1.2e-3
+
+
+# ---------------------------------------------------------------- quads
+# include/geometry/shape/shape_polygon.h:126-176 with N == 4.  On the unit
+# square (v0, v1, v2, v3) = (0,0), (1,0), (1,1), (0,1) the first triangle has
+# u = x - y, v = y and returns {u + v, v}; the second (v0, v2, v3) has u = x,
+# v = y - x and returns {u, u + v}: both give uv == (x, y).
+SQUARE = ((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0))
+
+
+@pytest.mark.parametrize("xy", [(0.75, 0.25), (0.25, 0.75), (0.5, 0.5), (1.0, 1.0), (0.0, 1.0)])
+def test_quad_closed_form_uv(oracle, xy):
+    hit, t, u, v = oracle.quad_intersect(*SQUARE, (xy[0], xy[1], 1.0), (0, 0, -1))
+    assert hit and (t, u, v) == (1.0, xy[0], xy[1])
+
+
+def test_quad_second_triangle_only_when_u_leaves_unit_interval(oracle):
+    # v3 folded out: the point (1.2, 0.5) lies inside the second triangle
+    # (u 0.36, v 0.56), but the first triangle's u = x - y = 0.7 is inside
+    # [0, 1] and its v test fails (u + v = 1.2): the reference returns a miss
+    quad = ((0, 0, 0), (1, 0, 0), (1, 1, 0), (1.5, 0.25, 0))
+    assert oracle.triangle_intersect(quad[0], quad[2], quad[3], (1.2, 0.5, 1.0), (0, 0, -1))[0]
+    assert not oracle.quad_intersect(*quad, (1.2, 0.5, 1.0), (0, 0, -1))[0]
+    # below the diagonal the first triangle's u = x - y = -0.25 < 0 hands over
+    hit, t, u, v = oracle.quad_intersect(*quad, (0.5, 0.75, 1.0), (0, 0, -1))
+    assert not hit  # outside the second triangle too
+    # beyond the first triangle's u = 1 (x - y = 1.1) the second triangle
+    # hits, at its (u, v) = (0.08, 0.88), returned as {u, u + v}
+    hit, t, u, v = oracle.quad_intersect(*quad, (1.4, 0.3, 1.0), (0, 0, -1))
+    assert hit and abs(t - 1.0) < 1e-6 and abs(u - 0.08) < 1e-6 and abs(v - 0.96) < 1e-6
+
+
+def test_quad_first_triangle_parallel_is_a_miss(oracle):
+    # non-planar quad: the ray runs parallel to the first triangle (det 0) but
+    # crosses the second one at (0.2, 0.3, 0.1); the reference returns a miss
+    quad = ((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 1))
+    assert oracle.triangle_intersect(quad[0], quad[2], quad[3], (0.2, -1.0, 0.1), (0, 1, 0))[0]
+    assert not oracle.quad_intersect(*quad, (0.2, -1.0, 0.1), (0, 1, 0))[0]
+
+
+def test_quad_double_sided_and_behind(oracle):
+    hit, t, u, v = oracle.quad_intersect(*SQUARE, (0.25, 0.75, -2.0), (0, 0, 1))
+    assert hit and (t, u, v) == (2.0, 0.25, 0.75)
+    assert not oracle.quad_intersect(*SQUARE, (0.25, 0.75, 1.0), (0, 0, 1))[0]
+    assert not oracle.quad_intersect(*SQUARE, (1.25, 0.5, 1.0), (0, 0, -1))[0]
+
+
+def test_quad_faces_in_brute_force_and_kdtree(oracle):
+    """A face list mixing quads and triangles: every hit re-tests, the tree
+    bound covers the fourth vertices, and the kd-tree port agrees."""
+    rng = np.random.default_rng(11)
+    n = 400
+    c = rng.uniform(0.1, 0.9, size=(n, 3)).astype(F32)
+    a = rng.normal(size=(n, 3))
+    a /= np.linalg.norm(a, axis=1, keepdims=True)
+    b = np.cross(a, rng.normal(size=(n, 3)))
+    b /= np.linalg.norm(b, axis=1, keepdims=True)
+    s = rng.uniform(0.02, 0.06, size=(n, 1))
+    quad = np.stack([c - s * a - s * b, c + s * a - s * b, c + s * a + s * b, c - s * a + s * b], 1)
+    quad[:, 3] += rng.normal(scale=0.004, size=(n, 3))  # slightly non-planar
+    verts = quad.reshape(-1, 3).astype(F32)
+    faces = np.arange(4 * n, dtype=np.uint32).reshape(n, 4)
+    faces[::3, 3] = 0xFFFFFFFF  # every third face is a triangle
+    lo, hi = verts.min(0), verts.max(0)
+    rays = scenes.incoherent_rays(rng, 20000, lo, hi)
+    exp = oracle.closest_brute(verts, faces, rays)
+    assert (exp["prim"] != oracle.NO_HIT).mean() > 0.2
+    ta, tg = oracle.tree_bound(verts, faces)
+    used = verts[faces[faces != 0xFFFFFFFF]]
+    assert np.all(ta < used.min(0)) and np.all(tg > used.max(0))
+    quad_v3 = verts[faces[faces[:, 3] != 0xFFFFFFFF, 3]]
+    assert np.all(ta < quad_v3.min(0)) and np.all(tg > quad_v3.max(0))
+    hit = exp["prim"] != oracle.NO_HIT
+    r = oracle.retest(verts, faces, rays, exp["prim"])
+    assert np.array_equal(r["t"][hit].view(np.uint32), exp["t"][hit].view(np.uint32))
+    kd = oracle.KdTree(verts, faces).closest(rays)
+    same = kd["prim"] == exp["prim"]
+    assert same.mean() > 0.999
+    assert np.array_equal(kd["t"][~same].view(np.uint32), exp["t"][~same].view(np.uint32))  # exact ties only
+    srays = rays.copy()
+    srays[:, 7] = scenes.shadow_tmax(rng, len(rays))
+    assert np.array_equal(oracle.shadow_brute(verts, faces, srays), oracle.KdTree(verts, faces).shadow(srays)[0])
+
+
+def test_quad_tree_bound_includes_fourth_vertex(oracle):
+    verts = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [-1, 2, 0]], dtype=F32)
+    a, g = oracle.tree_bound(verts, np.array([[0, 1, 2, 3]], np.uint32))
+    assert a[0] == -F32(1) - F32(0.002) and g[1] == F32(2) + F32(0.002)
+    a3, g3 = oracle.tree_bound(verts, np.array([[0, 1, 2, 0xFFFFFFFF]], np.uint32))
+    assert a3[0] == -F32(0.001) and g3[1] == F32(1) + F32(0.001)
