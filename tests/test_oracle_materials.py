@@ -373,3 +373,60 @@ def test_glossy_branches_average_their_samples(P):
         acc += e_col * (s["col"] * s["w"])
     assert np.allclose(col[:3], acc / 8, rtol=1e-4)
     assert col[0] > 0.5
+
+
+# ---------------------------------------------------------------- shinydiffuse
+def test_shinydiffuse_eval_pdf_closed_form(P):
+    """Lambert without 1/pi: eval = diffuse_reflect * color on the side of wo,
+    pdf = |cos wi| (material_shiny_diffuse.cc:297-347, :441-485)."""
+    m = P.Material(type="shinydiffuse", color=(0.9, 0.5, 0.25), diffuse_reflect=0.8)
+    assert m.flags & P.F_DIFFUSE
+    rng = np.random.default_rng(8)
+    for wo, wi in zip(rand_dirs(rng, 100), rand_dirs(rng, 100)):
+        assert np.allclose(m.eval(Z, wo, wi), 0.8 * np.float32([0.9, 0.5, 0.25]), rtol=1e-6)
+        assert m.pdf(Z, wo, wi) == pytest.approx(abs(float(wi[2])), rel=1e-6)
+        # the transmit side gets nothing: no translucency built
+        assert np.all(m.eval(Z, wo, wi * np.float32([1, 1, -1])) == 0)
+
+
+def test_shinydiffuse_sampler_matches_cosine_density(P):
+    """sample() draws wi from cosHemisphere (include/sampler/sample.h:43-52):
+    cos(theta) = sqrt(s1) exactly, azimuth 2 pi s2 through the fast sine and
+    cosine.  A histogram of 60k samples against the cosine density cos/pi
+    over (theta, phi) bins; the returned pdf is |cos wi| and the weight
+    |cos| / (0.99 pdf + 0.01) (:432); the colour is eval's."""
+    m = P.Material(type="shinydiffuse", color=(0.7, 0.7, 0.7))
+    wo = unit([0.5, -0.2, 0.8]).astype(np.float32)
+    rng = np.random.default_rng(9)
+    n = 60000
+    n_t, n_p = 10, 16
+    hist = np.zeros((n_t, n_p))
+    for s1, s2 in rng.random((n, 2)):
+        s = m.sample(Z, wo, float(s1), float(s2))
+        wi = s["wi"]
+        assert wi[2] > 0 and s["sampled"] & P.F_DIFFUSE
+        assert float(wi[2]) == pytest.approx(math.sqrt(s1), abs=2e-7)
+        assert s["pdf"] == pytest.approx(float(wi[2]), rel=1e-6)
+        assert s["w"] == pytest.approx(float(wi[2]) / (s["pdf"] * 0.99 + 0.01), rel=1e-6)
+        assert np.allclose(s["col"], 0.7, rtol=1e-6)
+        # the fast sine/cosine leave the direction unit only to about 2e-3
+        assert abs(float(np.linalg.norm(wi)) - 1.0) < 3e-3
+        th = math.acos(min(1.0, float(wi[2])))
+        ph = math.atan2(float(wi[1]), float(wi[0])) % (2 * math.pi)
+        hist[min(n_t - 1, int(th / (math.pi / 2) * n_t)), min(n_p - 1, int(ph / (2 * math.pi) * n_p))] += 1
+    # cos/pi integrated over a bin: (sin^2 t1 - sin^2 t0) / 2 * dphi / pi
+    t = np.linspace(0, math.pi / 2, n_t + 1)
+    ring = (np.sin(t[1:]) ** 2 - np.sin(t[:-1]) ** 2) / 2 * (2 * math.pi / n_p) / math.pi
+    exp = np.repeat(ring[:, None], n_p, axis=1) * n
+    assert np.all(np.abs(hist - exp) < 5 * np.sqrt(exp) + 0.02 * exp)
+    assert abs(exp.sum() - n) < 1e-6 * n
+
+
+def test_shinydiffuse_sampler_below_the_surface(P):
+    """From below (wo on the back), sample() flips to the side of wo:
+    the reference's face-forwarded normal."""
+    m = P.Material(type="shinydiffuse", color=(0.7, 0.7, 0.7))
+    wo = unit([0.3, 0.1, -0.9]).astype(np.float32)
+    for s1, s2 in ((0.1, 0.2), (0.5, 0.7), (0.9, 0.05)):
+        s = m.sample(Z, wo, s1, s2)
+        assert s["wi"][2] < 0 and float(-s["wi"][2]) == pytest.approx(math.sqrt(s1), abs=2e-7)

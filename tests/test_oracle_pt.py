@@ -195,3 +195,126 @@ def test_power_scales_linearly_and_shadows_block(oracle):
     sc["camera"]["up"] = (0.0, 1.0, 0.4)
     rgba, w = oracle.PathTracer(sc).render()
     assert np.all(rgba[..., :3] == 0.0)
+
+
+# ---------------------------------------------------------------- multi-bounce closed form
+C_WEIGHT = 2 * (1 / (2 * 0.99) - 0.01 / 0.99 ** 2 + 0.01 ** 2 / 0.99 ** 3 * math.log(100.0))
+
+
+def test_sample_weight_constant():
+    # E over cosine-distributed directions of the reference's diffuse sample
+    # weight |cos| / (0.99 pdf + 0.01), pdf = |cos| (material_shiny_diffuse.cc:432):
+    # 2 * integral_0^1 x^2 / (0.99 x + 0.01) dx
+    from scipy.integrate import quad
+    assert abs(C_WEIGHT - 2 * quad(lambda x: x * x / (0.99 * x + 0.01), 0, 1)[0]) < 1e-12
+    assert abs(C_WEIGHT - 0.99064416) < 1e-7
+
+
+def two_plates(albedo=0.8, bounces=5, rr=10, n_lights=1, spp=1024, res=8, seed=0, half=1000.0, h=0.5):
+    """Two facing Lambertian plates, z = 0 and z = 1, 2000 x 2000 (infinite
+    for the rays that matter), and a downward area light plane at z = h
+    covering them (one quad, or two halves).  The light has no geometry, so
+    paths pass it; the ceiling sees only its back (one-sided), so direct
+    light reaches the floor only, the same at every floor point."""
+    S = half
+    verts = np.array([[-S, -S, 0], [S, -S, 0], [S, S, 0], [-S, S, 0],
+                      [-S, -S, 1], [S, -S, 1], [S, S, 1], [-S, S, 1]], np.float32)
+    tris = np.array([[0, 1, 2], [0, 2, 3], [4, 6, 5], [4, 7, 6]], np.uint32)
+    if n_lights == 1:
+        lights = [dict(type="area", corner=(-S, -S, h), point1=(-S, S, h), point2=(S, -S, h), power=1.0)]
+    else:
+        lights = [dict(type="area", corner=(-S, -S, h), point1=(-S, S, h), point2=(0.0, -S, h), power=1.0),
+                  dict(type="area", corner=(0.0, -S, h), point1=(0.0, S, h), point2=(S, -S, h), power=1.0)]
+    return dict(verts=verts, tris=tris, mat_id=np.zeros(4, np.uint32), materials=[dict(color=(albedo,) * 3)],
+                lights=lights,
+                camera=dict(from_=(0.0, 0.0, 0.9), to=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.9), focal=2000.0,
+                            resx=res, resy=res),
+                settings=dict(path_samples=1, bounces=bounces, rr_min_bounces=rr, aa_samples=spp,
+                              aa_pixelwidth=1.5, background=None, seed=seed))
+
+
+def direct_light_constant(area, h=0.5):
+    """Expected direct light on the floor per unit albedo and power, from the
+    reference's estimator: light sampling (pdf d^2 pi / (A cos'),
+    light_area.cc:99-124) and BSDF sampling (weight W = x / (0.99 x + 0.01))
+    combined by the power heuristic (integrator_montecarlo.cc:254-264,
+    :354-368).  With x = cos(theta) and pdf_l = pi h^2 / (A x^3) for the
+    plane: pi * 2 * integral x (w_l + (1 - w_l) W) dx; w_l vanishes except
+    at grazing angles, so this is pi * C_WEIGHT to 2e-4."""
+    from scipy.integrate import quad
+
+    def f(x):
+        pl = math.pi * h * h / (area * x ** 3)
+        wl = pl * pl / (pl * pl + x * x)
+        return x * (wl + (1 - wl) * x / (0.99 * x + 0.01))
+    return math.pi * 2 * quad(f, 1e-9, 1, limit=400, points=[0.001, 0.01, 0.03, 0.1])[0]
+
+
+@pytest.mark.parametrize("n_lights,rr,bounces", [(1, 10, 1), (1, 10, 5), (1, 1, 5), (2, 10, 5), (2, 1, 5),
+                                                 (1, 10, 4), (2, 0, 3)])
+def test_two_plate_furnace_closed_form(oracle, n_lights, rr, bounces):
+    """The whole path loop against a closed form (integrator_path_tracer.cc:
+    138-314).  Direct light D reaches floor vertices only and is the same at
+    every one; each bounce multiplies the throughput by albedo * W, whose
+    mean is a * C_WEIGHT; ceiling vertices add nothing; diffuse paths add no
+    emission past the first vertex.  So the floor pixel reads
+    D * (1 + sum over even k in 2..bounces of (a c)^k), with D from the camera
+    vertex's estimateAllDirectLight and the later floor vertices' one-light
+    pick times the number of lights; Russian roulette (from depth rr + 1)
+    must leave the expectation unchanged.  Dropping the 0.01 sample-weight
+    term moves the value by 1.3 %, the x L of the light pick by 15-30 %, the
+    roulette division by more."""
+    a = 0.8
+    sc = two_plates(albedo=a, bounces=bounces, rr=rr, n_lights=n_lights)
+    rgba, w = oracle.PathTracer(sc).render()
+    img = rgba[..., :3] / w[..., None]
+    assert np.allclose(img[..., 0], img[..., 1]) and np.allclose(img[..., 0], img[..., 2])
+    ac = a * C_WEIGHT
+    series = 1 + sum(ac ** k for k in range(2, bounces + 1, 2))
+    expected = a * direct_light_constant(4e6 / n_lights) * series
+    got = float(img[..., 0].mean())
+    assert abs(got / expected - 1) < 0.005, (got, expected)
+    wrong_c = a * math.pi * (1 + sum(a ** k for k in range(2, bounces + 1, 2)))
+    assert abs(got / wrong_c - 1) > 0.008
+
+
+def test_film_splat_footprint_against_gaussian(oracle):
+    """ImageFilm::addSample (imagefilm.cc:771-822): a sample at pixel p with
+    sub-pixel offset (dx, dy) adds filter::gauss of the distance from its
+    position p + (dx, dy) to each film pixel centre, over the pixel width 1.5
+    (Gauss: 2 x AA_pixelwidth / 2), through a 16 x 16 table
+    (include/math/filter.h:32-37).  The weight film of a 1-spp render must
+    equal, pixel by pixel, the sum of the continuous Gaussian over the samples
+    near it, to the table's quantisation (slope 2.1 x half a cell diagonal);
+    a footprint shifted by half a pixel misses by several times that."""
+    sc = scenes.cornell_path_scene(n_random=50, res=12, spp=1)
+    sc["camera"]["resy"] = 9
+    pt = oracle.PathTracer(sc)
+    _, w = pt.render()
+    _, _, width = oracle.filter_table(1.5)
+    H, Wd = w.shape
+    exp = np.zeros((H, Wd))
+    shifted = np.zeros((H, Wd))
+    count = np.zeros((H, Wd))
+
+    def gauss(rx, ry):
+        return np.maximum(0.0, np.exp(-6.0 * (rx * rx + ry * ry)) - 0.00247875)
+
+    for y in range(H):
+        for x in range(Wd):
+            _, (dx, dy) = pt.pixel_sample(x, y, 0)
+            assert 0.0 <= dx < 1.0 and 0.0 <= dy < 1.0
+            for j in range(max(0, y - 1), min(H, y + 2)):
+                for i in range(max(0, x - 1), min(Wd, x + 2)):
+                    rx = abs(i + 0.5 - (x + dx)) / width
+                    ry = abs(j + 0.5 - (y + dy)) / width
+                    if rx < 1 and ry < 1:
+                        exp[j, i] += gauss(rx, ry)
+                        count[j, i] += 1
+                    shifted[j, i] += gauss(min(1.0, abs(i - (x + dx)) / width), min(1.0, abs(j - (y + dy)) / width))
+    # |grad exp(-6 r^2)| <= 12 r exp(-6 r^2) <= 2.1; the table holds G at the
+    # centre of the 1/16-wide cell the distance falls in (0.9999 scale aside)
+    per_sample = 2.1 * (math.sqrt(2) / 32 + 1e-4)
+    err = np.abs(w - exp)
+    assert np.all(err <= count * per_sample), (err / np.maximum(count, 1)).max()
+    assert np.abs(w - shifted).max() > 4 * err.max()
