@@ -128,8 +128,29 @@ def build_oracle(verbose: bool = False) -> Path:
     return ROOT / "oracle" / "liboracle.so"
 
 
+REF_ROOT = Path("/root/reference")
+
+
+def build_ref(verbose: bool = False) -> Path | None:
+    """oracle/_ref/libref.so: the reference's own polygon test, samplers, fast
+    math and filter, compiled from its sources (test infrastructure).  Only
+    where the reference is mounted; elsewhere the tests that use it skip."""
+    if not (REF_ROOT / "include" / "geometry" / "shape" / "shape_polygon.h").exists():
+        if verbose:
+            print("reference sources not mounted: oracle/_ref not built")
+        return None
+    res = subprocess.run(["make", "-C", str(ROOT / "oracle"), "-f", "ref.mk", "-s", f"REF={REF_ROOT}"],
+                         capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"oracle/_ref build failed:\n{res.stdout}\n{res.stderr}")
+    if verbose:
+        print("built oracle/_ref/libref.so")
+    return ROOT / "oracle" / "_ref" / "libref.so"
+
+
 if __name__ == "__main__":
     build_library(verbose=True)
     build_host(verbose=True)
     build_oracle(verbose=True)
+    build_ref(verbose=True)
     build_cpp_tests(verbose=True)

@@ -221,3 +221,20 @@ def vmf_sample(mu, kappa, u1, u2):
     t = np.stack([1 + sign * mu[:, 0] ** 2 * a, sign * b, -sign * mu[:, 0]], -1)
     bb = np.stack([b, sign + mu[:, 1] ** 2 * a, -mu[:, 1]], -1)
     return (wc[:, None] * mu + s[:, None] * (np.cos(phi)[:, None] * t + np.sin(phi)[:, None] * bb))
+
+
+def adam_step(p, m, v, g, t, lr=1e-2, beta1=0.9, beta2=0.99, eps=1e-8):
+    """One Adam step (Kingma & Ba, "Adam: A Method for Stochastic
+    Optimization", 2015, Algorithm 1) in float32 in the kernels' order:
+    m = b1 m + (1 - b1) g; v = b2 v + (1 - b2) g^2;
+    p -= lr (m / (1 - b1^t)) / (sqrt(v / (1 - b2^t)) + eps).
+    Returns the new (p, m, v)."""
+    f = np.float32
+    p, m, v, g = (np.asarray(a, np.float32) for a in (p, m, v, g))
+    b1, b2 = f(beta1), f(beta2)
+    c1 = f(1) - f(np.power(np.float64(b1), t))
+    c2 = f(1) - f(np.power(np.float64(b2), t))
+    m = b1 * m + (f(1) - b1) * g
+    v = b2 * v + (f(1) - b2) * g * g
+    p = p - f(lr) * (m / c1) / (np.sqrt(v / c2) + f(eps))
+    return p.astype(np.float32), m.astype(np.float32), v.astype(np.float32)

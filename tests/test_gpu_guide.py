@@ -104,6 +104,33 @@ def test_gradients_match_oracle():
     assert np.count_nonzero(ggrid[~touched]) == 0
 
 
+def test_adam_steps_match_oracle():
+    """Two Adam steps (k_adam on the MLP, k_adam_grid on the lattice entries in
+    use) on the gradients the GPU formed: the FP32 master weights equal the
+    oracle's Adam (Kingma & Ba 2015) applied to those gradients, to float32
+    rounding of the bias corrections (the host forms 1 - b^t with powf)."""
+    from libyafaray_amd.guide import flat_to_mlp, mlp_to_flat, pack_samples
+    p, pos, nrm, wo, w, c = case(2, 3000)
+    say("guide two Adam steps")
+    g = gpu_guide(p)
+    grid0, mlp0 = g.get_weights()
+    state = [(grid0.reshape(-1), np.zeros(grid0.size, np.float32), np.zeros(grid0.size, np.float32)),
+             (mlp0, np.zeros(mlp0.size, np.float32), np.zeros(mlp0.size, np.float32))]
+    for step, (lo, hi) in enumerate(((0, 1500), (1500, 3000)), start=1):
+        g.backward(pack_samples(pos[lo:hi], nrm[lo:hi], wo[lo:hi], w[lo:hi], c[lo:hi]))
+        ggrid, gmlp, _ = g.gradients()
+        g.adam_step()
+        grid, mlp = g.get_weights()
+        state = [G.adam_step(*state[0], ggrid.reshape(-1), step), G.adam_step(*state[1], gmlp, step)]
+        for name, got, exp, prev in (("grid", grid.reshape(-1), state[0][0], None), ("mlp", mlp, state[1][0], None)):
+            err = np.abs(got - exp)
+            moved = np.abs(exp - (grid0.reshape(-1) if name == "grid" else mlp0)) > 0
+            say(f"  step {step} {name}: {moved.sum()} weights moved, max abs err {err.max():.2e}")
+            assert moved.sum() > 100
+            assert np.all(err <= 2e-6 * np.maximum(1.0, np.abs(exp))), (name, step, err.max())
+    g.close()
+
+
 def test_training_learns_a_position_dependent_lobe():
     """Directions follow a vMF lobe (kappa 20) whose axis turns with the
     position; Adam on the network's own gradients raises the held-out log

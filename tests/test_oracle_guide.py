@@ -204,3 +204,30 @@ def test_backward_matches_finite_differences():
         p["grid"] = grid64.copy()
         num = (lp - lm) / (2 * hh)
         assert ggrid[lev, e, f] == pytest.approx(num, rel=2e-3, abs=1e-6), (lev, e, f)
+
+
+# ---------------------------------------------------------------- Adam
+def test_adam_first_steps_closed_form():
+    """From zero moments the bias-corrected moments of step 1 are exactly g
+    and g^2, so the step is lr g / (|g| + eps); with the same gradient again
+    they stay g and g^2.  With g1 then g2 the corrected first moment is
+    (b1 (1 - b1) g1 + (1 - b1) g2) / (1 - b1^2) and the second
+    (b2 (1 - b2) g1^2 + (1 - b2) g2^2) / (1 - b2^2)."""
+    from oracle import ora_guide as G
+    rng = np.random.default_rng(0)
+    p0 = rng.normal(size=1000).astype(np.float32)
+    g1 = rng.normal(size=1000).astype(np.float32)
+    g2 = rng.normal(size=1000).astype(np.float32)
+    z = np.zeros(1000, np.float32)
+    lr, b1, b2, eps = 1e-2, 0.9, 0.99, 1e-8
+    p1, m1, v1 = G.adam_step(p0, z, z, g1, 1, lr, b1, b2, eps)
+    assert np.allclose(p1, p0 - lr * g1 / (np.abs(g1) + eps), rtol=0, atol=2e-8)
+    assert np.allclose(m1, 0.1 * g1, rtol=1e-6) and np.allclose(v1, 0.01 * g1 * g1, rtol=1e-5)
+    p1b, _, _ = G.adam_step(p1, m1, v1, g1, 2, lr, b1, b2, eps)
+    assert np.allclose(p1b, p1 - lr * g1 / (np.abs(g1) + eps), rtol=0, atol=5e-8)
+    p2, m2, v2 = G.adam_step(p1, m1, v1, g2, 2, lr, b1, b2, eps)
+    mh = (b1 * (1 - b1) * g1.astype(float) + (1 - b1) * g2) / (1 - b1 ** 2)
+    vh = (b2 * (1 - b2) * g1.astype(float) ** 2 + (1 - b2) * g2.astype(float) ** 2) / (1 - b2 ** 2)
+    assert np.allclose(p2, p1 - lr * mh / (np.sqrt(vh) + eps), rtol=0, atol=4e-7)
+    # a zero gradient from zero moments leaves the weight where it is
+    assert np.array_equal(G.adam_step(p0, z, z, z, 1)[0], p0)
