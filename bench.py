@@ -49,6 +49,9 @@ def parse():
     p.add_argument("--res", type=int, default=1024)
     p.add_argument("--n-random", type=int, default=100_000)
     p.add_argument("--quiet", action="store_true")
+    p.add_argument("--treelet-passes", type=int, default=0,
+                   help="SAH treelet restructuring passes of the ray-cast tree (0: the library default, -1: none)")
+    p.add_argument("--builder", type=str, default="lbvh", help="tree builder of the ray-cast batches")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run only a plain 1 GiB device-to-device copy (HBM counter calibration)")
     p.add_argument("--no-pt", action="store_true", help="skip the path-tracing measurement")
@@ -390,9 +393,10 @@ def main():
     verts, tris, coherent, incoherent, shadow = scenes.raycast_workload(args.res, args.n_random)
     log(args, f"workload: {len(tris)} tris, {len(coherent)} coherent + {len(incoherent)} incoherent rays "
               f"({time.perf_counter() - t0:.1f}s)")
-    acc = GpuAccelerator(verts, tris)
+    acc = GpuAccelerator(verts, tris, builder=args.builder, treelet_passes=args.treelet_passes)
     info = acc.info()
-    log(args, f"lbvh: {info['n_nodes']} nodes, depth {info['max_depth']}, build {info['build_ms']:.1f} ms")
+    log(args, f"{args.builder}: {info['n_nodes']} nodes, depth {info['max_depth']}, build {info['build_ms']:.1f} ms, "
+              f"treelet passes {info['treelet_passes']} ({info['treelets_rewritten']} rewritten)")
 
     d_inc = torch.from_numpy(incoherent).cuda()
     d_coh = torch.from_numpy(coherent).cuda()
@@ -480,6 +484,8 @@ def main():
             "coherent_tris_per_ray": round(nt_coh / len(coherent), 2),
             "anyhit_nodes_per_ray": round(nf_sha / n, 2),
             "anyhit_tris_per_ray": round(nt_sha / n, 2),
+            "bvh_builder": args.builder,
+            "bvh_treelet_passes": info["treelet_passes"],
             "bvh_build_ms": round(info["build_ms"], 2),
             "bvh_nodes": info["n_nodes"],
             "bvh_depth": info["max_depth"],

@@ -80,7 +80,10 @@ typedef struct ymx_accel_params
 	uint32_t struct_size;   /* sizeof(ymx_accel_params) */
 	int32_t max_leaf_size;  /* at most this many triangles per leaf (1..8), 0 = default (4) */
 	int32_t builder;        /* YMX_BUILDER_LBVH (0, default) or YMX_BUILDER_PLOC */
-	int32_t reserved[5];
+	int32_t treelet_passes; /* SAH treelet restructuring passes over the built tree
+	                           (Karras & Aila 2013): 1..8, -1 = none,
+	                           0 = default (YMX_TREELET_PASSES_DEFAULT) */
+	int32_t reserved[4];
 } ymx_accel_params;
 
 /* Tree builders.  LBVH (Karras 2012) splits at Morton-code prefixes; PLOC
@@ -91,6 +94,7 @@ typedef struct ymx_accel_params
  * path-traced rooms render 4-12 % faster on PLOC trees. */
 #define YMX_BUILDER_LBVH 0
 #define YMX_BUILDER_PLOC 1
+#define YMX_TREELET_PASSES_DEFAULT 0
 
 typedef struct ymx_accel_info
 {
@@ -104,6 +108,8 @@ typedef struct ymx_accel_info
 	float bound_min[3];      /* tree bound (getBound()) */
 	float bound_max[3];
 	uint32_t n_quads;        /* quad primitives (ymx_accel_create_faces) */
+	uint32_t treelet_passes; /* SAH treelet restructuring passes run */
+	uint32_t treelets_rewritten;  /* treelets whose topology those passes changed */
 } ymx_accel_info;
 
 /* Build the accelerator over n_tris triangles on the current HIP device.

@@ -36,7 +36,7 @@ class GpuAccelerator:
     type_name = "mi355x-lbvh"
 
     def __init__(self, verts: np.ndarray, tris: np.ndarray, vis: np.ndarray | None = None,
-                 max_leaf_size: int = 0, builder: str = "lbvh"):
+                 max_leaf_size: int = 0, builder: str = "lbvh", treelet_passes: int = 0):
         self._lib = native.lib()
         verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
         tris = np.ascontiguousarray(tris, dtype=np.uint32)
@@ -48,7 +48,8 @@ class GpuAccelerator:
                 raise ValueError("one visibility mask per triangle is required")
         if builder not in native.BUILDERS:
             raise ValueError(f"unknown BVH builder {builder!r} (one of {sorted(native.BUILDERS)})")
-        params = YmxAccelParams(C.sizeof(YmxAccelParams), int(max_leaf_size), native.BUILDERS[builder])
+        params = YmxAccelParams(C.sizeof(YmxAccelParams), int(max_leaf_size), native.BUILDERS[builder],
+                                int(treelet_passes))
         handle = C.c_void_p()
         create = self._lib.ymx_accel_create_faces if faces else self._lib.ymx_accel_create
         rc = create(
@@ -88,6 +89,7 @@ class GpuAccelerator:
             "n_tris": inf.n_tris, "n_nodes": inf.n_nodes, "max_depth": inf.max_depth,
             "node_bytes": inf.node_bytes, "tri_bytes": inf.tri_bytes, "build_ms": inf.build_ms,
             "bound_min": list(inf.bound_min), "bound_max": list(inf.bound_max), "n_quads": inf.n_quads,
+            "treelet_passes": inf.treelet_passes, "treelets_rewritten": inf.treelets_rewritten,
         }
 
     # --- batched queries on device tensors -------------------------------

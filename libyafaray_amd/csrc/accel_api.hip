@@ -56,7 +56,7 @@ static int create_accel(const char *fn, const float *verts, uint32_t n_verts, co
 	}
 	// 4 triangles per leaf: on the 2.19M-triangle heightfield +2 % incoherent
 	// and +14 % coherent closest hit over 1 (LBVH; PLOC keeps 1)
-	int max_leaf = 4, builder = YMX_BUILDER_LBVH;
+	int max_leaf = 4, builder = YMX_BUILDER_LBVH, treelet_passes = YMX_TREELET_PASSES_DEFAULT;
 	if(params)
 	{
 		if(params->struct_size != sizeof(ymx_accel_params))
@@ -76,6 +76,12 @@ static int create_accel(const char *fn, const float *verts, uint32_t n_verts, co
 			return YMX_ERR_INVALID_ARG;
 		}
 		builder = params->builder;
+		if(params->treelet_passes < -1 || params->treelet_passes > 8)
+		{
+			ymx_set_error("%s: treelet_passes %d outside -1..8", fn, params->treelet_passes);
+			return YMX_ERR_INVALID_ARG;
+		}
+		if(params->treelet_passes != 0) treelet_passes = params->treelet_passes < 0 ? 0 : params->treelet_passes;
 	}
 	// a face's 4th index is YMX_NO_VERTEX for a triangle; quads get e3 slots
 	std::vector<uint32_t> slot;
@@ -111,8 +117,9 @@ static int create_accel(const char *fn, const float *verts, uint32_t n_verts, co
 		return YMX_ERR_HIP;
 	}
 	acc->n_quads = n_quads;
+	acc->treelet_passes = (uint32_t)treelet_passes;
 	int rc = ymx_build_device(acc, verts, n_verts, idx, stride, n_quads ? slot.data() : nullptr, vis, n_tris, max_leaf,
-	                          builder);
+	                          builder, treelet_passes);
 	if(rc == YMX_OK) rc = ymx_trace_setup(acc);
 	if(rc != YMX_OK)
 	{
@@ -193,6 +200,8 @@ int ymx_accel_get_info(const ymx_accel *accel, ymx_accel_info *info)
 	info->tri_bytes = 48ull * (accel->n_tris ? accel->n_tris : 1);
 	info->build_ms = accel->build_ms;
 	info->n_quads = accel->n_quads;
+	info->treelet_passes = accel->treelet_passes;
+	info->treelets_rewritten = accel->treelets_rewritten;
 	memcpy(info->bound_min, accel->bmin, sizeof info->bound_min);
 	memcpy(info->bound_max, accel->bmax, sizeof info->bound_max);
 	return YMX_OK;

@@ -23,6 +23,7 @@
 // (src/accelerator/accelerator_kdtree_original.cc:65-645); only its query
 // results and its tree bound (:89-103) are part of the contract.
 #include "accel_internal.h"
+#include "treelet.h"
 
 #include <hipcub/hipcub.hpp>
 
@@ -327,7 +328,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_depth(const int32_t *__restrict
 	if(j < n)
 	{
 		int32_t node = parent_leaf[j];
-		while(node >= 0)
+		while(node >= 0 && d <= n)  // at most n - 1 internal nodes on a path
 		{
 			++d;
 			node = (node == 0) ? -1 : parent_internal[node];
@@ -335,6 +336,53 @@ __global__ __launch_bounds__(kBuildBlock) void k_depth(const int32_t *__restrict
 	}
 	for(int off = 32; off > 0; off >>= 1) d = max(d, (uint32_t)__shfl_xor((int)d, off));
 	if((threadIdx.x & 63) == 0) atomicMax(max_depth, d);
+}
+
+// ---- SAH treelet restructuring (treelet.h), bottom-up over the whole tree:
+// one thread per leaf climbs towards the root; the second arrival at a node
+// finds both of its subtrees final, restructures the treelet rooted there
+// (when it holds at least min_tris triangles) and goes on up.  The treelet's
+// DP tables live in LDS, subset-major so that the lanes of a wave touch
+// adjacent words.
+constexpr int kTreeletBlock = 64;
+
+__global__ __launch_bounds__(kTreeletBlock) void k_treelets(float4 *nodes, int32_t *parent_int, int32_t *parent_leaf,
+                                                            uint32_t n, uint32_t *flags, float *cost, uint32_t *count,
+                                                            uint32_t min_tris, uint32_t *rewritten)
+{
+	// rewritten[0]: treelets rewritten; rewritten[1]: set on an inconsistent tree
+	__shared__ float dp_cost[ymx::kTreeletSubsets * kTreeletBlock];
+	__shared__ uint8_t dp_part[ymx::kTreeletSubsets * kTreeletBlock];
+	const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+	if(j >= n) return;
+	const ymx::TreeletTree t{nodes, parent_int, parent_leaf, cost, count};
+	uint32_t done = 0;
+	int32_t node = parent_leaf[j];
+	while(node >= 0)
+	{
+		// the first arrival leaves; the second finds both subtrees final (the
+		// fences order the other thread's restructuring writes before its
+		// arrival and this thread's reads after it)
+		__threadfence();
+		const uint32_t arrivals = atomicAdd(&flags[node], 1u);
+		if(arrivals == 0u) break;
+		if(arrivals > 1u)
+		{
+			// a node reached three times: the parent links are not a tree
+			atomicOr(rewritten + 1, 1u);
+			break;
+		}
+		__threadfence();
+		uint32_t c = 0;
+		int32_t c0, c1;
+		ymx::tl_refs(nodes, node, c0, c1);
+		c = (c0 >= 0 ? count[c0] : 1u) + (c1 >= 0 ? count[c1] : 1u);
+		if(c >= min_tris) done += (uint32_t)ymx::tl_restructure(t, node, dp_cost + threadIdx.x, dp_part + threadIdx.x,
+		                                                        kTreeletBlock);
+		else ymx::tl_node_cost(t, node);
+		node = (node == 0) ? -1 : parent_int[node];
+	}
+	if(done) atomicAdd(rewritten, done);
 }
 
 // ---- PLOC: parallel locally-ordered clustering (D. Meister, J. Bittner,
@@ -840,7 +888,8 @@ int relayout_tree(DevBuf &d_nodes, DevBuf &d_tris, uint32_t n_tris, bool ploc_le
 }  // namespace
 
 int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const uint32_t *idx, uint32_t stride,
-                     const uint32_t *quad_slot, const uint8_t *vis, uint32_t n_tris, int max_leaf, int builder)
+                     const uint32_t *quad_slot, const uint8_t *vis, uint32_t n_tris, int max_leaf, int builder,
+                     int treelet_passes)
 {
 	acc->n_tris = n_tris;
 	acc->build_ms = 0.f;
@@ -1025,7 +1074,7 @@ int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const
 		                                                      n_tris, d_parent_int.as<int32_t>(),
 		                                                      d_parent_leaf.as<int32_t>(), d_flags.as<uint32_t>(), pad);
 		YMX_HIP_CHECK(hipGetLastError());
-		if(max_leaf > 1)
+		if(max_leaf > 1 && treelet_passes == 0)
 		{
 			k_collapse_leaves<<<grid_for(n_tris - 1), kBuildBlock, 0, stream>>>(d_nodes.as<float4>(), d_range.as<uint2>(),
 			                                                                    n_tris, (uint32_t)max_leaf);
@@ -1108,6 +1157,37 @@ int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const
 			return YMX_ERR_HIP;
 		}
 	}
+	if(treelet_passes > 0 && n_tris >= 3)
+	{
+		// SAH treelet restructuring of the one-triangle-leaf tree; leaves of
+		// up to max_leaf triangles are then formed by the depth-first
+		// triangle rewrite of relayout_tree (the LBVH key ranges no longer
+		// describe the subtrees)
+		DevBuf d_tflags, d_tcost, d_tcount, d_trew;
+		YMX_HIP_CHECK(d_tflags.alloc(sizeof(uint32_t) * (size_t)(n_tris - 1)));
+		YMX_HIP_CHECK(d_tcost.alloc(sizeof(float) * (size_t)(n_tris - 1)));
+		YMX_HIP_CHECK(d_tcount.alloc(sizeof(uint32_t) * (size_t)(n_tris - 1)));
+		YMX_HIP_CHECK(d_trew.alloc(2 * sizeof(uint32_t)));
+		YMX_HIP_CHECK(hipMemsetAsync(d_trew.p, 0, 2 * sizeof(uint32_t), stream));
+		for(int pass = 0; pass < treelet_passes; ++pass)
+		{
+			YMX_HIP_CHECK(hipMemsetAsync(d_tflags.p, 0, sizeof(uint32_t) * (size_t)(n_tris - 1), stream));
+			k_treelets<<<(n_tris + kTreeletBlock - 1) / kTreeletBlock, kTreeletBlock, 0, stream>>>(
+			    d_nodes.as<float4>(), d_parent_int.as<int32_t>(), d_parent_leaf.as<int32_t>(), n_tris,
+			    d_tflags.as<uint32_t>(), d_tcost.as<float>(), d_tcount.as<uint32_t>(), (uint32_t)max_leaf + 1u,
+			    d_trew.as<uint32_t>());
+			YMX_HIP_CHECK(hipGetLastError());
+		}
+		uint32_t hr[2] = {0u, 0u};
+		YMX_HIP_CHECK(hipMemcpyAsync(hr, d_trew.p, sizeof hr, hipMemcpyDeviceToHost, stream));
+		YMX_HIP_CHECK(hipStreamSynchronize(stream));
+		if(hr[1])
+		{
+			ymx_set_error("ymx_accel_create: treelet restructuring found an inconsistent tree");
+			return YMX_ERR_HIP;
+		}
+		acc->treelets_rewritten = hr[0];
+	}
 	k_depth<<<grid_for(n_tris), kBuildBlock, 0, stream>>>(d_parent_int.as<int32_t>(), d_parent_leaf.as<int32_t>(),
 	                                                      n_tris, d_depth.as<uint32_t>());
 	YMX_HIP_CHECK(hipGetLastError());
@@ -1118,10 +1198,10 @@ int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const
 	// and traces 2 % faster.  PLOC: leaves of up to max_leaf triangles need the
 	// triangles in depth-first order first; with one triangle per leaf the tree
 	// is left as built (the relayout alone gains nothing on it).
-	if(builder != YMX_BUILDER_PLOC || max_leaf > 1)
+	if(builder != YMX_BUILDER_PLOC || max_leaf > 1 || treelet_passes > 0)
 	{
-		const int rc = relayout_tree(d_nodes, d_tris, n_tris, builder == YMX_BUILDER_PLOC, (uint32_t)max_leaf, stream,
-		                             n_nodes);
+		const bool rewrite_leaves = builder == YMX_BUILDER_PLOC || treelet_passes > 0;
+		const int rc = relayout_tree(d_nodes, d_tris, n_tris, rewrite_leaves, (uint32_t)max_leaf, stream, n_nodes);
 		if(rc != YMX_OK) return rc;
 	}
 	YMX_HIP_CHECK(hipEventRecord(ev.b, stream));

@@ -67,6 +67,8 @@ struct ymx_accel
 	float4 *d_tris;
 	float4 *d_quads;         // e3 of each quad primitive (NULL without quads)
 	uint32_t n_quads;
+	uint32_t treelet_passes;      // SAH treelet restructuring passes run at build
+	uint32_t treelets_rewritten;  // treelets whose topology they changed
 	// Traversal stack: YMX_LDS_STACK entries in LDS plus spill_depth in HBM,
 	// max_depth + 1 in all.  A lane at an internal node of depth D holds at
 	// most D deferred siblings (one per level of its root path), and internal
@@ -118,7 +120,8 @@ struct ymx_trace_args
 // quad_slot (NULL: no quads): 0 for a triangle, else the 1-based slot of the
 // quad's e3 in acc->d_quads (acc->n_quads slots).
 int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const uint32_t *idx, uint32_t stride,
-                     const uint32_t *quad_slot, const uint8_t *vis, uint32_t n_tris, int max_leaf, int builder);
+                     const uint32_t *quad_slot, const uint8_t *vis, uint32_t n_tris, int max_leaf, int builder,
+                     int treelet_passes);
 int ymx_launch_closest(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t n,
                        hipStream_t stream, unsigned long long *stats);
 int ymx_launch_shadow(const ymx_accel *acc, const ymx_ray *rays, uint32_t *occ, uint32_t *prim,

@@ -122,8 +122,8 @@ class Accelerator
 		IntersectData intersect(const Ray &ray, float t_max) const;
 		IntersectData intersectShadow(const Ray &ray, float t_max) const;
 		// transparent[p]: primitive p's material isTransparent(); transparency
-		// (p, dir) its getTransparency, multiplied in the order met front to
-		// back.  A hit is an occlusion.
+		// (p, dir, t) its getTransparency at the caster's hit ray.from + t * dir,
+		// multiplied in the order met front to back.  A hit is an occlusion.
 		template <typename F>
 		IntersectData intersectTransparentShadow(const Ray &ray, int max_depth, float t_max,
 		                                         const std::vector<uint8_t> &transparent, F transparency) const;
@@ -330,7 +330,7 @@ IntersectData Accelerator::intersectTransparentShadow(const Ray &ray, int max_de
 	std::sort(kept.begin(), kept.end());
 	for(const auto &k : kept)
 	{
-		const std::array<float, 3> c = transparency(k.second, ray.dir);
+		const std::array<float, 3> c = transparency(k.second, ray.dir, k.first);
 		for(int i = 0; i < 3; ++i) d.color[i] *= c[i];
 	}
 	return d;

@@ -29,7 +29,7 @@ class YmxHit(C.Structure):
 
 class YmxAccelParams(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("max_leaf_size", C.c_int32), ("builder", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("treelet_passes", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 BUILDERS = {"lbvh": 0, "ploc": 1}
@@ -47,6 +47,8 @@ class YmxAccelInfo(C.Structure):
         ("bound_min", C.c_float * 3),
         ("bound_max", C.c_float * 3),
         ("n_quads", C.c_uint32),
+        ("treelet_passes", C.c_uint32),
+        ("treelets_rewritten", C.c_uint32),
     ]
 
 

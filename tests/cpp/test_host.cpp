@@ -234,7 +234,7 @@ static void test_accel_transparent()
 	for(size_t i = 0; i < rays.size(); ++i)
 	{
 		std::set<uint32_t> met;
-		auto tr = [&](uint32_t p, const float *) {
+		auto tr = [&](uint32_t p, const float *, float) {
 			met.insert(p);
 			return std::array<float, 3>{0.5f, 0.75f, 1.f};
 		};
