@@ -156,6 +156,22 @@ int ymx_accel_get_info(const ymx_accel *accel, ymx_accel_info *info);
 int ymx_accel_intersect(const ymx_accel *accel, const ymx_ray *d_rays, ymx_hit *d_hits,
                         uint32_t n_rays, void *stream);
 
+/* Closest hit as ymx_accel_intersect (same kernels, same results), allocating
+ * nothing.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The traversal's ray counters and stack
+ * spill area live in a fixed 64 MiB device workspace inside libymx.so, which
+ * every GPU the library runs on holds from the library's load until the
+ * process ends (its address is looked up by ymx_accel_create on the handle's
+ * device).  A tree too deep for the workspace (spill columns of more than
+ * 64 MiB for one 128-lane block, max_depth above about 131,000) is refused
+ * with YMX_ERR_UNSUPPORTED.  Every call on a GPU shares that GPU's workspace:
+ * calls that overlap on one GPU (on different streams or from different
+ * threads) race, and the caller must order them; calls on one stream are
+ * ordered by the stream. */
+int ymx_accel_intersect_noalloc(const ymx_accel *accel, const ymx_ray *d_rays, ymx_hit *d_hits,
+                                uint32_t n_rays, void *stream);
+
 /* Any hit for n rays: Accelerator::intersectShadow(ray, t_max) with
  * t_max = ray.tmax >= 0 ? ray.tmax : FLT_MAX; ray.tmin is ignored, the
  * dynamic bias alone is t_min (include/accelerator/accelerator.h:51,

@@ -253,6 +253,21 @@ int ymx_accel_intersect(const ymx_accel *accel, const ymx_ray *d_rays, ymx_hit *
 	return ymx_launch_closest(accel, d_rays, d_hits, n_rays, (hipStream_t)stream, nullptr);
 }
 
+int ymx_accel_intersect_noalloc(const ymx_accel *accel, const ymx_ray *d_rays, ymx_hit *d_hits, uint32_t n_rays,
+                                void *stream)
+{
+	if(n_rays == 0) return YMX_OK;
+	int rc = check_count(n_rays, "ymx_accel_intersect_noalloc");
+	if(rc == YMX_OK) rc = check_query(accel, d_rays, d_hits, "ymx_accel_intersect_noalloc");
+	if(rc != YMX_OK) return rc;
+	if((uintptr_t)d_hits & 15u)
+	{
+		ymx_set_error("ymx_accel_intersect_noalloc: hit buffer must be 16-byte aligned");
+		return YMX_ERR_INVALID_ARG;
+	}
+	return ymx_launch_closest_noalloc(accel, d_rays, d_hits, n_rays, (hipStream_t)stream);
+}
+
 int ymx_accel_intersect_shadow(const ymx_accel *accel, const ymx_ray *d_rays, uint32_t *d_occluded,
                                uint32_t *d_prim, uint32_t n_rays, void *stream)
 {

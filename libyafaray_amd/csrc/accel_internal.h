@@ -139,6 +139,12 @@ int ymx_launch_closest_queue(const ymx_accel *acc, const ymx_ray *rays, ymx_hit 
                              uint32_t capacity, hipStream_t stream);
 int ymx_launch_shadow_queue(const ymx_accel *acc, const ymx_ray *rays, uint32_t *occ, const uint32_t *n_dev,
                             uint32_t capacity, hipStream_t stream);
+// Allocation-free closest hit over the fixed per-device workspace of
+// accel_trace.hip; ymx_noalloc_setup (run by ymx_trace_setup with the handle's
+// device current) looks up that device's workspace address once.
+int ymx_noalloc_setup(int device);
+int ymx_launch_closest_noalloc(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t n,
+                               hipStream_t stream);
 
 void ymx_set_error(const char *fmt, ...);
 

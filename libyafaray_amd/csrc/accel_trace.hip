@@ -617,6 +617,9 @@ int ymx_trace_setup(ymx_accel *acc)
 	                                                                         : acc->grid_threads_shadow;
 	acc->grid_threads_max = gt;
 	acc->spill_depth = acc->max_depth > (uint32_t)kStack ? acc->max_depth - (uint32_t)kStack + 1u : 1u;
+	// the allocation-free query's workspace on this device
+	const int rc = ymx_noalloc_setup(acc->device);
+	if(rc != YMX_OK) return rc;
 	// the work area of the default stream, so its first query allocates nothing
 	std::lock_guard<std::mutex> lock(acc->work_mutex);
 	return ymx_work_area_for(acc, nullptr) ? YMX_OK : YMX_ERR_OUT_OF_MEMORY;
@@ -699,4 +702,95 @@ int ymx_launch_transparent(const ymx_accel *acc, const ymx_ray *rays, const uint
 {
 	return launch<kTransparent, false>(acc, rays, nullptr, occ, nullptr, n, stream, nullptr, n_dev, transp, max_depth,
 	                                   counts, hits);
+}
+
+// ---------------------------------------------------------------------------
+// Fixed workspace of the allocation-free closest-hit query: the ray counter
+// and the finished-wave counter in its first kScratchHead words, then the
+// spill columns of the persistent lanes.  Every device holds its own copy
+// from the load of this library until the process ends.  Host code must not
+// pass `scratch` itself to a kernel (on the host it only names the symbol):
+// ymx_noalloc_setup looks up each device's address once, and the launches use
+// that address.
+namespace {
+constexpr size_t kScratchWords = (size_t)16 << 20;  // 64 MiB of int32
+constexpr size_t kScratchHead = 64;                 // 256 bytes: the two counters, padded
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mutex;
+int32_t *g_scratch_ptr[kScratchDevices];            // per device, set once
+}  // namespace
+
+static __device__ __align__(256) int32_t scratch[kScratchWords];
+
+int ymx_noalloc_setup(int device)
+{
+	if(device < 0 || device >= kScratchDevices)
+	{
+		ymx_set_error("query workspace: device %d outside 0..%d", device, kScratchDevices - 1);
+		return YMX_ERR_UNSUPPORTED;
+	}
+	std::lock_guard<std::mutex> lock(g_scratch_mutex);
+	if(g_scratch_ptr[device]) return YMX_OK;
+	void *p = nullptr;
+	const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+	if(e != hipSuccess || !p)
+	{
+		ymx_set_error("query workspace: hipGetSymbolAddress on device %d failed: %s", device, hipGetErrorString(e));
+		return YMX_ERR_HIP;
+	}
+	g_scratch_ptr[device] = static_cast<int32_t *>(p);
+	return YMX_OK;
+}
+
+int ymx_launch_closest_noalloc(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t n,
+                               hipStream_t stream)
+{
+	if(n == 0) return YMX_OK;
+	int32_t *ws = nullptr;
+	{
+		std::lock_guard<std::mutex> lock(g_scratch_mutex);
+		if(acc->device >= 0 && acc->device < kScratchDevices) ws = g_scratch_ptr[acc->device];
+	}
+	if(!ws)
+	{
+		ymx_set_error("ymx_accel_intersect_noalloc: no query workspace on device %d", acc->device);
+		return YMX_ERR_HIP;
+	}
+	// as many persistent lanes as the closest-hit grid has, or as many whole
+	// blocks as the spill area holds columns of spill_depth entries
+	const size_t fit = (kScratchWords - kScratchHead) / acc->spill_depth;
+	uint32_t lanes = acc->grid_threads_closest;
+	if(fit < lanes) lanes = (uint32_t)(fit / kTraceBlock) * (uint32_t)kTraceBlock;
+	if(lanes < (uint32_t)kTraceBlock)
+	{
+		ymx_set_error("ymx_accel_intersect_noalloc: a tree of depth %u needs %zu bytes of workspace per %d-lane "
+		              "block; the fixed workspace holds %zu",
+		              acc->max_depth, sizeof(int32_t) * (size_t)acc->spill_depth * kTraceBlock, kTraceBlock,
+		              sizeof(int32_t) * (kScratchWords - kScratchHead));
+		return YMX_ERR_UNSUPPORTED;
+	}
+	ymx_trace_args a = {};
+	a.nodes = acc->d_nodes;
+	a.tris = acc->d_tris;
+	a.quads = acc->d_quads;
+	a.rays = rays;
+	a.n_rays = n;
+	for(int k = 0; k < 3; ++k)
+	{
+		a.bmin[k] = acc->bmin[k];
+		a.bmax[k] = acc->bmax[k];
+	}
+	a.work = reinterpret_cast<uint32_t *>(ws);
+	a.spill = ws + kScratchHead;
+	a.spill_depth = acc->spill_depth;
+	a.grid_threads = lanes;  // the spill columns' stride
+	a.hits = hits;
+	const uint32_t needed = ((n + 63u) / 64u) * 64u;
+	const uint32_t threads = lanes < needed ? lanes : needed;
+	const uint32_t blocks = (threads + kTraceBlock - 1) / kTraceBlock;
+	// the counters start at zero: a fill kernel, not a memset node
+	k_clear_words<<<1, 64, 0, stream>>>(a.work, 2u);
+	k_dyn<kClosest, false><<<blocks, kTraceBlock, 0, stream>>>(a);
+	YMX_HIP_CHECK(hipGetLastError());
+	return YMX_OK;
 }

@@ -82,6 +82,7 @@ SIGNATURES = {
     "ymx_accel_bound": (C.c_int, [C.c_void_p, C.c_float * 3, C.c_float * 3]),
     "ymx_accel_get_info": (C.c_int, [C.c_void_p, C.POINTER(YmxAccelInfo)]),
     "ymx_accel_intersect": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "ymx_accel_intersect_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "ymx_accel_intersect_shadow": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                                              C.c_void_p]),
     "ymx_accel_export": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64]),
