@@ -110,6 +110,20 @@ typedef struct ymx_pt_settings
 	int32_t raydepth;             /* "raydepth" of the recursive raytrace, default 5 (0..10) */
 	int32_t transparent_shadows;  /* "transpShad": shadow rays pass transparent materials, default 0 */
 	int32_t shadow_depth;         /* "shadowDepth": most transparent casters a shadow ray passes, default 4 (0..64) */
+	/* Multi-pass adaptive anti-aliasing (TiledIntegrator::render, ymx_pt_render_passes),
+	 * ImageFilm's AA params (include/render/imagefilm.h:212-226); aa_passes 0 means 1 */
+	int32_t aa_passes;                        /* "AA_passes" [1] */
+	int32_t aa_inc_samples;                   /* "AA_inc_samples": samples of each further pass [1] */
+	float aa_threshold;                       /* "AA_threshold" [0.05] */
+	float aa_resampled_floor;                 /* "AA_resampled_floor": % of the pixels [0] */
+	float aa_sample_multiplier_factor;        /* "AA_sample_multiplier_factor" [1] */
+	float aa_light_sample_multiplier_factor;  /* "AA_light_sample_multiplier_factor" [1] */
+	int32_t aa_detect_color_noise;            /* "AA_detect_color_noise" [0] */
+	int32_t aa_dark_detection_type;           /* "AA_dark_detection_type": 0 none, 1 linear, 2 curve [0] */
+	float aa_dark_threshold_factor;           /* "AA_dark_threshold_factor" [0] */
+	int32_t aa_variance_edge_size;            /* "AA_variance_edge_size" [10] */
+	int32_t aa_variance_pixels;               /* "AA_variance_pixels" [0] */
+	float aa_clamp_samples;                   /* "AA_clamp_samples": proportional clamp of each sample, 0 off [0] */
 } ymx_pt_settings;
 
 typedef struct ymx_pt ymx_pt;
@@ -125,12 +139,39 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 
 int ymx_pt_destroy(ymx_pt *pt);
 
+/* Shading normals of smooth meshes and meshes with vertex normals
+ * (PrimitivePolygon::getSurfacePolygon, src/geometry/primitive/primitive_polygon.cc:45-51;
+ * the normals come from yafaray_addNormal or yafaray_smoothObjectMesh,
+ * src/geometry/object/object_mesh.cc:150-264).  corner_normals: host, 12
+ * floats per triangle, corner k at 4 k as (nx, ny, nz, w): w > 0 a corner
+ * normal, w == 0 a corner without one (the face normal stands in, as
+ * FacePrimitive::getVertexNormal does), and w < 0 in corner 0 a flat triangle
+ * (shading normal = face normal).  A hit's shading normal is the corners'
+ * normals weighted by its barycentric coordinates (1 - u - v, u, v), then
+ * normalised.  NULL makes every triangle flat again.  Not with transparent
+ * shadows through smooth fake-shadow glass (YMX_ERR_UNSUPPORTED). */
+int ymx_pt_set_normals(ymx_pt *pt, const float *corner_normals);
+
 /* Zero the film (colour and weight sums). */
 int ymx_pt_clear_film(ymx_pt *pt, void *stream);
 
 /* Trace AA samples [first_sample, first_sample + n_samples) of every pixel and
  * splat them into the film, asynchronously on `stream`. */
 int ymx_pt_render(ymx_pt *pt, int32_t first_sample, int32_t n_samples, void *stream);
+
+/* The whole multi-pass render of TiledIntegrator::render
+ * (src/integrator/surface/integrator_tiled.cc:160-214): pass 0 renders
+ * AA_minsamples samples of every pixel; each further pass of aa_passes renders
+ * ceil(AA_inc_samples x sample multiplier) more samples of the pixels that
+ * ImageFilm::nextPass (src/render/imagefilm.cc:365-520) flags from the film so
+ * far, with that pass's light sample multiplier, and lowers the threshold when
+ * fewer pixels than AA_resampled_floor were flagged.  The flags are computed
+ * on the device; one host synchronisation per pass reads their count.  With
+ * aa_passes > 1 every pass takes its sub-pixel offsets from the scrambled
+ * (0, 2) sequence (riVdC, riS), as the reference does.  Renders into the film
+ * as it is (clear it first for a new image).  Returns the pixels resampled in
+ * each pass in resampled[] (aa_passes entries; may be NULL). */
+int ymx_pt_render_passes(ymx_pt *pt, int32_t *resampled, void *stream);
 
 /* Copy the film sums to host memory (synchronous): rgba[4 * (y * resx + x)]
  * holds sum(weight * colour, weight * alpha) and weight[y * resx + x] the

@@ -90,7 +90,8 @@ void free_pt(ymx_pt *pt)
 	                pt->st.cam_hit, pt->st.dxy, pt->st.flags, pt->st.light, pt->st.vert_emit, pt->st.ext_rays[0],
 	                pt->st.ext_rays[1], pt->st.ext_path[0], pt->st.ext_path[1], pt->st.ext_count, pt->st.ext_hits,
 	                pt->st.slot_contrib, pt->st.slot_queue, pt->st.sh_rays, pt->st.sh_occ, pt->st.sh_count,
-	                pt->st.sh_tcount, pt->st.sh_thits, pt->d_transp, pt->st.totals, pt->nd.spawn};
+	                pt->st.sh_tcount, pt->st.sh_thits, pt->d_transp, pt->st.totals, pt->nd.spawn, pt->d_aa,
+	                pt->d_cnorm};
 	for(void *p : ptrs)
 		if(p) (void)hipFree(p);
 	if(pt->ev0) (void)hipEventDestroy(pt->ev0);
@@ -307,6 +308,43 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 		}
 	}
 
+	const int32_t aa_passes = st->aa_passes > 1 ? st->aa_passes : 1;
+	if(aa_passes > 1 && (aa_passes > 1000 || st->aa_inc_samples < 1 || !(st->aa_sample_multiplier_factor > 0.f) ||
+	                     !(st->aa_light_sample_multiplier_factor > 0.f) || st->aa_dark_detection_type < 0 ||
+	                     st->aa_dark_detection_type > 2 || st->aa_variance_edge_size < 0 ||
+	                     st->aa_variance_edge_size > 256))
+	{
+		ymx_set_error("ymx_pt_create: unsupported AA pass settings (passes %d, inc samples %d, multipliers %g %g)",
+		              st->aa_passes, st->aa_inc_samples, st->aa_sample_multiplier_factor,
+		              st->aa_light_sample_multiplier_factor);
+		return YMX_ERR_UNSUPPORTED;
+	}
+	// the pass sample counts and light multipliers of TiledIntegrator::render,
+	// the most samples a pixel can receive (integrator_tiled.cc:258-262) and
+	// the largest light multiplier (it sizes the shadow slots)
+	float max_light_mult = 1.f;
+	int32_t max_pass_samples = st->aa_samples;
+	int32_t aa_total = st->aa_samples;
+	{
+		float sm = 1.f, lm = 1.f;
+		for(int32_t i = 1; i < aa_passes; ++i)
+		{
+			sm *= st->aa_sample_multiplier_factor;
+			lm *= st->aa_light_sample_multiplier_factor;
+			const int32_t n = (int32_t)ceilf((float)st->aa_inc_samples * sm);
+			max_pass_samples = n > max_pass_samples ? n : max_pass_samples;
+			max_light_mult = lm > max_light_mult ? lm : max_light_mult;
+			aa_total = (int32_t)((float)aa_total + ceilf((float)((double)st->aa_inc_samples *
+			                                                     pow((double)st->aa_sample_multiplier_factor, i))));
+		}
+	}
+	for(uint32_t l = 0; l < n_lights; ++l)
+		if(lights[l].type != YMX_LIGHT_POINT && ceilf((float)lights[l].samples * max_light_mult) > 4096.f)
+		{
+			ymx_set_error("ymx_pt_create: light %u takes more than 4096 samples in an AA pass", l);
+			return YMX_ERR_UNSUPPORTED;
+		}
+
 	ymx_pt *pt = new(std::nothrow) ymx_pt();
 	if(!pt)
 	{
@@ -314,6 +352,9 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 		return YMX_ERR_OUT_OF_MEMORY;
 	}
 	memset(pt, 0, sizeof *pt);
+	pt->settings = *st;
+	pt->settings.aa_passes = aa_passes;
+	pt->max_light_mult = max_light_mult;
 	pt->guide_depth = -1;
 	pt->accel = accel;
 	pt->n_tris = n_tris;
@@ -451,7 +492,7 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 			d.color = cp;
 			d.from = load3(s.from);
 		}
-		const uint32_t k = s.type == YMX_LIGHT_POINT ? 1u : 2u * (uint32_t)s.samples;
+		const uint32_t k = s.type == YMX_LIGHT_POINT ? 1u : 2u * (uint32_t)ceilf((float)s.samples * max_light_mult);
 		slots_camera += k;
 		slots_path = k > slots_path ? k : slots_path;
 	}
@@ -487,6 +528,9 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 	c.bounces = st->bounces;
 	c.rr_min_bounces = st->rr_min_bounces;
 	c.aa_samples = st->aa_samples;
+	c.aa_passes = aa_passes;
+	c.aa_total = (uint32_t)aa_total;
+	c.aa_clamp = st->aa_clamp_samples;
 	c.raydepth = st->raydepth;
 	c.transparent_shadows = st->transparent_shadows ? 1u : 0u;
 	c.shadow_depth = st->transparent_shadows ? (uint32_t)st->shadow_depth : 0u;
@@ -532,7 +576,7 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 	if(max_paths * s.slots > (1ull << 28)) max_paths = (1ull << 28) / s.slots;
 	uint64_t batch = max_paths / npix;
 	if(batch < 1) batch = 1;
-	if(batch > (uint64_t)st->aa_samples) batch = (uint64_t)st->aa_samples;
+	if(batch > (uint64_t)max_pass_samples) batch = (uint64_t)max_pass_samples;
 	pt->batch = (uint32_t)batch;
 	const uint32_t n = (uint32_t)(npix * batch);
 	s.capacity = n;
@@ -547,6 +591,11 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 	PT_TRY(dalloc(pt->d_film_w, npix));
 	PT_TRY(hipMemset(pt->d_film, 0, sizeof(float4) * npix));
 	PT_TRY(hipMemset(pt->d_film_w, 0, sizeof(float) * npix));
+	if(aa_passes > 1)
+	{
+		// flags, pair bits, pixel list, slots; per-block counts and the count word
+		PT_TRY(dalloc(pt->d_aa, 4 * (size_t)npix + (npix + 255) / 256 + 1));
+	}
 	PT_TRY(dalloc(s.path_col, n));
 	PT_TRY(dalloc(s.tp_next, n));
 	PT_TRY(dalloc(s.tp_vertex, n));
@@ -676,6 +725,39 @@ int ymx_pt_set_guide(ymx_pt *pt, ymx_guide *guide, float bsdf_prob, int32_t trai
 	return YMX_OK;
 }
 
+int ymx_pt_set_normals(ymx_pt *pt, const float *corner_normals)
+{
+	if(!pt)
+	{
+		ymx_set_error("ymx_pt_set_normals: NULL handle");
+		return YMX_ERR_INVALID_ARG;
+	}
+	YMX_HIP_CHECK(hipDeviceSynchronize());
+	if(!corner_normals)
+	{
+		if(pt->d_cnorm) YMX_HIP_CHECK(hipFree(pt->d_cnorm));
+		pt->d_cnorm = nullptr;
+		return YMX_OK;
+	}
+	// the transparent-shadow query returns no barycentrics, so a smooth
+	// fake-shadow glass caster's Fresnel term cannot use its shading normal
+	if(pt->host_const.transparent_shadows && pt->d_transp)
+	{
+		std::vector<uint32_t> bits(((size_t)pt->n_tris + 31) / 32 + 1);
+		YMX_HIP_CHECK(hipMemcpy(bits.data(), pt->d_transp, sizeof(uint32_t) * bits.size(), hipMemcpyDeviceToHost));
+		for(uint32_t i = 0; i < pt->n_tris; ++i)
+			if(((bits[i >> 5] >> (i & 31u)) & 1u) && corner_normals[12 * (size_t)i + 3] >= 0.f)
+			{
+				ymx_set_error("ymx_pt_set_normals: triangle %u is smooth fake-shadow glass under transparent shadows", i);
+				return YMX_ERR_UNSUPPORTED;
+			}
+	}
+	if(!pt->d_cnorm) YMX_HIP_CHECK(hipMalloc(&pt->d_cnorm, sizeof(float4) * 3 * (size_t)(pt->n_tris ? pt->n_tris : 1)));
+	YMX_HIP_CHECK(hipMemcpy(pt->d_cnorm, corner_normals, sizeof(float) * 12 * (size_t)pt->n_tris,
+	                        hipMemcpyHostToDevice));
+	return YMX_OK;
+}
+
 int ymx_pt_destroy(ymx_pt *pt)
 {
 	if(!pt) return YMX_OK;
@@ -698,6 +780,39 @@ int ymx_pt_clear_film(ymx_pt *pt, void *stream)
 	return YMX_OK;
 }
 
+// Samples [first, first + n) of n_active pixels (pix_list / pix_slot NULL:
+// every pixel) in wavefronts of as many samples as the buffers hold.
+static int render_range(ymx_pt *pt, int32_t first, int32_t n, const uint32_t *pix_list, const uint32_t *pix_slot,
+                        uint32_t n_active, float light_mult, int32_t pass_samples, hipStream_t s)
+{
+	uint64_t per = pt->st.capacity / n_active;
+	if(per > pt->nd.cap / n_active) per = pt->nd.cap / n_active;
+	if(per < 1) per = 1;
+	for(int32_t k = 0; k < n; k += (int32_t)per)
+	{
+		const int32_t count = (uint64_t)(n - k) < per ? (n - k) : (int32_t)per;
+		const int rc = ymx_pt_trace_subset(pt, first + k, count, pix_list, pix_slot, n_active, light_mult, pass_samples, s);
+		if(rc != YMX_OK) return rc;
+	}
+	return YMX_OK;
+}
+
+static int begin_render(ymx_pt *pt, hipStream_t s)
+{
+	YMX_HIP_CHECK(hipMemsetAsync(pt->st.totals, 0, 3 * sizeof(unsigned long long), s));
+	pt->stats.guide_samples = 0;
+	pt->stats.guide_steps = 0;
+	YMX_HIP_CHECK(hipEventRecord(pt->ev0, s));
+	return YMX_OK;
+}
+
+static int end_render(ymx_pt *pt, hipStream_t s)
+{
+	YMX_HIP_CHECK(hipEventRecord(pt->ev1, s));
+	pt->stats.struct_size = sizeof(ymx_pt_stats);
+	return YMX_OK;
+}
+
 int ymx_pt_render(ymx_pt *pt, int32_t first_sample, int32_t n_samples, void *stream)
 {
 	if(!pt || first_sample < 0 || n_samples < 0)
@@ -706,19 +821,82 @@ int ymx_pt_render(ymx_pt *pt, int32_t first_sample, int32_t n_samples, void *str
 		return YMX_ERR_INVALID_ARG;
 	}
 	hipStream_t s = (hipStream_t)stream;
-	YMX_HIP_CHECK(hipMemsetAsync(pt->st.totals, 0, 3 * sizeof(unsigned long long), s));
-	pt->stats.guide_samples = 0;
-	pt->stats.guide_steps = 0;
-	YMX_HIP_CHECK(hipEventRecord(pt->ev0, s));
+	int rc = begin_render(pt, s);
+	if(rc != YMX_OK) return rc;
+	const uint32_t npix = (uint32_t)(pt->host_const.resx * pt->host_const.resy);
 	for(int32_t k = 0; k < n_samples; k += (int32_t)pt->batch)
 	{
 		const int32_t count = (n_samples - k) < (int32_t)pt->batch ? (n_samples - k) : (int32_t)pt->batch;
-		const int rc = ymx_pt_trace_pass(pt, first_sample + k, count, s);
+		rc = ymx_pt_trace_subset(pt, first_sample + k, count, nullptr, nullptr, npix, 1.f, pt->host_const.aa_samples,
+		                         s);
 		if(rc != YMX_OK) return rc;
 	}
-	YMX_HIP_CHECK(hipEventRecord(pt->ev1, s));
-	pt->stats.struct_size = sizeof(ymx_pt_stats);
-	return YMX_OK;
+	return end_render(pt, s);
+}
+
+int ymx_pt_render_passes(ymx_pt *pt, int32_t *resampled, void *stream)
+{
+	if(!pt)
+	{
+		ymx_set_error("ymx_pt_render_passes: NULL handle");
+		return YMX_ERR_INVALID_ARG;
+	}
+	hipStream_t s = (hipStream_t)stream;
+	const ymx_pt_settings &st = pt->settings;
+	const uint32_t npix = (uint32_t)(pt->host_const.resx * pt->host_const.resy);
+	int rc = begin_render(pt, s);
+	if(rc != YMX_OK) return rc;
+	// TiledIntegrator::render, src/integrator/surface/integrator_tiled.cc:160-214
+	rc = render_range(pt, 0, st.aa_samples, nullptr, nullptr, npix, 1.f, st.aa_samples, s);
+	if(rc != YMX_OK) return rc;
+	if(resampled) resampled[0] = (int32_t)npix;
+	const int32_t floor_pixels = (int32_t)floorf(st.aa_resampled_floor * (float)npix / 100.f);
+	float sample_mult = 1.f, light_mult = 1.f, threshold = st.aa_threshold;
+	int32_t acum = st.aa_samples, res = 0;
+	bool threshold_changed = true;
+	const uint32_t *list = pt->d_aa ? pt->d_aa + 2 * (size_t)npix : nullptr;
+	const uint32_t *slot = pt->d_aa ? pt->d_aa + 3 * (size_t)npix : nullptr;
+	const uint32_t *count = pt->d_aa ? pt->d_aa + 4 * (size_t)npix + (npix + 255) / 256 : nullptr;
+	for(int32_t i = 1; i < st.aa_passes; ++i)
+	{
+		sample_mult *= st.aa_sample_multiplier_factor;
+		light_mult *= st.aa_light_sample_multiplier_factor;
+		bool all = false;
+		if(res <= 0 && !threshold_changed) res = 0;  // ImageFilm::nextPass skipped
+		else if(threshold <= 0.f)
+		{
+			// ImageFilm::doMoreSamples: every pixel
+			res = (int32_t)npix;
+			all = true;
+			threshold_changed = false;
+		}
+		else
+		{
+			rc = ymx_pt_aa_flags(pt, threshold, s);
+			if(rc != YMX_OK) return rc;
+			uint32_t hc = 0;
+			YMX_HIP_CHECK(hipMemcpyAsync(&hc, count, sizeof hc, hipMemcpyDeviceToHost, s));
+			YMX_HIP_CHECK(hipStreamSynchronize(s));
+			res = (int32_t)hc;
+			threshold_changed = false;
+		}
+		const int32_t n = (int32_t)ceilf((float)st.aa_inc_samples * sample_mult);
+		if(res > 0)
+		{
+			rc = all ? render_range(pt, acum, n, nullptr, nullptr, npix, light_mult, n, s)
+			         : render_range(pt, acum, n, list, slot, (uint32_t)res, light_mult, n, s);
+			if(rc != YMX_OK) return rc;
+		}
+		if(resampled) resampled[i] = res;
+		acum += n;
+		if(res < floor_pixels)
+		{
+			const float ratio = fminf(8.f, (float)floor_pixels / (float)res);
+			threshold = threshold * (1.f - 0.1f * ratio);
+			if(st.aa_threshold > 0.f) threshold_changed = true;
+		}
+	}
+	return end_render(pt, s);
 }
 
 int ymx_pt_read_film(ymx_pt *pt, float *rgba, float *weight)

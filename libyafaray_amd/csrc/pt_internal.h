@@ -81,6 +81,9 @@ struct PtConst
 	V3 cam_from, vright, vup, vto;
 	int32_t resx, resy;
 	int32_t path_samples, bounces, rr_min_bounces, aa_samples;
+	int32_t aa_passes;       // "AA_passes"
+	uint32_t aa_total;       // the most AA samples a pixel can receive over all passes
+	float aa_clamp;          // "AA_clamp_samples"
 	int32_t raydepth;
 	uint32_t transparent_shadows;  // "transpShad"
 	uint32_t shadow_depth;         // "shadowDepth"
@@ -175,6 +178,7 @@ struct ymx_pt
 	int device;
 	uint32_t n_tris, n_materials;
 	float4 *d_prim;                  // (face normal xyz, material id bits) per primitive
+	float4 *d_cnorm;                 // corner normals, 3 per triangle (NULL: all flat; ymx_pt_set_normals)
 	ymx::PtMaterial *d_materials;
 	ymx::PtLight *d_lights;
 	ymx::PtLightTri *d_ltris;        // triangles of the object lights
@@ -197,6 +201,9 @@ struct ymx_pt
 	uint32_t g_seen, g_filled;       // training samples written so far (counter value), valid in the ring
 	uint64_t g_rng;                  // mini-batch offsets
 	uint32_t batch;                  // AA samples per wavefront
+	ymx_pt_settings settings;        // as given at create (the AA pass params)
+	float max_light_mult;            // largest light sample multiplier of any AA pass
+	uint32_t *d_aa;                  // AA passes: flags, pair bits, variance hits, pixel list, slots (npix each), count
 	ymx::PtConst host_const;
 	ymx_pt_stats stats;
 	hipEvent_t ev0, ev1;
@@ -205,6 +212,19 @@ struct ymx_pt
 // Kernel launches of one wavefront: AA samples [first_sample, first_sample + count)
 // of every pixel (pt_kernels.hip); count * resx * resy <= st.capacity.
 int ymx_pt_trace_pass(ymx_pt *pt, int32_t first_sample, int32_t count, hipStream_t stream);
+
+// One AA pass's wavefront over a subset of the pixels: pix_list (device, n_active
+// pixel indices) and pix_slot (device, per pixel its position in the list or
+// 0xFFFFFFFF), NULL for every pixel; light_mult the pass's light sample
+// multiplier; pass_samples its sample count (the sub-pixel offsets of a
+// single-pass render depend on it).
+int ymx_pt_trace_subset(ymx_pt *pt, int32_t first_sample, int32_t count, const uint32_t *pix_list,
+                        const uint32_t *pix_slot, uint32_t n_active, float light_mult, int32_t pass_samples,
+                        hipStream_t stream);
+
+// ImageFilm::nextPass on the device: flags the pixels to resample, lists them
+// and leaves their count in d_aa's count word (pt_kernels.hip).
+int ymx_pt_aa_flags(ymx_pt *pt, float threshold, hipStream_t stream);
 
 // Grows the node pool to hold at least `need` nodes, keeping the first `keep`
 // (pt_api.hip).  Synchronous.

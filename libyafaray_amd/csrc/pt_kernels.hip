@@ -47,6 +47,7 @@ struct PtArgs
 	const PtLight *lights;
 	const PtLightTri *ltris;
 	const float4 *prim;
+	const float4 *cnorm;   // corner normals, 3 per triangle (NULL: every triangle flat)
 	const int32_t *faure;
 	const double *inv_prims;
 	const uint32_t *prims;
@@ -56,7 +57,12 @@ struct PtArgs
 	float *film_w;
 	int32_t first_sample;  // AA sample index of the wavefront's first slice
 	uint32_t npix;         // resx * resy
-	uint32_t n_paths;      // npix * count (level-0 nodes)
+	uint32_t n_active;     // pixels of this wavefront's AA pass (npix, or those listed in pix_list)
+	const uint32_t *pix_list;  // the pass's pixels in ascending order (NULL: every pixel)
+	const uint32_t *pix_slot;  // per pixel its index in pix_list, kNoSlot if not in the pass
+	float light_mult;      // the AA pass's light sample multiplier
+	int32_t pass_samples;  // AA samples of the pass (single-pass sub-pixel offsets)
+	uint32_t n_paths;      // n_active * count (level-0 nodes)
 	int32_t count;         // AA samples in this wavefront
 	uint32_t base;         // first node of the chunk: slot i is node base + i
 	uint32_t cnt;          // nodes in the chunk
@@ -66,6 +72,16 @@ struct PtArgs
 	int32_t depth;
 	PtGuide g;             // path guiding (g.on == 0: BSDF sampling only)
 };
+
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+
+// Camera sample `path` is sample first_sample + path / n_active of the pass's
+// (path % n_active)-th pixel.
+__device__ __forceinline__ uint32_t path_pixel(const PtArgs &a, uint32_t path)
+{
+	const uint32_t k = path % a.n_active;
+	return a.pix_list ? a.pix_list[k] : k;
+}
 
 // What a kernel needs to know of the integrate() call behind slot i.
 struct NodeCtx
@@ -87,8 +103,8 @@ __device__ __forceinline__ NodeCtx node_ctx(const PtArgs &a, uint32_t i)
 	x.n = a.base + i;
 	const uint4 info = a.nd.info[x.n];
 	x.path = info.x;
-	x.pixel = info.x % a.npix;
-	x.sample = (uint32_t)a.first_sample + info.x / a.npix;
+	x.pixel = path_pixel(a, info.x);
+	x.sample = (uint32_t)a.first_sample + info.x / a.n_active;
 	const uint32_t px = x.pixel % (uint32_t)a.c->resx, py = x.pixel / (uint32_t)a.c->resx;
 	x.offset = fnv32a(py * fnv32a(px));
 	x.level = info.y & 0xFFu;
@@ -152,6 +168,23 @@ __device__ void make_surf(const PtArgs &a, V3 from, V3 dir, const ymx_hit &h, Su
 	const float4 pi = a.prim[h.prim];
 	s.ng = v3(pi.x, pi.y, pi.z);
 	s.n = s.ng;
+	if(a.cnorm)
+	{
+		// a smooth mesh or one with vertex normals (primitive_polygon.cc:45-51):
+		// corner normals (the face normal for a corner without one,
+		// FacePrimitive::getVertexNormal) weighted by (1 - u - v, u, v)
+		const float4 *cn = a.cnorm + 3 * (size_t)h.prim;
+		const float4 c0 = cn[0];
+		if(c0.w >= 0.f)
+		{
+			const float4 c1 = cn[1], c2 = cn[2];
+			const V3 n0 = c0.w > 0.f ? v3(c0.x, c0.y, c0.z) : s.ng;
+			const V3 n1 = c1.w > 0.f ? v3(c1.x, c1.y, c1.z) : s.ng;
+			const V3 n2 = c2.w > 0.f ? v3(c2.x, c2.y, c2.z) : s.ng;
+			const float bu = 1.f - h.u - h.v, bv = h.u, bw = h.v;
+			s.n = normalize(add(add(mul(bu, n0), mul(bv, n1)), mul(bw, n2)));
+		}
+	}
 	coords_system(s.n, s.u, s.v);
 	s.mat = __float_as_uint(pi.w);
 	s.flags = a.mats[s.mat].flags;
@@ -707,18 +740,18 @@ __device__ float quad_intersect(const V3 q[4], V3 from, V3 dir)
 	return 0.f;
 }
 
-// doLightEstimation's sample count: ceil(samples * 1), divided by the ray
-// division (integrator_montecarlo.cc:427-429)
-__device__ __forceinline__ int light_samples(const PtLight &l, uint32_t division)
+// doLightEstimation's sample count: ceil(samples x the AA pass's light sample
+// multiplier), divided by the ray division (integrator_montecarlo.cc:427-429)
+__device__ __forceinline__ int light_samples(const PtLight &l, uint32_t division, float mult)
 {
-	int num_samples = (int)ceilf((float)l.samples * 1.f);
+	int num_samples = (int)ceilf((float)l.samples * mult);
 	if(division > 1) num_samples = num_samples / (int)division > 1 ? num_samples / (int)division : 1;
 	return num_samples;
 }
 
-__device__ __forceinline__ uint32_t light_slots(const PtLight &l, uint32_t division)
+__device__ __forceinline__ uint32_t light_slots(const PtLight &l, uint32_t division, float mult)
 {
-	return l.type == YMX_LIGHT_POINT ? 1u : 2u * (uint32_t)light_samples(l, division);
+	return l.type == YMX_LIGHT_POINT ? 1u : 2u * (uint32_t)light_samples(l, division, mult);
 }
 
 // Accelerator::isShadowed (include/accelerator/accelerator.h:103-111): origin
@@ -914,7 +947,7 @@ __device__ void queue_light(const PtArgs &a, const Surf &sp, V3 wo, const PtLigh
 		return;
 	}
 	const uint32_t l_offs = loffs * 4567u;
-	const int num_samples = light_samples(l, division);
+	const int num_samples = light_samples(l, division, a.light_mult);
 	const uint32_t offs = (uint32_t)num_samples * psd_sample + psd_offset + l_offs;
 	// areaLightSampleLight (:192-316) with Light::illumSample
 	Halton h2, h3;
@@ -1044,7 +1077,7 @@ __device__ Rgb resolve_light(const PtArgs &a, const PtLight &l, uint32_t divisio
 		if(slot_lit(a, slot0, c)) col = cadd(col, c);
 		return col;
 	}
-	const int num_samples = light_samples(l, division);
+	const int num_samples = light_samples(l, division, a.light_mult);
 	const float inv = 1.f / (float)num_samples;
 	Rgb col = rgb(0.f, 0.f, 0.f);
 	Rgb acc = rgb(0.f, 0.f, 0.f);
@@ -1091,14 +1124,20 @@ __global__ __launch_bounds__(kBlock) void k_camera(PtArgs a)
 	const uint32_t n = a.n_paths;
 	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
 	if(i >= n) return;
-	const uint32_t p = i % a.npix;
-	const uint32_t sample = (uint32_t)a.first_sample + i / a.npix;
+	const uint32_t p = path_pixel(a, i);
+	const uint32_t sample = (uint32_t)a.first_sample + i / a.n_active;
 	const uint32_t x = p % (uint32_t)c.resx, y = p / (uint32_t)c.resx;
 	const uint32_t offset = fnv32a(y * fnv32a(x));
 	float dx = 0.5f, dy = 0.5f;
-	if(c.aa_samples > 1)
+	if(c.aa_passes > 1)
 	{
-		const float d_1 = 1.f / (float)c.aa_samples;
+		// scrambled (0, 2) sequence: the total sample count is not known
+		dx = ri_vdc(sample, offset);
+		dy = ri_s(sample, offset);
+	}
+	else if(a.pass_samples > 1)
+	{
+		const float d_1 = 1.f / (float)a.pass_samples;
 		dx = (0.5f + (float)sample) * d_1;
 		dy = ri_lp(sample + offset, 0);
 	}
@@ -1286,7 +1325,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_node(PtArgs a)
 		for(uint32_t l = 0; l < c.n_lights; ++l)
 		{
 			queue_light(a, sp, wo, a.lights[l], l, x.division, x.sample, x.offset, slot);
-			slot += light_slots(a.lights[l], x.division);
+			slot += light_slots(a.lights[l], x.division, a.light_mult);
 		}
 		f |= kDirect;
 	}
@@ -1318,7 +1357,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve_node(PtArgs a)
 	for(uint32_t l = 0; l < c.n_lights; ++l)
 	{
 		col_all = cadd(col_all, resolve_light(a, a.lights[l], division, slot));
-		slot += light_slots(a.lights[l], division);
+		slot += light_slots(a.lights[l], division, a.light_mult);
 	}
 	const float4 c4 = a.nd.col[n];
 	const Rgb col = cadd(rgb(c4.x, c4.y, c4.z), col_all);
@@ -1459,7 +1498,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_path(PtArgs a)
 		{
 			// estimateOneDirectLight (integrator_montecarlo.cc:108-116) with the
 			// deterministic Halton base-2 light-pick index
-			const uint32_t corr = (x.pixel * (uint32_t)c.aa_samples + x.sample) *
+			const uint32_t corr = (x.pixel * c.aa_total + x.sample) *
 			                          ((uint32_t)c.path_samples * (uint32_t)c.bounces) +
 			                      (uint32_t)a.sub * (uint32_t)c.bounces + (uint32_t)d + 1u + x.tag * 0x9E3779B9u;
 			const float hv = halton_next(2, corr - 1u);
@@ -1733,6 +1772,38 @@ __device__ __forceinline__ int round_bias(double v) { return (int)(v + (.5 - 1.4
 // ImageFilm::addSample (src/render/imagefilm.cc:771-822) as a gather: each
 // film pixel adds the samples whose footprint covers it, visiting the source
 // pixels in row-major order -- the order a row-major scatter adds them.
+// Rgb::clampProportionalRgb (include/color/color.h:399-431): the largest RGB
+// component clamped to max_value, the others scaled by max_value / it; alpha
+// untouched.  max_value 0: no clamp.
+__device__ __forceinline__ float4 clamp_proportional(float4 c, float max_value)
+{
+	if(max_value <= 0.f) return c;
+	const float max_rgb = fmaxf(c.x, fmaxf(c.y, c.z));
+	const float adj = max_value / max_rgb;
+	if(max_rgb > max_value)
+	{
+		if(c.x >= max_rgb)
+		{
+			c.x = max_value;
+			c.y *= adj;
+			c.z *= adj;
+		}
+		else if(c.y >= max_rgb)
+		{
+			c.y = max_value;
+			c.x *= adj;
+			c.z *= adj;
+		}
+		else
+		{
+			c.z = max_value;
+			c.x *= adj;
+			c.y *= adj;
+		}
+	}
+	return c;
+}
+
 __global__ __launch_bounds__(kBlock) void k_film(PtArgs a)
 {
 	const PtConst &c = *a.c;
@@ -1751,7 +1822,10 @@ __global__ __launch_bounds__(kBlock) void k_film(PtArgs a)
 		for(int sx = tx - radius; sx <= tx + radius; ++sx)
 		{
 			if(sx < 0 || sx >= c.resx) continue;
-			const uint32_t s = (uint32_t)j * a.npix + (uint32_t)(sy * c.resx + sx);
+			const uint32_t src = (uint32_t)(sy * c.resx + sx);
+			const uint32_t slot = a.pix_slot ? a.pix_slot[src] : src;
+			if(slot == kNoSlot) continue;  // not rendered in this pass
+			const uint32_t s = (uint32_t)j * a.n_active + slot;
 			const float2 dd = a.st.dxy[s];
 			int dx0 = round_bias((double)dd.x - fw);
 			int dx1 = round_bias((double)dd.x + fw - 1.0);
@@ -1768,7 +1842,7 @@ __global__ __launch_bounds__(kBlock) void k_film(PtArgs a)
 			xi = xi > 15 ? 15 : xi;
 			yi = yi > 15 ? 15 : yi;
 			const float wt = c.filter_table[yi * 16 + xi];
-			const float4 col = a.nd.col[s];  // camera sample s is node s
+			const float4 col = clamp_proportional(a.nd.col[s], c.aa_clamp);  // camera sample s is node s
 			w = w + wt;
 			acc.x += wt * col.x;
 			acc.y += wt * col.y;
@@ -1778,6 +1852,205 @@ __global__ __launch_bounds__(kBlock) void k_film(PtArgs a)
 	}
 	a.film_w[t] = w;
 	a.film[t] = acc;
+}
+
+// ---- ImageFilm::nextPass (src/render/imagefilm.cc:398-513) on the device.
+// d_aa holds, npix words each: flags, pair bits, the pass's pixel list and
+// each pixel's slot in it; then per-block counts / offsets and the count word.
+struct AaArgs
+{
+	const float4 *film;
+	const float *film_w;
+	int32_t w, h;
+	float threshold;          // ImageFilm's aa_threshold_calculated_
+	int32_t dark_type;        // 0 none, 1 linear, 2 curve
+	float dark_factor;
+	int32_t color_noise;      // AA_detect_color_noise
+	int32_t var_half;         // AA_variance_edge_size / 2
+	int32_t var_pixels;       // AA_variance_pixels
+	uint32_t *flags, *bits, *list, *slot, *block_count, *count;
+};
+
+// Rgba::normalized (include/color/color.h:538-542) of film pixel p.
+__device__ __forceinline__ float4 film_pixel(const AaArgs &g, uint32_t p)
+{
+	float w = g.film_w[p];
+	if(w == 0.f) return make_float4(0.f, 0.f, 0.f, 0.f);
+	w = 1.f / w;
+	const float4 c = g.film[p];
+	return make_float4(c.x * w, c.y * w, c.z * w, c.w * w);
+}
+
+// Rgba::colorDifference (include/color/color.h:434-451) with col2Bri (:57)
+__device__ __forceinline__ float color_difference(float4 a, float4 b, int rgb_components)
+{
+	const float bri_a = 0.2126f * a.x + 0.7152f * a.y + 0.0722f * a.z;
+	const float bri_b = 0.2126f * b.x + 0.7152f * b.y + 0.0722f * b.z;
+	float d = fabsf(bri_b - bri_a);
+	if(rgb_components)
+	{
+		const float dr = fabsf(b.x - a.x), dg = fabsf(b.y - a.y), db = fabsf(b.z - a.z), da = fabsf(b.w - a.w);
+		if(d < dr) d = dr;
+		if(d < dg) d = dg;
+		if(d < db) d = db;
+		if(d < da) d = da;
+	}
+	return d;
+}
+
+// ImageFilm::darkThresholdCurveInterpolate (src/render/imagefilm.cc:882-899)
+__device__ __forceinline__ float dark_curve(float b)
+{
+	if(b <= 0.10f) return 0.0001f;
+	else if(b <= 0.20f) return (0.0001f + (b - 0.10f) * (0.0010f - 0.0001f) / 0.10f);
+	else if(b <= 0.30f) return (0.0010f + (b - 0.20f) * (0.0020f - 0.0010f) / 0.10f);
+	else if(b <= 0.40f) return (0.0020f + (b - 0.30f) * (0.0035f - 0.0020f) / 0.10f);
+	else if(b <= 0.50f) return (0.0035f + (b - 0.40f) * (0.0055f - 0.0035f) / 0.10f);
+	else if(b <= 0.60f) return (0.0055f + (b - 0.50f) * (0.0075f - 0.0055f) / 0.10f);
+	else if(b <= 0.70f) return (0.0075f + (b - 0.60f) * (0.0100f - 0.0075f) / 0.10f);
+	else if(b <= 0.80f) return (0.0100f + (b - 0.70f) * (0.0150f - 0.0100f) / 0.10f);
+	else if(b <= 0.90f) return (0.0150f + (b - 0.80f) * (0.0250f - 0.0150f) / 0.10f);
+	else if(b <= 1.00f) return (0.0250f + (b - 0.90f) * (0.0400f - 0.0250f) / 0.10f);
+	else if(b <= 1.20f) return (0.0400f + (b - 1.00f) * (0.0800f - 0.0400f) / 0.20f);
+	else if(b <= 1.40f) return (0.0800f + (b - 1.20f) * (0.0950f - 0.0800f) / 0.20f);
+	else if(b <= 1.80f) return (0.0950f + (b - 1.40f) * (0.1000f - 0.0950f) / 0.40f);
+	return 0.1000f;
+}
+
+// Per source pixel (x, y) of the reference's loop (x < w - 1, y < h - 1):
+// bit k set when its k-th forward pair -- (x+1, y), (x, y+1), (x+1, y+1),
+// (x-1, y+1) -- differs by at least its threshold; bit 4 when its variance
+// window counts AA_variance_pixels such differences.
+__global__ __launch_bounds__(kBlock) void k_aa_pairs(AaArgs g)
+{
+	const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+	if(p >= (uint32_t)(g.w * g.h)) return;
+	const int x = (int)(p % (uint32_t)g.w), y = (int)(p / (uint32_t)g.w);
+	uint32_t b = 0u;
+	if(x < g.w - 1 && y < g.h - 1)
+	{
+		const float4 pix = film_pixel(g, p);
+		float thr = g.threshold;
+		const float bri = 0.2126f * fabsf(pix.x) + 0.7152f * fabsf(pix.y) + 0.0722f * fabsf(pix.z);
+		if(g.dark_type == 1 && g.dark_factor > 0.f) thr = g.threshold * ((1.f - g.dark_factor) + (bri * g.dark_factor));
+		else if(g.dark_type == 2) thr = dark_curve(bri);
+		const int nx[4] = {x + 1, x, x + 1, x - 1}, ny[4] = {y, y + 1, y + 1, y + 1};
+		for(int k = 0; k < 4; ++k)
+		{
+			if(k == 3 && x == 0) continue;
+			const float4 q = film_pixel(g, (uint32_t)(ny[k] * g.w + nx[k]));
+			if(color_difference(pix, q, g.color_noise) >= thr) b |= 1u << k;
+		}
+		if(g.var_pixels > 0)
+		{
+			int vx = 0, vy = 0;
+			for(int xd = -g.var_half; xd < g.var_half - 1; ++xd)
+			{
+				int xi = x + xd;
+				if(xi < 0) xi = 0;
+				else if(xi >= g.w - 1) xi = g.w - 2;
+				const float4 c0 = film_pixel(g, (uint32_t)(y * g.w + xi));
+				const float4 c1 = film_pixel(g, (uint32_t)(y * g.w + xi + 1));
+				if(color_difference(c0, c1, g.color_noise) >= thr) ++vx;
+			}
+			for(int yd = -g.var_half; yd < g.var_half - 1; ++yd)
+			{
+				int yi = y + yd;
+				if(yi < 0) yi = 0;
+				else if(yi >= g.h - 1) yi = g.h - 2;
+				const float4 c0 = film_pixel(g, (uint32_t)(yi * g.w + x));
+				const float4 c1 = film_pixel(g, (uint32_t)((yi + 1) * g.w + x));
+				if(color_difference(c0, c1, g.color_noise) >= thr) ++vy;
+			}
+			if(vx + vy >= g.var_pixels) b |= 16u;
+		}
+	}
+	g.bits[p] = b;
+}
+
+// Pixel (px, py) is flagged when it has no weight yet, when one of the pairs
+// it belongs to differs, or when it lies in the (clamped) variance window
+// [sx - half, sx + half) x [sy - half, sy + half) of a source with bit 4:
+// with clamping, that is sx in [px - half + 1, px + half] and likewise in y.
+// Each block also counts its flagged pixels.
+__global__ __launch_bounds__(kBlock) void k_aa_flags(AaArgs g)
+{
+	const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+	const uint32_t npix = (uint32_t)(g.w * g.h);
+	uint32_t f = 0u;
+	if(p < npix)
+	{
+		const int x = (int)(p % (uint32_t)g.w), y = (int)(p / (uint32_t)g.w);
+		f = g.film_w[p] > 0.f ? 0u : 1u;
+		if(g.bits[p] & 15u) f = 1u;
+		if(x > 0 && (g.bits[p - 1] & 1u)) f = 1u;                               // (x-1, y) -> (x, y)
+		if(y > 0 && (g.bits[p - (uint32_t)g.w] & 2u)) f = 1u;                   // (x, y-1) -> (x, y)
+		if(x > 0 && y > 0 && (g.bits[p - (uint32_t)g.w - 1] & 4u)) f = 1u;      // (x-1, y-1) -> (x, y)
+		if(x + 1 < g.w && y > 0 && (g.bits[p - (uint32_t)g.w + 1] & 8u)) f = 1u;  // (x+1, y-1) -> (x, y)
+		if(g.var_pixels > 0 && g.var_half > 0 && !f)
+		{
+			const int sx0 = max(x - g.var_half + 1, 0), sx1 = min(x + g.var_half, g.w - 2);
+			const int sy0 = max(y - g.var_half + 1, 0), sy1 = min(y + g.var_half, g.h - 2);
+			for(int sy = sy0; sy <= sy1 && !f; ++sy)
+				for(int sx = sx0; sx <= sx1; ++sx)
+					if(g.bits[(uint32_t)(sy * g.w + sx)] & 16u)
+					{
+						f = 1u;
+						break;
+					}
+		}
+		g.flags[p] = f;
+	}
+	const int n = __syncthreads_count((int)f);
+	if(threadIdx.x == 0) g.block_count[blockIdx.x] = (uint32_t)n;
+}
+
+// Exclusive scan of the per-block counts (one block), total into *count.
+__global__ __launch_bounds__(1024) void k_aa_scan(uint32_t *block_count, uint32_t n_blocks, uint32_t *count)
+{
+	__shared__ uint32_t part[1024];
+	const uint32_t per = (n_blocks + 1023u) / 1024u;
+	const uint32_t b0 = threadIdx.x * per;
+	uint32_t sum = 0u;
+	for(uint32_t k = 0; k < per && b0 + k < n_blocks; ++k) sum += block_count[b0 + k];
+	part[threadIdx.x] = sum;
+	__syncthreads();
+	for(uint32_t off = 1; off < 1024u; off <<= 1)
+	{
+		const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+		__syncthreads();
+		part[threadIdx.x] += v;
+		__syncthreads();
+	}
+	uint32_t run = part[threadIdx.x] - sum;  // exclusive
+	for(uint32_t k = 0; k < per && b0 + k < n_blocks; ++k)
+	{
+		const uint32_t c = block_count[b0 + k];
+		block_count[b0 + k] = run;
+		run += c;
+	}
+	if(threadIdx.x == 1023) *count = part[1023];
+}
+
+// The flagged pixels in ascending order, and each pixel's slot.
+__global__ __launch_bounds__(kBlock) void k_aa_list(AaArgs g)
+{
+	__shared__ uint32_t wave_base[kBlock / 64];
+	const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+	const uint32_t npix = (uint32_t)(g.w * g.h);
+	const bool f = p < npix && g.flags[p];
+	const unsigned long long m = __ballot(f);
+	const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+	if(lane == 0) wave_base[wave] = (uint32_t)__popcll(m);
+	__syncthreads();
+	uint32_t base = g.block_count[blockIdx.x];
+	for(uint32_t k = 0; k < wave; ++k) base += wave_base[k];
+	const uint32_t pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+	if(p < npix)
+	{
+		g.slot[p] = f ? pos : kNoSlot;
+		if(f) g.list[pos] = p;
+	}
 }
 
 __global__ void k_count_rays(unsigned long long *total, uint32_t n)
@@ -1872,13 +2145,57 @@ int trace_chunk(ymx_pt *pt, PtArgs &a, hipStream_t stream)
 
 int ymx_pt_trace_pass(ymx_pt *pt, int32_t first_sample, int32_t count, hipStream_t stream)
 {
+	const uint32_t npix = (uint32_t)(pt->host_const.resx * pt->host_const.resy);
+	return ymx_pt_trace_subset(pt, first_sample, count, nullptr, nullptr, npix, 1.f, pt->host_const.aa_samples, stream);
+}
+
+int ymx_pt_aa_flags(ymx_pt *pt, float threshold, hipStream_t stream)
+{
+	using namespace ymx;
+	const PtConst &hc = pt->host_const;
+	const ymx_pt_settings &st = pt->settings;
+	const uint32_t npix = (uint32_t)(hc.resx * hc.resy);
+	const uint32_t n_blocks = blocks_for(npix);
+	AaArgs g;
+	g.film = pt->d_film;
+	g.film_w = pt->d_film_w;
+	g.w = hc.resx;
+	g.h = hc.resy;
+	g.threshold = threshold;
+	g.dark_type = st.aa_dark_detection_type;
+	g.dark_factor = st.aa_dark_threshold_factor;
+	g.color_noise = st.aa_detect_color_noise ? 1 : 0;
+	g.var_half = st.aa_variance_edge_size / 2;
+	g.var_pixels = st.aa_variance_pixels;
+	g.flags = pt->d_aa;
+	g.bits = pt->d_aa + npix;
+	g.list = pt->d_aa + 2 * (size_t)npix;
+	g.slot = pt->d_aa + 3 * (size_t)npix;
+	g.block_count = pt->d_aa + 4 * (size_t)npix;
+	g.count = g.block_count + n_blocks;
+	k_aa_pairs<<<n_blocks, kBlock, 0, stream>>>(g);
+	YMX_HIP_CHECK(hipGetLastError());
+	k_aa_flags<<<n_blocks, kBlock, 0, stream>>>(g);
+	YMX_HIP_CHECK(hipGetLastError());
+	k_aa_scan<<<1, 1024, 0, stream>>>(g.block_count, n_blocks, g.count);
+	YMX_HIP_CHECK(hipGetLastError());
+	k_aa_list<<<n_blocks, kBlock, 0, stream>>>(g);
+	YMX_HIP_CHECK(hipGetLastError());
+	return YMX_OK;
+}
+
+int ymx_pt_trace_subset(ymx_pt *pt, int32_t first_sample, int32_t count, const uint32_t *pix_list,
+                        const uint32_t *pix_slot, uint32_t n_active, float light_mult, int32_t pass_samples,
+                        hipStream_t stream)
+{
 	using namespace ymx;
 	const PtConst &hc = pt->host_const;
 	const uint32_t npix = (uint32_t)(hc.resx * hc.resy);
-	const uint32_t n = npix * (uint32_t)count;
-	if(count < 1 || n > pt->st.capacity || n > pt->nd.cap)
+	const uint64_t n64 = (uint64_t)n_active * (uint64_t)(count > 0 ? count : 0);
+	const uint32_t n = (uint32_t)n64;
+	if(count < 1 || n_active < 1 || n_active > npix || n64 > pt->st.capacity || n > pt->nd.cap)
 	{
-		ymx_set_error("ymx_pt_trace_pass: %d samples exceed the wavefront capacity", count);
+		ymx_set_error("ymx_pt_trace_pass: %d samples of %u pixels exceed the wavefront capacity", count, n_active);
 		return YMX_ERR_INVALID_ARG;
 	}
 	PtArgs a;
@@ -1887,6 +2204,7 @@ int ymx_pt_trace_pass(ymx_pt *pt, int32_t first_sample, int32_t count, hipStream
 	a.lights = pt->d_lights;
 	a.ltris = pt->d_ltris;
 	a.prim = pt->d_prim;
+	a.cnorm = pt->d_cnorm;
 	a.faure = pt->d_faure;
 	a.inv_prims = pt->d_inv_prims;
 	a.prims = pt->d_prims;
@@ -1896,6 +2214,11 @@ int ymx_pt_trace_pass(ymx_pt *pt, int32_t first_sample, int32_t count, hipStream
 	a.film_w = pt->d_film_w;
 	a.first_sample = first_sample;
 	a.npix = npix;
+	a.n_active = n_active;
+	a.pix_list = pix_list;
+	a.pix_slot = pix_slot;
+	a.light_mult = light_mult;
+	a.pass_samples = pass_samples;
 	a.n_paths = n;
 	a.count = count;
 	a.sub = 0;

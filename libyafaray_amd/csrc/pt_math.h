@@ -7,11 +7,16 @@
 // x86 arithmetic bit for bit.
 #pragma once
 
-#include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
 
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
 #define YMX_HD __host__ __device__ __forceinline__
+#else
+// plain C++ (the host layer's mesh smoothing uses the fast cosine)
+#define YMX_HD inline
+#endif
 
 namespace ymx {
 
@@ -151,6 +156,13 @@ YMX_HD float ri_vdc(uint32_t bits, uint32_t r)
 	bits = ((bits & 0x33333333u) << 2) | ((bits & 0xccccccccu) >> 2);
 	bits = ((bits & 0x55555555u) << 1) | ((bits & 0xaaaaaaaau) >> 1);
 	return clamp01((float)((double)(bits ^ r) * 0.00000000023283064365386962890625));
+}
+// Sobol' second dimension (include/sampler/sample.h:110-115)
+YMX_HD float ri_s(uint32_t i, uint32_t r)
+{
+	for(uint32_t v = 1u << 31; i; i >>= 1, v ^= v >> 1)
+		if(i & 1u) r ^= v;
+	return clamp01((float)((double)r * 0.00000000023283064365386962890625));
 }
 YMX_HD float ri_lp(uint32_t i, uint32_t r)
 {

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "../csrc/pt_math.h"
 #include "../csrc/ray_tri.h"
 
 namespace ymx_host {
@@ -337,6 +338,15 @@ size_t Scene::addVertex(size_t object_id, double x, double y, double z)
 	return v.size() / 3 - 1;
 }
 
+bool Scene::addNormal(size_t object_id, double nx, double ny, double nz)
+{
+	if(object_id >= objects_.size()) return false;
+	std::vector<float> &n = objects_[object_id].normals;
+	n.insert(n.end(), {(float)nx, (float)ny, (float)nz});
+	modified_ |= ModifiedObjects;
+	return true;
+}
+
 bool Scene::addTriangle(size_t object_id, size_t a, size_t b, size_t c, size_t material_id)
 {
 	if(object_id >= objects_.size() || material_id >= materials_.size()) return false;
@@ -345,6 +355,168 @@ bool Scene::addTriangle(size_t object_id, size_t a, size_t b, size_t c, size_t m
 	if(a >= nv || b >= nv || c >= nv) return false;
 	o.tris.insert(o.tris.end(), {(uint32_t)a, (uint32_t)b, (uint32_t)c});
 	o.mats.push_back((uint32_t)material_id);
+	// a mesh with vertex normals: the corners take their vertices' normals
+	// (src/geometry/object/object_mesh.cc:110)
+	const bool has_normals = !o.normals.empty();
+	for(size_t v : {a, b, c}) o.corner_normal.push_back(has_normals ? (int32_t)v : -1);
+	modified_ |= ModifiedObjects;
+	return true;
+}
+
+namespace {
+
+using ymx::V3;
+
+V3 load_v3(const std::vector<float> &a, size_t i) { return ymx::v3(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
+
+// Vec::length, sinFromVectors and math::asin (include/geometry/vector.h:63,342-349,
+// include/math/math.h:280-286)
+float length(V3 a) { return std::sqrt(ymx::dot(a, a)); }
+float sin_from_vectors(V3 a, V3 b)
+{
+	const float div = (length(a) * length(b)) * 0.99999f + 0.00001f;
+	float arg = (length(ymx::cross(a, b)) / div) * 0.99999f;
+	if(arg > 1.f) arg = 1.f;
+	if(arg <= -1.f) return -1.5707963267948966f;
+	if(arg >= 1.f) return 1.5707963267948966f;
+	return std::asin(arg);
+}
+
+}  // namespace
+
+void Scene::appendCornerNormals(const Object &o, const float *m, std::vector<float> &out)
+{
+	// PrimitivePolygon::getSurfacePolygon shades a smooth mesh, or one with
+	// vertex normals, with its corners' normals (the face normal for a corner
+	// without one); an instance's normals are transformed by its matrix and
+	// normalised (FacePrimitive::getVertexNormal, include/geometry/matrix.h:122-129)
+	const bool shaded = o.smooth || !o.normals.empty();
+	for(size_t t = 0; t < o.mats.size(); ++t)
+		for(int k = 0; k < 3; ++k)
+		{
+			const int32_t idx = shaded ? o.corner_normal[3 * t + k] : -1;
+			float c[4] = {0.f, 0.f, 0.f, 0.f};
+			if(!shaded && k == 0) c[3] = -1.f;
+			if(idx >= 0 && 3 * (size_t)idx + 2 < o.normals.size())
+			{
+				V3 n = load_v3(o.normals, (size_t)idx);
+				if(m)
+				{
+					float r[3] = {0.f, 0.f, 0.f};
+					for(int i = 0; i < 3; ++i)
+						for(int j = 0; j < 3; ++j) r[i] += m[4 * i + j] * (&n.x)[j];
+					n = ymx::normalize(ymx::v3(r[0], r[1], r[2]));
+				}
+				c[0] = n.x;
+				c[1] = n.y;
+				c[2] = n.z;
+				c[3] = 1.f;
+			}
+			out.insert(out.end(), c, c + 4);
+		}
+}
+
+std::vector<float> Scene::cornerNormals(size_t object_id) const
+{
+	std::vector<float> out;
+	if(object_id < objects_.size()) appendCornerNormals(objects_[object_id], nullptr, out);
+	return out;
+}
+
+bool Scene::smoothMesh(size_t object_id, double angle_d)
+{
+	if(object_id >= objects_.size()) return false;
+	Object &o = objects_[object_id];
+	const float angle = (float)angle_d;
+	const size_t nv = o.verts.size() / 3, nt = o.tris.size() / 3;
+	if(o.normals.size() < 3 * nv) o.normals.resize(3 * nv, 0.f);
+	// MeshObject::getAngleSine of corner k of triangle t, and the face normals
+	// (ShapePolygon::calculateFaceNormal)
+	auto corner_sine = [&](size_t t, int k) {
+		const uint32_t *ix = &o.tris[3 * t];
+		const V3 p0 = load_v3(o.verts, ix[k]), p1 = load_v3(o.verts, ix[(k + 1) % 3]), p2 = load_v3(o.verts, ix[(k + 2) % 3]);
+		return sin_from_vectors(ymx::sub(p1, p0), ymx::sub(p2, p0));
+	};
+	std::vector<V3> face_n(nt);
+	for(size_t t = 0; t < nt; ++t)
+	{
+		const V3 p0 = load_v3(o.verts, o.tris[3 * t]), p1 = load_v3(o.verts, o.tris[3 * t + 1]),
+		         p2 = load_v3(o.verts, o.tris[3 * t + 2]);
+		face_n[t] = ymx::normalize(ymx::cross(ymx::sub(p1, p0), ymx::sub(p2, p0)));
+	}
+	if(angle >= 180.f)
+	{
+		for(size_t t = 0; t < nt; ++t)
+			for(int k = 0; k < 3; ++k)
+			{
+				const uint32_t v = o.tris[3 * t + k];
+				const V3 add = ymx::mul(corner_sine(t, k), face_n[t]);
+				for(int c = 0; c < 3; ++c) o.normals[3 * v + c] += (&add.x)[c];
+				o.corner_normal[3 * t + k] = (int32_t)v;
+			}
+		for(size_t v = 0; v < o.normals.size() / 3; ++v)
+		{
+			const V3 n = ymx::normalize(load_v3(o.normals, v));
+			o.normals[3 * v] = n.x;
+			o.normals[3 * v + 1] = n.y;
+			o.normals[3 * v + 2] = n.z;
+		}
+	}
+	else if(angle > 0.1f)
+	{
+		// math::cos(math::degToRad(angle)) with the reference's fast cosine
+		const float threshold = ymx::fast_cos(angle * 0.017453292519943295f);
+		std::vector<std::vector<std::pair<size_t, int>>> point_faces(nv);  // (triangle, corner)
+		std::vector<std::vector<float>> point_sines(nv);
+		for(size_t t = 0; t < nt; ++t)
+			for(int k = 0; k < 3; ++k)
+			{
+				const uint32_t v = o.tris[3 * t + k];
+				point_sines[v].push_back(corner_sine(t, k));
+				point_faces[v].emplace_back(t, k);
+			}
+		for(size_t v = 0; v < nv; ++v)
+		{
+			std::vector<V3> made;
+			std::vector<int32_t> made_idx;
+			for(size_t j = 0; j < point_faces[v].size(); ++j)
+			{
+				const size_t t = point_faces[v][j].first;
+				bool smooth = false;
+				V3 vn = ymx::mul(point_sines[v][j], face_n[t]);
+				for(size_t k = 0; k < point_faces[v].size(); ++k)
+				{
+					if(k == j) continue;
+					const V3 n2 = face_n[point_faces[v][k].first];
+					if(ymx::dot(face_n[t], n2) > threshold)
+					{
+						smooth = true;
+						vn = ymx::add(vn, ymx::mul(point_sines[v][k], n2));
+					}
+				}
+				int32_t idx = -1;
+				if(smooth)
+				{
+					vn = ymx::normalize(vn);
+					for(size_t m = 0; m < made.size(); ++m)
+						if(ymx::dot(vn, made[m]) > 0.999f)
+						{
+							idx = made_idx[m];
+							break;
+						}
+					if(idx < 0)
+					{
+						idx = (int32_t)(o.normals.size() / 3);
+						made.push_back(vn);
+						made_idx.push_back(idx);
+						o.normals.insert(o.normals.end(), {vn.x, vn.y, vn.z});
+					}
+				}
+				o.corner_normal[3 * t + point_faces[v][j].second] = idx;
+			}
+		}
+	}
+	o.smooth = true;
 	modified_ |= ModifiedObjects;
 	return true;
 }
@@ -477,7 +649,11 @@ bool Scene::preprocess(uint32_t flags, std::string *error)
 	ranges_.clear();
 	// Scene::preprocess skips base objects (src/scene/scene.cc:322-326); they
 	// enter through their instances
+	flat_cnorm_.clear();
+	bool any_smooth = false;
+	for(const Object &o : objects_) any_smooth |= o.smooth || !o.normals.empty();
 	auto append = [&](const Object &o, const float *m) {
+		if(any_smooth) appendCornerNormals(o, m, flat_cnorm_);
 		const uint32_t v0 = (uint32_t)(flat_verts_.size() / 3);
 		if(!m) flat_verts_.insert(flat_verts_.end(), o.verts.begin(), o.verts.end());
 		else
@@ -594,6 +770,20 @@ std::unique_ptr<PathIntegrator> PathIntegrator::create(Scene &scene, const Param
 	s.seed = (uint32_t)integrator.getInt("seed", 0, &ok);
 	s.aa_samples = film.getInt("AA_minsamples", 1, &ok);
 	s.aa_pixelwidth = (float)film.getFloat("AA_pixelwidth", 1.5, &ok);
+	// the AA pass params of ImageFilm (include/render/imagefilm.h:212-226)
+	s.aa_passes = film.getInt("AA_passes", 1, &ok);
+	s.aa_inc_samples = film.getInt("AA_inc_samples", 1, &ok);
+	s.aa_threshold = (float)film.getFloat("AA_threshold", 0.05, &ok);
+	s.aa_resampled_floor = (float)film.getFloat("AA_resampled_floor", 0.0, &ok);
+	s.aa_sample_multiplier_factor = (float)film.getFloat("AA_sample_multiplier_factor", 1.0, &ok);
+	s.aa_light_sample_multiplier_factor = (float)film.getFloat("AA_light_sample_multiplier_factor", 1.0, &ok);
+	s.aa_detect_color_noise = film.getBool("AA_detect_color_noise", false, &ok) ? 1 : 0;
+	const std::string dark = film.getString("AA_dark_detection_type", "none", &ok);
+	s.aa_dark_detection_type = dark == "linear" ? 1 : (dark == "curve" ? 2 : 0);
+	s.aa_dark_threshold_factor = (float)film.getFloat("AA_dark_threshold_factor", 0.0, &ok);
+	s.aa_variance_edge_size = film.getInt("AA_variance_edge_size", 10, &ok);
+	s.aa_variance_pixels = film.getInt("AA_variance_pixels", 0, &ok);
+	s.aa_clamp_samples = (float)film.getFloat("AA_clamp_samples", 0.0, &ok);
 	if(background)
 	{
 		s.has_background = 1;
@@ -603,6 +793,8 @@ std::unique_ptr<PathIntegrator> PathIntegrator::create(Scene &scene, const Param
 	if(ymx_pt_create(pi->accel_->handle(), verts.data(), (uint32_t)(verts.size() / 3), tris.data(), mats.data(),
 	                 vis.data(), (uint32_t)mats.size(), scene.materials_.data(), (uint32_t)scene.materials_.size(),
 	                 lights.data(), (uint32_t)lights.size(), &s, &pi->pt_) != YMX_OK)
+		return fail(ymx_last_error());
+	if(!scene.flat_cnorm_.empty() && ymx_pt_set_normals(pi->pt_, scene.flat_cnorm_.data()) != YMX_OK)
 		return fail(ymx_last_error());
 	return pi;
 }
@@ -616,6 +808,15 @@ int PathIntegrator::render(int first, int n)
 {
 	const int rc = ymx_pt_render(pt_, first, n > 0 ? n : settings_.aa_samples, nullptr);
 	if(rc != YMX_OK) error_ = ymx_last_error();
+	return rc;
+}
+
+int PathIntegrator::renderPasses(std::vector<int> *resampled)
+{
+	std::vector<int32_t> res((size_t)(settings_.aa_passes > 1 ? settings_.aa_passes : 1), 0);
+	const int rc = ymx_pt_render_passes(pt_, res.data(), nullptr);
+	if(rc != YMX_OK) error_ = ymx_last_error();
+	if(resampled) resampled->assign(res.begin(), res.end());
 	return rc;
 }
 

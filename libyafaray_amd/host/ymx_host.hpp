@@ -185,7 +185,22 @@ class Scene
 		bool addInstanceObject(size_t instance_id, size_t base_object_id);
 		bool addInstanceMatrix(size_t instance_id, const double m[16], float time);
 		size_t addVertex(size_t object_id, double x, double y, double z);
+		// yafaray_addNormal (include/public_api/yafaray_c_api.h:217): the normal
+		// of the object's next vertex; triangles added while the object has
+		// normals take them at their corners (FacePrimitive::generateInitialVerticesNormalsIndices).
+		bool addNormal(size_t object_id, double nx, double ny, double nz);
 		bool addTriangle(size_t object_id, size_t a, size_t b, size_t c, size_t material_id);
+		// yafaray_smoothObjectMesh (include/public_api/yafaray_c_api.h:224),
+		// MeshObject::smoothVerticesNormals (src/geometry/object/object_mesh.cc:150-264):
+		// angle >= 180 gives every vertex the sine-weighted mean of its faces'
+		// normals; 0.1 < angle < 180 averages, per face corner, the faces whose
+		// normals lie within `angle` of the face's (the others keep the face
+		// normal at that corner).  The object is then shaded smooth.
+		bool smoothMesh(size_t object_id, double angle);
+		// The object's shading normals at its triangles' corners in the form of
+		// ymx_pt_set_normals (12 floats per triangle; w < 0 in corner 0 for a
+		// flat object, w == 0 for a corner without a normal).
+		std::vector<float> cornerNormals(size_t object_id) const;
 		// Material params (type "shinydiffusemat", "glossy", "mirror", "glass",
 		// "light_mat") and light params (type "arealight", "pointlight",
 		// "objectlight" with "object_name") use the reference's names.
@@ -232,6 +247,9 @@ class Scene
 			bool base = false;
 			std::vector<float> verts;
 			std::vector<uint32_t> tris, mats;  // local vertex indices
+			std::vector<float> normals;        // vertex normals, 3 floats each (may outgrow verts when smoothed)
+			std::vector<int32_t> corner_normal;  // per triangle corner: index into normals, -1 for none
+			bool smooth = false;               // isSmooth(): shaded with interpolated normals
 		};
 		std::vector<Object> objects_;
 		struct Instance
@@ -251,7 +269,10 @@ class Scene
 		std::vector<float> flat_verts_;
 		std::vector<uint32_t> flat_tris_, flat_mats_;
 		std::vector<uint8_t> flat_vis_;
+		std::vector<float> flat_cnorm_;  // 12 floats per triangle (ymx_pt_set_normals), empty if none is smooth
 		std::map<std::string, std::pair<uint32_t, uint32_t>> ranges_;  // object name -> (first triangle, count)
+		// appends the corner normals of o (instanced under the row-major matrix m, or nullptr)
+		static void appendCornerNormals(const Object &o, const float *m, std::vector<float> &out);
 };
 
 class PathIntegrator
@@ -272,9 +293,18 @@ class PathIntegrator
 		PathIntegrator(const PathIntegrator &) = delete;
 		PathIntegrator &operator=(const PathIntegrator &) = delete;
 
-		// Renders AA samples [first, first + n) into the film (n <= 0: all of
-		// "AA_minsamples"); synchronous.
+		// Renders AA samples [first, first + n) of every pixel into the film
+		// (n <= 0: all of "AA_minsamples"); synchronous.
 		int render(int first = 0, int n = 0);
+		// TiledIntegrator::render: pass 0 of "AA_minsamples", then the adaptive
+		// passes of the film params "AA_passes", "AA_inc_samples", "AA_threshold",
+		// "AA_resampled_floor", "AA_sample_multiplier_factor",
+		// "AA_light_sample_multiplier_factor", "AA_detect_color_noise",
+		// "AA_dark_detection_type", "AA_dark_threshold_factor",
+		// "AA_variance_edge_size", "AA_variance_pixels" (and "AA_clamp_samples"
+		// for every sample); resampled: the pixels of each pass.  Returns once
+		// the last pass is queued (film() waits for it).
+		int renderPasses(std::vector<int> *resampled = nullptr);
 		int clearFilm();
 		// The film as ImageFilm holds it: colour and alpha sums weighted by the
 		// filter, and the weight sums; pixel (x, y) at index y * resx + x.

@@ -64,8 +64,10 @@ SIGNATURES = {
                                 C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
                                 C.POINTER(C.c_void_p)]),
     "ymx_pt_destroy": (C.c_int, [C.c_void_p]),
+    "ymx_pt_set_normals": (C.c_int, [C.c_void_p, C.c_void_p]),
     "ymx_pt_clear_film": (C.c_int, [C.c_void_p, C.c_void_p]),
     "ymx_pt_render": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
+    "ymx_pt_render_passes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "ymx_pt_read_film": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "ymx_pt_get_stats": (C.c_int, [C.c_void_p, C.POINTER(YmxPtStats)]),
     "ymx_pt_set_guide": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_int32, C.c_uint32]),

@@ -51,8 +51,28 @@ class YmxPtSettings(C.Structure):
                 ("path_samples", C.c_int32), ("bounces", C.c_int32), ("rr_min_bounces", C.c_int32),
                 ("aa_samples", C.c_int32), ("aa_pixelwidth", C.c_float), ("has_background", C.c_int32),
                 ("background", C.c_float * 3), ("seed", C.c_uint32), ("raydepth", C.c_int32),
-                ("transparent_shadows", C.c_int32), ("shadow_depth", C.c_int32)]
+                ("transparent_shadows", C.c_int32), ("shadow_depth", C.c_int32),
+                ("aa_passes", C.c_int32), ("aa_inc_samples", C.c_int32), ("aa_threshold", C.c_float),
+                ("aa_resampled_floor", C.c_float), ("aa_sample_multiplier_factor", C.c_float),
+                ("aa_light_sample_multiplier_factor", C.c_float), ("aa_detect_color_noise", C.c_int32),
+                ("aa_dark_detection_type", C.c_int32), ("aa_dark_threshold_factor", C.c_float),
+                ("aa_variance_edge_size", C.c_int32), ("aa_variance_pixels", C.c_int32),
+                ("aa_clamp_samples", C.c_float)]
 
+
+# ImageFilm's multi-pass AA params (include/render/imagefilm.h:212-226):
+# the settings key (the reference's param name), the struct field, the default
+AA_PARAMS = [("AA_passes", "aa_passes", 1), ("AA_inc_samples", "aa_inc_samples", 1),
+             ("AA_threshold", "aa_threshold", 0.05), ("AA_resampled_floor", "aa_resampled_floor", 0.0),
+             ("AA_sample_multiplier_factor", "aa_sample_multiplier_factor", 1.0),
+             ("AA_light_sample_multiplier_factor", "aa_light_sample_multiplier_factor", 1.0),
+             ("AA_detect_color_noise", "aa_detect_color_noise", 0),
+             ("AA_dark_detection_type", "aa_dark_detection_type", "none"),
+             ("AA_dark_threshold_factor", "aa_dark_threshold_factor", 0.0),
+             ("AA_variance_edge_size", "aa_variance_edge_size", 10),
+             ("AA_variance_pixels", "aa_variance_pixels", 0), ("AA_clamp_samples", "aa_clamp_samples", 0.0)]
+# "AA_dark_detection_type" values (include/common/aa_noise_params.h:30-37)
+DARK_DETECTION = {"none": 0, "linear": 1, "curve": 2}
 
 YmxPtStats = native.YmxPtStats
 
@@ -125,6 +145,11 @@ def build_structs(scene):
     st.raydepth = int(s.get("raydepth", 5))
     st.transparent_shadows = 1 if s.get("transpShad", False) else 0
     st.shadow_depth = int(s.get("shadowDepth", 4))
+    for key, field, default in AA_PARAMS:
+        v = s.get(key, default)
+        if field == "aa_dark_detection_type":
+            v = DARK_DETECTION[v] if isinstance(v, str) else int(v)
+        setattr(st, field, float(v) if isinstance(default, float) else int(v))
     return mats, lights, st
 
 
@@ -154,6 +179,18 @@ class GpuPathTracer:
                                      len(scene["lights"]), C.addressof(self.st), C.byref(h))
         check(rc, "ymx_pt_create")
         self._h = h
+        cn = scene.get("corner_normals")
+        if cn is not None:
+            cn = np.ascontiguousarray(cn, dtype=np.float32).reshape(-1, 12)
+            if cn.shape[0] != self.tris.shape[0]:
+                raise ValueError("one row of corner normals per triangle is required")
+            self.set_normals(cn)
+
+    def set_normals(self, corner_normals):
+        """Shading normals at the triangles' corners ((n_tris, 12) float32, see
+        ymx_pt_set_normals), or None for flat triangles."""
+        arr = None if corner_normals is None else np.ascontiguousarray(corner_normals, dtype=np.float32)
+        check(self._lib.ymx_pt_set_normals(self._h, None if arr is None else arr.ctypes.data), "ymx_pt_set_normals")
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -204,6 +241,14 @@ class GpuPathTracer:
     def render(self, first_sample: int = 0, n_samples: int | None = None):
         n = self.st.aa_samples if n_samples is None else n_samples
         check(self._lib.ymx_pt_render(self._h, int(first_sample), int(n), self._stream()), "ymx_pt_render")
+
+    def render_passes(self) -> list[int]:
+        """The whole multi-pass AA render (TiledIntegrator::render): returns the
+        pixels resampled in each pass."""
+        passes = max(1, int(self.st.aa_passes))
+        res = (C.c_int32 * passes)()
+        check(self._lib.ymx_pt_render_passes(self._h, res, self._stream()), "ymx_pt_render_passes")
+        return list(res)
 
     def film(self):
         rgba = np.zeros((self.resy, self.resx, 4), dtype=np.float32)

@@ -565,3 +565,49 @@ def terrain_spheres_scene(res=(1920, 1080), spp: int = 64, seed: int = 0, hf_res
                     background=(0.1, 0.1, 0.1), seed=seed, raydepth=depth)
     return dict(verts=verts, tris=tris, mat_id=mat_id, materials=materials, lights=lights, camera=camera,
                 settings=settings)
+
+
+def smooth_spheres_scene(res: int = 64, spp: int = 4, seed: int = 0, subdiv: int = 2, flat_every: int = 3):
+    """Cornell box with icospheres of every material whose shading normals are
+    the sphere's radial vertex normals (as yafaray_addNormal gives them), so a
+    hit's shading normal n is the barycentric blend of its corners' normals
+    while the geometric normal ng stays the facet's.  corner_normals is the
+    (n_tris, 12) array of ymx_pt_set_normals: the walls are flat (w = -1);
+    on every `flat_every`-th sphere one corner of each triangle has no normal
+    (w = 0: the face normal stands in)."""
+    rng = np.random.default_rng(seed)
+    wv, wt, wall = cornell_walls()
+    meshes, normals, mats = [(wv, wt)], [np.full((len(wt), 12), -1.0, F32)], []
+    materials = [dict(color=(0.7, 0.7, 0.7)), dict(color=(0.7, 0.1, 0.1)), dict(color=(0.1, 0.7, 0.1))]
+    kinds = [dict(color=(0.8, 0.8, 0.8)),
+             dict(type="glossy", color=(0.8, 0.7, 0.6), exponent=60.0, diffuse_reflect=0.3,
+                  diffuse_color=(0.6, 0.7, 0.8), glossy_reflect=0.8),
+             dict(type="glossy", color=(0.7, 0.8, 0.6), exponent=20.0, diffuse_reflect=0.2, as_diffuse=False),
+             dict(type="mirror", color=(0.9, 0.9, 0.85), reflect=0.9),
+             dict(type="glass", IOR=1.5, filter_color=(0.95, 0.9, 1.0))]
+    centers = [(0.3, 0.35, 0.18), (0.7, 0.35, 0.2), (0.5, 0.65, 0.22), (0.28, 0.7, 0.5), (0.72, 0.68, 0.55)]
+    mat_id = [np.array([0, 0, 0, 1, 2])[wall]]
+    for k, (c, kind) in enumerate(zip(centers, kinds)):
+        r = 0.15 + 0.03 * rng.uniform()
+        sv, st = icosphere_tris(c, r, subdiv)
+        vn = (sv.astype(np.float64) - np.asarray(c)) / r
+        vn = (vn / np.linalg.norm(vn, axis=1, keepdims=True)).astype(F32)
+        cn = np.zeros((len(st), 3, 4), F32)
+        cn[:, :, :3] = vn[st]
+        cn[:, :, 3] = 1.0
+        if flat_every and k % flat_every == 0:
+            cn[:, 1, :] = 0.0
+        meshes.append((sv, st))
+        normals.append(cn.reshape(-1, 12))
+        materials.append(kind)
+        mat_id.append(np.full(len(st), len(materials) - 1))
+    verts, tris = merge(meshes)
+    lights = [dict(type="area", corner=(0.4, 0.4, 0.999), point1=(0.4, 0.6, 0.999), point2=(0.6, 0.4, 0.999),
+                   color=(1.0, 1.0, 1.0), power=10.0 / np.pi, samples=4),
+              dict(type="point", **{"from": (0.5, 0.2, 0.8)}, color=(1.0, 0.95, 0.9), power=0.05)]
+    camera = dict(from_=(0.5, -1.6, 0.5), to=(0.5, 0.5, 0.5), up=(0.5, -1.6, 1.5),
+                  focal=float(1.0 / (2.0 * np.tan(np.radians(20.0)))), resx=res, resy=res)
+    settings = dict(path_samples=1, bounces=5, rr_min_bounces=2, aa_samples=spp, aa_pixelwidth=1.5,
+                    background=(0.1, 0.1, 0.1), seed=0, raydepth=5)
+    return dict(verts=verts, tris=tris, mat_id=np.concatenate(mat_id).astype(np.uint32), materials=materials,
+                lights=lights, camera=camera, settings=settings, corner_normals=np.concatenate(normals))

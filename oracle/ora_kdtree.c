@@ -664,3 +664,5 @@ void ora_kd_shadow(const ora_kdtree *kd, const ora_ray *rays, uint32_t n_rays, u
 	j.kd = kd; j.rays = rays; j.n_rays = n_rays; j.occ = occluded; j.occ_prim = prim; j.test = TEST_SHADOW;
 	kd_run(&j, threads);
 }
+
+uint32_t ora_kd_size(const ora_kdtree *kd) { return kd ? kd->n_tris : 0u; }

@@ -277,7 +277,9 @@ typedef struct
 	const ora_pt_settings *st;
 	/* camera (src/camera/camera.cc:89-110, camera_perspective.cc:97-111) */
 	v3 cam_from, vright, vup, vto;
-	uint32_t total_samples;
+	uint32_t total_samples;   /* the most AA samples a pixel can receive (all passes) */
+	float light_mult;         /* the AA pass's light sample multiplier */
+	int pass_samples;         /* AA samples of the pass being rendered */
 } pt_ctx;
 
 enum
@@ -343,7 +345,13 @@ static int intersect_surface(const pt_ctx *c, v3 from, v3 dir, float tmin, float
 	ora_ray r = {{from.x, from.y, from.z}, tmin, {dir.x, dir.y, dir.z}, tmax};
 	const float t_max = (tmax >= 0.f) ? tmax : FLT_MAX;
 	ora_hit h;
-	if(!ora_kd_intersect1(c->kd, &r, t_max, 0, &h) || h.prim == ORA_NO_HIT) return 0;
+	if(c->st->trace_brute)
+	{
+		/* every triangle in id order: exact ties in t keep the lowest id */
+		ora_closest_brute(c->verts, c->idx, c->vis, ora_kd_size(c->kd), &r, 1, &h, 1);
+		if(h.prim == ORA_NO_HIT) return 0;
+	}
+	else if(!ora_kd_intersect1(c->kd, &r, t_max, 0, &h) || h.prim == ORA_NO_HIT) return 0;
 	sp->p = vadd(from, vmul(h.t, dir));
 	const uint32_t *ix = c->idx + 3 * (size_t)h.prim;
 	const v3 p0 = vload(c->verts + 3 * (size_t)ix[0]);
@@ -351,6 +359,18 @@ static int intersect_surface(const pt_ctx *c, v3 from, v3 dir, float tmin, float
 	const v3 p2 = vload(c->verts + 3 * (size_t)ix[2]);
 	sp->ng = vnorm(vcross(vsub(p1, p0), vsub(p2, p0)));   /* shape_polygon.h:58-62 */
 	sp->n = sp->ng;
+	const float *cn = c->st->corner_normals ? c->st->corner_normals + 12 * (size_t)h.prim : NULL;
+	if(cn && cn[3] >= 0.f)
+	{
+		/* a smooth mesh or one with vertex normals (primitive_polygon.cc:45-51):
+		 * the corners' normals (the face normal where a corner has none,
+		 * FacePrimitive::getVertexNormal) weighted by the barycentric
+		 * coordinates (1 - u - v, u, v) of the hit, then normalised */
+		v3 vn[3];
+		for(int k = 0; k < 3; ++k) vn[k] = cn[4 * k + 3] > 0.f ? vload(cn + 4 * k) : sp->ng;
+		const float bu = 1.f - h.u - h.v, bv = h.u, bw = h.v;
+		sp->n = vnorm(vadd(vadd(vmul(bu, vn[0]), vmul(bv, vn[1])), vmul(bw, vn[2])));
+	}
 	coords_system(sp->n, &sp->u, &sp->v);
 	sp->prim = h.prim;
 	sp->mat = &c->mats[c->mat_id[h.prim]];
@@ -1187,7 +1207,7 @@ static rgb light_estimation(const pt_ctx *c, const ora_light *l, const surf *sp,
 		return col;
 	}
 	const uint32_t l_offs = loffs * 4567u;
-	int num_samples = (int)ceilf((float)l->samples * 1.f);
+	int num_samples = (int)ceilf((float)l->samples * c->light_mult);
 	if(rd->division > 1) num_samples = num_samples / rd->division > 1 ? num_samples / rd->division : 1;
 	const float inv_num_samples = 1.f / (float)num_samples;
 	const uint32_t offs = (uint32_t)num_samples * psd_sample + psd_offset + l_offs;
@@ -1510,7 +1530,9 @@ static int setup_ctx(pt_ctx *c, const ora_kdtree *kd, const float *verts, const 
 			return 0;
 		}
 	}
-	c->total_samples = (uint32_t)st->aa_samples;
+	c->total_samples = (uint32_t)ora_aa_max_samples(st);
+	c->light_mult = 1.f;
+	c->pass_samples = st->aa_samples;
 	/* Camera::Camera, src/camera/camera.cc:89-104 */
 	const v3 from = vload(st->from);
 	v3 cy = vsub(vload(st->up), from);
@@ -1531,7 +1553,8 @@ static int setup_ctx(pt_ctx *c, const ora_kdtree *kd, const float *verts, const 
 	return 1;
 }
 
-/* TiledIntegrator::renderTile per-sample set-up, integrator_tiled.cc:290-335 (one pass) */
+/* TiledIntegrator::renderTile per-sample set-up, integrator_tiled.cc:290-335:
+ * sample is the pass offset plus the sample's index in the pass. */
 static void pixel_sample(const pt_ctx *c, int x, int y, int sample, float col[4], float *dx_out, float *dy_out)
 {
 	const ora_pt_settings *st = c->st;
@@ -1539,9 +1562,15 @@ static void pixel_sample(const pt_ctx *c, int x, int y, int sample, float col[4]
 	const uint32_t offset = ora_fnv32a((uint32_t)y * ora_fnv32a((uint32_t)x));
 	const uint32_t psd_sample = (uint32_t)sample;
 	float dx = 0.5f, dy = 0.5f;
-	if(st->aa_samples > 1)
+	if(st->aa_passes > 1)
 	{
-		const float d_1 = 1.f / (float)st->aa_samples;
+		/* scrambled (0, 2) sequence, as the total sample count is not known */
+		dx = ora_ri_vdc(psd_sample, offset);
+		dy = ora_ri_s(psd_sample, offset);
+	}
+	else if(c->pass_samples > 1)
+	{
+		const float d_1 = 1.f / (float)c->pass_samples;
 		dx = (0.5f + (float)sample) * d_1;
 		dy = ora_ri_lp((uint32_t)sample + offset, 0);
 	}
@@ -1589,6 +1618,7 @@ typedef struct
 {
 	const pt_ctx *c;
 	int sample;
+	const uint8_t *flags;  /* pixels of an adaptive pass (NULL: all) */
 	float *cols;   /* resx * resy * 4 */
 	float *dxy;    /* resx * resy * 2 */
 	int tid, nthreads;
@@ -1602,9 +1632,39 @@ static void *pass_worker(void *arg)
 		for(int x = 0; x < st->resx; ++x)
 		{
 			const size_t k = (size_t)y * st->resx + x;
+			if(j->flags && !j->flags[k]) continue;
 			pixel_sample(j->c, x, y, j->sample, j->cols + 4 * k, &j->dxy[2 * k], &j->dxy[2 * k + 1]);
 		}
 	return NULL;
+}
+
+/* Rgb::clampProportionalRgb, include/color/color.h:399-431 (alpha untouched) */
+static void clamp_proportional(float col[4], float max_value)
+{
+	if(max_value <= 0.f) return;
+	const float max_rgb = fmaxf(col[0], fmaxf(col[1], col[2]));
+	const float adj = max_value / max_rgb;
+	if(max_rgb > max_value)
+	{
+		if(col[0] >= max_rgb)
+		{
+			col[0] = max_value;
+			col[1] *= adj;
+			col[2] *= adj;
+		}
+		else if(col[1] >= max_rgb)
+		{
+			col[1] = max_value;
+			col[0] *= adj;
+			col[2] *= adj;
+		}
+		else
+		{
+			col[2] = max_value;
+			col[0] *= adj;
+			col[1] *= adj;
+		}
+	}
 }
 
 /* roundToIntWithBias, include/render/imagefilm.h:344-348 */
@@ -1612,8 +1672,10 @@ static inline int round_bias(double v) { return (int)(v + (.5 - 1.4e-11)); }
 
 /* ImageFilm::addSample, src/render/imagefilm.cc:771-822 */
 static void add_sample(int resx, int resy, const float table[256], float table_scale, float filter_width, int x,
-                       int y, float dx, float dy, const float col[4], float *film_rgba, float *film_w)
+                       int y, float dx, float dy, const float col_in[4], float clamp, float *film_rgba, float *film_w)
 {
+	float col[4] = {col_in[0], col_in[1], col_in[2], col_in[3]};
+	clamp_proportional(col, clamp);
 	int dx0 = round_bias((double)dx - filter_width);
 	int dx1 = round_bias((double)dx + filter_width - 1.0);
 	int dy0 = round_bias((double)dy - filter_width);
@@ -1637,27 +1699,20 @@ static void add_sample(int resx, int resy, const float table[256], float table_s
 		}
 }
 
-void ora_pt_render(const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx, const uint32_t *mat_id,
-                   const uint8_t *vis, const ora_material *mats, const ora_light *lights, uint32_t n_lights,
-                   const ora_pt_settings *st, float *film_rgba, float *film_w, int threads)
+/* renderPass of n samples from sample `offset` over the flagged pixels (all
+ * when flags is NULL), splatted sample by sample, pixels in row-major order. */
+static void render_pass(pt_ctx *c, int offset, int n, const uint8_t *flags, const float table[256], float table_scale,
+                        float filter_width, float *cols, float *dxy, float *film_rgba, float *film_w, int threads)
 {
-	pt_ctx c;
-	if(!setup_ctx(&c, kd, verts, tri_idx, mat_id, vis, mats, lights, n_lights, st)) return;
-	float table[256], table_scale, filter_width;
-	ora_filter_table(st->aa_pixelwidth, table, &table_scale, &filter_width);
-	const size_t npix = (size_t)st->resx * st->resy;
-	float *cols = (float *)malloc(npix * 4 * sizeof(float));
-	float *dxy = (float *)malloc(npix * 2 * sizeof(float));
-	if(!cols || !dxy) { free(cols); free(dxy); free_ctx(&c); return; }
-	if(threads < 1) threads = 1;
-	if(threads > 256) threads = 256;
+	const ora_pt_settings *st = c->st;
 	pass_job jobs[256];
 	pthread_t th[256];
-	for(int s = 0; s < st->aa_samples; ++s)
+	c->pass_samples = n;
+	for(int s = 0; s < n; ++s)
 	{
 		for(int t = 0; t < threads; ++t)
 		{
-			jobs[t].c = &c; jobs[t].sample = s; jobs[t].cols = cols; jobs[t].dxy = dxy;
+			jobs[t].c = c; jobs[t].sample = offset + s; jobs[t].flags = flags; jobs[t].cols = cols; jobs[t].dxy = dxy;
 			jobs[t].tid = t; jobs[t].nthreads = threads;
 		}
 		if(threads == 1) pass_worker(&jobs[0]);
@@ -1670,12 +1725,190 @@ void ora_pt_render(const ora_kdtree *kd, const float *verts, const uint32_t *tri
 			for(int x = 0; x < st->resx; ++x)
 			{
 				const size_t k = (size_t)y * st->resx + x;
+				if(flags && !flags[k]) continue;
 				add_sample(st->resx, st->resy, table, table_scale, filter_width, x, y, dxy[2 * k], dxy[2 * k + 1],
-				           cols + 4 * k, film_rgba, film_w);
+				           cols + 4 * k, st->aa_clamp_samples, film_rgba, film_w);
 			}
+	}
+}
+
+int ora_aa_max_samples(const ora_pt_settings *st)
+{
+	/* integrator_tiled.cc:258-262: int += ceilf(int * pow(float, int)) */
+	int m = st->aa_samples;
+	for(int i = 1; i < st->aa_passes; ++i)
+		m = (int)((float)m + ceilf((float)((double)st->aa_inc_samples * pow((double)st->aa_sample_multiplier_factor, i))));
+	return m;
+}
+
+/* Rgba::col2Bri / abscol2Bri / colorDifference, include/color/color.h:57-58,434-451 */
+static inline float col2bri(const float c[4]) { return 0.2126f * c[0] + 0.7152f * c[1] + 0.0722f * c[2]; }
+static inline float abscol2bri(const float c[4])
+{
+	return 0.2126f * fabsf(c[0]) + 0.7152f * fabsf(c[1]) + 0.0722f * fabsf(c[2]);
+}
+static float color_difference(const float a[4], const float b[4], int rgb_components)
+{
+	float d = fabsf(col2bri(b) - col2bri(a));
+	if(rgb_components)
+		for(int k = 0; k < 4; ++k)
+		{
+			const float dk = fabsf(b[k] - a[k]);
+			if(d < dk) d = dk;
+		}
+	return d;
+}
+
+/* Rgba::normalized (include/color/color.h:538-542) of a film pixel: the sums
+ * times 1 / weight, zero for a zero weight. */
+static void film_pixel(const float *film_rgba, const float *film_w, int resx, int x, int y, float out[4])
+{
+	const size_t p = (size_t)y * resx + x;
+	float w = film_w[p];
+	if(w != 0.f)
+	{
+		w = 1.f / w;
+		for(int k = 0; k < 4; ++k) out[k] = film_rgba[4 * p + k] * w;
+	}
+	else out[0] = out[1] = out[2] = out[3] = 0.f;
+}
+
+/* ImageFilm::darkThresholdCurveInterpolate, src/render/imagefilm.cc:882-899 */
+static float dark_curve(float b)
+{
+	if(b <= 0.10f) return 0.0001f;
+	else if(b <= 0.20f) return (0.0001f + (b - 0.10f) * (0.0010f - 0.0001f) / 0.10f);
+	else if(b <= 0.30f) return (0.0010f + (b - 0.20f) * (0.0020f - 0.0010f) / 0.10f);
+	else if(b <= 0.40f) return (0.0020f + (b - 0.30f) * (0.0035f - 0.0020f) / 0.10f);
+	else if(b <= 0.50f) return (0.0035f + (b - 0.40f) * (0.0055f - 0.0035f) / 0.10f);
+	else if(b <= 0.60f) return (0.0055f + (b - 0.50f) * (0.0075f - 0.0055f) / 0.10f);
+	else if(b <= 0.70f) return (0.0075f + (b - 0.60f) * (0.0100f - 0.0075f) / 0.10f);
+	else if(b <= 0.80f) return (0.0100f + (b - 0.70f) * (0.0150f - 0.0100f) / 0.10f);
+	else if(b <= 0.90f) return (0.0150f + (b - 0.80f) * (0.0250f - 0.0150f) / 0.10f);
+	else if(b <= 1.00f) return (0.0250f + (b - 0.90f) * (0.0400f - 0.0250f) / 0.10f);
+	else if(b <= 1.20f) return (0.0400f + (b - 1.00f) * (0.0800f - 0.0400f) / 0.20f);
+	else if(b <= 1.40f) return (0.0800f + (b - 1.20f) * (0.0950f - 0.0800f) / 0.20f);
+	else if(b <= 1.80f) return (0.0950f + (b - 1.40f) * (0.1000f - 0.0950f) / 0.40f);
+	return 0.1000f;
+}
+
+int ora_aa_next_pass(const ora_pt_settings *st, float threshold, const float *film_rgba, const float *film_w,
+                     uint8_t *flags)
+{
+	const int w = st->resx, h = st->resy;
+	if(threshold <= 0.f)
+	{
+		memset(flags, 1, (size_t)w * h);
+		return w * h;
+	}
+	for(size_t p = 0; p < (size_t)w * h; ++p) flags[p] = film_w[p] > 0.f ? 0 : 1;
+	const int half = st->aa_variance_edge_size / 2;
+	float thr = threshold;
+	for(int y = 0; y < h - 1; ++y)
+		for(int x = 0; x < w - 1; ++x)
+		{
+			float pix[4], q[4];
+			film_pixel(film_rgba, film_w, w, x, y, pix);
+			const float bri = abscol2bri(pix);
+			if(st->aa_dark_detection_type == 1 && st->aa_dark_threshold_factor > 0.f)
+				thr = threshold * ((1.f - st->aa_dark_threshold_factor) + (bri * st->aa_dark_threshold_factor));
+			else if(st->aa_dark_detection_type == 2) thr = dark_curve(bri);
+			const int nb[4][2] = {{x + 1, y}, {x, y + 1}, {x + 1, y + 1}, {x - 1, y + 1}};
+			for(int k = 0; k < 4; ++k)
+			{
+				if(k == 3 && x == 0) continue;
+				film_pixel(film_rgba, film_w, w, nb[k][0], nb[k][1], q);
+				if(color_difference(pix, q, st->aa_detect_color_noise) >= thr)
+				{
+					flags[(size_t)y * w + x] = 1;
+					flags[(size_t)nb[k][1] * w + nb[k][0]] = 1;
+				}
+			}
+			if(st->aa_variance_pixels > 0)
+			{
+				int var_x = 0, var_y = 0;
+				for(int xd = -half; xd < half - 1; ++xd)
+				{
+					int xi = x + xd;
+					if(xi < 0) xi = 0;
+					else if(xi >= w - 1) xi = w - 2;
+					film_pixel(film_rgba, film_w, w, xi, y, pix);
+					film_pixel(film_rgba, film_w, w, xi + 1, y, q);
+					if(color_difference(pix, q, st->aa_detect_color_noise) >= thr) ++var_x;
+				}
+				for(int yd = -half; yd < half - 1; ++yd)
+				{
+					int yi = y + yd;
+					if(yi < 0) yi = 0;
+					else if(yi >= h - 1) yi = h - 2;
+					film_pixel(film_rgba, film_w, w, x, yi, pix);
+					film_pixel(film_rgba, film_w, w, x, yi + 1, q);
+					if(color_difference(pix, q, st->aa_detect_color_noise) >= thr) ++var_y;
+				}
+				if(var_x + var_y >= st->aa_variance_pixels)
+					for(int xd = -half; xd < half; ++xd)
+						for(int yd = -half; yd < half; ++yd)
+						{
+							int xi = x + xd, yi = y + yd;
+							xi = xi < 0 ? 0 : (xi >= w ? w - 1 : xi);
+							yi = yi < 0 ? 0 : (yi >= h ? h - 1 : yi);
+							flags[(size_t)yi * w + xi] = 1;
+						}
+			}
+		}
+	int n = 0;
+	for(size_t p = 0; p < (size_t)w * h; ++p) n += flags[p] ? 1 : 0;
+	return n;
+}
+
+void ora_pt_render(const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx, const uint32_t *mat_id,
+                   const uint8_t *vis, const ora_material *mats, const ora_light *lights, uint32_t n_lights,
+                   const ora_pt_settings *st, float *film_rgba, float *film_w, int threads)
+{
+	pt_ctx c;
+	if(!setup_ctx(&c, kd, verts, tri_idx, mat_id, vis, mats, lights, n_lights, st)) return;
+	float table[256], table_scale, filter_width;
+	ora_filter_table(st->aa_pixelwidth, table, &table_scale, &filter_width);
+	const size_t npix = (size_t)st->resx * st->resy;
+	float *cols = (float *)malloc(npix * 4 * sizeof(float));
+	float *dxy = (float *)malloc(npix * 2 * sizeof(float));
+	uint8_t *flags = (uint8_t *)malloc(npix);
+	if(!cols || !dxy || !flags) { free(cols); free(dxy); free(flags); free_ctx(&c); return; }
+	if(threads < 1) threads = 1;
+	if(threads > 256) threads = 256;
+	/* TiledIntegrator::render, integrator_tiled.cc:160-214 (not resumed) */
+	c.light_mult = 1.f;
+	render_pass(&c, 0, st->aa_samples, NULL, table, table_scale, filter_width, cols, dxy, film_rgba, film_w, threads);
+	const int floor_pixels = (int)floorf(st->aa_resampled_floor * (float)npix / 100.f);
+	float sample_mult = 1.f, light_mult = 1.f, threshold = st->aa_threshold;
+	int acum = st->aa_samples, resampled = 0, threshold_changed = 1;
+	for(int i = 1; i < st->aa_passes; ++i)
+	{
+		sample_mult *= st->aa_sample_multiplier_factor;
+		light_mult *= st->aa_light_sample_multiplier_factor;
+		if(resampled <= 0 && !threshold_changed) resampled = 0;  /* nextPass skipped: nothing rendered */
+		else
+		{
+			resampled = ora_aa_next_pass(st, threshold, film_rgba, film_w, flags);
+			threshold_changed = 0;
+		}
+		const int n = (int)ceilf((float)st->aa_inc_samples * sample_mult);
+		if(resampled > 0)
+		{
+			c.light_mult = light_mult;
+			render_pass(&c, acum, n, flags, table, table_scale, filter_width, cols, dxy, film_rgba, film_w, threads);
+		}
+		acum += n;
+		if(resampled < floor_pixels)
+		{
+			const float ratio = fminf(8.f, (float)floor_pixels / (float)resampled);
+			threshold = threshold * (1.f - 0.1f * ratio);
+			if(st->aa_threshold > 0.f) threshold_changed = 1;
+		}
 	}
 	free(cols);
 	free(dxy);
+	free(flags);
 	free_ctx(&c);
 }
 

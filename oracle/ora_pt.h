@@ -113,11 +113,40 @@ typedef struct
 	int32_t raydepth;        /* MonteCarloIntegrator "raydepth" (5) */
 	int32_t transparent_shadows;  /* "transpShad" (false) */
 	int32_t shadow_depth;    /* "shadowDepth" (4), at most ORA_MAX_SHADOW_DEPTH */
+	/* multi-pass adaptive anti-aliasing, ImageFilm's AA params
+	 * (include/render/imagefilm.h:212-226; 0 for aa_passes means 1) */
+	int32_t aa_passes;                        /* "AA_passes" (1) */
+	int32_t aa_inc_samples;                   /* "AA_inc_samples" (1) */
+	float aa_threshold;                       /* "AA_threshold" (0.05) */
+	float aa_resampled_floor;                 /* "AA_resampled_floor" (0, % of the pixels) */
+	float aa_sample_multiplier_factor;        /* "AA_sample_multiplier_factor" (1) */
+	float aa_light_sample_multiplier_factor;  /* "AA_light_sample_multiplier_factor" (1) */
+	int32_t aa_detect_color_noise;            /* "AA_detect_color_noise" (false) */
+	int32_t aa_dark_detection_type;           /* "AA_dark_detection_type": 0 none, 1 linear, 2 curve */
+	float aa_dark_threshold_factor;           /* "AA_dark_threshold_factor" (0) */
+	int32_t aa_variance_edge_size;            /* "AA_variance_edge_size" (10) */
+	int32_t aa_variance_pixels;               /* "AA_variance_pixels" (0) */
+	float aa_clamp_samples;                   /* "AA_clamp_samples" (0: no clamp) */
+	/* vertex normals at the triangles' corners, 12 floats per triangle (NULL:
+	 * every triangle flat): corner k is (nx, ny, nz, w) at 4 k; w < 0 in
+	 * corner 0 marks a flat triangle (n = ng), w == 0 a corner without a
+	 * normal (the face normal stands in), w > 0 a corner normal */
+	const float *corner_normals;
+	/* closest hits by brute force over every triangle (exact ties in t to the
+	 * lowest id, as the GPU accelerator and ora_closest_brute resolve them)
+	 * instead of the kd-tree port, whose leaf order decides ties and whose
+	 * unclipped leaves can miss a ray grazing an edge; for scenes small
+	 * enough to test every triangle per ray */
+	int32_t trace_brute;
 } ora_pt_settings;
 
 #define ORA_MAX_SHADOW_DEPTH 64
 
-/* Render aa_samples samples per pixel into film sums: film_rgba[4 * (y * resx + x)]
+/* TiledIntegrator::render (src/integrator/surface/integrator_tiled.cc:98-216):
+ * pass 0 renders aa_samples samples of every pixel; each further pass of
+ * aa_passes renders ceil(aa_inc_samples * multiplier) more samples of the
+ * pixels ImageFilm::nextPass flags as noisy, with the light sample multiplier
+ * of that pass.  Film sums: film_rgba[4 * (y * resx + x)]
  * (colour * weight, alpha * weight) and film_w[y * resx + x] (weights).
  * Buffers are accumulated into (zero them first).  vis: the visibility mask of
  * every triangle (the one kd was built with; object lights read it), NULL: all
@@ -131,6 +160,16 @@ void ora_pt_render(const ora_kdtree *kd, const float *verts, const uint32_t *tri
 void ora_pt_pixel_sample(const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx, const uint32_t *mat_id,
                          const uint8_t *vis, const ora_material *mats, const ora_light *lights, uint32_t n_lights,
                          const ora_pt_settings *st, int x, int y, int sample, float col_alpha[4], float dxdy[2]);
+
+/* ImageFilm::nextPass's resampling flags (src/render/imagefilm.cc:398-513)
+ * for the film sums given and the current AA threshold: flags[y * resx + x]
+ * set for the pixels the next pass renders; returns their count.  A
+ * threshold <= 0 flags every pixel (ImageFilm::doMoreSamples). */
+int ora_aa_next_pass(const ora_pt_settings *st, float threshold, const float *film_rgba, const float *film_w,
+                     uint8_t *flags);
+
+/* The most AA samples a pixel can receive over all passes (integrator_tiled.cc:258-262). */
+int ora_aa_max_samples(const ora_pt_settings *st);
 
 /* PerspectiveCamera::shootRay direction for film coordinates (px, py). */
 void ora_pt_camera_dir(const ora_pt_settings *st, float px, float py, float dir[3]);

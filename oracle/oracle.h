@@ -126,6 +126,8 @@ uint32_t ora_kd_num_prims(const ora_kdtree *kd);
 /* One ray through the kd-tree port: shadow = 0 closest (t_max rule of
  * Accelerator::intersect applied by the caller), 1 any hit.  Returns 1 on hit. */
 int ora_kd_intersect1(const ora_kdtree *kd, const ora_ray *ray, float t_max, int shadow, ora_hit *hit);
+/* Triangles the tree was built over. */
+uint32_t ora_kd_size(const ora_kdtree *kd);
 
 /* accelerator_kdtree_common.h:107-255, TransparentShadow, with
  * Accelerator::primitiveIntersectionTransparentShadow (accelerator.h:147-168):

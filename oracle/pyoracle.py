@@ -288,7 +288,37 @@ class OraPtSettings(C.Structure):
                 ("path_samples", C.c_int32), ("bounces", C.c_int32), ("rr_min_bounces", C.c_int32),
                 ("aa_samples", C.c_int32), ("aa_pixelwidth", C.c_float), ("has_background", C.c_int32),
                 ("background", C.c_float * 3), ("seed", C.c_uint32), ("raydepth", C.c_int32),
-                ("transparent_shadows", C.c_int32), ("shadow_depth", C.c_int32)]
+                ("transparent_shadows", C.c_int32), ("shadow_depth", C.c_int32),
+                ("aa_passes", C.c_int32), ("aa_inc_samples", C.c_int32), ("aa_threshold", C.c_float),
+                ("aa_resampled_floor", C.c_float), ("aa_sample_multiplier_factor", C.c_float),
+                ("aa_light_sample_multiplier_factor", C.c_float), ("aa_detect_color_noise", C.c_int32),
+                ("aa_dark_detection_type", C.c_int32), ("aa_dark_threshold_factor", C.c_float),
+                ("aa_variance_edge_size", C.c_int32), ("aa_variance_pixels", C.c_int32),
+                ("aa_clamp_samples", C.c_float), ("corner_normals", C.c_void_p), ("trace_brute", C.c_int32)]
+
+
+# ImageFilm's multi-pass AA params (include/render/imagefilm.h:212-226): the
+# settings key, the struct field, the reference's default
+AA_PARAMS = [("AA_passes", "aa_passes", 1), ("AA_inc_samples", "aa_inc_samples", 1),
+             ("AA_threshold", "aa_threshold", 0.05), ("AA_resampled_floor", "aa_resampled_floor", 0.0),
+             ("AA_sample_multiplier_factor", "aa_sample_multiplier_factor", 1.0),
+             ("AA_light_sample_multiplier_factor", "aa_light_sample_multiplier_factor", 1.0),
+             ("AA_detect_color_noise", "aa_detect_color_noise", 0),
+             ("AA_dark_detection_type", "aa_dark_detection_type", "none"),
+             ("AA_dark_threshold_factor", "aa_dark_threshold_factor", 0.0),
+             ("AA_variance_edge_size", "aa_variance_edge_size", 10),
+             ("AA_variance_pixels", "aa_variance_pixels", 0), ("AA_clamp_samples", "aa_clamp_samples", 0.0)]
+DARK_DETECTION = {"none": 0, "linear": 1, "curve": 2}
+
+
+def fill_aa(st, s):
+    """The AA pass params of a scene's settings dict into a settings struct
+    (ora_pt_settings or the library's ymx_pt_settings, same field names)."""
+    for key, field, default in AA_PARAMS:
+        v = s.get(key, default)
+        if field == "aa_dark_detection_type":
+            v = DARK_DETECTION[v] if isinstance(v, str) else int(v)
+        setattr(st, field, float(v) if isinstance(default, float) else int(v))
 
 
 def _f3v(v):
@@ -320,7 +350,20 @@ def pt_structs(scene):
     st.raydepth = int(s.get("raydepth", 5))
     st.transparent_shadows = int(bool(s.get("transpShad", False)))
     st.shadow_depth = int(s.get("shadowDepth", 4))
+    fill_aa(st, s)
     return mats, lights, st
+
+
+def corner_normals_array(scene):
+    """The scene's per-corner normals as the float32 (n_tris, 12) array the
+    path tracers take, or None."""
+    cn = scene.get("corner_normals")
+    if cn is None:
+        return None
+    cn = np.ascontiguousarray(cn, dtype=np.float32).reshape(-1, 12)
+    if cn.shape[0] != len(scene["tris"]):
+        raise ValueError("one row of corner normals per triangle is required")
+    return cn
 
 
 def _pt_lib():
@@ -353,6 +396,10 @@ def _pt_lib():
         L.ora_rr_random.argtypes = [C.c_uint32] * 4
         L.ora_rr_random.restype = C.c_float
         L.ora_filter_table.argtypes = [C.c_float, vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.ora_aa_next_pass.argtypes = [vp, C.c_float, vp, vp, vp]
+        L.ora_aa_next_pass.restype = C.c_int
+        L.ora_aa_max_samples.argtypes = [vp]
+        L.ora_aa_max_samples.restype = C.c_int
         L._pt_ready = True
     return L
 
@@ -360,7 +407,9 @@ def _pt_lib():
 class PathTracer:
     """CPU restatement of the reference's pathtracing integrator (oracle)."""
 
-    def __init__(self, scene, kd: "KdTree | None" = None):
+    def __init__(self, scene, kd: "KdTree | None" = None, brute: bool = False):
+        """brute: closest hits by brute force (exact ties to the lowest id, as
+        the GPU) instead of the kd-tree port; for small scenes."""
         self.scene = scene
         self.verts = np.ascontiguousarray(scene["verts"], dtype=np.float32)
         self.tris = np.ascontiguousarray(scene["tris"], dtype=np.uint32)
@@ -369,6 +418,9 @@ class PathTracer:
         self.vis = self.kd.vis
         self.mats, self.lights, self.st = pt_structs(scene)
         self.n_lights = len(scene["lights"])
+        self._cn = corner_normals_array(scene)
+        self.st.corner_normals = None if self._cn is None else self._cn.ctypes.data
+        self.st.trace_brute = 1 if brute else 0
 
     def render(self, threads=None, aa_samples=None):
         L = _pt_lib()
@@ -390,6 +442,24 @@ class PathTracer:
                               self.vis.ctypes.data, C.addressof(self.mats), C.addressof(self.lights), self.n_lights, C.addressof(self.st),
                               int(x), int(y), int(sample), out.ctypes.data, dxdy.ctypes.data)
         return out, dxdy
+
+
+def aa_next_pass(settings_scene, threshold, rgba, w):
+    """ImageFilm::nextPass's resampling flags for film sums rgba (h, w, 4) and
+    weights (h, w): returns (flags uint8 (h, w), count)."""
+    _, _, st = pt_structs(settings_scene)
+    L = _pt_lib()
+    rgba = np.ascontiguousarray(rgba, dtype=np.float32)
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    flags = np.zeros(w.shape, dtype=np.uint8)
+    n = L.ora_aa_next_pass(C.addressof(st), C.c_float(threshold), rgba.ctypes.data, w.ctypes.data,
+                           flags.ctypes.data)
+    return flags, n
+
+
+def aa_max_samples(settings_scene):
+    _, _, st = pt_structs(settings_scene)
+    return _pt_lib().ora_aa_max_samples(C.addressof(st))
 
 
 def camera_dir(scene, px, py):

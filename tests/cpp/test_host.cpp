@@ -12,11 +12,14 @@
 #include <hip/hip_runtime.h>
 #include <unistd.h>
 
+#include <array>
+#include <algorithm>
 #include <cstdlib>
 
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <random>
 #include <set>
 #include <string>
@@ -702,6 +705,319 @@ static void test_slab_nan_order()
 	}
 }
 
+// MeshObject::smoothVerticesNormals (src/geometry/object/object_mesh.cc:150-264)
+// against closed forms: a flat grid keeps its face normal, a cube corner's
+// normal averages its three faces equally (each face meets the corner with a
+// total angle of pi/2, one right-angled corner or two 45-degree ones), a cube
+// smoothed below 90 degrees keeps face normals at corners with a coplanar
+// partner and none elsewhere, and an icosphere's normals are radial.
+static size_t add_cube(Scene &sc, size_t mat, const float lo[3] = nullptr, float size = 1.f)
+{
+	ParamMap normal;
+	const size_t obj = sc.createObject("cube", normal);
+	const float o[3] = {lo ? lo[0] : 0.f, lo ? lo[1] : 0.f, lo ? lo[2] : 0.f};
+	for(int i = 0; i < 8; ++i)
+		sc.addVertex(obj, o[0] + size * (float)(i & 1), o[1] + size * (float)((i >> 1) & 1),
+		             o[2] + size * (float)((i >> 2) & 1));
+	const int faces[6][4] = {{0, 2, 3, 1}, {4, 5, 7, 6}, {0, 1, 5, 4}, {2, 6, 7, 3}, {0, 4, 6, 2}, {1, 3, 7, 5}};
+	for(const auto &f : faces)
+	{
+		sc.addTriangle(obj, f[0], f[1], f[2], mat);
+		sc.addTriangle(obj, f[0], f[2], f[3], mat);
+	}
+	return obj;
+}
+
+static void test_smooth_cpu()
+{
+	Scene sc;
+	std::string err;
+	ParamMap m;
+	m.setString("type", "shinydiffusemat");
+	size_t mat = 0;
+	CHECK(sc.createMaterial(&mat, "m", m, &err), "material");
+	ParamMap normal;
+	// flat grid
+	const size_t grid = sc.createObject("grid", normal);
+	for(int j = 0; j < 4; ++j)
+		for(int i = 0; i < 4; ++i) sc.addVertex(grid, i * 0.3, j * 0.25, 0.5);
+	for(int j = 0; j < 3; ++j)
+		for(int i = 0; i < 3; ++i)
+		{
+			const size_t a = 4 * j + i;
+			sc.addTriangle(grid, a, a + 1, a + 5, mat);
+			sc.addTriangle(grid, a, a + 5, a + 4, mat);
+		}
+	CHECK(sc.cornerNormals(grid).size() == 18 * 12 && sc.cornerNormals(grid)[3] == -1.f, "unsmoothed grid is flat");
+	CHECK(sc.smoothMesh(grid, 180.0), "smooth grid");
+	const std::vector<float> gn = sc.cornerNormals(grid);
+	bool flat_ok = true;
+	for(size_t c = 0; c < gn.size() / 4; ++c)
+		flat_ok = flat_ok && gn[4 * c + 3] == 1.f && fabsf(gn[4 * c]) < 1e-7f && fabsf(gn[4 * c + 1]) < 1e-7f &&
+		          fabsf(gn[4 * c + 2] - 1.f) < 1e-7f;
+	CHECK(flat_ok, "a flat grid smooths to its face normal");
+	// cube at 180: corner normals along the cube's diagonals
+	const size_t cube = add_cube(sc, mat);
+	CHECK(sc.smoothMesh(cube, 180.0), "smooth cube");
+	const std::vector<float> cn = sc.cornerNormals(cube);
+	double worst = 0.0;
+	const int faces[6][4] = {{0, 2, 3, 1}, {4, 5, 7, 6}, {0, 1, 5, 4}, {2, 6, 7, 3}, {0, 4, 6, 2}, {1, 3, 7, 5}};
+	for(int f = 0; f < 6; ++f)
+		for(int t = 0; t < 2; ++t)
+			for(int k = 0; k < 3; ++k)
+			{
+				const int v = faces[f][t == 0 ? k : (k == 0 ? 0 : k + 1)];
+				const double e[3] = {(v & 1) ? 1.0 : -1.0, ((v >> 1) & 1) ? 1.0 : -1.0, ((v >> 2) & 1) ? 1.0 : -1.0};
+				const float *c = &cn[12 * (2 * f + t) + 4 * k];
+				double d = 0.0;
+				for(int i = 0; i < 3; ++i) d += c[i] * e[i] / sqrt(3.0);
+				worst = std::max(worst, 1.0 - d);
+			}
+	printf("  cube corners: 1 - cos(error) at most %.2e\n", worst);
+	CHECK(worst < 1e-4, "smoothed cube corners point along the diagonals");
+	// cube at 60 degrees: neighbouring faces are 90 degrees apart
+	Scene sc2;
+	size_t mat2 = 0;
+	sc2.createMaterial(&mat2, "m", m, &err);
+	const size_t cube2 = add_cube(sc2, mat2);
+	CHECK(sc2.smoothMesh(cube2, 60.0), "smooth cube at 60");
+	const std::vector<float> c2 = sc2.cornerNormals(cube2);
+	const float fn[6][3] = {{0, 0, -1}, {0, 0, 1}, {0, -1, 0}, {0, 1, 0}, {-1, 0, 0}, {1, 0, 0}};
+	int with = 0, without = 0;
+	bool sharp_ok = true;
+	for(int f = 0; f < 6; ++f)
+		for(int t = 0; t < 2; ++t)
+			for(int k = 0; k < 3; ++k)
+			{
+				const float *c = &c2[12 * (2 * f + t) + 4 * k];
+				if(c[3] == 0.f) ++without;
+				else
+				{
+					++with;
+					for(int i = 0; i < 3; ++i) sharp_ok = sharp_ok && fabsf(c[i] - fn[f][i]) < 1e-6f;
+				}
+			}
+	printf("  cube at 60 degrees: %d corners with the face normal, %d without\n", with, without);
+	CHECK(sharp_ok && with == 24 && without == 12, "sharp cube edges keep face normals");
+	// icosphere: radial normals
+	Scene sc3;
+	size_t mat3 = 0;
+	sc3.createMaterial(&mat3, "m", m, &err);
+	const size_t sph = sc3.createObject("sphere", normal);
+	std::vector<float> sv;
+	std::vector<uint32_t> st;
+	{
+		// octahedron subdivided twice
+		std::vector<std::array<double, 3>> v = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+		std::vector<std::array<int, 3>> f = {{0, 2, 4}, {2, 1, 4}, {1, 3, 4}, {3, 0, 4}, {2, 0, 5}, {1, 2, 5}, {3, 1, 5}, {0, 3, 5}};
+		for(int s_ = 0; s_ < 2; ++s_)
+		{
+			std::map<std::pair<int, int>, int> mid;
+			auto m_ = [&](int a, int b) {
+				const auto key = std::make_pair(std::min(a, b), std::max(a, b));
+				auto it = mid.find(key);
+				if(it != mid.end()) return it->second;
+				std::array<double, 3> p = {v[a][0] + v[b][0], v[a][1] + v[b][1], v[a][2] + v[b][2]};
+				const double l = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+				for(double &x : p) x /= l;
+				v.push_back(p);
+				return mid[key] = (int)v.size() - 1;
+			};
+			std::vector<std::array<int, 3>> nf;
+			for(const auto &t : f)
+			{
+				const int ab = m_(t[0], t[1]), bc = m_(t[1], t[2]), ca = m_(t[2], t[0]);
+				nf.push_back({t[0], ab, ca});
+				nf.push_back({ab, t[1], bc});
+				nf.push_back({ca, bc, t[2]});
+				nf.push_back({ab, bc, ca});
+			}
+			f = nf;
+		}
+		for(const auto &p : v) sc3.addVertex(sph, p[0], p[1], p[2]);
+		for(const auto &t : f) sc3.addTriangle(sph, t[0], t[1], t[2], mat3);
+		CHECK(sc3.smoothMesh(sph, 180.0), "smooth sphere");
+		const std::vector<float> c3 = sc3.cornerNormals(sph);
+		double worst3 = 0.0;
+		for(size_t t = 0; t < f.size(); ++t)
+			for(int k = 0; k < 3; ++k)
+			{
+				const auto &p = v[f[t][k]];
+				const float *c = &c3[12 * t + 4 * k];
+				worst3 = std::max(worst3, 1.0 - (c[0] * p[0] + c[1] * p[1] + c[2] * p[2]));
+			}
+		printf("  icosphere: 1 - cos(error) at most %.2e\n", worst3);
+		CHECK(worst3 < 2e-3, "smoothed sphere normals are radial");
+	}
+	// explicit vertex normals (yafaray_addNormal): the corners take them
+	Scene sc4;
+	size_t mat4 = 0;
+	sc4.createMaterial(&mat4, "m", m, &err);
+	const size_t tri = sc4.createObject("tri", normal);
+	for(int i = 0; i < 3; ++i)
+	{
+		sc4.addVertex(tri, i == 1, i == 2, 0);
+		sc4.addNormal(tri, 0.1 * i, 0.0, 1.0);
+	}
+	sc4.addTriangle(tri, 0, 1, 2, mat4);
+	const std::vector<float> c4 = sc4.cornerNormals(tri);
+	CHECK(c4.size() == 12 && c4[3] == 1.f && c4[4] == 0.1f && c4[8] == 0.2f && c4[11] == 1.f,
+	      "explicit vertex normals at the corners");
+}
+
+// A scene built through the host layer with a smoothed sphere and multi-pass
+// adaptive AA renders the oracle's film bit for bit.
+static void test_pathtracer_smooth_aa()
+{
+	Scene sc;
+	std::string err;
+	ParamMap white, glossy;
+	white.setString("type", "shinydiffusemat");
+	white.setColor("color", 0.7, 0.7, 0.7);
+	glossy.setString("type", "glossy");
+	glossy.setColor("color", 0.8, 0.7, 0.6);
+	glossy.setFloat("diffuse_reflect", 0.3);
+	glossy.setFloat("exponent", 40.0);
+	size_t mw = 0, mg = 0;
+	CHECK(sc.createMaterial(&mw, "white", white, &err) && sc.createMaterial(&mg, "glossy", glossy, &err), "materials");
+	ParamMap normal;
+	const size_t walls = sc.createObject("walls", normal);
+	std::vector<float> verts;
+	std::vector<uint32_t> tris, mats;
+	const float o[3] = {0, 0, 0}, x[3] = {1, 0, 0}, y[3] = {0, 1, 0}, z[3] = {0, 0, 1};
+	const float top[3] = {0, 0, 1}, back[3] = {0, 1, 0}, right[3] = {1, 0, 0};
+	quad(sc, walls, mw, o, y, x, verts, tris, mats);
+	quad(sc, walls, mw, top, x, y, verts, tris, mats);
+	quad(sc, walls, mw, back, z, x, verts, tris, mats);
+	quad(sc, walls, mw, o, z, y, verts, tris, mats);
+	quad(sc, walls, mw, right, y, z, verts, tris, mats);
+	// a glossy cube standing in the box, shaded smooth (its corner normals
+	// along the diagonals)
+	const float cube_lo[3] = {0.35f, 0.35f, 0.0625f}, cube_size = 0.3f;
+	const size_t cube = add_cube(sc, mg, cube_lo, cube_size);
+	{
+		// the oracle's copy of the cube's vertices, as add_cube added them
+		const uint32_t v0 = (uint32_t)(verts.size() / 3);
+		for(int i = 0; i < 8; ++i)
+		{
+			verts.push_back((float)((double)cube_lo[0] + (double)(cube_size * (float)(i & 1))));
+			verts.push_back((float)((double)cube_lo[1] + (double)(cube_size * (float)((i >> 1) & 1))));
+			verts.push_back((float)((double)cube_lo[2] + (double)(cube_size * (float)((i >> 2) & 1))));
+		}
+		const int faces[6][4] = {{0, 2, 3, 1}, {4, 5, 7, 6}, {0, 1, 5, 4}, {2, 6, 7, 3}, {0, 4, 6, 2}, {1, 3, 7, 5}};
+		for(const auto &f : faces)
+		{
+			tris.insert(tris.end(), {v0 + f[0], v0 + f[1], v0 + f[2], v0 + f[0], v0 + f[2], v0 + f[3]});
+			mats.insert(mats.end(), {(uint32_t)mg, (uint32_t)mg});
+		}
+	}
+	CHECK(sc.smoothMesh(cube, 180.0), "smooth");
+	ParamMap al;
+	al.setString("type", "arealight");
+	al.setVector("corner", 0.4, 0.4, 0.999);
+	al.setVector("point1", 0.4, 0.6, 0.999);
+	al.setVector("point2", 0.6, 0.4, 0.999);
+	al.setFloat("power", 10.0 / M_PI);
+	CHECK(sc.createLight("area", al, &err), "light");
+	ParamMap integ, cam, film;
+	integ.setString("type", "pathtracing");
+	integ.setInt("bounces", 3);
+	integ.setInt("path_samples", 1);
+	cam.setString("type", "perspective");
+	cam.setVector("from", 0.5, -1.6, 0.5);
+	cam.setVector("to", 0.5, 0.5, 0.5);
+	cam.setVector("up", 0.5, -1.6, 1.5);
+	cam.setFloat("focal", 1.0 / (2.0 * tan(20.0 * M_PI / 180.0)));
+	cam.setInt("resx", 32);
+	cam.setInt("resy", 24);
+	film.setInt("AA_minsamples", 2);
+	film.setInt("AA_passes", 3);
+	film.setInt("AA_inc_samples", 2);
+	film.setFloat("AA_threshold", 0.02);
+	film.setFloat("AA_light_sample_multiplier_factor", 1.5);
+	auto pi = PathIntegrator::create(sc, integ, cam, film, nullptr, &err);
+	CHECK(pi != nullptr, "create: %s", err.c_str());
+	if(!pi) return;
+	std::vector<int> res;
+	CHECK(pi->clearFilm() == YMX_OK && pi->renderPasses(&res) == YMX_OK, "render: %s", pi->lastError().c_str());
+	std::vector<float> rgba, w;
+	CHECK(pi->film(rgba, w) == YMX_OK, "film");
+	printf("  resampled per pass: %d %d %d\n", res.size() > 0 ? res[0] : -1, res.size() > 1 ? res[1] : -1,
+	       res.size() > 2 ? res[2] : -1);
+	// the oracle with the same arrays, corner normals and AA params
+	std::vector<float> cnorm = sc.cornerNormals(walls);
+	const std::vector<float> cc = sc.cornerNormals(cube);
+	cnorm.insert(cnorm.end(), cc.begin(), cc.end());
+	std::vector<ora_material> omats(2);
+	for(auto &mm : omats)
+	{
+		mm = ora_material{};
+		mm.diffuse_reflect = 1.f;
+		mm.receive_shadows = 1;
+		mm.diffuse_color[0] = mm.diffuse_color[1] = mm.diffuse_color[2] = 1.f;
+		mm.glossy_reflect = 1.f;
+		mm.exponent = 50.f;
+		mm.as_diffuse = 1;
+		mm.reflect = 1.f;
+		mm.ior = 1.4f;
+		mm.filter_color[0] = mm.filter_color[1] = mm.filter_color[2] = 1.f;
+		mm.mirror_color[0] = mm.mirror_color[1] = mm.mirror_color[2] = 1.f;
+		mm.power = 1.f;
+	}
+	omats[0].type = ORA_MAT_SHINYDIFFUSE;
+	omats[0].color[0] = omats[0].color[1] = omats[0].color[2] = 0.7f;
+	omats[1].type = ORA_MAT_GLOSSY;
+	omats[1].color[0] = 0.8f;
+	omats[1].color[1] = 0.7f;
+	omats[1].color[2] = 0.6f;
+	omats[1].diffuse_reflect = 0.3f;
+	omats[1].exponent = 40.f;
+	ora_light light{};
+	light.type = ORA_LIGHT_AREA;
+	const float corner[3] = {0.4f, 0.4f, 0.999f}, p1[3] = {0.4f, 0.6f, 0.999f}, p2[3] = {0.6f, 0.4f, 0.999f};
+	memcpy(light.corner, corner, 12);
+	memcpy(light.point1, p1, 12);
+	memcpy(light.point2, p2, 12);
+	light.color[0] = light.color[1] = light.color[2] = 1.f;
+	light.power = (float)(10.0 / M_PI);
+	light.samples = 4;
+	light.cast_shadows = 1;
+	ora_pt_settings st{};
+	const float from[3] = {0.5f, -1.6f, 0.5f}, to[3] = {0.5f, 0.5f, 0.5f}, up[3] = {0.5f, -1.6f, 1.5f};
+	memcpy(st.from, from, 12);
+	memcpy(st.to, to, 12);
+	memcpy(st.up, up, 12);
+	st.focal = (float)(1.0 / (2.0 * tan(20.0 * M_PI / 180.0)));
+	st.aspect_factor = 1.f;
+	st.resx = 32;
+	st.resy = 24;
+	st.path_samples = 1;
+	st.bounces = 3;
+	st.aa_samples = 2;
+	st.aa_pixelwidth = 1.5f;
+	st.raydepth = 5;
+	st.shadow_depth = 4;
+	st.aa_passes = 3;
+	st.aa_inc_samples = 2;
+	st.aa_threshold = 0.02f;
+	st.aa_sample_multiplier_factor = 1.f;
+	st.aa_light_sample_multiplier_factor = 1.5f;
+	st.aa_variance_edge_size = 10;
+	st.corner_normals = cnorm.data();
+	st.trace_brute = 1;  // ties in t to the lowest id, as the GPU
+	std::vector<uint8_t> vis(mats.size(), 3);
+	ora_kdtree *kd = ora_kd_build(verts.data(), tris.data(), vis.data(), (uint32_t)mats.size());
+	std::vector<float> orgba(4 * 32 * 24, 0.f), ow(32 * 24, 0.f);
+	ora_pt_render(kd, verts.data(), tris.data(), mats.data(), vis.data(), omats.data(), &light, 1, &st, orgba.data(),
+	              ow.data(), 8);
+	ora_kd_free(kd);
+	size_t bad = 0;
+	for(size_t i = 0; i < rgba.size(); ++i) bad += memcmp(&rgba[i], &orgba[i], 4) != 0;
+	for(size_t i = 0; i < w.size(); ++i) bad += memcmp(&w[i], &ow[i], 4) != 0;
+	printf("  smooth + AA passes: %zu film values differ from the oracle\n", bad);
+	CHECK(bad == 0 && res.size() == 3 && res[1] > 0 && res[1] < 32 * 24, "film differs from the oracle");
+}
+
 int main(int argc, char **argv)
 {
 	const bool cpu_only = argc > 1 && strcmp(argv[1], "--cpu") == 0;
@@ -712,6 +1028,8 @@ int main(int argc, char **argv)
 		bool gpu;
 	} tests[] = {{"errors", test_errors, false},
 	             {"slab_nan_order", test_slab_nan_order, false},
+	             {"smooth_cpu", test_smooth_cpu, false},
+	             {"pathtracer_smooth_aa", test_pathtracer_smooth_aa, true},
 	             {"accel_closest", test_accel_closest, true},
 	             {"accel_closest_ploc", [] { g_builder = "ploc"; test_accel_closest(); g_builder = "lbvh"; }, true},
 	             {"accel_shadow_ploc", [] { g_builder = "ploc"; test_accel_shadow(); g_builder = "lbvh"; }, true},
