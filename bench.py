@@ -52,6 +52,8 @@ def parse():
     p.add_argument("--treelet-passes", type=int, default=0,
                    help="SAH treelet restructuring passes of the ray-cast tree (0: the library default, -1: none)")
     p.add_argument("--builder", type=str, default="lbvh", help="tree builder of the ray-cast batches")
+    p.add_argument("--pt-treelet-passes", type=int, default=0,
+                   help="treelet passes of the path tracer's tree (0: the library default, -1: none)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run only a plain 1 GiB device-to-device copy (HBM counter calibration)")
     p.add_argument("--no-pt", action="store_true", help="skip the path-tracing measurement")
@@ -81,7 +83,7 @@ def measure_path_tracer(args, torch):
     from libyafaray_amd import scenes
     from libyafaray_amd.pt import GpuPathTracer
     sc = scenes.cornell_path_scene()
-    pt = GpuPathTracer(sc)
+    pt = GpuPathTracer(sc, treelet_passes=args.pt_treelet_passes)
     pt.clear()
     pt.render(0, 16)  # warm-up (kernel loads, first-touch)
     torch.cuda.synchronize()
@@ -158,7 +160,7 @@ def measure_guiding(args, torch):
     def tracer(aa, seed):
         s = dict(sc)
         s["settings"] = dict(sc["settings"], aa_samples=aa, seed=seed)
-        return GpuPathTracer(s)
+        return GpuPathTracer(s, treelet_passes=getattr(args, "pt_treelet_passes", 0))
 
     def ratio(rgba, w):
         return rgba[..., :3].astype(np.float64) / w[..., None]

@@ -94,7 +94,11 @@ typedef struct ymx_accel_params
  * path-traced rooms render 4-12 % faster on PLOC trees. */
 #define YMX_BUILDER_LBVH 0
 #define YMX_BUILDER_PLOC 1
-#define YMX_TREELET_PASSES_DEFAULT 0
+/* Two treelet passes: on the 2.19M-triangle heightfield (LBVH, 4-triangle
+ * leaves) 45.8 -> 42.8 nodes and 20.2 -> 18.0 triangles per incoherent ray,
+ * +9 % incoherent and +5 % coherent closest hit, +8 % any hit, for 155 ms
+ * more build time (one pass: +7 %, three: +9.5 %). */
+#define YMX_TREELET_PASSES_DEFAULT 2
 
 typedef struct ymx_accel_info
 {

@@ -160,7 +160,8 @@ class GpuPathTracer:
     results; 4-triangle leaves are faster on the Cornell frame but 5 % slower
     on the occluded room)."""
 
-    def __init__(self, scene, accel: GpuAccelerator | None = None, builder: str = "ploc", max_leaf_size: int = 2):
+    def __init__(self, scene, accel: GpuAccelerator | None = None, builder: str = "ploc", max_leaf_size: int = 2,
+                 treelet_passes: int = 0):
         self._lib = native.lib()
         self.verts = np.ascontiguousarray(scene["verts"], dtype=np.float32)
         self.tris = np.ascontiguousarray(scene["tris"], dtype=np.uint32)
@@ -168,7 +169,7 @@ class GpuPathTracer:
         vis = scene.get("vis")
         self.vis = None if vis is None else np.ascontiguousarray(vis, dtype=np.uint8)
         self.accel = accel or GpuAccelerator(self.verts, self.tris, self.vis, max_leaf_size=max_leaf_size,
-                                              builder=builder)
+                                              builder=builder, treelet_passes=treelet_passes)
         self.mats, self.lights, self.st = build_structs(scene)
         self.resx, self.resy = self.st.resx, self.st.resy
         h = C.c_void_p()
