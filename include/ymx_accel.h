@@ -83,7 +83,17 @@ typedef struct ymx_accel_params
 	int32_t treelet_passes; /* SAH treelet restructuring passes over the built tree
 	                           (Karras & Aila 2013): 1..8, -1 = none,
 	                           0 = default (YMX_TREELET_PASSES_DEFAULT) */
-	int32_t reserved[4];
+	int32_t ray_reorder;    /* closest- and any-hit batches trace their rays grouped by
+	                           direction octant and origin cell (a device counting
+	                           sort of the ray indices; results stay in the caller's
+	                           order): 0 = batches of at least 2^18 rays that are not
+	                           already coherent (decided on the device, no host wait),
+	                           1 = always, -1 = never */
+	int32_t reorder_cell_bits; /* sort granularity of ray reordering: origin cells per
+	                              axis of the tree bound, 2^bits (3: 4,096 buckets with
+	                              the octant, 4: 32,768); 0 = default
+	                              (YMX_REORDER_CELL_BITS_DEFAULT) */
+	int32_t reserved[2];
 } ymx_accel_params;
 
 /* Tree builders.  LBVH (Karras 2012) splits at Morton-code prefixes; PLOC
@@ -99,6 +109,7 @@ typedef struct ymx_accel_params
  * +9 % incoherent and +5 % coherent closest hit, +8 % any hit, for 155 ms
  * more build time (one pass: +7 %, three: +9.5 %). */
 #define YMX_TREELET_PASSES_DEFAULT 2
+#define YMX_REORDER_CELL_BITS_DEFAULT 3
 
 typedef struct ymx_accel_info
 {

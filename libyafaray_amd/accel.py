@@ -36,7 +36,8 @@ class GpuAccelerator:
     type_name = "mi355x-lbvh"
 
     def __init__(self, verts: np.ndarray, tris: np.ndarray, vis: np.ndarray | None = None,
-                 max_leaf_size: int = 0, builder: str = "lbvh", treelet_passes: int = 0):
+                 max_leaf_size: int = 0, builder: str = "lbvh", treelet_passes: int = 0, ray_reorder: int = 0,
+                 reorder_cell_bits: int = 0):
         self._lib = native.lib()
         verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
         tris = np.ascontiguousarray(tris, dtype=np.uint32)
@@ -49,7 +50,7 @@ class GpuAccelerator:
         if builder not in native.BUILDERS:
             raise ValueError(f"unknown BVH builder {builder!r} (one of {sorted(native.BUILDERS)})")
         params = YmxAccelParams(C.sizeof(YmxAccelParams), int(max_leaf_size), native.BUILDERS[builder],
-                                int(treelet_passes))
+                                int(treelet_passes), int(ray_reorder), int(reorder_cell_bits))
         handle = C.c_void_p()
         create = self._lib.ymx_accel_create_faces if faces else self._lib.ymx_accel_create
         rc = create(

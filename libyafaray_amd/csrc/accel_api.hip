@@ -56,7 +56,8 @@ static int create_accel(const char *fn, const float *verts, uint32_t n_verts, co
 	}
 	// 4 triangles per leaf: on the 2.19M-triangle heightfield +2 % incoherent
 	// and +14 % coherent closest hit over 1 (LBVH; PLOC keeps 1)
-	int max_leaf = 4, builder = YMX_BUILDER_LBVH, treelet_passes = YMX_TREELET_PASSES_DEFAULT;
+	int max_leaf = 4, builder = YMX_BUILDER_LBVH, treelet_passes = YMX_TREELET_PASSES_DEFAULT, ray_reorder = 0;
+	int cell_bits = YMX_REORDER_CELL_BITS_DEFAULT;
 	if(params)
 	{
 		if(params->struct_size != sizeof(ymx_accel_params))
@@ -82,6 +83,18 @@ static int create_accel(const char *fn, const float *verts, uint32_t n_verts, co
 			return YMX_ERR_INVALID_ARG;
 		}
 		if(params->treelet_passes != 0) treelet_passes = params->treelet_passes < 0 ? 0 : params->treelet_passes;
+		if(params->ray_reorder < -1 || params->ray_reorder > 1)
+		{
+			ymx_set_error("%s: ray_reorder %d outside -1..1", fn, params->ray_reorder);
+			return YMX_ERR_INVALID_ARG;
+		}
+		ray_reorder = params->ray_reorder;
+		if(params->reorder_cell_bits != 0 && params->reorder_cell_bits != 3 && params->reorder_cell_bits != 4)
+		{
+			ymx_set_error("%s: reorder_cell_bits %d is not 0, 3 or 4", fn, params->reorder_cell_bits);
+			return YMX_ERR_INVALID_ARG;
+		}
+		if(params->reorder_cell_bits != 0) cell_bits = params->reorder_cell_bits;
 	}
 	// a face's 4th index is YMX_NO_VERTEX for a triangle; quads get e3 slots
 	std::vector<uint32_t> slot;
@@ -118,6 +131,8 @@ static int create_accel(const char *fn, const float *verts, uint32_t n_verts, co
 	}
 	acc->n_quads = n_quads;
 	acc->treelet_passes = (uint32_t)treelet_passes;
+	acc->ray_reorder = ray_reorder;
+	acc->reorder_cell_bits = cell_bits;
 	int rc = ymx_build_device(acc, verts, n_verts, idx, stride, n_quads ? slot.data() : nullptr, vis, n_tris, max_leaf,
 	                          builder, treelet_passes);
 	if(rc == YMX_OK) rc = ymx_trace_setup(acc);

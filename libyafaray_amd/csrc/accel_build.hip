@@ -25,7 +25,8 @@
 #include "accel_internal.h"
 #include "treelet.h"
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <float.h>
 #include <math.h>
@@ -1044,14 +1045,16 @@ int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const
 	                                                       d_keys.as<uint64_t>(), d_vals.as<uint32_t>());
 	YMX_HIP_CHECK(hipGetLastError());
 
-	hipcub::DoubleBuffer<uint64_t> kb(d_keys.as<uint64_t>(), d_keys_alt.as<uint64_t>());
-	hipcub::DoubleBuffer<uint32_t> vb(d_vals.as<uint32_t>(), d_vals_alt.as<uint32_t>());
+	// the 63-bit Morton keys, ping-ponged between the two buffers: the sorted
+	// run ends in whichever current() names
+	rocprim::double_buffer<uint64_t> kb(d_keys.as<uint64_t>(), d_keys_alt.as<uint64_t>());
+	rocprim::double_buffer<uint32_t> vb(d_vals.as<uint32_t>(), d_vals_alt.as<uint32_t>());
 	size_t temp_bytes = 0;
-	YMX_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, kb, vb, (int)n_tris, 0, 63, stream));
+	YMX_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, temp_bytes, kb, vb, (size_t)n_tris, 0u, 63u, stream));
 	YMX_HIP_CHECK(d_temp.alloc(temp_bytes));
-	YMX_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(d_temp.p, temp_bytes, kb, vb, (int)n_tris, 0, 63, stream));
-	const uint64_t *keys = kb.Current();
-	const uint32_t *order = vb.Current();
+	YMX_HIP_CHECK(rocprim::radix_sort_pairs(d_temp.p, temp_bytes, kb, vb, (size_t)n_tris, 0u, 63u, stream));
+	const uint64_t *keys = kb.current();
+	const uint32_t *order = vb.current();
 
 	k_reorder<<<grid_for(n_tris), kBuildBlock, 0, stream>>>(d_tris_unsorted.as<float4>(), order, n_tris,
 	                                                        d_tris.as<float4>());
@@ -1094,8 +1097,8 @@ int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const
 		YMX_HIP_CHECK(d_mpos.alloc(sizeof(uint32_t) * (size_t)n_tris));
 		YMX_HIP_CHECK(d_tot.alloc(sizeof(uint32_t) * 2));
 		size_t scan_bytes = 0;
-		YMX_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, d_keep.as<uint32_t>(), d_kpos.as<uint32_t>(),
-		                                               (int)n_tris, stream));
+		YMX_HIP_CHECK(rocprim::exclusive_scan(nullptr, scan_bytes, d_keep.as<uint32_t>(), d_kpos.as<uint32_t>(), 0u,
+		                                      (size_t)n_tris, rocprim::plus<uint32_t>(), stream));
 		YMX_HIP_CHECK(d_scan.alloc(scan_bytes));
 		k_ploc_init<<<grid_for(n_tris), kBuildBlock, 0, stream>>>(d_prim_box.as<float>(), order, n_tris, pad,
 		                                                          d_ca.as<Cluster>());
@@ -1116,11 +1119,11 @@ int ymx_build_device(ymx_accel *acc, const float *verts, uint32_t n_verts, const
 			                                                          d_merge.as<uint32_t>());
 			YMX_HIP_CHECK(hipGetLastError());
 			size_t sb = scan_bytes;
-			YMX_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(d_scan.p, sb, d_keep.as<uint32_t>(), d_kpos.as<uint32_t>(),
-			                                               (int)count, stream));
+			YMX_HIP_CHECK(rocprim::exclusive_scan(d_scan.p, sb, d_keep.as<uint32_t>(), d_kpos.as<uint32_t>(), 0u,
+			                                      (size_t)count, rocprim::plus<uint32_t>(), stream));
 			sb = scan_bytes;
-			YMX_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(d_scan.p, sb, d_merge.as<uint32_t>(), d_mpos.as<uint32_t>(),
-			                                               (int)count, stream));
+			YMX_HIP_CHECK(rocprim::exclusive_scan(d_scan.p, sb, d_merge.as<uint32_t>(), d_mpos.as<uint32_t>(), 0u,
+			                                      (size_t)count, rocprim::plus<uint32_t>(), stream));
 			k_ploc_totals<<<1, 1, 0, stream>>>(d_keep.as<uint32_t>(), d_kpos.as<uint32_t>(), d_merge.as<uint32_t>(),
 			                                   d_mpos.as<uint32_t>(), count, d_tot.as<uint32_t>());
 			YMX_HIP_CHECK(hipGetLastError());

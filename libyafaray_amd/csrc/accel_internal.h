@@ -53,6 +53,8 @@ struct ymx_work_area
 	hipEvent_t done;    // recorded after each launch on the area
 	uint32_t *d_work;   // [0] ray counter, [1] finished waves
 	int32_t *d_spill;
+	void *d_sort;       // ray reordering: change count, buckets, order, per-tile bucket counts, bucket bases
+	size_t sort_bytes;
 };
 
 struct ymx_accel
@@ -67,6 +69,8 @@ struct ymx_accel
 	float4 *d_tris;
 	float4 *d_quads;         // e3 of each quad primitive (NULL without quads)
 	uint32_t n_quads;
+	int32_t ray_reorder;          // ymx_accel_params.ray_reorder: 0 auto, 1 always, -1 never
+	int32_t reorder_cell_bits;    // its sort granularity: 3 or 4 bits per axis
 	uint32_t treelet_passes;      // SAH treelet restructuring passes run at build
 	uint32_t treelets_rewritten;  // treelets whose topology they changed
 	// Traversal stack: YMX_LDS_STACK entries in LDS plus spill_depth in HBM,
@@ -96,6 +100,8 @@ struct ymx_trace_args
 	const float4 *tris;
 	const float4 *quads;
 	const ymx_ray *rays;
+	const uint32_t *order;    // optional: the k-th ray traced is rays[order[k]] (ray reordering)
+	const uint32_t *order_changes;  // with order: the batch was reordered when this count is >= the threshold
 	uint32_t n_rays;          // ray count, or the capacity when n_dev is set
 	const uint32_t *n_dev;    // optional device-side ray count (queues)
 	float bmin[3], bmax[3];

@@ -20,6 +20,7 @@
 
 #include <float.h>
 
+
 namespace {
 
 constexpr int kTraceBlock = 128;
@@ -361,6 +362,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(ymx_trace_args args)
 // together, mixing rays of different progress costs more than it gains, so
 // those use k_trace.
 constexpr int kRefill = 12;
+// Ray reordering's coherence test (below): neighbouring pairs sampled, and
+// the count of them changing bucket from which a batch is reordered (1 in 8).
+constexpr uint32_t kCoherenceSamples = 16384;
+constexpr uint32_t kReorderChanges = kCoherenceSamples / 8u;
 
 template <int kMode, bool kStats>
 __global__ __launch_bounds__(kTraceBlock) void k_dyn(ymx_trace_args args)
@@ -377,6 +382,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_dyn(ymx_trace_args args)
 	st.sp = 0;
 	unsigned long long n_fetch = 0, n_tests = 0;
 	const uint32_t n_rays = args.n_dev ? *args.n_dev : args.n_rays;
+	const bool reordered = args.order && *args.order_changes >= kReorderChanges;
 
 	bool alive = false, exhausted = false;
 	uint32_t rid = 0;
@@ -401,6 +407,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_dyn(ymx_trace_args args)
 				if(rid >= n_rays) exhausted = true;
 				else
 				{
+					// with ray reordering the k-th ray traced is the order[k]-th of the
+					// batch; it reads and writes the caller's slot
+					if(reordered) rid = args.order[rid];
 					const ymx_ray ray = load_ray(args.rays, rid);
 					setup_ray(ray, r);
 					t_max = (ray.tmax >= 0.f) ? ray.tmax : FLT_MAX;
@@ -534,6 +543,263 @@ __global__ void k_clear_words(uint32_t *p, uint32_t n)
 	if(i < n) p[i] = 0u;
 }
 
+// ---- ray reordering.  Rays that start near one another and head into the
+// same octant take similar paths through the tree; traced in that order, a
+// wave's lanes stay together longer.  The bucket of a ray: its direction
+// octant (3 bits) above a Morton code of its origin's cell in the tree bound,
+// kCellBits bits per axis (the sort granularity: 3 gives 4,096 buckets, 4
+// gives 32,768).  One counting-sort pass over the buckets gives the order
+// (ray indices bucket by bucket; within a bucket in no fixed order, which
+// changes no result: every ray is traced on its own and writes its own slot),
+// and the trace kernels read rays[order[k]].
+//
+// No host round trip decides whether to reorder: a first kernel counts how
+// many of kCoherenceSamples neighbouring ray pairs change bucket, and every
+// later kernel of the pass, the trace kernel included, reads that count and
+// skips the reordering (or ignores the order) when the batch is coherent
+// already (camera rays in pixel order).  Device work only, on the caller's
+// stream.
+constexpr uint32_t kReorderMin = 1u << 18;
+
+template <int kCellBits>
+struct Bucketing
+{
+	static constexpr uint32_t kBuckets = 8u << (3 * kCellBits);
+	static constexpr uint32_t kBlock = kCellBits <= 3 ? 512u : 1024u;     // histogram and scatter blocks
+	static constexpr uint32_t kTilesMax = kCellBits <= 3 ? 512u : 256u;   // their count per pass (at most)
+};
+
+struct KeyGrid
+{
+	float lo[3], scale[3];
+};
+
+template <int kCellBits>
+__device__ __forceinline__ uint32_t spread_bits(uint32_t v)
+{
+	uint32_t out = 0u;
+	for(int b = 0; b < kCellBits; ++b) out |= ((v >> b) & 1u) << (3 * b);
+	return out;
+}
+
+template <int kCellBits>
+__device__ __forceinline__ uint32_t ray_bucket(const ymx_ray &r, const KeyGrid &g)
+{
+	uint32_t q[3];
+	for(int k = 0; k < 3; ++k)
+	{
+		// NaN and out-of-bound origins clamp to the grid
+		const float c = fminf(fmaxf((r.org[k] - g.lo[k]) * g.scale[k], 0.f), (float)((1 << kCellBits) - 1));
+		q[k] = c == c ? (uint32_t)c : 0u;
+	}
+	const uint32_t oct = (r.dir[0] < 0.f ? 1u : 0u) | (r.dir[1] < 0.f ? 2u : 0u) | (r.dir[2] < 0.f ? 4u : 0u);
+	return (oct << (3 * kCellBits)) | spread_bits<kCellBits>(q[0]) | (spread_bits<kCellBits>(q[1]) << 1) |
+	       (spread_bits<kCellBits>(q[2]) << 2);
+}
+
+// The work area's reordering buffers for one pass of n rays over `tiles`
+// histogram blocks of `tile` rays.
+struct ReorderBufs
+{
+	uint32_t *changes;   // 1 word: changing neighbour pairs among the samples (or kReorderChanges when forced)
+	uint16_t *bucket;    // n: each ray's bucket
+	uint32_t *order;     // n: the ray indices in bucket order
+	uint32_t *table;     // buckets x tiles, bucket-major: rays of bucket b in tile t, then its exclusive offset
+	uint32_t *base;      // buckets: first slot of each bucket
+	uint32_t n, tile, tiles;
+};
+
+__device__ __forceinline__ bool reorder_off(const uint32_t *changes)
+{
+	return *changes < kReorderChanges;
+}
+
+// Neighbouring pairs (i, i + 1) at positions of the golden-ratio (Weyl)
+// sequence over the batch, so that no stride of the batch's own layout (an
+// image row of camera rays) lines up with the samples.
+template <int kCellBits>
+__global__ __launch_bounds__(256) void k_ray_coherence(const ymx_ray *rays, uint32_t n, KeyGrid g, uint32_t *changes)
+{
+	const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+	uint32_t diff = 0u;
+	if(s < kCoherenceSamples)
+	{
+		const uint32_t i = (uint32_t)(((uint64_t)(s * 2654435769u) * (uint64_t)(n - 1u)) >> 32);
+		diff = ray_bucket<kCellBits>(load_ray(rays, i), g) != ray_bucket<kCellBits>(load_ray(rays, i + 1u), g) ? 1u
+		                                                                                                         : 0u;
+	}
+	const int c = __syncthreads_count((int)diff);
+	if(threadIdx.x == 0 && c) atomicAdd(changes, (uint32_t)c);
+}
+
+__global__ void k_set_word(uint32_t *p, uint32_t v)
+{
+	if(threadIdx.x == 0) *p = v;
+}
+
+// Per tile: each ray's bucket, and the tile's count of every bucket.
+template <int kCellBits>
+__global__ __launch_bounds__(Bucketing<kCellBits>::kBlock) void k_bucket_count(const ymx_ray *rays, KeyGrid g,
+                                                                               ReorderBufs b)
+{
+	using B = Bucketing<kCellBits>;
+	if(reorder_off(b.changes)) return;
+	__shared__ uint32_t hist[B::kBuckets];
+	for(uint32_t k = threadIdx.x; k < B::kBuckets; k += B::kBlock) hist[k] = 0u;
+	__syncthreads();
+	const uint64_t first = (uint64_t)blockIdx.x * b.tile;
+	const uint64_t end = first + b.tile < b.n ? first + b.tile : b.n;
+	for(uint64_t i = first + threadIdx.x; i < end; i += B::kBlock)
+	{
+		const uint32_t k = ray_bucket<kCellBits>(load_ray(rays, (uint32_t)i), g);
+		b.bucket[i] = (uint16_t)k;
+		atomicAdd(&hist[k], 1u);
+	}
+	__syncthreads();
+	for(uint32_t k = threadIdx.x; k < B::kBuckets; k += B::kBlock) b.table[k * b.tiles + blockIdx.x] = hist[k];
+}
+
+// One block per bucket: exclusive prefix of its counts over the tiles, in
+// place, and the bucket's total into base[].
+__global__ __launch_bounds__(256) void k_bucket_tiles(ReorderBufs b)
+{
+	if(reorder_off(b.changes)) return;
+	__shared__ uint32_t wsum[4];
+	uint32_t *row = b.table + (size_t)blockIdx.x * b.tiles;
+	const int lane = (int)__lane_id(), wave = (int)(threadIdx.x >> 6);
+	uint32_t carry = 0u;
+	for(uint32_t c0 = 0; c0 < b.tiles; c0 += 256u)
+	{
+		const uint32_t t = c0 + threadIdx.x;
+		const uint32_t v = t < b.tiles ? row[t] : 0u;
+		uint32_t x = v;  // inclusive scan within the wave
+		for(int off = 1; off < 64; off <<= 1)
+		{
+			const uint32_t y = __shfl_up(x, off);
+			if(lane >= off) x += y;
+		}
+		if(lane == 63) wsum[wave] = x;
+		__syncthreads();
+		uint32_t before = carry;
+		for(int w = 0; w < wave; ++w) before += wsum[w];
+		if(t < b.tiles) row[t] = before + x - v;
+		carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+		__syncthreads();
+	}
+	if(threadIdx.x == 0) b.base[blockIdx.x] = carry;
+}
+
+// One block: the buckets' totals into their exclusive prefix (first slots).
+template <int kCellBits>
+__global__ __launch_bounds__(1024) void k_bucket_base(ReorderBufs b)
+{
+	if(reorder_off(b.changes)) return;
+	constexpr uint32_t kPer = Bucketing<kCellBits>::kBuckets / 1024u;
+	__shared__ uint32_t wsum[16];
+	const int lane = (int)__lane_id(), wave = (int)(threadIdx.x >> 6);
+	uint32_t sum = 0u;
+	for(uint32_t k = 0; k < kPer; ++k) sum += b.base[threadIdx.x * kPer + k];
+	uint32_t x = sum;
+	for(int off = 1; off < 64; off <<= 1)
+	{
+		const uint32_t y = __shfl_up(x, off);
+		if(lane >= off) x += y;
+	}
+	if(lane == 63) wsum[wave] = x;
+	__syncthreads();
+	uint32_t run = x - sum;
+	for(int w = 0; w < wave; ++w) run += wsum[w];
+	for(uint32_t k = 0; k < kPer; ++k)
+	{
+		const uint32_t v = b.base[threadIdx.x * kPer + k];
+		b.base[threadIdx.x * kPer + k] = run;
+		run += v;
+	}
+}
+
+// Per tile: each ray's slot is its bucket's first slot + the tile's offset in
+// the bucket + its rank among the tile's rays of the bucket.
+template <int kCellBits>
+__global__ __launch_bounds__(Bucketing<kCellBits>::kBlock) void k_bucket_scatter(ReorderBufs b)
+{
+	using B = Bucketing<kCellBits>;
+	if(reorder_off(b.changes)) return;
+	__shared__ uint32_t next[B::kBuckets];
+	for(uint32_t k = threadIdx.x; k < B::kBuckets; k += B::kBlock)
+		next[k] = b.base[k] + b.table[k * b.tiles + blockIdx.x];
+	__syncthreads();
+	const uint64_t first = (uint64_t)blockIdx.x * b.tile;
+	const uint64_t end = first + b.tile < b.n ? first + b.tile : b.n;
+	for(uint64_t i = first + threadIdx.x; i < end; i += B::kBlock)
+	{
+		const uint32_t slot = atomicAdd(&next[b.bucket[i]], 1u);
+		b.order[slot] = (uint32_t)i;
+	}
+}
+
+template <int kCellBits>
+int reorder_rays_bits(const ymx_accel *acc, ymx_work_area *wa, const ymx_ray *rays, uint32_t n, bool always,
+                      hipStream_t stream, const uint32_t *&order, const uint32_t *&changes)
+{
+	using B = Bucketing<kCellBits>;
+	ReorderBufs b;
+	b.n = n;
+	b.tiles = (n + 4095u) / 4096u;
+	if(b.tiles > B::kTilesMax) b.tiles = B::kTilesMax;
+	b.tile = (n + b.tiles - 1u) / b.tiles;
+	b.tiles = (n + b.tile - 1u) / b.tile;
+	auto up = [](size_t v) { return (v + 255u) / 256u * 256u; };
+	const size_t o_bucket = 256, o_order = o_bucket + up(2ull * n), o_table = o_order + up(4ull * n);
+	const size_t o_base = o_table + up(4ull * B::kBuckets * b.tiles), need = o_base + up(4ull * B::kBuckets);
+	if(wa->sort_bytes < need)
+	{
+		// the previous launches on this area may still read the old buffer
+		YMX_HIP_CHECK(hipStreamSynchronize(stream));
+		if(wa->d_sort) YMX_HIP_CHECK(hipFree(wa->d_sort));
+		wa->d_sort = nullptr;
+		wa->sort_bytes = 0;
+		YMX_HIP_CHECK(hipMalloc(&wa->d_sort, need));
+		wa->sort_bytes = need;
+	}
+	char *base = static_cast<char *>(wa->d_sort);
+	b.changes = reinterpret_cast<uint32_t *>(base);
+	b.bucket = reinterpret_cast<uint16_t *>(base + o_bucket);
+	b.order = reinterpret_cast<uint32_t *>(base + o_order);
+	b.table = reinterpret_cast<uint32_t *>(base + o_table);
+	b.base = reinterpret_cast<uint32_t *>(base + o_base);
+	KeyGrid g;
+	for(int k = 0; k < 3; ++k)
+	{
+		const float e = acc->bmax[k] - acc->bmin[k];
+		g.lo[k] = acc->bmin[k];
+		g.scale[k] = e > 0.f ? (float)(1 << kCellBits) / e : 0.f;
+	}
+	if(always || n < 2u) k_set_word<<<1, 64, 0, stream>>>(b.changes, always ? kReorderChanges : 0u);
+	else
+	{
+		k_clear_words<<<1, 64, 0, stream>>>(b.changes, 1u);
+		k_ray_coherence<kCellBits><<<kCoherenceSamples / 256u, 256, 0, stream>>>(rays, n, g, b.changes);
+	}
+	k_bucket_count<kCellBits><<<b.tiles, B::kBlock, 0, stream>>>(rays, g, b);
+	k_bucket_tiles<<<B::kBuckets, 256, 0, stream>>>(b);
+	k_bucket_base<kCellBits><<<1, 1024, 0, stream>>>(b);
+	k_bucket_scatter<kCellBits><<<b.tiles, B::kBlock, 0, stream>>>(b);
+	YMX_HIP_CHECK(hipGetLastError());
+	order = b.order;
+	changes = b.changes;
+	return YMX_OK;
+}
+
+// Queues the reordering of one launch's rays on `stream` into the work area
+// (grown on demand).  order receives the index array and changes the word the
+// trace kernel reads to know whether the array was written.
+int reorder_rays(const ymx_accel *acc, ymx_work_area *wa, const ymx_ray *rays, uint32_t n, bool always,
+                 hipStream_t stream, const uint32_t *&order, const uint32_t *&changes)
+{
+	if(acc->reorder_cell_bits == 4) return reorder_rays_bits<4>(acc, wa, rays, n, always, stream, order, changes);
+	return reorder_rays_bits<3>(acc, wa, rays, n, always, stream, order, changes);
+}
+
 template <int kMode, bool kStats, bool kDynamic = false>
 int launch(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t *occ, uint32_t *prim, uint32_t n,
            hipStream_t stream, unsigned long long *stats, const uint32_t *n_dev = nullptr,
@@ -572,8 +838,17 @@ int launch(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t *o
 	const uint32_t needed = ((n + 63u) / 64u) * 64u;
 	uint32_t threads = a.grid_threads < needed ? a.grid_threads : needed;
 	const uint32_t blocks = (threads + kTraceBlock - 1) / kTraceBlock;
+	a.order = nullptr;
+	a.order_changes = nullptr;
 	if constexpr(kDynamic && kMode != kTransparent)
 	{
+		const bool reorder = n_dev == nullptr && !kStats &&
+		                     (acc->ray_reorder > 0 || (acc->ray_reorder == 0 && n >= kReorderMin));
+		if(reorder)
+		{
+			const int rc = reorder_rays(acc, wa, rays, n, acc->ray_reorder > 0, stream, a.order, a.order_changes);
+			if(rc != YMX_OK) return rc;
+		}
 		if(kShadow)
 		{
 			// occlusion bits are OR-ed in: clear the mask (a kernel, not a
@@ -659,6 +934,7 @@ void ymx_free_work_areas(ymx_accel *acc)
 	{
 		(void)hipFree(w.d_work);
 		(void)hipFree(w.d_spill);
+		if(w.d_sort) (void)hipFree(w.d_sort);
 		(void)hipEventDestroy(w.done);
 	}
 	acc->work_areas.clear();

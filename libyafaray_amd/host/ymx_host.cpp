@@ -132,6 +132,8 @@ std::unique_ptr<Accelerator> Accelerator::create(const ParamMap &params, const f
 	}
 	p.builder = builder == "ploc" ? YMX_BUILDER_PLOC : YMX_BUILDER_LBVH;
 	p.treelet_passes = params.getInt("treelet_passes", 0);
+	p.ray_reorder = params.getInt("ray_reorder", 0);
+	p.reorder_cell_bits = params.getInt("reorder_cell_bits", 0);
 	std::unique_ptr<Accelerator> a(new Accelerator());
 	const int rc = faces ? ymx_accel_create_faces(verts, n_verts, tri_idx, vis, n_tris, &p, &a->acc_)
 	                     : ymx_accel_create(verts, n_verts, tri_idx, vis, n_tris, &p, &a->acc_);

@@ -29,7 +29,8 @@ class YmxHit(C.Structure):
 
 class YmxAccelParams(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("max_leaf_size", C.c_int32), ("builder", C.c_int32),
-                ("treelet_passes", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("treelet_passes", C.c_int32), ("ray_reorder", C.c_int32), ("reorder_cell_bits", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
 
 BUILDERS = {"lbvh": 0, "ploc": 1}
