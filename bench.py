@@ -11,16 +11,21 @@ A step is one closest-hit pass over the 8,388,608 incoherent rays, with rays
 and BVH already resident in HBM.  The coherent closest-hit and incoherent
 any-hit rates are reported alongside.  The same JSON line carries, under
 `extra`, the path-traced Cornell frame (256 x 256, 16 spp) beside the CPU
-restatement of the reference's integrator, the guided vs plain time to
+restatement of the reference's integrator, the mixed-material terrain frame
+(1920 x 1080, 64 spp, 2.73M triangles), the guided vs plain time to
 relMSE 1e-3 on the occluded room (1280 x 720), and the guiding network's
 kernels on their own.  Prints one JSON line.
 """
 from __future__ import annotations
 
 import argparse
+import csv
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 from pathlib import Path
 
@@ -45,18 +50,27 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--cpu-sample", type=int, default=2_097_152,
                    help="incoherent rays timed on the CPU kd-tree port (0: skip)")
-    p.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cores)")
+    p.add_argument("--cpu-threads", type=int, default=0, help="0: the CPU share of one GPU, min(16, cores)")
     p.add_argument("--res", type=int, default=1024)
     p.add_argument("--n-random", type=int, default=100_000)
     p.add_argument("--quiet", action="store_true")
     p.add_argument("--treelet-passes", type=int, default=0,
                    help="SAH treelet restructuring passes of the ray-cast tree (0: the library default, -1: none)")
     p.add_argument("--builder", type=str, default="lbvh", help="tree builder of the ray-cast batches")
+    p.add_argument("--reorder-cell-bits", type=int, default=0, help="sort granularity of ray reordering (3 or 4)")
+    p.add_argument("--ray-reorder", type=int, default=0,
+                   help="ray reordering of the ray-cast batches (0: library default, 1: always, -1: never)")
     p.add_argument("--pt-treelet-passes", type=int, default=0,
                    help="treelet passes of the path tracer's tree (0: the library default, -1: none)")
+    p.add_argument("--no-counters", action="store_true",
+                   help="skip the rocprofv3 counter passes (traffic, VALU busy, MFMA utilisation)")
+    p.add_argument("--counter-pass", choices=["raycast", "pathtrace", "guide"], default=None,
+                   help="only launch one workload's kernels a few times, for a profiler's counter pass")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run only a plain 1 GiB device-to-device copy (HBM counter calibration)")
-    p.add_argument("--no-pt", action="store_true", help="skip the path-tracing measurement")
+    p.add_argument("--no-pt", action="store_true", help="skip the path-tracing measurements")
+    p.add_argument("--no-terrain", action="store_true", help="skip the 1920x1080x64 terrain frame")
+    p.add_argument("--terrain-only", action="store_true", help="only the 1920x1080x64 terrain frame")
     p.add_argument("--pt-only", action="store_true", help="only the path-tracing measurement")
     p.add_argument("--no-guided", action="store_true", help="skip the guided path-tracing measurement")
     p.add_argument("--guided-only", action="store_true", help="only the guided path-tracing measurement")
@@ -105,7 +119,7 @@ def measure_path_tracer(args, torch):
            "shadow_rays": int(st["shadow_rays"])}
     if args.cpu_sample > 0:
         from oracle import pyoracle
-        threads = args.cpu_threads or max(1, min(16, os.cpu_count() or 1))
+        threads = args.cpu_threads or cpu_share()
         cpu = pyoracle.PathTracer(sc)
         t0 = time.perf_counter()
         cpu.render(threads=threads)
@@ -114,6 +128,43 @@ def measure_path_tracer(args, torch):
         out["cpu_port_threads"] = threads
         out["speedup_vs_cpu_port"] = round(cpu_s / best, 1)
     log(args, f"path tracer: {out}")
+    return out
+
+
+def measure_terrain_frame(args, torch):
+    """The mixed-material terrain frame at its full size: 1024^2 heightfield
+    + 500 icospheres of 1280 triangles, 1920 x 1080, 64 spp, depth 8, 64 point
+    lights + an area light, background 0.1.  One warm-up frame at 1 spp, then
+    the best of two 64-spp frames: render time, samples/s and rays/s."""
+    from libyafaray_amd import scenes
+    from libyafaray_amd.pt import GpuPathTracer
+    sc = scenes.terrain_spheres_scene()
+    cam, st0 = sc["camera"], sc["settings"]
+    tb = time.perf_counter()
+    pt = GpuPathTracer(sc, treelet_passes=args.pt_treelet_passes)
+    setup_s = time.perf_counter() - tb
+    pt.clear()
+    pt.render(0, 1)
+    torch.cuda.synchronize()
+    times, st = [], None
+    for _ in range(2):
+        pt.clear()
+        t0 = time.perf_counter()
+        pt.render(0, st0["aa_samples"])
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        st = pt.stats()
+    pt.close()
+    best = min(times)
+    rays = st["camera_rays"] + st["extension_rays"] + st["shadow_rays"]
+    samples = cam["resx"] * cam["resy"] * st0["aa_samples"]
+    out = {"workload": f"terrain 1024^2 + 500 icospheres ({len(sc['tris'])} tris), {cam['resx']}x{cam['resy']}, "
+                       f"{st0['aa_samples']} spp, depth {st0['bounces']}, {len(sc['lights'])} lights",
+           "render_ms": round(best * 1e3, 1), "msamples_per_s": round(samples / best / 1e6, 1),
+           "rays": int(rays), "mrays_per_s": round(rays / best / 1e6, 1),
+           "camera_rays": int(st["camera_rays"]), "extension_rays": int(st["extension_rays"]),
+           "shadow_rays": int(st["shadow_rays"]), "setup_ms": round(setup_s * 1e3, 1)}
+    log(args, f"terrain frame: {out}")
     return out
 
 
@@ -344,6 +395,178 @@ def calibrate_copy():
     print(json.dumps({"calibrate_copy_bytes": 4 * n, "launches": 3}), flush=True)
 
 
+def cpu_share() -> int:
+    """Host threads for the CPU baselines: the machine's CPU share of one GPU
+    (16), or fewer when fewer CPUs are visible."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def counter_pass(args, torch):
+    """The kernels a counter pass profiles, launched a few times each with
+    nothing else on the device: the ray-cast batches (incoherent closest hit,
+    coherent closest hit, incoherent any hit), one path-traced Cornell frame,
+    or the guide's inference and training step."""
+    from libyafaray_amd import scenes
+    if args.counter_pass == "raycast":
+        from libyafaray_amd.accel import GpuAccelerator
+        verts, tris, coherent, incoherent, shadow = scenes.raycast_workload(args.res, args.n_random)
+        acc = GpuAccelerator(verts, tris, builder=args.builder, treelet_passes=args.treelet_passes,
+                             ray_reorder=args.ray_reorder)
+        d_inc = torch.from_numpy(incoherent).cuda()
+        d_coh = torch.from_numpy(coherent).cuda()
+        d_sha = torch.from_numpy(shadow).cuda()
+        hits = torch.empty((d_inc.shape[0], 4), dtype=torch.int32, device="cuda")
+        occ = torch.empty(((d_inc.shape[0] + 31) // 32,), dtype=torch.int32, device="cuda")
+        for _ in range(3):
+            acc.intersect_device(d_inc, hits)
+            acc.intersect_device(d_coh, hits)
+            acc.intersect_shadow_device(d_sha, occ)
+        torch.cuda.synchronize()
+        acc.close()
+        del d_inc, d_coh, d_sha, hits, occ
+        # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE scale the counters to bytes
+        a = torch.ones(COPY_FLOATS, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+    elif args.counter_pass == "pathtrace":
+        from libyafaray_amd.pt import GpuPathTracer
+        pt = GpuPathTracer(scenes.cornell_path_scene(), treelet_passes=args.pt_treelet_passes)
+        pt.clear()
+        pt.render(0, 16)
+        torch.cuda.synchronize()
+        pt.close()
+    else:
+        measure_guide_kernels(args, torch)
+    print(json.dumps({"counter_pass": args.counter_pass, "done": True}), flush=True)
+
+
+COPY_FLOATS = 1 << 28  # 1 GiB of float32
+
+# rocprofv3 counter passes, each within one run's hardware limits (at most 8
+# SQ and 4 TCC counters: FETCH_SIZE takes 3 TCC, WRITE_SIZE 2).  Derived
+# counters as rocprofv3 defines them: VALUBusy = 100 SQ_ACTIVE_INST_VALU /
+# CU_NUM / GRBM_GUI_ACTIVE, VALUUtilization = 100 SQ_THREAD_CYCLES_VALU /
+# (SQ_ACTIVE_INST_VALU x 64), MfmaUtil = 100 SQ_VALU_MFMA_BUSY_CYCLES /
+# (GRBM_GUI_ACTIVE x SIMD_NUM).
+COUNTER_PASSES = {
+    "valu": ["VALUBusy", "VALUUtilization", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_WAVES", "SQ_INSTS_VMEM_RD",
+             "SQ_INSTS_LDS"],
+    "fetch": ["FETCH_SIZE", "TCC_HIT_sum", "MfmaUtil", "SQ_INSTS_VALU_MFMA_MOPS_BF16"],
+    "write": ["WRITE_SIZE", "TCC_MISS_sum", "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES"],
+}
+
+
+def counter_run(args, workload, counters, regex):
+    """One rocprofv3 counter pass over `bench.py --counter-pass workload` in a
+    child process; returns [(kernel name, {counter: value})] in dispatch order,
+    or None when rocprofv3 is missing or the pass fails."""
+    rocprof = shutil.which("rocprofv3")
+    if not rocprof:
+        return None
+    out = tempfile.mkdtemp(prefix="ymx_pmc_", dir="/tmp")
+    cmd = ["timeout", "-s", "KILL", "180", rocprof, "--pmc", *counters, "--kernel-include-regex", regex, "-d", out,
+           "-o", "run", "--output-format", "csv", "--", sys.executable, str(Path(__file__).resolve()),
+           "--counter-pass", workload, "--builder", args.builder, "--treelet-passes", str(args.treelet_passes),
+           "--pt-treelet-passes", str(args.pt_treelet_passes), "--ray-reorder", str(args.ray_reorder),
+           "--res", str(args.res), "--n-random",
+           str(args.n_random), "--quiet"]
+    env = dict(os.environ, TMPDIR="/tmp")
+    res = subprocess.run(cmd, capture_output=True, text=True, env=env, cwd="/tmp")
+    path = Path(out) / "run_counter_collection.csv"
+    if not path.exists():
+        log(args, f"counter pass {workload} {counters[0]}...: no output (rc {res.returncode}) {res.stderr[-400:]}")
+        return None
+    rows = {}
+    for r in csv.DictReader(path.open()):
+        key = int(r["Dispatch_Id"])
+        rows.setdefault(key, (r["Kernel_Name"], {}))[1][r["Counter_Name"]] = float(r["Counter_Value"])
+    shutil.rmtree(out, ignore_errors=True)
+    return [rows[k] for k in sorted(rows)]
+
+
+def short_kernel(name):
+    """'void (anonymous namespace)::k_dyn<0, false>(ymx_trace_args)' -> 'k_dyn<0, false>'."""
+    n = name.replace("(anonymous namespace)::", "")
+    if n.startswith("void "):
+        n = n[5:]
+    n = n.split("(")[0]
+    base, _, tmpl = n.partition("<")
+    base = base.split("::")[-1]
+    return base + ("<" + tmpl[:40] if tmpl else "")
+
+
+def profile_counters(args):
+    """Counter passes of the ray-cast batches (with the 1 GiB calibration
+    copy), the path-traced Cornell frame and the guide kernels; averages per
+    kernel and batch over the dispatches of each pass."""
+    out = {}
+
+    def add(label, counters):
+        slot = out.setdefault(label, {"dispatches": 0})
+        for k, v in counters.items():
+            slot.setdefault(k, []).append(v)
+
+    # ray cast: dispatches repeat (incoherent closest, coherent closest, any hit), then the copies
+    for pname in ("valu", "fetch", "write"):
+        rows = counter_run(args, "raycast", COUNTER_PASSES[pname], "k_dyn|copy|elementwise")
+        if rows is None:
+            return None
+        order = ["incoherent closest-hit k_dyn<0>", "coherent closest-hit k_dyn<0>", "incoherent any-hit k_dyn<1>"]
+        k = 0
+        for name, c in rows:
+            s = short_kernel(name)
+            if s.startswith("k_dyn"):
+                add(order[k % 3], c)
+                k += 1
+            else:
+                add("copy 1 GiB", c)
+    for workload, regex, passes in (("pathtrace", "k_trace|k_dyn|k_shade|k_subpath|k_resolve|k_camera|k_film",
+                                     ("valu",)),
+                                    ("guide", "k_infer|k_forward|k_backward|k_grid|k_wgrad|k_adam", ("valu", "fetch"))):
+        for pname in passes:
+            rows = counter_run(args, workload, COUNTER_PASSES[pname], regex)
+            if rows is None:
+                continue
+            for name, c in rows:
+                add(f"{workload} {short_kernel(name)}", c)
+    res = {}
+    for label, slot in out.items():
+        r = {}
+        for k, v in slot.items():
+            if k == "dispatches":
+                continue
+            r[k] = float(np.mean(v))
+        r["dispatches"] = max(len(v) for k, v in slot.items() if k != "dispatches")
+        res[label] = r
+    copy = res.get("copy 1 GiB", {})
+    fs = ws = None
+    if copy.get("FETCH_SIZE") and copy.get("WRITE_SIZE"):
+        fs = 4.0 * COPY_FLOATS / (copy["FETCH_SIZE"] * 1024.0)
+        ws = 4.0 * COPY_FLOATS / (copy["WRITE_SIZE"] * 1024.0)
+    for label, r in res.items():
+        if fs and "FETCH_SIZE" in r:
+            r["hbm_read_bytes"] = r["FETCH_SIZE"] * 1024.0 * fs
+        if ws and "WRITE_SIZE" in r:
+            r["hbm_write_bytes"] = r["WRITE_SIZE"] * 1024.0 * ws
+        if "TCC_HIT_sum" in r and "TCC_MISS_sum" in r and r["TCC_HIT_sum"] + r["TCC_MISS_sum"] > 0:
+            r["l2_hit_rate"] = r["TCC_HIT_sum"] / (r["TCC_HIT_sum"] + r["TCC_MISS_sum"])
+    return {"kernels": res, "fetch_scale": fs, "write_scale": ws,
+            "note": "per-dispatch means over rocprofv3 --pmc passes; bytes = counter KiB x 1024 x the factor that "
+                    "maps a 1 GiB device copy's FETCH_SIZE / WRITE_SIZE to 1 GiB"}
+
+
 def log(args, msg):
     if not args.quiet:
         print(msg, file=sys.stderr, flush=True)
@@ -383,8 +606,14 @@ def main():
     if args.calibrate_copy:
         calibrate_copy()
         return
+    if args.counter_pass:
+        counter_pass(args, torch)
+        return
     if args.pt_only:
         print(json.dumps({"path_tracing": measure_path_tracer(args, torch)}), flush=True)
+        return
+    if args.terrain_only:
+        print(json.dumps({"terrain_frame": measure_terrain_frame(args, torch)}), flush=True)
         return
     if args.guided_only:
         print(json.dumps({"guide_kernels": measure_guide_kernels(args, torch),
@@ -395,7 +624,8 @@ def main():
     verts, tris, coherent, incoherent, shadow = scenes.raycast_workload(args.res, args.n_random)
     log(args, f"workload: {len(tris)} tris, {len(coherent)} coherent + {len(incoherent)} incoherent rays "
               f"({time.perf_counter() - t0:.1f}s)")
-    acc = GpuAccelerator(verts, tris, builder=args.builder, treelet_passes=args.treelet_passes)
+    acc = GpuAccelerator(verts, tris, builder=args.builder, treelet_passes=args.treelet_passes,
+                         ray_reorder=args.ray_reorder, reorder_cell_bits=args.reorder_cell_bits)
     info = acc.info()
     log(args, f"{args.builder}: {info['n_nodes']} nodes, depth {info['max_depth']}, build {info['build_ms']:.1f} ms, "
               f"treelet passes {info['treelet_passes']} ({info['treelets_rewritten']} rewritten)")
@@ -422,18 +652,19 @@ def main():
     bytes_per_ray = RAY_BYTES + HIT_BYTES + NODE_BYTES * nodes_per_ray + TRI_BYTES * tris_per_ray
     achieved = bytes_per_ray * n / (ev_ms * 1e-3) / 1e9
 
-    traffic = None
-    tfile = ROOT / "profiles" / "raycast_traffic.json"
-    if tfile.exists():
-        try:
-            traffic = json.loads(tfile.read_text()).get("closest_incoherent_bytes_per_launch")
-        except Exception:
-            traffic = None
+    counters = None if args.no_counters else profile_counters(args)
+    traffic = valu_busy = None
+    if counters:
+        k = counters["kernels"].get("incoherent closest-hit k_dyn<0>", {})
+        if "hbm_read_bytes" in k and "hbm_write_bytes" in k:
+            traffic = int(k["hbm_read_bytes"] + k["hbm_write_bytes"])
+        valu_busy = k.get("VALUBusy")
+        log(args, f"counters: traffic {traffic}, VALU busy {valu_busy}")
 
     cpu = None
     if args.cpu_sample > 0:
         from oracle import pyoracle
-        threads = args.cpu_threads or max(1, min(16, os.cpu_count() or 1))
+        threads = args.cpu_threads or cpu_share()
         sample = incoherent[: args.cpu_sample]
         tb = time.perf_counter()
         kd = pyoracle.KdTree(verts, tris)
@@ -441,10 +672,17 @@ def main():
         tq = time.perf_counter()
         kd.closest(sample, threads=threads)
         q_s = time.perf_counter() - tq
+        ssample = shadow[: args.cpu_sample]
+        ts = time.perf_counter()
+        kd.shadow(ssample, threads=threads)
+        s_s = time.perf_counter() - ts
         cpu = {"value": round(len(sample) / q_s / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
                "sample": f"{len(sample)} of the incoherent rays, closest-hit, C port of the reference kd-tree "
-                         f"(SAH build {build_s:.1f}s not timed)"}
-        log(args, f"cpu kd-tree port: {cpu['value']} Mrays/s on {threads} threads")
+                         f"(SAH build {build_s:.1f}s not timed)",
+               "anyhit_value": round(len(ssample) / s_s / 1e6, 4),
+               "cpu_model": cpu_model(), "logical_cpus_visible": os.cpu_count(),
+               "threads_note": "threads = the host-core share of one GPU on this machine (16), all of it used"}
+        log(args, f"cpu kd-tree port: {cpu['value']} Mrays/s closest, {cpu['anyhit_value']} any hit on {threads} threads")
 
     out = {
         "metric": METRIC,
@@ -473,6 +711,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "valu_busy_pct": None if valu_busy is None else round(valu_busy, 2),
             "bytes_per_ray": round(bytes_per_ray, 1),
             "nodes_per_ray": round(nodes_per_ray, 2),
             "tris_per_ray": round(tris_per_ray, 2),
@@ -493,8 +732,12 @@ def main():
             "bvh_depth": info["max_depth"],
         },
     }
+    if counters:
+        out["extra"]["counters"] = counters
     if not args.no_pt:
         out["extra"]["path_tracing"] = measure_path_tracer(args, torch)
+        if not args.no_terrain:
+            out["extra"]["terrain_frame"] = measure_terrain_frame(args, torch)
     if not args.no_guided:
         out["extra"]["guided_path_tracing"] = measure_guiding(args, torch)
         out["extra"]["guide_kernels"] = measure_guide_kernels(args, torch)

@@ -107,7 +107,7 @@ typedef struct ymx_pt_settings
 	int32_t has_background;       /* constant background seen by camera rays */
 	float background[3];
 	uint32_t seed;                /* seed of the Russian-roulette stream */
-	int32_t raydepth;             /* "raydepth" of the recursive raytrace, default 5 (0..10) */
+	int32_t raydepth;             /* "raydepth" of the recursive raytrace, default 5 (0..19) */
 	int32_t transparent_shadows;  /* "transpShad": shadow rays pass transparent materials, default 0 */
 	int32_t shadow_depth;         /* "shadowDepth": most transparent casters a shadow ray passes, default 4 (0..64) */
 	/* Multi-pass adaptive anti-aliasing (TiledIntegrator::render, ymx_pt_render_passes),

@@ -42,7 +42,10 @@ struct PtMaterial
 // glossy without as_diffuse, mirror, glass.
 YMX_HD bool spawns_children(uint32_t flags) { return (flags & (kGlossy | kSpecular | kFilter)) != 0; }
 
-constexpr int kMaxRayDepth = 10;
+// raydepth accepted: recursive rays stop below level 20 whatever raydepth says
+// (src/integrator/surface/integrator_montecarlo.cc:713), so 19 is the deepest
+// that changes anything; the sampler dimensions 2 level + 2 stay below kLdsDims
+constexpr int kMaxRayDepth = 19;
 constexpr int kGuideCells = 16;  // regions per axis over which training weights are normalised
 constexpr uint32_t kGlossyBranches = 8;  // MonteCarloIntegrator::initial_ray_samples_glossy_
 

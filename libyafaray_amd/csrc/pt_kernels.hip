@@ -771,7 +771,13 @@ __device__ int32_t queue_shadow(const PtArgs &a, V3 from, V3 dir, float tmin, fl
 
 // A slot holds one light sample's contribution when unoccluded.  A zero
 // contribution is stored as no contribution (adding +0 changes no sum) and
-// needs no shadow ray.
+// needs no shadow ray.  Slots are laid out column by column: slot j of path i
+// is entry j x capacity + i, so the lanes of a wave, which shade consecutive
+// paths, write and read each slot column in whole cache lines.
+__device__ __forceinline__ uint32_t slot_at(const PtArgs &a, uint32_t slot0, uint32_t j)
+{
+	return slot0 + j * a.st.capacity;
+}
 __device__ __forceinline__ void set_slot_empty(const PtArgs &a, uint32_t slot)
 {
 	a.st.slot_contrib[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -957,7 +963,7 @@ __device__ void queue_light(const PtArgs &a, const Surf &sp, V3 wo, const PtLigh
 	{
 		const float s1 = h2.next();
 		const float s2 = h3.next();
-		const uint32_t slot = slot0 + (uint32_t)i;
+		const uint32_t slot = slot_at(a, slot0, (uint32_t)i);
 		V3 ldir;
 		float dist, lpdf;
 		if(!illum_sample(a, l, sp.p, s1, s2, ldir, dist, lpdf) || !(lpdf > 1e-6f))
@@ -986,7 +992,7 @@ __device__ void queue_light(const PtArgs &a, const Surf &sp, V3 wo, const PtLigh
 	h3.start(3, offs - 1);
 	for(int i = 0; i < num_samples; ++i)
 	{
-		const uint32_t slot = slot0 + (uint32_t)num_samples + (uint32_t)i;
+		const uint32_t slot = slot_at(a, slot0, (uint32_t)num_samples + (uint32_t)i);
 		BSample s;
 		s.s1 = h2.next();
 		s.s2 = h3.next();
@@ -1082,11 +1088,11 @@ __device__ Rgb resolve_light(const PtArgs &a, const PtLight &l, uint32_t divisio
 	Rgb col = rgb(0.f, 0.f, 0.f);
 	Rgb acc = rgb(0.f, 0.f, 0.f);
 	for(int i = 0; i < num_samples; ++i)
-		if(slot_lit(a, slot0 + (uint32_t)i, c)) acc = cadd(acc, c);
+		if(slot_lit(a, slot_at(a, slot0, (uint32_t)i), c)) acc = cadd(acc, c);
 	col = cadd(col, cscale(acc, inv));
 	acc = rgb(0.f, 0.f, 0.f);
 	for(int i = 0; i < num_samples; ++i)
-		if(slot_lit(a, slot0 + (uint32_t)num_samples + (uint32_t)i, c)) acc = cadd(acc, c);
+		if(slot_lit(a, slot_at(a, slot0, (uint32_t)num_samples + (uint32_t)i), c)) acc = cadd(acc, c);
 	col = cadd(col, cscale(acc, inv));
 	return col;
 }
@@ -1321,11 +1327,11 @@ __global__ __launch_bounds__(kBlock) void k_shade_node(PtArgs a)
 	uint32_t f = 0u;
 	if(sp.flags & kDiffuse)
 	{
-		uint32_t slot = i * a.st.slots;
+		uint32_t slot = i;
 		for(uint32_t l = 0; l < c.n_lights; ++l)
 		{
 			queue_light(a, sp, wo, a.lights[l], l, x.division, x.sample, x.offset, slot);
-			slot += light_slots(a.lights[l], x.division, a.light_mult);
+			slot = slot_at(a, slot, light_slots(a.lights[l], x.division, a.light_mult));
 		}
 		f |= kDirect;
 	}
@@ -1353,11 +1359,11 @@ __global__ __launch_bounds__(kBlock) void k_resolve_node(PtArgs a)
 	const uint32_t n = a.base + i;
 	const uint32_t division = a.nd.info[n].y >> 8;
 	Rgb col_all = rgb(0.f, 0.f, 0.f);
-	uint32_t slot = i * a.st.slots;
+	uint32_t slot = i;
 	for(uint32_t l = 0; l < c.n_lights; ++l)
 	{
 		col_all = cadd(col_all, resolve_light(a, a.lights[l], division, slot));
-		slot += light_slots(a.lights[l], division, a.light_mult);
+		slot = slot_at(a, slot, light_slots(a.lights[l], division, a.light_mult));
 	}
 	const float4 c4 = a.nd.col[n];
 	const Rgb col = cadd(rgb(c4.x, c4.y, c4.z), col_all);
@@ -1504,7 +1510,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_path(PtArgs a)
 			const float hv = halton_next(2, corr - 1u);
 			int lnum = (int)(hv * (float)c.n_lights);
 			if(lnum > (int)c.n_lights - 1) lnum = (int)c.n_lights - 1;
-			queue_light(a, sp, pwo, a.lights[lnum], (uint32_t)lnum, x.division, x.sample, x.offset, i * a.st.slots);
+			queue_light(a, sp, pwo, a.lights[lnum], (uint32_t)lnum, x.division, x.sample, x.offset, i);
 			a.st.light[i] = lnum;
 			f |= kDirect;
 		}
@@ -1580,7 +1586,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve_path(PtArgs a)
 	{
 		const int lnum = a.st.light[i];
 		const uint32_t division = a.nd.info[a.base + i].y >> 8;
-		lcol = cscale(resolve_light(a, a.lights[lnum], division, i * a.st.slots), (float)c.n_lights);
+		lcol = cscale(resolve_light(a, a.lights[lnum], division, i), (float)c.n_lights);
 	}
 	if(f & kAddEmit)
 	{

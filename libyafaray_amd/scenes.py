@@ -19,6 +19,8 @@ Workloads
 """
 from __future__ import annotations
 
+import functools
+
 import numpy as np
 
 F32 = np.float32
@@ -39,6 +41,10 @@ def _hash2(ix: np.ndarray, iy: np.ndarray, seed: int) -> np.ndarray:
 
 
 def value_noise(x: np.ndarray, y: np.ndarray, seed: int) -> np.ndarray:
+    """Bilinear value noise with a smoothstep fade over the integer lattice of
+    _hash2.  The lattice values are hashed once per lattice point of the
+    covered range and gathered, which gives the same numbers as hashing every
+    corner of every sample."""
     ix = np.floor(x)
     iy = np.floor(y)
     fx = x - ix
@@ -47,10 +53,20 @@ def value_noise(x: np.ndarray, y: np.ndarray, seed: int) -> np.ndarray:
     iy = iy.astype(np.int64)
     sx = fx * fx * (3.0 - 2.0 * fx)
     sy = fy * fy * (3.0 - 2.0 * fy)
-    v00 = _hash2(ix, iy, seed)
-    v10 = _hash2(ix + 1, iy, seed)
-    v01 = _hash2(ix, iy + 1, seed)
-    v11 = _hash2(ix + 1, iy + 1, seed)
+    x0, y0 = int(ix.min()), int(iy.min())
+    lx, ly = np.meshgrid(np.arange(x0, int(ix.max()) + 2), np.arange(y0, int(iy.max()) + 2), indexing="xy")
+    lattice = _hash2(lx, ly, seed)
+    cx, cy = ix - x0, iy - y0
+    if x.ndim == 2 and y.ndim == 2 and x.shape[0] == 1 and y.shape[1] == 1:
+        # x along the columns, y along the rows (a separable grid): gather rows, then columns
+        r0, r1 = lattice[cy[:, 0]], lattice[cy[:, 0] + 1]
+        v00, v10 = r0[:, cx[0]], r0[:, cx[0] + 1]
+        v01, v11 = r1[:, cx[0]], r1[:, cx[0] + 1]
+    else:
+        v00 = lattice[cy, cx]
+        v10 = lattice[cy, cx + 1]
+        v01 = lattice[cy + 1, cx]
+        v11 = lattice[cy + 1, cx + 1]
     a = v00 + (v10 - v00) * sx
     b = v01 + (v11 - v01) * sx
     return a + (b - a) * sy
@@ -59,8 +75,8 @@ def value_noise(x: np.ndarray, y: np.ndarray, seed: int) -> np.ndarray:
 def fractal_heights(res: int, octaves: int = 6, amplitude: float = 0.2, base_freq: float = 4.0,
                     seed: int = 0) -> np.ndarray:
     t = np.linspace(0.0, 1.0, res)
-    x, y = np.meshgrid(t, t, indexing="xy")
-    h = np.zeros_like(x)
+    x, y = t[None, :], t[:, None]  # the "xy" meshgrid, broadcast
+    h = np.zeros((res, res))
     amp = 1.0
     freq = base_freq
     for o in range(octaves):
@@ -450,12 +466,46 @@ def occluded_room_scene(res=(1280, 720), spp: int = 16, seed: int = 0, wall_cell
                 settings=settings)
 
 
-def icosphere_tris(center, radius: float, subdiv: int = 2) -> tuple[np.ndarray, np.ndarray]:
-    """Sphere mesh: an octahedron with each face split into 4 `subdiv` times and
-    its vertices pushed to the sphere (8 * 4^subdiv triangles, outward face
-    normals)."""
-    v = [np.array(p, np.float64) for p in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1))]
-    f = [(0, 2, 4), (2, 1, 4), (1, 3, 4), (3, 0, 4), (2, 0, 5), (1, 2, 5), (3, 1, 5), (0, 3, 5)]
+def _icosahedron():
+    """The 12 vertices (0, +-1, +-phi) and their cyclic permutations, scaled to
+    the unit sphere, and the 20 faces: the vertex triples at mutual distance 2
+    (the edge length), each ordered so its normal points outward."""
+    phi = (1.0 + 5.0 ** 0.5) / 2.0
+    raw = []
+    for a in (-1.0, 1.0):
+        for b in (-phi, phi):
+            raw += [(0.0, a, b), (a, b, 0.0), (b, 0.0, a)]
+    p = np.array(raw)
+    d = np.linalg.norm(p[:, None] - p[None], axis=-1)
+    edge = np.abs(d - 2.0) < 1e-9
+    faces = []
+    for i in range(12):
+        for j in range(i + 1, 12):
+            for k in range(j + 1, 12):
+                if edge[i, j] and edge[j, k] and edge[i, k]:
+                    n = np.cross(p[j] - p[i], p[k] - p[i])
+                    faces.append((i, j, k) if n @ (p[i] + p[j] + p[k]) > 0 else (i, k, j))
+    assert len(faces) == 20
+    return [q / np.linalg.norm(q) for q in p], faces
+
+
+def icosphere_tris(center, radius: float, subdiv: int = 2, base: str = "octahedron") -> tuple[np.ndarray, np.ndarray]:
+    """Sphere mesh: a base solid with each face split into 4 `subdiv` times and
+    its vertices pushed to the sphere, outward face normals.  base
+    "octahedron": 8 * 4^subdiv triangles; "icosahedron": 20 * 4^subdiv (1280
+    at subdiv 3)."""
+    v, f = _unit_sphere(subdiv, base)
+    verts = (v * radius + np.asarray(center, np.float64)).astype(F32)
+    return verts, f.copy()
+
+
+@functools.lru_cache(maxsize=None)
+def _unit_sphere(subdiv: int, base: str):
+    if base == "icosahedron":
+        v, f = _icosahedron()
+    else:
+        v = [np.array(p, np.float64) for p in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1))]
+        f = [(0, 2, 4), (2, 1, 4), (1, 3, 4), (3, 0, 4), (2, 0, 5), (1, 2, 5), (3, 1, 5), (0, 3, 5)]
     for _ in range(subdiv):
         mid, nf = {}, []
 
@@ -470,8 +520,10 @@ def icosphere_tris(center, radius: float, subdiv: int = 2) -> tuple[np.ndarray, 
             ab, bc, ca = m(a, b), m(b, c), m(c, a)
             nf += [(a, ab, ca), (ab, b, bc), (ca, bc, c), (ab, bc, ca)]
         f = nf
-    verts = (np.array(v) * radius + np.asarray(center, np.float64)).astype(F32)
-    return verts, np.array(f, np.uint32)
+    v, f = np.array(v), np.array(f, np.uint32)
+    v.flags.writeable = False
+    f.flags.writeable = False
+    return v, f
 
 
 def object_light_scene(n_random: int = 2000, seed: int = 0, res: int = 64, spp: int = 4,
@@ -523,13 +575,14 @@ def object_light_scene(n_random: int = 2000, seed: int = 0, res: int = 64, spp: 
 
 
 def terrain_spheres_scene(res=(1920, 1080), spp: int = 64, seed: int = 0, hf_res: int = 1024, n_spheres: int = 500,
-                          subdiv: int = 3, n_point_lights: int = 64, depth: int = 8):
+                          subdiv: int = 3, n_point_lights: int = 64, depth: int = 8, area_lights=None):
     """Mixed-material path-tracing workload: the ray-cast heightfield
-    (`hf_res`^2 vertices) with `n_spheres` icospheres above it (8 * 4^subdiv
-    triangles each), sphere materials drawn 40 % shinydiffuse / 30 % glossy
+    (`hf_res`^2 vertices) with `n_spheres` icospheres above it (20 * 4^subdiv
+    triangles each: 1280 at the default subdiv 3), sphere materials drawn 40 % shinydiffuse / 30 % glossy
     (roughness r ~ U[0.05, 0.5], Blinn exponent 2 / r^2 - 2) / 15 % mirror /
     15 % glass (IOR 1.5), `n_point_lights` point lights of power log-U[1, 100],
-    one area light and a constant background of 0.1; path depth `depth`.  The
+    one area light (or the `area_lights` given, as (corner, point1, point2,
+    power) tuples) and a constant background of 0.1; path depth `depth`.  The
     defaults are the full frame; tests use a small one."""
     rng = np.random.default_rng(seed)
     hf = grid_mesh(fractal_heights(hf_res, seed=seed))
@@ -537,7 +590,7 @@ def terrain_spheres_scene(res=(1920, 1080), spp: int = 64, seed: int = 0, hf_res
     materials = [dict(color=(0.5, 0.45, 0.4))]
     for k in range(n_spheres):
         c = (float(rng.uniform(0.1, 0.9)), float(rng.uniform(0.1, 0.9)), float(rng.uniform(0.3, 0.5)))
-        parts.append(icosphere_tris(c, float(rng.uniform(0.02, 0.05)), subdiv))
+        parts.append(icosphere_tris(c, float(rng.uniform(0.02, 0.05)), subdiv, base="icosahedron"))
         mats.append(len(materials))
         u, a = rng.uniform(), float(rng.uniform(0.2, 0.8))
         if u < 0.4:
@@ -553,8 +606,10 @@ def terrain_spheres_scene(res=(1920, 1080), spp: int = 64, seed: int = 0, hf_res
             materials.append(dict(type="glass", IOR=1.5))
     verts, tris = merge(parts)
     mat_id = np.concatenate([np.full(len(t), m, np.uint32) for (_, t), m in zip(parts, mats)])
-    lights = [dict(type="area", corner=(0.4, 0.4, 1.2), point1=(0.4, 0.6, 1.2), point2=(0.6, 0.4, 1.2),
-                   color=(1.0, 1.0, 1.0), power=20.0 / np.pi, samples=4)]
+    if area_lights is None:
+        area_lights = [((0.4, 0.4, 1.2), (0.4, 0.6, 1.2), (0.6, 0.4, 1.2), 20.0 / np.pi)]
+    lights = [dict(type="area", corner=c, point1=p1, point2=p2, color=(1.0, 1.0, 1.0), power=float(pw), samples=4)
+              for c, p1, p2, pw in area_lights]
     for k in range(n_point_lights):
         p = (float(rng.uniform(0.0, 1.0)), float(rng.uniform(0.0, 1.0)), float(rng.uniform(0.6, 1.0)))
         lights.append(dict(type="point", **{"from": p}, color=(1.0, 0.95, 0.9),
@@ -565,6 +620,20 @@ def terrain_spheres_scene(res=(1920, 1080), spp: int = 64, seed: int = 0, hf_res
                     background=(0.1, 0.1, 0.1), seed=seed, raydepth=depth)
     return dict(verts=verts, tris=tris, mat_id=mat_id, materials=materials, lights=lights, camera=camera,
                 settings=settings)
+
+
+def large_terrain_scene(res=(3840, 2160), spp: int = 256, seed: int = 0, hf_res: int = 4096, n_spheres: int = 2000,
+                        n_point_lights: int = 256, depth: int = 12):
+    """Large-scene workload: a `hf_res`^2 fractal heightfield (33.5M triangles
+    at 4096) with `n_spheres` 1280-triangle icospheres (2.56M at 2000), the
+    mixed material draw of terrain_spheres_scene, `n_point_lights` point
+    lights, and two downward-facing 0.2 x 0.2 area lights hung low over the
+    terrain (z = 0.12 and 0.16, below its highest ridges at 0.2), so that the
+    terrain hides them from part of the scene; path depth `depth`."""
+    area = [((0.2, 0.25, 0.12), (0.2, 0.45, 0.12), (0.4, 0.25, 0.12), 8.0 / np.pi),
+            ((0.6, 0.6, 0.16), (0.6, 0.8, 0.16), (0.8, 0.6, 0.16), 8.0 / np.pi)]
+    return terrain_spheres_scene(res=res, spp=spp, seed=seed, hf_res=hf_res, n_spheres=n_spheres, subdiv=3,
+                                 n_point_lights=n_point_lights, depth=depth, area_lights=area)
 
 
 def smooth_spheres_scene(res: int = 64, spp: int = 4, seed: int = 0, subdiv: int = 2, flat_every: int = 3):

@@ -1628,13 +1628,15 @@ static void *pass_worker(void *arg)
 {
 	pass_job *j = (pass_job *)arg;
 	const ora_pt_settings *st = j->c->st;
-	for(int y = j->tid; y < st->resy; y += j->nthreads)
-		for(int x = 0; x < st->resx; ++x)
-		{
-			const size_t k = (size_t)y * st->resx + x;
-			if(j->flags && !j->flags[k]) continue;
-			pixel_sample(j->c, x, y, j->sample, j->cols + 4 * k, &j->dxy[2 * k], &j->dxy[2 * k + 1]);
-		}
+	/* pixels dealt out one by one, so a sparse flag set (a window) still
+	 * spreads over every thread; the film is splatted afterwards in pixel order */
+	const size_t npix = (size_t)st->resx * st->resy;
+	for(size_t k = (size_t)j->tid; k < npix; k += (size_t)j->nthreads)
+	{
+		if(j->flags && !j->flags[k]) continue;
+		const int x = (int)(k % (size_t)st->resx), y = (int)(k / (size_t)st->resx);
+		pixel_sample(j->c, x, y, j->sample, j->cols + 4 * k, &j->dxy[2 * k], &j->dxy[2 * k + 1]);
+	}
 	return NULL;
 }
 
@@ -1909,6 +1911,28 @@ void ora_pt_render(const ora_kdtree *kd, const float *verts, const uint32_t *tri
 	free(cols);
 	free(dxy);
 	free(flags);
+	free_ctx(&c);
+}
+
+void ora_pt_render_pixels(const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx, const uint32_t *mat_id,
+                          const uint8_t *vis, const ora_material *mats, const ora_light *lights, uint32_t n_lights,
+                          const ora_pt_settings *st, const uint8_t *flags, float *film_rgba, float *film_w,
+                          int threads)
+{
+	pt_ctx c;
+	if(!setup_ctx(&c, kd, verts, tri_idx, mat_id, vis, mats, lights, n_lights, st)) return;
+	float table[256], table_scale, filter_width;
+	ora_filter_table(st->aa_pixelwidth, table, &table_scale, &filter_width);
+	const size_t npix = (size_t)st->resx * st->resy;
+	float *cols = (float *)malloc(npix * 4 * sizeof(float));
+	float *dxy = (float *)malloc(npix * 2 * sizeof(float));
+	if(!cols || !dxy) { free(cols); free(dxy); free_ctx(&c); return; }
+	if(threads < 1) threads = 1;
+	if(threads > 256) threads = 256;
+	c.light_mult = 1.f;
+	render_pass(&c, 0, st->aa_samples, flags, table, table_scale, filter_width, cols, dxy, film_rgba, film_w, threads);
+	free(cols);
+	free(dxy);
 	free_ctx(&c);
 }
 

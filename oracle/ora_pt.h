@@ -155,6 +155,17 @@ void ora_pt_render(const ora_kdtree *kd, const float *verts, const uint32_t *tri
                    const uint8_t *vis, const ora_material *mats, const ora_light *lights, uint32_t n_lights,
                    const ora_pt_settings *st, float *film_rgba, float *film_w, int threads);
 
+/* The first pass of ora_pt_render (aa_samples samples, light multiplier 1)
+ * over the pixels with flags[y * resx + x] set only: each of their samples is
+ * splatted into the full-size film as in ora_pt_render, so a pixel whose
+ * neighbours within the filter width are all flagged gets the sums a full
+ * render gives it.  For checks on windows of frames too large to render on
+ * the CPU. */
+void ora_pt_render_pixels(const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx, const uint32_t *mat_id,
+                          const uint8_t *vis, const ora_material *mats, const ora_light *lights, uint32_t n_lights,
+                          const ora_pt_settings *st, const uint8_t *flags, float *film_rgba, float *film_w,
+                          int threads);
+
 /* Same, but only the sample values of one pixel sample (no film), for
  * sample-level checks: col_alpha[4], and the sub-pixel offsets dx, dy. */
 void ora_pt_pixel_sample(const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx, const uint32_t *mat_id,

@@ -323,11 +323,11 @@ static void *brute_worker(void *arg)
 	return NULL;
 }
 
-static void brute_run(brute_job *proto, int threads)
+static void brute_run(brute_job *proto, int threads, int bound_given)
 {
 	if(threads < 1) threads = 1;
 	if(threads > 256) threads = 256;
-	ora_tree_bound_faces(proto->verts, proto->idx, proto->stride, proto->n_tris, proto->a, proto->g);
+	if(!bound_given) ora_tree_bound_faces(proto->verts, proto->idx, proto->stride, proto->n_tris, proto->a, proto->g);
 	brute_job jobs[256];
 	pthread_t th[256];
 	for(int i = 0; i < threads; ++i)
@@ -348,7 +348,7 @@ void ora_closest_brute_faces(const float *verts, const uint32_t *idx, uint32_t s
 	memset(&j, 0, sizeof j);
 	j.verts = verts; j.idx = idx; j.stride = stride; j.vis = vis; j.n_tris = n_tris;
 	j.rays = rays; j.n_rays = n_rays; j.hits = hits; j.shadow = 0;
-	brute_run(&j, threads);
+	brute_run(&j, threads, 0);
 }
 
 void ora_shadow_brute_faces(const float *verts, const uint32_t *idx, uint32_t stride, const uint8_t *vis,
@@ -358,7 +358,19 @@ void ora_shadow_brute_faces(const float *verts, const uint32_t *idx, uint32_t st
 	memset(&j, 0, sizeof j);
 	j.verts = verts; j.idx = idx; j.stride = stride; j.vis = vis; j.n_tris = n_tris;
 	j.rays = rays; j.n_rays = n_rays; j.occluded = occluded; j.shadow = 1;
-	brute_run(&j, threads);
+	brute_run(&j, threads, 0);
+}
+
+void ora_closest_brute_in_bound(const float *verts, const uint32_t *idx, uint32_t stride, const uint8_t *vis,
+                                uint32_t n_faces, const float a[3], const float g[3], const ora_ray *rays,
+                                uint32_t n_rays, ora_hit *hits, int threads)
+{
+	brute_job j;
+	memset(&j, 0, sizeof j);
+	j.verts = verts; j.idx = idx; j.stride = stride; j.vis = vis; j.n_tris = n_faces;
+	j.rays = rays; j.n_rays = n_rays; j.hits = hits; j.shadow = 0;
+	for(int k = 0; k < 3; ++k) { j.a[k] = a[k]; j.g[k] = g[k]; }
+	brute_run(&j, threads, 1);
 }
 
 void ora_closest_brute(const float *verts, const uint32_t *tri_idx, const uint8_t *vis,

@@ -100,6 +100,15 @@ void ora_shadow_brute_faces(const float *verts, const uint32_t *idx, uint32_t st
 void ora_retest_faces(const float *verts, const uint32_t *idx, uint32_t stride, const ora_ray *rays,
                       const uint32_t *prim, uint32_t n_rays, ora_hit *out);
 
+/* Closest hit by brute force over a subset of a scene's faces, with the
+ * whole scene's tree bound a, g (ora_tree_bound of all faces): the bound the
+ * queries clip to and take their dynamic bias from.  When the subset holds
+ * every face a ray segment can reach, this is the whole scene's answer (with
+ * ids into the subset). */
+void ora_closest_brute_in_bound(const float *verts, const uint32_t *idx, uint32_t stride, const uint8_t *vis,
+                                uint32_t n_faces, const float a[3], const float g[3], const ora_ray *rays,
+                                uint32_t n_rays, ora_hit *hits, int threads);
+
 /* Re-run the triangle test for given (ray, primitive) pairs: out[i] is the
  * ora_hit of ray i against primitive prim[i] (prim[i] == ORA_NO_HIT: skipped,
  * out[i] = {ORA_NO_HIT, 0, 0, 0}). */

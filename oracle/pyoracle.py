@@ -34,6 +34,8 @@ def lib() -> C.CDLL:
         L.ora_tree_bound_faces.argtypes = [vp, vp, C.c_uint32, C.c_uint32, fp, fp]
         L.ora_closest_brute_faces.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32, vp, C.c_uint32, vp, C.c_int]
         L.ora_shadow_brute_faces.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32, vp, C.c_uint32, vp, C.c_int]
+        L.ora_closest_brute_in_bound.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32, fp, fp, vp, C.c_uint32, vp,
+                                                 C.c_int]
         L.ora_retest_faces.argtypes = [vp, vp, C.c_uint32, vp, vp, C.c_uint32, vp]
         L.ora_kd_build_faces.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32]
         L.ora_kd_build_faces.restype = vp
@@ -122,6 +124,20 @@ def closest_brute(verts, tris, rays, vis=None, threads=None) -> dict:
     lib().ora_closest_brute_faces(verts.ctypes.data, tris.ctypes.data, tris.shape[1], vis.ctypes.data,
                                   tris.shape[0], rays.ctypes.data, rays.shape[0], out.ctypes.data,
                                   threads or default_threads())
+    return _hits(out)
+
+
+def closest_brute_in_bound(verts, tris, rays, bound, vis=None, threads=None) -> dict:
+    """closest_brute over the faces given, clipped to and biased by the
+    scene's tree bound (a, g) = tree_bound(all faces) instead of theirs."""
+    verts, tris, vis = _prep(verts, tris, vis)
+    rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+    out = np.zeros((rays.shape[0], 4), dtype=np.uint32)
+    a = (C.c_float * 3)(*[float(x) for x in bound[0]])
+    g = (C.c_float * 3)(*[float(x) for x in bound[1]])
+    lib().ora_closest_brute_in_bound(verts.ctypes.data, tris.ctypes.data, tris.shape[1], vis.ctypes.data,
+                                     tris.shape[0], a, g, rays.ctypes.data, rays.shape[0], out.ctypes.data,
+                                     threads or default_threads())
     return _hits(out)
 
 
@@ -371,6 +387,7 @@ def _pt_lib():
     if not getattr(L, "_pt_ready", False):
         vp = C.c_void_p
         L.ora_pt_render.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_uint32, vp, vp, vp, C.c_int]
+        L.ora_pt_render_pixels.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_uint32, vp, vp, vp, vp, C.c_int]
         L.ora_pt_pixel_sample.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_uint32, vp, C.c_int, C.c_int, C.c_int,
                                           vp, vp]
         L.ora_object_light_setup.argtypes = [vp, vp, vp, vp]
@@ -432,6 +449,21 @@ class PathTracer:
         L.ora_pt_render(self.kd._h, self.verts.ctypes.data, self.tris.ctypes.data, self.mat_id.ctypes.data,
                         self.vis.ctypes.data, C.addressof(self.mats), C.addressof(self.lights), self.n_lights, C.addressof(st),
                         rgba.ctypes.data, w.ctypes.data, threads or default_threads())
+        return rgba, w
+
+    def render_pixels(self, flags, threads=None):
+        """Pass 0 over the pixels where flags (resy, resx) is set, splatted
+        into a full-size film (ora_pt_render_pixels)."""
+        L = _pt_lib()
+        st = self.st
+        flags = np.ascontiguousarray(flags, dtype=np.uint8)
+        assert flags.shape == (st.resy, st.resx)
+        rgba = np.zeros((st.resy, st.resx, 4), dtype=np.float32)
+        w = np.zeros((st.resy, st.resx), dtype=np.float32)
+        L.ora_pt_render_pixels(self.kd._h, self.verts.ctypes.data, self.tris.ctypes.data, self.mat_id.ctypes.data,
+                               self.vis.ctypes.data, C.addressof(self.mats), C.addressof(self.lights), self.n_lights,
+                               C.addressof(st), flags.ctypes.data, rgba.ctypes.data, w.ctypes.data,
+                               threads or default_threads())
         return rgba, w
 
     def pixel_sample(self, x, y, sample):

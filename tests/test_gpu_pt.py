@@ -162,8 +162,9 @@ def test_glossy_branches_match_oracle(oracle):
 
 
 def test_specular_depths_match_oracle(oracle):
-    """The glass cube and mirror panel alone, at every ray depth from 0 to 6."""
-    for rd in (0, 1, 2, 3, 6):
+    """The glass cube and mirror panel alone, at ray depths from 0 to 19 (the
+    deepest that recurses: levels stop below 20)."""
+    for rd in (0, 1, 2, 3, 6, 12, 19):
         sc = scenes.cornell_materials_scene(n_random=0, res=48, spp=2, seed=6, raydepth=rd, point_lights=1)
         say(f"pt specular raydepth {rd}")
         g = render_gpu(sc)

@@ -302,6 +302,43 @@ def test_kdtree_port_equals_brute(oracle, seed):
     assert np.array_equal(occ_b, occ_k)
 
 
+def test_brute_in_bound_equals_whole_scene(oracle):
+    """Brute force over a culled subset in the whole scene's bound equals
+    brute force over the whole scene for short segments the subset holds
+    (ids mapped back).  The subset's own bound differs from the scene's, and
+    the queries clip to the bound and scale their bias by its chord, so the
+    scene's bound is the one to pass."""
+    verts, tris = scenes.heightfield_scene(res=64, n_random=800, seed=5)
+    rng = np.random.default_rng(5)
+    # origins near the terrain surface of a small patch: centroids of its
+    # triangles moved up or down by at most 0.02
+    cen = verts[tris[: 2 * 63 * 63]].mean(1)
+    patch = cen[(np.abs(cen[:, 0] - 0.42) < 0.03) & (np.abs(cen[:, 1] - 0.52) < 0.03)]
+    org = patch[rng.integers(0, len(patch), 3000)] + np.stack(
+        [np.zeros(3000), np.zeros(3000), rng.uniform(-0.02, 0.02, 3000)], 1)
+    rays = scenes.incoherent_rays(rng, 3000, (0, 0, 0), (1, 1, 1))
+    rays[:, 0:3] = org
+    rays[:, 7] = rng.uniform(0.001, 0.05, 3000)
+    full = oracle.closest_brute(verts, tris, rays)
+    ends = rays[:, 0:3] + rays[:, 4:7] * rays[:, 7:8]
+    lo = np.minimum(rays[:, 0:3], ends).min(0) - 1e-4
+    hi = np.maximum(rays[:, 0:3], ends).max(0) + 1e-4
+    tv = verts[tris]
+    sub = np.nonzero(np.all((tv.max(1) >= lo) & (tv.min(1) <= hi), axis=1))[0]
+    assert 0 < len(sub) < len(tris) // 4
+    part = oracle.closest_brute_in_bound(verts, tris[sub], rays, oracle.tree_bound(verts, tris))
+    hit = part["prim"] != 0xFFFFFFFF
+    assert 0.2 < hit.mean() < 0.95
+    mapped = np.where(hit, sub[np.where(hit, part["prim"], 0)], 0xFFFFFFFF)
+    assert np.array_equal(mapped, full["prim"])
+    for k in ("t", "u", "v"):
+        assert np.array_equal(part[k].view(np.uint32), full[k].view(np.uint32))
+    # the subset's own bound is another bound (so closest_brute over the subset would clip and bias differently)
+    a_sub, g_sub = oracle.tree_bound(verts, tris[sub])
+    a_all, g_all = oracle.tree_bound(verts, tris)
+    assert np.any(a_sub != a_all) and np.any(g_sub != g_all)
+
+
 def test_retest_reproduces_hits(oracle):
     verts, tris = scenes.heightfield_scene(res=32, n_random=300, seed=2)
     rng = np.random.default_rng(2)
