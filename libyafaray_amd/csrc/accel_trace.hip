@@ -95,20 +95,114 @@ struct LaneStack
 	}
 };
 
-// End of a persistent wave: the last wave of the launch to finish resets the
-// ray counter and the finished-wave counter, so the next launch on the same
-// work area starts from zero without a reset of its own.  Every other wave
-// has made its last ray fetch (an atomic whose result it waited for) before
-// its increment here, so none fetches after the reset.
+// ---- work distribution of the persistent grids.  One counter shared by the
+// whole grid serialises: device-scope atomics on one address complete at about
+// 90 M/s on this part (11 ns each), against about 16 G/s spread over many
+// addresses, so 8,192 waves claiming 64 rays at a time queue at it, and even
+// an empty launch costs each wave's claim and finish in turn (about 0.2 ms).
+// The batch is cut into kSegments contiguous segments, each with its own
+// claim counter a page apart from the others.  A wave starts at a home
+// segment among the kSegments / 8 of its XCD's eighth of the batch (blocks
+// are dealt to the 8 XCDs in turn, so one XCD's L2 holds the nodes that its
+// contiguous share of rays visits).  When that one is used up, the wave reads
+// every segment's counter at once (one load per lane) and moves to the next
+// segment with rays left, so it claims only where claims can succeed.
+//
+// Work area words (a page, kWorkStride words, apart): slot s < kSegments:
+// units claimed from segment s; slot kSegments + g: waves finished in block
+// group g = blockIdx % kSegments; slot 2 kSegments: groups finished.  The
+// last wave of the launch resets them all, so the next launch on the work
+// area starts from zero.
+constexpr uint32_t kSegments = 64;  // one per lane of the wave that scans them
+static_assert(kTraceBlock % 64 == 0, "waves of 64 lanes");
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kWorkStride = 1024;
+constexpr uint32_t kWorkSlots = 2 * kSegments + 1;
+constexpr size_t kWorkWords = (size_t)kWorkSlots * kWorkStride;
+
+__device__ __forceinline__ uint32_t seg_lo(uint32_t s, uint32_t n_units)
+{
+	return (uint32_t)(((uint64_t)n_units * s) / kSegments);
+}
+
+struct WaveWork
+{
+	uint32_t seg;   // segment the wave claims from
+	bool done;      // every segment used up
+};
+
+__device__ __forceinline__ WaveWork wave_work_start(uint32_t n_units)
+{
+	constexpr uint32_t per_xcd = kSegments / kXcds;
+	const uint32_t xcd = blockIdx.x % kXcds;
+	const uint32_t j = (blockIdx.x / kXcds) * (uint32_t)(kTraceBlock / 64) + (threadIdx.x >> 6);
+	return WaveWork{xcd * per_xcd + j % per_xcd, n_units == 0u};
+}
+
+// After segment ww.seg ran out: lane k reads segment k's counter, and the
+// wave moves to the first segment after ww.seg (cyclically) with units left,
+// or sets done.  Every lane computes the same choice from the ballot.
+__device__ __forceinline__ void next_segment(const uint32_t *work, WaveWork &ww, uint32_t n_units)
+{
+	const uint32_t k = __lane_id();
+	const uint32_t claimed = __hip_atomic_load(work + (size_t)k * kWorkStride, __ATOMIC_RELAXED,
+	                                           __HIP_MEMORY_SCOPE_AGENT);
+	const bool left = claimed < seg_lo(k + 1u, n_units) - seg_lo(k, n_units) && k != ww.seg;
+	const unsigned long long free_bits = __ballot(left);
+	if(free_bits == 0ull)
+	{
+		ww.done = true;
+		return;
+	}
+	const uint32_t start = (ww.seg + 1u) % kSegments;
+	const unsigned long long rot = (free_bits >> start) | (start ? (free_bits << (64u - start)) : 0ull);
+	ww.seg = (start + (uint32_t)(__ffsll((long long)rot) - 1)) % kSegments;
+}
+
+// Claims up to `want` consecutive units of n_units for the wave: returns the
+// first and sets got, 0 once every segment is used up.  Called with every
+// lane of the wave active; lane 0 makes the atomic.
+__device__ __forceinline__ uint32_t wave_claim(uint32_t *work, WaveWork &ww, uint32_t n_units, uint32_t want,
+                                               uint32_t &got)
+{
+	while(!ww.done)
+	{
+		const uint32_t lo = seg_lo(ww.seg, n_units);
+		const uint32_t size = seg_lo(ww.seg + 1u, n_units) - lo;
+		uint32_t b = size;
+		if(__lane_id() == 0 && size > 0u) b = atomicAdd(work + (size_t)ww.seg * kWorkStride, want);
+		b = __shfl(b, 0);
+		if(b < size)
+		{
+			got = size - b < want ? size - b : want;
+			if(b + want >= size) next_segment(work, ww, n_units);
+			return lo + b;
+		}
+		next_segment(work, ww, n_units);
+	}
+	got = 0u;
+	return 0u;
+}
+
+// End of a persistent wave.  Waves count themselves finished per block group
+// (blockIdx % kSegments), the last of each group counts the group, and the
+// last group's resets every counter.  A wave finishes after its last claim
+// (an atomic whose result it waited for), so nothing of this launch claims
+// after the reset.
 __device__ __forceinline__ void finish_wave(uint32_t *work)
 {
 	if(__lane_id() == 0)
 	{
-		const uint32_t waves = gridDim.x * (uint32_t)(kTraceBlock / 64);
-		if(atomicAdd(work + 1, 1u) == waves - 1u)
+		const uint32_t g = blockIdx.x % kSegments;
+		const uint32_t blocks = gridDim.x / kSegments + (g < gridDim.x % kSegments ? 1u : 0u);
+		const uint32_t waves = blocks * (uint32_t)(kTraceBlock / 64);
+		if(atomicAdd(work + (size_t)(kSegments + g) * kWorkStride, 1u) == waves - 1u)
 		{
-			atomicExch(work, 0u);
-			atomicExch(work + 1, 0u);
+			const uint32_t groups = gridDim.x < kSegments ? gridDim.x : kSegments;
+			if(atomicAdd(work + (size_t)(2 * kSegments) * kWorkStride, 1u) == groups - 1u)
+			{
+				for(uint32_t k = 0; k < kWorkSlots; ++k) atomicExch(work + (size_t)k * kWorkStride, 0u);
+			}
 		}
 	}
 }
@@ -140,12 +234,6 @@ __device__ __forceinline__ ymx_ray load_ray(const ymx_ray *rays, uint32_t i)
 	return r;
 }
 
-__device__ __forceinline__ uint32_t wave_fetch(uint32_t *counter)
-{
-	uint32_t base = 0;
-	if(__lane_id() == 0) base = atomicAdd(counter, 64u);
-	return __shfl(base, 0);
-}
 
 // kMode: kClosest (Accelerator::intersect), kShadowAny (intersectShadow) or
 // kTransparent (intersectTransparentShadow).
@@ -164,11 +252,16 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(ymx_trace_args args)
 	st.cap = args.spill_depth;
 	unsigned long long n_fetch = 0, n_tests = 0;
 	const uint32_t n_rays = args.n_dev ? *args.n_dev : args.n_rays;
+	const uint32_t n_groups = (uint32_t)(((uint64_t)n_rays + 63u) / 64u);
+	WaveWork ww = wave_work_start(n_groups);
 
 	for(;;)
 	{
-		const uint32_t base = wave_fetch(args.work);
-		if(base >= n_rays) break;
+		// one group of 64 consecutive rays per claim
+		uint32_t got = 0;
+		const uint32_t group = wave_claim(args.work, ww, n_groups, 1u, got);
+		if(got == 0u) break;
+		const uint32_t base = group * 64u;
 		const uint32_t rid = base + __lane_id();
 		const bool valid = rid < n_rays;
 
@@ -383,6 +476,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_dyn(ymx_trace_args args)
 	unsigned long long n_fetch = 0, n_tests = 0;
 	const uint32_t n_rays = args.n_dev ? *args.n_dev : args.n_rays;
 	const bool reordered = args.order && *args.order_changes >= kReorderChanges;
+	WaveWork ww = wave_work_start(n_rays);
 
 	bool alive = false, exhausted = false;
 	uint32_t rid = 0;
@@ -397,16 +491,17 @@ __global__ __launch_bounds__(kTraceBlock) void k_dyn(ymx_trace_args args)
 		const unsigned long long need_mask = __ballot(need);
 		if(need_mask)
 		{
-			const int leader = __ffsll((long long)need_mask) - 1;
-			uint32_t base = 0;
-			if(lane == leader) base = atomicAdd(args.work, (uint32_t)__popcll(need_mask));
-			base = __shfl(base, leader);
+			uint32_t got = 0;
+			const uint32_t first = wave_claim(args.work, ww, n_rays, (uint32_t)__popcll(need_mask), got);
 			if(need)
 			{
-				rid = base + (uint32_t)__popcll(need_mask & ((1ull << lane) - 1ull));
-				if(rid >= n_rays) exhausted = true;
-				else
+				const uint32_t rank = (uint32_t)__popcll(need_mask & ((1ull << lane) - 1ull));
+				// nothing left anywhere: done; a short claim at a segment's end
+				// leaves the lanes past it to claim again on the next pass
+				if(got == 0u) exhausted = true;
+				else if(rank < got)
 				{
+					rid = first + rank;
 					// with ray reordering the k-th ray traced is the order[k]-th of the
 					// batch; it reads and writes the caller's slot
 					if(reordered) rid = args.order[rid];
@@ -915,13 +1010,15 @@ ymx_work_area *ymx_work_area_for(const ymx_accel *acc, hipStream_t stream)
 	ymx_work_area w = {};
 	w.stream = stream;
 	const size_t spill_bytes = sizeof(int32_t) * (size_t)acc->spill_depth * acc->grid_threads_max;
-	if(hipMalloc(&w.d_work, 64) != hipSuccess || hipMemset(w.d_work, 0, 64) != hipSuccess ||
+	if(hipMalloc(&w.d_work, sizeof(uint32_t) * kWorkWords) != hipSuccess ||
+	   hipMemset(w.d_work, 0, sizeof(uint32_t) * kWorkWords) != hipSuccess ||
 	   hipMalloc(&w.d_spill, spill_bytes) != hipSuccess ||
 	   hipEventCreateWithFlags(&w.done, hipEventDisableTiming) != hipSuccess)
 	{
 		if(w.d_work) (void)hipFree(w.d_work);
 		if(w.d_spill) (void)hipFree(w.d_spill);
-		ymx_set_error("query work area: device allocation of %zu bytes failed", spill_bytes + 64);
+		ymx_set_error("query work area: device allocation of %zu bytes failed",
+		              spill_bytes + sizeof(uint32_t) * kWorkWords);
 		return nullptr;
 	}
 	acc->work_areas.push_back(w);
@@ -981,8 +1078,8 @@ int ymx_launch_transparent(const ymx_accel *acc, const ymx_ray *rays, const uint
 }
 
 // ---------------------------------------------------------------------------
-// Fixed workspace of the allocation-free closest-hit query: the ray counter
-// and the finished-wave counter in its first kScratchHead words, then the
+// Fixed workspace of the allocation-free closest-hit query: the work
+// counters of the persistent grid in its first kScratchHead words, then the
 // spill columns of the persistent lanes.  Every device holds its own copy
 // from the load of this library until the process ends.  Host code must not
 // pass `scratch` itself to a kernel (on the host it only names the symbol):
@@ -990,7 +1087,7 @@ int ymx_launch_transparent(const ymx_accel *acc, const ymx_ray *rays, const uint
 // that address.
 namespace {
 constexpr size_t kScratchWords = (size_t)16 << 20;  // 64 MiB of int32
-constexpr size_t kScratchHead = 64;                 // 256 bytes: the two counters, padded
+constexpr size_t kScratchHead = kWorkWords;         // the work counters (520 KiB)
 constexpr int kScratchDevices = 64;
 std::mutex g_scratch_mutex;
 int32_t *g_scratch_ptr[kScratchDevices];            // per device, set once
@@ -1065,7 +1162,7 @@ int ymx_launch_closest_noalloc(const ymx_accel *acc, const ymx_ray *rays, ymx_hi
 	const uint32_t threads = lanes < needed ? lanes : needed;
 	const uint32_t blocks = (threads + kTraceBlock - 1) / kTraceBlock;
 	// the counters start at zero: a fill kernel, not a memset node
-	k_clear_words<<<1, 64, 0, stream>>>(a.work, 2u);
+	k_clear_words<<<(uint32_t)((kWorkWords + 255u) / 256u), 256, 0, stream>>>(a.work, (uint32_t)kWorkWords);
 	k_dyn<kClosest, false><<<blocks, kTraceBlock, 0, stream>>>(a);
 	YMX_HIP_CHECK(hipGetLastError());
 	return YMX_OK;
