@@ -94,6 +94,14 @@ struct ymx_accel
 ymx_work_area *ymx_work_area_for(const ymx_accel *acc, hipStream_t stream);
 void ymx_free_work_areas(ymx_accel *acc);
 
+// A segmented ray queue: its producers spread their appends over
+// kQueueSegments counters, a page (kQueueStride words) apart, so that no one
+// address takes every append; segment s holds counts[s * kQueueStride] rays
+// at [s * seg_cap, s * seg_cap + count).  The trace kernels claim their work
+// segment by segment, so they read such a queue as it is.
+constexpr uint32_t kQueueSegments = 64;
+constexpr uint32_t kQueueStride = 1024;
+
 struct ymx_trace_args
 {
 	const float4 *nodes;
@@ -102,8 +110,10 @@ struct ymx_trace_args
 	const ymx_ray *rays;
 	const uint32_t *order;    // optional: the k-th ray traced is rays[order[k]] (ray reordering)
 	const uint32_t *order_changes;  // with order: the batch was reordered when this count is >= the threshold
-	uint32_t n_rays;          // ray count, or the capacity when n_dev is set
+	uint32_t n_rays;          // ray count, or the capacity when n_dev or seg_counts is set
 	const uint32_t *n_dev;    // optional device-side ray count (queues)
+	const uint32_t *seg_counts;  // optional: a segmented queue's counters (n_rays = kQueueSegments x seg_cap)
+	uint32_t seg_cap;            // its rays per segment, a multiple of 64
 	float bmin[3], bmax[3];
 	uint32_t *work;
 	int32_t *spill;
@@ -145,6 +155,13 @@ int ymx_launch_closest_queue(const ymx_accel *acc, const ymx_ray *rays, ymx_hit 
                              uint32_t capacity, hipStream_t stream);
 int ymx_launch_shadow_queue(const ymx_accel *acc, const ymx_ray *rays, uint32_t *occ, const uint32_t *n_dev,
                             uint32_t capacity, hipStream_t stream);
+// Segmented-queue variants (see kQueueSegments): occlusion bits and caster
+// lists by queue position.
+int ymx_launch_shadow_segmented(const ymx_accel *acc, const ymx_ray *rays, uint32_t *occ, const uint32_t *seg_counts,
+                                uint32_t seg_cap, hipStream_t stream);
+int ymx_launch_transparent_segmented(const ymx_accel *acc, const ymx_ray *rays, const uint32_t *transp,
+                                     uint32_t max_depth, uint32_t *occ, uint32_t *counts, uint2 *hits,
+                                     const uint32_t *seg_counts, uint32_t seg_cap, hipStream_t stream);
 // Allocation-free closest hit over the fixed per-device workspace of
 // accel_trace.hip; ymx_noalloc_setup (run by ymx_trace_setup with the handle's
 // device current) looks up that device's workspace address once.

@@ -115,6 +115,7 @@ struct LaneStack
 // area starts from zero.
 constexpr uint32_t kSegments = 64;  // one per lane of the wave that scans them
 static_assert(kTraceBlock % 64 == 0, "waves of 64 lanes");
+static_assert(kSegments == kQueueSegments, "a segmented queue's segments are the claim segments");
 constexpr uint32_t kXcds = 8;
 constexpr uint32_t kWorkStride = 1024;
 constexpr uint32_t kWorkSlots = 2 * kSegments + 1;
@@ -125,29 +126,72 @@ __device__ __forceinline__ uint32_t seg_lo(uint32_t s, uint32_t n_units)
 	return (uint32_t)(((uint64_t)n_units * s) / kSegments);
 }
 
+// What the segments hold, in claim units of 2^unit_shift rays: an even split
+// of n_units, or the segments of a segmented queue (counts, cap_units each).
+struct SegView
+{
+	uint32_t n_units;
+	const uint32_t *counts;
+	uint32_t cap_units;
+	uint32_t unit_shift;
+};
+
+__device__ __forceinline__ SegView seg_view(const ymx_trace_args &args, uint32_t n_rays, uint32_t unit_shift)
+{
+	SegView v;
+	v.counts = args.seg_counts;
+	v.unit_shift = unit_shift;
+	v.cap_units = args.seg_cap >> unit_shift;
+	v.n_units = (uint32_t)(((uint64_t)n_rays + (1u << unit_shift) - 1u) >> unit_shift);
+	return v;
+}
+
+// Segment s: first unit and unit count; ray_end, the end of its rays.
+__device__ __forceinline__ void seg_bounds(const SegView &v, uint32_t s, uint32_t &lo, uint32_t &size,
+                                           uint32_t &ray_end)
+{
+	if(v.counts)
+	{
+		uint32_t c = v.counts[(size_t)s * kQueueStride];
+		const uint32_t cap = v.cap_units << v.unit_shift;
+		if(c > cap) c = cap;
+		lo = s * v.cap_units;
+		size = (c + (1u << v.unit_shift) - 1u) >> v.unit_shift;
+		ray_end = (lo << v.unit_shift) + c;
+	}
+	else
+	{
+		lo = seg_lo(s, v.n_units);
+		size = seg_lo(s + 1u, v.n_units) - lo;
+		ray_end = 0xFFFFFFFFu;  // only the batch's own end bounds the last unit
+	}
+}
+
 struct WaveWork
 {
 	uint32_t seg;   // segment the wave claims from
 	bool done;      // every segment used up
 };
 
-__device__ __forceinline__ WaveWork wave_work_start(uint32_t n_units)
+__device__ __forceinline__ WaveWork wave_work_start(const SegView &v)
 {
 	constexpr uint32_t per_xcd = kSegments / kXcds;
 	const uint32_t xcd = blockIdx.x % kXcds;
 	const uint32_t j = (blockIdx.x / kXcds) * (uint32_t)(kTraceBlock / 64) + (threadIdx.x >> 6);
-	return WaveWork{xcd * per_xcd + j % per_xcd, n_units == 0u};
+	return WaveWork{xcd * per_xcd + j % per_xcd, !v.counts && v.n_units == 0u};
 }
 
 // After segment ww.seg ran out: lane k reads segment k's counter, and the
 // wave moves to the first segment after ww.seg (cyclically) with units left,
 // or sets done.  Every lane computes the same choice from the ballot.
-__device__ __forceinline__ void next_segment(const uint32_t *work, WaveWork &ww, uint32_t n_units)
+__device__ __forceinline__ void next_segment(const uint32_t *work, WaveWork &ww, const SegView &v)
 {
 	const uint32_t k = __lane_id();
 	const uint32_t claimed = __hip_atomic_load(work + (size_t)k * kWorkStride, __ATOMIC_RELAXED,
 	                                           __HIP_MEMORY_SCOPE_AGENT);
-	const bool left = claimed < seg_lo(k + 1u, n_units) - seg_lo(k, n_units) && k != ww.seg;
+	uint32_t lo, size, ray_end;
+	seg_bounds(v, k, lo, size, ray_end);
+	const bool left = claimed < size && k != ww.seg;
 	const unsigned long long free_bits = __ballot(left);
 	if(free_bits == 0ull)
 	{
@@ -159,26 +203,27 @@ __device__ __forceinline__ void next_segment(const uint32_t *work, WaveWork &ww,
 	ww.seg = (start + (uint32_t)(__ffsll((long long)rot) - 1)) % kSegments;
 }
 
-// Claims up to `want` consecutive units of n_units for the wave: returns the
-// first and sets got, 0 once every segment is used up.  Called with every
-// lane of the wave active; lane 0 makes the atomic.
-__device__ __forceinline__ uint32_t wave_claim(uint32_t *work, WaveWork &ww, uint32_t n_units, uint32_t want,
-                                               uint32_t &got)
+// Claims up to `want` consecutive units for the wave: returns the first and
+// sets got (0 once every segment is used up) and ray_end (the end of the
+// claimed segment's rays).  Called with every lane of the wave active; lane 0
+// makes the atomic.
+__device__ __forceinline__ uint32_t wave_claim(uint32_t *work, WaveWork &ww, const SegView &v, uint32_t want,
+                                               uint32_t &got, uint32_t &ray_end)
 {
 	while(!ww.done)
 	{
-		const uint32_t lo = seg_lo(ww.seg, n_units);
-		const uint32_t size = seg_lo(ww.seg + 1u, n_units) - lo;
+		uint32_t lo, size;
+		seg_bounds(v, ww.seg, lo, size, ray_end);
 		uint32_t b = size;
 		if(__lane_id() == 0 && size > 0u) b = atomicAdd(work + (size_t)ww.seg * kWorkStride, want);
 		b = __shfl(b, 0);
 		if(b < size)
 		{
 			got = size - b < want ? size - b : want;
-			if(b + want >= size) next_segment(work, ww, n_units);
+			if(b + want >= size) next_segment(work, ww, v);
 			return lo + b;
 		}
-		next_segment(work, ww, n_units);
+		next_segment(work, ww, v);
 	}
 	got = 0u;
 	return 0u;
@@ -252,18 +297,18 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(ymx_trace_args args)
 	st.cap = args.spill_depth;
 	unsigned long long n_fetch = 0, n_tests = 0;
 	const uint32_t n_rays = args.n_dev ? *args.n_dev : args.n_rays;
-	const uint32_t n_groups = (uint32_t)(((uint64_t)n_rays + 63u) / 64u);
-	WaveWork ww = wave_work_start(n_groups);
+	const SegView sv = seg_view(args, n_rays, 6u);
+	WaveWork ww = wave_work_start(sv);
 
 	for(;;)
 	{
 		// one group of 64 consecutive rays per claim
-		uint32_t got = 0;
-		const uint32_t group = wave_claim(args.work, ww, n_groups, 1u, got);
+		uint32_t got = 0, ray_end = 0;
+		const uint32_t group = wave_claim(args.work, ww, sv, 1u, got, ray_end);
 		if(got == 0u) break;
 		const uint32_t base = group * 64u;
 		const uint32_t rid = base + __lane_id();
-		const bool valid = rid < n_rays;
+		const bool valid = rid < n_rays && rid < ray_end;
 
 		ymx_ray ray = {};
 		if(valid) ray = load_ray(args.rays, rid);
@@ -476,7 +521,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_dyn(ymx_trace_args args)
 	unsigned long long n_fetch = 0, n_tests = 0;
 	const uint32_t n_rays = args.n_dev ? *args.n_dev : args.n_rays;
 	const bool reordered = args.order && *args.order_changes >= kReorderChanges;
-	WaveWork ww = wave_work_start(n_rays);
+	const SegView sv = seg_view(args, n_rays, 0u);
+	WaveWork ww = wave_work_start(sv);
 
 	bool alive = false, exhausted = false;
 	uint32_t rid = 0;
@@ -491,8 +537,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_dyn(ymx_trace_args args)
 		const unsigned long long need_mask = __ballot(need);
 		if(need_mask)
 		{
-			uint32_t got = 0;
-			const uint32_t first = wave_claim(args.work, ww, n_rays, (uint32_t)__popcll(need_mask), got);
+			uint32_t got = 0, ray_end = 0;
+			const uint32_t first = wave_claim(args.work, ww, sv, (uint32_t)__popcll(need_mask), got, ray_end);
 			if(need)
 			{
 				const uint32_t rank = (uint32_t)__popcll(need_mask & ((1ull << lane) - 1ull));
@@ -899,7 +945,7 @@ template <int kMode, bool kStats, bool kDynamic = false>
 int launch(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t *occ, uint32_t *prim, uint32_t n,
            hipStream_t stream, unsigned long long *stats, const uint32_t *n_dev = nullptr,
            const uint32_t *transp = nullptr, uint32_t max_depth = 0, uint32_t *t_count = nullptr,
-           uint2 *t_hits = nullptr)
+           uint2 *t_hits = nullptr, const uint32_t *seg_counts = nullptr, uint32_t seg_cap = 0)
 {
 	constexpr bool kShadow = kMode != kClosest;
 	if(n == 0) return YMX_OK;
@@ -910,6 +956,8 @@ int launch(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t *o
 	a.rays = rays;
 	a.n_rays = n;
 	a.n_dev = n_dev;
+	a.seg_counts = seg_counts;
+	a.seg_cap = seg_cap;
 	for(int k = 0; k < 3; ++k)
 	{
 		a.bmin[k] = acc->bmin[k];
@@ -937,7 +985,7 @@ int launch(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t *o
 	a.order_changes = nullptr;
 	if constexpr(kDynamic && kMode != kTransparent)
 	{
-		const bool reorder = n_dev == nullptr && !kStats &&
+		const bool reorder = n_dev == nullptr && seg_counts == nullptr && !kStats &&
 		                     (acc->ray_reorder > 0 || (acc->ray_reorder == 0 && n >= kReorderMin));
 		if(reorder)
 		{
@@ -1075,6 +1123,31 @@ int ymx_launch_transparent(const ymx_accel *acc, const ymx_ray *rays, const uint
 {
 	return launch<kTransparent, false>(acc, rays, nullptr, occ, nullptr, n, stream, nullptr, n_dev, transp, max_depth,
 	                                   counts, hits);
+}
+
+int ymx_launch_shadow_segmented(const ymx_accel *acc, const ymx_ray *rays, uint32_t *occ, const uint32_t *seg_counts,
+                                uint32_t seg_cap, hipStream_t stream)
+{
+	if(seg_cap % 64u)
+	{
+		ymx_set_error("segmented shadow queue: %u rays per segment is not a multiple of 64", seg_cap);
+		return YMX_ERR_INVALID_ARG;
+	}
+	return launch<kShadowAny, false>(acc, rays, nullptr, occ, nullptr, kQueueSegments * seg_cap, stream, nullptr,
+	                                 nullptr, nullptr, 0, nullptr, nullptr, seg_counts, seg_cap);
+}
+
+int ymx_launch_transparent_segmented(const ymx_accel *acc, const ymx_ray *rays, const uint32_t *transp,
+                                     uint32_t max_depth, uint32_t *occ, uint32_t *counts, uint2 *hits,
+                                     const uint32_t *seg_counts, uint32_t seg_cap, hipStream_t stream)
+{
+	if(seg_cap % 64u)
+	{
+		ymx_set_error("segmented shadow queue: %u rays per segment is not a multiple of 64", seg_cap);
+		return YMX_ERR_INVALID_ARG;
+	}
+	return launch<kTransparent, false>(acc, rays, nullptr, occ, nullptr, kQueueSegments * seg_cap, stream, nullptr,
+	                                   nullptr, transp, max_depth, counts, hits, seg_counts, seg_cap);
 }
 
 // ---------------------------------------------------------------------------

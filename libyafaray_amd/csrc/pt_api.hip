@@ -618,10 +618,23 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 	PT_TRY(dalloc(s.ext_hits, n));
 	PT_TRY(dalloc(s.slot_contrib, slot_cap));
 	PT_TRY(dalloc(s.slot_queue, slot_cap));
-	PT_TRY(dalloc(s.sh_rays, slot_cap));
-	PT_TRY(dalloc(s.sh_occ, (slot_cap + 31) / 32 + 2));
-	PT_TRY(dalloc(s.sh_count, 1));
-	PT_TRY(hipMemset(s.sh_count, 0, sizeof(uint32_t)));
+	// the segmented shadow queue: a segment takes the appends of every
+	// kQueueSegments-th wave of the shading grid (at most capacity paths, 64
+	// per wave), each appending at most `slots` rays per path
+	const uint64_t waves = ((uint64_t)n + 255u) / 256u * 4u;
+	const uint64_t seg_cap = (waves + kQueueSegments - 1u) / kQueueSegments * 64u * s.slots;
+	const uint64_t sh_cap = seg_cap * kQueueSegments;
+	if(sh_cap > 0xFFFFFFC0ull)
+	{
+		ymx_set_error("ymx_pt_create: a %llu-ray shadow queue exceeds 2^32", (unsigned long long)sh_cap);
+		free_pt(pt);
+		return YMX_ERR_UNSUPPORTED;
+	}
+	s.sh_seg_cap = (uint32_t)seg_cap;
+	PT_TRY(dalloc(s.sh_rays, sh_cap));
+	PT_TRY(dalloc(s.sh_occ, (sh_cap + 31) / 32 + 2));
+	PT_TRY(dalloc(s.sh_count, (size_t)kQueueSegments * kQueueStride));
+	PT_TRY(hipMemset(s.sh_count, 0, sizeof(uint32_t) * kQueueSegments * kQueueStride));
 	if(c.transparent_shadows)
 	{
 		// Material::isTransparent: glass with fake shadows (include/material/material_glass.h:82)
@@ -633,8 +646,8 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 		}
 		PT_TRY(dalloc(pt->d_transp, bits.size()));
 		PT_TRY(hipMemcpy(pt->d_transp, bits.data(), sizeof(uint32_t) * bits.size(), hipMemcpyHostToDevice));
-		PT_TRY(dalloc(s.sh_tcount, slot_cap));
-		PT_TRY(dalloc(s.sh_thits, slot_cap * (size_t)(c.shadow_depth ? c.shadow_depth : 1)));
+		PT_TRY(dalloc(s.sh_tcount, sh_cap));
+		PT_TRY(dalloc(s.sh_thits, sh_cap * (size_t)(c.shadow_depth ? c.shadow_depth : 1)));
 	}
 	PT_TRY(dalloc(s.totals, 3));
 	PT_TRY(hipMemset(s.totals, 0, 3 * sizeof(unsigned long long)));

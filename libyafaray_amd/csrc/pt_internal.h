@@ -140,7 +140,8 @@ struct PtState
 	int32_t *slot_queue;   // queue index, -1: contributes without a shadow test
 	ymx_ray *sh_rays;
 	uint32_t *sh_occ;      // bit per queued shadow ray
-	uint32_t *sh_count;
+	uint32_t *sh_count;    // segmented queue: kQueueSegments counters, kQueueStride words apart
+	uint32_t sh_seg_cap;   // rays per segment (a multiple of 64)
 	uint32_t *sh_tcount;   // transparent shadows: casters kept per queued shadow ray
 	uint2 *sh_thits;       // ... and their (primitive, t bits), shadow_depth per ray
 	uint32_t capacity;     // slots (nodes per chunk); also the camera paths per wavefront

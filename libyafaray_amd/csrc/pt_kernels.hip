@@ -757,12 +757,28 @@ __device__ __forceinline__ uint32_t light_slots(const PtLight &l, uint32_t divis
 // Accelerator::isShadowed (include/accelerator/accelerator.h:103-111): origin
 // moved by tmin, t_max = tmax - 2 tmin.  Returns the queue index, or -1 when no
 // test is needed (t_max <= 0 can never be occluded).
+//
+// The shadow queue is segmented (kQueueSegments segments of sh_seg_cap rays):
+// a wave appends to segment (its index mod kQueueSegments) with one atomic
+// for the lanes appending together, so the appends spread over the
+// segments' counters.  sh_seg_cap is sized for every wave of a segment
+// filling all its slots, so no segment overflows.
 __device__ int32_t queue_shadow(const PtArgs &a, V3 from, V3 dir, float tmin, float tmax)
 {
 	const float t_max = (tmax >= 0.f) ? tmax - 2 * tmin : FLT_MAX;
 	if(!(t_max > 0.f)) return -1;
 	const V3 f = add(from, mul(tmin, dir));
-	const uint32_t q = atomicAdd(a.st.sh_count, 1u);
+	const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * kBlock + threadIdx.x) >> 6);
+	const uint32_t seg = wave % kQueueSegments;
+	const unsigned long long active = __ballot(true);
+	const int leader = __ffsll((long long)active) - 1;
+	const int lane = (int)__lane_id();
+	uint32_t base = 0;
+	if(lane == leader) base = atomicAdd(a.st.sh_count + (size_t)seg * kQueueStride, (uint32_t)__popcll(active));
+	base = __shfl(base, leader);
+	const uint32_t off = base + (uint32_t)__popcll(active & ((1ull << lane) - 1ull));
+	if(off >= a.st.sh_seg_cap) return -1;  // cannot happen with the sizing above; keeps the write in bounds
+	const uint32_t q = seg * a.st.sh_seg_cap + off;
 	float4 *r = reinterpret_cast<float4 *>(a.st.sh_rays + q);
 	r[0] = make_float4(f.x, f.y, f.z, tmin);
 	r[1] = make_float4(dir.x, dir.y, dir.z, t_max);
@@ -1345,11 +1361,15 @@ __global__ __launch_bounds__(kBlock) void k_resolve_node(PtArgs a)
 {
 	const PtConst &c = *a.c;
 	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-	if(i == 0)
+	if(i < kQueueSegments)
 	{
 		// consumed by the launches before this one; the next enqueues come after it
-		a.st.totals[2] += *a.st.sh_count;
-		*a.st.sh_count = 0u;
+		uint32_t *cnt = a.st.sh_count + (size_t)i * kQueueStride;
+		atomicAdd(a.st.totals + 2, (unsigned long long)*cnt);
+		*cnt = 0u;
+	}
+	if(i == 0)
+	{
 		a.st.ext_count[0] = 0u;
 		a.st.ext_count[1] = 0u;
 	}
@@ -1571,11 +1591,15 @@ __global__ __launch_bounds__(kBlock) void k_resolve_path(PtArgs a)
 {
 	const PtConst &c = *a.c;
 	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if(i < kQueueSegments)
+	{
+		uint32_t *cnt = a.st.sh_count + (size_t)i * kQueueStride;
+		atomicAdd(a.st.totals + 2, (unsigned long long)*cnt);
+		*cnt = 0u;
+	}
 	if(i == 0)
 	{
-		a.st.totals[2] += *a.st.sh_count;
 		a.st.totals[1] += a.st.ext_count[a.depth & 1];
-		*a.st.sh_count = 0u;
 		a.st.ext_count[a.depth & 1] = 0u;  // this segment's queue, consumed by shade_path
 	}
 	if(i >= a.cnt) return;
@@ -2068,13 +2092,15 @@ inline unsigned blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
 // The queued shadow rays: any-hit, or with transparent shadows the
 // transparent-shadow query (Accelerator::isShadowed / isShadowedTransparentShadow).
-int shadow_pass(ymx_pt *pt, uint32_t slot_cap, hipStream_t stream)
+int shadow_pass(ymx_pt *pt, hipStream_t stream)
 {
 	const PtConst &hc = pt->host_const;
 	if(hc.transparent_shadows)
-		return ymx_launch_transparent(pt->accel, pt->st.sh_rays, pt->d_transp, hc.shadow_depth, pt->st.sh_occ,
-		                              pt->st.sh_tcount, pt->st.sh_thits, slot_cap, pt->st.sh_count, stream);
-	return ymx_launch_shadow_queue(pt->accel, pt->st.sh_rays, pt->st.sh_occ, pt->st.sh_count, slot_cap, stream);
+		return ymx_launch_transparent_segmented(pt->accel, pt->st.sh_rays, pt->d_transp, hc.shadow_depth,
+		                                        pt->st.sh_occ, pt->st.sh_tcount, pt->st.sh_thits, pt->st.sh_count,
+		                                        pt->st.sh_seg_cap, stream);
+	return ymx_launch_shadow_segmented(pt->accel, pt->st.sh_rays, pt->st.sh_occ, pt->st.sh_count, pt->st.sh_seg_cap,
+	                                   stream);
 }
 
 // The stages of one chunk of nodes [a.base, a.base + a.cnt).
@@ -2083,7 +2109,6 @@ int trace_chunk(ymx_pt *pt, PtArgs &a, hipStream_t stream)
 	const PtConst &hc = pt->host_const;
 	const uint32_t n = a.cnt;
 	const unsigned g = blocks_for(n);
-	const uint32_t slot_cap = n * pt->st.slots;
 	k_count_rays<<<1, 64, 0, stream>>>(pt->st.totals, n);
 	YMX_HIP_CHECK(hipGetLastError());
 	// camera rays are coherent: the wave-ordered kernel (ray replacement costs there)
@@ -2093,7 +2118,7 @@ int trace_chunk(ymx_pt *pt, PtArgs &a, hipStream_t stream)
 	YMX_HIP_CHECK(hipGetLastError());
 	if(hc.n_lights > 0)
 	{
-		rc = shadow_pass(pt, slot_cap, stream);
+		rc = shadow_pass(pt, stream);
 		if(rc != YMX_OK) return rc;
 	}
 	k_resolve_node<<<g, kBlock, 0, stream>>>(a);
@@ -2129,7 +2154,7 @@ int trace_chunk(ymx_pt *pt, PtArgs &a, hipStream_t stream)
 			YMX_HIP_CHECK(hipGetLastError());
 			if(hc.n_lights > 0)
 			{
-				rc = shadow_pass(pt, slot_cap, stream);
+				rc = shadow_pass(pt, stream);
 				if(rc != YMX_OK) return rc;
 			}
 			k_resolve_path<<<g, kBlock, 0, stream>>>(a);
