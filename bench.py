@@ -134,8 +134,10 @@ def measure_path_tracer(args, torch):
 def measure_terrain_frame(args, torch):
     """The mixed-material terrain frame at its full size: 1024^2 heightfield
     + 500 icospheres of 1280 triangles, 1920 x 1080, 64 spp, depth 8, 64 point
-    lights + an area light, background 0.1.  One warm-up frame at 1 spp, then
-    the best of two 64-spp frames: render time, samples/s and rays/s."""
+    lights + an area light, background 0.1.  One warm-up frame at 1 spp, the
+    autotuner's choice of launch configuration (its trial renders reported
+    apart), then the best of two 64-spp frames: render time, samples/s and
+    rays/s."""
     from libyafaray_amd import scenes
     from libyafaray_amd.pt import GpuPathTracer
     sc = scenes.terrain_spheres_scene()
@@ -146,6 +148,9 @@ def measure_terrain_frame(args, torch):
     pt.clear()
     pt.render(0, 1)
     torch.cuda.synchronize()
+    ta = time.perf_counter()
+    tuning = pt.autotune()  # launch configuration for this scene (not timed with the frame)
+    tune_s = time.perf_counter() - ta
     times, st = [], None
     for _ in range(2):
         pt.clear()
@@ -163,7 +168,8 @@ def measure_terrain_frame(args, torch):
            "render_ms": round(best * 1e3, 1), "msamples_per_s": round(samples / best / 1e6, 1),
            "rays": int(rays), "mrays_per_s": round(rays / best / 1e6, 1),
            "camera_rays": int(st["camera_rays"]), "extension_rays": int(st["extension_rays"]),
-           "shadow_rays": int(st["shadow_rays"]), "setup_ms": round(setup_s * 1e3, 1)}
+           "shadow_rays": int(st["shadow_rays"]), "setup_ms": round(setup_s * 1e3, 1),
+           "autotune_ms": round(tune_s * 1e3, 1), "tuning": tuning}
     log(args, f"terrain frame: {out}")
     return out
 

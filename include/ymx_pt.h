@@ -124,6 +124,11 @@ typedef struct ymx_pt_settings
 	int32_t aa_variance_edge_size;            /* "AA_variance_edge_size" [10] */
 	int32_t aa_variance_pixels;               /* "AA_variance_pixels" [0] */
 	float aa_clamp_samples;                   /* "AA_clamp_samples": proportional clamp of each sample, 0 off [0] */
+	/* Device memory for the wavefront state in MiB: a wavefront holds as many AA
+	 * samples of every pixel as fit (at least one); 0 = a quarter of the free
+	 * device memory at creation, at most 64 GiB.  No counterpart in the reference
+	 * (its tiles hold one sample at a time). */
+	int32_t wavefront_mb;
 } ymx_pt_settings;
 
 typedef struct ymx_pt ymx_pt;
@@ -231,6 +236,38 @@ typedef struct ymx_pt_stats
 } ymx_pt_stats;
 
 int ymx_pt_get_stats(const ymx_pt *pt, ymx_pt_stats *stats);
+
+/* Launch configuration of the wavefront, chosen per scene by ymx_pt_autotune
+ * or set directly.  No counterpart in the reference (its tiles run on CPU
+ * threads).  Every configuration gives the same film bit for bit. */
+#define YMX_TRACE_WAVE_ORDERED 0    /* each wave traces 64 consecutive queued rays */
+#define YMX_TRACE_REPLACEMENT 1     /* a lane whose ray is done takes the next one */
+#define YMX_PT_TUNING_TRIALS 8
+typedef struct ymx_pt_tuning
+{
+	uint32_t struct_size;
+	int32_t samples_per_wavefront;  /* AA samples of every pixel per wavefront, 1 .. the allocated number */
+	int32_t shadow_kernel;          /* YMX_TRACE_*, for the shadow queues (wave-ordered only with transparent shadows) */
+	int32_t extension_kernel;       /* YMX_TRACE_*, for the extension queues */
+	/* ymx_pt_autotune's measurements: per trial configuration, ms per AA sample */
+	int32_t n_trials;
+	int32_t trial_samples_per_wavefront[YMX_PT_TUNING_TRIALS];
+	int32_t trial_shadow_kernel[YMX_PT_TUNING_TRIALS];
+	int32_t trial_extension_kernel[YMX_PT_TUNING_TRIALS];
+	float trial_ms_per_sample[YMX_PT_TUNING_TRIALS];
+} ymx_pt_tuning;
+
+/* Times short renders of the scene under each candidate configuration (both
+ * kernels for each queue kind at the allocated wavefront size, then the best
+ * of those at half and a quarter of that size), keeps the fastest, and
+ * returns it with the measurements.  trial_samples: AA samples per trial
+ * (0: the allocated samples per wavefront).  Synchronous; the film is left as
+ * it was, and guide training is paused during the trials. */
+int ymx_pt_autotune(ymx_pt *pt, int32_t trial_samples, ymx_pt_tuning *result, void *stream);
+
+/* The configuration in use (struct_size set by the caller), or a new one. */
+int ymx_pt_get_tuning(const ymx_pt *pt, ymx_pt_tuning *tuning);
+int ymx_pt_set_tuning(ymx_pt *pt, const ymx_pt_tuning *tuning);
 
 #ifdef __cplusplus
 }

@@ -150,15 +150,16 @@ int ymx_launch_transparent(const ymx_accel *acc, const ymx_ray *rays, const uint
 // Closest hit of n coherent rays with the wave-ordered kernel (no ray replacement).
 int ymx_launch_closest_ordered(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, uint32_t n,
                                hipStream_t stream);
-// Queue variants: the ray count is read on the device from *n_dev (<= capacity).
+// Queue variants: the ray count is read on the device from *n_dev (<= capacity);
+// replacement: the ray-replacement kernel (k_dyn) instead of the wave-ordered one.
 int ymx_launch_closest_queue(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, const uint32_t *n_dev,
-                             uint32_t capacity, hipStream_t stream);
+                             uint32_t capacity, bool replacement, hipStream_t stream);
 int ymx_launch_shadow_queue(const ymx_accel *acc, const ymx_ray *rays, uint32_t *occ, const uint32_t *n_dev,
                             uint32_t capacity, hipStream_t stream);
 // Segmented-queue variants (see kQueueSegments): occlusion bits and caster
 // lists by queue position.
 int ymx_launch_shadow_segmented(const ymx_accel *acc, const ymx_ray *rays, uint32_t *occ, const uint32_t *seg_counts,
-                                uint32_t seg_cap, hipStream_t stream);
+                                uint32_t seg_cap, bool replacement, hipStream_t stream);
 int ymx_launch_transparent_segmented(const ymx_accel *acc, const ymx_ray *rays, const uint32_t *transp,
                                      uint32_t max_depth, uint32_t *occ, uint32_t *counts, uint2 *hits,
                                      const uint32_t *seg_counts, uint32_t seg_cap, hipStream_t stream);

@@ -496,9 +496,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(ymx_trace_args args)
 // result is the same; the any-hit occlusion bits are set by atomicOr into a
 // cleared mask, since the rays of a wave are no longer one 64-ray group.  On
 // the incoherent 8.4M-ray batch this is 17 % faster than k_trace for closest
-// hit; on the path tracer's queues, where the lanes of a wave finish
-// together, mixing rays of different progress costs more than it gains, so
-// those use k_trace.
+// hit.  The path tracer's extension queues, whose lanes finish together, keep
+// k_trace; its shadow queues use this kernel.
 constexpr int kRefill = 12;
 // Ray reordering's coherence test (below): neighbouring pairs sampled, and
 // the count of them changing bucket from which a batch is reordered (1 in 8).
@@ -1099,8 +1098,10 @@ int ymx_launch_closest_ordered(const ymx_accel *acc, const ymx_ray *rays, ymx_hi
 }
 
 int ymx_launch_closest_queue(const ymx_accel *acc, const ymx_ray *rays, ymx_hit *hits, const uint32_t *n_dev,
-                             uint32_t capacity, hipStream_t stream)
+                             uint32_t capacity, bool replacement, hipStream_t stream)
 {
+	if(replacement)
+		return launch<kClosest, false, true>(acc, rays, hits, nullptr, nullptr, capacity, stream, nullptr, n_dev);
 	return launch<kClosest, false>(acc, rays, hits, nullptr, nullptr, capacity, stream, nullptr, n_dev);
 }
 
@@ -1126,13 +1127,16 @@ int ymx_launch_transparent(const ymx_accel *acc, const ymx_ray *rays, const uint
 }
 
 int ymx_launch_shadow_segmented(const ymx_accel *acc, const ymx_ray *rays, uint32_t *occ, const uint32_t *seg_counts,
-                                uint32_t seg_cap, hipStream_t stream)
+                                uint32_t seg_cap, bool replacement, hipStream_t stream)
 {
 	if(seg_cap % 64u)
 	{
 		ymx_set_error("segmented shadow queue: %u rays per segment is not a multiple of 64", seg_cap);
 		return YMX_ERR_INVALID_ARG;
 	}
+	if(replacement)
+		return launch<kShadowAny, false, true>(acc, rays, nullptr, occ, nullptr, kQueueSegments * seg_cap, stream,
+		                                       nullptr, nullptr, nullptr, 0, nullptr, nullptr, seg_counts, seg_cap);
 	return launch<kShadowAny, false>(acc, rays, nullptr, occ, nullptr, kQueueSegments * seg_cap, stream, nullptr,
 	                                 nullptr, nullptr, 0, nullptr, nullptr, seg_counts, seg_cap);
 }

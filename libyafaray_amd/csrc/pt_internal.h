@@ -204,7 +204,10 @@ struct ymx_pt
 	int32_t guide_depth;             // ymx_pt_set_guide_depth (-1: every vertex)
 	uint32_t g_seen, g_filled;       // training samples written so far (counter value), valid in the ring
 	uint64_t g_rng;                  // mini-batch offsets
-	uint32_t batch;                  // AA samples per wavefront
+	uint32_t batch;                  // AA samples per wavefront (<= batch_max; ymx_pt_autotune may lower it)
+	uint32_t batch_max;              // AA samples per wavefront the state is allocated for
+	bool shadow_replacement;         // shadow queues traced with ray replacement (k_dyn), else wave-ordered
+	bool ext_replacement;            // the same for the extension queues
 	ymx_pt_settings settings;        // as given at create (the AA pass params)
 	float max_light_mult;            // largest light sample multiplier of any AA pass
 	uint32_t *d_aa;                  // AA passes: flags, pair bits, variance hits, pixel list, slots (npix each), count

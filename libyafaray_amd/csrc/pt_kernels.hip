@@ -2100,7 +2100,7 @@ int shadow_pass(ymx_pt *pt, hipStream_t stream)
 		                                        pt->st.sh_occ, pt->st.sh_tcount, pt->st.sh_thits, pt->st.sh_count,
 		                                        pt->st.sh_seg_cap, stream);
 	return ymx_launch_shadow_segmented(pt->accel, pt->st.sh_rays, pt->st.sh_occ, pt->st.sh_count, pt->st.sh_seg_cap,
-	                                   stream);
+	                                   pt->shadow_replacement, stream);
 }
 
 // The stages of one chunk of nodes [a.base, a.base + a.cnt).
@@ -2141,7 +2141,7 @@ int trace_chunk(ymx_pt *pt, PtArgs &a, hipStream_t stream)
 			a.depth = d;
 			const int qsel = d & 1;
 			rc = ymx_launch_closest_queue(pt->accel, pt->st.ext_rays[qsel], pt->st.ext_hits, pt->st.ext_count + qsel, n,
-			                              stream);
+			                              pt->ext_replacement, stream);
 			if(rc != YMX_OK) return rc;
 			if(a.g.on && d + 1 < hc.bounces && d < a.g.max_depth)
 			{

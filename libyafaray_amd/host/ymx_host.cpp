@@ -786,6 +786,7 @@ std::unique_ptr<PathIntegrator> PathIntegrator::create(Scene &scene, const Param
 	s.aa_variance_edge_size = film.getInt("AA_variance_edge_size", 10, &ok);
 	s.aa_variance_pixels = film.getInt("AA_variance_pixels", 0, &ok);
 	s.aa_clamp_samples = (float)film.getFloat("AA_clamp_samples", 0.0, &ok);
+	s.wavefront_mb = integrator.getInt("wavefront_mb", 0, &ok);
 	if(background)
 	{
 		s.has_background = 1;
@@ -819,6 +820,16 @@ int PathIntegrator::renderPasses(std::vector<int> *resampled)
 	const int rc = ymx_pt_render_passes(pt_, res.data(), nullptr);
 	if(rc != YMX_OK) error_ = ymx_last_error();
 	if(resampled) resampled->assign(res.begin(), res.end());
+	return rc;
+}
+
+int PathIntegrator::autotune(int trial_samples, ymx_pt_tuning *result)
+{
+	ymx_pt_tuning t{};
+	t.struct_size = sizeof t;
+	const int rc = ymx_pt_autotune(pt_, trial_samples, &t, nullptr);
+	if(rc != YMX_OK) error_ = ymx_last_error();
+	else if(result) *result = t;
 	return rc;
 }
 

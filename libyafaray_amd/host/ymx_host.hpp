@@ -305,6 +305,10 @@ class PathIntegrator
 		// for every sample); resampled: the pixels of each pass.  Returns once
 		// the last pass is queued (film() waits for it).
 		int renderPasses(std::vector<int> *resampled = nullptr);
+		// ymx_pt_autotune: times short renders under each launch configuration
+		// and keeps the fastest (the film is left as it was); trial_samples 0:
+		// the samples one wavefront holds.  Synchronous.
+		int autotune(int trial_samples = 0, ymx_pt_tuning *result = nullptr);
 		int clearFilm();
 		// The film as ImageFilm holds it: colour and alpha sums weighted by the
 		// filter, and the weight sums; pixel (x, y) at index y * resx + x.

@@ -59,6 +59,18 @@ class YmxPtStats(C.Structure):
                 ("guide_steps", C.c_uint64)]
 
 
+TUNING_TRIALS = 8
+
+
+class YmxPtTuning(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("samples_per_wavefront", C.c_int32), ("shadow_kernel", C.c_int32),
+                ("extension_kernel", C.c_int32), ("n_trials", C.c_int32),
+                ("trial_samples_per_wavefront", C.c_int32 * TUNING_TRIALS),
+                ("trial_shadow_kernel", C.c_int32 * TUNING_TRIALS),
+                ("trial_extension_kernel", C.c_int32 * TUNING_TRIALS),
+                ("trial_ms_per_sample", C.c_float * TUNING_TRIALS)]
+
+
 # Every symbol include/ymx_accel.h and include/ymx_pt.h declare, with its ctypes signature.
 SIGNATURES = {
     "ymx_pt_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -73,6 +85,9 @@ SIGNATURES = {
     "ymx_pt_get_stats": (C.c_int, [C.c_void_p, C.POINTER(YmxPtStats)]),
     "ymx_pt_set_guide": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_int32, C.c_uint32]),
     "ymx_pt_set_guide_depth": (C.c_int, [C.c_void_p, C.c_int32]),
+    "ymx_pt_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(YmxPtTuning), C.c_void_p]),
+    "ymx_pt_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(YmxPtTuning)]),
+    "ymx_pt_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(YmxPtTuning)]),
     "ymx_pt_save_film": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32, C.c_int32]),
     "ymx_pt_load_film": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                    C.POINTER(C.c_int32)]),

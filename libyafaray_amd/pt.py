@@ -57,7 +57,7 @@ class YmxPtSettings(C.Structure):
                 ("aa_light_sample_multiplier_factor", C.c_float), ("aa_detect_color_noise", C.c_int32),
                 ("aa_dark_detection_type", C.c_int32), ("aa_dark_threshold_factor", C.c_float),
                 ("aa_variance_edge_size", C.c_int32), ("aa_variance_pixels", C.c_int32),
-                ("aa_clamp_samples", C.c_float)]
+                ("aa_clamp_samples", C.c_float), ("wavefront_mb", C.c_int32)]
 
 
 # ImageFilm's multi-pass AA params (include/render/imagefilm.h:212-226):
@@ -145,6 +145,7 @@ def build_structs(scene):
     st.raydepth = int(s.get("raydepth", 5))
     st.transparent_shadows = 1 if s.get("transpShad", False) else 0
     st.shadow_depth = int(s.get("shadowDepth", 4))
+    st.wavefront_mb = int(s.get("wavefront_mb", 0))
     for key, field, default in AA_PARAMS:
         v = s.get(key, default)
         if field == "aa_dark_detection_type":
@@ -256,6 +257,40 @@ class GpuPathTracer:
         w = np.zeros((self.resy, self.resx), dtype=np.float32)
         check(self._lib.ymx_pt_read_film(self._h, rgba.ctypes.data, w.ctypes.data), "ymx_pt_read_film")
         return rgba, w
+
+    KERNELS = {0: "wave-ordered", 1: "replacement"}
+
+    def autotune(self, trial_samples: int = 0) -> dict:
+        """ymx_pt_autotune: time short renders under each launch configuration
+        and keep the fastest (the film is left as it was)."""
+        t = native.YmxPtTuning()
+        t.struct_size = C.sizeof(native.YmxPtTuning)
+        check(self._lib.ymx_pt_autotune(self._h, int(trial_samples), C.byref(t), self._stream()), "ymx_pt_autotune")
+        out = self._tuning_dict(t)
+        out["trials"] = [{"samples_per_wavefront": t.trial_samples_per_wavefront[k],
+                          "shadow_kernel": self.KERNELS[t.trial_shadow_kernel[k]],
+                          "extension_kernel": self.KERNELS[t.trial_extension_kernel[k]],
+                          "ms_per_sample": round(float(t.trial_ms_per_sample[k]), 4)} for k in range(t.n_trials)]
+        return out
+
+    def _tuning_dict(self, t) -> dict:
+        return {"samples_per_wavefront": t.samples_per_wavefront, "shadow_kernel": self.KERNELS[t.shadow_kernel],
+                "extension_kernel": self.KERNELS[t.extension_kernel]}
+
+    def tuning(self) -> dict:
+        t = native.YmxPtTuning()
+        t.struct_size = C.sizeof(native.YmxPtTuning)
+        check(self._lib.ymx_pt_get_tuning(self._h, C.byref(t)), "ymx_pt_get_tuning")
+        return self._tuning_dict(t)
+
+    def set_tuning(self, samples_per_wavefront: int, shadow_kernel: str, extension_kernel: str):
+        inv = {v: k for k, v in self.KERNELS.items()}
+        t = native.YmxPtTuning()
+        t.struct_size = C.sizeof(native.YmxPtTuning)
+        t.samples_per_wavefront = int(samples_per_wavefront)
+        t.shadow_kernel = inv[shadow_kernel]
+        t.extension_kernel = inv[extension_kernel]
+        check(self._lib.ymx_pt_set_tuning(self._h, C.byref(t)), "ymx_pt_set_tuning")
 
     def stats(self) -> dict:
         s = YmxPtStats()

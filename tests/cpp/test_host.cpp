@@ -939,6 +939,9 @@ static void test_pathtracer_smooth_aa()
 	CHECK(pi != nullptr, "create: %s", err.c_str());
 	if(!pi) return;
 	std::vector<int> res;
+	// the autotuner leaves the film alone and changes no pixel of a render
+	ymx_pt_tuning tune{};
+	CHECK(pi->autotune(1, &tune) == YMX_OK && tune.n_trials >= 4, "autotune: %s", pi->lastError().c_str());
 	CHECK(pi->clearFilm() == YMX_OK && pi->renderPasses(&res) == YMX_OK, "render: %s", pi->lastError().c_str());
 	std::vector<float> rgba, w;
 	CHECK(pi->film(rgba, w) == YMX_OK, "film");
