@@ -79,6 +79,9 @@ def parse():
     p.add_argument("--guided-spp", type=int, default=640,
                    help="samples of each tracer; near the 1e-3 target so that little is extrapolated")
     p.add_argument("--guided-bsdf-prob", type=float, default=0.5)
+    p.add_argument("--guided-selection", choices=["learn", "fixed"], default="learn",
+                   help="BSDF selection probability: learned per region, or --guided-bsdf-prob everywhere")
+    p.add_argument("--guided-selection-lr", type=float, default=0.05)
     p.add_argument("--guided-steps", type=int, default=16, help="Adam steps after each training pass")
     p.add_argument("--guided-lr", type=float, default=1e-2)
     p.add_argument("--guided-depth", type=int, default=GUIDE_DEPTH,
@@ -265,6 +268,9 @@ def measure_guiding(args, torch):
         raise SystemExit(f"--guided-train-passes must divide {n0}")
     p.set_guide_depth(None if depth < 0 else depth)
     p.set_guide(g, bsdf_prob=alpha, train=True, train_steps=steps)
+    selection = getattr(args, "guided_selection", "learn")
+    if selection == "learn":
+        p.set_guide_selection("learn", lr=args.guided_selection_lr)
     p.clear()
 
     def train():
@@ -275,6 +281,12 @@ def measure_guiding(args, torch):
     rgba_t, w_t = p.film()
     e_train = relmse(ratio(rgba_t, w_t), ref) - ref_err
     p.set_guide(g, bsdf_prob=alpha, train=False)
+    sel_stats = None
+    if selection == "learn":
+        p.set_guide_selection("keep")
+        a_sel = p.guide_selection()
+        sel_stats = {"min": round(float(a_sel.min()), 3), "median": round(float(np.median(a_sel)), 3),
+                     "max": round(float(a_sel.max()), 3)}
     m = spp - n0
     t_rest = timed(lambda: p.render(n0, m))
     rgba_c, w_c = p.film()
@@ -298,6 +310,8 @@ def measure_guiding(args, torch):
         pg = tracer(n0 + m_v, 0)  # a tracer whose AA sample count covers every sample rendered
         pg.set_guide_depth(None if depth < 0 else depth)
         pg.set_guide(g2, bsdf_prob=alpha, train=True, train_steps=steps)
+        if selection == "learn":
+            pg.set_guide_selection("learn", lr=args.guided_selection_lr)
         pg.clear()
 
         def train_v():
@@ -306,6 +320,8 @@ def measure_guiding(args, torch):
 
         t_gv = timed(train_v)
         pg.set_guide(g2, bsdf_prob=alpha, train=False)
+        if selection == "learn":
+            pg.set_guide_selection("keep")
         t_gv += timed(lambda: pg.render(n0, m_v) if m_v > 0 else None)
         e_gv = relmse(image(pg), ref) - ref_err
         pg.set_guide(None)
@@ -323,6 +339,9 @@ def measure_guiding(args, torch):
     a_, b_, c_ = target, 2 * target * n0 - v_g, target * n0 * n0 - n0 * v_t
     m_need = 0.0 if c_ >= 0 else (-b_ + (b_ * b_ - 4 * a_ * c_) ** 0.5) / (2 * a_)
     tt_guided = t_train + m_need * t_rest / m
+    # equal time: the guided image rendered in the plain tracer's time for `spp`
+    m_eq = max(0.0, (t_plain - t_train) / (t_rest / m))
+    e_guided_eq = (n0 * v_t + m_eq * v_g) / (n0 + m_eq) ** 2
     out = {"workload": f"occluded room, {len(sc['tris'])} tris, {resx}x{resy}, depth 10, slit 0.1 m",
            "reference_spp": ref_spp, "reference_relmse": ref_err, "spp": spp,
            "plain_ms": round(t_plain * 1e3, 1),
@@ -332,6 +351,10 @@ def measure_guiding(args, torch):
            "guided_vertices": "all" if depth < 0 else f"camera vertex and the next {depth}",
            "post_train_ms_per_spp": round(t_rest / m * 1e3, 2), "plain_ms_per_spp": round(t_plain / spp * 1e3, 2),
            "bsdf_prob": alpha, "lr": args.guided_lr, "grid_levels": f"16 (resolution {n_min}..{n_max})",
+           "selection": selection if selection == "fixed" else f"learned per region (lr {args.guided_selection_lr})",
+           "learned_selection": sel_stats,
+           "equal_time_relmse_plain": e_plain, "equal_time_relmse_guided": e_guided_eq,
+           "equal_time_gain": round(e_plain / e_guided_eq, 3) if e_guided_eq > 0 else None,
            "relmse_plain": e_plain, "relmse_guided": e_all,
            "relmse_training_part": e_train, "relmse_post_training_part": e_rest,
            "equal_spp_gain": round(e_plain / e_all, 3) if e_all > 0 else None,

@@ -51,7 +51,7 @@ typedef struct ymx_guide_sample
 	float pos[3], c;            /* position; target weight c >= 0 */
 	float normal[3], pad0;
 	float wo[3], pad1;
-	float w[3], pad2;           /* the sampled direction */
+	float w[3], pad2;           /* the sampled direction (the pad fields are not read by the guide) */
 } ymx_guide_sample;
 
 typedef struct ymx_guide ymx_guide;

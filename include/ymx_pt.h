@@ -222,6 +222,22 @@ int ymx_pt_set_guide(ymx_pt *pt, ymx_guide *guide, float bsdf_prob, int32_t trai
  * reference.  Applies to the guide set now and to later ones. */
 int ymx_pt_set_guide_depth(ymx_pt *pt, int32_t max_depth);
 
+/* Learned selection probability (no counterpart in the reference).  The
+ * probability of sampling the BSDF at a guided vertex is kept per region of
+ * the guide's bounds (16 x 16 x 16 cells) in a table of the guide, and learned
+ * while the guide trains: Adam (learning rate lr) on the gradient of the
+ * second moment of the one-sample MIS estimator, estimated from the training
+ * samples, between 0.2 and 1.  The estimator stays unbiased for any value.
+ * mode 1: start every region at the bsdf_prob of ymx_pt_set_guide and learn;
+ * mode 2: use (and while training, go on learning) the guide's table as
+ * another tracer left it; mode 0: back to the fixed bsdf_prob.  Call after
+ * ymx_pt_set_guide; later ymx_pt_set_guide calls on this tracer keep the mode. */
+int ymx_pt_set_guide_selection(ymx_pt *pt, int32_t mode, float lr);
+
+/* The selection probability of every region in use: n >= 4096 floats, cell
+ * (x, y, z) at (z 16 + y) 16 + x. */
+int ymx_pt_get_guide_selection(const ymx_pt *pt, float *alpha, uint32_t n);
+
 /* Counters of the last ymx_pt_render: rays cast, and its duration in ms on
  * the stream (events around the whole render). */
 typedef struct ymx_pt_stats

@@ -51,6 +51,7 @@ struct ymx_guide
 	float *wgrad_part;                           // [kWgradBlocks][4][64 x 64] partial weight gradients
 	double *loss;
 	int step;
+	float *selection;                            // learned selection probability of the path tracer (NULL until asked)
 };
 
 namespace {
@@ -849,7 +850,8 @@ void free_guide(ymx_guide *g)
 {
 	if(!g) return;
 	void *ptrs[] = {g->grid, g->grid_m, g->grid_v, g->grid_g, g->grid_bf, g->mlp, g->mlp_m, g->mlp_v, g->mlp_g,
-	                g->w_bf, g->wt_bf, g->X, g->H1, g->H2, g->H3, g->dO, g->dH3, g->dH2, g->dH1, g->dXg, g->dgrad_part, g->wgrad_part, g->loss};
+	                g->w_bf, g->wt_bf, g->X, g->H1, g->H2, g->H3, g->dO, g->dH3, g->dH2, g->dH1, g->dXg, g->dgrad_part, g->wgrad_part, g->loss,
+	                g->selection};
 	for(void *p : ptrs)
 		if(p) (void)hipFree(p);
 	delete g;
@@ -867,6 +869,23 @@ int refresh_bf16(ymx_guide *g, hipStream_t s)
 }  // namespace
 
 uint32_t ymx_guide_batch_limit(const ymx_guide *g) { return g ? g->p.max_batch : 0u; }
+
+float *ymx_guide_selection_table(ymx_guide *g, size_t floats)
+{
+	if(!g) return nullptr;
+	if(!g->selection)
+	{
+		// a new table: every logit and moment zero (probability 0.6 until learned)
+		if(hipMalloc((void **)&g->selection, sizeof(float) * floats) != hipSuccess ||
+		   hipMemset(g->selection, 0, sizeof(float) * floats) != hipSuccess)
+		{
+			if(g->selection) (void)hipFree(g->selection);
+			g->selection = nullptr;
+			return nullptr;
+		}
+	}
+	return g->selection;
+}
 
 void ymx_guide_bounds(const ymx_guide *g, float bmin[3], float bmax[3])
 {

@@ -77,7 +77,7 @@ void free_guide_buffers(PtGuide &g)
 	                g.query_count};
 	for(void *p : ptrs)
 		if(p) (void)hipFree(p);
-	memset(&g, 0, sizeof g);
+	memset(&g, 0, sizeof g);  // the learned-selection arrays belong to ymx_pt
 }
 
 void free_pt(ymx_pt *pt)
@@ -728,9 +728,29 @@ int ymx_pt_set_guide(ymx_pt *pt, ymx_guide *guide, float bsdf_prob, int32_t trai
 	GB_TRY(dalloc(g.mix_node, (size_t)cap * YMX_GUIDE_MIX_FLOATS));
 	GB_TRY(dalloc(g.mix_path, (size_t)cap * YMX_GUIDE_MIX_FLOATS));
 	GB_TRY(dalloc(g.query_count, 1));
-	g.bsdf_prob = bsdf_prob < 0.2f ? 0.2f : (bsdf_prob > 1.f ? 1.f : bsdf_prob);
+	g.bsdf_prob = bsdf_prob < kAlphaMin ? kAlphaMin : (bsdf_prob > 1.f ? 1.f : bsdf_prob);
 	g.max_depth = pt->guide_depth < 0 ? INT32_MAX : pt->guide_depth;
 	g.on = 1u;
+	{
+		// regions of the guide's bounds (training-weight normalisation, learned selection)
+		float bmax[3];
+		ymx_guide_bounds(guide, g.bmin, bmax);
+		for(int k = 0; k < 3; ++k) g.scale[k] = (float)kGuideCells / (bmax[k] - g.bmin[k]);
+	}
+	if(pt->alpha_learned)
+	{
+		const size_t cells = (size_t)kGuideCells * kGuideCells * kGuideCells;
+		float *t = ymx_guide_selection_table(guide, 4 * cells);
+		if(t)
+		{
+			g.learn_alpha = 1u;
+			g.alpha_theta = t;
+			g.alpha_m = t + cells;
+			g.alpha_v = t + 2 * cells;
+			g.alpha_grad = t + 3 * cells;
+		}
+		else pt->alpha_learned = false;
+	}
 	if(train)
 	{
 		// ring of recent samples: a power of two, at least 4 mini-batches, 1M by default
@@ -745,9 +765,6 @@ int ymx_pt_set_guide(ymx_pt *pt, ymx_guide *guide, float bsdf_prob, int32_t trai
 		const size_t cells = (size_t)kGuideCells * kGuideCells * kGuideCells;
 		GB_TRY(dalloc(g.cell_sum, cells));
 		GB_TRY(hipMemset(g.cell_sum, 0, sizeof(float2) * cells));
-		float bmax[3];
-		ymx_guide_bounds(guide, g.bmin, bmax);
-		for(int k = 0; k < 3; ++k) g.scale[k] = (float)kGuideCells / (bmax[k] - g.bmin[k]);
 		GB_TRY(dalloc(g.train_count, 1));
 		GB_TRY(hipMemset(g.train_count, 0, sizeof(uint32_t)));
 		g.train_cap = (uint32_t)tcap;
@@ -759,6 +776,69 @@ int ymx_pt_set_guide(ymx_pt *pt, ymx_guide *guide, float bsdf_prob, int32_t trai
 		pt->g_rng = 0x853C49E6748FEA9Bull;
 	}
 #undef GB_TRY
+	return YMX_OK;
+}
+
+int ymx_pt_set_guide_selection(ymx_pt *pt, int32_t mode, float lr)
+{
+	if(!pt || !pt->guide || !(lr > 0.f) || lr > 1.f || mode < 0 || mode > 2)
+	{
+		ymx_set_error("ymx_pt_set_guide_selection: no guide set, mode outside 0..2 or lr outside (0, 1]");
+		return YMX_ERR_INVALID_ARG;
+	}
+	YMX_HIP_CHECK(hipDeviceSynchronize());
+	const size_t cells = (size_t)kGuideCells * kGuideCells * kGuideCells;
+	PtGuide &g = pt->g;
+	pt->alpha_lr = lr;
+	pt->alpha_steps = 0;
+	if(mode == 0)
+	{
+		pt->alpha_learned = false;
+		g.learn_alpha = 0u;
+		return YMX_OK;
+	}
+	float *t = ymx_guide_selection_table(pt->guide, 4 * cells);
+	if(!t)
+	{
+		ymx_set_error("ymx_pt_set_guide_selection: no device memory for the selection table");
+		return YMX_ERR_OUT_OF_MEMORY;
+	}
+	if(mode == 1)
+	{
+		// every region starts at the fixed probability:
+		// theta = logit((bsdf_prob - min) / (1 - min)), moments zero
+		const float s = (g.bsdf_prob - kAlphaMin) / (1.f - kAlphaMin);
+		const float sc = s < 1e-4f ? 1e-4f : (s > 1.f - 1e-4f ? 1.f - 1e-4f : s);
+		std::vector<float> init(4 * cells, 0.f);
+		for(size_t i = 0; i < cells; ++i) init[i] = logf(sc / (1.f - sc));
+		YMX_HIP_CHECK(hipMemcpy(t, init.data(), sizeof(float) * init.size(), hipMemcpyHostToDevice));
+	}
+	pt->alpha_learned = true;
+	g.learn_alpha = 1u;
+	g.alpha_theta = t;
+	g.alpha_m = t + cells;
+	g.alpha_v = t + 2 * cells;
+	g.alpha_grad = t + 3 * cells;
+	return YMX_OK;
+}
+
+int ymx_pt_get_guide_selection(const ymx_pt *pt, float *alpha, uint32_t n)
+{
+	const size_t cells = (size_t)kGuideCells * kGuideCells * kGuideCells;
+	if(!pt || !alpha || n < cells)
+	{
+		ymx_set_error("ymx_pt_get_guide_selection: NULL argument or fewer than %zu entries", cells);
+		return YMX_ERR_INVALID_ARG;
+	}
+	YMX_HIP_CHECK(hipDeviceSynchronize());
+	if(!pt->g.learn_alpha)
+	{
+		for(size_t i = 0; i < cells; ++i) alpha[i] = pt->g.bsdf_prob;
+		return YMX_OK;
+	}
+	std::vector<float> theta(cells);
+	YMX_HIP_CHECK(hipMemcpy(theta.data(), pt->g.alpha_theta, sizeof(float) * cells, hipMemcpyDeviceToHost));
+	for(size_t i = 0; i < cells; ++i) alpha[i] = kAlphaMin + (1.f - kAlphaMin) / (1.f + expf(-theta[i]));
 	return YMX_OK;
 }
 

@@ -172,7 +172,15 @@ struct PtGuide
 	uint32_t *train_count;
 	uint32_t train_cap;
 	uint32_t *query_count;     // queries of the current path-vertex batch
+	// learned selection probability (ymx_pt_set_guide_selection): per region,
+	// alpha = kAlphaMin + (1 - kAlphaMin) sigmoid(theta); the arrays are the
+	// guide's (ymx_guide_selection_table)
+	uint32_t learn_alpha;
+	float *alpha_theta;        // [kGuideCells^3]
+	float *alpha_m, *alpha_v;  // Adam moments
+	float *alpha_grad;         // gradient sums of one mini-batch
 };
+constexpr float kAlphaMin = 0.2f;  // the BSDF is sampled with probability >= 0.2 at a guided vertex
 
 }  // namespace ymx
 
@@ -204,6 +212,9 @@ struct ymx_pt
 	int32_t guide_depth;             // ymx_pt_set_guide_depth (-1: every vertex)
 	uint32_t g_seen, g_filled;       // training samples written so far (counter value), valid in the ring
 	uint64_t g_rng;                  // mini-batch offsets
+	bool alpha_learned;              // ymx_pt_set_guide_selection: the guide's learned selection is used
+	float alpha_lr;
+	uint32_t alpha_steps;
 	uint32_t batch;                  // AA samples per wavefront (<= batch_max; ymx_pt_autotune may lower it)
 	uint32_t batch_max;              // AA samples per wavefront the state is allocated for
 	bool shadow_replacement;         // shadow queues traced with ray replacement (k_dyn), else wave-ordered
@@ -243,3 +254,7 @@ int ymx_pt_train_guide(ymx_pt *pt, hipStream_t stream);
 // Most samples one training step of the guide takes, and its bounding box (guide.hip).
 uint32_t ymx_guide_batch_limit(const ymx_guide *g);
 void ymx_guide_bounds(const ymx_guide *g, float bmin[3], float bmax[3]);
+// The guide's table of the path tracer's learned selection probability
+// (`floats` floats, allocated on first use and freed with the guide; NULL
+// when the allocation fails).  Tracers sharing the guide share the table.
+float *ymx_guide_selection_table(ymx_guide *g, size_t floats);

@@ -600,6 +600,36 @@ __device__ __forceinline__ float guide_rand(const NodeCtx &x, uint32_t ps, uint3
 	                 seed ^ 0xA511E9B3u);
 }
 
+// Region of a position (training-weight normalisation, learned selection).
+__device__ __forceinline__ uint32_t guide_cell(const PtGuide &g, const float p[3])
+{
+	uint32_t c[3];
+	for(int k = 0; k < 3; ++k)
+	{
+		const float f = (p[k] - g.bmin[k]) * g.scale[k];
+		c[k] = !(f > 0.f) ? 0u : (f >= (float)(kGuideCells - 1) ? (uint32_t)(kGuideCells - 1) : (uint32_t)f);
+	}
+	return (c[2] * kGuideCells + c[1]) * kGuideCells + c[0];
+}
+
+// The probability of sampling the BSDF at a guided vertex at p: the fixed
+// bsdf_prob, or the region's learned value.
+__device__ __forceinline__ float selection_prob(const PtGuide &g, V3 p)
+{
+	if(!g.learn_alpha) return g.bsdf_prob;
+	const float pp[3] = {p.x, p.y, p.z};
+	const float t = g.alpha_theta[guide_cell(g, pp)];
+	return kAlphaMin + (1.f - kAlphaMin) / (1.f + __expf(-t));
+}
+
+// The densities a guided direction was drawn with: the BSDF's cosine lobe p,
+// the mixture q and the selection probability alpha (kept in the training
+// record for the selection's gradient).
+struct GuidePdfs
+{
+	float p, q, alpha;
+};
+
 // Next direction at a guided vertex: the BSDF's own sample with probability
 // alpha, otherwise the mixture's; weight = F / (alpha p + (1 - alpha) q) with
 // F = sw * p the plain sampler's integrand, p = cos / pi the density of its
@@ -610,10 +640,10 @@ __device__ __forceinline__ float guide_rand(const NodeCtx &x, uint32_t ps, uint3
 // kappa.  w_plain: the sampler's w for wi (what a later vertex that leaves w
 // unset reuses).  Returns false when the weight is 0.
 __device__ bool guided_sample(const PtArgs &a, const Surf &sp, V3 wo, const float *mix, BSample &s, float u_sel,
-                              float u0, float u1, float u2, V3 &wi, Rgb &weight, float &w_plain)
+                              float u0, float u1, float u2, V3 &wi, Rgb &weight, float &w_plain, GuidePdfs &pdfs)
 {
 	const PtMaterial &m = a.mats[sp.mat];
-	const float alpha = a.g.bsdf_prob;
+	const float alpha = selection_prob(a.g, sp.p);
 	Rgb sw;
 	if(u_sel < alpha)
 	{
@@ -641,6 +671,9 @@ __device__ bool guided_sample(const PtArgs &a, const Surf &sp, V3 wo, const floa
 	const float p = cos_n > 0.f ? cos_n * k1Pi : 0.f;
 	const float q = ymx::guide::mix_pdf(mix, wn.x, wn.y, wn.z);
 	const float denom = alpha * p + (1.f - alpha) * q;
+	pdfs.p = p;
+	pdfs.q = q;
+	pdfs.alpha = alpha;
 	if(!(p > 0.f) || !(denom > 0.f))
 	{
 		weight = rgb(0.f, 0.f, 0.f);
@@ -650,24 +683,12 @@ __device__ bool guided_sample(const PtArgs &a, const Surf &sp, V3 wo, const floa
 	return true;
 }
 
-// Region of a guide sample's position for the training-weight normalisation.
-__device__ __forceinline__ uint32_t guide_cell(const PtGuide &g, const float p[3])
-{
-	uint32_t c[3];
-	for(int k = 0; k < 3; ++k)
-	{
-		const float f = (p[k] - g.bmin[k]) * g.scale[k];
-		c[k] = !(f > 0.f) ? 0u : (f >= (float)(kGuideCells - 1) ? (uint32_t)(kGuideCells - 1) : (uint32_t)f);
-	}
-	return (c[2] * kGuideCells + c[1]) * kGuideCells + c[0];
-}
-
 __device__ __forceinline__ float luminance(Rgb c) { return 0.2126f * c.r + 0.7152f * c.g + 0.0722f * c.b; }
 
 // A pending training sample of segment r (position, normal, wo, direction) and
 // the throughput before it; the path colour before it is set in base.xyz.
 __device__ __forceinline__ void record_segment(const PtArgs &a, uint32_t slot, uint32_t r, const Surf &sp, V3 wo, V3 wi,
-                                               float tp_lum)
+                                               float tp_lum, const GuidePdfs &pdfs)
 {
 	ymx_guide_sample rec;
 	const V3 n = face_forward(sp.ng, sp.n, wo);
@@ -678,15 +699,15 @@ __device__ __forceinline__ void record_segment(const PtArgs &a, uint32_t slot, u
 	rec.normal[0] = n.x;
 	rec.normal[1] = n.y;
 	rec.normal[2] = n.z;
-	rec.pad0 = 0.f;
+	rec.pad0 = pdfs.p;      // the densities of the draw, for the selection's gradient
 	rec.wo[0] = wo.x;
 	rec.wo[1] = wo.y;
 	rec.wo[2] = wo.z;
-	rec.pad1 = 0.f;
+	rec.pad1 = pdfs.q;
 	rec.w[0] = wi.x;
 	rec.w[1] = wi.y;
 	rec.w[2] = wi.z;
-	rec.pad2 = 0.f;
+	rec.pad2 = pdfs.alpha;
 	const size_t k = (size_t)r * a.cnt + slot;
 	a.g.rec[k] = rec;
 	a.g.base[k].w = tp_lum;
@@ -1432,13 +1453,14 @@ __global__ __launch_bounds__(kBlock) void k_subpath_init(PtArgs a)
 	if(guided_material(a, sp))
 	{
 		const uint32_t ps = x.sample * (uint32_t)c.path_samples + (uint32_t)a.sub;
+		GuidePdfs pdfs;
 		guided_sample(a, sp, wo, a.g.mix_node + (size_t)i * ymx::guide::kMix, s, guide_rand(x, ps, 0, c.seed, 0),
 		              guide_rand(x, ps, 0, c.seed, 1), guide_rand(x, ps, 0, c.seed, 2), guide_rand(x, ps, 0, c.seed, 3),
-		              pdir, scol, w);
+		              pdir, scol, w, pdfs);
 		if(a.g.train)
 		{
 			// the path colour before this sub-path, throughput 1
-			record_segment(a, i, 0, sp, wo, pdir, 1.f);
+			record_segment(a, i, 0, sp, wo, pdir, 1.f, pdfs);
 			const float4 pc = a.st.path_col[i];
 			a.g.base[i].x = pc.x;
 			a.g.base[i].y = pc.y;
@@ -1560,13 +1582,14 @@ __global__ __launch_bounds__(kBlock) void k_shade_path(PtArgs a)
 		V3 ndir = dir;
 		Rgb scol;
 		const bool guided = d < a.g.max_depth && guided_material(a, sp);
+		GuidePdfs pdfs = {0.f, 0.f, 0.f};
 		if(guided)
 		{
 			const uint32_t ps = x.sample * (uint32_t)c.path_samples + (uint32_t)a.sub;
 			const uint32_t seg = (uint32_t)d + 1u;
 			guided_sample(a, sp, pwo, a.g.mix_path + (size_t)q * ymx::guide::kMix, s, guide_rand(x, ps, seg, c.seed, 0),
 			              guide_rand(x, ps, seg, c.seed, 1), guide_rand(x, ps, seg, c.seed, 2),
-			              guide_rand(x, ps, seg, c.seed, 3), ndir, scol, w);
+			              guide_rand(x, ps, seg, c.seed, 3), ndir, scol, w, pdfs);
 		}
 		else
 		{
@@ -1575,7 +1598,8 @@ __global__ __launch_bounds__(kBlock) void k_shade_path(PtArgs a)
 		}
 		if(!cblack(scol))
 		{
-			if(guided && a.g.train && !cblack(tp)) record_segment(a, i, (uint32_t)d + 1u, sp, pwo, ndir, luminance(tp));
+			if(guided && a.g.train && !cblack(tp))
+				record_segment(a, i, (uint32_t)d + 1u, sp, pwo, ndir, luminance(tp), pdfs);
 			const Rgb tpn2 = cmul(tp, scol);
 			a.st.tp_next[i] = make_float4(tpn2.r, tpn2.g, tpn2.b, w);
 			if(s.sampled & (kSpecular | kGlossy | kFilter)) f |= kCaustic;
@@ -1686,6 +1710,50 @@ __global__ __launch_bounds__(kBlock) void k_guide_gather(ymx_guide_sample *dst, 
 	const float2 cs = g.cell_sum[guide_cell(g, s.pos)];
 	s.c = cs.x > 0.f ? s.c * (cs.y / cs.x) : s.c;
 	dst[k] = s;
+}
+
+// Learned selection probability (ymx_pt_set_guide_selection).  A record drawn
+// with densities p (BSDF), q (mixture) and alpha_old has target weight c =
+// F / p_old, F the integrand and p_old = alpha_old p + (1 - alpha_old) q.
+// The second moment of the one-sample estimator F / p_mix(alpha) over its own
+// density is M = integral of F^2 / p_mix; its derivative in alpha, estimated
+// from the records' draws, is -F^2 (p - q) / (p_mix^2 p_old) per record.  The
+// region's logit theta takes the sum times d alpha / d theta; Adam (per
+// region) then moves alpha towards the strategy of smaller variance.  The
+// records' weights carry their region's normalisation, one factor per region,
+// which Adam's scaling removes.
+__global__ __launch_bounds__(kBlock) void k_alpha_grad(const ymx_guide_sample *batch, uint32_t n, PtGuide g)
+{
+	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if(i >= n) return;
+	const ymx_guide_sample r = batch[i];
+	const float p = r.pad0, q = r.pad1, a_old = r.pad2;
+	const float p_old = a_old * p + (1.f - a_old) * q;
+	if(!(r.c > 0.f) || !(p_old > 0.f) || !(a_old > 0.f)) return;
+	const uint32_t cell = guide_cell(g, r.pos);
+	const float sg = 1.f / (1.f + __expf(-g.alpha_theta[cell]));
+	const float alpha = kAlphaMin + (1.f - kAlphaMin) * sg;
+	const float pm = alpha * p + (1.f - alpha) * q;
+	const float f = r.c * p_old;
+	const float d_alpha = -f * f * (p - q) / (pm * pm * p_old);
+	const float d_theta = d_alpha * (1.f - kAlphaMin) * sg * (1.f - sg);
+	if(isfinite(d_theta)) atomicAdd(g.alpha_grad + cell, d_theta);
+}
+
+__global__ __launch_bounds__(kBlock) void k_alpha_adam(PtGuide g, uint32_t cells, float lr, float bc1, float bc2)
+{
+	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if(i >= cells) return;
+	const float gr = g.alpha_grad[i];
+	g.alpha_grad[i] = 0.f;
+	if(gr == 0.f) return;  // no sample of this region in the batch
+	const float m = 0.9f * g.alpha_m[i] + 0.1f * gr;
+	const float v = 0.999f * g.alpha_v[i] + 0.001f * gr * gr;
+	g.alpha_m[i] = m;
+	g.alpha_v[i] = v;
+	float t = g.alpha_theta[i] - lr * (m / bc1) / (sqrtf(v / bc2) + 1e-30f);
+	// past +-8 the probability no longer moves (sigmoid within 3e-4 of its ends)
+	g.alpha_theta[i] = t > 8.f ? 8.f : (t < -8.f ? -8.f : t);
 }
 
 // Queries of the guide at the node vertices that start sub-paths.
@@ -2344,6 +2412,16 @@ int ymx_pt_train_guide(ymx_pt *pt, hipStream_t stream)
 			YMX_HIP_CHECK(hipGetLastError());
 			const int rc = ymx_guide_train_step(pt->guide, pt->g.batch_buf, batch, nullptr, stream);
 			if(rc != YMX_OK) return rc;
+			if(pt->g.learn_alpha)
+			{
+				constexpr uint32_t cells = (uint32_t)(kGuideCells * kGuideCells * kGuideCells);
+				k_alpha_grad<<<blocks_for(batch), kBlock, 0, stream>>>(pt->g.batch_buf, batch, pt->g);
+				++pt->alpha_steps;
+				const float bc1 = 1.f - powf(0.9f, (float)pt->alpha_steps);
+				const float bc2 = 1.f - powf(0.999f, (float)pt->alpha_steps);
+				k_alpha_adam<<<blocks_for(cells), kBlock, 0, stream>>>(pt->g, cells, pt->alpha_lr, bc1, bc2);
+				YMX_HIP_CHECK(hipGetLastError());
+			}
 		}
 	}
 	const double records = kept / (double)(pt->g.accept > 0.f ? pt->g.accept : 1.f);

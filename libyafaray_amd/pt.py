@@ -258,6 +258,21 @@ class GpuPathTracer:
         check(self._lib.ymx_pt_read_film(self._h, rgba.ctypes.data, w.ctypes.data), "ymx_pt_read_film")
         return rgba, w
 
+    SELECTION = {"fixed": 0, "learn": 1, "keep": 2}
+
+    def set_guide_selection(self, mode: str, lr: float = 0.05):
+        """ymx_pt_set_guide_selection (after set_guide): "learn" the
+        BSDF-sampling probability per region from the fixed one, "keep" the
+        guide's learned table, or go back to "fixed"."""
+        check(self._lib.ymx_pt_set_guide_selection(self._h, self.SELECTION[mode], C.c_float(lr)),
+              "ymx_pt_set_guide_selection")
+
+    def guide_selection(self) -> np.ndarray:
+        """Selection probability per region, shape (16, 16, 16) as [z, y, x]."""
+        a = np.zeros(16 ** 3, dtype=np.float32)
+        check(self._lib.ymx_pt_get_guide_selection(self._h, a.ctypes.data, a.size), "ymx_pt_get_guide_selection")
+        return a.reshape(16, 16, 16)
+
     KERNELS = {0: "wave-ordered", 1: "replacement"}
 
     def autotune(self, trial_samples: int = 0) -> dict:

@@ -238,3 +238,25 @@ def adam_step(p, m, v, g, t, lr=1e-2, beta1=0.9, beta2=0.99, eps=1e-8):
     v = b2 * v + (f(1) - b2) * g * g
     p = p - f(lr) * (m / c1) / (np.sqrt(v / c2) + f(eps))
     return p.astype(np.float32), m.astype(np.float32), v.astype(np.float32)
+
+
+ALPHA_MIN = 0.2
+
+
+def selection_alpha(theta):
+    """Learned selection probability of a region from its logit."""
+    return ALPHA_MIN + (1.0 - ALPHA_MIN) / (1.0 + np.exp(-np.asarray(theta, np.float64)))
+
+
+def selection_grad(c, p, q, alpha_old, theta):
+    """Per-record gradient of the one-sample MIS estimator's second moment in
+    the region's logit theta (the path tracer's k_alpha_grad): a record drawn
+    with densities p (BSDF), q (mixture) at alpha_old has weight c = F / p_old;
+    d M / d alpha = -F^2 (p - q) / (p_mix(alpha)^2 p_old), times d alpha / d theta."""
+    c, p, q, a0 = (np.asarray(v, np.float64) for v in (c, p, q, alpha_old))
+    s = 1.0 / (1.0 + np.exp(-np.asarray(theta, np.float64)))
+    alpha = ALPHA_MIN + (1.0 - ALPHA_MIN) * s
+    p_old = a0 * p + (1.0 - a0) * q
+    pm = alpha * p + (1.0 - alpha) * q
+    f = c * p_old
+    return -f * f * (p - q) / (pm * pm * p_old) * (1.0 - ALPHA_MIN) * s * (1.0 - s)

@@ -168,20 +168,29 @@ def test_training_learns_a_position_dependent_lobe():
 
 
 # ---------------------------------------------------------------- guided path tracing
-def _render(sc, spp, guide=None, train_passes=0, seed=0, seeds=(0,), depth=None):
+def _render(sc, spp, guide=None, train_passes=0, seed=0, seeds=(0,), depth=None, learned=False, selection=None):
     """Film images (one per seed) of a plain or guided render; with
-    train_passes the guide first trains on that many 1-spp passes; depth: the
-    path vertices after the sub-path start that are guided (None: all)."""
+    train_passes the guide first trains on that many 1-spp passes (learned:
+    with the selection probability learned per region, returned in
+    selection[0]); depth: the path vertices after the sub-path start that are
+    guided (None: all)."""
     from libyafaray_amd.pt import GpuPathTracer
     s = dict(sc)
     s["settings"] = dict(sc["settings"], aa_samples=spp, seed=seed)
     pt = GpuPathTracer(s)
     pt.set_guide_depth(depth)
+    alpha = None
     if guide is not None and train_passes:
         pt.set_guide(guide, bsdf_prob=0.3, train=True, train_steps=8)
+        if learned:
+            pt.set_guide_selection("learn", lr=0.05)
         pt.clear()
         for k in range(train_passes):
             pt.render(k, 1)
+        if learned:
+            alpha = pt.guide_selection()
+            if selection is not None:
+                selection.append(alpha)
     pt.close()
     imgs = []
     for sd in seeds:
@@ -190,6 +199,8 @@ def _render(sc, spp, guide=None, train_passes=0, seed=0, seeds=(0,), depth=None)
         pt.set_guide_depth(depth)
         if guide is not None:
             pt.set_guide(guide, bsdf_prob=0.3, train=False)
+            if learned:
+                pt.set_guide_selection("keep")
         pt.clear()
         pt.render(0, spp)
         rgba, w = pt.film()
@@ -198,15 +209,17 @@ def _render(sc, spp, guide=None, train_passes=0, seed=0, seeds=(0,), depth=None)
     return imgs
 
 
-@pytest.mark.parametrize("depth", [None, 1])
-def test_guided_render_converges_to_the_plain_image(depth):
+@pytest.mark.parametrize("depth,learned", [(None, False), (1, False), (1, True)])
+def test_guided_render_converges_to_the_plain_image(depth, learned):
     """One-sample MIS keeps the estimator unbiased: with an untrained and with a
     trained guide, the image mean of independent guided renders (seeds 1..8,
     256 spp each: the guided directions and the roulette draw from the seed)
     agrees with a 4096-spp plain render's within four standard errors of
     their spread (a wrong MIS weight or a biased roulette is off by per cents),
     and the guided images are finite and of the same order of error.  Guiding
-    every vertex, and only the camera vertex and the next one."""
+    every vertex, and only the camera vertex and the next one; with the
+    selection probability learned per region, which also stays within
+    [0.2, 1] and moves away from its start."""
     from libyafaray_amd import scenes
     from libyafaray_amd.guide import GpuGuide
     sc = scenes.cornell_path_scene(n_random=2000, res=48, spp=16)
@@ -217,8 +230,15 @@ def test_guided_render_converges_to_the_plain_image(depth):
     say(f"  plain 256 spp: mean {plain.mean():.5f} (reference {ref.mean():.5f}), relMSE {rel(plain):.2e}")
     for train in (0, 8):
         g = GpuGuide(bmin, bmax, log2_table_size=16, max_batch=1 << 14, seed=3)
-        imgs = _render(sc, 256, guide=g, train_passes=train, seeds=tuple(range(1, 9)), depth=depth)
+        sel = []
+        imgs = _render(sc, 256, guide=g, train_passes=train, seeds=tuple(range(1, 9)), depth=depth, learned=learned,
+                       selection=sel)
         g.close()
+        if learned and train:
+            a = sel[0]
+            moved = np.abs(a - 0.3) > 0.05
+            say(f"  learned selection: min {a.min():.3f} max {a.max():.3f}, {moved.sum()} of {a.size} regions moved")
+            assert np.all((a >= 0.2 - 1e-6) & (a <= 1 + 1e-6)) and moved.sum() > 0
         means = np.array([im.mean() for im in imgs], np.float64)
         m, se = means.mean(), means.std(ddof=1) / np.sqrt(len(means))
         errs = [rel(im) for im in imgs]

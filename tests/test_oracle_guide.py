@@ -231,3 +231,33 @@ def test_adam_first_steps_closed_form():
     assert np.allclose(p2, p1 - lr * mh / (np.sqrt(vh) + eps), rtol=0, atol=4e-7)
     # a zero gradient from zero moments leaves the weight where it is
     assert np.array_equal(G.adam_step(p0, z, z, z, 1)[0], p0)
+
+
+# ---------------------------------------------------------------- learned selection probability
+def test_selection_gradient_is_the_second_moments_derivative():
+    """Over a discrete set of directions with integrand F, BSDF density p and
+    mixture density q, the second moment of F / p_mix(alpha) under its own
+    density is M(theta) = sum F^2 / p_mix.  The expectation of the per-record
+    gradient under the drawing density p_old (any alpha_old) equals dM/dtheta
+    by central differences; its sign sends alpha towards the BSDF when the
+    mixture misses F and towards the mixture when it matches F."""
+    from oracle import ora_guide as G
+    rng = np.random.default_rng(3)
+    n = 50
+    F = rng.gamma(0.5, 1.0, n)
+    p = rng.dirichlet(np.ones(n))
+    for q, expect_sign in ((F / F.sum(), +1), (rng.dirichlet(np.ones(n) * 0.1), None)):
+        for a_old, theta in ((0.5, -0.5), (0.9, 1.3), (0.25, -2.0)):
+            p_old = a_old * p + (1 - a_old) * q
+            c = F / p_old
+            grad = float((p_old * G.selection_grad(c, p, q, a_old, theta)).sum())
+
+            def M(th):
+                al = G.selection_alpha(th)
+                return float((F * F / (al * p + (1 - al) * q)).sum())
+            h = 1e-5
+            num = (M(theta + h) - M(theta - h)) / (2 * h)
+            assert grad == pytest.approx(num, rel=1e-5)
+            if expect_sign is not None:
+                # q proportional to F is the zero-variance density: lowering alpha lowers M
+                assert grad > 0
