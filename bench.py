@@ -71,6 +71,8 @@ def parse():
     p.add_argument("--no-pt", action="store_true", help="skip the path-tracing measurements")
     p.add_argument("--no-terrain", action="store_true", help="skip the 1920x1080x64 terrain frame")
     p.add_argument("--terrain-only", action="store_true", help="only the 1920x1080x64 terrain frame")
+    p.add_argument("--large-only", action="store_true", help="only the 3840x2160 large-terrain frame")
+    p.add_argument("--large-spp", type=int, default=4, help="samples of the large-terrain frame")
     p.add_argument("--pt-only", action="store_true", help="only the path-tracing measurement")
     p.add_argument("--no-guided", action="store_true", help="skip the guided path-tracing measurement")
     p.add_argument("--guided-only", action="store_true", help="only the guided path-tracing measurement")
@@ -174,6 +176,53 @@ def measure_terrain_frame(args, torch):
            "shadow_rays": int(st["shadow_rays"]), "setup_ms": round(setup_s * 1e3, 1),
            "autotune_ms": round(tune_s * 1e3, 1), "tuning": tuning}
     log(args, f"terrain frame: {out}")
+    return out
+
+
+def measure_large_frame(args, torch):
+    """The large terrain at its full size (`scenes.large_terrain_scene`: 4096^2
+    heightfield + 2000 icospheres, 36.1M triangles, 256 point lights + 2 area
+    lights, depth 12) at 3840 x 2160 and args.large_spp samples: BVH build,
+    wavefront state, one frame.  Not in the default run (the scene takes about
+    half a minute to generate on the host)."""
+    from libyafaray_amd import scenes
+    from libyafaray_amd.pt import GpuPathTracer
+    tg = time.perf_counter()
+    sc = scenes.large_terrain_scene(spp=args.large_spp)
+    gen_s = time.perf_counter() - tg
+    tb = time.perf_counter()
+    pt = GpuPathTracer(sc, treelet_passes=args.pt_treelet_passes)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - tb
+    pt.clear()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    st = {"camera_rays": 0, "extension_rays": 0, "shadow_rays": 0}
+    while done < args.large_spp:  # in slices of 16 samples, each reported (stats are per render call)
+        k = min(16, args.large_spp - done)
+        pt.render(done, k)
+        torch.cuda.synchronize()
+        done += k
+        s_k = pt.stats()
+        for key in st:
+            st[key] += s_k[key]
+        log(args, f"large frame: {done} of {args.large_spp} spp, {time.perf_counter() - t0:.1f} s")
+    frame_s = time.perf_counter() - t0
+    tuning = pt.tuning()
+    rgba, w = pt.film()
+    pt.close()
+    img = rgba[..., :3] / w[..., None]
+    cam = sc["camera"]
+    rays = st["camera_rays"] + st["extension_rays"] + st["shadow_rays"]
+    samples = cam["resx"] * cam["resy"] * args.large_spp
+    out = {"workload": f"large terrain 4096^2 + 2000 icospheres ({len(sc['tris'])} tris), {cam['resx']}x{cam['resy']}, "
+                       f"{args.large_spp} spp, depth {sc['settings']['bounces']}, {len(sc['lights'])} lights",
+           "scene_generation_s": round(gen_s, 1), "setup_s": round(setup_s, 2), "frame_s": round(frame_s, 3),
+           "msamples_per_s": round(samples / frame_s / 1e6, 2), "mrays_per_s": round(rays / frame_s / 1e6, 1),
+           "rays": int(rays), "shadow_rays": int(st["shadow_rays"]), "samples_per_wavefront": tuning["samples_per_wavefront"],
+           "finite": bool(np.isfinite(img).all()), "mean": round(float(img.mean()), 5)}
+    log(args, f"large frame: {out}")
     return out
 
 
@@ -643,6 +692,9 @@ def main():
         return
     if args.terrain_only:
         print(json.dumps({"terrain_frame": measure_terrain_frame(args, torch)}), flush=True)
+        return
+    if args.large_only:
+        print(json.dumps({"large_frame": measure_large_frame(args, torch)}), flush=True)
         return
     if args.guided_only:
         print(json.dumps({"guide_kernels": measure_guide_kernels(args, torch),

@@ -86,9 +86,9 @@ typedef struct ymx_accel_params
 	int32_t ray_reorder;    /* closest- and any-hit batches trace their rays grouped by
 	                           direction octant and origin cell (a device counting
 	                           sort of the ray indices; results stay in the caller's
-	                           order): 0 = batches of at least 2^18 rays that are not
-	                           already coherent (decided on the device, no host wait),
-	                           1 = always, -1 = never */
+	                           order): 0 = closest-hit batches of at least 2^18 rays
+	                           that are not already coherent (decided on the device,
+	                           no host wait), 1 = always (any hit too), -1 = never */
 	int32_t reorder_cell_bits; /* sort granularity of ray reordering: origin cells per
 	                              axis of the tree bound, 2^bits (3: 4,096 buckets with
 	                              the octant, 4: 32,768); 0 = default
