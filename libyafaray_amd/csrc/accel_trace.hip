@@ -345,26 +345,20 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(ymx_trace_args args)
 				const float thi = best_t + best_t * kWindow;
 				float tn0, tn1;
 				const int m = box2(f0, f1, f2, r, thi, tn0, tn1);
-				int32_t c0 = __float_as_int(f3.x), c1 = __float_as_int(f3.y);
-				if(m == 3)
+				const int32_t c0 = __float_as_int(f3.x), c1 = __float_as_int(f3.y);
+				// both children hit: the nearer next and the farther on the stack; one
+				// hit: that one; none: the stack's top.  Selects for the next node, so
+				// only the push and the pop stay branches.
+				const bool swap = tn1 < tn0;
+				int32_t next = m == 3 ? (swap ? c1 : c0) : (m == 1 ? c0 : c1);
+				if(m == 3) st.push(swap ? c0 : c1);
+				if(m == 0) next = st.pop();
+				if(next < 0 && next != YMX_REF_EMPTY && leaf == YMX_REF_EMPTY)
 				{
-					if(tn1 < tn0)
-					{
-						const int32_t tmp = c0;
-						c0 = c1;
-						c1 = tmp;
-					}
-					st.push(c1);
-					ref = c0;
+					leaf = next;
+					next = st.pop();
 				}
-				else if(m == 1) ref = c0;
-				else if(m == 2) ref = c1;
-				else ref = st.pop();
-				if(ref < 0 && ref != YMX_REF_EMPTY && leaf == YMX_REF_EMPTY)
-				{
-					leaf = ref;
-					ref = st.pop();
-				}
+				ref = next;
 				if(__all(leaf != YMX_REF_EMPTY || ref < 0)) break;
 			}
 			if(leaf == YMX_REF_EMPTY)
@@ -583,26 +577,20 @@ __global__ __launch_bounds__(kTraceBlock) void k_dyn(ymx_trace_args args)
 					const float thi = best_t + best_t * kWindow;
 					float tn0, tn1;
 					const int m = box2(f0, f1, f2, r, thi, tn0, tn1);
-					int32_t c0 = __float_as_int(f3.x), c1 = __float_as_int(f3.y);
-					if(m == 3)
+					const int32_t c0 = __float_as_int(f3.x), c1 = __float_as_int(f3.y);
+					// both children hit: the nearer next and the farther on the stack; one
+					// hit: that one; none: the stack's top.  Selects for the next node, so
+					// only the push and the pop stay branches.
+					const bool swap = tn1 < tn0;
+					int32_t next = m == 3 ? (swap ? c1 : c0) : (m == 1 ? c0 : c1);
+					if(m == 3) st.push(swap ? c0 : c1);
+					if(m == 0) next = st.pop();
+					if(next < 0 && next != YMX_REF_EMPTY && leaf == YMX_REF_EMPTY)
 					{
-						if(tn1 < tn0)
-						{
-							const int32_t tmp = c0;
-							c0 = c1;
-							c1 = tmp;
-						}
-						st.push(c1);
-						ref = c0;
+						leaf = next;
+						next = st.pop();
 					}
-					else if(m == 1) ref = c0;
-					else if(m == 2) ref = c1;
-					else ref = st.pop();
-					if(ref < 0 && ref != YMX_REF_EMPTY && leaf == YMX_REF_EMPTY)
-					{
-						leaf = ref;
-						ref = st.pop();
-					}
+					ref = next;
 					if(__all(leaf != YMX_REF_EMPTY || ref < 0)) break;
 				}
 				if(leaf == YMX_REF_EMPTY && ref < 0 && ref != YMX_REF_EMPTY)
