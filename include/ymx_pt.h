@@ -23,14 +23,19 @@
  * glass (no dispersion or absorption) and light_mat; specular and glossy
  * (as_diffuse = 0) surfaces recurse as the reference's recursiveRaytrace does,
  * up to `raydepth`.  Lights: area (corner / point1 / point2, colour, power,
- * samples), point (from, colour, power) and object (a mesh of the scene: a
- * triangle range, colour, power, samples, double_sided).  Param meanings and defaults follow
+ * samples), point (from, colour, power), object (a mesh of the scene: a
+ * triangle range, colour, power, samples, double_sided) and background portal
+ * (a mesh of the scene through which the constant background lights the
+ * scene: a triangle range, power, samples, ibl_clamp_sampling; one-sided, the
+ * background's colour times power).  Object and portal lights of more than 16
+ * triangles test BSDF-sampled rays against a BVH of their own, as the
+ * reference's lights do with their private kd-trees.  Param meanings and defaults follow
  * the reference's ParamMaps (include/material/material_shiny_diffuse.h:85-96,
  * include/material/material_glossy.h:68-80, include/material/material_mirror.h:57-58,
  * include/material/material_glass.h:68-75, include/material/material_light.h:59-61,
  * include/integrator/surface/integrator_montecarlo.h:47,
  * include/light/light_area.h:50-57, include/light/light_point.h:52-54,
- * include/light/light_object_light.h:53-61,
+ * include/light/light_object_light.h:53-62, include/light/light_background_portal.h:58-61,
  * include/integrator/surface/integrator_path_tracer.h:55-58,
  * include/render/imagefilm.h:196-225, include/camera/camera.h:82-89).
  */
@@ -54,6 +59,7 @@ extern "C" {
 #define YMX_LIGHT_AREA 0u
 #define YMX_LIGHT_POINT 1u
 #define YMX_LIGHT_OBJECT 2u     /* "objectlight" */
+#define YMX_LIGHT_PORTAL 3u     /* "bgPortalLight" */
 
 /* One record for every material type; each type reads its own fields, named
  * after the reference's params (defaults in brackets). */
@@ -85,12 +91,15 @@ typedef struct ymx_light
 	float from[3];                          /* point light position */
 	float color[3];
 	float power;
-	int32_t samples;                        /* area / object light samples, default 4 */
+	int32_t samples;                        /* area / object light samples, default 4; portal 16 */
 	uint32_t cast_shadows;                  /* default 1 */
-	/* object light: its object ("object_name") is the triangle range
+	/* object and portal lights: the object ("object_name") is the triangle range
 	 * [first_tri, first_tri + n_tris) of the path tracer's triangles */
 	uint32_t first_tri, n_tris;
 	uint32_t double_sided;                  /* object light "double_sided", default 0 */
+	/* portal light "ibl_clamp_sampling": the colour its intersect returns is
+	 * clamped proportionally to this maximum, 0 off [0] */
+	float ibl_clamp_sampling;
 } ymx_light;
 
 typedef struct ymx_pt_settings
@@ -249,6 +258,7 @@ typedef struct ymx_pt_stats
 	float render_ms;
 	uint64_t guide_samples;       /* training samples the paths produced (before subsampling) */
 	uint64_t guide_steps;         /* guide training steps taken */
+	uint32_t light_trees;         /* object and portal lights tested through their own BVH */
 } ymx_pt_stats;
 
 int ymx_pt_get_stats(const ymx_pt *pt, ymx_pt_stats *stats);

@@ -26,8 +26,8 @@ namespace {
 constexpr int kTraceBlock = 128;
 constexpr int kStack = YMX_LDS_STACK;
 constexpr float kBiasScale = 0.1f * 0.00005f;  // include/accelerator/accelerator.h:64,85
-constexpr float kSlabWiden = 1.0000004f;       // >= 1 + 2 gamma_3 for FP32
-constexpr float kWindow = 0x1p-12f;
+constexpr float kSlabWiden = ymx::kSlabWiden;
+constexpr float kWindow = ymx::kBoxWindow;
 
 struct RayState
 {

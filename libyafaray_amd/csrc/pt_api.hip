@@ -7,6 +7,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <new>
 #include <vector>
 
@@ -96,6 +97,7 @@ void free_pt(ymx_pt *pt)
 		if(p) (void)hipFree(p);
 	if(pt->ev0) (void)hipEventDestroy(pt->ev0);
 	if(pt->ev1) (void)hipEventDestroy(pt->ev1);
+	for(ymx_accel *tree : pt->light_trees) (void)ymx_accel_destroy(tree);
 	delete pt;
 }
 
@@ -143,6 +145,40 @@ void filter_table(float pixelwidth, PtConst &c)
 		}
 	c.filter_table_scale = 0.9999f * 16.f / fw;
 	c.filter_width = fw;
+}
+
+// Rgb::clampProportionalRgb (include/color/color.h:399-431): the largest
+// component clamped to max_value and the others scaled by the same factor;
+// max_value <= 0 leaves the colour as it is.
+Rgb clamp_proportional(Rgb c, float max_value)
+{
+	if(max_value > 0.f)
+	{
+		const float max_rgb = std::max(c.r, std::max(c.g, c.b));
+		const float adj = max_value / max_rgb;
+		if(max_rgb > max_value)
+		{
+			if(c.r >= max_rgb)
+			{
+				c.r = max_value;
+				c.g *= adj;
+				c.b *= adj;
+			}
+			else if(c.g >= max_rgb)
+			{
+				c.g = max_value;
+				c.r *= adj;
+				c.b *= adj;
+			}
+			else
+			{
+				c.b = max_value;
+				c.r *= adj;
+				c.g *= adj;
+			}
+		}
+	}
+	return c;
 }
 
 // ObjectLight::initIs (src/light/light_object_light.cc:87-104) for the
@@ -295,15 +331,25 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 	for(uint32_t l = 0; l < n_lights; ++l)
 	{
 		const ymx_light &s = lights[l];
-		if(s.type > YMX_LIGHT_OBJECT || (s.type != YMX_LIGHT_POINT && (s.samples < 1 || s.samples > 64)))
+		if(s.type > YMX_LIGHT_PORTAL || (s.type != YMX_LIGHT_POINT && (s.samples < 1 || s.samples > 64)))
 		{
 			ymx_set_error("ymx_pt_create: light %u unsupported (type %u, samples %d)", l, s.type, s.samples);
 			return YMX_ERR_UNSUPPORTED;
 		}
-		if(s.type == YMX_LIGHT_OBJECT && (s.n_tris == 0 || s.first_tri > n_tris || s.n_tris > n_tris - s.first_tri))
+		const bool mesh = s.type == YMX_LIGHT_OBJECT || s.type == YMX_LIGHT_PORTAL;
+		if(mesh && (s.n_tris == 0 || s.first_tri > n_tris || s.n_tris > n_tris - s.first_tri))
 		{
-			ymx_set_error("ymx_pt_create: object light %u covers triangles [%u, %u + %u) of %u", l, s.first_tri,
-			              s.first_tri, s.n_tris, n_tris);
+			ymx_set_error("ymx_pt_create: %s light %u covers triangles [%u, %u + %u) of %u",
+			              s.type == YMX_LIGHT_PORTAL ? "portal" : "object", l, s.first_tri, s.first_tri, s.n_tris,
+			              n_tris);
+			return YMX_ERR_INVALID_ARG;
+		}
+		// BackgroundPortalLight::init takes the scene's background
+		// (src/light/light_background_portal.cc:96-99) and evaluates it in every
+		// sample: a scene without one has nothing to send through the portal
+		if(s.type == YMX_LIGHT_PORTAL && !st->has_background)
+		{
+			ymx_set_error("ymx_pt_create: portal light %u needs a background", l);
 			return YMX_ERR_INVALID_ARG;
 		}
 	}
@@ -477,10 +523,15 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 			d.quad[2] = sub(add(p1, p2), c0);
 			d.quad[3] = p2;
 		}
-		else if(s.type == YMX_LIGHT_OBJECT)
+		else if(s.type == YMX_LIGHT_OBJECT || s.type == YMX_LIGHT_PORTAL)
 		{
-			d.color = cscale(cp, kPi);
-			d.double_sided = s.double_sided ? 1u : 0u;
+			if(s.type == YMX_LIGHT_OBJECT)
+			{
+				d.color = cscale(cp, kPi);
+				d.double_sided = s.double_sided ? 1u : 0u;
+			}
+			else  // ConstantBackground::eval (src/background/background_constant.cc:63-66) * power
+				d.color = cscale(rgb(st->background[0], st->background[1], st->background[2]), s.power);
 			d.first = (uint32_t)ltris.size();
 			d.count = s.n_tris;
 			d.area = object_light_tables(verts, tri_idx + 3 * (size_t)s.first_tri, s.n_tris,
@@ -492,9 +543,45 @@ int ymx_pt_create(const ymx_accel *accel, const float *verts, uint32_t n_verts, 
 			d.color = cp;
 			d.from = load3(s.from);
 		}
+		// BackgroundPortalLight::intersect clamps its colour
+		// (src/light/light_background_portal.cc:219-220); the other lights return theirs
+		d.hit_color = s.type == YMX_LIGHT_PORTAL ? clamp_proportional(d.color, s.ibl_clamp_sampling) : d.color;
 		const uint32_t k = s.type == YMX_LIGHT_POINT ? 1u : 2u * (uint32_t)ceilf((float)s.samples * max_light_mult);
 		slots_camera += k;
 		slots_path = k > slots_path ? k : slots_path;
+	}
+	// the lights' own BVHs (ObjectLight::initIs, BackgroundPortalLight::initIs:
+	// src/light/light_object_light.cc:95-98, src/light/light_background_portal.cc:90-93)
+	for(uint32_t l = 0; l < n_lights; ++l)
+	{
+		const ymx_light &s = lights[l];
+		if((s.type != YMX_LIGHT_OBJECT && s.type != YMX_LIGHT_PORTAL) || s.n_tris <= kLightTreeMin) continue;
+		ymx_accel_params ap;
+		memset(&ap, 0, sizeof ap);
+		ap.struct_size = sizeof ap;
+		// the light's triangles on their own vertices, so the build reads no
+		// more than the light's mesh
+		std::vector<float> lv(9 * (size_t)s.n_tris);
+		std::vector<uint32_t> li(3 * (size_t)s.n_tris);
+		for(size_t k = 0; k < 3 * (size_t)s.n_tris; ++k)
+		{
+			memcpy(&lv[3 * k], verts + 3 * (size_t)tri_idx[3 * (size_t)s.first_tri + k], 3 * sizeof(float));
+			li[k] = (uint32_t)k;
+		}
+		ymx_accel *tree = nullptr;
+		const int rc = ymx_accel_create(lv.data(), 3u * s.n_tris, li.data(), vis ? vis + s.first_tri : nullptr,
+		                                s.n_tris, &ap, &tree);
+		if(rc != YMX_OK)
+		{
+			free_pt(pt);
+			return rc;
+		}
+		pt->light_trees.push_back(tree);
+		if(tree->max_depth < kLightStack)
+		{
+			lts[l].tree_nodes = tree->d_nodes;
+			lts[l].tree_tris = tree->d_tris;
+		}
 	}
 	PT_TRY(dalloc(pt->d_lights, lts.size()));
 	PT_TRY(hipMemcpy(pt->d_lights, lts.data(), sizeof(PtLight) * lts.size(), hipMemcpyHostToDevice));
@@ -1279,6 +1366,8 @@ int ymx_pt_get_stats(const ymx_pt *pt, ymx_pt_stats *stats)
 	stats->shadow_rays = t[2];
 	stats->render_ms = ms;
 	stats->guide_samples = pt->stats.guide_samples;
+	stats->light_trees = 0;
+	for(const ymx_accel *tree : pt->light_trees) stats->light_trees += tree->max_depth < kLightStack ? 1u : 0u;
 	stats->guide_steps = pt->stats.guide_steps;
 	return YMX_OK;
 }

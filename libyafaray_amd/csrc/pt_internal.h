@@ -6,6 +6,8 @@
 #include "accel_internal.h"
 #include "pt_math.h"
 
+#include <vector>
+
 namespace ymx {
 
 enum BsdfBits : uint32_t
@@ -54,17 +56,29 @@ struct PtLight
 	uint32_t type;
 	uint32_t cast_shadows;
 	int32_t samples;
-	float area;           // area light: normal_flipped_.normalized().length(); object light: its surface area
+	float area;           // area light: normal_flipped_.normalized().length(); object / portal light: its surface area
 	V3 corner, to_x, to_y, normal_flipped;
 	V3 quad[4];           // corner, point1, point1 + point2 - corner, point2
 	V3 from;              // point light
-	Rgb color;            // colour * power (* pi for area and object lights)
-	// object light: its triangles are ltris[first, first + count), and the
-	// tree bound of its own kd-tree (include/accelerator/accelerator_kdtree_original.cc)
+	Rgb color;            // colour * power (* pi for area and object lights); portal: background * power
+	Rgb hit_color;        // the colour Light::intersect returns: color, clamped for a portal
+	// object and portal lights: the triangles are ltris[first, first + count),
+	// with the tree bound of the light's own kd-tree
+	// (include/accelerator/accelerator_kdtree_original.cc) and, for a mesh of
+	// more than kLightTreeMin triangles, the light's own BVH over them
+	// (accel_internal.h layout, ids 0 .. count - 1; NULL: test every triangle)
 	uint32_t first, count;
 	uint32_t double_sided;
 	float bound_a[3], bound_g[3];
+	const float4 *tree_nodes;
+	const float4 *tree_tris;
 };
+
+// Object and portal lights with more triangles than this get their own BVH;
+// its traversal stack holds kLightStack entries, so a tree deeper than that
+// keeps the triangle loop.
+constexpr uint32_t kLightTreeMin = 16;
+constexpr uint32_t kLightStack = 64;
 
 // One triangle of an object light (64 B): v0 and its cumulative area
 // fraction (the light's Pdf1D cdf), v1 and its visibility mask, v2, and the
@@ -193,7 +207,8 @@ struct ymx_pt
 	float4 *d_cnorm;                 // corner normals, 3 per triangle (NULL: all flat; ymx_pt_set_normals)
 	ymx::PtMaterial *d_materials;
 	ymx::PtLight *d_lights;
-	ymx::PtLightTri *d_ltris;        // triangles of the object lights
+	ymx::PtLightTri *d_ltris;        // triangles of the object and portal lights
+	std::vector<ymx_accel *> light_trees;  // their own BVHs (PtLight::tree_nodes), destroyed with the tracer
 	uint32_t *d_transp;              // bit per primitive: its material is transparent (transparent shadows)
 	ymx::PtConst *d_const;
 	int32_t *d_faure;                // [kLdsDims][kLdsMaxBase] scrambling permutations

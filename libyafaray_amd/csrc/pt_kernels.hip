@@ -873,14 +873,16 @@ __device__ void obj_sample_surface(const PtArgs &a, const PtLight &l, float s1, 
 	n = v3(t.n.x, t.n.y, t.n.z);
 }
 
-// Light::illumSample of the area and object lights (src/light/light_area.cc:99-124,
-// src/light/light_object_light.cc:141-171).  False when the light gives no sample.
+// Light::illumSample of the area, object and portal lights (src/light/light_area.cc:99-124,
+// src/light/light_object_light.cc:141-171, src/light/light_background_portal.cc:157-181;
+// the portal is one-sided and has no guard on a zero area * cos).  False when
+// the light gives no sample.
 __device__ bool illum_sample(const PtArgs &a, const PtLight &l, V3 sp_p, float s1, float s2, V3 &ldir_out,
                              float &dist_out, float &pdf_out)
 {
 	V3 ldir;
 	float dist_sqr, cos_angle, area_cos;
-	if(l.type == YMX_LIGHT_OBJECT)
+	if(l.type == YMX_LIGHT_OBJECT || l.type == YMX_LIGHT_PORTAL)
 	{
 		V3 p, n;
 		obj_sample_surface(a, l, s1, s2, p, n);
@@ -896,7 +898,7 @@ __device__ bool illum_sample(const PtArgs &a, const PtLight &l, V3 sp_p, float s
 			else return false;
 		}
 		area_cos = l.area * cos_angle;
-		pdf_out = dist_sqr * kPi / ((area_cos == 0.f) ? 1e-8f : area_cos);
+		pdf_out = dist_sqr * kPi / ((area_cos == 0.f && l.type == YMX_LIGHT_OBJECT) ? 1e-8f : area_cos);
 		dist_out = dist;
 	}
 	else
@@ -916,12 +918,88 @@ __device__ bool illum_sample(const PtArgs &a, const PtLight &l, V3 sp_p, float s
 	return true;
 }
 
-// ObjectLight::intersect (src/light/light_object_light.cc:211-229): the closest
-// hit among the light's visible triangles past max(tmin, dynamic bias of the
-// light's tree bound) (include/accelerator/accelerator.h:64,122-131,
-// include/accelerator/accelerator_kdtree_common.h:139), by testing every
-// triangle of the light.  The reference leaves its `t` argument, the ray's
-// tmax_ = -1, unchanged, so the 1 / (t * t) factor of the inverse pdf is 1.
+// The closest visible triangle of a light's own BVH (the node and triangle
+// layout of accel_internal.h, ids 0 .. count - 1) along o + t d with t in
+// [t_min, FLT_MAX): the one the loop over the light's triangles in id order
+// finds, since the box tests only cull (ray_tri.h) and an exact tie keeps the
+// lower id.  -1 when none.  The stack holds at most the tree's depth
+// (< kLightStack) entries: one deferred sibling per level of the path.
+__device__ int32_t light_tree_closest(const float4 *nodes, const float4 *tris, const float o[3], const float d[3],
+                                      float t_min)
+{
+	float idir[3];
+	for(int k = 0; k < 3; ++k)
+	{
+		const float dk = (fabsf(d[k]) > 1e-30f) ? d[k] : copysignf(1e-30f, d[k]);
+		idir[k] = 1.f / dk;
+	}
+	const float tlo = t_min - t_min * ymx::kBoxWindow;
+	float best_t = FLT_MAX;
+	int32_t best = -1;
+	int32_t stack[kLightStack];
+	uint32_t sp = 0;
+	int32_t ref = 0;
+	for(;;)
+	{
+		if(ref >= 0)
+		{
+			const float4 *np = nodes + 4 * (size_t)ref;
+			const float4 f0 = np[0], f1 = np[1], f2 = np[2], f3 = np[3];
+			const float thi = best_t + best_t * ymx::kBoxWindow;
+			const float a0x = (f0.x - o[0]) * idir[0], b0x = (f0.y - o[0]) * idir[0];
+			const float a0y = (f0.z - o[1]) * idir[1], b0y = (f0.w - o[1]) * idir[1];
+			const float a0z = (f2.x - o[2]) * idir[2], b0z = (f2.y - o[2]) * idir[2];
+			const float a1x = (f1.x - o[0]) * idir[0], b1x = (f1.y - o[0]) * idir[0];
+			const float a1y = (f1.z - o[1]) * idir[1], b1y = (f1.w - o[1]) * idir[1];
+			const float a1z = (f2.z - o[2]) * idir[2], b1z = (f2.w - o[2]) * idir[2];
+			const float n0 = fmaxf(fmaxf(fminf(a0x, b0x), fminf(a0y, b0y)), fmaxf(fminf(a0z, b0z), tlo));
+			const float x0 = fminf(fminf(fmaxf(a0x, b0x), fmaxf(a0y, b0y)), fmaxf(a0z, b0z)) * ymx::kSlabWiden;
+			const float n1 = fmaxf(fmaxf(fminf(a1x, b1x), fminf(a1y, b1y)), fmaxf(fminf(a1z, b1z), tlo));
+			const float x1 = fminf(fminf(fmaxf(a1x, b1x), fmaxf(a1y, b1y)), fmaxf(a1z, b1z)) * ymx::kSlabWiden;
+			const bool h0 = n0 <= fminf(x0, thi), h1 = n1 <= fminf(x1, thi);
+			const int32_t c0 = __float_as_int(f3.x), c1 = __float_as_int(f3.y);
+			if(h0 && h1)
+			{
+				const bool swap = n1 < n0;
+				if(sp < kLightStack) stack[sp++] = swap ? c0 : c1;
+				ref = swap ? c1 : c0;
+			}
+			else ref = h0 ? c0 : (h1 ? c1 : YMX_REF_EMPTY);
+			if(ref != YMX_REF_EMPTY) continue;
+		}
+		else if(ref != YMX_REF_EMPTY)
+		{
+			const uint32_t first = ymx_leaf_first(ref), count = ymx_leaf_count(ref);
+			for(uint32_t k = 0; k < count; ++k)
+			{
+				const float4 *tp = tris + 3 * (size_t)(first + k);
+				const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+				if(!(__float_as_uint(t1.w) & YMX_VIS_VISIBLE)) continue;
+				float u, v;
+				const float th = ymx::tri_test(t0, t1, t2, o, d, u, v);
+				if(th <= 0.f || th < t_min) continue;
+				const int32_t id = (int32_t)__float_as_uint(t0.w);
+				if(th < best_t || (th == best_t && best >= 0 && id < best))
+				{
+					best_t = th;
+					best = id;
+				}
+			}
+		}
+		if(sp == 0) break;
+		ref = stack[--sp];
+	}
+	return best;
+}
+
+// ObjectLight::intersect and BackgroundPortalLight::intersect
+// (src/light/light_object_light.cc:211-229, src/light/light_background_portal.cc:207-222):
+// the closest hit among the light's visible triangles past max(tmin, dynamic
+// bias of the light's tree bound) (include/accelerator/accelerator.h:64,122-131,
+// include/accelerator/accelerator_kdtree_common.h:139), from the light's own
+// BVH or, for a small mesh, by testing every triangle.  A portal is one-sided.
+// The reference leaves its `t` argument, the ray's tmax_ = -1, unchanged, so
+// the 1 / (t * t) factor of the inverse pdf is 1.
 __device__ bool obj_intersect(const PtArgs &a, const PtLight &l, V3 from, V3 dir, float tmin, float &ipdf)
 {
 	const float f[3] = {from.x, from.y, from.z}, d[3] = {dir.x, dir.y, dir.z};
@@ -929,20 +1007,24 @@ __device__ bool obj_intersect(const PtArgs &a, const PtLight &l, V3 from, V3 dir
 	if(!ymx::tree_cross(l.bound_a, l.bound_g, f, d, FLT_MAX, enter, leave)) return false;
 	const float bias = (0.1f * 0.00005f) * fabsf(leave - enter);
 	const float t_min = (tmin < bias) ? bias : tmin;
-	float best = FLT_MAX;
-	int32_t hit = -1;
 	const PtLightTri *lt = a.ltris + l.first;
-	for(uint32_t i = 0; i < l.count; ++i)
+	int32_t hit = -1;
+	if(l.tree_nodes) hit = light_tree_closest(l.tree_nodes, l.tree_tris, f, d, t_min);
+	else
 	{
-		const PtLightTri t = lt[i];
-		if(!(__float_as_uint(t.v1_vis.w) & YMX_VIS_VISIBLE)) continue;
-		const float4 e1 = make_float4(t.v1_vis.x - t.v0_cdf.x, t.v1_vis.y - t.v0_cdf.y, t.v1_vis.z - t.v0_cdf.z, 0.f);
-		const float4 e2 = make_float4(t.v2.x - t.v0_cdf.x, t.v2.y - t.v0_cdf.y, t.v2.z - t.v0_cdf.z, 0.f);
-		float u, v;
-		const float th = ymx::tri_test(t.v0_cdf, e1, e2, f, d, u, v);
-		if(th <= 0.f || th < t_min || th >= best) continue;
-		best = th;
-		hit = (int32_t)i;
+		float best = FLT_MAX;
+		for(uint32_t i = 0; i < l.count; ++i)
+		{
+			const PtLightTri t = lt[i];
+			if(!(__float_as_uint(t.v1_vis.w) & YMX_VIS_VISIBLE)) continue;
+			const float4 e1 = make_float4(t.v1_vis.x - t.v0_cdf.x, t.v1_vis.y - t.v0_cdf.y, t.v1_vis.z - t.v0_cdf.z, 0.f);
+			const float4 e2 = make_float4(t.v2.x - t.v0_cdf.x, t.v2.y - t.v0_cdf.y, t.v2.z - t.v0_cdf.z, 0.f);
+			float u, v;
+			const float th = ymx::tri_test(t.v0_cdf, e1, e2, f, d, u, v);
+			if(th <= 0.f || th < t_min || th >= best) continue;
+			best = th;
+			hit = (int32_t)i;
+		}
 	}
 	if(hit < 0) return false;
 	const float4 n = lt[hit].n;
@@ -1041,7 +1123,7 @@ __device__ void queue_light(const PtArgs &a, const Surf &sp, V3 wo, const PtLigh
 		const Rgb surf_col = bsdf_sample(a, sp, wo, dir, s, W);
 		float t = -1.f, ipdf = 0.f;
 		bool hit = false;
-		if(l.type == YMX_LIGHT_OBJECT) hit = obj_intersect(a, l, sp.p, dir, btmin, ipdf);
+		if(l.type == YMX_LIGHT_OBJECT || l.type == YMX_LIGHT_PORTAL) hit = obj_intersect(a, l, sp.p, dir, btmin, ipdf);
 		else
 		{
 			// AreaLight::intersect (src/light/light_area.cc:146-155)
@@ -1062,7 +1144,7 @@ __device__ void queue_light(const PtArgs &a, const Surf &sp, V3 wo, const PtLigh
 			const float l_2 = l_pdf * l_pdf;
 			const float m_2 = s.pdf * s.pdf;
 			const float w = m_2 / (l_2 + m_2);
-			const Rgb c = cscale(cscale(cmul(surf_col, l.color), w), W);
+			const Rgb c = cscale(cscale(cmul(surf_col, l.hit_color), w), W);
 			set_slot(a, slot, c, cast_shadows, sp.p, dir, btmin, t);
 		}
 		else set_slot_empty(a, slot);

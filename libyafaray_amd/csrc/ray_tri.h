@@ -22,6 +22,13 @@ struct float4
 
 namespace ymx {
 
+// Box tests of the BVH traversals only cull: slab intervals are widened by
+// kSlabWiden >= 1 + 2 gamma_3 for FP32 (T. Ize, "Robust BVH Ray Traversal",
+// JCGT 2013) and the accepted [t_min, t_best] window by kBoxWindow relative,
+// so every triangle whose exact test would be accepted is tested.
+constexpr float kSlabWiden = 1.0000004f;
+constexpr float kBoxWindow = 0x1p-12f;
+
 // include/geometry/bound.h:157-198 on the tree bound, exactly as the reference
 // evaluates it.
 YMX_RT_FN bool tree_cross(const float a[3], const float g[3], const float from[3],

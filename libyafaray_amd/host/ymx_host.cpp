@@ -593,6 +593,11 @@ bool Scene::createLight(const std::string &name, const ParamMap &p, std::string 
 		l.type = YMX_LIGHT_OBJECT;
 		l.double_sided = p.getBool("double_sided", false, &ok) ? 1u : 0u;
 	}
+	else if(type == "bgPortalLight")  // include/light/light_background_portal.h:58-61
+	{
+		l.type = YMX_LIGHT_PORTAL;
+		l.ibl_clamp_sampling = (float)p.getFloat("ibl_clamp_sampling", 0.0, &ok);
+	}
 	else
 	{
 		if(error) *error = "Scene: light '" + name + "' has unsupported type '" + type + "'";
@@ -600,10 +605,10 @@ bool Scene::createLight(const std::string &name, const ParamMap &p, std::string 
 	}
 	set3(l.color, p.getColor("color", {1, 1, 1, 1}, &ok));
 	l.power = (float)p.getFloat("power", 1.0, &ok);
-	l.samples = p.getInt("samples", 4, &ok);
+	l.samples = p.getInt("samples", l.type == YMX_LIGHT_PORTAL ? 16 : 4, &ok);
 	l.cast_shadows = p.getBool("cast_shadows", true, &ok) ? 1u : 0u;
 	const std::string object = p.getString("object_name", "", &ok);
-	if(!ok || (type == "objectlight" && object.empty()))
+	if(!ok || ((l.type == YMX_LIGHT_OBJECT || l.type == YMX_LIGHT_PORTAL) && object.empty()))
 	{
 		if(error) *error = "Scene: light '" + name + "' has a missing or mistyped param";
 		return false;
@@ -738,15 +743,15 @@ std::unique_ptr<PathIntegrator> PathIntegrator::create(Scene &scene, const Param
 	const std::vector<uint32_t> &tris = scene.flat_tris_, &mats = scene.flat_mats_;
 	const std::vector<uint8_t> &vis = scene.flat_vis_;
 	const auto &ranges = scene.ranges_;
-	// an object light covers its object's triangle range
+	// an object or portal light covers its object's triangle range
 	std::vector<ymx_light> lights;
 	for(const auto &[l, object] : scene.lights_)
 	{
 		ymx_light x = l;
-		if(x.type == YMX_LIGHT_OBJECT)
+		if(x.type == YMX_LIGHT_OBJECT || x.type == YMX_LIGHT_PORTAL)
 		{
 			const auto it = ranges.find(object);
-			if(it == ranges.end() || it->second.second == 0) return fail("object light's object '" + object + "' not found");
+			if(it == ranges.end() || it->second.second == 0) return fail("light's object '" + object + "' not found");
 			x.first_tri = it->second.first;
 			x.n_tris = it->second.second;
 		}

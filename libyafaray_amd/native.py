@@ -56,7 +56,7 @@ class YmxAccelInfo(C.Structure):
 class YmxPtStats(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("camera_rays", C.c_uint64), ("extension_rays", C.c_uint64),
                 ("shadow_rays", C.c_uint64), ("render_ms", C.c_float), ("guide_samples", C.c_uint64),
-                ("guide_steps", C.c_uint64)]
+                ("guide_steps", C.c_uint64), ("light_trees", C.c_uint32)]
 
 
 TUNING_TRIALS = 8

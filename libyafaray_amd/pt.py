@@ -35,11 +35,13 @@ class YmxLight(C.Structure):
     _fields_ = [("type", C.c_uint32), ("corner", C.c_float * 3), ("point1", C.c_float * 3),
                 ("point2", C.c_float * 3), ("from_", C.c_float * 3), ("color", C.c_float * 3),
                 ("power", C.c_float), ("samples", C.c_int32), ("cast_shadows", C.c_uint32),
-                ("first_tri", C.c_uint32), ("n_tris", C.c_uint32), ("double_sided", C.c_uint32)]
+                ("first_tri", C.c_uint32), ("n_tris", C.c_uint32), ("double_sided", C.c_uint32),
+                ("ibl_clamp_sampling", C.c_float)]
 
 
 # light "type" param values of the reference (include/light/light.h:117-128), plus short aliases
-LIGHT_TYPES = {"area": 0, "arealight": 0, "point": 1, "pointlight": 1, "object": 2, "objectlight": 2}
+LIGHT_TYPES = {"area": 0, "arealight": 0, "point": 1, "pointlight": 1, "object": 2, "objectlight": 2,
+               "portal": 3, "bgPortalLight": 3}
 
 # object "visibility" param values (include/common/visibility.h:31-37) as YMX_VIS_* masks
 VISIBILITY = {"normal": 3, "no_shadows": 1, "shadow_only": 2, "invisible": 0}
@@ -120,11 +122,12 @@ def build_structs(scene):
         lights[i].from_ = _f3(l.get("from", (0, 0, 0)))
         lights[i].color = _f3(l.get("color", (1, 1, 1)))
         lights[i].power = float(l.get("power", 1.0))
-        lights[i].samples = int(l.get("samples", 4))
+        lights[i].samples = int(l.get("samples", 16 if LIGHT_TYPES[l["type"]] == 3 else 4))
         lights[i].cast_shadows = 1 if l.get("cast_shadows", True) else 0
         lights[i].first_tri = int(l.get("first_tri", 0))
         lights[i].n_tris = int(l.get("n_tris", 0))
         lights[i].double_sided = 1 if l.get("double_sided", False) else 0
+        lights[i].ibl_clamp_sampling = float(l.get("ibl_clamp_sampling", 0.0))
     cam, s = scene["camera"], scene["settings"]
     st = YmxPtSettings()
     st.from_ = _f3(cam["from_"])
@@ -312,4 +315,5 @@ class GpuPathTracer:
         s.struct_size = C.sizeof(YmxPtStats)
         check(self._lib.ymx_pt_get_stats(self._h, C.byref(s)), "ymx_pt_get_stats")
         return {"camera_rays": s.camera_rays, "extension_rays": s.extension_rays, "shadow_rays": s.shadow_rays,
-                "render_ms": s.render_ms, "guide_samples": s.guide_samples, "guide_steps": s.guide_steps}
+                "render_ms": s.render_ms, "guide_samples": s.guide_samples, "guide_steps": s.guide_steps,
+                "light_trees": s.light_trees}

@@ -574,6 +574,65 @@ def object_light_scene(n_random: int = 2000, seed: int = 0, res: int = 64, spp: 
                 camera=camera, settings=settings)
 
 
+def portal_scene(n_random: int = 2000, seed: int = 0, res: int = 64, spp: int = 4, visibility: str = "invisible",
+                 cells: int = 6, power: float = 2.0, ibl_clamp_sampling: float = 0.0,
+                 background=(0.9, 0.95, 1.0), as_object_light: bool = False):
+    """Cornell box lit by a background portal light
+    (include/light/light_background_portal.h): a window x, z in [0.25, 0.75]
+    cut into the back wall (y = 1), covered by the portal's mesh of `cells` x
+    `cells` quads (2 * cells^2 triangles, face normal -y into the box)
+    carrying the object `visibility`, with the constant `background` behind it
+    (and seen through the open side), the portal's `power` and
+    "ibl_clamp_sampling".  Nothing lies behind the window, so the BSDF half of
+    the light's MIS, whose shadow ray has no end, reaches the background.  Random triangles
+    are shinydiffuse and glossy, the floor a sharp glossy (so the BSDF half
+    of the light's MIS matters); one point light.  With `as_object_light` the
+    window is instead a one-sided object light of colour `background` and
+    power `power` / pi: the same light up to float rounding of its colour."""
+    from .pt import VISIBILITY
+    rng = np.random.default_rng(seed)
+    wv, wt, wall = cornell_walls()
+    keep = wall != 2
+    back = grid_quad_tris((0.0, 1.0, 0.0), (1.0, 0.0, 0.0), (0.0, 0.0, 1.0), 4, 4,
+                          skip=lambda a, b: a in (1, 2) and b in (1, 2))
+    wall = np.concatenate([wall[keep], np.full(len(back[1]), 2)])
+    tv = random_triangles(rng, n_random, (0.15, 0.15, 0.05), (0.85, 0.85, 0.6))
+    portal = grid_quad_tris((0.25, 1.0, 0.25), (0.5, 0.0, 0.0), (0.0, 0.0, 0.5), cells, cells)
+    verts, tris = merge([(wv, wt[keep]), back, soup(tv), portal])
+    n_wall, n_portal = len(wall), len(portal[1])
+    materials = [dict(color=(0.7, 0.7, 0.7)), dict(color=(0.7, 0.1, 0.1)), dict(color=(0.1, 0.7, 0.1)),
+                 dict(color=(0.2, 0.2, 0.25)),
+                 dict(type="glossy", color=(0.8, 0.8, 0.8), exponent=400.0, diffuse_reflect=0.2,
+                      diffuse_color=(0.6, 0.6, 0.6), glossy_reflect=0.8)]
+    wall_mat = np.array([4, 0, 0, 1, 2])
+    for k in range(n_random):
+        a = float(rng.uniform(0.2, 0.8))
+        if rng.uniform() < 0.6:
+            materials.append(dict(color=(a, a, a)))
+        else:
+            r = float(rng.uniform(0.1, 0.5))
+            materials.append(dict(type="glossy", color=(a, 0.9 * a, 0.8 * a), exponent=2.0 / (r * r) - 2.0,
+                                  diffuse_reflect=float(rng.uniform(0.0, 0.6)), diffuse_color=(0.8 * a, a, a),
+                                  glossy_reflect=float(rng.uniform(0.2, 1.0))))
+    mat_id = np.concatenate([wall_mat[wall], 5 + np.arange(n_random), np.full(n_portal, 3)]).astype(np.uint32)
+    first = n_wall + n_random
+    vis = np.full(len(tris), 3, np.uint8)
+    vis[first:] = VISIBILITY[visibility]
+    if as_object_light:
+        light = dict(type="objectlight", first_tri=first, n_tris=n_portal, color=tuple(background),
+                     power=float(power / np.pi), samples=16)
+    else:
+        light = dict(type="bgPortalLight", first_tri=first, n_tris=n_portal, power=power, samples=16,
+                     ibl_clamp_sampling=ibl_clamp_sampling)
+    lights = [light, dict(type="point", **{"from": (0.8, 0.6, 0.9)}, color=(1.0, 1.0, 1.0), power=0.05)]
+    camera = dict(from_=(0.5, -1.6, 0.5), to=(0.5, 0.5, 0.5), up=(0.5, -1.6, 1.5),
+                  focal=float(1.0 / (2.0 * np.tan(np.radians(20.0)))), resx=res, resy=res)
+    settings = dict(path_samples=1, bounces=4, rr_min_bounces=2, aa_samples=spp, aa_pixelwidth=1.5,
+                    background=tuple(background), seed=seed, raydepth=2)
+    return dict(verts=verts, tris=tris, mat_id=mat_id, vis=vis, materials=materials, lights=lights,
+                camera=camera, settings=settings)
+
+
 def terrain_spheres_scene(res=(1920, 1080), spp: int = 64, seed: int = 0, hf_res: int = 1024, n_spheres: int = 500,
                           subdiv: int = 3, n_point_lights: int = 64, depth: int = 8, area_lights=None):
     """Mixed-material path-tracing workload: the ray-cast heightfield

@@ -272,7 +272,7 @@ typedef struct
 	const uint8_t *vis;
 	const ora_light *lights;
 	uint32_t n_lights;
-	struct obj_light *objs;   /* [n_lights], set up for ORA_LIGHT_OBJECT lights */
+	struct obj_light *objs;   /* [n_lights], set up for object and portal lights */
 	uint8_t *transparent;     /* per triangle, with transparent shadows: its material isTransparent() */
 	const ora_pt_settings *st;
 	/* camera (src/camera/camera.cc:89-110, camera_perspective.cc:97-111) */
@@ -922,12 +922,22 @@ static int shadow_ray(const pt_ctx *c, v3 from, v3 dir, float tmin, float tmax, 
 /* ------------------------------------------------------------------ */
 /* lights                                                                */
 
-static rgb light_color(const ora_light *l)
+static rgb light_color(const pt_ctx *c, const ora_light *l)
 {
-	const rgb c = cscale(C(l->color[0], l->color[1], l->color[2]), l->power);
+	/* BackgroundPortalLight: the background's colour (ConstantBackground::eval,
+	 * src/background/background_constant.cc:63-66) times the portal's power
+	 * (src/light/light_background_portal.cc:173) */
+	if(l->type == ORA_LIGHT_PORTAL)
+		return cscale(C(c->st->background[0], c->st->background[1], c->st->background[2]), l->power);
+	const rgb col = cscale(C(l->color[0], l->color[1], l->color[2]), l->power);
 	/* light_area.h:78, light_object_light.h:82, light_point.h:64 */
-	return l->type == ORA_LIGHT_POINT ? c : cscale(c, K_PI);
+	return l->type == ORA_LIGHT_POINT ? col : cscale(col, K_PI);
 }
+
+static void clamp_proportional(float col[4], float max_value);
+
+/* ObjectLight's double_sided param; a portal is one-sided */
+static int two_sided(const ora_light *l) { return l->type == ORA_LIGHT_OBJECT && l->double_sided; }
 
 static v3 area_normal_flipped(const ora_light *l)
 {
@@ -1089,13 +1099,15 @@ static void obj_sample_surface(const pt_ctx *c, const ora_light *l, const struct
 	*n = vnorm(vcross(vsub(v1, v0), vsub(v2, v0)));   /* face_normal_geometric_ */
 }
 
-/* Light::illumSample of the area and object lights (src/light/light_area.cc:99-124,
- * src/light/light_object_light.cc:141-171): the direction and distance to a
- * point on the light and the sample's pdf; 0 when the light gives no sample. */
+/* Light::illumSample of the area, object and portal lights (src/light/light_area.cc:99-124,
+ * src/light/light_object_light.cc:141-171, src/light/light_background_portal.cc:157-181):
+ * the direction and distance to a point on the light and the sample's pdf; 0
+ * when the light gives no sample.  The portal is one-sided and divides by
+ * area * cos without the object light's 1e-8 floor. */
 static int illum_sample(const pt_ctx *c, const ora_light *l, const struct obj_light *o, v3 sp_p, float s1, float s2,
                         v3 *ldir_out, float *dist_out, float *pdf_out)
 {
-	if(l->type == ORA_LIGHT_OBJECT)
+	if(o)
 	{
 		v3 p, n;
 		obj_sample_surface(c, l, o, s1, s2, &p, &n);
@@ -1107,11 +1119,12 @@ static int illum_sample(const pt_ctx *c, const ora_light *l, const struct obj_li
 		float cos_angle = -vdot(ldir, n);
 		if(cos_angle <= 0.f)
 		{
-			if(l->double_sided) cos_angle = -cos_angle;
+			if(two_sided(l)) cos_angle = -cos_angle;
 			else return 0;
 		}
 		const float area_mul_cosangle = o->area * cos_angle;
-		*pdf_out = dist_sqr * K_PI / ((area_mul_cosangle == 0.f) ? 1e-8f : area_mul_cosangle);
+		if(l->type == ORA_LIGHT_PORTAL) *pdf_out = dist_sqr * K_PI / area_mul_cosangle;
+		else *pdf_out = dist_sqr * K_PI / ((area_mul_cosangle == 0.f) ? 1e-8f : area_mul_cosangle);
 		*ldir_out = ldir;
 		*dist_out = dist;
 		return 1;
@@ -1132,9 +1145,11 @@ static int illum_sample(const pt_ctx *c, const ora_light *l, const struct obj_li
 	return 1;
 }
 
-/* ObjectLight::intersect (src/light/light_object_light.cc:211-229): the
- * closest hit of the light's own kd-tree (t_min = max(ray tmin, dynamic
- * bias), visible triangles only, include/accelerator/accelerator.h:122-131).
+/* ObjectLight::intersect and BackgroundPortalLight::intersect
+ * (src/light/light_object_light.cc:211-229, src/light/light_background_portal.cc:207-222):
+ * the closest hit of the light's own kd-tree (t_min = max(ray tmin, dynamic
+ * bias), visible triangles only, include/accelerator/accelerator.h:122-131),
+ * found here by testing every triangle; the lowest index wins an exact tie.
  * The reference never writes the hit distance back into its `t` argument,
  * which areaLightSampleMaterial passes as the ray's tmax_ = -1, so 1 / (t * t)
  * is 1: the returned inverse pdf ignores the distance. */
@@ -1166,7 +1181,7 @@ static int obj_intersect(const pt_ctx *c, const ora_light *l, const struct obj_l
 	float cos_angle = vdot(dir, vneg(n));
 	if(cos_angle <= 0.f)
 	{
-		if(l->double_sided) cos_angle = fabsf(cos_angle);
+		if(two_sided(l)) cos_angle = fabsf(cos_angle);
 		else return 0;
 	}
 	const float t_arg = -1.f;
@@ -1193,7 +1208,7 @@ static rgb light_estimation(const pt_ctx *c, const ora_light *l, const surf *sp,
 		if(dist == 0.f) return col;
 		const float idist_sqr = 1.f / dist_sqr;
 		ldir = vmul(1.f / dist, ldir);
-		const rgb lcol = cscale(light_color(l), idist_sqr);
+		const rgb lcol = cscale(light_color(c, l), idist_sqr);
 		int shadowed = 0;
 		rgb scol = C(1.f, 1.f, 1.f);
 		if(cast_shadows) shadowed = shadow_ray(c, sp->p, ldir, shadow_tmin, dist, &scol);
@@ -1211,8 +1226,14 @@ static rgb light_estimation(const pt_ctx *c, const ora_light *l, const surf *sp,
 	if(rd->division > 1) num_samples = num_samples / rd->division > 1 ? num_samples / rd->division : 1;
 	const float inv_num_samples = 1.f / (float)num_samples;
 	const uint32_t offs = (uint32_t)num_samples * psd_sample + psd_offset + l_offs;
-	const rgb lcolor = light_color(l);
-	const struct obj_light *o = (l->type == ORA_LIGHT_OBJECT) ? &c->objs[l - c->lights] : NULL;
+	const rgb lcolor = light_color(c, l);
+	/* the colour Light::intersect returns: the portal clamps it
+	 * (src/light/light_background_portal.cc:219-220) */
+	float hc[4] = {lcolor.r, lcolor.g, lcolor.b, 0.f};
+	if(l->type == ORA_LIGHT_PORTAL) clamp_proportional(hc, l->ibl_clamp_sampling);
+	const rgb hit_color = C(hc[0], hc[1], hc[2]);
+	const struct obj_light *o =
+	    (l->type == ORA_LIGHT_OBJECT || l->type == ORA_LIGHT_PORTAL) ? &c->objs[l - c->lights] : NULL;
 	halton h2, h3;
 	/* areaLightSampleLight, :192-316 */
 	halton_start(&h2, 2, offs - 1);
@@ -1272,7 +1293,7 @@ static rgb light_estimation(const pt_ctx *c, const ora_light *l, const surf *sp,
 			if(cast_shadows) shadowed = shadow_ray(c, sp->p, dir, btmin, t, &scol);
 			if(!shadowed && ipdf > 1e-6f)
 			{
-				const rgb lc = (c->st->transparent_shadows && cast_shadows) ? cmul(lcolor, scol) : lcolor;
+				const rgb lc = (c->st->transparent_shadows && cast_shadows) ? cmul(hit_color, scol) : hit_color;
 				const float l_pdf = 1.f / ipdf;
 				const float l_2 = l_pdf * l_pdf;
 				const float m_2 = s.pdf * s.pdf;
@@ -1518,7 +1539,13 @@ static int setup_ctx(pt_ctx *c, const ora_kdtree *kd, const float *verts, const 
 	}
 	for(uint32_t l = 0; l < n_lights; ++l)
 	{
-		if(lights[l].type != ORA_LIGHT_OBJECT) continue;
+		if(lights[l].type != ORA_LIGHT_OBJECT && lights[l].type != ORA_LIGHT_PORTAL) continue;
+		/* a portal sends the scene's background through: none, no light */
+		if(lights[l].type == ORA_LIGHT_PORTAL && !st->has_background)
+		{
+			free_ctx(c);
+			return 0;
+		}
 		if(!c->objs && !(c->objs = (struct obj_light *)calloc(n_lights, sizeof(struct obj_light))))
 		{
 			free_ctx(c);
@@ -1863,19 +1890,19 @@ int ora_aa_next_pass(const ora_pt_settings *st, float threshold, const float *fi
 	return n;
 }
 
-void ora_pt_render(const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx, const uint32_t *mat_id,
+int ora_pt_render(const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx, const uint32_t *mat_id,
                    const uint8_t *vis, const ora_material *mats, const ora_light *lights, uint32_t n_lights,
                    const ora_pt_settings *st, float *film_rgba, float *film_w, int threads)
 {
 	pt_ctx c;
-	if(!setup_ctx(&c, kd, verts, tri_idx, mat_id, vis, mats, lights, n_lights, st)) return;
+	if(!setup_ctx(&c, kd, verts, tri_idx, mat_id, vis, mats, lights, n_lights, st)) return 0;
 	float table[256], table_scale, filter_width;
 	ora_filter_table(st->aa_pixelwidth, table, &table_scale, &filter_width);
 	const size_t npix = (size_t)st->resx * st->resy;
 	float *cols = (float *)malloc(npix * 4 * sizeof(float));
 	float *dxy = (float *)malloc(npix * 2 * sizeof(float));
 	uint8_t *flags = (uint8_t *)malloc(npix);
-	if(!cols || !dxy || !flags) { free(cols); free(dxy); free(flags); free_ctx(&c); return; }
+	if(!cols || !dxy || !flags) { free(cols); free(dxy); free(flags); free_ctx(&c); return 0; }
 	if(threads < 1) threads = 1;
 	if(threads > 256) threads = 256;
 	/* TiledIntegrator::render, integrator_tiled.cc:160-214 (not resumed) */
@@ -1912,21 +1939,22 @@ void ora_pt_render(const ora_kdtree *kd, const float *verts, const uint32_t *tri
 	free(dxy);
 	free(flags);
 	free_ctx(&c);
+	return 1;
 }
 
-void ora_pt_render_pixels(const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx, const uint32_t *mat_id,
+int ora_pt_render_pixels(const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx, const uint32_t *mat_id,
                           const uint8_t *vis, const ora_material *mats, const ora_light *lights, uint32_t n_lights,
                           const ora_pt_settings *st, const uint8_t *flags, float *film_rgba, float *film_w,
                           int threads)
 {
 	pt_ctx c;
-	if(!setup_ctx(&c, kd, verts, tri_idx, mat_id, vis, mats, lights, n_lights, st)) return;
+	if(!setup_ctx(&c, kd, verts, tri_idx, mat_id, vis, mats, lights, n_lights, st)) return 0;
 	float table[256], table_scale, filter_width;
 	ora_filter_table(st->aa_pixelwidth, table, &table_scale, &filter_width);
 	const size_t npix = (size_t)st->resx * st->resy;
 	float *cols = (float *)malloc(npix * 4 * sizeof(float));
 	float *dxy = (float *)malloc(npix * 2 * sizeof(float));
-	if(!cols || !dxy) { free(cols); free(dxy); free_ctx(&c); return; }
+	if(!cols || !dxy) { free(cols); free(dxy); free_ctx(&c); return 0; }
 	if(threads < 1) threads = 1;
 	if(threads > 256) threads = 256;
 	c.light_mult = 1.f;
@@ -1934,6 +1962,7 @@ void ora_pt_render_pixels(const ora_kdtree *kd, const float *verts, const uint32
 	free(cols);
 	free(dxy);
 	free_ctx(&c);
+	return 1;
 }
 
 /* ------------------------------------------------------------------ */
@@ -2014,14 +2043,14 @@ void ora_mat_emit(const ora_material *m, const float ng[3], const float wo[3], f
 	out[0] = c.r; out[1] = c.g; out[2] = c.b;
 }
 
-/* ObjectLight pieces on their own: set-up (area_ and the area cdf),
- * illumSample from a point, and intersect along a direction. */
+/* ObjectLight (and BackgroundPortalLight) pieces on their own: set-up (area_
+ * and the area cdf), illumSample from a point, and intersect along a direction. */
 static int light_ctx(pt_ctx *c, const float *verts, const uint32_t *tri_idx, const uint8_t *vis, const ora_light *l,
                      struct obj_light *o)
 {
 	memset(c, 0, sizeof *c);
 	c->verts = verts; c->idx = tri_idx; c->vis = vis;
-	return l->type == ORA_LIGHT_OBJECT && l->n_tris > 0 && obj_light_init(c, l, o);
+	return (l->type == ORA_LIGHT_OBJECT || l->type == ORA_LIGHT_PORTAL) && l->n_tris > 0 && obj_light_init(c, l, o);
 }
 
 float ora_object_light_setup(const float *verts, const uint32_t *tri_idx, const ora_light *l, float *cdf)

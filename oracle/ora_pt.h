@@ -81,6 +81,7 @@ typedef struct
 #define ORA_LIGHT_AREA 0u
 #define ORA_LIGHT_POINT 1u
 #define ORA_LIGHT_OBJECT 2u
+#define ORA_LIGHT_PORTAL 3u   /* "bgPortalLight" */
 
 typedef struct
 {
@@ -89,11 +90,12 @@ typedef struct
 	float from[3];                          /* point light */
 	float color[3];
 	float power;
-	int32_t samples;                        /* area / object light samples (default 4) */
+	int32_t samples;                        /* area / object light samples (default 4), portal (16) */
 	uint32_t cast_shadows;
-	/* object light: the object is the triangle range [first_tri, first_tri + n_tris) */
+	/* object and portal lights: the object is the triangle range [first_tri, first_tri + n_tris) */
 	uint32_t first_tri, n_tris;
 	uint32_t double_sided;                  /* object light "double_sided" (false) */
+	float ibl_clamp_sampling;               /* portal "ibl_clamp_sampling" (0: off) */
 } ora_light;
 
 typedef struct
@@ -150,8 +152,10 @@ typedef struct
  * (colour * weight, alpha * weight) and film_w[y * resx + x] (weights).
  * Buffers are accumulated into (zero them first).  vis: the visibility mask of
  * every triangle (the one kd was built with; object lights read it), NULL: all
- * ORA_VIS_VISIBLE | ORA_VIS_SHADOWS. */
-void ora_pt_render(const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx, const uint32_t *mat_id,
+ * ORA_VIS_VISIBLE | ORA_VIS_SHADOWS.  Returns 0 (film untouched) for a scene
+ * it cannot set up (a portal light without a background, an empty object
+ * light, too deep transparent shadows, no memory), else 1. */
+int ora_pt_render(const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx, const uint32_t *mat_id,
                    const uint8_t *vis, const ora_material *mats, const ora_light *lights, uint32_t n_lights,
                    const ora_pt_settings *st, float *film_rgba, float *film_w, int threads);
 
@@ -160,8 +164,8 @@ void ora_pt_render(const ora_kdtree *kd, const float *verts, const uint32_t *tri
  * splatted into the full-size film as in ora_pt_render, so a pixel whose
  * neighbours within the filter width are all flagged gets the sums a full
  * render gives it.  For checks on windows of frames too large to render on
- * the CPU. */
-void ora_pt_render_pixels(const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx, const uint32_t *mat_id,
+ * the CPU.  Returns 0 or 1 as ora_pt_render. */
+int ora_pt_render_pixels(const ora_kdtree *kd, const float *verts, const uint32_t *tri_idx, const uint32_t *mat_id,
                           const uint8_t *vis, const ora_material *mats, const ora_light *lights, uint32_t n_lights,
                           const ora_pt_settings *st, const uint8_t *flags, float *film_rgba, float *film_w,
                           int threads);

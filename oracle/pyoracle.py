@@ -275,11 +275,13 @@ class OraLight(C.Structure):
     _fields_ = [("type", C.c_uint32), ("corner", C.c_float * 3), ("point1", C.c_float * 3),
                 ("point2", C.c_float * 3), ("from_", C.c_float * 3), ("color", C.c_float * 3),
                 ("power", C.c_float), ("samples", C.c_int32), ("cast_shadows", C.c_uint32),
-                ("first_tri", C.c_uint32), ("n_tris", C.c_uint32), ("double_sided", C.c_uint32)]
+                ("first_tri", C.c_uint32), ("n_tris", C.c_uint32), ("double_sided", C.c_uint32),
+                ("ibl_clamp_sampling", C.c_float)]
 
 
 # light "type" param values of the reference (src/light/light.cc factory), as ORA_LIGHT_*
-LIGHT_TYPES = {"area": 0, "arealight": 0, "point": 1, "pointlight": 1, "object": 2, "objectlight": 2}
+LIGHT_TYPES = {"area": 0, "arealight": 0, "point": 1, "pointlight": 1, "object": 2, "objectlight": 2,
+               "portal": 3, "bgPortalLight": 3}
 
 
 def fill_light(dst, l):
@@ -291,11 +293,12 @@ def fill_light(dst, l):
     dst.from_ = _f3v(l.get("from", (0, 0, 0)))
     dst.color = _f3v(l.get("color", (1, 1, 1)))
     dst.power = float(l.get("power", 1.0))
-    dst.samples = int(l.get("samples", 4))
+    dst.samples = int(l.get("samples", 16 if dst.type == 3 else 4))
     dst.cast_shadows = int(bool(l.get("cast_shadows", True)))
     dst.first_tri = int(l.get("first_tri", 0))
     dst.n_tris = int(l.get("n_tris", 0))
     dst.double_sided = int(bool(l.get("double_sided", False)))
+    dst.ibl_clamp_sampling = float(l.get("ibl_clamp_sampling", 0.0))
 
 
 class OraPtSettings(C.Structure):
@@ -388,6 +391,8 @@ def _pt_lib():
         vp = C.c_void_p
         L.ora_pt_render.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_uint32, vp, vp, vp, C.c_int]
         L.ora_pt_render_pixels.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_uint32, vp, vp, vp, vp, C.c_int]
+        L.ora_pt_render.restype = C.c_int
+        L.ora_pt_render_pixels.restype = C.c_int
         L.ora_pt_pixel_sample.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_uint32, vp, C.c_int, C.c_int, C.c_int,
                                           vp, vp]
         L.ora_object_light_setup.argtypes = [vp, vp, vp, vp]
@@ -446,9 +451,11 @@ class PathTracer:
             st.aa_samples = int(aa_samples)
         rgba = np.zeros((st.resy, st.resx, 4), dtype=np.float32)
         w = np.zeros((st.resy, st.resx), dtype=np.float32)
-        L.ora_pt_render(self.kd._h, self.verts.ctypes.data, self.tris.ctypes.data, self.mat_id.ctypes.data,
-                        self.vis.ctypes.data, C.addressof(self.mats), C.addressof(self.lights), self.n_lights, C.addressof(st),
-                        rgba.ctypes.data, w.ctypes.data, threads or default_threads())
+        ok = L.ora_pt_render(self.kd._h, self.verts.ctypes.data, self.tris.ctypes.data, self.mat_id.ctypes.data,
+                             self.vis.ctypes.data, C.addressof(self.mats), C.addressof(self.lights), self.n_lights,
+                             C.addressof(st), rgba.ctypes.data, w.ctypes.data, threads or default_threads())
+        if not ok:
+            raise RuntimeError("ora_pt_render: scene not supported (see ora_pt.h)")
         return rgba, w
 
     def render_pixels(self, flags, threads=None):
@@ -460,10 +467,12 @@ class PathTracer:
         assert flags.shape == (st.resy, st.resx)
         rgba = np.zeros((st.resy, st.resx, 4), dtype=np.float32)
         w = np.zeros((st.resy, st.resx), dtype=np.float32)
-        L.ora_pt_render_pixels(self.kd._h, self.verts.ctypes.data, self.tris.ctypes.data, self.mat_id.ctypes.data,
-                               self.vis.ctypes.data, C.addressof(self.mats), C.addressof(self.lights), self.n_lights,
-                               C.addressof(st), flags.ctypes.data, rgba.ctypes.data, w.ctypes.data,
-                               threads or default_threads())
+        ok = L.ora_pt_render_pixels(self.kd._h, self.verts.ctypes.data, self.tris.ctypes.data,
+                                    self.mat_id.ctypes.data, self.vis.ctypes.data, C.addressof(self.mats),
+                                    C.addressof(self.lights), self.n_lights, C.addressof(st), flags.ctypes.data,
+                                    rgba.ctypes.data, w.ctypes.data, threads or default_threads())
+        if not ok:
+            raise RuntimeError("ora_pt_render_pixels: scene not supported (see ora_pt.h)")
         return rgba, w
 
     def pixel_sample(self, x, y, sample):

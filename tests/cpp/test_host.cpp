@@ -661,6 +661,10 @@ static void test_errors()
 	CHECK(!sc.createLight("l", l, &err) && !err.empty(), "object light without object_name refused");
 	l.setString("object_name", "nowhere");
 	CHECK(sc.createLight("l", l, &err), "object light accepted at creation: %s", err.c_str());
+	ParamMap portal;
+	portal.setString("type", "bgPortalLight");
+	err.clear();
+	CHECK(!sc.createLight("p", portal, &err) && !err.empty(), "portal light without object_name refused");
 	ParamMap integ, cam, film;
 	err.clear();
 	CHECK(PathIntegrator::create(sc, integ, cam, film, nullptr, &err) == nullptr && !err.empty(),

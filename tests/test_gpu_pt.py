@@ -205,6 +205,41 @@ def test_object_lights_match_oracle(oracle, visibility, double_sided):
     compare_films(g[:2], render_cpu(oracle, sc), f"object lights {visibility}")
 
 
+def test_large_object_light_tree_matches_oracle(oracle):
+    """A 2048-triangle sphere light: its BSDF-sampled rays go through the
+    light's own BVH (as the reference's ObjectLight through its private
+    kd-tree), with the same closest hit as testing every triangle (the
+    oracle's loop); the two-triangle panel keeps the loop."""
+    sc = scenes.object_light_scene(n_random=1000, res=40, spp=4, visibility="no_shadows", subdiv=4)
+    say(f"pt object light tree 40x40x4 ({sc['lights'][0]['n_tris']} light triangles)")
+    g = render_gpu(sc)
+    say(f"  stats {g[2]}")
+    assert g[2]["light_trees"] == 1
+    compare_films(g[:2], render_cpu(oracle, sc), "object light tree")
+
+
+@pytest.mark.parametrize("visibility,clamp", [("invisible", 0.0), ("no_shadows", 0.0), ("no_shadows", 0.7)])
+def test_portal_light_matches_oracle(oracle, visibility, clamp):
+    """A background portal (bgPortalLight) over a 72-triangle window: the
+    background's colour times power through it, one-sided, sampled by area,
+    its own BVH in the BSDF half of the MIS, whose colour ibl_clamp_sampling
+    clamps."""
+    sc = scenes.portal_scene(n_random=2000, res=48, spp=4, visibility=visibility, ibl_clamp_sampling=clamp)
+    say(f"pt portal 48x48x4 ({visibility}, clamp {clamp})")
+    g = render_gpu(sc)
+    say(f"  stats {g[2]}")
+    assert g[2]["light_trees"] == 1
+    compare_films(g[:2], render_cpu(oracle, sc), f"portal {visibility} clamp {clamp}")
+
+
+def test_portal_needs_a_background():
+    from libyafaray_amd.native import YmxError
+    sc = scenes.portal_scene(n_random=0, res=8, spp=1)
+    sc["settings"] = dict(sc["settings"], background=None)
+    with pytest.raises(YmxError):
+        render_gpu(sc)
+
+
 def test_transparent_shadows_match_oracle(oracle):
     """"transpShad" with fake-shadow glass: shadow rays pass up to "shadowDepth"
     glass surfaces, scaled by their kt * filter colour.  The product is taken
