@@ -84,6 +84,8 @@ def parse():
     p.add_argument("--guided-selection", choices=["learn", "fixed"], default="learn",
                    help="BSDF selection probability: learned per region, or --guided-bsdf-prob everywhere")
     p.add_argument("--guided-selection-lr", type=float, default=0.05)
+    p.add_argument("--guided-fallback", action="store_true",
+                   help="per-region fallback to BSDF sampling where guiding's running second moment is the larger")
     p.add_argument("--guided-steps", type=int, default=16, help="Adam steps after each training pass")
     p.add_argument("--guided-lr", type=float, default=1e-2)
     p.add_argument("--guided-depth", type=int, default=GUIDE_DEPTH,
@@ -320,6 +322,9 @@ def measure_guiding(args, torch):
     selection = getattr(args, "guided_selection", "learn")
     if selection == "learn":
         p.set_guide_selection("learn", lr=args.guided_selection_lr)
+    fallback = getattr(args, "guided_fallback", False)
+    if fallback:
+        p.set_guide_fallback("start")
     p.clear()
 
     def train():
@@ -336,6 +341,12 @@ def measure_guiding(args, torch):
         a_sel = p.guide_selection()
         sel_stats = {"min": round(float(a_sel.min()), 3), "median": round(float(np.median(a_sel)), 3),
                      "max": round(float(a_sel.max()), 3)}
+    fb_stats = None
+    if fallback:
+        p.set_guide_fallback("keep")
+        mg, mb = p.guide_fallback()
+        seen = (mg > 0) | (mb > 0)
+        fb_stats = {"regions_with_estimates": int(seen.sum()), "fallen_back": int((seen & (mg > mb)).sum())}
     m = spp - n0
     t_rest = timed(lambda: p.render(n0, m))
     rgba_c, w_c = p.film()
@@ -361,6 +372,8 @@ def measure_guiding(args, torch):
         pg.set_guide(g2, bsdf_prob=alpha, train=True, train_steps=steps)
         if selection == "learn":
             pg.set_guide_selection("learn", lr=args.guided_selection_lr)
+        if fallback:
+            pg.set_guide_fallback("start")
         pg.clear()
 
         def train_v():
@@ -371,6 +384,8 @@ def measure_guiding(args, torch):
         pg.set_guide(g2, bsdf_prob=alpha, train=False)
         if selection == "learn":
             pg.set_guide_selection("keep")
+        if fallback:
+            pg.set_guide_fallback("keep")
         t_gv += timed(lambda: pg.render(n0, m_v) if m_v > 0 else None)
         e_gv = relmse(image(pg), ref) - ref_err
         pg.set_guide(None)
@@ -402,6 +417,7 @@ def measure_guiding(args, torch):
            "bsdf_prob": alpha, "lr": args.guided_lr, "grid_levels": f"16 (resolution {n_min}..{n_max})",
            "selection": selection if selection == "fixed" else f"learned per region (lr {args.guided_selection_lr})",
            "learned_selection": sel_stats,
+           "fallback": fb_stats,
            "equal_time_relmse_plain": e_plain, "equal_time_relmse_guided": e_guided_eq,
            "equal_time_gain": round(e_plain / e_guided_eq, 3) if e_guided_eq > 0 else None,
            "relmse_plain": e_plain, "relmse_guided": e_all,

@@ -78,7 +78,31 @@ void free_guide_buffers(PtGuide &g)
 	                g.query_count};
 	for(void *p : ptrs)
 		if(p) (void)hipFree(p);
-	memset(&g, 0, sizeof g);  // the learned-selection arrays belong to ymx_pt
+	memset(&g, 0, sizeof g);  // the selection table belongs to the guide
+}
+
+// Points the tracer's PtGuide at its guide's selection table (allocated on
+// first use) for the learned selection and the fallback the tracer has
+// switched on.  False when the table cannot be allocated.
+bool attach_selection(ymx_pt *pt)
+{
+	PtGuide &g = pt->g;
+	g.learn_alpha = 0u;
+	g.fallback = 0u;
+	g.alpha_theta = g.alpha_m = g.alpha_v = g.alpha_grad = g.fb_sum = g.fb_run = nullptr;
+	if(!pt->guide || (!pt->alpha_learned && !pt->fallback_on)) return true;
+	float *t = ymx_guide_selection_table(pt->guide, kSelectionTableFloats);
+	if(!t) return false;
+	const size_t cells = (size_t)kGuideCells * kGuideCells * kGuideCells;
+	g.alpha_theta = t;
+	g.alpha_m = t + cells;
+	g.alpha_v = t + 2 * cells;
+	g.alpha_grad = t + 3 * cells;
+	g.fb_sum = t + 4 * cells;
+	g.fb_run = t + 7 * cells;
+	g.learn_alpha = pt->alpha_learned ? 1u : 0u;
+	g.fallback = pt->fallback_on ? 1u : 0u;
+	return true;
 }
 
 void free_pt(ymx_pt *pt)
@@ -824,19 +848,10 @@ int ymx_pt_set_guide(ymx_pt *pt, ymx_guide *guide, float bsdf_prob, int32_t trai
 		ymx_guide_bounds(guide, g.bmin, bmax);
 		for(int k = 0; k < 3; ++k) g.scale[k] = (float)kGuideCells / (bmax[k] - g.bmin[k]);
 	}
-	if(pt->alpha_learned)
+	if(!attach_selection(pt))
 	{
-		const size_t cells = (size_t)kGuideCells * kGuideCells * kGuideCells;
-		float *t = ymx_guide_selection_table(guide, 4 * cells);
-		if(t)
-		{
-			g.learn_alpha = 1u;
-			g.alpha_theta = t;
-			g.alpha_m = t + cells;
-			g.alpha_v = t + 2 * cells;
-			g.alpha_grad = t + 3 * cells;
-		}
-		else pt->alpha_learned = false;
+		pt->alpha_learned = false;
+		pt->fallback_on = false;
 	}
 	if(train)
 	{
@@ -881,10 +896,10 @@ int ymx_pt_set_guide_selection(ymx_pt *pt, int32_t mode, float lr)
 	if(mode == 0)
 	{
 		pt->alpha_learned = false;
-		g.learn_alpha = 0u;
+		attach_selection(pt);
 		return YMX_OK;
 	}
-	float *t = ymx_guide_selection_table(pt->guide, 4 * cells);
+	float *t = ymx_guide_selection_table(pt->guide, kSelectionTableFloats);
 	if(!t)
 	{
 		ymx_set_error("ymx_pt_set_guide_selection: no device memory for the selection table");
@@ -901,11 +916,47 @@ int ymx_pt_set_guide_selection(ymx_pt *pt, int32_t mode, float lr)
 		YMX_HIP_CHECK(hipMemcpy(t, init.data(), sizeof(float) * init.size(), hipMemcpyHostToDevice));
 	}
 	pt->alpha_learned = true;
-	g.learn_alpha = 1u;
-	g.alpha_theta = t;
-	g.alpha_m = t + cells;
-	g.alpha_v = t + 2 * cells;
-	g.alpha_grad = t + 3 * cells;
+	attach_selection(pt);
+	return YMX_OK;
+}
+
+int ymx_pt_set_guide_fallback(ymx_pt *pt, int32_t mode)
+{
+	if(!pt || !pt->guide || mode < 0 || mode > 2)
+	{
+		ymx_set_error("ymx_pt_set_guide_fallback: no guide set or mode outside 0..2");
+		return YMX_ERR_INVALID_ARG;
+	}
+	YMX_HIP_CHECK(hipDeviceSynchronize());
+	pt->fallback_on = mode != 0;
+	if(mode != 0)
+	{
+		float *t = ymx_guide_selection_table(pt->guide, kSelectionTableFloats);
+		if(!t)
+		{
+			pt->fallback_on = false;
+			ymx_set_error("ymx_pt_set_guide_fallback: no device memory for the selection table");
+			return YMX_ERR_OUT_OF_MEMORY;
+		}
+		// mode 1: no region has fallen back yet: sums and running moments zero
+		const size_t cells = (size_t)kGuideCells * kGuideCells * kGuideCells;
+		if(mode == 1) YMX_HIP_CHECK(hipMemset(t + 4 * cells, 0, sizeof(float) * 5 * cells));
+	}
+	attach_selection(pt);
+	return YMX_OK;
+}
+
+int ymx_pt_get_guide_fallback(const ymx_pt *pt, float *moments, uint32_t n)
+{
+	const size_t cells = (size_t)kGuideCells * kGuideCells * kGuideCells;
+	if(!pt || !moments || n < 2 * cells || !pt->g.fallback)
+	{
+		ymx_set_error("ymx_pt_get_guide_fallback: NULL argument, fewer than %zu entries or no fallback set",
+		              2 * cells);
+		return YMX_ERR_INVALID_ARG;
+	}
+	YMX_HIP_CHECK(hipDeviceSynchronize());
+	YMX_HIP_CHECK(hipMemcpy(moments, pt->g.fb_run, sizeof(float) * 2 * cells, hipMemcpyDeviceToHost));
 	return YMX_OK;
 }
 
@@ -919,13 +970,20 @@ int ymx_pt_get_guide_selection(const ymx_pt *pt, float *alpha, uint32_t n)
 	}
 	YMX_HIP_CHECK(hipDeviceSynchronize());
 	if(!pt->g.learn_alpha)
-	{
 		for(size_t i = 0; i < cells; ++i) alpha[i] = pt->g.bsdf_prob;
-		return YMX_OK;
+	else
+	{
+		std::vector<float> theta(cells);
+		YMX_HIP_CHECK(hipMemcpy(theta.data(), pt->g.alpha_theta, sizeof(float) * cells, hipMemcpyDeviceToHost));
+		for(size_t i = 0; i < cells; ++i) alpha[i] = kAlphaMin + (1.f - kAlphaMin) / (1.f + expf(-theta[i]));
 	}
-	std::vector<float> theta(cells);
-	YMX_HIP_CHECK(hipMemcpy(theta.data(), pt->g.alpha_theta, sizeof(float) * cells, hipMemcpyDeviceToHost));
-	for(size_t i = 0; i < cells; ++i) alpha[i] = kAlphaMin + (1.f - kAlphaMin) / (1.f + expf(-theta[i]));
+	if(pt->g.fallback)
+	{
+		std::vector<float> run(2 * cells);
+		YMX_HIP_CHECK(hipMemcpy(run.data(), pt->g.fb_run, sizeof(float) * 2 * cells, hipMemcpyDeviceToHost));
+		for(size_t i = 0; i < cells; ++i)
+			if(run[i] > run[cells + i]) alpha[i] = 1.f;
+	}
 	return YMX_OK;
 }
 

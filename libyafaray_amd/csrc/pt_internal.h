@@ -193,8 +193,19 @@ struct PtGuide
 	float *alpha_theta;        // [kGuideCells^3]
 	float *alpha_m, *alpha_v;  // Adam moments
 	float *alpha_grad;         // gradient sums of one mini-batch
+	// per-region fallback to BSDF sampling (ymx_pt_set_guide_fallback): running
+	// second moments of the guided estimator (at the region's selection
+	// probability) and of BSDF sampling alone, estimated from the training
+	// records; a region whose guided moment is the larger samples the BSDF only
+	uint32_t fallback;
+	float *fb_sum;             // [3][kGuideCells^3] one mini-batch's sums: guided, BSDF, records
+	float *fb_run;             // [2][kGuideCells^3] running moments: guided, BSDF
 };
 constexpr float kAlphaMin = 0.2f;  // the BSDF is sampled with probability >= 0.2 at a guided vertex
+constexpr float kFallbackDecay = 0.9f;  // weight of the running moments against one mini-batch's
+// The guide's selection table: theta, the two Adam moments and the gradient
+// sums of the learned selection, then the fallback's sums and running moments.
+constexpr size_t kSelectionTableFloats = (size_t)9 * kGuideCells * kGuideCells * kGuideCells;
 
 }  // namespace ymx
 
@@ -228,6 +239,7 @@ struct ymx_pt
 	uint32_t g_seen, g_filled;       // training samples written so far (counter value), valid in the ring
 	uint64_t g_rng;                  // mini-batch offsets
 	bool alpha_learned;              // ymx_pt_set_guide_selection: the guide's learned selection is used
+	bool fallback_on;                // ymx_pt_set_guide_fallback: regions may fall back to BSDF sampling
 	float alpha_lr;
 	uint32_t alpha_steps;
 	uint32_t batch;                  // AA samples per wavefront (<= batch_max; ymx_pt_autotune may lower it)

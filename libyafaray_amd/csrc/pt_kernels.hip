@@ -612,14 +612,25 @@ __device__ __forceinline__ uint32_t guide_cell(const PtGuide &g, const float p[3
 	return (c[2] * kGuideCells + c[1]) * kGuideCells + c[0];
 }
 
-// The probability of sampling the BSDF at a guided vertex at p: the fixed
+// The selection probability of region `cell` before the fallback: the fixed
 // bsdf_prob, or the region's learned value.
-__device__ __forceinline__ float selection_prob(const PtGuide &g, V3 p)
+__device__ __forceinline__ float region_alpha(const PtGuide &g, uint32_t cell)
 {
 	if(!g.learn_alpha) return g.bsdf_prob;
+	return kAlphaMin + (1.f - kAlphaMin) / (1.f + __expf(-g.alpha_theta[cell]));
+}
+
+// The probability of sampling the BSDF at a guided vertex at p: 1 in a region
+// fallen back to BSDF sampling (its running guided second moment above the
+// BSDF's), else the region's alpha.
+__device__ __forceinline__ float selection_prob(const PtGuide &g, V3 p)
+{
+	if(!g.learn_alpha && !g.fallback) return g.bsdf_prob;
 	const float pp[3] = {p.x, p.y, p.z};
-	const float t = g.alpha_theta[guide_cell(g, pp)];
-	return kAlphaMin + (1.f - kAlphaMin) / (1.f + __expf(-t));
+	const uint32_t cell = guide_cell(g, pp);
+	constexpr uint32_t kCells = (uint32_t)(kGuideCells * kGuideCells * kGuideCells);
+	if(g.fallback && g.fb_run[cell] > g.fb_run[kCells + cell]) return 1.f;
+	return region_alpha(g, cell);
 }
 
 // The densities a guided direction was drawn with: the BSDF's cosine lobe p,
@@ -1822,6 +1833,55 @@ __global__ __launch_bounds__(kBlock) void k_alpha_grad(const ymx_guide_sample *b
 	if(isfinite(d_theta)) atomicAdd(g.alpha_grad + cell, d_theta);
 }
 
+// Per-region fallback (ymx_pt_set_guide_fallback).  A record drawn with
+// densities p, q, alpha_old (p_old = alpha_old p + (1 - alpha_old) q) and
+// target weight c has integrand F = c p_old.  The second moment of a
+// one-sample estimator with density r, M_r = integral of F^2 / r, is
+// estimated from the records' draws by the mean of F^2 / (r p_old): here for
+// r = the region's mixture at its alpha (before the fallback) and r = p.
+// Both means share the region's weight normalisation, so their comparison
+// does not depend on it.
+__global__ __launch_bounds__(kBlock) void k_fallback_sum(const ymx_guide_sample *batch, uint32_t n, PtGuide g)
+{
+	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if(i >= n) return;
+	const ymx_guide_sample r = batch[i];
+	const float p = r.pad0, q = r.pad1, a_old = r.pad2;
+	const float p_old = a_old * p + (1.f - a_old) * q;
+	if(!(r.c > 0.f) || !(p_old > 0.f) || !(p > 0.f)) return;
+	const uint32_t cell = guide_cell(g, r.pos);
+	const float alpha = region_alpha(g, cell);
+	const float pm = alpha * p + (1.f - alpha) * q;
+	const float f = r.c * p_old;
+	const float m_guided = f * f / (pm * p_old);
+	const float m_bsdf = f * f / (p * p_old);
+	if(!isfinite(m_guided) || !isfinite(m_bsdf)) return;
+	constexpr uint32_t kCells = (uint32_t)(kGuideCells * kGuideCells * kGuideCells);
+	atomicAdd(g.fb_sum + cell, m_guided);
+	atomicAdd(g.fb_sum + kCells + cell, m_bsdf);
+	atomicAdd(g.fb_sum + 2 * kCells + cell, 1.f);
+}
+
+// The running moments of each region with records in the mini-batch: the
+// first batch's means, then kFallbackDecay * running + (1 - kFallbackDecay) *
+// the batch's mean.  Clears the sums.
+__global__ __launch_bounds__(kBlock) void k_fallback_update(PtGuide g)
+{
+	constexpr uint32_t kCells = (uint32_t)(kGuideCells * kGuideCells * kGuideCells);
+	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if(i >= kCells) return;
+	const float cnt = g.fb_sum[2 * kCells + i];
+	if(cnt == 0.f) return;
+	const float mg = g.fb_sum[i] / cnt, mb = g.fb_sum[kCells + i] / cnt;
+	g.fb_sum[i] = 0.f;
+	g.fb_sum[kCells + i] = 0.f;
+	g.fb_sum[2 * kCells + i] = 0.f;
+	const float rg = g.fb_run[i], rb = g.fb_run[kCells + i];
+	const bool first = rg == 0.f && rb == 0.f;
+	g.fb_run[i] = first ? mg : kFallbackDecay * rg + (1.f - kFallbackDecay) * mg;
+	g.fb_run[kCells + i] = first ? mb : kFallbackDecay * rb + (1.f - kFallbackDecay) * mb;
+}
+
 __global__ __launch_bounds__(kBlock) void k_alpha_adam(PtGuide g, uint32_t cells, float lr, float bc1, float bc2)
 {
 	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -2502,6 +2562,14 @@ int ymx_pt_train_guide(ymx_pt *pt, hipStream_t stream)
 				const float bc1 = 1.f - powf(0.9f, (float)pt->alpha_steps);
 				const float bc2 = 1.f - powf(0.999f, (float)pt->alpha_steps);
 				k_alpha_adam<<<blocks_for(cells), kBlock, 0, stream>>>(pt->g, cells, pt->alpha_lr, bc1, bc2);
+				YMX_HIP_CHECK(hipGetLastError());
+			}
+			if(pt->g.fallback)
+			{
+				// the moments at the alpha the selection step just set
+				constexpr uint32_t cells = (uint32_t)(kGuideCells * kGuideCells * kGuideCells);
+				k_fallback_sum<<<blocks_for(batch), kBlock, 0, stream>>>(pt->g.batch_buf, batch, pt->g);
+				k_fallback_update<<<blocks_for(cells), kBlock, 0, stream>>>(pt->g);
 				YMX_HIP_CHECK(hipGetLastError());
 			}
 		}

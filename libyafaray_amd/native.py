@@ -87,6 +87,8 @@ SIGNATURES = {
     "ymx_pt_set_guide_depth": (C.c_int, [C.c_void_p, C.c_int32]),
     "ymx_pt_set_guide_selection": (C.c_int, [C.c_void_p, C.c_int32, C.c_float]),
     "ymx_pt_get_guide_selection": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    "ymx_pt_set_guide_fallback": (C.c_int, [C.c_void_p, C.c_int32]),
+    "ymx_pt_get_guide_fallback": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "ymx_pt_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(YmxPtTuning), C.c_void_p]),
     "ymx_pt_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(YmxPtTuning)]),
     "ymx_pt_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(YmxPtTuning)]),

@@ -262,6 +262,7 @@ class GpuPathTracer:
         return rgba, w
 
     SELECTION = {"fixed": 0, "learn": 1, "keep": 2}
+    FALLBACK = {"off": 0, "start": 1, "keep": 2}
 
     def set_guide_selection(self, mode: str, lr: float = 0.05):
         """ymx_pt_set_guide_selection (after set_guide): "learn" the
@@ -271,10 +272,25 @@ class GpuPathTracer:
               "ymx_pt_set_guide_selection")
 
     def guide_selection(self) -> np.ndarray:
-        """Selection probability per region, shape (16, 16, 16) as [z, y, x]."""
+        """Selection probability per region, shape (16, 16, 16) as [z, y, x]
+        (1 where the region has fallen back to BSDF sampling)."""
         a = np.zeros(16 ** 3, dtype=np.float32)
         check(self._lib.ymx_pt_get_guide_selection(self._h, a.ctypes.data, a.size), "ymx_pt_get_guide_selection")
         return a.reshape(16, 16, 16)
+
+    def set_guide_fallback(self, mode: str = "start"):
+        """ymx_pt_set_guide_fallback (after set_guide): regions whose running
+        guided second moment exceeds BSDF sampling's sample the BSDF alone;
+        "start" from no estimates, "keep" the guide's table as another tracer
+        left it, or "off"."""
+        check(self._lib.ymx_pt_set_guide_fallback(self._h, self.FALLBACK[mode]), "ymx_pt_set_guide_fallback")
+
+    def guide_fallback(self) -> tuple[np.ndarray, np.ndarray]:
+        """The fallback's running second moments per region (guided, BSDF),
+        each of shape (16, 16, 16) as [z, y, x]; 0 where no estimate yet."""
+        m = np.zeros(2 * 16 ** 3, dtype=np.float32)
+        check(self._lib.ymx_pt_get_guide_fallback(self._h, m.ctypes.data, m.size), "ymx_pt_get_guide_fallback")
+        return m[:16 ** 3].reshape(16, 16, 16), m[16 ** 3:].reshape(16, 16, 16)
 
     KERNELS = {0: "wave-ordered", 1: "replacement"}
 

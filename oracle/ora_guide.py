@@ -248,6 +248,35 @@ def selection_alpha(theta):
     return ALPHA_MIN + (1.0 - ALPHA_MIN) / (1.0 + np.exp(-np.asarray(theta, np.float64)))
 
 
+FALLBACK_DECAY = 0.9
+
+
+def fallback_terms(c, p, q, alpha_old, alpha):
+    """Per-record terms of the per-region fallback (the path tracer's
+    k_fallback_sum): a record drawn from p_old = alpha_old p + (1 - alpha_old) q
+    with weight c has integrand F = c p_old; F^2 / (r p_old) has expectation
+    integral F^2 / r under the draws, the second moment of the one-sample
+    estimator with density r.  Returns the terms for r = the mixture at the
+    region's alpha and for r = p (BSDF sampling alone)."""
+    c, p, q, a0, al = (np.asarray(v, np.float64) for v in (c, p, q, alpha_old, alpha))
+    p_old = a0 * p + (1.0 - a0) * q
+    f = c * p_old
+    return f * f / ((al * p + (1.0 - al) * q) * p_old), f * f / (p * p_old)
+
+
+def fallback_update(run_guided, run_bsdf, batch_guided, batch_bsdf, decay=FALLBACK_DECAY):
+    """Running moments after one mini-batch with records in the region (its
+    means given; k_fallback_update): the batch's means when the region has no
+    estimate yet (both 0), else decay * running + (1 - decay) * batch.  The
+    region samples the BSDF alone while the guided moment is the larger."""
+    if run_guided == 0.0 and run_bsdf == 0.0:
+        g, b = batch_guided, batch_bsdf
+    else:
+        g = decay * run_guided + (1.0 - decay) * batch_guided
+        b = decay * run_bsdf + (1.0 - decay) * batch_bsdf
+    return g, b, g > b
+
+
 def selection_grad(c, p, q, alpha_old, theta):
     """Per-record gradient of the one-sample MIS estimator's second moment in
     the region's logit theta (the path tracer's k_alpha_grad): a record drawn

@@ -261,3 +261,49 @@ def test_selection_gradient_is_the_second_moments_derivative():
             if expect_sign is not None:
                 # q proportional to F is the zero-variance density: lowering alpha lowers M
                 assert grad > 0
+
+
+def test_fallback_moments_estimate_the_second_moments():
+    """The fallback's per-record terms, averaged over draws from p_old, are
+    unbiased for the second moments sum F^2 / p_mix(alpha) and sum F^2 / p of
+    the guided and the BSDF-only one-sample estimators (exact expectation over
+    a discrete set of directions, and a Monte Carlo average of 200k draws within
+    5 of its standard errors).  The decision follows the exact comparison: a mixture proportional to
+    F (zero variance) stays guided, a mixture that misses F falls back, and
+    with F proportional to p (uniform incoming radiance, diffuse BSDF) every
+    mixture other than p falls back."""
+    from oracle import ora_guide as G
+    rng = np.random.default_rng(11)
+    n = 64
+    F = rng.gamma(0.5, 1.0, n)
+    p = rng.dirichlet(np.ones(n))
+    cases = ((F / F.sum(), False), (rng.dirichlet(np.ones(n) * 0.05), True))
+    for q, falls_back in cases:
+        for a_old, alpha in ((0.5, 0.5), (0.2, 0.7), (1.0, 0.3)):
+            p_old = a_old * p + (1 - a_old) * q
+            c = F / p_old
+            tg, tb = G.fallback_terms(c, p, q, a_old, alpha)
+            pm = alpha * p + (1 - alpha) * q
+            exact_g, exact_b = float((F * F / pm).sum()), float((F * F / p).sum())
+            assert float((p_old * tg).sum()) == pytest.approx(exact_g, rel=1e-12)
+            assert float((p_old * tb).sum()) == pytest.approx(exact_b, rel=1e-12)
+            # Monte Carlo: within 5 standard errors (exact variance of the terms)
+            m = 200_000
+            draws = rng.choice(n, size=m, p=p_old / p_old.sum())
+            for t, exact in ((tg, exact_g), (tb, exact_b)):
+                se = np.sqrt(max(float((p_old * t * t).sum()) - exact * exact, 0.0) / m)
+                assert abs(t[draws].mean() - exact) <= 5 * se + 1e-12 * exact
+            assert G.fallback_update(0.0, 0.0, exact_g, exact_b)[2] == falls_back
+    # F proportional to p: BSDF sampling has zero variance, any other mixture falls back
+    Fp = 3.0 * p
+    for q in (rng.dirichlet(np.ones(n)), 0.5 * p + 0.5 * rng.dirichlet(np.ones(n))):
+        tg, tb = G.fallback_terms(Fp / p, p, q, 1.0, 0.5)
+        assert float((p * tg).sum()) > float((p * tb).sum()) == pytest.approx(9.0)
+    # the running update: first batch taken as is, then an exponential average
+    g, b, fb = G.fallback_update(0.0, 0.0, 2.0, 1.0)
+    assert (g, b, fb) == (2.0, 1.0, True)
+    g, b, fb = G.fallback_update(g, b, 0.5, 1.0)
+    assert g == pytest.approx(1.85) and b == pytest.approx(1.0) and fb
+    for _ in range(20):
+        g, b, fb = G.fallback_update(g, b, 0.5, 1.0)
+    assert not fb
