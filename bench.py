@@ -77,6 +77,8 @@ def parse():
     p.add_argument("--no-guided", action="store_true", help="skip the guided path-tracing measurement")
     p.add_argument("--guided-only", action="store_true", help="only the guided path-tracing measurement")
     p.add_argument("--guided-res", type=str, default="1280x720")
+    p.add_argument("--guided-scene", choices=["room", "large"], default="room",
+                   help="the occluded room, or the large terrain (at --guided-res) for its guided half")
     p.add_argument("--guided-ref-spp", type=int, default=4096)
     p.add_argument("--guided-spp", type=int, default=640,
                    help="samples of each tracer; near the 1e-3 target so that little is extrapolated")
@@ -231,12 +233,13 @@ def measure_large_frame(args, torch):
 GUIDE_N_MIN, GUIDE_N_MAX = 4, 32   # hash-grid resolutions of the guided occluded-room run
 GUIDE_TRAIN_SPP = 32               # the guide trains on the first 32 spp of the render
 GUIDE_DEPTH = 1                    # guided: the camera vertex and the next path vertex
-_GUIDING_REFERENCE = {}            # (resx, resy, spp) -> (reference image, its own relMSE)
+_GUIDING_REFERENCE = {}            # (large scene, resx, resy, spp) -> (reference image, its own relMSE)
 
 
 def measure_guiding(args, torch):
     """Occluded room (4 x 3 x 3 m, about 533k triangles, lit through a 0.1 m
-    slit from an adjacent room; max depth 10): per-pixel relMSE
+    slit from an adjacent room; max depth 10), or with --guided-scene large the
+    large terrain (36.1M triangles, depth 12) at --guided-res: per-pixel relMSE
     (I - R)^2 / (R^2 + 0.01) against a `guided_ref_spp` plain reference (two
     independent halves, whose own error is subtracted), and the time each
     tracer needs to reach relMSE 1e-3.
@@ -262,7 +265,14 @@ def measure_guiding(args, torch):
     from libyafaray_amd.guide import GpuGuide
     from libyafaray_amd.pt import GpuPathTracer
     resx, resy = (int(v) for v in args.guided_res.split("x"))
-    sc = scenes.occluded_room_scene(res=(resx, resy), spp=16)
+    large = getattr(args, "guided_scene", "room") == "large"
+    if large:
+        sc = scenes.large_terrain_scene(res=(resx, resy), spp=16)
+        workload = (f"large terrain 4096^2 + 2000 icospheres, {len(sc['tris'])} tris, {resx}x{resy}, "
+                    f"depth {sc['settings']['bounces']}, {len(sc['lights'])} lights")
+    else:
+        sc = scenes.occluded_room_scene(res=(resx, resy), spp=16)
+        workload = f"occluded room, {len(sc['tris'])} tris, {resx}x{resy}, depth 10, slit 0.1 m"
     spp, ref_spp = args.guided_spp, args.guided_ref_spp
     n0 = GUIDE_TRAIN_SPP
     if spp <= n0:
@@ -282,7 +292,8 @@ def measure_guiding(args, torch):
     def relmse(a, b):
         return float(((a - b) ** 2 / (b.astype(np.float64) ** 2 + 1e-2)).mean())
 
-    key = (resx, resy, ref_spp)
+    log(args, f"guided: {workload}")
+    key = (large, resx, resy, ref_spp)
     if key not in _GUIDING_REFERENCE:
         pt = tracer(ref_spp // 2, 777)
         pt.clear()
@@ -293,6 +304,7 @@ def measure_guiding(args, torch):
         r2 = image(pt)
         pt.close()
         _GUIDING_REFERENCE[key] = ((r1 + r2) / 2, relmse(r1, r2) / 4)
+        log(args, f"guided: {ref_spp}-spp reference rendered")
     ref, ref_err = _GUIDING_REFERENCE[key]
 
     def timed(fn):
@@ -406,7 +418,7 @@ def measure_guiding(args, torch):
     # equal time: the guided image rendered in the plain tracer's time for `spp`
     m_eq = max(0.0, (t_plain - t_train) / (t_rest / m))
     e_guided_eq = (n0 * v_t + m_eq * v_g) / (n0 + m_eq) ** 2
-    out = {"workload": f"occluded room, {len(sc['tris'])} tris, {resx}x{resy}, depth 10, slit 0.1 m",
+    out = {"workload": workload,
            "reference_spp": ref_spp, "reference_relmse": ref_err, "spp": spp,
            "plain_ms": round(t_plain * 1e3, 1),
            "guided_ms": round((t_train + t_rest) * 1e3, 1),
