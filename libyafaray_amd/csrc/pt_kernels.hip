@@ -648,7 +648,9 @@ struct GuidePdfs
 // direction and weight (its direction is unit only to the accuracy of the
 // fast sine and cosine); both densities are evaluated at the unit direction,
 // since a sharp lobe's exp(kappa (mu . w - 1)) amplifies any length error by
-// kappa.  w_plain: the sampler's w for wi (what a later vertex that leaves w
+// kappa.  At alpha 1 (a region fallen back to BSDF sampling, or bsdf_prob 1)
+// the vertex is the plain tracer's: its own low-discrepancy sample and weight
+// sw.  w_plain: the sampler's w for wi (what a later vertex that leaves w
 // unset reuses).  Returns false when the weight is 0.
 __device__ bool guided_sample(const PtArgs &a, const Surf &sp, V3 wo, const float *mix, BSample &s, float u_sel,
                               float u0, float u1, float u2, V3 &wi, Rgb &weight, float &w_plain, GuidePdfs &pdfs)
@@ -663,8 +665,13 @@ __device__ bool guided_sample(const PtArgs &a, const Surf &sp, V3 wo, const floa
 		// the light samples of the next vertex, and weighting them by the MIS
 		// factor p / (alpha p + (1 - alpha) q) would no longer average to the
 		// plain estimate (it leaves the camera-vertex bounce about 1 % high).
-		s.s1 = u1;
-		s.s2 = u2;
+		// At alpha 1 (a region fallen back, or bsdf_prob 1) the factor is 1 and
+		// the vertex is the plain tracer's: it keeps the sample's own dimensions.
+		if(alpha < 1.f)
+		{
+			s.s1 = u1;
+			s.s2 = u2;
+		}
 		w_plain = 0.f;
 		sw = cscale(sd_sample(m, sp, wo, wi, s, w_plain), w_plain);
 	}
@@ -685,6 +692,11 @@ __device__ bool guided_sample(const PtArgs &a, const Surf &sp, V3 wo, const floa
 	pdfs.p = p;
 	pdfs.q = q;
 	pdfs.alpha = alpha;
+	if(alpha >= 1.f)
+	{
+		weight = sw;
+		return !cblack(sw);
+	}
 	if(!(p > 0.f) || !(denom > 0.f))
 	{
 		weight = rgb(0.f, 0.f, 0.f);

@@ -633,6 +633,26 @@ def portal_scene(n_random: int = 2000, seed: int = 0, res: int = 64, spp: int = 
                 camera=camera, settings=settings)
 
 
+def spot_room_scene(res: int = 64, spp: int = 16, seed: int = 0, h: float = 0.1, size: float = 0.1,
+                    bounces: int = 2):
+    """Guiding workload where the incoming light is concentrated: the five
+    Cornell walls (albedo 0.7, open at y = 0) lit by one small area light
+    (`size` square, power 200) facing down at height `h` above the floor
+    centre, so it lights a small spot of the floor and the rest of the box
+    sees mostly that spot.  No roulette; path depth `bounces`."""
+    wv, wt, _ = cornell_walls()
+    c = (0.5 - size / 2, 0.5 - size / 2, h)
+    # point1 / point2 chosen so that the light faces -z (towards the floor)
+    lights = [dict(type="area", corner=c, point1=(c[0], c[1] + size, h), point2=(c[0] + size, c[1], h),
+                   color=(1.0, 1.0, 1.0), power=200.0, samples=1)]
+    camera = dict(from_=(0.5, -1.6, 0.5), to=(0.5, 0.5, 0.5), up=(0.5, -1.6, 1.5),
+                  focal=float(1.0 / (2.0 * np.tan(np.radians(20.0)))), resx=res, resy=res)
+    settings = dict(path_samples=1, bounces=bounces, rr_min_bounces=10, aa_samples=spp, aa_pixelwidth=1.5,
+                    background=None, seed=seed, raydepth=0)
+    return dict(verts=wv, tris=wt, mat_id=np.zeros(len(wt), np.uint32), materials=[dict(color=(0.7, 0.7, 0.7))],
+                lights=lights, camera=camera, settings=settings)
+
+
 def terrain_spheres_scene(res=(1920, 1080), spp: int = 64, seed: int = 0, hf_res: int = 1024, n_spheres: int = 500,
                           subdiv: int = 3, n_point_lights: int = 64, depth: int = 8, area_lights=None):
     """Mixed-material path-tracing workload: the ray-cast heightfield

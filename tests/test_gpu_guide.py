@@ -328,3 +328,53 @@ def test_fallback_to_bsdf_sampling_where_guiding_loses():
         assert abs(m - ref.mean()) < 4 * se + 1e-3 * ref.mean()
         results[name] = var
     assert results["guided + fallback"] < results["guided"]
+
+
+def test_trained_guide_beats_bsdf_sampling_on_a_spot_lit_room():
+    """What the trained network gains over the method without it: on a box lit
+    mostly by a small bright spot of the floor (scenes.spot_room_scene), the
+    guide trained on the render's first 32 spp and combined with BSDF
+    sampling (probability 0.5) gives a lower relMSE at 64 further spp than
+    plain BSDF sampling, against a 8192-spp reference, averaged over 4 seeds
+    (measured: about 2x; asserted: 1.4x), and stays unbiased in the mean."""
+    from libyafaray_amd import scenes
+    from libyafaray_amd.guide import GpuGuide
+    from libyafaray_amd.pt import GpuPathTracer
+    sc = scenes.spot_room_scene(res=64)
+
+    def render(spp, seed, guide=None):
+        s = dict(sc)
+        s["settings"] = dict(sc["settings"], aa_samples=spp + (32 if guide else 0), seed=seed)
+        pt = GpuPathTracer(s)
+        first = 0
+        if guide is not None:
+            pt.set_guide_depth(1)
+            pt.set_guide(guide, bsdf_prob=0.5, train=True, train_steps=16)
+            pt.clear()
+            for k in range(8):
+                pt.render(4 * k, 4)
+            pt.set_guide(guide, bsdf_prob=0.5, train=False)
+            first = 32
+        pt.clear()
+        pt.render(first, spp)
+        rgba, w = pt.film()
+        pt.close()
+        return rgba[..., :3].astype(np.float64) / w[..., None]
+
+    ref = (render(4096, 91) + render(4096, 92)) / 2
+    rel = lambda x: float(((x - ref) ** 2 / (ref ** 2 + 1e-2)).mean())
+    plain, guided, means = [], [], []
+    bmin, bmax = sc["verts"].min(0) - 1e-3, sc["verts"].max(0) + 1e-3
+    for sd in range(1, 5):
+        plain.append(rel(render(64, sd)))
+        g = GpuGuide(bmin, bmax, log2_table_size=19, n_min=4, n_max=32, max_batch=1 << 16, lr=1e-2, seed=sd)
+        img = render(64, sd, guide=g)
+        g.close()
+        guided.append(rel(img))
+        means.append(img.mean())
+    gain = np.mean(plain) / np.mean(guided)
+    m, se = np.mean(means), np.std(means, ddof=1) / 2
+    say(f"  spot room 64 spp: plain relMSE {np.mean(plain):.3e}, guided {np.mean(guided):.3e}, gain {gain:.2f}; "
+        f"guided mean {m:.5f} +- {se:.5f} (reference {ref.mean():.5f})")
+    assert gain > 1.4
+    assert abs(m - ref.mean()) < 4 * se + 2e-3 * ref.mean()
