@@ -593,11 +593,21 @@ __device__ Rgb sd_weight(const PtMaterial &m, const Surf &sp, V3 wo, V3 wi, floa
 	return scolor;
 }
 
-// Counter-based uniforms of the guided sampling, distinct from the roulette stream.
+// Uniforms of the guided sampling (selection and direction, k = 0..3): over a
+// pixel's samples ps, the 4-D Kronecker sequence frac(ps g_k) with
+// g_k = phi^-(k+1), phi the real root of x^5 = x + 1 (M. Roberts' R4
+// sequence), in 32-bit fixed point, shifted by a random per-pixel, per-segment
+// offset (Cranley-Patterson rotation, from the counter-based stream, distinct
+// from the roulette's).  Each point is uniform, so the estimator stays
+// unbiased, and the rotation is independent of the reference's sample
+// dimensions; the sequence keeps a pixel's guided samples stratified.
 __device__ __forceinline__ float guide_rand(const NodeCtx &x, uint32_t ps, uint32_t seg, uint32_t seed, uint32_t k)
 {
-	return rr_random(x.pixel ^ (0x632BE59Bu * (k + 1u)), ps, seg + 256u * x.tag + 0x10000u * (k + 1u),
-	                 seed ^ 0xA511E9B3u);
+	constexpr uint32_t kGen[4] = {3679390609u, 3152041523u, 2700274806u, 2313257605u};  // g_k * 2^32
+	const float rot = rr_random(x.pixel ^ (0x632BE59Bu * (k + 1u)), 0u, seg + 256u * x.tag + 0x10000u * (k + 1u),
+	                            seed ^ 0xA511E9B3u);
+	const uint32_t u = ps * kGen[k & 3u] + (uint32_t)(rot * 4294967296.f);
+	return (float)(u >> 8) * 0x1p-24f;
 }
 
 // Region of a position (training-weight normalisation, learned selection).
