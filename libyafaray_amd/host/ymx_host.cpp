@@ -794,8 +794,12 @@ std::unique_ptr<PathIntegrator> PathIntegrator::create(Scene &scene, const Param
 	s.wavefront_mb = integrator.getInt("wavefront_mb", 0, &ok);
 	if(background)
 	{
+		// ConstantBackground: color * power (include/background/background_constant.h:55,
+		// include/background/background.h:79)
 		s.has_background = 1;
 		set3(s.background, background->getColor("color", {0, 0, 0, 1}, &ok));
+		const float power = (float)background->getFloat("power", 1.0, &ok);
+		for(int k = 0; k < 3; ++k) s.background[k] *= power;
 	}
 	if(!ok) return fail("a param has the wrong type");
 	if(ymx_pt_create(pi->accel_->handle(), verts.data(), (uint32_t)(verts.size() / 3), tris.data(), mats.data(),

@@ -465,6 +465,131 @@ static void test_pathtracer()
 	unlink(path);
 }
 
+// BackgroundPortalLight through the host layer (type "bgPortalLight",
+// src/light/light_background_portal.cc): an open-backed box whose back is an
+// invisible "window" object, the portal's mesh, under a constant background of
+// colour (0.9, 0.95, 1) and power 2; film against the oracle on the same arrays.
+static void test_portal_light()
+{
+	Scene sc;
+	std::string err;
+	ParamMap mp;
+	mp.setString("type", "shinydiffusemat");
+	mp.setColor("color", 0.7, 0.7, 0.7);
+	size_t white = 0;
+	CHECK(sc.createMaterial(&white, "white", mp, &err), "material: %s", err.c_str());
+	ora_material om{};
+	om.type = ORA_MAT_SHINYDIFFUSE;
+	om.color[0] = om.color[1] = om.color[2] = 0.7f;
+	om.diffuse_reflect = 1.f;
+	om.receive_shadows = 1;
+	om.diffuse_color[0] = om.diffuse_color[1] = om.diffuse_color[2] = 1.f;
+	om.glossy_reflect = 1.f;
+	om.exponent = 50.f;
+	om.as_diffuse = 1;
+	om.reflect = 1.f;
+	om.ior = 1.4f;
+	om.filter_color[0] = om.filter_color[1] = om.filter_color[2] = 1.f;
+	om.mirror_color[0] = om.mirror_color[1] = om.mirror_color[2] = 1.f;
+	om.power = 1.f;
+	std::vector<float> verts;
+	std::vector<uint32_t> tris, mats;
+	ParamMap normal;
+	const size_t walls = sc.createObject("walls", normal);
+	const float o[3] = {0, 0, 0}, x[3] = {1, 0, 0}, y[3] = {0, 1, 0}, z[3] = {0, 0, 1};
+	const float top[3] = {0, 0, 1}, back[3] = {0, 1, 0}, right[3] = {1, 0, 0};
+	quad(sc, walls, white, o, y, x, verts, tris, mats);      // floor
+	quad(sc, walls, white, top, x, y, verts, tris, mats);    // ceiling
+	quad(sc, walls, white, o, z, y, verts, tris, mats);      // left wall
+	quad(sc, walls, white, right, y, z, verts, tris, mats);  // right wall
+	ParamMap inv;
+	inv.setString("visibility", "invisible");
+	const size_t window = sc.createObject("window", inv);
+	quad(sc, window, white, back, x, z, verts, tris, mats);  // the open back, face normal -y
+	ParamMap pl;
+	pl.setString("type", "bgPortalLight");
+	pl.setString("object_name", "window");
+	pl.setFloat("power", 2.0);
+	pl.setInt("samples", 4);
+	CHECK(sc.createLight("portal", pl, &err), "light: %s", err.c_str());
+	ParamMap integ, cam, film, bg;
+	integ.setString("type", "pathtracing");
+	integ.setInt("bounces", 3);
+	integ.setInt("path_samples", 1);
+	integ.setInt("russian_roulette_min_bounces", 2);
+	cam.setString("type", "perspective");
+	cam.setVector("from", 0.5, -1.6, 0.5);
+	cam.setVector("to", 0.5, 0.5, 0.5);
+	cam.setVector("up", 0.5, -1.6, 1.5);
+	cam.setFloat("focal", 1.0 / (2.0 * tan(20.0 * M_PI / 180.0)));
+	cam.setInt("resx", 32);
+	cam.setInt("resy", 24);
+	film.setInt("AA_minsamples", 4);
+	bg.setString("type", "constant");
+	bg.setColor("color", 0.45, 0.475, 0.5);
+	bg.setFloat("power", 2.0);
+	CHECK(PathIntegrator::create(sc, integ, cam, film, nullptr, &err) == nullptr && !err.empty(),
+	      "portal light without a background refused");
+	err.clear();
+	auto pi = PathIntegrator::create(sc, integ, cam, film, &bg, &err);
+	CHECK(pi != nullptr, "create: %s", err.c_str());
+	if(!pi) return;
+	CHECK(pi->clearFilm() == YMX_OK && pi->render() == YMX_OK, "render: %s", pi->lastError().c_str());
+	std::vector<float> rgba, w;
+	CHECK(pi->film(rgba, w) == YMX_OK, "film");
+	ora_light light{};
+	light.type = ORA_LIGHT_PORTAL;
+	light.power = 2.f;
+	light.samples = 4;
+	light.cast_shadows = 1;
+	light.first_tri = 8;  // the invisible window follows the walls' 8 triangles
+	light.n_tris = 2;
+	ora_pt_settings st{};
+	const float from[3] = {0.5f, -1.6f, 0.5f}, to[3] = {0.5f, 0.5f, 0.5f}, up[3] = {0.5f, -1.6f, 1.5f};
+	memcpy(st.from, from, 12);
+	memcpy(st.to, to, 12);
+	memcpy(st.up, up, 12);
+	st.focal = (float)(1.0 / (2.0 * tan(20.0 * M_PI / 180.0)));
+	st.aspect_factor = 1.f;
+	st.resx = 32;
+	st.resy = 24;
+	st.path_samples = 1;
+	st.bounces = 3;
+	st.rr_min_bounces = 2;
+	st.aa_samples = 4;
+	st.aa_pixelwidth = 1.5f;
+	st.raydepth = 5;
+	st.shadow_depth = 4;
+	st.has_background = 1;
+	st.background[0] = 0.45f * 2.f;
+	st.background[1] = 0.475f * 2.f;
+	st.background[2] = 0.5f * 2.f;
+	std::vector<uint8_t> vis(mats.size(), 3);
+	vis[8] = vis[9] = 0;
+	std::vector<ora_material> omats(1, om);
+	ora_kdtree *kd = ora_kd_build(verts.data(), tris.data(), vis.data(), (uint32_t)mats.size());
+	std::vector<float> orgba(4 * 32 * 24, 0.f), ow(32 * 24, 0.f);
+	const int ok = ora_pt_render(kd, verts.data(), tris.data(), mats.data(), vis.data(), omats.data(), &light, 1, &st,
+	                             orgba.data(), ow.data(), 8);
+	ora_kd_free(kd);
+	CHECK(ok == 1, "oracle render");
+	size_t bad_w = 0, bad_c = 0;
+	double mean = 0.0;
+	for(size_t i = 0; i < w.size(); ++i)
+	{
+		bad_w += w[i] != ow[i];
+		for(int k = 0; k < 3; ++k)
+		{
+			const float g = rgba[4 * i + k] / w[i], r = orgba[4 * i + k] / ow[i];
+			mean += r;
+			if(fabsf(g - r) > 1e-4f * fabsf(r) + 1e-6f) ++bad_c;
+		}
+	}
+	printf("  portal light: 32x24x4, mean %.5f, weight mismatches %zu, colour values off %zu\n",
+	       mean / (3.0 * w.size()), bad_w, bad_c);
+	CHECK(bad_w == 0 && bad_c == 0 && mean > 0.0, "film differs from the oracle");
+}
+
 // Scene::preprocess keeps the accelerator unless objects or accelerator
 // params changed (src/scene/scene.cc:314-362).
 static void test_scene_preprocess()
@@ -1045,7 +1170,8 @@ int main(int argc, char **argv)
 	             {"scene_preprocess", test_scene_preprocess, true},
 	             {"instances", test_instances, true},
 	             {"visibility_order", test_visibility_order, true},
-	             {"pathtracer", test_pathtracer, true}};
+	             {"pathtracer", test_pathtracer, true},
+	             {"portal_light", test_portal_light, true}};
 	for(const auto &t : tests)
 	{
 		if(cpu_only && t.gpu) continue;
