@@ -356,9 +356,9 @@ def measure_guiding(args, torch):
     fb_stats = None
     if fallback:
         p.set_guide_fallback("keep")
-        mg, mb = p.guide_fallback()
-        seen = (mg > 0) | (mb > 0)
-        fb_stats = {"regions_with_estimates": int(seen.sum()), "fallen_back": int((seen & (mg > mb)).sum())}
+        mg, mb, records = p.guide_fallback()
+        seen = records >= 256
+        fb_stats = {"regions_with_256_records": int(seen.sum()), "fallen_back": int((seen & (mg > mb)).sum())}
     m = spp - n0
     t_rest = timed(lambda: p.render(n0, m))
     rgba_c, w_c = p.film()

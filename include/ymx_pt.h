@@ -252,16 +252,18 @@ int ymx_pt_get_guide_selection(const ymx_pt *pt, float *alpha, uint32_t n);
  * estimates (weight 0.9 on the past, 0.1 on each training step) of the second
  * moment of its guided one-sample estimator, at the region's selection
  * probability, and of BSDF sampling alone, both from the training samples'
- * draws; a region whose guided moment is the larger samples the BSDF alone
- * (selection probability 1) until the estimates cross back.  The estimator
+ * draws; a region whose guided moment is the larger, once it has seen 256
+ * training records, samples the BSDF alone (selection probability 1) until
+ * the estimates cross back.  The estimator
  * stays unbiased either way.  mode 1: start every region with no estimate
  * (guided); mode 2: use (and while training, go on updating) the estimates
  * another tracer left in the guide's table; mode 0: off.  Call after
  * ymx_pt_set_guide; later ymx_pt_set_guide calls on this tracer keep it. */
 int ymx_pt_set_guide_fallback(ymx_pt *pt, int32_t mode);
 
-/* The running second moments of the fallback, n >= 8192 floats: the guided
- * estimator's for the 4096 regions, then BSDF sampling's (0: no estimate yet). */
+/* The fallback's state, n >= 12288 floats: the running second moments of the
+ * guided estimator for the 4096 regions, then BSDF sampling's (0: no estimate
+ * yet), then the training records each region has seen. */
 int ymx_pt_get_guide_fallback(const ymx_pt *pt, float *moments, uint32_t n);
 
 /* Counters of the last ymx_pt_render: rays cast, and its duration in ms on

@@ -940,7 +940,7 @@ int ymx_pt_set_guide_fallback(ymx_pt *pt, int32_t mode)
 		}
 		// mode 1: no region has fallen back yet: sums and running moments zero
 		const size_t cells = (size_t)kGuideCells * kGuideCells * kGuideCells;
-		if(mode == 1) YMX_HIP_CHECK(hipMemset(t + 4 * cells, 0, sizeof(float) * 5 * cells));
+		if(mode == 1) YMX_HIP_CHECK(hipMemset(t + 4 * cells, 0, sizeof(float) * 6 * cells));
 	}
 	attach_selection(pt);
 	return YMX_OK;
@@ -949,14 +949,14 @@ int ymx_pt_set_guide_fallback(ymx_pt *pt, int32_t mode)
 int ymx_pt_get_guide_fallback(const ymx_pt *pt, float *moments, uint32_t n)
 {
 	const size_t cells = (size_t)kGuideCells * kGuideCells * kGuideCells;
-	if(!pt || !moments || n < 2 * cells || !pt->g.fallback)
+	if(!pt || !moments || n < 3 * cells || !pt->g.fallback)
 	{
 		ymx_set_error("ymx_pt_get_guide_fallback: NULL argument, fewer than %zu entries or no fallback set",
-		              2 * cells);
+		              3 * cells);
 		return YMX_ERR_INVALID_ARG;
 	}
 	YMX_HIP_CHECK(hipDeviceSynchronize());
-	YMX_HIP_CHECK(hipMemcpy(moments, pt->g.fb_run, sizeof(float) * 2 * cells, hipMemcpyDeviceToHost));
+	YMX_HIP_CHECK(hipMemcpy(moments, pt->g.fb_run, sizeof(float) * 3 * cells, hipMemcpyDeviceToHost));
 	return YMX_OK;
 }
 
@@ -979,10 +979,10 @@ int ymx_pt_get_guide_selection(const ymx_pt *pt, float *alpha, uint32_t n)
 	}
 	if(pt->g.fallback)
 	{
-		std::vector<float> run(2 * cells);
-		YMX_HIP_CHECK(hipMemcpy(run.data(), pt->g.fb_run, sizeof(float) * 2 * cells, hipMemcpyDeviceToHost));
+		std::vector<float> run(3 * cells);
+		YMX_HIP_CHECK(hipMemcpy(run.data(), pt->g.fb_run, sizeof(float) * 3 * cells, hipMemcpyDeviceToHost));
 		for(size_t i = 0; i < cells; ++i)
-			if(run[i] > run[cells + i]) alpha[i] = 1.f;
+			if(run[2 * cells + i] >= kFallbackMinRecords && run[i] > run[cells + i]) alpha[i] = 1.f;
 	}
 	return YMX_OK;
 }

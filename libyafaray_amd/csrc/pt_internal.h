@@ -199,13 +199,14 @@ struct PtGuide
 	// records; a region whose guided moment is the larger samples the BSDF only
 	uint32_t fallback;
 	float *fb_sum;             // [3][kGuideCells^3] one mini-batch's sums: guided, BSDF, records
-	float *fb_run;             // [2][kGuideCells^3] running moments: guided, BSDF
+	float *fb_run;             // [3][kGuideCells^3] running moments (guided, BSDF) and records seen
 };
 constexpr float kAlphaMin = 0.2f;  // the BSDF is sampled with probability >= 0.2 at a guided vertex
 constexpr float kFallbackDecay = 0.9f;  // weight of the running moments against one mini-batch's
+constexpr float kFallbackMinRecords = 256.f;  // records a region needs before it may fall back
 // The guide's selection table: theta, the two Adam moments and the gradient
 // sums of the learned selection, then the fallback's sums and running moments.
-constexpr size_t kSelectionTableFloats = (size_t)9 * kGuideCells * kGuideCells * kGuideCells;
+constexpr size_t kSelectionTableFloats = (size_t)10 * kGuideCells * kGuideCells * kGuideCells;
 
 }  // namespace ymx
 

@@ -632,14 +632,15 @@ __device__ __forceinline__ float region_alpha(const PtGuide &g, uint32_t cell)
 
 // The probability of sampling the BSDF at a guided vertex at p: 1 in a region
 // fallen back to BSDF sampling (its running guided second moment above the
-// BSDF's), else the region's alpha.
+// BSDF's, after at least kFallbackMinRecords records), else the region's alpha.
 __device__ __forceinline__ float selection_prob(const PtGuide &g, V3 p)
 {
 	if(!g.learn_alpha && !g.fallback) return g.bsdf_prob;
 	const float pp[3] = {p.x, p.y, p.z};
 	const uint32_t cell = guide_cell(g, pp);
 	constexpr uint32_t kCells = (uint32_t)(kGuideCells * kGuideCells * kGuideCells);
-	if(g.fallback && g.fb_run[cell] > g.fb_run[kCells + cell]) return 1.f;
+	if(g.fallback && g.fb_run[2 * kCells + cell] >= kFallbackMinRecords && g.fb_run[cell] > g.fb_run[kCells + cell])
+		return 1.f;
 	return region_alpha(g, cell);
 }
 
@@ -1886,7 +1887,7 @@ __global__ __launch_bounds__(kBlock) void k_fallback_sum(const ymx_guide_sample 
 
 // The running moments of each region with records in the mini-batch: the
 // first batch's means, then kFallbackDecay * running + (1 - kFallbackDecay) *
-// the batch's mean.  Clears the sums.
+// the batch's mean; and the count of records seen.  Clears the sums.
 __global__ __launch_bounds__(kBlock) void k_fallback_update(PtGuide g)
 {
 	constexpr uint32_t kCells = (uint32_t)(kGuideCells * kGuideCells * kGuideCells);
@@ -1902,6 +1903,7 @@ __global__ __launch_bounds__(kBlock) void k_fallback_update(PtGuide g)
 	const bool first = rg == 0.f && rb == 0.f;
 	g.fb_run[i] = first ? mg : kFallbackDecay * rg + (1.f - kFallbackDecay) * mg;
 	g.fb_run[kCells + i] = first ? mb : kFallbackDecay * rb + (1.f - kFallbackDecay) * mb;
+	g.fb_run[2 * kCells + i] += cnt;
 }
 
 __global__ __launch_bounds__(kBlock) void k_alpha_adam(PtGuide g, uint32_t cells, float lr, float bc1, float bc2)

@@ -285,12 +285,13 @@ class GpuPathTracer:
         left it, or "off"."""
         check(self._lib.ymx_pt_set_guide_fallback(self._h, self.FALLBACK[mode]), "ymx_pt_set_guide_fallback")
 
-    def guide_fallback(self) -> tuple[np.ndarray, np.ndarray]:
-        """The fallback's running second moments per region (guided, BSDF),
-        each of shape (16, 16, 16) as [z, y, x]; 0 where no estimate yet."""
-        m = np.zeros(2 * 16 ** 3, dtype=np.float32)
+    def guide_fallback(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """The fallback's running second moments per region (guided, BSDF;
+        0 where no estimate yet) and the training records each region has
+        seen, each of shape (16, 16, 16) as [z, y, x]."""
+        m = np.zeros(3 * 16 ** 3, dtype=np.float32)
         check(self._lib.ymx_pt_get_guide_fallback(self._h, m.ctypes.data, m.size), "ymx_pt_get_guide_fallback")
-        return m[:16 ** 3].reshape(16, 16, 16), m[16 ** 3:].reshape(16, 16, 16)
+        return tuple(m[k * 16 ** 3:(k + 1) * 16 ** 3].reshape(16, 16, 16) for k in range(3))
 
     KERNELS = {0: "wave-ordered", 1: "replacement"}
 

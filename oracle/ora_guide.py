@@ -264,17 +264,24 @@ def fallback_terms(c, p, q, alpha_old, alpha):
     return f * f / ((al * p + (1.0 - al) * q) * p_old), f * f / (p * p_old)
 
 
-def fallback_update(run_guided, run_bsdf, batch_guided, batch_bsdf, decay=FALLBACK_DECAY):
-    """Running moments after one mini-batch with records in the region (its
-    means given; k_fallback_update): the batch's means when the region has no
-    estimate yet (both 0), else decay * running + (1 - decay) * batch.  The
-    region samples the BSDF alone while the guided moment is the larger."""
+FALLBACK_MIN_RECORDS = 256
+
+
+def fallback_update(run_guided, run_bsdf, batch_guided, batch_bsdf, seen=0, batch_n=FALLBACK_MIN_RECORDS,
+                    decay=FALLBACK_DECAY):
+    """Running moments after one mini-batch of batch_n records in the region
+    (their means given; k_fallback_update): the batch's means when the region
+    has no estimate yet (both 0), else decay * running + (1 - decay) * batch;
+    and the records seen.  The region samples the BSDF alone while the guided
+    moment is the larger, once it has seen FALLBACK_MIN_RECORDS records.
+    Returns (guided, bsdf, seen, falls back)."""
     if run_guided == 0.0 and run_bsdf == 0.0:
         g, b = batch_guided, batch_bsdf
     else:
         g = decay * run_guided + (1.0 - decay) * batch_guided
         b = decay * run_bsdf + (1.0 - decay) * batch_bsdf
-    return g, b, g > b
+    seen = seen + batch_n
+    return g, b, seen, seen >= FALLBACK_MIN_RECORDS and g > b
 
 
 def selection_grad(c, p, q, alpha_old, theta):

@@ -293,17 +293,18 @@ def test_fallback_moments_estimate_the_second_moments():
             for t, exact in ((tg, exact_g), (tb, exact_b)):
                 se = np.sqrt(max(float((p_old * t * t).sum()) - exact * exact, 0.0) / m)
                 assert abs(t[draws].mean() - exact) <= 5 * se + 1e-12 * exact
-            assert G.fallback_update(0.0, 0.0, exact_g, exact_b)[2] == falls_back
+            assert G.fallback_update(0.0, 0.0, exact_g, exact_b)[3] == falls_back
     # F proportional to p: BSDF sampling has zero variance, any other mixture falls back
     Fp = 3.0 * p
     for q in (rng.dirichlet(np.ones(n)), 0.5 * p + 0.5 * rng.dirichlet(np.ones(n))):
         tg, tb = G.fallback_terms(Fp / p, p, q, 1.0, 0.5)
         assert float((p * tg).sum()) > float((p * tb).sum()) == pytest.approx(9.0)
-    # the running update: first batch taken as is, then an exponential average
-    g, b, fb = G.fallback_update(0.0, 0.0, 2.0, 1.0)
-    assert (g, b, fb) == (2.0, 1.0, True)
-    g, b, fb = G.fallback_update(g, b, 0.5, 1.0)
-    assert g == pytest.approx(1.85) and b == pytest.approx(1.0) and fb
+    # the running update: first batch taken as is, then an exponential average;
+    # no fallback before 256 records
+    g, b, seen, fb = G.fallback_update(0.0, 0.0, 2.0, 1.0, 0, 100)
+    assert (g, b, seen, fb) == (2.0, 1.0, 100, False)
+    g, b, seen, fb = G.fallback_update(g, b, 0.5, 1.0, seen, 156)
+    assert g == pytest.approx(1.85) and b == pytest.approx(1.0) and seen == 256 and fb
     for _ in range(20):
-        g, b, fb = G.fallback_update(g, b, 0.5, 1.0)
+        g, b, seen, fb = G.fallback_update(g, b, 0.5, 1.0, seen, 100)
     assert not fb
