@@ -216,7 +216,7 @@ def test_guided_render_converges_to_the_plain_image(depth, learned):
     256 spp each: the guided directions and the roulette draw from the seed)
     agrees with a 4096-spp plain render's within four standard errors of
     their spread (a wrong MIS weight or a biased roulette is off by per cents),
-    and the guided images are finite and within 6x the plain relMSE.  Guiding
+    and the guided images are finite and within 10x the plain relMSE.  Guiding
     every vertex, and only the camera vertex and the next one; with the
     selection probability learned per region, which also stays within
     [0.2, 1] and moves away from its start."""
@@ -246,10 +246,12 @@ def test_guided_render_converges_to_the_plain_image(depth, learned):
             f"ratio {m / ref.mean():.4f}, relMSE median {np.median(errs):.2e}")
         assert all(np.isfinite(im).all() for im in imgs)
         assert abs(m - ref.mean()) < 4 * se + 1e-3 * ref.mean()
-        # measured 1.4-4.2x plain's: the guided vertices sample half their
-        # directions from the mixture and draw their own uniforms where the
-        # plain tracer has the reference's low-discrepancy samples
-        assert np.median(errs) < 6 * rel(plain)
+        # measured 1.4-6.9x plain's (the trained guide varies between runs:
+        # its training sums gradients with float atomics): with bsdf_prob
+        # 0.3 the guided vertices draw 70 % of their directions from the
+        # mixture, and their own uniforms where the plain tracer has the
+        # reference's low-discrepancy samples
+        assert np.median(errs) < 10 * rel(plain)
 
 
 def _plates(res=24, spp=16, seed=0):
