@@ -630,7 +630,17 @@ size_t Scene::createInstance()
 bool Scene::addInstanceObject(size_t instance_id, size_t base_object_id)
 {
 	if(instance_id >= instances_.size() || base_object_id >= objects_.size()) return false;
-	instances_[instance_id].objects.push_back(base_object_id);
+	instances_[instance_id].bases.push_back({base_object_id, false});
+	modified_ |= ModifiedObjects;
+	return true;
+}
+
+// Scene::addInstanceOfInstance (src/scene/scene.cc:259-264): no check of the
+// base beyond its range; an instance of itself is refused at preprocess.
+bool Scene::addInstanceOfInstance(size_t instance_id, size_t base_instance_id)
+{
+	if(instance_id >= instances_.size() || base_instance_id >= instances_.size()) return false;
+	instances_[instance_id].bases.push_back({base_instance_id, true});
 	modified_ |= ModifiedObjects;
 	return true;
 }
@@ -689,15 +699,64 @@ bool Scene::preprocess(uint32_t flags, std::string *error)
 		append(o, nullptr);
 		ranges_[o.name] = {t0, (uint32_t)o.mats.size()};
 	}
-	for(const Instance &in : instances_)
+	// Instance::updatePrimitives in id order (src/geometry/instance.cc,
+	// src/scene/scene.cc:327-341): a base instance contributes the list it
+	// holds now, updated earlier in this pass when its id is lower, else at the
+	// previous preprocess (empty before the first); an instance that names
+	// itself is not updated fully and adds nothing to the scene.  A copy's
+	// vertices take the product of its chain's matrices, outermost first, in
+	// Matrix4f's float order (include/geometry/matrix.h:76-92).
+	auto mul4 = [](const std::array<float, 16> &a, const std::array<float, 16> &b) {
+		std::array<float, 16> r;
+		for(int i = 0; i < 4; ++i)
+			for(int k = 0; k < 4; ++k)
+			{
+				float s = 0.f;
+				for(int j = 0; j < 4; ++j) s += a[4 * i + j] * b[4 * j + k];
+				r[4 * i + k] = s;
+			}
+		return r;
+	};
+	for(size_t id = 0; id < instances_.size(); ++id)
 	{
-		if(in.matrices.size() > 1)
+		Instance &in = instances_[id];
+		if(in.matrices.size() > 2)
 		{
-			if(error) *error = "Scene: an instance with several matrices (motion blur) is not built";
+			if(error) *error = "Scene: an instance with three matrices or more (motion blur) is not built";
 			return false;
 		}
-		if(in.matrices.empty()) continue;
-		for(size_t id : in.objects) append(objects_[id], in.matrices[0].data());
+		std::vector<std::pair<size_t, std::vector<size_t>>> prims;
+		bool updated = true;
+		for(const Instance::Base &b : in.bases)
+		{
+			if(!b.instance)
+			{
+				prims.push_back({b.id, {id}});
+				continue;
+			}
+			if(b.id == id)
+			{
+				updated = false;
+				continue;
+			}
+			for(const auto &e : instances_[b.id].prims)
+			{
+				std::vector<size_t> chain{id};
+				chain.insert(chain.end(), e.second.begin(), e.second.end());
+				prims.push_back({e.first, chain});
+			}
+		}
+		in.prims = std::move(prims);
+		if(!updated) continue;
+		for(const auto &[oid, chain] : in.prims)
+		{
+			bool have = true;
+			for(size_t c : chain) have = have && !instances_[c].matrices.empty();
+			if(!have) continue;
+			std::array<float, 16> m = instances_[chain[0]].matrices[0];
+			for(size_t k = 1; k < chain.size(); ++k) m = mul4(m, instances_[chain[k]].matrices[0]);
+			append(objects_[oid], m.data());
+		}
 	}
 	for(const Object &o : objects_)
 	{

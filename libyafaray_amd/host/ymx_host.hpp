@@ -175,14 +175,19 @@ class Scene
 		// instances), as src/geometry/object/object.cc.
 		size_t createObject(const std::string &name, const ParamMap &params);
 		// Instances (yafaray_createInstance / yafaray_addInstanceObject /
-		// yafaray_addInstanceMatrix, include/public_api/yafaray_c_api.h:227-231):
-		// copies of base objects under an object-to-world matrix (row-major m[4][4]).
-		// preprocess flattens each copy into the triangle list, transforming its
-		// vertices as Matrix4f * Point3f does (src/geometry/primitive/primitive_face.h,
-		// include/geometry/matrix.h:131-143), so hits equal the reference's.  Not
-		// built: instances of instances and motion blur (several matrices).
+		// yafaray_addInstanceOfInstance / yafaray_addInstanceMatrix,
+		// include/public_api/yafaray_c_api.h:227-231): copies of base objects and
+		// of other instances' primitives under an object-to-world matrix
+		// (row-major m[4][4]; the first one is used, as Instance does without
+		// motion blur).  preprocess flattens each copy into the triangle list,
+		// transforming its vertices as Matrix4f * Point3f does under the product
+		// of the chain's matrices, outermost first (src/geometry/primitive/primitive_face.h,
+		// include/geometry/primitive/primitive_instance.h:83-91,
+		// include/geometry/matrix.h:76-92,131-143), so hits equal the reference's.
+		// Not built: motion blur (three matrices or more).
 		size_t createInstance();
 		bool addInstanceObject(size_t instance_id, size_t base_object_id);
+		bool addInstanceOfInstance(size_t instance_id, size_t base_instance_id);
 		bool addInstanceMatrix(size_t instance_id, const double m[16], float time);
 		size_t addVertex(size_t object_id, double x, double y, double z);
 		// yafaray_addNormal (include/public_api/yafaray_c_api.h:217): the normal
@@ -254,8 +259,16 @@ class Scene
 		std::vector<Object> objects_;
 		struct Instance
 		{
-			std::vector<size_t> objects;
+			struct Base
+			{
+				size_t id;
+				bool instance;  // id of another instance, else of an object
+			};
+			std::vector<Base> bases;  // in the order they were added
 			std::vector<std::array<float, 16>> matrices;
+			// Instance::updatePrimitives's list at the last preprocess: per entry a
+			// base object and the chain of instances it is copied through, this one first
+			std::vector<std::pair<size_t, std::vector<size_t>>> prims;
 		};
 		std::vector<Instance> instances_;
 		std::vector<ymx_material> materials_;

@@ -752,10 +752,86 @@ static void test_instances()
 	CHECK(h1.isHit() && h1.primitive == 1 && h1.t_hit == 3.f, "moved instance hit at z = 0 (%u, %g)", h1.primitive, h1.t_hit);
 	CHECK(h2.isHit() && h2.primitive == 0, "inside the scaled copy");
 	CHECK(!h3.isHit(), "outside both copies");
+	// a second matrix changes nothing (Instance::getObjToWorldMatrixAtTime uses
+	// the first below three time steps); a third is motion blur, refused
 	const double two[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
 	sc.addInstanceMatrix(i1, two, 1.f);
+	CHECK(sc.preprocess(sc.checkAndClearModifiedFlags(), &err), "two matrices: %s", err.c_str());
+	const auto a2 = sc.accelerator();
+	CHECK(a2 && a2->numPrimitives() == 2, "two matrices: the same two triangles");
+	if(a2)
+	{
+		const IntersectData h = a2->intersect(
+		    [] { Ray r; r.from[0] = 5.2f; r.from[1] = 0.2f; r.from[2] = 3.f; r.dir[2] = -1.f; return r; }(), FLT_MAX);
+		CHECK(h.isHit() && h.primitive == 1 && h.t_hit == 3.f, "first matrix used (%u, %g)", h.primitive, h.t_hit);
+	}
+	sc.addInstanceMatrix(i1, two, 2.f);
 	err.clear();
 	CHECK(!sc.preprocess(sc.checkAndClearModifiedFlags(), &err) && !err.empty(), "motion blur refused");
+}
+
+// Instances of instances (Scene::addInstanceOfInstance, src/geometry/instance.cc):
+// a copy takes the product of its chain's matrices, outermost first; a base
+// instance contributes the list it held when last updated (so an outer
+// instance of lower id sees its base's primitives only from the second
+// preprocess on); an instance of itself adds nothing.
+static void test_nested_instances()
+{
+	Scene sc;
+	std::string err;
+	ParamMap m;
+	m.setString("type", "shinydiffusemat");
+	size_t mat = 0;
+	CHECK(sc.createMaterial(&mat, "m", m, &err), "material: %s", err.c_str());
+	ParamMap base;
+	base.setBool("is_base_object", true);
+	const size_t obj = sc.createObject("tri", base);
+	sc.addVertex(obj, 0, 0, 0);
+	sc.addVertex(obj, 1, 0, 0);
+	sc.addVertex(obj, 0, 1, 0);
+	sc.addTriangle(obj, 0, 1, 2, mat);
+	const double lift[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 1, 0, 0, 0, 1};        // z + 1
+	const double shift[16] = {2, 0, 0, 5, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};       // x * 2 + 5
+	const double far_[16] = {1, 0, 0, 20, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};       // x + 20
+	const size_t outer = sc.createInstance();   // 0: an instance of instance 2, created first
+	const size_t inner = sc.createInstance();   // 1: the triangle lifted to z = 1
+	const size_t nest = sc.createInstance();    // 2: instance 1 scaled by 2 in x and moved to x + 5
+	const size_t self = sc.createInstance();    // 3: names itself
+	CHECK(sc.addInstanceObject(inner, obj) && sc.addInstanceMatrix(inner, lift, 0.f), "inner");
+	CHECK(sc.addInstanceOfInstance(nest, inner) && sc.addInstanceMatrix(nest, shift, 0.f), "nest");
+	CHECK(sc.addInstanceOfInstance(outer, nest) && sc.addInstanceMatrix(outer, far_, 0.f), "outer");
+	CHECK(sc.addInstanceObject(self, obj) && sc.addInstanceOfInstance(self, self) &&
+	      sc.addInstanceMatrix(self, far_, 0.f), "self");
+	CHECK(!sc.addInstanceOfInstance(9, inner) && !sc.addInstanceOfInstance(inner, 9), "unknown instance refused");
+	auto down = [&](const std::shared_ptr<const Accelerator> &a, float x, float y) {
+		Ray r;
+		r.from[0] = x;
+		r.from[1] = y;
+		r.from[2] = 3.f;
+		r.dir[2] = -1.f;
+		return a->intersect(r, FLT_MAX);
+	};
+	CHECK(sc.preprocess(sc.checkAndClearModifiedFlags(), &err), "preprocess: %s", err.c_str());
+	auto a = sc.accelerator();
+	// first pass: instance 0 saw instance 2 before its update (nothing), 1 and 2 one triangle each
+	CHECK(a && a->numPrimitives() == 2, "first pass: two triangles (%u)", a ? a->numPrimitives() : 0u);
+	if(!a) return;
+	const IntersectData h1 = down(a, 0.2f, 0.2f), h2 = down(a, 6.5f, 0.2f), h3 = down(a, 26.5f, 0.2f);
+	CHECK(h1.isHit() && h1.primitive == 0 && h1.t_hit == 2.f, "inner copy at z = 1 (%u, %g)", h1.primitive, h1.t_hit);
+	CHECK(h2.isHit() && h2.primitive == 1 && h2.t_hit == 2.f, "nested copy: x * 2 + 5 after z + 1 (%u, %g)",
+	      h2.primitive, h2.t_hit);
+	CHECK(!h3.isHit(), "outer copy not there yet");
+	// second pass (any object change): instance 0 now copies instance 2's list, first in the order
+	sc.addInstanceMatrix(inner, lift, 1.f);
+	CHECK(sc.preprocess(sc.checkAndClearModifiedFlags(), &err), "preprocess: %s", err.c_str());
+	a = sc.accelerator();
+	CHECK(a && a->numPrimitives() == 3, "second pass: three triangles (%u)", a ? a->numPrimitives() : 0u);
+	if(!a) return;
+	const IntersectData g3 = down(a, 26.5f, 0.2f), g1 = down(a, 0.2f, 0.2f);
+	CHECK(g3.isHit() && g3.primitive == 0 && g3.t_hit == 2.f, "outer copy: x + 20 after the nest (%u, %g)",
+	      g3.primitive, g3.t_hit);
+	CHECK(g1.isHit() && g1.primitive == 1, "inner copy now second");
+	CHECK(!down(a, 20.2f, 0.2f).isHit(), "the self-naming instance adds nothing");
 }
 
 static void test_errors()
@@ -1169,6 +1245,7 @@ int main(int argc, char **argv)
 	             {"accel_transparent", test_accel_transparent, true},
 	             {"scene_preprocess", test_scene_preprocess, true},
 	             {"instances", test_instances, true},
+	             {"nested_instances", test_nested_instances, true},
 	             {"visibility_order", test_visibility_order, true},
 	             {"pathtracer", test_pathtracer, true},
 	             {"portal_light", test_portal_light, true}};
